@@ -1,0 +1,139 @@
+/*
+ * spira_amd.h -- C-ABI of libspira_amd: Spira's locked-mode hot path
+ * (lock() and SpMV) on one MI355X (gfx950).
+ *
+ * Plain C types only: no torch, no HIP types in the signatures.  The stream is
+ * passed as an opaque `void *` (a hipStream_t).  Every function returns a
+ * status code (SPIRA_OK == 0); the message of the last failure on a handle is
+ * spira_dev_last_error().  Status codes map 1:1 onto the reference's
+ * exceptions so a C++ wrapper can rethrow exactly what Spira throws.
+ *
+ * Threading: calls on one spira_dev are serialised on its stream and must not
+ * overlap from several host threads; separate handles may run concurrently.
+ * Host arrays are borrowed for the duration of a call; device objects
+ * (spira_dcsr) are owned by the caller until spira_dcsr_destroy().
+ */
+#ifndef SPIRA_AMD_H
+#define SPIRA_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct spira_dev spira_dev;   /* device ordinal, stream, workspace */
+typedef struct spira_dcsr spira_dcsr; /* device CSR: u64 offsets, u32 cols, f64 vals */
+
+/* ---- status codes ------------------------------------------------------- */
+#define SPIRA_OK 0
+#define SPIRA_ERR_INVALID_ARGUMENT 1 /* std::invalid_argument */
+#define SPIRA_ERR_LOGIC 2            /* std::logic_error */
+#define SPIRA_ERR_OUT_OF_RANGE 3     /* std::out_of_range */
+#define SPIRA_ERR_NOMEM 4            /* std::bad_alloc */
+#define SPIRA_ERR_DEVICE 5           /* HIP runtime failure */
+#define SPIRA_ERR_UNSUPPORTED 6      /* input outside this build's limits */
+
+/* ---- pointer-location and behaviour flags ------------------------------ */
+#define SPIRA_PTR_HOST 0u
+#define SPIRA_X_DEVICE 1u   /* x (or X) is a device pointer */
+#define SPIRA_Y_DEVICE 2u   /* y (or Y) is a device pointer */
+#define SPIRA_IN_DEVICE 4u  /* upload / staged input arrays are device pointers */
+#define SPIRA_OUT_DEVICE 8u /* download destinations are device pointers */
+#define SPIRA_NO_SYNC 16u   /* return without synchronising the stream
+                               (device pointers only) */
+
+/* ---- SpMV kernel variants ------------------------------------------------ */
+#define SPIRA_SPMV_AUTO (-1)      /* cached choice, else the built-in heuristic */
+#define SPIRA_SPMV_VEC1 0         /* one lane per row */
+#define SPIRA_SPMV_VEC2 1         /* G lanes per row, shuffle reduction ... */
+#define SPIRA_SPMV_VEC4 2
+#define SPIRA_SPMV_VEC8 3
+#define SPIRA_SPMV_VEC16 4
+#define SPIRA_SPMV_VEC32 5
+#define SPIRA_SPMV_VEC64 6        /* one wavefront per row */
+#define SPIRA_SPMV_STREAM1K 7     /* LDS-staged row blocks of <= 1024 rows+nnz */
+#define SPIRA_SPMV_STREAM2K 8     /* LDS-staged row blocks of <= 2048 rows+nnz */
+#define SPIRA_SPMV_N_VARIANTS 9
+
+/* ---- device handle --------------------------------------------------------
+ * Replaces the reference's CPU dispatch role (dispatch.cpp:34-67,
+ * runtime_config.hpp:7-31): one handle per GPU and stream. */
+int spira_dev_create(spira_dev **out, int device_ordinal, void *stream);
+void spira_dev_destroy(spira_dev *dev);
+const char *spira_dev_last_error(const spira_dev *dev);
+void *spira_dev_stream(const spira_dev *dev);
+int spira_dev_synchronize(spira_dev *dev);
+
+/* ---- device CSR ------------------------------------------------------------
+ * csr_storage<soa_tag, uint32_t, double> (csr_storage_soa.hpp:17-76) on the
+ * device.  A fresh handle is "not built", like matrix::csr_ before the first
+ * lock (matrix.hpp:178, :306). */
+int spira_dcsr_create(spira_dev *dev, uint64_t n_rows, uint64_t n_cols, spira_dcsr **out);
+void spira_dcsr_destroy(spira_dcsr *A);
+/* n_rows, n_cols, nnz and whether a CSR has been built (any output may be NULL). */
+int spira_dcsr_info(const spira_dcsr *A, uint64_t *n_rows, uint64_t *n_cols, uint64_t *nnz, int *built);
+
+/* matrix::load_csr (matrix.hpp:520-605): validate and install an external CSR.
+ * Validation order and messages follow matrix.hpp:524-575 (offsets[0] == 0,
+ * offsets[n] == nnz, non-decreasing offsets, in-bounds and strictly increasing
+ * columns); every failure is SPIRA_ERR_INVALID_ARGUMENT.  Explicit zero values
+ * are accepted, as in the reference.  flags: SPIRA_IN_DEVICE. */
+int spira_dcsr_upload_f64(spira_dcsr *A, uint64_t nnz, const uint64_t *offsets,
+                          const uint32_t *cols, const double *vals, uint32_t flags);
+
+/* matrix::csr() read-out (matrix.hpp:88-91): copy offsets[n_rows+1],
+ * cols[nnz], vals[nnz] out (any pointer may be NULL to skip that array).
+ * flags: SPIRA_OUT_DEVICE.  SPIRA_ERR_LOGIC if no CSR has been built. */
+int spira_dcsr_download_f64(const spira_dcsr *A, uint64_t *offsets, uint32_t *cols,
+                            double *vals, uint32_t flags);
+
+/* ---- lock() ---------------------------------------------------------------
+ * matrix::lock() / parallel_matrix::lock_partition (matrix.hpp:277-342,
+ * parallel_matrix.hpp:321-386) with build_csr / merge_csr
+ * (csr_build.hpp:32-98, :126-488) on the device.
+ *   stage_ptr[n_rows+1], stage_col[m], stage_val[m]: the open-mode inserts of
+ *   every row, in insertion order (row r owns [stage_ptr[r], stage_ptr[r+1])).
+ *   dirty: optional u8[n_rows]; NULL means dirty[r] = (row r has inserts), the
+ *   parallel_matrix rule (parallel_matrix.hpp:351-355).  matrix::dirty_ is
+ *   passed explicitly when it differs (row_at_mut without an insert).
+ * First call on an unbuilt A = build_csr; later calls = merge_csr.  Result is
+ * bit-identical to the reference's offsets/cols/vals.  *out_nnz (may be NULL)
+ * receives the new nnz.  Column indices >= n_cols give SPIRA_ERR_OUT_OF_RANGE
+ * (matrix.hpp:167-171).  A row with more than SPIRA_LOCK_MAX_ROW_INSERTS
+ * staged inserts in one call gives SPIRA_ERR_UNSUPPORTED.
+ * flags: SPIRA_IN_DEVICE. */
+#define SPIRA_LOCK_MAX_ROW_INSERTS 8192u
+int spira_lock_f64(spira_dcsr *A, const uint64_t *stage_ptr, const uint32_t *stage_col,
+                   const double *stage_val, const uint8_t *dirty, uint32_t flags,
+                   uint64_t *out_nnz);
+
+/* ---- SpMV -----------------------------------------------------------------
+ * serial::algorithms::spmv / parallel::algorithms::spmv for
+ * <soa_tag, uint32_t, double> (serial/spmv.hpp:142-186,
+ * parallel/algorithms/spmv.hpp:127-161): y = A*x, y overwritten, empty rows
+ * give +0.0.  Checks in the reference's order: x_len != n_cols and
+ * y_len != n_rows give SPIRA_ERR_INVALID_ARGUMENT, an unbuilt A gives
+ * SPIRA_ERR_LOGIC.  FP64 throughout; normwise relative error vs. the
+ * sequential FP64 loop <= 1e-12.  variant: SPIRA_SPMV_*.
+ * flags: SPIRA_X_DEVICE | SPIRA_Y_DEVICE | SPIRA_NO_SYNC. */
+int spira_spmv_f64(spira_dcsr *A, const double *x, uint64_t x_len, double *y,
+                   uint64_t y_len, int variant, uint32_t flags);
+
+/* Variant the AUTO path would use now for A (after plan build). */
+int spira_spmv_choose(spira_dcsr *A, int *variant);
+/* Pin the AUTO choice for A (autotuner / selector output). */
+int spira_spmv_set_variant(spira_dcsr *A, int variant);
+
+/* Exhaustive autotuner (replaces the reference's one-time CPUID kernel pick,
+ * dispatch.cpp:34-67, with a per-matrix timed pick): runs every variant `reps`
+ * times on device vectors x (n_cols) and y (n_rows), writes the mean time per
+ * call in milliseconds to ms[SPIRA_SPMV_N_VARIANTS], pins the fastest as A's
+ * AUTO choice and returns it in *best (either output may be NULL). */
+int spira_spmv_autotune_f64(spira_dcsr *A, const double *x_dev, double *y_dev, int reps, float *ms, int *best);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPIRA_AMD_H */

@@ -1,0 +1,606 @@
+"""spira_amd -- Spira's locked-mode hot path (``lock()`` and SpMV) on MI355X.
+
+Python mirror of the reference's operator surface for this path, over the
+C-ABI of ``libspira_amd.so`` (``include/spira_amd.h``):
+
+* :class:`Matrix` mirrors ``spira::matrix<soa_tag, uint32_t, double>``
+  (``include/spira/matrix/matrix.hpp``): open-mode ``insert`` into per-row
+  buffers kept in insertion order on the host, ``lock()`` on the device
+  (``build_csr`` / ``merge_csr``), ``open()``, ``clear()``, ``load_csr``,
+  ``csr()``, ``get``/``contains``/``row_nnz``/``nnz``.
+* :class:`ParallelMatrix` mirrors ``spira::parallel::parallel_matrix``
+  (``include/spira/parallel/parallel_matrix.hpp``): same lifecycle, uniform row
+  partitions; the device holds one concatenated CSR and ``partition_at(t)``
+  returns the partition's rebased slice.
+* :func:`spmv` mirrors ``serial::algorithms::spmv`` / ``parallel::algorithms::spmv``
+  (``serial/spmv.hpp:142-186``, ``parallel/algorithms/spmv.hpp:127-161``).
+
+Exceptions mirror the reference's: ``InvalidArgument`` (std::invalid_argument),
+``LogicError`` (std::logic_error), ``OutOfRange`` (std::out_of_range).
+
+There is no CPU fallback: importing this package fails if the HIP library has
+not been built, and every compute call runs the HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = [
+    "Device", "DeviceCSR", "Matrix", "ParallelMatrix", "spmv", "lib", "lib_path",
+    "InvalidArgument", "LogicError", "OutOfRange", "UnsupportedInput", "DeviceError",
+    "VARIANTS", "SPMV_AUTO",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libspira_amd.so")
+
+# ---- constants mirrored from include/spira_amd.h ---------------------------
+OK = 0
+ERR_INVALID_ARGUMENT = 1
+ERR_LOGIC = 2
+ERR_OUT_OF_RANGE = 3
+ERR_NOMEM = 4
+ERR_DEVICE = 5
+ERR_UNSUPPORTED = 6
+
+X_DEVICE = 1
+Y_DEVICE = 2
+IN_DEVICE = 4
+OUT_DEVICE = 8
+NO_SYNC = 16
+
+SPMV_AUTO = -1
+VARIANTS = {
+    "vec1": 0, "vec2": 1, "vec4": 2, "vec8": 3, "vec16": 4, "vec32": 5, "vec64": 6,
+    "stream1k": 7, "stream2k": 8,
+}
+VARIANT_NAMES = {v: k for k, v in VARIANTS.items()}
+N_VARIANTS = 9
+LOCK_MAX_ROW_INSERTS = 8192
+
+
+class SpiraError(Exception):
+    pass
+
+
+class InvalidArgument(SpiraError, ValueError):
+    """std::invalid_argument"""
+
+
+class LogicError(SpiraError, RuntimeError):
+    """std::logic_error"""
+
+
+class OutOfRange(SpiraError, IndexError):
+    """std::out_of_range"""
+
+
+class UnsupportedInput(SpiraError, ValueError):
+    """Input outside this build's limits (SPIRA_ERR_UNSUPPORTED)."""
+
+
+class DeviceError(SpiraError, RuntimeError):
+    """HIP runtime failure or device allocation failure."""
+
+
+_ERR = {
+    ERR_INVALID_ARGUMENT: InvalidArgument,
+    ERR_LOGIC: LogicError,
+    ERR_OUT_OF_RANGE: OutOfRange,
+    ERR_NOMEM: MemoryError,
+    ERR_DEVICE: DeviceError,
+    ERR_UNSUPPORTED: UnsupportedInput,
+}
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+_lib = None
+
+
+def lib_path() -> str:
+    return _LIB_PATH
+
+
+def lib():
+    """Load libspira_amd.so; fail loudly when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise ImportError(
+                f"{_LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+        L = ctypes.CDLL(_LIB_PATH)
+        sig = {
+            "spira_dev_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _vp]),
+            "spira_dev_destroy": (None, [_vp]),
+            "spira_dev_last_error": (ctypes.c_char_p, [_vp]),
+            "spira_dev_stream": (_vp, [_vp]),
+            "spira_dev_synchronize": (ctypes.c_int, [_vp]),
+            "spira_dcsr_create": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(_vp)]),
+            "spira_dcsr_destroy": (None, [_vp]),
+            "spira_dcsr_info": (ctypes.c_int, [_vp, _u64p, _u64p, _u64p, ctypes.POINTER(ctypes.c_int)]),
+            "spira_dcsr_upload_f64": (ctypes.c_int, [_vp, ctypes.c_uint64, _vp, _vp, _vp, ctypes.c_uint32]),
+            "spira_dcsr_download_f64": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_uint32]),
+            "spira_lock_f64": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_uint32, _u64p]),
+            "spira_spmv_f64": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64, _vp, ctypes.c_uint64, ctypes.c_int,
+                                              ctypes.c_uint32]),
+            "spira_spmv_choose": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
+            "spira_spmv_set_variant": (ctypes.c_int, [_vp, ctypes.c_int]),
+            "spira_spmv_autotune_f64": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
+                                                       ctypes.POINTER(ctypes.c_int)]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a) -> int:
+    """Address of a numpy array or a torch tensor (or an int address)."""
+    if a is None:
+        return 0
+    if isinstance(a, int):
+        return a
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return int(a.data_ptr())
+
+
+class Device:
+    """spira_dev: one GPU and one HIP stream (``stream`` = raw hipStream_t or None)."""
+
+    def __init__(self, ordinal: int = 0, stream: int | None = None):
+        h = _vp()
+        rc = lib().spira_dev_create(ctypes.byref(h), ordinal, stream or None)
+        if rc != OK:
+            raise _ERR.get(rc, DeviceError)(lib().spira_dev_last_error(None).decode())
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def check(self, rc: int):
+        if rc != OK:
+            msg = lib().spira_dev_last_error(self._h).decode()
+            raise _ERR.get(rc, DeviceError)(msg)
+
+    def synchronize(self):
+        self.check(lib().spira_dev_synchronize(self._h))
+
+    @property
+    def stream(self) -> int:
+        return lib().spira_dev_stream(self._h) or 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().spira_dev_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_device: Device | None = None
+
+
+def default_device() -> Device:
+    global _default_device
+    if _default_device is None:
+        _default_device = Device(0)
+    return _default_device
+
+
+class DeviceCSR:
+    """spira_dcsr: csr_storage<soa_tag, uint32_t, double> resident in HBM."""
+
+    def __init__(self, dev: Device, n_rows: int, n_cols: int):
+        self.dev = dev
+        h = _vp()
+        dev.check(lib().spira_dcsr_create(dev.handle, n_rows, n_cols, ctypes.byref(h)))
+        self._h = h
+        self.n_rows, self.n_cols = int(n_rows), int(n_cols)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self):
+        a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        built = ctypes.c_int()
+        self.dev.check(lib().spira_dcsr_info(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                             ctypes.byref(built)))
+        return {"n_rows": a.value, "n_cols": b.value, "nnz": c.value, "built": bool(built.value)}
+
+    @property
+    def nnz(self) -> int:
+        return self.info()["nnz"]
+
+    @property
+    def built(self) -> bool:
+        return self.info()["built"]
+
+    def upload(self, offsets, cols, vals, device: bool = False):
+        """load_csr (matrix.hpp:520-605): validate and install."""
+        if not device:
+            offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+            cols = np.ascontiguousarray(cols, dtype=np.uint32)
+            vals = np.ascontiguousarray(vals, dtype=np.float64)
+            if offsets.size != self.n_rows + 1:
+                raise InvalidArgument("spira::matrix::load_csr: CSR row count does not match matrix dimensions")
+            nnz = int(cols.size)
+            if vals.size != nnz:
+                raise InvalidArgument("load_csr: cols and vals differ in length")
+        else:
+            nnz = int(cols.numel())
+        self.dev.check(lib().spira_dcsr_upload_f64(self._h, nnz, _ptr(offsets), _ptr(cols), _ptr(vals),
+                                                   IN_DEVICE if device else 0))
+
+    def download(self):
+        """csr() read-out: (offsets u64[n+1], cols u32[nnz], vals f64[nnz])."""
+        nnz = self.nnz
+        off = np.empty(self.n_rows + 1, np.uint64)
+        col = np.empty(max(nnz, 1), np.uint32)
+        val = np.empty(max(nnz, 1), np.float64)
+        self.dev.check(lib().spira_dcsr_download_f64(self._h, _ptr(off), _ptr(col), _ptr(val), 0))
+        return off, col[:nnz], val[:nnz]
+
+    def lock(self, stage_ptr, stage_col, stage_val, dirty=None, device: bool = False) -> int:
+        """Device lock(): build (first call) or merge the staged inserts."""
+        out = ctypes.c_uint64()
+        if not device:
+            stage_ptr = np.ascontiguousarray(stage_ptr, dtype=np.uint64)
+            stage_col = np.ascontiguousarray(stage_col, dtype=np.uint32)
+            stage_val = np.ascontiguousarray(stage_val, dtype=np.float64)
+            if stage_ptr.size != self.n_rows + 1:
+                raise InvalidArgument("lock: stage_ptr must have n_rows+1 entries")
+            if dirty is not None:
+                dirty = np.ascontiguousarray(dirty, dtype=np.uint8)
+                if dirty.size != self.n_rows:
+                    raise InvalidArgument("lock: dirty must have n_rows entries")
+        self.dev.check(lib().spira_lock_f64(self._h, _ptr(stage_ptr), _ptr(stage_col), _ptr(stage_val),
+                                            _ptr(dirty), IN_DEVICE if device else 0, ctypes.byref(out)))
+        return int(out.value)
+
+    def spmv(self, x, y=None, variant: int = SPMV_AUTO, device: bool = False, sync: bool = True):
+        """y = A*x.  Host numpy arrays, or device tensors with device=True."""
+        if not device:
+            x = np.ascontiguousarray(x, dtype=np.float64)
+            if y is None:
+                y = np.empty(self.n_rows, np.float64)
+            xl, yl = x.size, y.size
+            flags = 0
+        else:
+            xl, yl = int(x.numel()), int(y.numel())
+            flags = X_DEVICE | Y_DEVICE | (0 if sync else NO_SYNC)
+        self.dev.check(lib().spira_spmv_f64(self._h, _ptr(x), xl, _ptr(y), yl, int(variant), flags))
+        return y
+
+    def choose_variant(self) -> int:
+        v = ctypes.c_int()
+        self.dev.check(lib().spira_spmv_choose(self._h, ctypes.byref(v)))
+        return v.value
+
+    def set_variant(self, variant: int):
+        self.dev.check(lib().spira_spmv_set_variant(self._h, int(variant)))
+
+    def autotune(self, x_dev, y_dev, reps: int = 5):
+        """Exhaustive per-matrix autotune on device vectors; pins the fastest.
+        Returns (best_variant, {variant: ms per call})."""
+        ms = (ctypes.c_float * N_VARIANTS)()
+        best = ctypes.c_int()
+        self.dev.check(lib().spira_spmv_autotune_f64(self._h, _ptr(x_dev), _ptr(y_dev), int(reps), ms,
+                                                     ctypes.byref(best)))
+        return best.value, {v: float(ms[v]) for v in range(N_VARIANTS)}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().spira_dcsr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def stage_from_coo(n_rows: int, rows, cols, vals):
+    """Per-row segments in insertion order from a COO insert stream (stable)."""
+    rows = np.asarray(rows, dtype=np.int64)
+    cols = np.asarray(cols, dtype=np.uint32)
+    vals = np.asarray(vals, dtype=np.float64)
+    order = np.argsort(rows, kind="stable")
+    counts = np.bincount(rows, minlength=n_rows) if rows.size else np.zeros(n_rows, np.int64)
+    ptr = np.zeros(n_rows + 1, np.uint64)
+    np.cumsum(counts, out=ptr[1:])
+    return ptr, cols[order], vals[order]
+
+
+class _RowBuffers:
+    """Open-mode staging: the insert stream, kept in insertion order."""
+
+    def __init__(self):
+        self.clear()
+
+    def clear(self):
+        self._r, self._c, self._v = [], [], []
+        self._bulk = []
+
+    def append(self, r, c, v):
+        self._r.append(r)
+        self._c.append(c)
+        self._v.append(v)
+
+    def extend(self, rows, cols, vals):
+        self._flush_small()
+        self._bulk.append((np.asarray(rows, np.int64), np.asarray(cols, np.uint32), np.asarray(vals, np.float64)))
+
+    def _flush_small(self):
+        if self._r:
+            self._bulk.append((np.asarray(self._r, np.int64), np.asarray(self._c, np.uint32),
+                               np.asarray(self._v, np.float64)))
+            self._r, self._c, self._v = [], [], []
+
+    def coo(self):
+        self._flush_small()
+        if not self._bulk:
+            return np.zeros(0, np.int64), np.zeros(0, np.uint32), np.zeros(0, np.float64)
+        if len(self._bulk) > 1:
+            self._bulk = [tuple(np.concatenate([b[i] for b in self._bulk]) for i in range(3))]
+        return self._bulk[0]
+
+    def __len__(self):
+        return len(self._r) + sum(b[0].size for b in self._bulk)
+
+    def lookup(self, r, c):
+        """Last write to (r, c) in the buffer, or None (buffer-first read)."""
+        for i in range(len(self._r) - 1, -1, -1):
+            if self._r[i] == r and self._c[i] == c:
+                return self._v[i]
+        for rows, cols, vals in reversed(self._bulk):
+            hit = np.nonzero((rows == r) & (cols == c))[0]
+            if hit.size:
+                return float(vals[hit[-1]])
+        return None
+
+    def row_count(self, r):
+        n = sum(1 for x in self._r if x == r)
+        return n + sum(int(np.count_nonzero(b[0] == r)) for b in self._bulk)
+
+    def rows_touched(self, n_rows):
+        d = np.zeros(n_rows, np.uint8)
+        rows, _, _ = self.coo()
+        d[rows] = 1
+        return d
+
+
+class Matrix:
+    """spira::matrix<soa_tag, uint32_t, double> with the device backend."""
+
+    def __init__(self, n_rows: int, n_cols: int, device: Device | None = None):
+        self.dev = device or default_device()
+        self._n_rows, self._n_cols = int(n_rows), int(n_cols)
+        self._locked = False
+        self._buf = _RowBuffers()
+        self._dirty = np.zeros(self._n_rows, np.uint8)
+        self._csr = DeviceCSR(self.dev, n_rows, n_cols)
+        self._host = None  # cached host copy of the CSR for point queries
+
+    # shape
+    def shape(self):
+        return (self._n_rows, self._n_cols)
+
+    def n_rows(self):
+        return self._n_rows
+
+    def n_cols(self):
+        return self._n_cols
+
+    # modes
+    def is_locked(self):
+        return self._locked
+
+    def is_open(self):
+        return not self._locked
+
+    def lock(self):
+        """matrix.hpp:277-342."""
+        if self._locked:
+            return
+        rows, cols, vals = self._buf.coo()
+        ptr, c, v = stage_from_coo(self._n_rows, rows, cols, vals)
+        dirty = self._dirty if self._csr.built else None
+        self._csr.lock(ptr, c, v, dirty=dirty)
+        self._buf.clear()
+        self._dirty[:] = 0
+        self._locked = True
+        self._host = None
+
+    def open(self):
+        """matrix.hpp:348-357: the committed CSR is kept."""
+        self._locked = False
+
+    def _check_row(self, r):
+        if r < 0 or r >= self._n_rows:
+            raise OutOfRange("spira::matrix: row_index out of range")
+
+    def _check_col(self, c):
+        if c < 0 or c >= self._n_cols:
+            raise OutOfRange("spira::matrix: col_index out of range")
+
+    def insert(self, r: int, c: int, v: float):
+        """matrix.hpp:453-462."""
+        if self._locked:
+            raise LogicError("matrix::insert() requires open mode")
+        self._check_row(r)
+        self._check_col(c)
+        self._buf.append(int(r), int(c), float(v))
+        self._dirty[r] = 1
+
+    def insert_many(self, rows, cols, vals):
+        """Bulk form of insert() (same checks, same insertion order)."""
+        if self._locked:
+            raise LogicError("matrix::insert() requires open mode")
+        rows = np.asarray(rows, np.int64)
+        cols = np.asarray(cols, np.int64)
+        if rows.size and (rows.min() < 0 or rows.max() >= self._n_rows):
+            raise OutOfRange("spira::matrix: row_index out of range")
+        if cols.size and (cols.min() < 0 or cols.max() >= self._n_cols):
+            raise OutOfRange("spira::matrix: col_index out of range")
+        self._buf.extend(rows, cols, vals)
+        self._dirty[rows] = 1
+
+    def clear(self):
+        """matrix.hpp:468-477: pending inserts dropped, CSR kept."""
+        if self._locked:
+            raise LogicError("matrix::clear() requires open mode")
+        self._buf.clear()
+        self._dirty[:] = 0
+
+    def mark_dirty(self, r: int):
+        """Effect of row_at_mut() without an insert (matrix.hpp:483-491)."""
+        if self._locked:
+            raise LogicError("matrix::row_at_mut() requires open mode")
+        self._check_row(r)
+        self._dirty[r] = 1
+
+    # queries
+    def csr(self):
+        """matrix.hpp:88-91: None before the first lock, else (offsets, cols, vals)."""
+        if not self._csr.built:
+            return None
+        if self._host is None:
+            self._host = self._csr.download()
+        return self._host
+
+    def device_csr(self) -> DeviceCSR:
+        return self._csr
+
+    def _committed(self, r, c):
+        h = self.csr()
+        if h is None:
+            return None
+        off, col, val = h
+        a, b = int(off[r]), int(off[r + 1])
+        i = a + int(np.searchsorted(col[a:b], c))
+        if i < b and int(col[i]) == c:
+            return float(val[i])
+        return None
+
+    def get(self, r, c):
+        self._check_row(r)
+        self._check_col(c)
+        if not self._locked:
+            v = self._buf.lookup(r, c)
+            if v is not None:
+                return v
+        v = self._committed(r, c)
+        return 0.0 if v is None else v
+
+    def contains(self, r, c):
+        self._check_row(r)
+        self._check_col(c)
+        if not self._locked and self._buf.lookup(r, c) is not None:
+            return True
+        return self._committed(r, c) is not None
+
+    def row_nnz(self, r):
+        self._check_row(r)
+        h = self.csr()
+        committed = 0 if h is None else int(h[0][r + 1] - h[0][r])
+        return committed + (0 if self._locked else self._buf.row_count(r))
+
+    def nnz(self):
+        committed = self._csr.nnz if self._csr.built else 0
+        return committed + (0 if self._locked else len(self._buf))
+
+    def load_csr(self, offsets, cols, vals):
+        """matrix.hpp:520-605 (validation on the device)."""
+        self._csr.upload(offsets, cols, vals)
+        self._buf.clear()
+        self._dirty[:] = 0
+        self._locked = True
+        self._host = None
+
+    def __matmul__(self, x):
+        y = np.empty(self._n_rows, np.float64)
+        spmv(self, x, y)
+        return y
+
+
+class ParallelMatrix(Matrix):
+    """parallel_matrix<soa_tag, uint32_t, double> with the device backend.
+
+    Rows are split uniformly into n_threads partitions (parallel_matrix.hpp:296-306);
+    the device keeps one concatenated CSR and partition_at(t) returns the
+    partition's rebased (offsets, cols, vals).  dirty follows the parallel rule
+    "row buffer non-empty" (parallel_matrix.hpp:351-355)."""
+
+    def __init__(self, n_rows, n_cols, n_threads, device: Device | None = None):
+        if n_threads < 1:
+            raise InvalidArgument("parallel_matrix requires at least one thread")
+        super().__init__(n_rows, n_cols, device)
+        self._n_threads = int(n_threads)
+        self._bounds = [t * n_rows // n_threads for t in range(n_threads + 1)]
+
+    def n_threads(self):
+        return self._n_threads
+
+    def lock(self):
+        if self._locked:
+            return
+        rows, cols, vals = self._buf.coo()
+        ptr, c, v = stage_from_coo(self._n_rows, rows, cols, vals)
+        self._csr.lock(ptr, c, v, dirty=None)
+        self._buf.clear()
+        self._dirty[:] = 0
+        self._locked = True
+        self._host = None
+
+    def partition_bounds(self):
+        return list(self._bounds)
+
+    def partition_at(self, t):
+        if t < 0 or t >= self._n_threads:
+            raise OutOfRange("parallel_matrix::partition_at: thread index out of range")
+        h = self.csr()
+        a, b = self._bounds[t], self._bounds[t + 1]
+        if h is None:
+            return None
+        off, col, val = h
+        base = int(off[a])
+        return (off[a:b + 1] - np.uint64(base), col[base:int(off[b])], val[base:int(off[b])])
+
+
+def spmv(mat: Matrix, x, y, variant: int = SPMV_AUTO):
+    """serial::algorithms::spmv / parallel::algorithms::spmv (FP64, device).
+
+    x and y are numpy float64 arrays (host) or torch float64 CUDA tensors."""
+    is_dev = not isinstance(x, np.ndarray)
+    xl = int(x.numel()) if is_dev else int(np.asarray(x).size)
+    yl = int(y.numel()) if is_dev else int(np.asarray(y).size)
+    if xl != mat.n_cols():
+        raise InvalidArgument("The size of the input vector x does not match the number of columns of the matrix")
+    if yl != mat.n_rows():
+        raise InvalidArgument("The size of the output vector y does not match the number of rows of the matrix")
+    if not mat.is_locked():
+        raise LogicError("spmv: matrix must be locked")
+    if is_dev:
+        mat.device_csr().spmv(x, y, variant=variant, device=True)
+    else:
+        if not (isinstance(y, np.ndarray) and y.dtype == np.float64 and y.flags.c_contiguous):
+            raise InvalidArgument("spmv: y must be a contiguous float64 array")
+        mat.device_csr().spmv(np.ascontiguousarray(x, dtype=np.float64), y, variant=variant)
+    return y

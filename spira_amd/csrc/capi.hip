@@ -1,0 +1,355 @@
+// capi.hip -- the extern "C" boundary of libspira_amd (include/spira_amd.h).
+// Converts status_error / bad_alloc into status codes and keeps the message on
+// the handle.
+#include "internal.hpp"
+
+#include <cstring>
+#include <new>
+
+using namespace spira_amd;
+
+namespace {
+
+thread_local std::string g_last_error;  // for calls without a usable handle
+
+template <class F>
+int guarded(spira_dev *dev, F &&f)
+{
+    std::string *sink = dev ? &dev->last_error : &g_last_error;
+    try {
+        if (dev) SPIRA_HIP(hipSetDevice(dev->device));
+        f();
+        return SPIRA_OK;
+    } catch (const status_error &e) {
+        *sink = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        *sink = "host allocation failed";
+        return SPIRA_ERR_NOMEM;
+    } catch (const std::exception &e) {
+        *sink = e.what();
+        return SPIRA_ERR_DEVICE;
+    }
+}
+
+void copy_in(spira_dev *dev, void *dst, const void *src, size_t bytes, bool src_device)
+{
+    if (bytes == 0) return;
+    SPIRA_HIP(hipMemcpyAsync(dst, src, bytes, src_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                             dev->stream));
+}
+
+}  // namespace
+
+namespace spira_amd {
+
+void free_dcsr_arrays(spira_dcsr *A)
+{
+    if (A->off) (void)hipFree(A->off);
+    if (A->col) (void)hipFree(A->col);
+    if (A->val) (void)hipFree(A->val);
+    if (A->val_f32) (void)hipFree(A->val_f32);
+    if (A->val_bf16) (void)hipFree(A->val_bf16);
+    if (A->val_bf16_lo) (void)hipFree(A->val_bf16_lo);
+    A->off = nullptr;
+    A->col = nullptr;
+    A->val = nullptr;
+    A->val_f32 = nullptr;
+    A->val_bf16 = nullptr;
+    A->val_bf16_lo = nullptr;
+    A->vals_ready = 0;
+    spmv_free_plan(A);
+}
+
+}  // namespace spira_amd
+
+extern "C" {
+
+int spira_dev_create(spira_dev **out, int device_ordinal, void *stream)
+{
+    if (!out) {
+        g_last_error = "spira_dev_create: out is NULL";
+        return SPIRA_ERR_INVALID_ARGUMENT;
+    }
+    *out = nullptr;
+    spira_dev *dev = nullptr;
+    int rc = guarded(nullptr, [&] {
+        int count = 0;
+        SPIRA_HIP(hipGetDeviceCount(&count));
+        if (device_ordinal < 0 || device_ordinal >= count)
+            fail(SPIRA_ERR_INVALID_ARGUMENT, "spira_dev_create: no such device ordinal");
+        SPIRA_HIP(hipSetDevice(device_ordinal));
+        dev = new spira_dev();
+        dev->device = device_ordinal;
+        hipDeviceProp_t prop;
+        SPIRA_HIP(hipGetDeviceProperties(&prop, device_ordinal));
+        dev->arch_major = prop.major;
+        dev->arch_minor = prop.minor;
+        dev->n_cu = prop.multiProcessorCount;
+        if (stream) {
+            dev->stream = (hipStream_t)stream;
+        } else {
+            SPIRA_HIP(hipStreamCreateWithFlags(&dev->stream, hipStreamNonBlocking));
+            dev->own_stream = true;
+        }
+        dev->ws.stream = dev->stream;
+    });
+    if (rc != SPIRA_OK) {
+        delete dev;
+        return rc;
+    }
+    *out = dev;
+    return SPIRA_OK;
+}
+
+void spira_dev_destroy(spira_dev *dev)
+{
+    if (!dev) return;
+    (void)hipSetDevice(dev->device);
+    (void)hipStreamSynchronize(dev->stream);
+    dev->ws.release();
+    if (dev->own_stream) (void)hipStreamDestroy(dev->stream);
+    delete dev;
+}
+
+const char *spira_dev_last_error(const spira_dev *dev)
+{
+    return dev ? dev->last_error.c_str() : g_last_error.c_str();
+}
+
+void *spira_dev_stream(const spira_dev *dev) { return dev ? (void *)dev->stream : nullptr; }
+
+int spira_dev_synchronize(spira_dev *dev)
+{
+    if (!dev) return SPIRA_ERR_INVALID_ARGUMENT;
+    return guarded(dev, [&] { dev_sync(dev); });
+}
+
+int spira_dcsr_create(spira_dev *dev, uint64_t n_rows, uint64_t n_cols, spira_dcsr **out)
+{
+    if (!dev || !out) {
+        g_last_error = "spira_dcsr_create: NULL argument";
+        return SPIRA_ERR_INVALID_ARGUMENT;
+    }
+    *out = nullptr;
+    return guarded(dev, [&] {
+        spira_dcsr *A = new spira_dcsr();
+        A->dev = dev;
+        A->n_rows = n_rows;
+        A->n_cols = n_cols;
+        *out = A;
+    });
+}
+
+void spira_dcsr_destroy(spira_dcsr *A)
+{
+    if (!A) return;
+    (void)hipSetDevice(A->dev->device);
+    (void)hipStreamSynchronize(A->dev->stream);
+    free_dcsr_arrays(A);
+    delete A;
+}
+
+int spira_dcsr_info(const spira_dcsr *A, uint64_t *n_rows, uint64_t *n_cols, uint64_t *nnz, int *built)
+{
+    if (!A) return SPIRA_ERR_INVALID_ARGUMENT;
+    if (n_rows) *n_rows = A->n_rows;
+    if (n_cols) *n_cols = A->n_cols;
+    if (nnz) *nnz = A->nnz;
+    if (built) *built = A->built ? 1 : 0;
+    return SPIRA_OK;
+}
+
+int spira_dcsr_upload_f64(spira_dcsr *A, uint64_t nnz, const uint64_t *offsets, const uint32_t *cols,
+                          const double *vals, uint32_t flags)
+{
+    if (!A) return SPIRA_ERR_INVALID_ARGUMENT;
+    spira_dev *dev = A->dev;
+    return guarded(dev, [&] {
+        if (!offsets) fail(SPIRA_ERR_INVALID_ARGUMENT, "spira::matrix::load_csr: CSR offsets array is null");
+        if (nnz > 0 && (!cols || !vals)) fail(SPIRA_ERR_INVALID_ARGUMENT, "load_csr: cols/vals are null");
+        const bool in_dev = (flags & SPIRA_IN_DEVICE) != 0;
+        uint64_t *off = nullptr;
+        uint32_t *col = nullptr;
+        double *val = nullptr;
+        try {
+            SPIRA_HIP(hipMalloc(&off, (A->n_rows + 1) * sizeof(uint64_t)));
+            SPIRA_HIP(hipMalloc(&col, (nnz ? nnz : 1) * sizeof(uint32_t)));
+            SPIRA_HIP(hipMalloc(&val, (nnz ? nnz : 1) * sizeof(double)));
+            copy_in(dev, off, offsets, (A->n_rows + 1) * sizeof(uint64_t), in_dev);
+            copy_in(dev, col, cols, nnz * sizeof(uint32_t), in_dev);
+            copy_in(dev, val, vals, nnz * sizeof(double), in_dev);
+            bool zeros = false;
+            validate_csr_device(A, nnz, off, col, val, &zeros);
+            free_dcsr_arrays(A);
+            A->off = off;
+            A->col = col;
+            A->val = val;
+            A->nnz = nnz;
+            A->capacity = nnz;
+            A->built = true;
+            A->has_zeros = zeros;
+        } catch (...) {
+            (void)hipStreamSynchronize(dev->stream);
+            if (off) (void)hipFree(off);
+            if (col) (void)hipFree(col);
+            if (val) (void)hipFree(val);
+            throw;
+        }
+    });
+}
+
+int spira_dcsr_download_f64(const spira_dcsr *A, uint64_t *offsets, uint32_t *cols, double *vals, uint32_t flags)
+{
+    if (!A) return SPIRA_ERR_INVALID_ARGUMENT;
+    spira_dev *dev = A->dev;
+    return guarded(dev, [&] {
+        if (!A->built) fail(SPIRA_ERR_LOGIC, "csr(): no CSR has been built (lock() first)");
+        const hipMemcpyKind kind = (flags & SPIRA_OUT_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+        if (offsets)
+            SPIRA_HIP(hipMemcpyAsync(offsets, A->off, (A->n_rows + 1) * sizeof(uint64_t), kind, dev->stream));
+        if (cols && A->nnz)
+            SPIRA_HIP(hipMemcpyAsync(cols, A->col, A->nnz * sizeof(uint32_t), kind, dev->stream));
+        if (vals && A->nnz)
+            SPIRA_HIP(hipMemcpyAsync(vals, A->val, A->nnz * sizeof(double), kind, dev->stream));
+        dev_sync(dev);
+    });
+}
+
+int spira_lock_f64(spira_dcsr *A, const uint64_t *stage_ptr, const uint32_t *stage_col, const double *stage_val,
+                   const uint8_t *dirty, uint32_t flags, uint64_t *out_nnz)
+{
+    if (!A) return SPIRA_ERR_INVALID_ARGUMENT;
+    spira_dev *dev = A->dev;
+    return guarded(dev, [&] {
+        if (!stage_ptr) fail(SPIRA_ERR_INVALID_ARGUMENT, "lock: stage_ptr is NULL");
+        const bool in_dev = (flags & SPIRA_IN_DEVICE) != 0;
+        const uint64_t n = A->n_rows;
+        uint64_t m = 0;
+        if (in_dev) {
+            m = read_scalar(dev, stage_ptr + n);
+        } else {
+            m = stage_ptr[n];
+        }
+        if (m > 0 && (!stage_col || !stage_val)) fail(SPIRA_ERR_INVALID_ARGUMENT, "lock: staged arrays are NULL");
+        const uint64_t *d_ptr = stage_ptr;
+        const uint32_t *d_col = stage_col;
+        const double *d_val = stage_val;
+        const uint8_t *d_dirty = dirty;
+        if (!in_dev) {
+            uint64_t *p = (uint64_t *)dev->ws.get(WS_STAGE_PTR, (n + 1) * sizeof(uint64_t));
+            uint32_t *c = (uint32_t *)dev->ws.get(WS_STAGE_COL, m * sizeof(uint32_t));
+            double *v = (double *)dev->ws.get(WS_STAGE_VAL, m * sizeof(double));
+            copy_in(dev, p, stage_ptr, (n + 1) * sizeof(uint64_t), false);
+            copy_in(dev, c, stage_col, m * sizeof(uint32_t), false);
+            copy_in(dev, v, stage_val, m * sizeof(double), false);
+            d_ptr = p;
+            d_col = c;
+            d_val = v;
+            if (dirty) {
+                uint8_t *d = (uint8_t *)dev->ws.get(WS_X, n);
+                copy_in(dev, d, dirty, n, false);
+                d_dirty = d;
+            }
+        }
+        const uint64_t nnz = lock_device_f64(A, d_ptr, d_col, d_val, m, d_dirty);
+        if (out_nnz) *out_nnz = nnz;
+    });
+}
+
+int spira_spmv_f64(spira_dcsr *A, const double *x, uint64_t x_len, double *y, uint64_t y_len, int variant,
+                   uint32_t flags)
+{
+    if (!A) return SPIRA_ERR_INVALID_ARGUMENT;
+    spira_dev *dev = A->dev;
+    return guarded(dev, [&] {
+        // serial/spmv.hpp:150-158: sizes first, then the mode.
+        if (x_len != A->n_cols)
+            fail(SPIRA_ERR_INVALID_ARGUMENT,
+                 "The size of the input vector x does not match the number of columns of the matrix");
+        if (y_len != A->n_rows)
+            fail(SPIRA_ERR_INVALID_ARGUMENT,
+                 "The size of the output vector y does not match the number of rows of the matrix");
+        if (!A->built) fail(SPIRA_ERR_LOGIC, "spmv: matrix must be locked");
+        if (variant < SPIRA_SPMV_AUTO || variant >= SPIRA_SPMV_N_VARIANTS)
+            fail(SPIRA_ERR_INVALID_ARGUMENT, "spmv: unknown kernel variant");
+        if ((x_len && !x) || (y_len && !y)) fail(SPIRA_ERR_INVALID_ARGUMENT, "spmv: x or y is NULL");
+        const double *dx = x;
+        double *dy = y;
+        const bool xd = flags & SPIRA_X_DEVICE, yd = flags & SPIRA_Y_DEVICE;
+        if (!xd) {
+            double *t = (double *)dev->ws.get(WS_X, x_len * sizeof(double));
+            copy_in(dev, t, x, x_len * sizeof(double), false);
+            dx = t;
+        }
+        if (!yd) dy = (double *)dev->ws.get(WS_Y, y_len * sizeof(double));
+        spmv_launch_f64(A, dx, dy, variant);
+        if (!yd && y_len)
+            SPIRA_HIP(hipMemcpyAsync(y, dy, y_len * sizeof(double), hipMemcpyDeviceToHost, dev->stream));
+        if (!(flags & SPIRA_NO_SYNC) || !xd || !yd) dev_sync(dev);
+    });
+}
+
+int spira_spmv_choose(spira_dcsr *A, int *variant)
+{
+    if (!A || !variant) return SPIRA_ERR_INVALID_ARGUMENT;
+    return guarded(A->dev, [&] {
+        if (!A->built) fail(SPIRA_ERR_LOGIC, "spmv: matrix must be locked");
+        spmv_build_plan(A);
+        *variant = spmv_heuristic_variant(A);
+    });
+}
+
+int spira_spmv_autotune_f64(spira_dcsr *A, const double *x_dev, double *y_dev, int reps, float *ms, int *best)
+{
+    if (!A) return SPIRA_ERR_INVALID_ARGUMENT;
+    spira_dev *dev = A->dev;
+    return guarded(dev, [&] {
+        if (!A->built) fail(SPIRA_ERR_LOGIC, "spmv: matrix must be locked");
+        if (reps < 1) fail(SPIRA_ERR_INVALID_ARGUMENT, "autotune: reps must be >= 1");
+        if ((A->n_cols && !x_dev) || (A->n_rows && !y_dev)) fail(SPIRA_ERR_INVALID_ARGUMENT, "autotune: NULL vector");
+        spmv_build_plan(A);
+        hipEvent_t e0, e1;
+        SPIRA_HIP(hipEventCreate(&e0));
+        SPIRA_HIP(hipEventCreate(&e1));
+        int best_v = 0;
+        float best_ms = 0.0f;
+        try {
+            for (int v = 0; v < SPIRA_SPMV_N_VARIANTS; ++v) {
+                spmv_launch_f64(A, x_dev, y_dev, v);  // warm-up (builds the plan parts)
+                SPIRA_HIP(hipEventRecord(e0, dev->stream));
+                for (int r = 0; r < reps; ++r) spmv_launch_f64(A, x_dev, y_dev, v);
+                SPIRA_HIP(hipEventRecord(e1, dev->stream));
+                SPIRA_HIP(hipEventSynchronize(e1));
+                float t = 0.0f;
+                SPIRA_HIP(hipEventElapsedTime(&t, e0, e1));
+                t /= (float)reps;
+                if (ms) ms[v] = t;
+                if (v == 0 || t < best_ms) {
+                    best_ms = t;
+                    best_v = v;
+                }
+            }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        A->plan.variant = best_v;
+        if (best) *best = best_v;
+    });
+}
+
+int spira_spmv_set_variant(spira_dcsr *A, int variant)
+{
+    if (!A) return SPIRA_ERR_INVALID_ARGUMENT;
+    return guarded(A->dev, [&] {
+        if (variant < SPIRA_SPMV_AUTO || variant >= SPIRA_SPMV_N_VARIANTS)
+            fail(SPIRA_ERR_INVALID_ARGUMENT, "spmv: unknown kernel variant");
+        A->plan.variant = variant;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,210 @@
+// common.hpp -- shared device helpers and the internal host-side objects of
+// libspira_amd (device handle, device CSR, workspace).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/spira_amd.h"
+
+namespace spira_amd {
+
+// ---------------------------------------------------------------------------
+// Errors.  Every C-ABI entry point converts these into status codes; the codes
+// map 1:1 onto the exceptions the reference throws (invalid_argument,
+// logic_error, out_of_range) plus device and allocation failures.
+// ---------------------------------------------------------------------------
+struct status_error : std::runtime_error {
+    int code;
+    status_error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] inline void fail(int code, const std::string &msg) { throw status_error(code, msg); }
+
+inline void hip_check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) {
+        if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+            fail(SPIRA_ERR_NOMEM, std::string(what) + ": " + hipGetErrorString(e));
+        fail(SPIRA_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+#define SPIRA_HIP(call) ::spira_amd::hip_check((call), #call)
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+constexpr int kWave = 64;
+
+template <class T>
+__device__ __forceinline__ T ld_stream(const T *p)
+{
+    // Streaming (non-temporal) load: A's values and columns are touched once
+    // per SpMV, so they should not evict x from L2 / MALL.
+    return __builtin_nontemporal_load(p);
+}
+
+// Sum over aligned groups of G lanes (G a power of two <= 64).
+template <int G>
+__device__ __forceinline__ double group_sum(double v)
+{
+#pragma unroll
+    for (int s = G / 2; s > 0; s >>= 1) v += __shfl_xor(v, s, G);
+    return v;
+}
+
+__device__ __forceinline__ double group_sum_rt(double v, int g)
+{
+    for (int s = g / 2; s > 0; s >>= 1) v += __shfl_xor(v, s, 64);
+    return v;
+}
+
+template <int G>
+__device__ __forceinline__ float group_sum_f(float v)
+{
+#pragma unroll
+    for (int s = G / 2; s > 0; s >>= 1) v += __shfl_xor(v, s, G);
+    return v;
+}
+
+// is_zero(v) <=> |v| <= 0 (traits.hpp:73-88): +0.0 and -0.0 only; NaN and
+// subnormals are not zero.  Bit test, so no dependence on denormal flushing.
+__device__ __forceinline__ bool is_zero_f64(double v)
+{
+    return (__double_as_longlong(v) & 0x7fffffffffffffffll) == 0;
+}
+
+// Largest r in [lo, hi) with ptr[r] <= k (the row that owns element k when the
+// rows are non-empty).  ptr is non-decreasing.
+template <class OffT>
+__device__ __forceinline__ uint64_t row_of(const OffT *ptr, uint64_t lo, uint64_t hi, uint64_t k)
+{
+    while (hi - lo > 1) {
+        uint64_t mid = lo + (hi - lo) / 2;
+        if ((uint64_t)ptr[mid] <= k) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// First index in [lo, hi) with a[i] >= key (a strictly increasing).
+__device__ __forceinline__ uint64_t lower_bound_u32(const uint32_t *a, uint64_t lo, uint64_t hi, uint32_t key)
+{
+    while (lo < hi) {
+        uint64_t mid = lo + (hi - lo) / 2;
+        if (a[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+inline unsigned grid_for(uint64_t items, unsigned per_block, unsigned cap = 0x7fffffffu)
+{
+    uint64_t g = (items + per_block - 1) / per_block;
+    if (g == 0) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+// ---------------------------------------------------------------------------
+// Host objects
+// ---------------------------------------------------------------------------
+
+// Growable device scratch owned by a handle.  Slot i keeps its own buffer so
+// one operation can hold several at once.
+struct workspace {
+    static constexpr int kSlots = 16;
+    void *ptr[kSlots] = {};
+    size_t cap[kSlots] = {};
+    hipStream_t stream = nullptr;  // work that may still read a slot runs here
+    void *get(int slot, size_t bytes)
+    {
+        if (bytes == 0) bytes = 8;
+        if (cap[slot] < bytes) {
+            // Growing frees the old buffer: let queued kernels finish first.
+            if (ptr[slot]) {
+                SPIRA_HIP(hipStreamSynchronize(stream));
+                SPIRA_HIP(hipFree(ptr[slot]));
+            }
+            ptr[slot] = nullptr;
+            cap[slot] = 0;
+            size_t want = bytes + bytes / 4;
+            SPIRA_HIP(hipMalloc(&ptr[slot], want));
+            cap[slot] = want;
+        }
+        return ptr[slot];
+    }
+    void release()
+    {
+        for (int i = 0; i < kSlots; ++i) {
+            if (ptr[i]) (void)hipFree(ptr[i]);
+            ptr[i] = nullptr;
+            cap[i] = 0;
+        }
+    }
+};
+
+// Workspace slot assignment (one operation at a time per handle).
+enum ws_slot : int {
+    WS_X = 0,        // host-x staging
+    WS_Y = 1,        // host-y staging
+    WS_SCAN = 2,     // scan block sums
+    WS_STAGE_PTR = 3,
+    WS_STAGE_COL = 4,
+    WS_STAGE_VAL = 5,
+    WS_DED_COL = 6,
+    WS_DED_VAL = 7,
+    WS_FLAGS = 8,    // lock: packed flags of staged slots, then their prefix
+    WS_ROWS = 9,     // lock: deduped count per row
+    WS_MISC = 10,    // small control words
+    WS_MISC2 = 11,   // spmv plan: block-start prefix
+    WS_QPOS = 12,    // lock: insertion point of each staged slot in its old row
+    WS_BIG = 13,     // lock: rows with many staged inserts
+    WS_CNT = 14,     // lock: new row lengths
+    WS_PZ = 15,      // lock: prefix of dropped old zeros
+};
+
+}  // namespace spira_amd
+
+// SpMV plan cached on a device CSR (dropped by every lock/upload).
+struct spira_plan {
+    bool stats_valid = false;
+    uint64_t max_len = 0;        // longest row
+    double mean_len = 0.0;       // nnz / n_rows
+    double cv_len = 0.0;         // coefficient of variation of row lengths
+    uint32_t *off32 = nullptr;   // 32-bit copy of the offsets (nnz < 2^32)
+    uint32_t *blk[2] = {nullptr, nullptr}; // stream row blocks, chunk 1024 / 2048
+    uint32_t nblk[2] = {0, 0};
+    int32_t variant = -1;        // pinned AUTO choice (-1 = heuristic)
+};
+
+struct spira_dcsr {
+    spira_dev *dev = nullptr;
+    uint64_t n_rows = 0, n_cols = 0, nnz = 0, capacity = 0;
+    bool built = false;          // false until the first lock()/upload
+    bool has_zeros = false;      // CSR may hold explicit zeros (only via upload)
+    uint64_t *off = nullptr;     // n_rows+1
+    uint32_t *col = nullptr;     // capacity
+    double *val = nullptr;       // capacity
+    spira_plan plan;
+    // Reduced-precision value copies (built on demand, dropped on lock).
+    float *val_f32 = nullptr;
+    uint16_t *val_bf16 = nullptr;    // hi part for split storage as well
+    uint16_t *val_bf16_lo = nullptr; // lo part of split storage
+    uint32_t vals_ready = 0;         // bitmask of SPIRA_VAL_* present
+};
+
+struct spira_dev {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    spira_amd::workspace ws;
+    std::string last_error;
+    int arch_major = 0, arch_minor = 0;
+    int n_cu = 0;
+};

@@ -1,0 +1,623 @@
+// lock.hip -- device lock(): sort+dedup of the open-mode row buffers, then
+// build_csr (first lock) or merge_csr (later locks), bit-identical to
+// csr_build.hpp:32-98 / :126-488 and aos_array_buffer.hpp:76-101.
+//
+// Pipeline (all on the handle's stream):
+//   1. check      stage_ptr well formed, columns < n_cols, rows binned by the
+//                 number of staged inserts (<= 32, <= 8192, larger = error).
+//   2. dedup      per row: last write wins, ascending columns, zeros kept.
+//                 <= 32 inserts: one lane per row, O(k^2) ranking in registers
+//                 and L1.  <= 8192: one block per row, bitonic sort of
+//                 (column, insertion index) keys in LDS.
+//   3. flags      per deduped entry: nz = "value is not zero and the row is
+//                 dirty", match = "column already present in the old row"
+//                 (binary search); packed (match << 32 | nz) and scanned.
+//   4. counts     new row length: clean rows keep their length; dirty rows
+//                 old_len - matches + nz (- old zeros, if the old CSR can hold
+//                 explicit zeros); exclusive scan -> new 64-bit offsets.
+//   5. scatter    every old entry and every surviving new entry computes its
+//                 own output slot from the prefix sums (merge by ranking), so
+//                 the work is element-parallel and independent of row lengths.
+// The output arrays are exactly sized (capacity == nnz), like build_csr's
+// csr_storage(n_rows, total_nnz).
+#include "internal.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace spira_amd {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr uint32_t kSmallSeg = 32;
+constexpr uint32_t kMaxSeg = SPIRA_LOCK_MAX_ROW_INSERTS;  // 8192, LDS bitonic limit
+constexpr int kItems = 4;                                   // elements per thread in scatter passes
+
+enum err_bits : uint32_t {
+    ERR_OUT_OF_RANGE = 1u,
+    ERR_BAD_PTR = 2u,
+    ERR_TOO_LONG = 4u,
+    ERR_INTERNAL = 8u,  // a scatter slot fell outside its row (never expected)
+};
+
+__device__ __forceinline__ bool row_dirty(const uint8_t *dirty, const uint64_t *sptr, uint64_t r)
+{
+    if (dirty) return dirty[r] != 0;
+    return sptr[r + 1] != sptr[r];
+}
+
+// ---------------------------------------------------------------------------
+// 1. checks and binning
+// ---------------------------------------------------------------------------
+__global__ void k_check_ptr(const uint64_t *__restrict__ sptr, uint64_t n, uint64_t m, uint32_t *err,
+                            uint32_t *big_rows, uint32_t *n_big)
+{
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > n) return;
+    if (r == 0 && sptr[0] != 0) atomicOr(err, ERR_BAD_PTR);
+    if (r == n) {
+        if (sptr[n] != m) atomicOr(err, ERR_BAD_PTR);
+        return;
+    }
+    const uint64_t a = sptr[r], b = sptr[r + 1];
+    if (b < a || b > m) {
+        atomicOr(err, ERR_BAD_PTR);
+        return;
+    }
+    const uint64_t k = b - a;
+    if (k > kMaxSeg) atomicOr(err, ERR_TOO_LONG);
+    else if (k > kSmallSeg) big_rows[atomicAdd(n_big, 1u)] = (uint32_t)r;
+}
+
+__global__ void k_check_cols(const uint32_t *__restrict__ scol, uint64_t m, uint64_t n_cols, uint32_t *err)
+{
+    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < m && (uint64_t)scol[s] >= n_cols) atomicOr(err, ERR_OUT_OF_RANGE);
+}
+
+// ---------------------------------------------------------------------------
+// 2. dedup
+// ---------------------------------------------------------------------------
+
+// Rows with <= 32 staged inserts: one lane per row.  keep_i = "no later insert
+// to the same column" (the reverse + stable_sort + keep-first of
+// aos_array_buffer.hpp:81-96 keeps exactly the last write); the output slot of
+// a kept entry is the number of kept entries with a smaller column.
+__global__ __launch_bounds__(kThreads) void k_dedup_small(uint64_t n, const uint64_t *__restrict__ sptr,
+                                                           const uint32_t *__restrict__ scol,
+                                                           const double *__restrict__ sval,
+                                                           uint32_t *__restrict__ dcol, double *__restrict__ dval,
+                                                           uint32_t *__restrict__ dcnt)
+{
+    const uint64_t r = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (r >= n) return;
+    const uint64_t s0 = sptr[r];
+    const uint32_t k = (uint32_t)(sptr[r + 1] - s0);
+    if (k > kSmallSeg) return;  // handled by the block kernel
+    const uint32_t *c = scol + s0;
+    uint32_t keep = 0;
+    for (uint32_t i = 0; i < k; ++i) {
+        const uint32_t ci = c[i];
+        bool last = true;
+        for (uint32_t j = i + 1; j < k; ++j) last &= (c[j] != ci);
+        keep |= (last ? 1u : 0u) << i;
+    }
+    for (uint32_t i = 0; i < k; ++i) {
+        if (!((keep >> i) & 1u)) continue;
+        const uint32_t ci = c[i];
+        uint32_t pos = 0;
+        for (uint32_t j = 0; j < k; ++j) pos += (((keep >> j) & 1u) && c[j] < ci) ? 1u : 0u;
+        dcol[s0 + pos] = ci;
+        dval[s0 + pos] = sval[s0 + i];
+    }
+    dcnt[r] = (uint32_t)__builtin_popcount(keep);
+}
+
+__device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t *total)
+{
+    __shared__ uint32_t s_w[kThreads / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t inc = v;
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+        uint32_t t = __shfl_up(inc, s, 64);
+        if (lane >= s) inc += t;
+    }
+    if (lane == 63) s_w[wid] = inc;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) {
+        if (w < wid) base += s_w[w];
+        tot += s_w[w];
+    }
+    __syncthreads();
+    *total = tot;
+    return base + inc - v;
+}
+
+// Rows with 33..8192 staged inserts: one block per row (grid-stride over the
+// list).  Keys (column << 32 | insertion index) are unique, so any sort gives
+// the order of the reference's stable sort; the last key of each column run is
+// the last write.
+__global__ __launch_bounds__(kThreads) void k_dedup_big(const uint32_t *__restrict__ rows,
+                                                         const uint32_t *__restrict__ n_rows_list,
+                                                         const uint64_t *__restrict__ sptr,
+                                                         const uint32_t *__restrict__ scol,
+                                                         const double *__restrict__ sval,
+                                                         uint32_t *__restrict__ dcol, double *__restrict__ dval,
+                                                         uint32_t *__restrict__ dcnt)
+{
+    extern __shared__ uint64_t s_key[];  // kMaxSeg entries
+    const uint32_t count = *n_rows_list;
+    for (uint32_t li = blockIdx.x; li < count; li += gridDim.x) {
+        const uint32_t r = rows[li];
+        const uint64_t s0 = sptr[r];
+        const uint32_t k = (uint32_t)(sptr[r + 1] - s0);
+        uint32_t N = 64;
+        while (N < k) N <<= 1;
+        for (uint32_t i = threadIdx.x; i < N; i += kThreads)
+            s_key[i] = i < k ? (((uint64_t)scol[s0 + i]) << 32) | i : ~0ull;
+        __syncthreads();
+        for (uint32_t size = 2; size <= N; size <<= 1) {
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                for (uint32_t i = threadIdx.x; i < N / 2; i += kThreads) {
+                    const uint32_t lo = 2 * i - (i & (stride - 1));
+                    const uint32_t hi = lo + stride;
+                    const bool asc = (lo & size) == 0;
+                    const uint64_t a = s_key[lo], b = s_key[hi];
+                    if ((a > b) == asc) {
+                        s_key[lo] = b;
+                        s_key[hi] = a;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        uint32_t carry = 0;
+        for (uint32_t base = 0; base < k; base += kThreads) {
+            const uint32_t i = base + threadIdx.x;
+            bool keep = false;
+            uint64_t key = 0;
+            if (i < k) {
+                key = s_key[i];
+                keep = (i + 1 == k) || ((uint32_t)(s_key[i + 1] >> 32) != (uint32_t)(key >> 32));
+            }
+            uint32_t tot;
+            const uint32_t ex = block_excl_scan_u32(keep ? 1u : 0u, &tot);
+            if (keep) {
+                dcol[s0 + carry + ex] = (uint32_t)(key >> 32);
+                dval[s0 + carry + ex] = sval[s0 + (uint32_t)key];
+            }
+            carry += tot;
+        }
+        if (threadIdx.x == 0) dcnt[r] = carry;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Row lookup for element-parallel passes: each block narrows the row range of
+// its first and last element once, then every element searches that range.
+// ---------------------------------------------------------------------------
+struct row_range {
+    uint64_t lo, hi;  // rows [lo, hi] own the block's elements
+};
+
+__device__ __forceinline__ row_range block_rows(const uint64_t *ptr, uint64_t n_rows, uint64_t first,
+                                                uint64_t last)
+{
+    row_range rr;
+    rr.lo = row_of(ptr, 0, n_rows, first);
+    rr.hi = row_of(ptr, rr.lo, n_rows, last);
+    return rr;
+}
+
+// ---------------------------------------------------------------------------
+// 3. flags of deduped entries
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_ded_flags(uint64_t n, uint64_t m, const uint64_t *__restrict__ sptr,
+                                                         const uint32_t *__restrict__ dcnt,
+                                                         const uint32_t *__restrict__ dcol,
+                                                         const double *__restrict__ dval,
+                                                         const uint8_t *__restrict__ dirty, int merge,
+                                                         const uint64_t *__restrict__ old_off,
+                                                         const uint32_t *__restrict__ old_col,
+                                                         uint64_t *__restrict__ flags, uint32_t *__restrict__ qpos)
+{
+    const uint64_t first = (uint64_t)blockIdx.x * kThreads * kItems;
+    if (first >= m) return;
+    const uint64_t last = min(first + kThreads * kItems, m) - 1;
+    const row_range rr = block_rows(sptr, n, first, last);
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+        const uint64_t s = first + (uint64_t)it * kThreads + threadIdx.x;
+        if (s >= m) break;
+        const uint64_t r = row_of(sptr, rr.lo, rr.hi + 1, s);
+        const uint64_t j = s - sptr[r];
+        uint64_t f = 0;
+        uint32_t q = 0;
+        if (j < dcnt[r]) {
+            const bool d = merge ? row_dirty(dirty, sptr, r) : true;
+            const uint32_t c = dcol[s];
+            const bool nz = d && !is_zero_f64(dval[s]);
+            bool match = false;
+            if (merge && d) {
+                const uint64_t ob = old_off[r], oe = old_off[r + 1];
+                const uint64_t p = lower_bound_u32(old_col, ob, oe, c);
+                match = p < oe && old_col[p] == c;
+                q = (uint32_t)(p - ob);
+            }
+            f = ((uint64_t)(match ? 1u : 0u) << 32) | (nz ? 1u : 0u);
+        }
+        flags[s] = f;
+        if (merge) qpos[s] = q;
+    }
+}
+
+// Old entries that a dirty row drops because they are explicit zeros (and not
+// overwritten): z = 1.  Only run when the old CSR can hold zeros.
+__global__ __launch_bounds__(kThreads) void k_old_zero_flags(uint64_t n, uint64_t nnz_old,
+                                                              const uint64_t *__restrict__ old_off,
+                                                              const uint32_t *__restrict__ old_col,
+                                                              const double *__restrict__ old_val,
+                                                              const uint64_t *__restrict__ sptr,
+                                                              const uint32_t *__restrict__ dcnt,
+                                                              const uint32_t *__restrict__ dcol,
+                                                              const uint8_t *__restrict__ dirty,
+                                                              uint64_t *__restrict__ zf)
+{
+    const uint64_t first = (uint64_t)blockIdx.x * kThreads * kItems;
+    if (first >= nnz_old) return;
+    const uint64_t last = min(first + kThreads * kItems, nnz_old) - 1;
+    const row_range rr = block_rows(old_off, n, first, last);
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+        const uint64_t k = first + (uint64_t)it * kThreads + threadIdx.x;
+        if (k >= nnz_old) break;
+        const uint64_t r = row_of(old_off, rr.lo, rr.hi + 1, k);
+        uint64_t z = 0;
+        if (row_dirty(dirty, sptr, r) && is_zero_f64(old_val[k])) {
+            const uint64_t db = sptr[r], de = db + dcnt[r];
+            const uint32_t c = old_col[k];
+            const uint64_t p = lower_bound_u32(dcol, db, de, c);
+            const bool over = p < de && dcol[p] == c;
+            z = over ? 0 : 1;
+        }
+        zf[k] = z;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 4. row counts
+// ---------------------------------------------------------------------------
+__global__ void k_row_counts(uint64_t n, const uint64_t *__restrict__ sptr, const uint64_t *__restrict__ P,
+                             const uint8_t *__restrict__ dirty, int merge, const uint64_t *__restrict__ old_off,
+                             const uint64_t *__restrict__ PZ, uint64_t *__restrict__ cnt)
+{
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const uint64_t pa = P[sptr[r]], pb = P[sptr[r + 1]];
+    const uint64_t nz = (pb & 0xffffffffull) - (pa & 0xffffffffull);
+    const uint64_t mt = (pb >> 32) - (pa >> 32);
+    if (!merge) {
+        cnt[r] = nz;
+        return;
+    }
+    const uint64_t ob = old_off[r], oe = old_off[r + 1];
+    if (!row_dirty(dirty, sptr, r)) {
+        cnt[r] = oe - ob;
+        return;
+    }
+    uint64_t zeros = PZ ? (PZ[oe] - PZ[ob]) : 0;
+    cnt[r] = (oe - ob) - mt - zeros + nz;
+}
+
+// ---------------------------------------------------------------------------
+// 5. scatter passes
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_scatter_old(uint64_t n, uint64_t nnz_old,
+                                                           const uint64_t *__restrict__ old_off,
+                                                           const uint32_t *__restrict__ old_col,
+                                                           const double *__restrict__ old_val,
+                                                           const uint64_t *__restrict__ sptr,
+                                                           const uint32_t *__restrict__ dcnt,
+                                                           const uint32_t *__restrict__ dcol,
+                                                           const uint64_t *__restrict__ P,
+                                                           const uint64_t *__restrict__ PZ,
+                                                           const uint8_t *__restrict__ dirty,
+                                                           const uint64_t *__restrict__ new_off,
+                                                           uint32_t *__restrict__ new_col, double *__restrict__ new_val,
+                                                           uint32_t *__restrict__ err)
+{
+    const uint64_t first = (uint64_t)blockIdx.x * kThreads * kItems;
+    if (first >= nnz_old) return;
+    const uint64_t last = min(first + kThreads * kItems, nnz_old) - 1;
+    const row_range rr = block_rows(old_off, n, first, last);
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+        const uint64_t k = first + (uint64_t)it * kThreads + threadIdx.x;
+        if (k >= nnz_old) break;
+        const uint64_t r = row_of(old_off, rr.lo, rr.hi + 1, k);
+        const uint64_t ob = old_off[r];
+        const uint64_t i = k - ob;
+        const uint32_t c = ld_stream(old_col + k);
+        const double v = ld_stream(old_val + k);
+        uint64_t out;
+        if (!row_dirty(dirty, sptr, r)) {
+            out = new_off[r] + i;  // clean row: verbatim (csr_build.hpp:362-377)
+        } else {
+            const uint64_t db = sptr[r], de = db + dcnt[r];
+            const uint64_t p = de > db ? lower_bound_u32(dcol, db, de, c) : db;
+            if (p < de && dcol[p] == c) continue;  // buffer wins (:430-436)
+            if (PZ && is_zero_f64(v)) continue;    // old zero dropped (:420-421)
+            const uint64_t pa = P[db], pp = P[p];
+            const uint64_t before_m = (pp >> 32) - (pa >> 32);
+            const uint64_t before_nz = (pp & 0xffffffffull) - (pa & 0xffffffffull);
+            const uint64_t before_z = PZ ? (PZ[k] - PZ[ob]) : 0;
+            out = new_off[r] + i - before_m - before_z + before_nz;
+        }
+        // Every slot must fall inside the row's new range; never write outside.
+        if (out - new_off[r] >= new_off[r + 1] - new_off[r]) {
+            atomicOr(err, ERR_INTERNAL);
+            continue;
+        }
+        new_col[out] = c;
+        new_val[out] = v;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_scatter_ded(uint64_t n, uint64_t m, const uint64_t *__restrict__ sptr,
+                                                           const uint32_t *__restrict__ dcnt,
+                                                           const uint32_t *__restrict__ dcol,
+                                                           const double *__restrict__ dval,
+                                                           const uint64_t *__restrict__ P,
+                                                           const uint32_t *__restrict__ qpos,
+                                                           const uint64_t *__restrict__ old_off,
+                                                           const uint64_t *__restrict__ PZ, int merge,
+                                                           const uint64_t *__restrict__ new_off,
+                                                           uint32_t *__restrict__ new_col, double *__restrict__ new_val,
+                                                           uint32_t *__restrict__ err)
+{
+    const uint64_t first = (uint64_t)blockIdx.x * kThreads * kItems;
+    if (first >= m) return;
+    const uint64_t last = min(first + kThreads * kItems, m) - 1;
+    const row_range rr = block_rows(sptr, n, first, last);
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+        const uint64_t s = first + (uint64_t)it * kThreads + threadIdx.x;
+        if (s >= m) break;
+        const uint64_t ps = P[s];
+        const uint64_t ps1 = P[s + 1];
+        if (((ps1 - ps) & 0xffffffffull) == 0) continue;  // not a surviving entry
+        const uint64_t r = row_of(sptr, rr.lo, rr.hi + 1, s);
+        const uint64_t db = sptr[r];
+        const uint64_t pa = P[db];
+        const uint64_t before_nz = (ps & 0xffffffffull) - (pa & 0xffffffffull);
+        uint64_t out = new_off[r] + before_nz;
+        if (merge) {
+            const uint64_t q = qpos[s];
+            const uint64_t before_m = (ps >> 32) - (pa >> 32);
+            const uint64_t ob = old_off[r];
+            const uint64_t before_z = PZ ? (PZ[ob + q] - PZ[ob]) : 0;
+            out += q - before_m - before_z;
+        }
+        if (out - new_off[r] >= new_off[r + 1] - new_off[r]) {
+            atomicOr(err, ERR_INTERNAL);
+            continue;
+        }
+        new_col[out] = dcol[s];
+        new_val[out] = dval[s];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// load_csr validation (matrix.hpp:524-575), first failure in the reference's
+// check order wins: key = row << 32 | step, step 0 = offsets of the row,
+// step 1 + 2t = bounds of entry t, step 2 + 2t = order of entry t.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_validate_rows(uint64_t n, uint64_t n_cols,
+                                                             const uint64_t *__restrict__ off,
+                                                             unsigned long long *__restrict__ first_bad)
+{
+    const uint64_t r = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (r >= n) return;
+    if (off[r] > off[r + 1]) atomicMin(first_bad, (unsigned long long)(r << 32));
+}
+
+__global__ __launch_bounds__(kThreads) void k_validate_entries(uint64_t n, uint64_t nnz, uint64_t n_cols,
+                                                                const uint64_t *__restrict__ off,
+                                                                const uint32_t *__restrict__ col,
+                                                                const double *__restrict__ val,
+                                                                unsigned long long *__restrict__ first_bad,
+                                                                uint32_t *__restrict__ zero_seen)
+{
+    const uint64_t first = (uint64_t)blockIdx.x * kThreads * kItems;
+    if (first >= nnz) return;
+    const uint64_t last = min(first + kThreads * kItems, nnz) - 1;
+    const row_range rr = block_rows(off, n, first, last);
+    bool z = false;
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+        const uint64_t k = first + (uint64_t)it * kThreads + threadIdx.x;
+        if (k >= nnz) break;
+        const uint64_t r = row_of(off, rr.lo, rr.hi + 1, k);
+        const uint64_t t = k - off[r];
+        const uint64_t tt = t < 0x7fffffffull ? t : 0x7fffffffull;
+        if ((uint64_t)col[k] >= n_cols) atomicMin(first_bad, (unsigned long long)((r << 32) | (1 + 2 * tt)));
+        else if (t > 0 && col[k] <= col[k - 1])
+            atomicMin(first_bad, (unsigned long long)((r << 32) | (2 + 2 * tt)));
+        z |= is_zero_f64(val[k]);
+    }
+    if (z) atomicOr(zero_seen, 1u);
+}
+
+}  // namespace
+
+void validate_csr_device(spira_dcsr *A, uint64_t nnz, const uint64_t *d_off, const uint32_t *d_col,
+                         const double *d_val, bool *has_zeros)
+{
+    spira_dev *dev = A->dev;
+    const uint64_t n = A->n_rows;
+    uint64_t ends[2];
+    SPIRA_HIP(hipMemcpyAsync(&ends[0], d_off, sizeof(uint64_t), hipMemcpyDeviceToHost, dev->stream));
+    SPIRA_HIP(hipMemcpyAsync(&ends[1], d_off + n, sizeof(uint64_t), hipMemcpyDeviceToHost, dev->stream));
+    dev_sync(dev);
+    if (ends[0] != 0) fail(SPIRA_ERR_INVALID_ARGUMENT, "spira::matrix::load_csr: CSR offsets[0] must be zero");
+    if (ends[1] != nnz)
+        fail(SPIRA_ERR_INVALID_ARGUMENT, "spira::matrix::load_csr: CSR offsets[n_rows] does not equal nnz");
+    unsigned long long *bad = (unsigned long long *)dev->ws.get(WS_MISC, 2 * sizeof(unsigned long long));
+    uint32_t *zs = (uint32_t *)(bad + 1);
+    const unsigned long long init[2] = {~0ull, 0ull};
+    SPIRA_HIP(hipMemcpyAsync(bad, init, sizeof(init), hipMemcpyHostToDevice, dev->stream));
+    if (n > 0) {
+        k_validate_rows<<<grid_for(n, kThreads), kThreads, 0, dev->stream>>>(n, A->n_cols, d_off, bad);
+        SPIRA_HIP(hipGetLastError());
+    }
+    // Entries are only checked once offsets are known to be monotone (the
+    // reference stops at the first bad row as well).
+    unsigned long long h[2];
+    SPIRA_HIP(hipMemcpyAsync(h, bad, sizeof(h), hipMemcpyDeviceToHost, dev->stream));
+    dev_sync(dev);
+    if (h[0] == ~0ull && nnz > 0) {
+        k_validate_entries<<<grid_for(nnz, kThreads * kItems), kThreads, 0, dev->stream>>>(n, nnz, A->n_cols, d_off,
+                                                                                           d_col, d_val, bad, zs);
+        SPIRA_HIP(hipGetLastError());
+        SPIRA_HIP(hipMemcpyAsync(h, bad, sizeof(h), hipMemcpyDeviceToHost, dev->stream));
+        dev_sync(dev);
+    }
+    if (h[0] != ~0ull) {
+        const uint64_t step = h[0] & 0xffffffffull;
+        if (step == 0) fail(SPIRA_ERR_INVALID_ARGUMENT, "spira::matrix::load_csr: CSR offsets are not non-decreasing");
+        if (step & 1ull) fail(SPIRA_ERR_INVALID_ARGUMENT, "spira::matrix::load_csr: column index out of bounds");
+        fail(SPIRA_ERR_INVALID_ARGUMENT,
+             "spira::matrix::load_csr: CSR row entries are not strictly sorted by column");
+    }
+    uint32_t zflag = 0;
+    std::memcpy(&zflag, &h[1], sizeof(uint32_t));
+    *has_zeros = zflag != 0;
+}
+
+uint64_t lock_device_f64(spira_dcsr *A, const uint64_t *sptr, const uint32_t *scol, const double *sval,
+                         uint64_t m, const uint8_t *dirty)
+{
+    spira_dev *dev = A->dev;
+    hipStream_t st = dev->stream;
+    const uint64_t n = A->n_rows;
+    const bool merge = A->built;
+    if (m >= (1ull << 32)) fail(SPIRA_ERR_UNSUPPORTED, "lock: more than 2^32-1 staged inserts in one call");
+
+    // ---- 1. checks + binning -------------------------------------------------
+    uint32_t *ctl = (uint32_t *)dev->ws.get(WS_MISC, 4 * sizeof(uint32_t));  // [err, n_big]
+    uint32_t *big = (uint32_t *)dev->ws.get(WS_BIG, (n + 1) * sizeof(uint32_t));
+    SPIRA_HIP(hipMemsetAsync(ctl, 0, 4 * sizeof(uint32_t), st));
+    k_check_ptr<<<grid_for(n + 1, kThreads), kThreads, 0, st>>>(sptr, n, m, ctl, big, ctl + 1);
+    SPIRA_HIP(hipGetLastError());
+    if (m > 0) {
+        k_check_cols<<<grid_for(m, kThreads), kThreads, 0, st>>>(scol, m, A->n_cols, ctl);
+        SPIRA_HIP(hipGetLastError());
+    }
+    uint32_t hctl[2];
+    SPIRA_HIP(hipMemcpyAsync(hctl, ctl, sizeof(hctl), hipMemcpyDeviceToHost, st));
+    dev_sync(dev);
+    if (hctl[0] & ERR_BAD_PTR)
+        fail(SPIRA_ERR_INVALID_ARGUMENT, "lock: stage_ptr must start at 0, be non-decreasing and end at m");
+    if (hctl[0] & ERR_OUT_OF_RANGE) fail(SPIRA_ERR_OUT_OF_RANGE, "spira::matrix: col_index out of range");
+    if (hctl[0] & ERR_TOO_LONG)
+        fail(SPIRA_ERR_UNSUPPORTED, "lock: a row has more staged inserts than SPIRA_LOCK_MAX_ROW_INSERTS");
+    const uint32_t n_big = hctl[1];
+
+    // ---- 2. dedup ----------------------------------------------------------
+    uint32_t *dcol = (uint32_t *)dev->ws.get(WS_DED_COL, m * sizeof(uint32_t));
+    double *dval = (double *)dev->ws.get(WS_DED_VAL, m * sizeof(double));
+    uint32_t *dcnt = (uint32_t *)dev->ws.get(WS_ROWS, (n + 1) * sizeof(uint32_t) + 8);
+    if (n > 0) {
+        k_dedup_small<<<grid_for(n, kThreads), kThreads, 0, st>>>(n, sptr, scol, sval, dcol, dval, dcnt);
+        SPIRA_HIP(hipGetLastError());
+    }
+    if (n_big > 0) {
+        const unsigned g = std::min<uint32_t>(n_big, 4096u);
+        k_dedup_big<<<g, kThreads, kMaxSeg * sizeof(uint64_t), st>>>(big, ctl + 1, sptr, scol, sval, dcol, dval,
+                                                                     dcnt);
+        SPIRA_HIP(hipGetLastError());
+    }
+
+    // ---- 3. flags + scan ----------------------------------------------------
+    uint64_t *P = (uint64_t *)dev->ws.get(WS_FLAGS, (m + 1) * sizeof(uint64_t));
+    uint32_t *qpos = merge ? (uint32_t *)dev->ws.get(WS_QPOS, m * sizeof(uint32_t)) : nullptr;
+    if (m > 0) {
+        k_ded_flags<<<grid_for(m, kThreads * kItems), kThreads, 0, st>>>(n, m, sptr, dcnt, dcol, dval, dirty,
+                                                                         merge ? 1 : 0, A->off, A->col, P, qpos);
+        SPIRA_HIP(hipGetLastError());
+    }
+    scan_excl_u64(dev, P, P, m);
+
+    // Old explicit zeros (only possible after load_csr with zeros).
+    uint64_t *PZ = nullptr;
+    if (merge && A->has_zeros && A->nnz > 0) {
+        PZ = (uint64_t *)dev->ws.get(WS_PZ, (A->nnz + 1) * sizeof(uint64_t));
+        k_old_zero_flags<<<grid_for(A->nnz, kThreads * kItems), kThreads, 0, st>>>(n, A->nnz, A->off, A->col, A->val,
+                                                                                   sptr, dcnt, dcol, dirty, PZ);
+        SPIRA_HIP(hipGetLastError());
+        scan_excl_u64(dev, PZ, PZ, A->nnz);
+    }
+
+    // ---- 4. counts + offsets ----------------------------------------------
+    uint64_t *cnt = (uint64_t *)dev->ws.get(WS_CNT, (n + 1) * sizeof(uint64_t));
+    uint64_t *new_off = nullptr;
+    SPIRA_HIP(hipMalloc(&new_off, (n + 1) * sizeof(uint64_t)));
+    try {
+        if (n > 0) {
+            k_row_counts<<<grid_for(n, kThreads), kThreads, 0, st>>>(n, sptr, P, dirty, merge ? 1 : 0, A->off, PZ,
+                                                                     cnt);
+            SPIRA_HIP(hipGetLastError());
+        }
+        scan_excl_u64(dev, cnt, new_off, n);
+        const uint64_t nnz_new = read_scalar(dev, new_off + n);
+
+        // ---- 5. scatter ------------------------------------------------------
+        uint32_t *new_col = nullptr;
+        double *new_val = nullptr;
+        SPIRA_HIP(hipMalloc(&new_col, std::max<uint64_t>(nnz_new, 1) * sizeof(uint32_t)));
+        if (hipMalloc(&new_val, std::max<uint64_t>(nnz_new, 1) * sizeof(double)) != hipSuccess) {
+            (void)hipFree(new_col);
+            fail(SPIRA_ERR_NOMEM, "lock: out of device memory for the new CSR");
+        }
+        if (merge && A->nnz > 0) {
+            k_scatter_old<<<grid_for(A->nnz, kThreads * kItems), kThreads, 0, st>>>(
+                n, A->nnz, A->off, A->col, A->val, sptr, dcnt, dcol, P, PZ, dirty, new_off, new_col, new_val, ctl);
+            SPIRA_HIP(hipGetLastError());
+        }
+        if (m > 0) {
+            k_scatter_ded<<<grid_for(m, kThreads * kItems), kThreads, 0, st>>>(
+                n, m, sptr, dcnt, dcol, dval, P, qpos, A->off, PZ, merge ? 1 : 0, new_off, new_col, new_val, ctl);
+            SPIRA_HIP(hipGetLastError());
+        }
+        uint32_t herr = 0;
+        SPIRA_HIP(hipMemcpyAsync(&herr, ctl, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        dev_sync(dev);
+        if (herr & ERR_INTERNAL) {
+            (void)hipFree(new_col);
+            (void)hipFree(new_val);
+            fail(SPIRA_ERR_DEVICE, "lock: internal error, a merged entry fell outside its row");
+        }
+        // install (the old arrays are released only after the kernels finished)
+        free_dcsr_arrays(A);
+        A->off = new_off;
+        A->col = new_col;
+        A->val = new_val;
+        A->nnz = nnz_new;
+        A->capacity = nnz_new;
+        A->built = true;
+        // Dirty rows come out without zeros; only clean rows can keep the
+        // explicit zeros an upload brought in.
+        if (!merge) A->has_zeros = false;
+        return nnz_new;
+    } catch (...) {
+        (void)hipFree(new_off);
+        throw;
+    }
+}
+
+}  // namespace spira_amd

@@ -1,0 +1,156 @@
+"""Seeded synthetic matrices for the benchmark configurations.
+
+Each generator returns numpy arrays in the reference's SoA CSR convention
+(u64 offsets, u32 columns sorted and distinct within a row, f64 values) or an
+open-mode insert stream (row, col, val in insertion order).  Seeds follow the
+reference benchmark: base seed 42 (bench/spira_bench.cpp:40).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+BASE_SEED = 42
+
+
+def _sorted_distinct_rows(rng, lens, n_cols, lo=None, hi=None):
+    """Per row i, lens[i] distinct sorted columns, uniform in [lo_i, hi_i)
+    (default [0, n_cols)).  Rows are generated as a padded 2-D block, sorted
+    along rows, and rows with a repeated column are redrawn."""
+    n = lens.size
+    width = int(lens.max()) if n else 0
+    out = np.empty((n, max(width, 1)), np.uint32)
+    lo = np.zeros(n, np.int64) if lo is None else lo.astype(np.int64)
+    hi = np.full(n, n_cols, np.int64) if hi is None else hi.astype(np.int64)
+    todo = np.arange(n)
+    pos = np.arange(max(width, 1))[None, :]
+    span = (hi[todo] - lo[todo])[:, None]
+    draw = lo[todo][:, None] + (rng.random((todo.size, max(width, 1))) * span).astype(np.int64)
+    draw = np.minimum(draw, hi[todo][:, None] - 1).astype(np.uint32)
+    valid = pos < lens[todo][:, None]
+    draw[~valid] = np.iinfo(np.uint32).max
+    draw.sort(axis=1)
+    dup = ((draw[:, 1:] == draw[:, :-1]) & valid[:, 1:]).any(axis=1)
+    out[:] = draw
+    # rows that drew a repeated column: sample without replacement instead
+    for i in np.nonzero(dup)[0]:
+        L = int(lens[i])
+        row = np.sort(rng.choice(int(hi[i] - lo[i]), size=L, replace=False) + lo[i]).astype(np.uint32)
+        out[i, :L] = row
+        out[i, L:] = np.iinfo(np.uint32).max
+    return out
+
+
+def _pack(lens, padded):
+    """Flatten a padded (n, w) column block with row lengths lens into CSR."""
+    n = lens.size
+    off = np.zeros(n + 1, np.uint64)
+    np.cumsum(lens, out=off[1:])
+    mask = np.arange(padded.shape[1])[None, :] < lens[:, None]
+    return off, padded[mask]
+
+
+def random_uniform_rows(n_rows, n_cols, lo, hi, seed=BASE_SEED, chunk=1 << 20):
+    """n_rows rows, nnz/row uniform in [lo, hi], distinct uniform columns,
+    values U(-1, 1).  Returns (offsets, cols, vals)."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(lo, hi + 1, size=n_rows).astype(np.int64)
+    off = np.zeros(n_rows + 1, np.uint64)
+    np.cumsum(lens, out=off[1:])
+    cols = np.empty(int(off[-1]), np.uint32)
+    for a in range(0, n_rows, chunk):
+        b = min(a + chunk, n_rows)
+        blk = _sorted_distinct_rows(rng, lens[a:b], n_cols)
+        _, c = _pack(lens[a:b], blk)
+        cols[int(off[a]):int(off[b])] = c
+    vals = rng.uniform(-1.0, 1.0, size=cols.size)
+    return off, cols, vals
+
+
+def laplacian_2d(nx, ny=None):
+    """5-point Laplacian on an nx x ny grid, Dirichlet boundary: diagonal 4,
+    neighbours -1, row-major grid numbering."""
+    ny = nx if ny is None else ny
+    n = nx * ny
+    i = np.arange(n, dtype=np.int64)
+    gx, gy = i % nx, i // nx
+    nb = [
+        (i - nx, gy > 0),
+        (i - 1, gx > 0),
+        (i, np.ones(n, bool)),
+        (i + 1, gx < nx - 1),
+        (i + nx, gy < ny - 1),
+    ]
+    lens = sum(m.astype(np.int64) for _, m in nb)
+    off = np.zeros(n + 1, np.uint64)
+    np.cumsum(lens, out=off[1:])
+    cols = np.empty(int(off[-1]), np.uint32)
+    vals = np.empty(int(off[-1]), np.float64)
+    cursor = off[:-1].astype(np.int64).copy()
+    for k, (c, m) in enumerate(nb):  # neighbours in ascending column order
+        idx = cursor[m]
+        cols[idx] = c[m]
+        vals[idx] = 4.0 if k == 2 else -1.0
+        cursor[m] += 1
+    return off, cols, vals
+
+
+def insert_stream_uniform(n_rows, n_cols, per_row, seed=BASE_SEED):
+    """Open-mode insert stream: per_row distinct uniform columns per row,
+    rows inserted in row-major order, values U(-1, 1).  (rows, cols, vals)."""
+    rng = np.random.default_rng(seed)
+    lens = np.full(n_rows, per_row, np.int64)
+    blk = _sorted_distinct_rows(rng, lens, n_cols)
+    # insertion order within a row is the draw order, not sorted: shuffle
+    perm = np.argsort(rng.random(blk.shape), axis=1)
+    blk = np.take_along_axis(blk, perm, axis=1)
+    rows = np.repeat(np.arange(n_rows, dtype=np.int64), per_row)
+    cols = blk[:, :per_row].reshape(-1)
+    vals = rng.uniform(-1.0, 1.0, size=rows.size)
+    return rows, cols, vals
+
+
+def relock_workload(n, nnz_per_row=16, inserts_per_row=2, overwrite_frac=0.25, seed=BASE_SEED):
+    """Re-lock merge: a locked n x n matrix with nnz_per_row distinct random
+    columns per row, then inserts_per_row inserts per row in row-major order of
+    which exactly round(overwrite_frac * total) hit an existing column of their
+    row; the others hit columns new to the row and distinct from each other.
+    Returns ((off, cols, vals), (rows, cols, vals) insert stream)."""
+    rng = np.random.default_rng(seed)
+    lens = np.full(n, nnz_per_row, np.int64)
+    blk = _sorted_distinct_rows(rng, np.full(n, nnz_per_row + inserts_per_row, np.int64), n)
+    # the first nnz_per_row columns (random subset) form the locked row, the
+    # rest are guaranteed-new columns for the inserts
+    perm = np.argsort(rng.random(blk.shape), axis=1)
+    blk = np.take_along_axis(blk, perm, axis=1)
+    base = np.sort(blk[:, :nnz_per_row], axis=1)
+    fresh = blk[:, nnz_per_row:nnz_per_row + inserts_per_row]
+    off, cols = _pack(lens, base)
+    vals = rng.uniform(-1.0, 1.0, size=cols.size)
+    total = n * inserts_per_row
+    n_over = int(round(overwrite_frac * total))
+    over = np.zeros(total, bool)
+    over[rng.permutation(total)[:n_over]] = True
+    over = over.reshape(n, inserts_per_row)
+    pick = rng.integers(0, nnz_per_row, size=(n, inserts_per_row))
+    ins_cols = np.where(over, np.take_along_axis(base, pick, axis=1), fresh).astype(np.uint32)
+    ins_rows = np.repeat(np.arange(n, dtype=np.int64), inserts_per_row)
+    ins_vals = rng.uniform(-1.0, 1.0, size=total)
+    return (off, cols, vals), (ins_rows, ins_cols.reshape(-1), ins_vals)
+
+
+def dense_x(n, seed=BASE_SEED ^ 0xC0FFEE):
+    """x ~ U(-1, 1) with the reference's x sub-seed (bench/spira_bench.cpp:268)."""
+    return np.random.default_rng(seed).uniform(-1.0, 1.0, size=n)
+
+
+def spmv_bytes(n_rows, n_cols, nnz, s_val=8, s_col=4, s_ptr=4, s_x=8, s_y=8, k=1):
+    """Algorithmic bytes per SpMV call, the reference's traffic model
+    (BENCHMARKS.md:173-187, bench/spira_bench.cpp:302-307) at device widths:
+    nnz*(s_val+s_col) + (n_rows+1)*s_ptr + k*(n_cols*s_x + n_rows*s_y)."""
+    return nnz * (s_val + s_col) + (n_rows + 1) * s_ptr + k * (n_cols * s_x + n_rows * s_y)
+
+
+def lock_bytes(n_rows, nnz_old, n_staged, nnz_new, s_col=4, s_val=8):
+    """Algorithmic bytes of one lock(): read old CSR + read staged + write new."""
+    return (nnz_old * (s_col + s_val) + (n_rows + 1) * 8 + n_staged * (4 + 8) + (n_rows + 1) * 8
+            + nnz_new * (s_col + s_val) + (n_rows + 1) * 8)

@@ -1,0 +1,266 @@
+"""Device lock() against the oracle, bit-exact (offsets, cols, vals), through
+the C-ABI: first lock (build_csr) and re-locks (merge_csr), dirty masks,
+explicit zeros from load_csr, +-0.0 deletes, NaN kept, rows with many staged
+inserts, the reference's lifecycle known answers, and error codes."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import spira_amd
+from spira_amd import workloads as W
+from oracle_model import GOLDEN, brute_lock, csr_equal, load_kats, random_stream, run_scenario, stage
+
+pytestmark = pytest.mark.gpu
+KATS = load_kats()
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = spira_amd.Device(0)
+    yield d
+    d.close()
+
+
+@pytest.mark.parametrize("case", KATS["lifecycle"], ids=lambda c: c["source"])
+def test_lifecycle_known_answers_on_device(dev, case):
+    if "threads" in case:
+        mat = spira_amd.ParallelMatrix(case["n_rows"], case["n_cols"], case["threads"], dev)
+    else:
+        mat = spira_amd.Matrix(case["n_rows"], case["n_cols"], dev)
+    run_scenario(mat, case)
+
+
+@pytest.mark.parametrize("case", KATS["build_csr"], ids=lambda c: c["source"])
+def test_build_csr_known_answers_on_device(dev, case):
+    n = len(case["rows"])
+    rows, cols, vals = [], [], []
+    for r, entries in enumerate(case["rows"]):
+        for c, v in entries:
+            rows.append(r)
+            cols.append(c)
+            vals.append(v)
+    A = spira_amd.DeviceCSR(dev, n, case["n_cols"])
+    A.lock(*stage(n, rows, cols, vals))
+    off, col, val = A.download()
+    assert off.tolist() == case["offsets"]
+    assert col.tolist() == case["cols"]
+    assert val.tolist() == case["vals"]
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_lock_multicycle_random_bit_exact(dev, seed):
+    rng = np.random.default_rng(1000 + seed)
+    n_rows = int(rng.integers(1, 3000))
+    n_cols = int(rng.integers(1, 5000))
+    A = spira_amd.DeviceCSR(dev, n_rows, n_cols)
+    old = None
+    for cycle in range(4):
+        n_ins = int(rng.integers(0, 20 * n_rows))
+        rows, cols, vals = random_stream(rng, n_rows, n_cols, n_ins, zero_frac=0.15,
+                                         col_pool=int(rng.integers(1, n_cols + 1)))
+        ptr, c, v = stage(n_rows, rows, cols, vals)
+        dirty = None
+        if old is not None and cycle == 2:
+            dirty = (rng.random(n_rows) < 0.3).astype(np.uint8)
+            dirty[np.bincount(rows, minlength=n_rows) > 0] = 1
+        nnz = A.lock(ptr, c, v, dirty=dirty)
+        exp = oracle.lock(ptr, c, v, old=old, dirty=dirty)
+        got = A.download()
+        assert nnz == exp[1].size
+        assert csr_equal(got, exp), f"seed {seed} cycle {cycle}"
+        old = exp
+
+
+def test_lock_matches_bruteforce_small(dev):
+    rng = np.random.default_rng(77)
+    for trial in range(20):
+        n_rows, n_cols = int(rng.integers(1, 12)), int(rng.integers(1, 12))
+        A = spira_amd.DeviceCSR(dev, n_rows, n_cols)
+        old = None
+        for cycle in range(3):
+            rows, cols, vals = random_stream(rng, n_rows, n_cols, int(rng.integers(0, 40)))
+            ptr, c, v = stage(n_rows, rows, cols, vals)
+            A.lock(ptr, c, v)
+            exp = brute_lock(n_rows, rows, cols, vals, old=old)
+            got = A.download()
+            assert csr_equal(got, exp)
+            old = got
+
+
+def test_lock_nan_and_signed_zero(dev):
+    rows = [0, 0, 0, 0, 1, 1]
+    cols = [4, 2, 4, 7, 1, 1]
+    vals = [1.0, float("nan"), -0.0, 2.0, 3.0, 0.0]
+    ptr, c, v = stage(2, rows, cols, vals)
+    A = spira_amd.DeviceCSR(dev, 2, 8)
+    A.lock(ptr, c, v)
+    assert csr_equal(A.download(), oracle.lock(ptr, c, v))
+
+
+@pytest.mark.parametrize("k", [33, 64, 257, 1000, 4096, 8192])
+def test_lock_rows_with_many_staged_inserts(dev, k):
+    """Rows past the one-lane path (> 32 staged inserts) use the LDS sort."""
+    rng = np.random.default_rng(k)
+    n_rows, n_cols = 5, 3000
+    rows = np.concatenate([np.full(k, 1), np.full(7, 3), np.full(k // 2 + 40, 4)])
+    rng.shuffle(rows)
+    cols = rng.integers(0, n_cols, rows.size)
+    vals = rng.uniform(-1, 1, rows.size)
+    vals[rng.random(rows.size) < 0.1] = 0.0
+    ptr, c, v = stage(n_rows, rows, cols, vals)
+    A = spira_amd.DeviceCSR(dev, n_rows, n_cols)
+    A.lock(ptr, c, v)
+    exp = oracle.lock(ptr, c, v)
+    assert csr_equal(A.download(), exp)
+    rows2 = np.full(k, 1)
+    cols2 = rng.integers(0, n_cols, k)
+    vals2 = rng.uniform(-1, 1, k)
+    ptr2, c2, v2 = stage(n_rows, rows2, cols2, vals2)
+    A.lock(ptr2, c2, v2)
+    assert csr_equal(A.download(), oracle.lock(ptr2, c2, v2, old=exp))
+
+
+def test_lock_after_load_csr_with_explicit_zeros(dev):
+    rng = np.random.default_rng(5)
+    n, nc = 500, 400
+    off, col, val = W.random_uniform_rows(n, nc, 0, 20, seed=9)
+    val = val.copy()
+    val[rng.random(val.size) < 0.25] = 0.0
+    A = spira_amd.DeviceCSR(dev, n, nc)
+    A.upload(off, col, val)
+    old = (off, col, val)
+    for cycle in range(3):
+        rows, cols, vals = random_stream(rng, n, nc, 800, zero_frac=0.2, col_pool=60)
+        ptr, c, v = stage(n, rows, cols, vals)
+        dirty = (rng.random(n) < 0.2).astype(np.uint8)
+        dirty[np.bincount(rows, minlength=n) > 0] = 1
+        A.lock(ptr, c, v, dirty=dirty)
+        exp = oracle.lock(ptr, c, v, old=old, dirty=dirty)
+        assert csr_equal(A.download(), exp), cycle
+        old = exp
+
+
+def test_open_lock_without_inserts_is_identity(dev):
+    off, col, val = W.random_uniform_rows(1000, 1000, 0, 9, seed=4)
+    A = spira_amd.DeviceCSR(dev, 1000, 1000)
+    A.upload(off, col, val)
+    empty = np.zeros(1001, np.uint64)
+    A.lock(empty, np.zeros(0, np.uint32), np.zeros(0))
+    assert csr_equal(A.download(), (off, col, val))
+
+
+def test_lock_empty_matrix_and_zero_rows(dev):
+    A = spira_amd.DeviceCSR(dev, 4, 5)
+    assert A.lock(np.zeros(5, np.uint64), np.zeros(0, np.uint32), np.zeros(0)) == 0
+    off, col, val = A.download()
+    assert off.tolist() == [0, 0, 0, 0, 0] and col.size == 0
+    B = spira_amd.DeviceCSR(dev, 0, 3)
+    assert B.lock(np.zeros(1, np.uint64), np.zeros(0, np.uint32), np.zeros(0)) == 0
+    assert B.download()[0].tolist() == [0]
+
+
+def test_lock_errors(dev):
+    A = spira_amd.DeviceCSR(dev, 3, 4)
+    ptr, c, v = stage(3, [0, 1], [1, 4], [1.0, 2.0])  # column 4 >= n_cols
+    with pytest.raises(spira_amd.OutOfRange):
+        A.lock(ptr, c, v)
+    bad_ptr = np.array([0, 2, 1, 2], np.uint64)
+    with pytest.raises(spira_amd.InvalidArgument):
+        A.lock(bad_ptr, np.array([0, 1], np.uint32), np.array([1.0, 2.0]))
+    k = spira_amd.LOCK_MAX_ROW_INSERTS + 1
+    ptr, c, v = stage(3, np.zeros(k, np.int64), np.zeros(k, np.uint32), np.ones(k))
+    with pytest.raises(spira_amd.UnsupportedInput):
+        A.lock(ptr, c, v)
+    assert not A.built  # failed locks leave the handle untouched
+    m = spira_amd.Matrix(2, 3, dev)
+    with pytest.raises(spira_amd.OutOfRange):
+        m.insert(2, 0, 1.0)
+    with pytest.raises(spira_amd.OutOfRange):
+        m.insert(0, 3, 1.0)
+    m.lock()
+    with pytest.raises(spira_amd.LogicError):
+        m.insert(0, 0, 1.0)
+
+
+def test_load_csr_validation_messages(dev):
+    A = spira_amd.DeviceCSR(dev, 2, 4)
+    cases = [
+        ((np.array([1, 1, 2]), [0, 1], [1.0, 2.0]), "offsets[0] must be zero"),
+        ((np.array([0, 1, 3]), [0, 1], [1.0, 2.0]), "does not equal nnz"),
+        ((np.array([0, 2, 1]), [0], [1.0]), "non-decreasing"),
+        ((np.array([0, 1, 2]), [0, 4], [1.0, 2.0]), "out of bounds"),
+        ((np.array([0, 2, 2]), [3, 3], [1.0, 2.0]), "strictly sorted"),
+    ]
+    for (off, col, val), msg in cases:
+        with pytest.raises(spira_amd.InvalidArgument, match=msg):
+            A.upload(off, col, val)
+    assert not A.built
+
+
+def test_relock_config3_scaled_then_spmv(dev):
+    (off, col, val), (rows, cols, vals) = W.relock_workload(1 << 16)
+    A = spira_amd.DeviceCSR(dev, 1 << 16, 1 << 16)
+    A.upload(off, col, val)
+    ptr, c, v = stage(1 << 16, rows, cols, vals)
+    nnz = A.lock(ptr, c, v)
+    exp = oracle.lock(ptr, c, v, old=(off, col, val))
+    assert nnz == exp[1].size
+    assert csr_equal(A.download(), exp)
+    x = W.dense_x(1 << 16)
+    y = A.spmv(x)
+    assert oracle.normwise_rel_err(y, oracle.spmv_f64(*exp, x)) <= 1e-12
+
+
+def test_relock_config3_full_size(dev):
+    """1,048,576 rows x 16 random nnz, 2 inserts/row, 25% overwrite: bit-exact."""
+    n = 1 << 20
+    (off, col, val), (rows, cols, vals) = W.relock_workload(n)
+    A = spira_amd.DeviceCSR(dev, n, n)
+    A.upload(off, col, val)
+    ptr, c, v = stage(n, rows, cols, vals)
+    nnz = A.lock(ptr, c, v)
+    exp = oracle.lock(ptr, c, v, old=(off, col, val))
+    assert nnz == exp[1].size == 16 * n + 2 * n - (2 * n) // 4
+    assert csr_equal(A.download(), exp)
+
+
+def test_stress_known_answer_on_device(dev, tmp_path):
+    """The reference's stress test (tests/large_matrix_stress_tests.cpp:11-90)
+    on the device: nnz == map size, CSR == map, y within 1e-10."""
+    exe = tmp_path / "gen_stress"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", os.path.join(GOLDEN, "gen_stress.cpp"), "-o",
+                    str(exe)], check=True)
+    out = subprocess.run([str(exe), str(tmp_path)], check=True, capture_output=True, text=True)
+    rd = lambda name, dt: np.fromfile(str(tmp_path / name), dt)
+    m = spira_amd.Matrix(10000, 10000, dev)
+    m.insert_many(rd("ins_row.u64", np.uint64).astype(np.int64), rd("ins_col.u64", np.uint64),
+                  rd("ins_val.f64", np.float64))
+    m.lock()
+    assert m.nnz() == int(out.stdout.strip())
+    assert csr_equal(m.csr(), (rd("exp_off.u64", np.uint64), rd("exp_col.u64", np.uint64),
+                               rd("exp_val.f64", np.float64)))
+    x, y_exp = rd("x.f64", np.float64), rd("y_exp.f64", np.float64)
+    y = np.empty(10000)
+    spira_amd.spmv(m, x, y)
+    tol = 1e-10 + 1e-10 * np.maximum(np.abs(y), np.abs(y_exp))
+    assert np.all(np.abs(y - y_exp) <= tol)
+
+
+def test_parallel_matrix_partitions_are_slices(dev):
+    pm = spira_amd.ParallelMatrix(10, 10, 3, dev)
+    rng = np.random.default_rng(2)
+    rows, cols, vals = random_stream(rng, 10, 10, 60, zero_frac=0.0)
+    pm.insert_many(rows, cols, vals)
+    pm.lock()
+    off, col, val = pm.csr()
+    assert pm.partition_bounds() == [0, 3, 6, 10]
+    for t in range(3):
+        po, pc, pv = pm.partition_at(t)
+        a, b = pm.partition_bounds()[t:t + 2]
+        assert po[0] == 0 and po.size == b - a + 1
+        assert np.array_equal(pc, col[int(off[a]):int(off[b])])
+    with pytest.raises(spira_amd.OutOfRange):
+        pm.partition_at(3)

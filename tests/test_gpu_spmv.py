@@ -1,0 +1,177 @@
+"""Device SpMV (FP64) against the oracle, through the C-ABI.
+
+Tolerance: normwise relative error ||y - y_ref||_2 / ||y_ref||_2 <= 1e-12
+against the oracle's sequential FP64 loop, plus the reference stress test's
+per-row bound |y - y_ref| <= 1e-10 + 1e-10 * max(|y|, |y_ref|)
+(tests/large_matrix_stress_tests.cpp:80-88).  Small integer-valued known
+answers must match exactly.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import spira_amd
+from spira_amd import workloads as W
+from oracle_model import load_kats, stage
+
+pytestmark = pytest.mark.gpu
+
+KATS = load_kats()
+VARIANTS = sorted(spira_amd.VARIANTS.values())
+NORMWISE_TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = spira_amd.Device(0)
+    yield d
+    d.close()
+
+
+def _upload(dev, off, col, val, n_cols):
+    A = spira_amd.DeviceCSR(dev, len(off) - 1, n_cols)
+    A.upload(off, col, val)
+    return A
+
+
+def _check(y, off, col, val, x):
+    y_ref = oracle.spmv_f64(off, col, val, x)
+    err = oracle.normwise_rel_err(y, y_ref)
+    assert err <= NORMWISE_TOL, err
+    tol = 1e-10 + 1e-10 * np.maximum(np.abs(y), np.abs(y_ref))
+    assert np.all(np.abs(y - y_ref) <= tol)
+    return err
+
+
+@pytest.mark.parametrize("case", KATS["spmv"], ids=lambda c: c["source"])
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_spmv_known_answers(dev, case, variant):
+    ins = case["inserts"]
+    ptr, c, v = stage(case["n_rows"], [t[0] for t in ins], [t[1] for t in ins], [t[2] for t in ins])
+    A = spira_amd.DeviceCSR(dev, case["n_rows"], case["n_cols"])
+    A.lock(ptr, c, v)
+    y = A.spmv(np.array(case["x"]), variant=variant)
+    assert y.tolist() == case["y"]
+
+
+@pytest.mark.parametrize("case", KATS["dot"], ids=lambda c: c["source"])
+def test_single_row_dot_known_answers(dev, case):
+    nnz = len(case["vals"])
+    if len(set(case["cols"])) != nnz:
+        pytest.skip("repeated column indices are not a valid CSR row (kernel-level case only)")
+    order = np.argsort(case["cols"])
+    off = np.array([0, nnz], np.uint64)
+    col = np.array(case["cols"], np.uint32)[order]
+    val = np.array(case["vals"])[order]
+    A = _upload(dev, off, col, val, len(case["x"]))
+    for variant in VARIANTS:
+        y = A.spmv(np.array(case["x"]), variant=variant)
+        assert y[0] == case["expect"]
+
+
+SHAPES = [
+    # (n_rows, n_cols, min_len, max_len, seed)
+    (1, 1, 0, 1, 1),
+    (7, 5, 0, 5, 2),
+    (1000, 1000, 0, 0, 3),          # every row empty
+    (3000, 2000, 0, 12, 4),         # many empty rows
+    (4096, 4096, 5, 5, 5),          # uniform short rows
+    (5000, 70000, 8, 32, 6),        # 2b-like
+    (2000, 10000, 60, 70, 7),       # rows just above a wavefront
+    (300, 200000, 900, 1100, 8),    # rows around the LDS chunk
+]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: f"{s[0]}x{s[1]}_{s[2]}-{s[3]}")
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_spmv_random_shapes(dev, shape, variant):
+    n_rows, n_cols, lo, hi, seed = shape
+    off, col, val = W.random_uniform_rows(n_rows, n_cols, lo, hi, seed=seed)
+    A = _upload(dev, off, col, val, n_cols)
+    x = np.random.default_rng(seed).uniform(-1, 1, n_cols)
+    y = A.spmv(x, variant=variant)
+    _check(y, off, col, val, x)
+
+
+def _powerlaw_csr(n, n_cols, seed, long_rows=()):
+    rng = np.random.default_rng(seed)
+    lens = np.minimum(rng.zipf(1.8, n), 3000).astype(np.int64)
+    for i, L in long_rows:
+        lens[i] = L
+    off = np.zeros(n + 1, np.uint64)
+    np.cumsum(lens, out=off[1:])
+    cols = np.concatenate([np.sort(rng.choice(n_cols, size=int(L), replace=False)) for L in lens]).astype(np.uint32)
+    vals = rng.standard_normal(cols.size)
+    return off, cols, vals
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_spmv_power_law_and_very_long_rows(dev, variant):
+    off, col, val = _powerlaw_csr(4000, 100000, 11, long_rows=[(0, 50000), (1777, 20000), (3999, 9000)])
+    A = _upload(dev, off, col, val, 100000)
+    x = np.random.default_rng(3).uniform(-1, 1, 100000)
+    _check(A.spmv(x, variant=variant), off, col, val, x)
+
+
+def test_spmv_auto_and_device_pointers(dev):
+    import torch
+    off, col, val = W.random_uniform_rows(20000, 30000, 4, 40, seed=12)
+    A = _upload(dev, off, col, val, 30000)
+    x = np.random.default_rng(5).uniform(-1, 1, 30000)
+    xt = torch.from_numpy(x).cuda()
+    yt = torch.full((20000,), 7.0, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    A.spmv(xt, yt, device=True)
+    _check(yt.cpu().numpy(), off, col, val, x)
+    v = A.choose_variant()
+    assert 0 <= v < spira_amd.N_VARIANTS
+
+
+def test_spmv_laplacian_full_size_exact_row_sums(dev):
+    """2048^2 5-point Laplacian (4,194,304 rows): A @ 1 = 4 - (#neighbours),
+    exactly, for every variant; and linearity on random vectors."""
+    off, col, val = W.laplacian_2d(2048)
+    n = len(off) - 1
+    A = _upload(dev, off, col, val, n)
+    deg = np.diff(off.astype(np.int64)) - 1
+    expect = 4.0 - deg
+    ones = np.ones(n)
+    rng = np.random.default_rng(0)
+    x1, x2 = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+    for variant in VARIANTS:
+        assert np.array_equal(A.spmv(ones, variant=variant), expect), variant
+    y1, y2, y12 = A.spmv(x1), A.spmv(x2), A.spmv(x1 + x2)
+    assert oracle.normwise_rel_err(y1 + y2, y12) <= 1e-14
+    _check(y1, off, col, val, x1)
+
+
+def test_spmv_errors_follow_reference_order(dev):
+    case = KATS["spmv_errors"][0]
+    A = spira_amd.DeviceCSR(dev, case["n_rows"], case["n_cols"])  # never locked
+    for c in case["cases"]:
+        with pytest.raises(spira_amd.InvalidArgument):
+            A.spmv(np.ones(c["x_len"]), np.zeros(c["y_len"]))
+    with pytest.raises(spira_amd.LogicError):
+        A.spmv(np.ones(case["n_cols"]), np.zeros(case["n_rows"]))
+    m = spira_amd.Matrix(3, 4, dev)
+    with pytest.raises(spira_amd.LogicError):
+        spira_amd.spmv(m, np.ones(4), np.zeros(3))
+    with pytest.raises(spira_amd.InvalidArgument):
+        spira_amd.spmv(m, np.ones(3), np.zeros(3))
+
+
+def test_spmv_config1_parity(dev):
+    """10,000 x 10,000, 10 distinct uniform columns per row, built in open mode
+    via insert then lock(); CSR arrays bit-exact and y within tolerance."""
+    rows, cols, vals = W.insert_stream_uniform(10000, 10000, 10)
+    m = spira_amd.Matrix(10000, 10000, dev)
+    m.insert_many(rows, cols, vals)
+    m.lock()
+    ptr, c, v = stage(10000, rows, cols, vals)
+    exp = oracle.lock(ptr, c, v)
+    got = m.csr()
+    assert all(np.array_equal(a, b) for a, b in zip(got, exp))
+    x = W.dense_x(10000)
+    y = np.empty(10000)
+    spira_amd.spmv(m, x, y)
+    _check(y, *exp, x)
