@@ -103,6 +103,25 @@ _vp = ctypes.c_void_p
 
 _lib = None
 
+# Live handles, released in order (matrices, then devices) before the
+# interpreter tears down the HIP runtime; afterwards finalizers do nothing.
+import atexit
+import weakref
+
+_live_csr = weakref.WeakSet()
+_live_dev = weakref.WeakSet()
+_shutdown = False
+
+
+@atexit.register
+def _release_all():
+    global _shutdown
+    for A in list(_live_csr):
+        A.close()
+    for d in list(_live_dev):
+        d.close()
+    _shutdown = True
+
 
 def lib_path() -> str:
     return _LIB_PATH
@@ -163,6 +182,7 @@ class Device:
         if rc != OK:
             raise _ERR.get(rc, DeviceError)(lib().spira_dev_last_error(None).decode())
         self._h = h
+        _live_dev.add(self)
 
     @property
     def handle(self):
@@ -181,9 +201,12 @@ class Device:
         return lib().spira_dev_stream(self._h) or 0
 
     def close(self):
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and not _shutdown:
+            for A in list(_live_csr):  # matrices of this device go first
+                if A.dev is self:
+                    A.close()
             lib().spira_dev_destroy(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         try:
@@ -210,6 +233,7 @@ class DeviceCSR:
         h = _vp()
         dev.check(lib().spira_dcsr_create(dev.handle, n_rows, n_cols, ctypes.byref(h)))
         self._h = h
+        _live_csr.add(self)
         self.n_rows, self.n_cols = int(n_rows), int(n_cols)
 
     @property
@@ -305,9 +329,9 @@ class DeviceCSR:
         return best.value, {v: float(ms[v]) for v in range(N_VARIANTS)}
 
     def close(self):
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and not _shutdown:
             lib().spira_dcsr_destroy(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         try:

@@ -188,8 +188,8 @@ def test_lock_errors(dev):
 def test_load_csr_validation_messages(dev):
     A = spira_amd.DeviceCSR(dev, 2, 4)
     cases = [
-        ((np.array([1, 1, 2]), [0, 1], [1.0, 2.0]), "offsets[0] must be zero"),
-        ((np.array([0, 1, 3]), [0, 1], [1.0, 2.0]), "does not equal nnz"),
+        ((np.array([1, 1, 2]), [0, 1], [1.0, 2.0]), r"offsets\[0\] must be zero"),
+        ((np.array([0, 1, 3]), [0, 1], [1.0, 2.0]), r"offsets\[n_rows\] does not equal nnz"),
         ((np.array([0, 2, 1]), [0], [1.0]), "non-decreasing"),
         ((np.array([0, 1, 2]), [0, 4], [1.0, 2.0]), "out of bounds"),
         ((np.array([0, 2, 2]), [3, 3], [1.0, 2.0]), "strictly sorted"),
