@@ -36,6 +36,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="bound on the CPU-baseline sample")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--no-lock", action="store_true", help="skip the lock() measurement")
     p.add_argument("--variant", default="autotune", help="autotune | auto | <variant name>")
     return p.parse_args()
 
@@ -163,6 +164,41 @@ def main():
         if "autotune_ms" in info[name]:
             detail[name]["autotune_ms"] = info[name]["autotune_ms"]
 
+    # ---- lock(): re-lock merge, 1,048,576^2, 16 nnz/row + 2 inserts/row (25% overwrite) ----
+    lock_detail = None
+    if not args.no_lock:
+        n3 = 1 << 20
+        (o3, c3, v3), (r3, ic3, iv3) = W.relock_workload(n3)
+        sp3, sc3, sv3 = spira_amd.stage_from_coo(n3, r3, ic3, iv3)
+        L = spira_amd.DeviceCSR(dev, n3, n3)
+        with torch.cuda.stream(stream):
+            d_sp = torch.from_numpy(sp3.astype(np.int64)).to("cuda")
+            d_sc = torch.from_numpy(sc3.astype(np.int32)).to("cuda")
+            d_sv = torch.from_numpy(sv3).to("cuda")
+        stream.synchronize()
+        t_dev, t_host = [], []
+        nnz_new = 0
+        for rep in range(6):
+            L.upload(o3, c3, v3)
+            dev.synchronize()
+            t = time.perf_counter()
+            nnz_new = L.lock(d_sp, d_sc, d_sv, device=True)
+            t_dev.append(time.perf_counter() - t)
+            L.upload(o3, c3, v3)
+            dev.synchronize()
+            t = time.perf_counter()
+            L.lock(sp3, sc3, sv3)
+            t_host.append(time.perf_counter() - t)
+        lbytes = W.lock_bytes(n3, int(c3.size), int(sc3.size), nnz_new)
+        tdev = float(np.median(t_dev[1:]))
+        lock_detail = {"workload": "re-lock merge 1,048,576^2: 16 random nnz/row, then 2 inserts/row, 25% overwrite",
+                       "nnz_old": int(c3.size), "staged": int(sc3.size), "nnz_new": int(nnz_new),
+                       "bytes_per_lock": lbytes, "ms_device_inputs": round(1e3 * tdev, 3),
+                       "GB_s": round(lbytes / tdev / 1e9, 1),
+                       "ms_host_inputs_pcie_inclusive": round(1e3 * float(np.median(t_host[1:])), 3)}
+        L.close()
+        lock_src = (o3, c3, v3, sp3, sc3, sv3, n3, lbytes)
+
     achieved = info[dominant]["bytes"] / (kern_ms * 1e-3) / 1e9
     traffic = profile_traffic(dominant + ":" + info[dominant]["variant"])
     roofline = {"bound": "hbm", "kernel": f"spmv {info[dominant]['variant']} on {dominant}",
@@ -214,6 +250,23 @@ def main():
                "ms_per_step": round(1e3 * t_cpu / reps, 3)}
         for p in pools.values():
             p.close()
+        if lock_detail is not None:
+            o3, c3, v3, sp3, sc3, sv3, n3, lbytes = lock_src
+            tl = []
+            for rep in range(3):
+                p = oracle.Pool(threads, n3, n3)
+                p.load_csr(o3, c3, v3)  # install the locked matrix as the partitions' CSR
+                t = time.perf_counter()
+                p.lock(sp3, sc3, sv3)
+                tl.append(time.perf_counter() - t)
+                if rep == 0:
+                    got = p.export_csr()
+                    exp = oracle.lock(sp3, sc3, sv3, old=(o3, c3, v3))
+                    lock_detail["bit_exact_vs_oracle"] = bool(
+                        all(np.array_equal(a.view(np.uint8), b.view(np.uint8)) for a, b in zip(got, exp)))
+                p.close()
+            lock_detail["cpu_baseline_ms"] = round(1e3 * float(np.median(tl)), 2)
+            lock_detail["cpu_baseline_threads"] = threads
 
     out = {
         "metric": "SpMV GB/s & % HBM peak; batched mixed-prec GFLOP/s vs plain FP64 at set error",
@@ -236,6 +289,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "detail": detail,
+        "lock": lock_detail,
     }
     print(json.dumps(out), flush=True)
 

@@ -54,7 +54,11 @@ typedef struct spira_dcsr spira_dcsr; /* device CSR: u64 offsets, u32 cols, f64 
 #define SPIRA_SPMV_VEC64 6        /* one wavefront per row */
 #define SPIRA_SPMV_STREAM1K 7     /* LDS-staged row blocks of <= 1024 rows+nnz */
 #define SPIRA_SPMV_STREAM2K 8     /* LDS-staged row blocks of <= 2048 rows+nnz */
-#define SPIRA_SPMV_N_VARIANTS 9
+#define SPIRA_SPMV_XSLICE1 9      /* column slice per XCD, 1 lane per row, then a sum of 8 partials */
+#define SPIRA_SPMV_XSLICE2 10     /* ... 2 lanes per row */
+#define SPIRA_SPMV_XSLICE4 11     /* ... 4 lanes per row */
+#define SPIRA_SPMV_XSTREAM 12     /* column slice per XCD, LDS row blocks per slice */
+#define SPIRA_SPMV_N_VARIANTS 13
 
 /* ---- device handle --------------------------------------------------------
  * Replaces the reference's CPU dispatch role (dispatch.cpp:34-67,

@@ -55,10 +55,10 @@ NO_SYNC = 16
 SPMV_AUTO = -1
 VARIANTS = {
     "vec1": 0, "vec2": 1, "vec4": 2, "vec8": 3, "vec16": 4, "vec32": 5, "vec64": 6,
-    "stream1k": 7, "stream2k": 8,
+    "stream1k": 7, "stream2k": 8, "xslice1": 9, "xslice2": 10, "xslice4": 11, "xstream": 12,
 }
 VARIANT_NAMES = {v: k for k, v in VARIANTS.items()}
-N_VARIANTS = 9
+N_VARIANTS = 13
 LOCK_MAX_ROW_INSERTS = 8192
 
 

@@ -180,6 +180,14 @@ struct spira_plan {
     uint32_t *off32 = nullptr;   // 32-bit copy of the offsets (nnz < 2^32)
     uint32_t *blk[2] = {nullptr, nullptr}; // stream row blocks, chunk 1024 / 2048
     uint32_t nblk[2] = {0, 0};
+    // XSLICE: a slice-major copy of A, one column slice per XCD
+    uint32_t *xs_off = nullptr;  // kSlices*n_rows+1 offsets into xs_col/xs_val
+    uint32_t *xs_col = nullptr;
+    double *xs_val = nullptr;
+    double *xs_part = nullptr;   // kSlices*n_rows partial sums
+    uint32_t *xs_blk = nullptr;  // XSTREAM row-block lists of all slices
+    uint32_t *xs_blk_start = nullptr;  // kSlices+1 list starts
+    uint32_t xs_nblk_max = 0;
     int32_t variant = -1;        // pinned AUTO choice (-1 = heuristic)
 };
 

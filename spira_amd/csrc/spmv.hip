@@ -82,17 +82,14 @@ __device__ __forceinline__ double block_sum(double v)
     return t;
 }
 
+// One row block [r0, r1) of a CSR (off/col/val), y[r] written for its rows.
 template <int C, class OffT>
-__global__ __launch_bounds__(kThreads) void k_spmv_stream(const uint32_t *__restrict__ blk_row,
-                                                           const OffT *__restrict__ off,
-                                                           const uint32_t *__restrict__ col,
-                                                           const double *__restrict__ val,
-                                                           const double *__restrict__ x, double *__restrict__ y)
+__device__ __forceinline__ void stream_block(uint32_t r0, uint32_t r1, const OffT *__restrict__ off,
+                                             const uint32_t *__restrict__ col, const double *__restrict__ val,
+                                             const double *__restrict__ x, double *__restrict__ y)
 {
     __shared__ double s_prod[2 * C];
     __shared__ uint32_t s_off[C + 1];
-    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
-    const uint32_t r0 = blk_row[b], r1 = blk_row[b + 1];
     const uint32_t nr = r1 - r0;
     const uint64_t k0 = off[r0], k1 = off[r1];
 
@@ -147,6 +144,138 @@ __global__ __launch_bounds__(kThreads) void k_spmv_stream(const uint32_t *__rest
     }
 }
 
+template <int C, class OffT>
+__global__ __launch_bounds__(kThreads) void k_spmv_stream(const uint32_t *__restrict__ blk_row,
+                                                           const OffT *__restrict__ off,
+                                                           const uint32_t *__restrict__ col,
+                                                           const double *__restrict__ val,
+                                                           const double *__restrict__ x, double *__restrict__ y)
+{
+    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+    stream_block<C, OffT>(blk_row[b], blk_row[b + 1], off, col, val, x, y);
+}
+
+// ---------------------------------------------------------------------------
+// XSLICE<G>: the columns are cut into kSlices slices, one per XCD, and the
+// plan keeps a slice-major copy of A (slice s holds every row's entries whose
+// column falls in slice s, rows in order).  Hardware block b runs on XCD
+// b % 8, so it takes slice b % 8: each XCD gathers only from its 1/8 of x,
+// which stays in that XCD's L2 instead of missing to the MALL on almost every
+// gather.  Each slice writes a partial y; k_xslice_sum adds the 8 partials in
+// slice order.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kSlices = 8;
+
+template <int G>
+__global__ __launch_bounds__(kThreads) void k_spmv_xslice(uint64_t n_rows, const uint32_t *__restrict__ xoff,
+                                                           const uint32_t *__restrict__ xcol,
+                                                           const double *__restrict__ xval,
+                                                           const double *__restrict__ x, double *__restrict__ part)
+{
+    const uint32_t s = blockIdx.x % kSlices;
+    const uint64_t rb = blockIdx.x / kSlices;
+    const uint64_t row = rb * (kThreads / G) + threadIdx.x / G;
+    const uint32_t lane = threadIdx.x % G;
+    if (row >= n_rows) return;
+    const uint64_t base = (uint64_t)s * n_rows + row;
+    const uint32_t b = xoff[base], e = xoff[base + 1];
+    double acc0 = 0.0, acc1 = 0.0;
+    uint32_t k = b + lane;
+    for (; k + G < e; k += 2 * G) {
+        const uint32_t c0 = ld_stream(xcol + k), c1 = ld_stream(xcol + k + G);
+        const double v0 = ld_stream(xval + k), v1 = ld_stream(xval + k + G);
+        acc0 += v0 * x[c0];
+        acc1 += v1 * x[c1];
+    }
+    if (k < e) acc0 += ld_stream(xval + k) * x[ld_stream(xcol + k)];
+    const double acc = group_sum<G>(acc0 + acc1);
+    if (lane == 0) part[base] = acc;
+}
+
+// XSTREAM<C>: the same slices, each processed with the LDS row-block kernel.
+// blk holds every slice's row-block list back to back (each list ends with
+// n_rows); blk_start[s] is where slice s's list begins.
+template <int C>
+__global__ __launch_bounds__(kThreads) void k_spmv_xstream(const uint32_t *__restrict__ blk,
+                                                            const uint32_t *__restrict__ blk_start, uint64_t n_rows,
+                                                            const uint32_t *__restrict__ xoff,
+                                                            const uint32_t *__restrict__ xcol,
+                                                            const double *__restrict__ xval,
+                                                            const double *__restrict__ x, double *__restrict__ part)
+{
+    const uint32_t s = blockIdx.x % kSlices;
+    const uint32_t j = blockIdx.x / kSlices;
+    const uint32_t a = blk_start[s];
+    const uint32_t nb = blk_start[s + 1] - a - 1;
+    if (j >= nb) return;
+    stream_block<C, uint32_t>(blk[a + j], blk[a + j + 1], xoff + (uint64_t)s * n_rows, xcol, xval, x,
+                              part + (uint64_t)s * n_rows);
+}
+
+__global__ __launch_bounds__(kThreads) void k_xslice_sum(uint64_t n_rows, const double *__restrict__ part,
+                                                          double *__restrict__ y)
+{
+    const uint64_t r = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (r >= n_rows) return;
+    double acc = ld_stream(part + r);
+#pragma unroll
+    for (uint32_t s = 1; s < kSlices; ++s) acc += ld_stream(part + (uint64_t)s * n_rows + r);
+    y[r] = acc;
+}
+
+// Plan: entries of row r per slice (columns are sorted, so one pass).
+__global__ __launch_bounds__(kThreads) void k_xslice_count(uint64_t n_rows, uint64_t width,
+                                                            const uint64_t *__restrict__ off,
+                                                            const uint32_t *__restrict__ col,
+                                                            uint64_t *__restrict__ cnt)
+{
+    const uint64_t r = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (r >= n_rows) return;
+    uint64_t c[kSlices] = {};
+    for (uint64_t k = off[r]; k < off[r + 1]; ++k) {
+        const uint32_t s = (uint32_t)(col[k] / width);
+#pragma unroll
+        for (uint32_t t = 0; t < kSlices; ++t) c[t] += (t == s) ? 1u : 0u;
+    }
+#pragma unroll
+    for (uint32_t t = 0; t < kSlices; ++t) cnt[(uint64_t)t * n_rows + r] = c[t];
+}
+
+// Plan: copy row r's entries into their slices at the scanned positions.
+__global__ __launch_bounds__(kThreads) void k_xslice_scatter(uint64_t n_rows, uint64_t width,
+                                                              const uint64_t *__restrict__ off,
+                                                              const uint32_t *__restrict__ col,
+                                                              const double *__restrict__ val,
+                                                              const uint64_t *__restrict__ pos,
+                                                              uint32_t *__restrict__ xcol, double *__restrict__ xval)
+{
+    const uint64_t r = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (r >= n_rows) return;
+    uint64_t w[kSlices];
+#pragma unroll
+    for (uint32_t t = 0; t < kSlices; ++t) w[t] = pos[(uint64_t)t * n_rows + r];
+    for (uint64_t k = off[r]; k < off[r + 1]; ++k) {
+        const uint32_t c = col[k];
+        const uint32_t s = (uint32_t)(c / width);
+        uint64_t d = 0;
+#pragma unroll
+        for (uint32_t t = 0; t < kSlices; ++t) {
+            if (t == s) {
+                d = w[t];
+                w[t] = d + 1;
+            }
+        }
+        xcol[d] = c;
+        xval[d] = val[k];
+    }
+}
+
+__global__ void k_u64_to_u32(const uint64_t *__restrict__ a, uint32_t *__restrict__ b, uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (uint32_t)a[i];
+}
+
 // ---------------------------------------------------------------------------
 // Plan kernels
 // ---------------------------------------------------------------------------
@@ -156,23 +285,65 @@ __global__ void k_off32(const uint64_t *__restrict__ off, uint32_t *__restrict__
     if (i < n) o32[i] = (uint32_t)off[i];
 }
 
+// Per-block max and sum of squares of the row lengths (one pass), then one
+// block folds the per-block results.
 __global__ __launch_bounds__(kThreads) void k_row_stats(const uint64_t *__restrict__ off, uint64_t n_rows,
-                                                         unsigned long long *__restrict__ out /* [max, sumsq] */)
+                                                         uint64_t *__restrict__ bmax, double *__restrict__ bsq)
 {
+    __shared__ uint64_t s_mx[kThreads / kWave];
+    __shared__ double s_sq[kThreads / kWave];
     const uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
-    uint64_t len = 0;
-    if (i < n_rows) len = off[i + 1] - off[i];
-    // wave reductions, then one atomic per wave
-    uint64_t mx = len;
-    double sq = (double)len * (double)len;
+    uint64_t mx = 0;
+    if (i < n_rows) mx = off[i + 1] - off[i];
+    double sq = (double)mx * (double)mx;
     for (int s = 32; s > 0; s >>= 1) {
-        uint64_t o = __shfl_xor(mx, s, 64);
+        const uint64_t o = __shfl_xor(mx, s, 64);
         mx = o > mx ? o : mx;
         sq += __shfl_xor(sq, s, 64);
     }
+    const int wid = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        atomicMax(&out[0], (unsigned long long)mx);
-        atomicAdd(reinterpret_cast<double *>(&out[1]), sq);
+        s_mx[wid] = mx;
+        s_sq[wid] = sq;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t m = 0;
+        double q = 0.0;
+        for (int w = 0; w < kThreads / kWave; ++w) {
+            m = s_mx[w] > m ? s_mx[w] : m;
+            q += s_sq[w];
+        }
+        bmax[blockIdx.x] = m;
+        bsq[blockIdx.x] = q;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_fold_stats(const uint64_t *__restrict__ bmax,
+                                                          const double *__restrict__ bsq, uint64_t nb,
+                                                          uint64_t *__restrict__ out_max, double *__restrict__ out_sq)
+{
+    __shared__ uint64_t s_mx[kThreads];
+    __shared__ double s_sq[kThreads];
+    uint64_t m = 0;
+    double q = 0.0;
+    for (uint64_t i = threadIdx.x; i < nb; i += kThreads) {
+        m = bmax[i] > m ? bmax[i] : m;
+        q += bsq[i];
+    }
+    s_mx[threadIdx.x] = m;
+    s_sq[threadIdx.x] = q;
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            s_mx[threadIdx.x] = s_mx[threadIdx.x + s] > s_mx[threadIdx.x] ? s_mx[threadIdx.x + s] : s_mx[threadIdx.x];
+            s_sq[threadIdx.x] += s_sq[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *out_max = s_mx[0];
+        *out_sq = s_sq[0];
     }
 }
 
@@ -235,12 +406,18 @@ void ensure_stats(spira_dcsr *A)
     if (P.stats_valid) return;
     spira_dev *dev = A->dev;
     if (A->n_rows > 0) {
-        unsigned long long *d = (unsigned long long *)dev->ws.get(WS_MISC, 2 * sizeof(unsigned long long));
-        SPIRA_HIP(hipMemsetAsync(d, 0, 2 * sizeof(unsigned long long), dev->stream));
-        k_row_stats<<<grid_for(A->n_rows, kThreads), kThreads, 0, dev->stream>>>(A->off, A->n_rows, d);
+        const uint64_t nb = grid_for(A->n_rows, kThreads);
+        char *w = (char *)dev->ws.get(WS_MISC, nb * 16 + 64);
+        uint64_t *bmax = (uint64_t *)w;
+        double *bsq = (double *)(w + nb * 8);
+        uint64_t *omax = (uint64_t *)(w + nb * 16);
+        double *osq = (double *)(w + nb * 16 + 8);
+        k_row_stats<<<(unsigned)nb, kThreads, 0, dev->stream>>>(A->off, A->n_rows, bmax, bsq);
         SPIRA_HIP(hipGetLastError());
-        unsigned long long h[2];
-        SPIRA_HIP(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, dev->stream));
+        k_fold_stats<<<1, kThreads, 0, dev->stream>>>(bmax, bsq, nb, omax, osq);
+        SPIRA_HIP(hipGetLastError());
+        uint64_t h[2];
+        SPIRA_HIP(hipMemcpyAsync(h, omax, sizeof(h), hipMemcpyDeviceToHost, dev->stream));
         dev_sync(dev);
         double sumsq;
         std::memcpy(&sumsq, &h[1], sizeof(double));
@@ -275,6 +452,102 @@ void ensure_blocks(spira_dcsr *A, int which)
     P.nblk[which] = (uint32_t)nb;
 }
 
+// Slice-major copy of A for the XSLICE kernels (one-time per locked matrix).
+void ensure_xslice(spira_dcsr *A)
+{
+    spira_plan &P = A->plan;
+    if (P.xs_off) return;
+    if (A->nnz >= (1ull << 32)) fail(SPIRA_ERR_UNSUPPORTED, "spmv: XSLICE needs nnz < 2^32");
+    spira_dev *dev = A->dev;
+    const uint64_t n = A->n_rows;
+    const uint64_t width = (A->n_cols + kSlices - 1) / kSlices > 0 ? (A->n_cols + kSlices - 1) / kSlices : 1;
+    const uint64_t N = kSlices * n;
+    uint64_t *pos = (uint64_t *)dev->ws.get(WS_MISC2, (N + 1) * sizeof(uint64_t));
+    k_xslice_count<<<grid_for(n, kThreads), kThreads, 0, dev->stream>>>(n, width, A->off, A->col, pos);
+    SPIRA_HIP(hipGetLastError());
+    scan_excl_u64(dev, pos, pos, N);
+    try {
+        SPIRA_HIP(hipMalloc(&P.xs_col, (A->nnz ? A->nnz : 1) * sizeof(uint32_t)));
+        SPIRA_HIP(hipMalloc(&P.xs_val, (A->nnz ? A->nnz : 1) * sizeof(double)));
+        SPIRA_HIP(hipMalloc(&P.xs_part, (N ? N : 1) * sizeof(double)));
+        SPIRA_HIP(hipMalloc(&P.xs_off, (N + 1) * sizeof(uint32_t)));
+    } catch (...) {
+        for (void *p : {(void *)P.xs_col, (void *)P.xs_val, (void *)P.xs_part, (void *)P.xs_off})
+            if (p) (void)hipFree(p);
+        P.xs_col = nullptr;
+        P.xs_val = nullptr;
+        P.xs_part = nullptr;
+        P.xs_off = nullptr;
+        throw;
+    }
+    k_xslice_scatter<<<grid_for(n, kThreads), kThreads, 0, dev->stream>>>(n, width, A->off, A->col, A->val, pos,
+                                                                          P.xs_col, P.xs_val);
+    SPIRA_HIP(hipGetLastError());
+    k_u64_to_u32<<<grid_for(N + 1, kThreads), kThreads, 0, dev->stream>>>(pos, P.xs_off, N + 1);
+    SPIRA_HIP(hipGetLastError());
+
+    // Row-block lists of every slice for XSTREAM (chunk 1024), back to back.
+    constexpr uint32_t C = 1024;
+    uint64_t *mark = (uint64_t *)dev->ws.get(WS_CNT, (n + 1) * sizeof(uint64_t));
+    std::vector<uint32_t> start(kSlices + 1, 0);
+    std::vector<uint32_t *> lists(kSlices, nullptr);
+    std::vector<uint64_t> nbs(kSlices, 0);
+    try {
+        for (uint32_t s = 0; s < kSlices; ++s) {
+            const uint64_t *so = pos + (uint64_t)s * n;
+            k_plan_mark<<<grid_for(n, kThreads), kThreads, 0, dev->stream>>>(so, n, C, mark);
+            SPIRA_HIP(hipGetLastError());
+            scan_excl_u64(dev, mark, mark, n);
+            nbs[s] = read_scalar(dev, mark + n);
+            start[s + 1] = start[s] + (uint32_t)nbs[s] + 1;
+            SPIRA_HIP(hipMalloc(&lists[s], (nbs[s] + 1) * sizeof(uint32_t)));
+            k_plan_scatter<<<grid_for(n, kThreads), kThreads, 0, dev->stream>>>(so, n, C, mark, lists[s]);
+            SPIRA_HIP(hipGetLastError());
+            dev_sync(dev);
+        }
+        SPIRA_HIP(hipMalloc(&P.xs_blk, (uint64_t)start[kSlices] * sizeof(uint32_t)));
+        SPIRA_HIP(hipMalloc(&P.xs_blk_start, (kSlices + 1) * sizeof(uint32_t)));
+        for (uint32_t s = 0; s < kSlices; ++s)
+            SPIRA_HIP(hipMemcpyAsync(P.xs_blk + start[s], lists[s], (nbs[s] + 1) * sizeof(uint32_t),
+                                     hipMemcpyDeviceToDevice, dev->stream));
+        SPIRA_HIP(hipMemcpyAsync(P.xs_blk_start, start.data(), (kSlices + 1) * sizeof(uint32_t),
+                                 hipMemcpyHostToDevice, dev->stream));
+        dev_sync(dev);  // pos lives in the workspace: finish before it can be reused
+    } catch (...) {
+        (void)hipStreamSynchronize(dev->stream);
+        for (uint32_t *l : lists)
+            if (l) (void)hipFree(l);
+        throw;
+    }
+    for (uint32_t *l : lists) (void)hipFree(l);
+    uint64_t mx = 0;
+    for (uint64_t v : nbs) mx = v > mx ? v : mx;
+    P.xs_nblk_max = (uint32_t)mx;
+}
+
+void launch_xstream(spira_dcsr *A, const double *x, double *y)
+{
+    const spira_plan &P = A->plan;
+    const uint64_t grid = (uint64_t)P.xs_nblk_max * kSlices;
+    if (grid > 0x7fffffffull) fail(SPIRA_ERR_UNSUPPORTED, "spmv: matrix too large for XSTREAM");
+    if (grid > 0)
+        k_spmv_xstream<1024><<<(unsigned)grid, kThreads, 0, A->dev->stream>>>(
+            P.xs_blk, P.xs_blk_start, A->n_rows, P.xs_off, P.xs_col, P.xs_val, x, P.xs_part);
+    k_xslice_sum<<<grid_for(A->n_rows, kThreads), kThreads, 0, A->dev->stream>>>(A->n_rows, P.xs_part, y);
+}
+
+template <int G>
+void launch_xslice(spira_dcsr *A, const double *x, double *y)
+{
+    const spira_plan &P = A->plan;
+    const uint64_t nrb = (A->n_rows * (uint64_t)G + kThreads - 1) / kThreads;
+    const uint64_t grid = nrb * kSlices;
+    if (grid > 0x7fffffffull) fail(SPIRA_ERR_UNSUPPORTED, "spmv: matrix too large for XSLICE");
+    k_spmv_xslice<G><<<(unsigned)grid, kThreads, 0, A->dev->stream>>>(A->n_rows, P.xs_off, P.xs_col, P.xs_val, x,
+                                                                       P.xs_part);
+    k_xslice_sum<<<grid_for(A->n_rows, kThreads), kThreads, 0, A->dev->stream>>>(A->n_rows, P.xs_part, y);
+}
+
 }  // namespace
 
 void spmv_free_plan(spira_dcsr *A)
@@ -283,6 +556,9 @@ void spmv_free_plan(spira_dcsr *A)
     if (P.off32) (void)hipFree(P.off32);
     for (int i = 0; i < 2; ++i)
         if (P.blk[i]) (void)hipFree(P.blk[i]);
+    for (void *p : {(void *)P.xs_off, (void *)P.xs_col, (void *)P.xs_val, (void *)P.xs_part, (void *)P.xs_blk,
+                    (void *)P.xs_blk_start})
+        if (p) (void)hipFree(p);
     int keep = P.variant;
     P = spira_plan{};
     (void)keep;  // a lock changes the matrix: the pinned variant is dropped too
@@ -333,6 +609,14 @@ void spmv_launch_f64(spira_dcsr *A, const double *x, double *y, int variant)
             if (o32) launch_stream<2048, uint32_t>(A, blk, nb, A->plan.off32, x, y);
             else launch_stream<2048, uint64_t>(A, blk, nb, A->off, x, y);
         }
+    } else if (variant >= SPIRA_SPMV_XSLICE1 && variant <= SPIRA_SPMV_XSLICE4) {
+        ensure_xslice(A);
+        if (variant == SPIRA_SPMV_XSLICE1) launch_xslice<1>(A, x, y);
+        else if (variant == SPIRA_SPMV_XSLICE2) launch_xslice<2>(A, x, y);
+        else launch_xslice<4>(A, x, y);
+    } else if (variant == SPIRA_SPMV_XSTREAM) {
+        ensure_xslice(A);
+        launch_xstream(A, x, y);
     } else {
         fail(SPIRA_ERR_INVALID_ARGUMENT, "spmv: unknown kernel variant");
     }
