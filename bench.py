@@ -38,6 +38,11 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--no-lock", action="store_true", help="skip the lock() measurement")
     p.add_argument("--variant", default="autotune", help="autotune | auto | <variant name>")
+    p.add_argument("--matrices", default="laplacian_2048,random_4M_8to32",
+                   help="comma list of the SpMV matrices to run (profiling runs pick one)")
+    p.add_argument("--lock-only", action="store_true", help="only the lock() measurement (profiling)")
+    p.add_argument("--calib-copy-mb", type=int, default=0,
+                   help="also run a plain device copy of this many MiB (PMC traffic calibration)")
     return p.parse_args()
 
 
@@ -52,6 +57,46 @@ def profile_traffic(workload_key):
         return None if e is None else e.get("traffic_bytes")
     except (OSError, ValueError):
         return None
+
+
+def run_lock(args, np, torch, spira_amd, W, dev, stream):
+    """lock(): re-lock merge on 1,048,576^2 with 16 random nnz/row, then 2
+    inserts/row of which 25% overwrite; staged inputs resident in HBM (and,
+    separately, handed over from host memory: PCIe-inclusive)."""
+    n3 = 1 << 20
+    (o3, c3, v3), (r3, ic3, iv3) = W.relock_workload(n3)
+    sp3, sc3, sv3 = spira_amd.stage_from_coo(n3, r3, ic3, iv3)
+    L = spira_amd.DeviceCSR(dev, n3, n3)
+    with torch.cuda.stream(stream):
+        d_sp = torch.from_numpy(sp3.astype(np.int64)).to("cuda")
+        d_sc = torch.from_numpy(sc3.astype(np.int32)).to("cuda")
+        d_sv = torch.from_numpy(sv3).to("cuda")
+    stream.synchronize()
+    t_dev, t_host = [], []
+    nnz_new = 0
+    for rep in range(6):
+        L.upload(o3, c3, v3)
+        dev.synchronize()
+        t = time.perf_counter()
+        nnz_new = L.lock(d_sp, d_sc, d_sv, device=True)
+        t_dev.append(time.perf_counter() - t)
+        if args.lock_only:
+            continue
+        L.upload(o3, c3, v3)
+        dev.synchronize()
+        t = time.perf_counter()
+        L.lock(sp3, sc3, sv3)
+        t_host.append(time.perf_counter() - t)
+    lbytes = W.lock_bytes(n3, int(c3.size), int(sc3.size), nnz_new)
+    tdev = float(np.median(t_dev[1:]))
+    detail = {"workload": "re-lock merge 1,048,576^2: 16 random nnz/row, then 2 inserts/row, 25% overwrite",
+              "nnz_old": int(c3.size), "staged": int(sc3.size), "nnz_new": int(nnz_new),
+              "bytes_per_lock": lbytes, "ms_device_inputs": round(1e3 * tdev, 3),
+              "GB_s": round(lbytes / tdev / 1e9, 1)}
+    if t_host:
+        detail["ms_host_inputs_pcie_inclusive"] = round(1e3 * float(np.median(t_host[1:])), 3)
+    L.close()
+    return detail, (o3, c3, v3, sp3, sc3, sv3, n3, lbytes)
 
 
 def main():
@@ -71,16 +116,36 @@ def main():
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(3)
 
-    t0 = time.time()
-    log("bench: generating the Laplacian 2048^2 and the 4M-row random matrix")
-    mats = {
-        "laplacian_2048": W.laplacian_2d(2048),
-        "random_4M_8to32": W.random_uniform_rows(1 << 22, 1 << 22, 8, 32, seed=43),
-    }
-    log(f"bench: generated in {time.time() - t0:.1f}s")
-
     stream = torch.cuda.Stream()
     dev = spira_amd.Device(0, stream=stream.cuda_stream)
+
+    if args.calib_copy_mb:
+        # A plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE per
+        # dispatch calibrate the PMC byte counters against true bytes.
+        nb = args.calib_copy_mb << 20
+        with torch.cuda.stream(stream):
+            a = torch.ones(nb // 8, dtype=torch.float64, device="cuda")
+            b = torch.empty_like(a)
+            for _ in range(3):
+                b.copy_(a)
+        stream.synchronize()
+        log(f"bench: calibration copy of {args.calib_copy_mb} MiB done")
+        del a, b
+
+    if args.lock_only:
+        out = run_lock(args, np, torch, spira_amd, W, dev, stream)
+        print(json.dumps({"lock": out[0]}), flush=True)
+        return
+
+    generators = {
+        "laplacian_2048": lambda: W.laplacian_2d(2048),
+        "random_4M_8to32": lambda: W.random_uniform_rows(1 << 22, 1 << 22, 8, 32, seed=43),
+    }
+    t0 = time.time()
+    wanted = [m for m in args.matrices.split(",") if m]
+    log(f"bench: generating {', '.join(wanted)}")
+    mats = {name: generators[name]() for name in wanted}
+    log(f"bench: generated in {time.time() - t0:.1f}s")
     dA, xs, ys, info = {}, {}, {}, {}
     for name, (off, col, val) in mats.items():
         n = len(off) - 1
@@ -164,44 +229,13 @@ def main():
         if "autotune_ms" in info[name]:
             detail[name]["autotune_ms"] = info[name]["autotune_ms"]
 
-    # ---- lock(): re-lock merge, 1,048,576^2, 16 nnz/row + 2 inserts/row (25% overwrite) ----
-    lock_detail = None
-    if not args.no_lock:
-        n3 = 1 << 20
-        (o3, c3, v3), (r3, ic3, iv3) = W.relock_workload(n3)
-        sp3, sc3, sv3 = spira_amd.stage_from_coo(n3, r3, ic3, iv3)
-        L = spira_amd.DeviceCSR(dev, n3, n3)
-        with torch.cuda.stream(stream):
-            d_sp = torch.from_numpy(sp3.astype(np.int64)).to("cuda")
-            d_sc = torch.from_numpy(sc3.astype(np.int32)).to("cuda")
-            d_sv = torch.from_numpy(sv3).to("cuda")
-        stream.synchronize()
-        t_dev, t_host = [], []
-        nnz_new = 0
-        for rep in range(6):
-            L.upload(o3, c3, v3)
-            dev.synchronize()
-            t = time.perf_counter()
-            nnz_new = L.lock(d_sp, d_sc, d_sv, device=True)
-            t_dev.append(time.perf_counter() - t)
-            L.upload(o3, c3, v3)
-            dev.synchronize()
-            t = time.perf_counter()
-            L.lock(sp3, sc3, sv3)
-            t_host.append(time.perf_counter() - t)
-        lbytes = W.lock_bytes(n3, int(c3.size), int(sc3.size), nnz_new)
-        tdev = float(np.median(t_dev[1:]))
-        lock_detail = {"workload": "re-lock merge 1,048,576^2: 16 random nnz/row, then 2 inserts/row, 25% overwrite",
-                       "nnz_old": int(c3.size), "staged": int(sc3.size), "nnz_new": int(nnz_new),
-                       "bytes_per_lock": lbytes, "ms_device_inputs": round(1e3 * tdev, 3),
-                       "GB_s": round(lbytes / tdev / 1e9, 1),
-                       "ms_host_inputs_pcie_inclusive": round(1e3 * float(np.median(t_host[1:])), 3)}
-        L.close()
-        lock_src = (o3, c3, v3, sp3, sc3, sv3, n3, lbytes)
+    lock_detail, lock_src = (None, None) if args.no_lock else run_lock(args, np, torch, spira_amd, W, dev, stream)
 
     achieved = info[dominant]["bytes"] / (kern_ms * 1e-3) / 1e9
     traffic = profile_traffic(dominant + ":" + info[dominant]["variant"])
-    roofline = {"bound": "hbm", "kernel": f"spmv {info[dominant]['variant']} on {dominant}",
+    two = info[dominant]["variant"].startswith("xs")
+    roofline = {"bound": "hbm", "kernel": f"spmv {info[dominant]['variant']} on {dominant}"
+                + (" (slice kernel + partial-sum kernel, timed together)" if two else ""),
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "bytes_per_launch": info[dominant]["bytes"], "avg_launch_ms": round(kern_ms, 5)}
