@@ -45,12 +45,13 @@ namespace spira_amd {
 
 void free_dcsr_arrays(spira_dcsr *A)
 {
-    if (A->off) (void)hipFree(A->off);
-    if (A->col) (void)hipFree(A->col);
-    if (A->val) (void)hipFree(A->val);
-    if (A->val_f32) (void)hipFree(A->val_f32);
-    if (A->val_bf16) (void)hipFree(A->val_bf16);
-    if (A->val_bf16_lo) (void)hipFree(A->val_bf16_lo);
+    spira_dev *dev = A->dev;
+    if (A->off) dev_release(dev, A->off, off_bytes(A));
+    if (A->col) dev_release(dev, A->col, col_bytes(A));
+    if (A->val) dev_release(dev, A->val, val_bytes(A));
+    if (A->val_f32) dev_release(dev, A->val_f32, (A->capacity ? A->capacity : 1) * sizeof(float));
+    if (A->val_bf16) dev_release(dev, A->val_bf16, (A->capacity ? A->capacity : 1) * sizeof(uint16_t));
+    if (A->val_bf16_lo) dev_release(dev, A->val_bf16_lo, (A->capacity ? A->capacity : 1) * sizeof(uint16_t));
     A->off = nullptr;
     A->col = nullptr;
     A->val = nullptr;
@@ -108,6 +109,7 @@ void spira_dev_destroy(spira_dev *dev)
     (void)hipSetDevice(dev->device);
     (void)hipStreamSynchronize(dev->stream);
     dev->ws.release();
+    dev->cache.release_all();
     if (dev->own_stream) (void)hipStreamDestroy(dev->stream);
     delete dev;
 }
@@ -172,10 +174,12 @@ int spira_dcsr_upload_f64(spira_dcsr *A, uint64_t nnz, const uint64_t *offsets, 
         uint64_t *off = nullptr;
         uint32_t *col = nullptr;
         double *val = nullptr;
+        const size_t ob = (A->n_rows + 1) * sizeof(uint64_t), cb = (nnz ? nnz : 1) * sizeof(uint32_t),
+                     vb = (nnz ? nnz : 1) * sizeof(double);
         try {
-            SPIRA_HIP(hipMalloc(&off, (A->n_rows + 1) * sizeof(uint64_t)));
-            SPIRA_HIP(hipMalloc(&col, (nnz ? nnz : 1) * sizeof(uint32_t)));
-            SPIRA_HIP(hipMalloc(&val, (nnz ? nnz : 1) * sizeof(double)));
+            off = (uint64_t *)dev_alloc(dev, ob);
+            col = (uint32_t *)dev_alloc(dev, cb);
+            val = (double *)dev_alloc(dev, vb);
             copy_in(dev, off, offsets, (A->n_rows + 1) * sizeof(uint64_t), in_dev);
             copy_in(dev, col, cols, nnz * sizeof(uint32_t), in_dev);
             copy_in(dev, val, vals, nnz * sizeof(double), in_dev);
@@ -191,9 +195,9 @@ int spira_dcsr_upload_f64(spira_dcsr *A, uint64_t nnz, const uint64_t *offsets, 
             A->has_zeros = zeros;
         } catch (...) {
             (void)hipStreamSynchronize(dev->stream);
-            if (off) (void)hipFree(off);
-            if (col) (void)hipFree(col);
-            if (val) (void)hipFree(val);
+            dev_release(dev, off, ob);
+            dev_release(dev, col, cb);
+            dev_release(dev, val, vb);
             throw;
         }
     });

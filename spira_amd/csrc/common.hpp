@@ -41,6 +41,12 @@ inline void hip_check(hipError_t e, const char *what)
 // ---------------------------------------------------------------------------
 constexpr int kWave = 64;
 
+// Native vector types (usable with the non-temporal load builtin).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
 template <class T>
 __device__ __forceinline__ T ld_stream(const T *p)
 {
@@ -149,6 +155,62 @@ struct workspace {
     }
 };
 
+// Caching allocator for CSR arrays and plans.  Every lock() allocates a new
+// CSR and releases the old one; on a handle all work runs on one stream, so a
+// released block can be handed out again at once (stream order protects any
+// kernel still reading it).  Blocks are reused when they are at least the
+// requested size and at most twice it; the cache is capped and is released
+// with the handle.
+struct block_cache {
+    struct blk {
+        void *p;
+        size_t bytes;
+    };
+    std::vector<blk> free_list;
+    size_t cached = 0;
+    size_t cap = size_t(24) << 30;  // bytes kept for reuse at most
+    void *alloc(size_t bytes)
+    {
+        if (bytes == 0) bytes = 8;
+        int best = -1;
+        for (size_t i = 0; i < free_list.size(); ++i) {
+            const size_t b = free_list[i].bytes;
+            if (b >= bytes && b <= 2 * bytes && (best < 0 || b < free_list[best].bytes)) best = (int)i;
+        }
+        if (best >= 0) {
+            void *p = free_list[best].p;
+            cached -= free_list[best].bytes;
+            free_list.erase(free_list.begin() + best);
+            return p;
+        }
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            release_all();  // give cached memory back and retry once
+            SPIRA_HIP(hipMalloc(&p, bytes));
+        }
+        return p;
+    }
+    void release(void *p, size_t bytes)
+    {
+        if (!p) return;
+        if (bytes == 0) bytes = 8;
+        if (cached + bytes > cap) {
+            (void)hipFree(p);
+            return;
+        }
+        free_list.push_back({p, bytes});
+        cached += bytes;
+    }
+    void release_all()
+    {
+        for (auto &b : free_list) (void)hipFree(b.p);
+        free_list.clear();
+        cached = 0;
+    }
+};
+
 // Workspace slot assignment (one operation at a time per handle).
 enum ws_slot : int {
     WS_X = 0,        // host-x staging
@@ -189,6 +251,7 @@ struct spira_plan {
     uint32_t *xs_blk_start = nullptr;  // kSlices+1 list starts
     uint32_t xs_nblk_max = 0;
     int32_t variant = -1;        // pinned AUTO choice (-1 = heuristic)
+    std::vector<std::pair<void *, size_t>> owned;  // every plan allocation
 };
 
 struct spira_dcsr {
@@ -212,6 +275,7 @@ struct spira_dev {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     spira_amd::workspace ws;
+    spira_amd::block_cache cache;
     std::string last_error;
     int arch_major = 0, arch_minor = 0;
     int n_cu = 0;

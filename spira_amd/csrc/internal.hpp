@@ -39,4 +39,21 @@ inline T read_scalar(spira_dev *dev, const T *dptr)
 
 void free_dcsr_arrays(spira_dcsr *A);
 
+// Device memory through the handle's block cache.
+inline void *dev_alloc(spira_dev *dev, size_t bytes) { return dev->cache.alloc(bytes); }
+inline void dev_release(spira_dev *dev, void *p, size_t bytes) { dev->cache.release(p, bytes); }
+
+// Plan memory: recorded on the plan and released with it.
+inline void *plan_alloc(spira_dcsr *A, size_t bytes)
+{
+    void *p = dev_alloc(A->dev, bytes);
+    A->plan.owned.push_back({p, bytes ? bytes : 8});
+    return p;
+}
+
+// Bytes of the CSR arrays a handle holds.
+inline size_t off_bytes(const spira_dcsr *A) { return (A->n_rows + 1) * sizeof(uint64_t); }
+inline size_t col_bytes(const spira_dcsr *A) { return (A->capacity ? A->capacity : 1) * sizeof(uint32_t); }
+inline size_t val_bytes(const spira_dcsr *A) { return (A->capacity ? A->capacity : 1) * sizeof(double); }
+
 }  // namespace spira_amd

@@ -214,6 +214,46 @@ __device__ __forceinline__ row_range block_rows(const uint64_t *ptr, uint64_t n_
     return rr;
 }
 
+// The block's row range with its pointers staged in LDS when the range is
+// short (the common case); lookups then stay in LDS.  Built by every thread of
+// the block before any of them leaves.
+constexpr uint32_t kRowCache = 2048;
+
+struct row_cache {
+    uint64_t lo, hi;
+    bool cached;
+    const uint64_t *s;  // LDS copy of ptr[lo .. hi+1] when cached
+    const uint64_t *g;  // the pointer array itself
+    __device__ __forceinline__ uint64_t find(uint64_t k) const
+    {
+        if (!cached) return row_of(g, lo, hi + 1, k);
+        uint32_t a = 0, b = (uint32_t)(hi - lo + 1);
+        while (b - a > 1) {
+            const uint32_t mid = (a + b) >> 1;
+            if (s[mid] <= k) a = mid;
+            else b = mid;
+        }
+        return lo + a;
+    }
+    __device__ __forceinline__ uint64_t at(uint64_t r) const { return cached ? s[r - lo] : g[r]; }  // r in [lo, hi+1]
+};
+
+__device__ __forceinline__ row_cache make_row_cache(const uint64_t *ptr, uint64_t n_rows, uint64_t first,
+                                                    uint64_t last, uint64_t *s_buf)
+{
+    row_cache rc;
+    const row_range rr = block_rows(ptr, n_rows, first, last);
+    rc.lo = rr.lo;
+    rc.hi = rr.hi;
+    rc.g = ptr;
+    rc.s = s_buf;
+    rc.cached = rr.hi - rr.lo + 1 <= kRowCache;
+    if (rc.cached)
+        for (uint64_t j = threadIdx.x; j <= rr.hi - rr.lo + 1; j += blockDim.x) s_buf[j] = ptr[rr.lo + j];
+    __syncthreads();
+    return rc;
+}
+
 // ---------------------------------------------------------------------------
 // 3. flags of deduped entries
 // ---------------------------------------------------------------------------
@@ -226,16 +266,17 @@ __global__ __launch_bounds__(kThreads) void k_ded_flags(uint64_t n, uint64_t m, 
                                                          const uint32_t *__restrict__ old_col,
                                                          uint64_t *__restrict__ flags, uint32_t *__restrict__ qpos)
 {
+    __shared__ uint64_t s_rows[kRowCache + 1];
     const uint64_t first = (uint64_t)blockIdx.x * kThreads * kItems;
     if (first >= m) return;
     const uint64_t last = min(first + kThreads * kItems, m) - 1;
-    const row_range rr = block_rows(sptr, n, first, last);
+    const row_cache rc = make_row_cache(sptr, n, first, last, s_rows);
 #pragma unroll
     for (int it = 0; it < kItems; ++it) {
         const uint64_t s = first + (uint64_t)it * kThreads + threadIdx.x;
         if (s >= m) break;
-        const uint64_t r = row_of(sptr, rr.lo, rr.hi + 1, s);
-        const uint64_t j = s - sptr[r];
+        const uint64_t r = rc.find(s);
+        const uint64_t j = s - rc.at(r);
         uint64_t f = 0;
         uint32_t q = 0;
         if (j < dcnt[r]) {
@@ -268,15 +309,16 @@ __global__ __launch_bounds__(kThreads) void k_old_zero_flags(uint64_t n, uint64_
                                                               const uint8_t *__restrict__ dirty,
                                                               uint64_t *__restrict__ zf)
 {
+    __shared__ uint64_t s_rows[kRowCache + 1];
     const uint64_t first = (uint64_t)blockIdx.x * kThreads * kItems;
     if (first >= nnz_old) return;
     const uint64_t last = min(first + kThreads * kItems, nnz_old) - 1;
-    const row_range rr = block_rows(old_off, n, first, last);
+    const row_cache rc = make_row_cache(old_off, n, first, last, s_rows);
 #pragma unroll
     for (int it = 0; it < kItems; ++it) {
         const uint64_t k = first + (uint64_t)it * kThreads + threadIdx.x;
         if (k >= nnz_old) break;
-        const uint64_t r = row_of(old_off, rr.lo, rr.hi + 1, k);
+        const uint64_t r = rc.find(k);
         uint64_t z = 0;
         if (row_dirty(dirty, sptr, r) && is_zero_f64(old_val[k])) {
             const uint64_t db = sptr[r], de = db + dcnt[r];
@@ -331,40 +373,83 @@ __global__ __launch_bounds__(kThreads) void k_scatter_old(uint64_t n, uint64_t n
                                                            uint32_t *__restrict__ new_col, double *__restrict__ new_val,
                                                            uint32_t *__restrict__ err)
 {
+    __shared__ uint64_t s_rows[kRowCache + 1];
     const uint64_t first = (uint64_t)blockIdx.x * kThreads * kItems;
     if (first >= nnz_old) return;
     const uint64_t last = min(first + kThreads * kItems, nnz_old) - 1;
-    const row_range rr = block_rows(old_off, n, first, last);
+    const row_cache rc = make_row_cache(old_off, n, first, last, s_rows);
+    // Each thread owns kItems consecutive entries: one row search, then a walk.
+    const uint64_t k0 = first + (uint64_t)threadIdx.x * kItems;
+    if (k0 >= nnz_old) return;
+    uint32_t c[kItems];
+    double v[kItems];
+    if (k0 + kItems <= nnz_old) {
+        static_assert(kItems == 4, "vector loads assume 4 entries per thread");
+        const u32x4 c4 = ld_stream(reinterpret_cast<const u32x4 *>(old_col + k0));
+        const f64x2 v01 = ld_stream(reinterpret_cast<const f64x2 *>(old_val + k0));
+        const f64x2 v23 = ld_stream(reinterpret_cast<const f64x2 *>(old_val + k0 + 2));
+        c[0] = c4[0];
+        c[1] = c4[1];
+        c[2] = c4[2];
+        c[3] = c4[3];
+        v[0] = v01[0];
+        v[1] = v01[1];
+        v[2] = v23[0];
+        v[3] = v23[1];
+    } else {
+#pragma unroll
+        for (int it = 0; it < kItems; ++it) {
+            c[it] = k0 + it < nnz_old ? old_col[k0 + it] : 0u;
+            v[it] = k0 + it < nnz_old ? old_val[k0 + it] : 0.0;
+        }
+    }
+    uint64_t r = rc.find(k0);
+    uint64_t ob = rc.at(r), oe = rc.at(r + 1);
+    // per-row state, refreshed when the walk enters a new row
+    uint64_t no = 0, nlen = 0, db = 0, de = 0, pa = 0;
+    bool dirty_r = false, have = false;
 #pragma unroll
     for (int it = 0; it < kItems; ++it) {
-        const uint64_t k = first + (uint64_t)it * kThreads + threadIdx.x;
+        const uint64_t k = k0 + it;
         if (k >= nnz_old) break;
-        const uint64_t r = row_of(old_off, rr.lo, rr.hi + 1, k);
-        const uint64_t ob = old_off[r];
+        while (k >= oe) {  // skip to the row owning k (empty rows included)
+            ++r;
+            ob = oe;
+            oe = rc.at(r + 1);
+            have = false;
+        }
+        if (!have) {
+            no = new_off[r];
+            nlen = new_off[r + 1] - no;
+            dirty_r = row_dirty(dirty, sptr, r);
+            if (dirty_r) {
+                db = sptr[r];
+                de = db + dcnt[r];
+                pa = P[db];
+            }
+            have = true;
+        }
         const uint64_t i = k - ob;
-        const uint32_t c = ld_stream(old_col + k);
-        const double v = ld_stream(old_val + k);
         uint64_t out;
-        if (!row_dirty(dirty, sptr, r)) {
-            out = new_off[r] + i;  // clean row: verbatim (csr_build.hpp:362-377)
+        if (!dirty_r) {
+            out = no + i;  // clean row: verbatim (csr_build.hpp:362-377)
         } else {
-            const uint64_t db = sptr[r], de = db + dcnt[r];
-            const uint64_t p = de > db ? lower_bound_u32(dcol, db, de, c) : db;
-            if (p < de && dcol[p] == c) continue;  // buffer wins (:430-436)
-            if (PZ && is_zero_f64(v)) continue;    // old zero dropped (:420-421)
-            const uint64_t pa = P[db], pp = P[p];
+            const uint64_t p = de > db ? lower_bound_u32(dcol, db, de, c[it]) : db;
+            if (p < de && dcol[p] == c[it]) continue;  // buffer wins (:430-436)
+            if (PZ && is_zero_f64(v[it])) continue;    // old zero dropped (:420-421)
+            const uint64_t pp = p == db ? pa : P[p];
             const uint64_t before_m = (pp >> 32) - (pa >> 32);
             const uint64_t before_nz = (pp & 0xffffffffull) - (pa & 0xffffffffull);
             const uint64_t before_z = PZ ? (PZ[k] - PZ[ob]) : 0;
-            out = new_off[r] + i - before_m - before_z + before_nz;
+            out = no + i - before_m - before_z + before_nz;
         }
         // Every slot must fall inside the row's new range; never write outside.
-        if (out - new_off[r] >= new_off[r + 1] - new_off[r]) {
+        if (out - no >= nlen) {
             atomicOr(err, ERR_INTERNAL);
             continue;
         }
-        new_col[out] = c;
-        new_val[out] = v;
+        new_col[out] = c[it];
+        new_val[out] = v[it];
     }
 }
 
@@ -380,10 +465,11 @@ __global__ __launch_bounds__(kThreads) void k_scatter_ded(uint64_t n, uint64_t m
                                                            uint32_t *__restrict__ new_col, double *__restrict__ new_val,
                                                            uint32_t *__restrict__ err)
 {
+    __shared__ uint64_t s_rows[kRowCache + 1];
     const uint64_t first = (uint64_t)blockIdx.x * kThreads * kItems;
     if (first >= m) return;
     const uint64_t last = min(first + kThreads * kItems, m) - 1;
-    const row_range rr = block_rows(sptr, n, first, last);
+    const row_cache rc = make_row_cache(sptr, n, first, last, s_rows);
 #pragma unroll
     for (int it = 0; it < kItems; ++it) {
         const uint64_t s = first + (uint64_t)it * kThreads + threadIdx.x;
@@ -391,8 +477,8 @@ __global__ __launch_bounds__(kThreads) void k_scatter_ded(uint64_t n, uint64_t m
         const uint64_t ps = P[s];
         const uint64_t ps1 = P[s + 1];
         if (((ps1 - ps) & 0xffffffffull) == 0) continue;  // not a surviving entry
-        const uint64_t r = row_of(sptr, rr.lo, rr.hi + 1, s);
-        const uint64_t db = sptr[r];
+        const uint64_t r = rc.find(s);
+        const uint64_t db = rc.at(r);
         const uint64_t pa = P[db];
         const uint64_t before_nz = (ps & 0xffffffffull) - (pa & 0xffffffffull);
         uint64_t out = new_off[r] + before_nz;
@@ -564,26 +650,27 @@ uint64_t lock_device_f64(spira_dcsr *A, const uint64_t *sptr, const uint32_t *sc
     }
 
     // ---- 4. counts + offsets ----------------------------------------------
+    // The new arrays are sized by the upper bound nnz_old + m (as merge_csr's
+    // total_ub, csr_build.hpp:139-145) so no host round trip is needed before
+    // the scatter; the exact nnz comes back with the final status word.
     uint64_t *cnt = (uint64_t *)dev->ws.get(WS_CNT, (n + 1) * sizeof(uint64_t));
-    uint64_t *new_off = nullptr;
-    SPIRA_HIP(hipMalloc(&new_off, (n + 1) * sizeof(uint64_t)));
+    const uint64_t ub = (merge ? A->nnz : 0) + m;
+    const size_t ob = (n + 1) * sizeof(uint64_t), cb = (ub ? ub : 1) * sizeof(uint32_t),
+                 vb = (ub ? ub : 1) * sizeof(double);
+    uint64_t *new_off = (uint64_t *)dev_alloc(dev, ob);
+    uint32_t *new_col = nullptr;
+    double *new_val = nullptr;
     try {
+        new_col = (uint32_t *)dev_alloc(dev, cb);
+        new_val = (double *)dev_alloc(dev, vb);
         if (n > 0) {
             k_row_counts<<<grid_for(n, kThreads), kThreads, 0, st>>>(n, sptr, P, dirty, merge ? 1 : 0, A->off, PZ,
                                                                      cnt);
             SPIRA_HIP(hipGetLastError());
         }
         scan_excl_u64(dev, cnt, new_off, n);
-        const uint64_t nnz_new = read_scalar(dev, new_off + n);
 
         // ---- 5. scatter ------------------------------------------------------
-        uint32_t *new_col = nullptr;
-        double *new_val = nullptr;
-        SPIRA_HIP(hipMalloc(&new_col, std::max<uint64_t>(nnz_new, 1) * sizeof(uint32_t)));
-        if (hipMalloc(&new_val, std::max<uint64_t>(nnz_new, 1) * sizeof(double)) != hipSuccess) {
-            (void)hipFree(new_col);
-            fail(SPIRA_ERR_NOMEM, "lock: out of device memory for the new CSR");
-        }
         if (merge && A->nnz > 0) {
             k_scatter_old<<<grid_for(A->nnz, kThreads * kItems), kThreads, 0, st>>>(
                 n, A->nnz, A->off, A->col, A->val, sptr, dcnt, dcol, P, PZ, dirty, new_off, new_col, new_val, ctl);
@@ -595,27 +682,29 @@ uint64_t lock_device_f64(spira_dcsr *A, const uint64_t *sptr, const uint32_t *sc
             SPIRA_HIP(hipGetLastError());
         }
         uint32_t herr = 0;
+        uint64_t nnz_new = 0;
         SPIRA_HIP(hipMemcpyAsync(&herr, ctl, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        SPIRA_HIP(hipMemcpyAsync(&nnz_new, new_off + n, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
         dev_sync(dev);
-        if (herr & ERR_INTERNAL) {
-            (void)hipFree(new_col);
-            (void)hipFree(new_val);
-            fail(SPIRA_ERR_DEVICE, "lock: internal error, a merged entry fell outside its row");
-        }
-        // install (the old arrays are released only after the kernels finished)
+        if (herr & ERR_INTERNAL) fail(SPIRA_ERR_DEVICE, "lock: internal error, a merged entry fell outside its row");
+        // install: the old arrays go back to the cache (stream order keeps
+        // them intact for anything still queued)
         free_dcsr_arrays(A);
         A->off = new_off;
         A->col = new_col;
         A->val = new_val;
         A->nnz = nnz_new;
-        A->capacity = nnz_new;
+        A->capacity = ub;
         A->built = true;
         // Dirty rows come out without zeros; only clean rows can keep the
         // explicit zeros an upload brought in.
         if (!merge) A->has_zeros = false;
         return nnz_new;
     } catch (...) {
-        (void)hipFree(new_off);
+        (void)hipStreamSynchronize(st);
+        dev_release(dev, new_off, ob);
+        dev_release(dev, new_col, cb);
+        dev_release(dev, new_val, vb);
         throw;
     }
 }

@@ -427,7 +427,7 @@ void ensure_stats(spira_dcsr *A)
         P.cv_len = P.mean_len > 0 ? std::sqrt(var > 0 ? var : 0.0) / P.mean_len : 0.0;
     }
     if (A->nnz < (1ull << 32) && !P.off32) {
-        SPIRA_HIP(hipMalloc(&P.off32, (A->n_rows + 1) * sizeof(uint32_t)));
+        P.off32 = (uint32_t *)plan_alloc(A, (A->n_rows + 1) * sizeof(uint32_t));
         k_off32<<<grid_for(A->n_rows + 1, kThreads), kThreads, 0, dev->stream>>>(A->off, P.off32, A->n_rows + 1);
         SPIRA_HIP(hipGetLastError());
     }
@@ -446,7 +446,7 @@ void ensure_blocks(spira_dcsr *A, int which)
     SPIRA_HIP(hipGetLastError());
     scan_excl_u64(dev, pos, pos, n);
     const uint64_t nb = read_scalar(dev, pos + n);
-    SPIRA_HIP(hipMalloc(&P.blk[which], (nb + 1) * sizeof(uint32_t)));
+    P.blk[which] = (uint32_t *)plan_alloc(A, (nb + 1) * sizeof(uint32_t));
     k_plan_scatter<<<grid_for(n, kThreads), kThreads, 0, dev->stream>>>(A->off, n, C, pos, P.blk[which]);
     SPIRA_HIP(hipGetLastError());
     P.nblk[which] = (uint32_t)nb;
@@ -466,63 +466,48 @@ void ensure_xslice(spira_dcsr *A)
     k_xslice_count<<<grid_for(n, kThreads), kThreads, 0, dev->stream>>>(n, width, A->off, A->col, pos);
     SPIRA_HIP(hipGetLastError());
     scan_excl_u64(dev, pos, pos, N);
-    try {
-        SPIRA_HIP(hipMalloc(&P.xs_col, (A->nnz ? A->nnz : 1) * sizeof(uint32_t)));
-        SPIRA_HIP(hipMalloc(&P.xs_val, (A->nnz ? A->nnz : 1) * sizeof(double)));
-        SPIRA_HIP(hipMalloc(&P.xs_part, (N ? N : 1) * sizeof(double)));
-        SPIRA_HIP(hipMalloc(&P.xs_off, (N + 1) * sizeof(uint32_t)));
-    } catch (...) {
-        for (void *p : {(void *)P.xs_col, (void *)P.xs_val, (void *)P.xs_part, (void *)P.xs_off})
-            if (p) (void)hipFree(p);
-        P.xs_col = nullptr;
-        P.xs_val = nullptr;
-        P.xs_part = nullptr;
-        P.xs_off = nullptr;
-        throw;
-    }
+    // (plan memory is recorded on the plan: a failure part-way leaves it to
+    // spmv_free_plan, and xs_off is set last so the plan is never half-used)
+    uint32_t *xs_off = nullptr;
+    P.xs_col = (uint32_t *)plan_alloc(A, (A->nnz ? A->nnz : 1) * sizeof(uint32_t));
+    P.xs_val = (double *)plan_alloc(A, (A->nnz ? A->nnz : 1) * sizeof(double));
+    P.xs_part = (double *)plan_alloc(A, (N ? N : 1) * sizeof(double));
+    xs_off = (uint32_t *)plan_alloc(A, (N + 1) * sizeof(uint32_t));
     k_xslice_scatter<<<grid_for(n, kThreads), kThreads, 0, dev->stream>>>(n, width, A->off, A->col, A->val, pos,
                                                                           P.xs_col, P.xs_val);
     SPIRA_HIP(hipGetLastError());
-    k_u64_to_u32<<<grid_for(N + 1, kThreads), kThreads, 0, dev->stream>>>(pos, P.xs_off, N + 1);
+    k_u64_to_u32<<<grid_for(N + 1, kThreads), kThreads, 0, dev->stream>>>(pos, xs_off, N + 1);
     SPIRA_HIP(hipGetLastError());
 
-    // Row-block lists of every slice for XSTREAM (chunk 1024), back to back.
+    // Row-block lists of every slice for XSTREAM (chunk 1024), back to back:
+    // one pass to size them, one to fill them.
     constexpr uint32_t C = 1024;
     uint64_t *mark = (uint64_t *)dev->ws.get(WS_CNT, (n + 1) * sizeof(uint64_t));
     std::vector<uint32_t> start(kSlices + 1, 0);
-    std::vector<uint32_t *> lists(kSlices, nullptr);
-    std::vector<uint64_t> nbs(kSlices, 0);
-    try {
-        for (uint32_t s = 0; s < kSlices; ++s) {
-            const uint64_t *so = pos + (uint64_t)s * n;
-            k_plan_mark<<<grid_for(n, kThreads), kThreads, 0, dev->stream>>>(so, n, C, mark);
-            SPIRA_HIP(hipGetLastError());
-            scan_excl_u64(dev, mark, mark, n);
-            nbs[s] = read_scalar(dev, mark + n);
-            start[s + 1] = start[s] + (uint32_t)nbs[s] + 1;
-            SPIRA_HIP(hipMalloc(&lists[s], (nbs[s] + 1) * sizeof(uint32_t)));
-            k_plan_scatter<<<grid_for(n, kThreads), kThreads, 0, dev->stream>>>(so, n, C, mark, lists[s]);
-            SPIRA_HIP(hipGetLastError());
-            dev_sync(dev);
-        }
-        SPIRA_HIP(hipMalloc(&P.xs_blk, (uint64_t)start[kSlices] * sizeof(uint32_t)));
-        SPIRA_HIP(hipMalloc(&P.xs_blk_start, (kSlices + 1) * sizeof(uint32_t)));
-        for (uint32_t s = 0; s < kSlices; ++s)
-            SPIRA_HIP(hipMemcpyAsync(P.xs_blk + start[s], lists[s], (nbs[s] + 1) * sizeof(uint32_t),
-                                     hipMemcpyDeviceToDevice, dev->stream));
-        SPIRA_HIP(hipMemcpyAsync(P.xs_blk_start, start.data(), (kSlices + 1) * sizeof(uint32_t),
-                                 hipMemcpyHostToDevice, dev->stream));
-        dev_sync(dev);  // pos lives in the workspace: finish before it can be reused
-    } catch (...) {
-        (void)hipStreamSynchronize(dev->stream);
-        for (uint32_t *l : lists)
-            if (l) (void)hipFree(l);
-        throw;
-    }
-    for (uint32_t *l : lists) (void)hipFree(l);
     uint64_t mx = 0;
-    for (uint64_t v : nbs) mx = v > mx ? v : mx;
+    for (uint32_t s = 0; s < kSlices; ++s) {
+        k_plan_mark<<<grid_for(n, kThreads), kThreads, 0, dev->stream>>>(pos + (uint64_t)s * n, n, C, mark);
+        SPIRA_HIP(hipGetLastError());
+        scan_excl_u64(dev, mark, mark, n);
+        const uint64_t nb = read_scalar(dev, mark + n);
+        start[s + 1] = start[s] + (uint32_t)nb + 1;
+        mx = nb > mx ? nb : mx;
+    }
+    P.xs_blk = (uint32_t *)plan_alloc(A, (uint64_t)start[kSlices] * sizeof(uint32_t));
+    P.xs_blk_start = (uint32_t *)plan_alloc(A, (kSlices + 1) * sizeof(uint32_t));
+    for (uint32_t s = 0; s < kSlices; ++s) {
+        const uint64_t *so = pos + (uint64_t)s * n;
+        k_plan_mark<<<grid_for(n, kThreads), kThreads, 0, dev->stream>>>(so, n, C, mark);
+        SPIRA_HIP(hipGetLastError());
+        scan_excl_u64(dev, mark, mark, n);
+        k_plan_scatter<<<grid_for(n, kThreads), kThreads, 0, dev->stream>>>(so, n, C, mark, P.xs_blk + start[s]);
+        SPIRA_HIP(hipGetLastError());
+    }
+    SPIRA_HIP(hipMemcpyAsync(P.xs_blk_start, start.data(), (kSlices + 1) * sizeof(uint32_t), hipMemcpyHostToDevice,
+                             dev->stream));
+    dev_sync(dev);  // pos and mark live in the workspace: finish before they can be reused
     P.xs_nblk_max = (uint32_t)mx;
+    P.xs_off = xs_off;  // the slice plan is complete
 }
 
 void launch_xstream(spira_dcsr *A, const double *x, double *y)
@@ -553,15 +538,8 @@ void launch_xslice(spira_dcsr *A, const double *x, double *y)
 void spmv_free_plan(spira_dcsr *A)
 {
     spira_plan &P = A->plan;
-    if (P.off32) (void)hipFree(P.off32);
-    for (int i = 0; i < 2; ++i)
-        if (P.blk[i]) (void)hipFree(P.blk[i]);
-    for (void *p : {(void *)P.xs_off, (void *)P.xs_col, (void *)P.xs_val, (void *)P.xs_part, (void *)P.xs_blk,
-                    (void *)P.xs_blk_start})
-        if (p) (void)hipFree(p);
-    int keep = P.variant;
-    P = spira_plan{};
-    (void)keep;  // a lock changes the matrix: the pinned variant is dropped too
+    for (auto &o : P.owned) dev_release(A->dev, o.first, o.second);
+    P = spira_plan{};  // a lock changes the matrix: the pinned variant is dropped too
 }
 
 void spmv_build_plan(spira_dcsr *A) { ensure_stats(A); }
