@@ -136,6 +136,25 @@ int spira_spmv_set_variant(spira_dcsr *A, int variant);
  * AUTO choice and returns it in *best (either output may be NULL). */
 int spira_spmv_autotune_f64(spira_dcsr *A, const double *x_dev, double *y_dev, int reps, float *ms, int *best);
 
+/* ---- mixed precision ------------------------------------------------------
+ * No counterpart in the reference (its float instantiation,
+ * serial/spmv.hpp:90-135, stores and accumulates in FP32).  y = A*x with A's
+ * values read from a reduced-precision copy (built once from the FP64 values,
+ * dropped by the next lock): */
+#define SPIRA_VAL_F32 1        /* 4 B per value */
+#define SPIRA_VAL_BF16 2       /* 2 B per value */
+#define SPIRA_VAL_SPLIT_BF16 3 /* hi = bf16(v), lo = bf16(v - hi): 4 B per value */
+/* x is rounded to FP32; each row is accumulated in FP32 with compensated
+ * (TwoSum / FMA TwoProd, "Dot2") summation and written to y in FP64.  Rows
+ * whose cancellation ratio sum|a*x| / |sum a*x| exceeds recompute_ratio (or
+ * whose result is not finite) are recomputed in FP64 from the FP64 values and
+ * x; recompute_ratio <= 0 disables that pass.  variant: SPIRA_SPMV_AUTO,
+ * VEC4..VEC32, STREAM1K or STREAM2K.  *n_recomputed (may be NULL; reading it
+ * synchronises) receives the number of recomputed rows.  Checks and flags as
+ * spira_spmv_f64. */
+int spira_spmv_mixed_f64(spira_dcsr *A, int storage, const double *x, uint64_t x_len, double *y, uint64_t y_len,
+                         int variant, float recompute_ratio, uint32_t flags, uint64_t *n_recomputed);
+
 #ifdef __cplusplus
 }
 #endif

@@ -58,6 +58,8 @@ VARIANTS = {
     "stream1k": 7, "stream2k": 8, "xslice1": 9, "xslice2": 10, "xslice4": 11, "xstream": 12,
 }
 VARIANT_NAMES = {v: k for k, v in VARIANTS.items()}
+STORAGE = {"f32": 1, "bf16": 2, "split_bf16": 3}
+MIXED_VARIANTS = [VARIANTS[k] for k in ("vec4", "vec8", "vec16", "vec32", "stream1k", "stream2k")]
 N_VARIANTS = 13
 LOCK_MAX_ROW_INSERTS = 8192
 
@@ -153,6 +155,8 @@ def lib():
             "spira_spmv_set_variant": (ctypes.c_int, [_vp, ctypes.c_int]),
             "spira_spmv_autotune_f64": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
                                                        ctypes.POINTER(ctypes.c_int)]),
+            "spira_spmv_mixed_f64": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_uint64, _vp, ctypes.c_uint64,
+                                                    ctypes.c_int, ctypes.c_float, ctypes.c_uint32, _u64p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -310,6 +314,27 @@ class DeviceCSR:
             flags = X_DEVICE | Y_DEVICE | (0 if sync else NO_SYNC)
         self.dev.check(lib().spira_spmv_f64(self._h, _ptr(x), xl, _ptr(y), yl, int(variant), flags))
         return y
+
+    def spmv_mixed(self, x, y=None, storage: str = "f32", variant: int = SPMV_AUTO, recompute_ratio: float = 0.0,
+                   device: bool = False, sync: bool = True, count: bool = True):
+        """Mixed-precision y = A*x (values stored as f32 / bf16 / split_bf16,
+        compensated FP32 accumulation, FP64 recompute of rows whose
+        cancellation ratio exceeds recompute_ratio).  Returns (y, n_recomputed)."""
+        st = STORAGE[storage]
+        if not device:
+            x = np.ascontiguousarray(x, dtype=np.float64)
+            if y is None:
+                y = np.empty(self.n_rows, np.float64)
+            xl, yl = x.size, y.size
+            flags = 0
+        else:
+            xl, yl = int(x.numel()), int(y.numel())
+            flags = X_DEVICE | Y_DEVICE | (0 if sync else NO_SYNC)
+        nrec = ctypes.c_uint64()
+        self.dev.check(lib().spira_spmv_mixed_f64(self._h, st, _ptr(x), xl, _ptr(y), yl, int(variant),
+                                                  float(recompute_ratio), flags,
+                                                  ctypes.byref(nrec) if count else None))
+        return y, (int(nrec.value) if count else None)
 
     def choose_variant(self) -> int:
         v = ctypes.c_int()

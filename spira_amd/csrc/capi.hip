@@ -49,16 +49,18 @@ void free_dcsr_arrays(spira_dcsr *A)
     if (A->off) dev_release(dev, A->off, off_bytes(A));
     if (A->col) dev_release(dev, A->col, col_bytes(A));
     if (A->val) dev_release(dev, A->val, val_bytes(A));
-    if (A->val_f32) dev_release(dev, A->val_f32, (A->capacity ? A->capacity : 1) * sizeof(float));
-    if (A->val_bf16) dev_release(dev, A->val_bf16, (A->capacity ? A->capacity : 1) * sizeof(uint16_t));
-    if (A->val_bf16_lo) dev_release(dev, A->val_bf16_lo, (A->capacity ? A->capacity : 1) * sizeof(uint16_t));
+    const size_t cap = A->capacity ? A->capacity : 1;
+    if (A->val_f32) dev_release(dev, A->val_f32, cap * sizeof(float));
+    if (A->val_bf16) dev_release(dev, A->val_bf16, cap * sizeof(uint16_t));
+    if (A->val_split_hi) dev_release(dev, A->val_split_hi, cap * sizeof(uint16_t));
+    if (A->val_bf16_lo) dev_release(dev, A->val_bf16_lo, cap * sizeof(uint16_t));
     A->off = nullptr;
     A->col = nullptr;
     A->val = nullptr;
     A->val_f32 = nullptr;
     A->val_bf16 = nullptr;
+    A->val_split_hi = nullptr;
     A->val_bf16_lo = nullptr;
-    A->vals_ready = 0;
     spmv_free_plan(A);
 }
 
@@ -290,6 +292,48 @@ int spira_spmv_f64(spira_dcsr *A, const double *x, uint64_t x_len, double *y, ui
         spmv_launch_f64(A, dx, dy, variant);
         if (!yd && y_len)
             SPIRA_HIP(hipMemcpyAsync(y, dy, y_len * sizeof(double), hipMemcpyDeviceToHost, dev->stream));
+        if (!(flags & SPIRA_NO_SYNC) || !xd || !yd) dev_sync(dev);
+    });
+}
+
+int spira_spmv_mixed_f64(spira_dcsr *A, int storage, const double *x, uint64_t x_len, double *y, uint64_t y_len,
+                         int variant, float recompute_ratio, uint32_t flags, uint64_t *n_recomputed)
+{
+    if (!A) return SPIRA_ERR_INVALID_ARGUMENT;
+    spira_dev *dev = A->dev;
+    return guarded(dev, [&] {
+        if (x_len != A->n_cols)
+            fail(SPIRA_ERR_INVALID_ARGUMENT,
+                 "The size of the input vector x does not match the number of columns of the matrix");
+        if (y_len != A->n_rows)
+            fail(SPIRA_ERR_INVALID_ARGUMENT,
+                 "The size of the output vector y does not match the number of rows of the matrix");
+        if (!A->built) fail(SPIRA_ERR_LOGIC, "spmv: matrix must be locked");
+        if (storage != SPIRA_VAL_F32 && storage != SPIRA_VAL_BF16 && storage != SPIRA_VAL_SPLIT_BF16)
+            fail(SPIRA_ERR_INVALID_ARGUMENT, "mixed spmv: unknown value storage");
+        if (variant != SPIRA_SPMV_AUTO && !mixed_variant_supported(variant))
+            fail(SPIRA_ERR_INVALID_ARGUMENT, "mixed spmv: variant must be VEC4..VEC32, STREAM1K or STREAM2K");
+        if ((x_len && !x) || (y_len && !y)) fail(SPIRA_ERR_INVALID_ARGUMENT, "spmv: x or y is NULL");
+        const double *dx = x;
+        double *dy = y;
+        const bool xd = flags & SPIRA_X_DEVICE, yd = flags & SPIRA_Y_DEVICE;
+        if (!xd) {
+            double *t = (double *)dev->ws.get(WS_X, x_len * sizeof(double));
+            copy_in(dev, t, x, x_len * sizeof(double), false);
+            dx = t;
+        }
+        if (!yd) dy = (double *)dev->ws.get(WS_Y, y_len * sizeof(double));
+        spmv_mixed_launch(A, storage, dx, dy, variant, recompute_ratio);
+        if (!yd && y_len)
+            SPIRA_HIP(hipMemcpyAsync(y, dy, y_len * sizeof(double), hipMemcpyDeviceToHost, dev->stream));
+        if (n_recomputed) {
+            uint32_t c = 0;
+            if (A->n_rows && recompute_ratio > 0.0f)
+                SPIRA_HIP(hipMemcpyAsync(&c, mixed_queue_count(A), sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                         dev->stream));
+            dev_sync(dev);
+            *n_recomputed = c;
+        }
         if (!(flags & SPIRA_NO_SYNC) || !xd || !yd) dev_sync(dev);
     });
 }

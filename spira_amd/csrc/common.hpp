@@ -124,7 +124,7 @@ inline unsigned grid_for(uint64_t items, unsigned per_block, unsigned cap = 0x7f
 // Growable device scratch owned by a handle.  Slot i keeps its own buffer so
 // one operation can hold several at once.
 struct workspace {
-    static constexpr int kSlots = 16;
+    static constexpr int kSlots = 20;
     void *ptr[kSlots] = {};
     size_t cap[kSlots] = {};
     hipStream_t stream = nullptr;  // work that may still read a slot runs here
@@ -229,6 +229,8 @@ enum ws_slot : int {
     WS_BIG = 13,     // lock: rows with many staged inserts
     WS_CNT = 14,     // lock: new row lengths
     WS_PZ = 15,      // lock: prefix of dropped old zeros
+    WS_MIXQ = 16,    // mixed spmv: queue of rows to recompute in FP64 + its count
+    WS_X32 = 17,     // mixed spmv: x rounded to FP32
 };
 
 }  // namespace spira_amd
@@ -265,9 +267,9 @@ struct spira_dcsr {
     spira_plan plan;
     // Reduced-precision value copies (built on demand, dropped on lock).
     float *val_f32 = nullptr;
-    uint16_t *val_bf16 = nullptr;    // hi part for split storage as well
-    uint16_t *val_bf16_lo = nullptr; // lo part of split storage
-    uint32_t vals_ready = 0;         // bitmask of SPIRA_VAL_* present
+    uint16_t *val_bf16 = nullptr;     // BF16 storage
+    uint16_t *val_split_hi = nullptr; // split BF16: hi = bf16(v)
+    uint16_t *val_bf16_lo = nullptr;  // split BF16: lo = bf16(v - hi)
 };
 
 struct spira_dev {

@@ -14,6 +14,13 @@ void spmv_build_plan(spira_dcsr *A);
 void spmv_free_plan(spira_dcsr *A);
 int spmv_heuristic_variant(const spira_dcsr *A);
 void spmv_launch_f64(spira_dcsr *A, const double *x, double *y, int variant);
+// Row-block list of the STREAM kernels (which: 0 = chunk 1024, 1 = 2048).
+void spmv_stream_blocks(spira_dcsr *A, int which, const uint32_t **blk, uint32_t *nblk);
+
+// spmv_mixed.hip
+void spmv_mixed_launch(spira_dcsr *A, int storage, const double *x, double *y, int variant, float ratio);
+uint32_t *mixed_queue_count(spira_dcsr *A);
+bool mixed_variant_supported(int variant);
 
 // lock.hip: runs the device lock pipeline on already-device-resident staged
 // arrays; installs the new CSR in A.  Returns the new nnz.

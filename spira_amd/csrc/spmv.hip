@@ -544,6 +544,13 @@ void spmv_free_plan(spira_dcsr *A)
 
 void spmv_build_plan(spira_dcsr *A) { ensure_stats(A); }
 
+void spmv_stream_blocks(spira_dcsr *A, int which, const uint32_t **blk, uint32_t *nblk)
+{
+    ensure_blocks(A, which);
+    *blk = A->plan.blk[which];
+    *nblk = A->plan.nblk[which];
+}
+
 int spmv_heuristic_variant(const spira_dcsr *A)
 {
     const spira_plan &P = A->plan;

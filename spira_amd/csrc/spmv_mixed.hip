@@ -1,0 +1,331 @@
+// spmv_mixed.hip -- reduced-precision value storage with compensated FP32
+// accumulation, and an FP64 recompute of rows that cancel.
+//
+// Values are stored as FP32 (4 B), BF16 (2 B) or split BF16 (hi + lo, 4 B),
+// copied once from A's FP64 values and dropped at the next lock.  x is
+// rounded to FP32 once per call.  Each row is accumulated with the Dot2
+// scheme of Ogita, Rump and Oishi ("Accurate sum and dot product", SIAM J. Sci.
+// Comput. 26(6), 2005): p = a*x, e = fma(a, x, -p) (the exact product error),
+// (s, q) = TwoSum(s, p) (Knuth), c += q + e, result s + c.  The same pass sums
+// |a*x|; a row whose cancellation ratio sum|a*x| / |sum a*x| exceeds the
+// caller's threshold (or whose result is not finite) is queued and recomputed
+// in FP64 from A's FP64 values and the FP64 x by a second kernel.
+#include "internal.hpp"
+
+#include <cmath>
+
+namespace spira_amd {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+// ---- value decoders --------------------------------------------------------
+__device__ __forceinline__ float bf16_to_f32(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
+
+// Round-to-nearest-even FP32 -> BF16 (NaN kept quiet).
+__device__ __forceinline__ uint16_t f32_to_bf16(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+struct vals_f32 {
+    const float *v;
+    __device__ __forceinline__ float operator()(uint64_t k) const { return ld_stream(v + k); }
+};
+struct vals_bf16 {
+    const uint16_t *v;
+    __device__ __forceinline__ float operator()(uint64_t k) const { return bf16_to_f32(ld_stream(v + k)); }
+};
+struct vals_split {
+    const uint16_t *hi, *lo;
+    __device__ __forceinline__ float operator()(uint64_t k) const
+    {
+        return bf16_to_f32(ld_stream(hi + k)) + bf16_to_f32(ld_stream(lo + k));
+    }
+};
+
+// ---- compensated accumulation ---------------------------------------------
+struct dot2 {
+    float s = 0.0f, c = 0.0f, a = 0.0f;  // sum, compensation, sum of |a*x|
+    __device__ __forceinline__ void add_prod(float p, float e)
+    {
+        const float t = s + p;
+        const float bp = t - s;
+        const float q = (s - (t - bp)) + (p - bp);  // TwoSum error
+        s = t;
+        c += q + e;
+        a += fabsf(p);
+    }
+    __device__ __forceinline__ void add(float av, float xv)
+    {
+        const float p = av * xv;
+        add_prod(p, fmaf(av, xv, -p));
+    }
+    __device__ __forceinline__ void merge(float s2, float c2, float a2)
+    {
+        const float t = s + s2;
+        const float bp = t - s;
+        const float q = (s - (t - bp)) + (s2 - bp);
+        s = t;
+        c += c2 + q;
+        a += a2;
+    }
+    // Merge across aligned groups of g lanes (g a power of two <= 64).
+    __device__ __forceinline__ void group_reduce(int g)
+    {
+        for (int o = g / 2; o > 0; o >>= 1) {
+            const float s2 = __shfl_xor(s, o, 64), c2 = __shfl_xor(c, o, 64), a2 = __shfl_xor(a, o, 64);
+            merge(s2, c2, a2);
+        }
+    }
+};
+
+__device__ __forceinline__ void finish_row(const dot2 &d, uint64_t row, double *y, float ratio,
+                                           uint32_t *queue, uint32_t *qcount)
+{
+    const double v = (double)d.s + (double)d.c;
+    y[row] = v;
+    if (ratio > 0.0f) {
+        const bool bad = !isfinite(v) || (double)d.a > (double)ratio * fabs(v);
+        if (bad) queue[atomicAdd(qcount, 1u)] = (uint32_t)row;
+    }
+}
+
+// ---- kernels ---------------------------------------------------------------
+__global__ void k_x_to_f32(const double *__restrict__ x, float *__restrict__ x32, uint64_t n, uint32_t *qcount)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x32[i] = (float)x[i];
+    if (i == 0 && qcount) *qcount = 0u;  // reset the recompute queue (no memset: graph-safe)
+}
+
+// VEC<G>: G lanes per row.
+template <int G, class Vals>
+__global__ __launch_bounds__(kThreads) void k_mixed_vec(uint64_t n_rows, const uint32_t *__restrict__ off,
+                                                         const uint32_t *__restrict__ col, Vals vals,
+                                                         const float *__restrict__ x, double *__restrict__ y,
+                                                         float ratio, uint32_t *queue, uint32_t *qcount)
+{
+    const uint64_t gid = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    const uint64_t row = gid / G;
+    const uint32_t lane = (uint32_t)(gid % G);
+    if (row >= n_rows) return;
+    const uint32_t b = off[row], e = off[row + 1];
+    dot2 d;
+    for (uint32_t k = b + lane; k < e; k += G) d.add(vals(k), x[ld_stream(col + k)]);
+    d.group_reduce(G);
+    if (lane == 0) finish_row(d, row, y, ratio, queue, qcount);
+}
+
+// STREAM<C>: LDS-staged row blocks (same plan as the FP64 kernel); the LDS
+// holds each product and its exact error.
+template <int C, class Vals>
+__global__ __launch_bounds__(kThreads) void k_mixed_stream(const uint32_t *__restrict__ blk_row,
+                                                            const uint32_t *__restrict__ off,
+                                                            const uint32_t *__restrict__ col, Vals vals,
+                                                            const float *__restrict__ x, double *__restrict__ y,
+                                                            float ratio, uint32_t *queue, uint32_t *qcount)
+{
+    __shared__ float s_p[2 * C];
+    __shared__ float s_e[2 * C];
+    __shared__ uint32_t s_off[C + 1];
+    __shared__ float s_red[3][kThreads / 64];
+    const uint32_t r0 = blk_row[blockIdx.x], r1 = blk_row[blockIdx.x + 1];
+    const uint32_t nr = r1 - r0;
+    const uint32_t k0 = off[r0], k1 = off[r1];
+    if (nr == 1) {  // one (possibly very long) row: all lanes
+        dot2 d;
+        for (uint32_t k = k0 + threadIdx.x; k < k1; k += kThreads) d.add(vals(k), x[ld_stream(col + k)]);
+        d.group_reduce(64);
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) {
+            s_red[0][w] = d.s;
+            s_red[1][w] = d.c;
+            s_red[2][w] = d.a;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            dot2 t;
+            for (int i = 0; i < kThreads / 64; ++i) t.merge(s_red[0][i], s_red[1][i], s_red[2][i]);
+            finish_row(t, r0, y, ratio, queue, qcount);
+        }
+        return;
+    }
+    const uint32_t nk = k1 - k0;  // < 2C
+    for (uint32_t i = threadIdx.x; i < nk; i += kThreads) {
+        const float a = vals(k0 + i);
+        const float xv = x[ld_stream(col + k0 + i)];
+        const float p = a * xv;
+        s_p[i] = p;
+        s_e[i] = fmaf(a, xv, -p);
+    }
+    for (uint32_t j = threadIdx.x; j <= nr; j += kThreads) s_off[j] = off[r0 + j] - k0;
+    __syncthreads();
+    int T = 1;
+    while (T < 64 && (uint32_t)(2 * T) * nr <= (uint32_t)kThreads) T <<= 1;
+    const uint32_t teams = kThreads / T;
+    const uint32_t lane = threadIdx.x & (T - 1);
+    for (uint32_t r = threadIdx.x / T; r < nr; r += teams) {
+        const uint32_t a = s_off[r], e = s_off[r + 1];
+        dot2 d;
+        for (uint32_t k = a + lane; k < e; k += T) d.add_prod(s_p[k], s_e[k]);
+        d.group_reduce(T);
+        if (lane == 0) finish_row(d, r0 + r, y, ratio, queue, qcount);
+    }
+}
+
+// FP64 recompute of queued rows: one wavefront per row.
+__global__ __launch_bounds__(kThreads) void k_recompute_f64(const uint32_t *__restrict__ queue,
+                                                             const uint32_t *__restrict__ qcount,
+                                                             const uint64_t *__restrict__ off,
+                                                             const uint32_t *__restrict__ col,
+                                                             const double *__restrict__ val,
+                                                             const double *__restrict__ x, double *__restrict__ y)
+{
+    const uint32_t count = *qcount;
+    const uint32_t waves = gridDim.x * (kThreads / 64);
+    const int lane = threadIdx.x & 63;
+    for (uint32_t w = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); w < count; w += waves) {
+        const uint32_t row = queue[w];
+        double acc = 0.0;
+        for (uint64_t k = off[row] + lane; k < off[row + 1]; k += 64) acc += val[k] * x[col[k]];
+        acc = group_sum<64>(acc);
+        if (lane == 0) y[row] = acc;
+    }
+}
+
+// ---- storage conversion ------------------------------------------------------
+__global__ void k_to_f32(const double *__restrict__ v, float *__restrict__ o, uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) o[i] = (float)v[i];
+}
+
+__global__ void k_to_bf16(const double *__restrict__ v, uint16_t *__restrict__ o, uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) o[i] = f32_to_bf16((float)v[i]);
+}
+
+__global__ void k_to_split(const double *__restrict__ v, uint16_t *__restrict__ hi, uint16_t *__restrict__ lo,
+                           uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint16_t h = f32_to_bf16((float)v[i]);
+    hi[i] = h;
+    lo[i] = f32_to_bf16((float)(v[i] - (double)bf16_to_f32(h)));
+}
+
+void ensure_storage(spira_dcsr *A, int storage)
+{
+    spira_dev *dev = A->dev;
+    const uint64_t n = A->nnz, cap = A->capacity ? A->capacity : 1;
+    const unsigned grid = grid_for(n ? n : 1, kThreads);
+    if (storage == SPIRA_VAL_F32 && !A->val_f32) {
+        A->val_f32 = (float *)dev_alloc(dev, cap * sizeof(float));
+        k_to_f32<<<grid, kThreads, 0, dev->stream>>>(A->val, A->val_f32, n);
+    } else if (storage == SPIRA_VAL_BF16 && !A->val_bf16) {
+        A->val_bf16 = (uint16_t *)dev_alloc(dev, cap * sizeof(uint16_t));
+        k_to_bf16<<<grid, kThreads, 0, dev->stream>>>(A->val, A->val_bf16, n);
+    } else if (storage == SPIRA_VAL_SPLIT_BF16 && !A->val_split_hi) {
+        uint16_t *hi = (uint16_t *)dev_alloc(dev, cap * sizeof(uint16_t));
+        uint16_t *lo = (uint16_t *)dev_alloc(dev, cap * sizeof(uint16_t));
+        k_to_split<<<grid, kThreads, 0, dev->stream>>>(A->val, hi, lo, n);
+        A->val_split_hi = hi;
+        A->val_bf16_lo = lo;
+    }
+    SPIRA_HIP(hipGetLastError());
+}
+
+template <class Vals>
+void launch_mixed(spira_dcsr *A, int variant, Vals vals, const float *x32, double *y, float ratio, uint32_t *queue,
+                  uint32_t *qcount, const uint32_t *blk, uint32_t nblk)
+{
+    hipStream_t st = A->dev->stream;
+    const uint32_t *off = A->plan.off32;
+    const uint64_t n = A->n_rows;
+    switch (variant) {
+    case SPIRA_SPMV_VEC4:
+        k_mixed_vec<4, Vals><<<grid_for(n * 4, kThreads), kThreads, 0, st>>>(n, off, A->col, vals, x32, y, ratio,
+                                                                            queue, qcount);
+        break;
+    case SPIRA_SPMV_VEC8:
+        k_mixed_vec<8, Vals><<<grid_for(n * 8, kThreads), kThreads, 0, st>>>(n, off, A->col, vals, x32, y, ratio,
+                                                                            queue, qcount);
+        break;
+    case SPIRA_SPMV_VEC16:
+        k_mixed_vec<16, Vals><<<grid_for(n * 16, kThreads), kThreads, 0, st>>>(n, off, A->col, vals, x32, y,
+                                                                              ratio, queue, qcount);
+        break;
+    case SPIRA_SPMV_VEC32:
+        k_mixed_vec<32, Vals><<<grid_for(n * 32, kThreads), kThreads, 0, st>>>(n, off, A->col, vals, x32, y,
+                                                                              ratio, queue, qcount);
+        break;
+    case SPIRA_SPMV_STREAM1K:
+        k_mixed_stream<1024, Vals><<<nblk, kThreads, 0, st>>>(blk, off, A->col, vals, x32, y, ratio, queue, qcount);
+        break;
+    default:
+        k_mixed_stream<2048, Vals><<<nblk, kThreads, 0, st>>>(blk, off, A->col, vals, x32, y, ratio, queue, qcount);
+        break;
+    }
+    SPIRA_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+bool mixed_variant_supported(int variant)
+{
+    return variant == SPIRA_SPMV_VEC4 || variant == SPIRA_SPMV_VEC8 || variant == SPIRA_SPMV_VEC16 ||
+           variant == SPIRA_SPMV_VEC32 || variant == SPIRA_SPMV_STREAM1K || variant == SPIRA_SPMV_STREAM2K;
+}
+
+uint32_t *mixed_queue_count(spira_dcsr *A)
+{
+    return (uint32_t *)A->dev->ws.get(WS_MIXQ, (A->n_rows + 16) * sizeof(uint32_t)) + A->n_rows + 8;
+}
+
+void spmv_mixed_launch(spira_dcsr *A, int storage, const double *x, double *y, int variant, float ratio)
+{
+    if (A->n_rows == 0) return;
+    if (A->nnz >= (1ull << 32)) fail(SPIRA_ERR_UNSUPPORTED, "mixed spmv: nnz must be < 2^32");
+    spmv_build_plan(A);
+    spira_dev *dev = A->dev;
+    hipStream_t st = dev->stream;
+    if (variant < 0) {
+        variant = spmv_heuristic_variant(A);
+        if (!mixed_variant_supported(variant)) variant = SPIRA_SPMV_STREAM2K;
+    }
+    if (!mixed_variant_supported(variant))
+        fail(SPIRA_ERR_INVALID_ARGUMENT, "mixed spmv: variant must be VEC4..VEC32, STREAM1K or STREAM2K");
+    ensure_storage(A, storage);
+    const uint32_t *blk = nullptr;
+    uint32_t nblk = 0;
+    if (variant == SPIRA_SPMV_STREAM1K || variant == SPIRA_SPMV_STREAM2K) {
+        const int which = variant == SPIRA_SPMV_STREAM1K ? 0 : 1;
+        spmv_stream_blocks(A, which, &blk, &nblk);
+    }
+    uint32_t *queue = (uint32_t *)dev->ws.get(WS_MIXQ, (A->n_rows + 16) * sizeof(uint32_t));
+    uint32_t *qcount = queue + A->n_rows + 8;
+    float *x32 = (float *)dev->ws.get(WS_X32, (A->n_cols ? A->n_cols : 1) * sizeof(float));
+    k_x_to_f32<<<grid_for(A->n_cols ? A->n_cols : 1, kThreads), kThreads, 0, st>>>(x, x32, A->n_cols, qcount);
+    SPIRA_HIP(hipGetLastError());
+    if (storage == SPIRA_VAL_F32)
+        launch_mixed(A, variant, vals_f32{A->val_f32}, x32, y, ratio, queue, qcount, blk, nblk);
+    else if (storage == SPIRA_VAL_BF16)
+        launch_mixed(A, variant, vals_bf16{A->val_bf16}, x32, y, ratio, queue, qcount, blk, nblk);
+    else
+        launch_mixed(A, variant, vals_split{A->val_split_hi, A->val_bf16_lo}, x32, y, ratio, queue, qcount, blk,
+                     nblk);
+    if (ratio > 0.0f) {
+        const unsigned g = std::min<uint64_t>(grid_for(A->n_rows, kThreads / 64), 2048u);
+        k_recompute_f64<<<g, kThreads, 0, st>>>(queue, qcount, A->off, A->col, A->val, x, y);
+        SPIRA_HIP(hipGetLastError());
+    }
+}
+
+}  // namespace spira_amd

@@ -66,6 +66,61 @@ def random_uniform_rows(n_rows, n_cols, lo, hi, seed=BASE_SEED, chunk=1 << 20):
     return off, cols, vals
 
 
+def powerlaw_rows(n_rows, s=1.8, clip=50000, local_frac=0.7, band=4096, seed=BASE_SEED ^ 0xDEADBEEF,
+                  values="normal"):
+    """Power-law rows: lengths Zipf(s) clipped to [1, clip] (numpy's zipf, so
+    the clipped mean for s = 1.8 is about 26, not 12); each column is, with
+    probability local_frac, uniform in [r - band, r + band] (clipped to the
+    matrix) and otherwise uniform over all columns; columns distinct within a
+    row; values N(0, 1) (or U(-1, 1) with values="uniform").  Square matrix."""
+    rng = np.random.default_rng(seed)
+    n = n_rows
+    lens = np.minimum(rng.zipf(s, size=n), min(clip, n)).astype(np.int64)
+    off = np.zeros(n + 1, np.uint64)
+    np.cumsum(lens, out=off[1:])
+    cols = np.empty(int(off[-1]), np.uint32)
+    rows_all = np.arange(n, dtype=np.int64)
+    # rows are handled in width buckets so each bucket is one padded 2-D draw
+    edges = [0, 8, 64, 512, 4096, clip + 1]
+    for lo_w, hi_w in zip(edges[:-1], edges[1:]):
+        sel = rows_all[(lens > lo_w) & (lens <= hi_w)]
+        if sel.size == 0:
+            continue
+        width = int(lens[sel].max())
+        for a in range(0, sel.size, max(1, (1 << 24) // width)):
+            rs = sel[a:a + max(1, (1 << 24) // width)]
+            L = lens[rs]
+            loc = rng.random((rs.size, width)) < local_frac
+            lo = np.maximum(rs - band, 0)[:, None]
+            hi = np.minimum(rs + band + 1, n)[:, None]
+            local = lo + (rng.random((rs.size, width)) * (hi - lo)).astype(np.int64)
+            glob = (rng.random((rs.size, width)) * n).astype(np.int64)
+            draw = np.where(loc, local, glob).astype(np.uint32)
+            pos = np.arange(width)[None, :]
+            valid = pos < L[:, None]
+            draw[~valid] = np.iinfo(np.uint32).max
+            draw.sort(axis=1)
+            dup = ((draw[:, 1:] == draw[:, :-1]) & valid[:, 1:]).any(axis=1)
+            for i in np.nonzero(dup)[0]:
+                want = int(L[i])
+                have = np.unique(draw[i, :want])
+                while have.size < want:  # top up with fresh uniform columns
+                    extra = rng.integers(0, n, size=want - have.size, dtype=np.int64).astype(np.uint32)
+                    have = np.unique(np.concatenate([have, extra]))
+                draw[i, :want] = have[:want]
+            mask = pos < L[:, None]
+            starts = off[rs].astype(np.int64)
+            # rows in rs are increasing, so the masked flatten lands contiguously per row
+            flat = draw[mask]
+            idx = np.repeat(starts, L) + (np.arange(flat.size) - np.repeat(np.cumsum(L) - L, L))
+            cols[idx] = flat
+    if values == "normal":
+        vals = rng.standard_normal(cols.size)
+    else:
+        vals = rng.uniform(-1.0, 1.0, size=cols.size)
+    return off, cols, vals
+
+
 def laplacian_2d(nx, ny=None):
     """5-point Laplacian on an nx x ny grid, Dirichlet boundary: diagonal 4,
     neighbours -1, row-major grid numbering."""

@@ -1,0 +1,121 @@
+"""Mixed-precision SpMV (FP32 / BF16 / split-BF16 value storage, compensated
+FP32 accumulation, FP64 recompute of cancelling rows) against the FP64 oracle.
+
+Tolerances, normwise relative error ||y - y_ref||_2 / ||y_ref||_2 against the
+oracle's FP64 loop on the FP64 values:
+  FP32 storage with recompute ratio 64:  <= 1e-6   (the target)
+  split-BF16 storage:                    <= 1e-4
+  BF16 storage:                          <= 2e-2
+Rows recomputed in FP64 match the FP64 oracle to 1e-12 relative.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import spira_amd
+from spira_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = spira_amd.Device(0)
+    yield d
+    d.close()
+
+
+@pytest.fixture(scope="module")
+def powerlaw(dev):
+    off, col, val = W.powerlaw_rows(200000, seed=5)
+    A = spira_amd.DeviceCSR(dev, 200000, 200000)
+    A.upload(off, col, val)
+    x = np.random.default_rng(1).uniform(-1, 1, 200000)
+    return A, (off, col, val), x, oracle.spmv_f64(off, col, val, x)
+
+
+TOL = {"f32": 1e-6, "split_bf16": 1e-4, "bf16": 2e-2}
+
+
+@pytest.mark.parametrize("storage", ["f32", "split_bf16", "bf16"])
+@pytest.mark.parametrize("variant", spira_amd.MIXED_VARIANTS)
+def test_mixed_error_bounds(powerlaw, storage, variant):
+    A, csr, x, y_ref = powerlaw
+    y, nrec = A.spmv_mixed(x, storage=storage, variant=variant, recompute_ratio=64.0)
+    err = oracle.normwise_rel_err(y, y_ref)
+    assert err <= TOL[storage], (storage, variant, err, nrec)
+
+
+def test_mixed_without_recompute_still_meets_fp32_bound(powerlaw):
+    A, csr, x, y_ref = powerlaw
+    y, nrec = A.spmv_mixed(x, storage="f32", recompute_ratio=0.0)
+    assert nrec == 0
+    assert oracle.normwise_rel_err(y, y_ref) <= 2e-6
+
+
+def test_recomputed_rows_are_fp64(powerlaw):
+    A, (off, col, val), x, y_ref = powerlaw
+    y_plain, _ = A.spmv_mixed(x, storage="bf16", recompute_ratio=0.0)
+    y_rec, nrec = A.spmv_mixed(x, storage="bf16", recompute_ratio=4.0)
+    assert nrec > 0
+    changed = y_rec != y_plain
+    assert changed.any()
+    # FP64 with a different summation order: error bounded by eps * sum|a||x|
+    absum = oracle.spmv_f64(off, col, np.abs(val), np.abs(x))
+    assert np.all(np.abs(y_rec[changed] - y_ref[changed]) <= 1e-14 * absum[changed])
+
+
+def test_dot2_recovers_cancellation_exactly(dev):
+    # 1e8 + 1 - 1e8: plain FP32 accumulation loses the 1; Dot2 keeps it.
+    off = np.array([0, 3, 5], np.uint64)
+    col = np.array([0, 1, 2, 0, 2], np.uint32)
+    val = np.array([1e8, 1.0, -1e8, 3.0, -3.0])
+    A = spira_amd.DeviceCSR(dev, 2, 3)
+    A.upload(off, col, val)
+    for variant in spira_amd.MIXED_VARIANTS:
+        y, _ = A.spmv_mixed(np.ones(3), storage="f32", variant=variant, recompute_ratio=0.0)
+        assert y.tolist() == [1.0, 0.0], variant
+
+
+def test_recompute_fixes_values_not_representable_in_fp32(dev):
+    # a = 1 + 2^-30 rounds to 1.0f: FP32 gives 0, the FP64 recompute gives 2^-30.
+    off = np.array([0, 2], np.uint64)
+    col = np.array([0, 1], np.uint32)
+    val = np.array([1.0 + 2.0 ** -30, -1.0])
+    A = spira_amd.DeviceCSR(dev, 1, 2)
+    A.upload(off, col, val)
+    y0, n0 = A.spmv_mixed(np.ones(2), storage="f32", recompute_ratio=0.0)
+    assert y0[0] == 0.0 and n0 == 0
+    y1, n1 = A.spmv_mixed(np.ones(2), storage="f32", recompute_ratio=1e6)
+    assert y1[0] == 2.0 ** -30 and n1 == 1
+
+
+def test_mixed_device_pointers_and_errors(dev, powerlaw):
+    import torch
+    A, csr, x, y_ref = powerlaw
+    xt = torch.from_numpy(x).cuda()
+    yt = torch.empty(A.n_rows, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    A.spmv_mixed(xt, yt, storage="f32", recompute_ratio=64.0, device=True)
+    assert oracle.normwise_rel_err(yt.cpu().numpy(), y_ref) <= 1e-6
+    with pytest.raises(spira_amd.InvalidArgument):
+        A.spmv_mixed(x[:-1], storage="f32")
+    with pytest.raises(spira_amd.InvalidArgument):
+        A.spmv_mixed(x, storage="f32", variant=spira_amd.VARIANTS["xstream"])
+    B = spira_amd.DeviceCSR(dev, 3, 3)
+    with pytest.raises(spira_amd.LogicError):
+        B.spmv_mixed(np.ones(3), storage="f32")
+
+
+def test_storage_copies_dropped_by_lock(dev):
+    off, col, val = W.random_uniform_rows(1000, 1000, 1, 20, seed=3)
+    A = spira_amd.DeviceCSR(dev, 1000, 1000)
+    A.upload(off, col, val)
+    x = np.random.default_rng(0).uniform(-1, 1, 1000)
+    A.spmv_mixed(x, storage="f32")
+    rows = np.arange(1000)
+    cols = np.zeros(1000, np.uint32)
+    A.lock(*spira_amd.stage_from_coo(1000, rows, cols, np.full(1000, 2.5)))
+    exp = oracle.lock(*spira_amd.stage_from_coo(1000, rows, cols, np.full(1000, 2.5)), old=(off, col, val))
+    y, _ = A.spmv_mixed(x, storage="f32", recompute_ratio=64.0)
+    assert oracle.normwise_rel_err(y, oracle.spmv_f64(*exp, x)) <= 1e-6
