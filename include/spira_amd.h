@@ -155,6 +155,29 @@ int spira_spmv_autotune_f64(spira_dcsr *A, const double *x_dev, double *y_dev, i
 int spira_spmv_mixed_f64(spira_dcsr *A, int storage, const double *x, uint64_t x_len, double *y, uint64_t y_len,
                          int variant, float recompute_ratio, uint32_t flags, uint64_t *n_recomputed);
 
+/* ---- batches: many matrices, k right-hand sides, one launch ---------------
+ * No counterpart in the reference (it runs one matrix and one vector at a
+ * time, serial/spmv.hpp:142-186).  A batch holds locked matrices as one
+ * block-diagonal CSR with reduced-precision values (SPIRA_VAL_*): X is the
+ * column-wise concatenation of every matrix's n_cols x k block (row major,
+ * FP32), Y likewise with n_rows (FP32).  spira_batch_bases gives each matrix's
+ * first row and column.  A schedule fixes k (1, 4, 16 or 32) and, per matrix,
+ * the lanes per row (1 << elog2[m], at most 64 / k) and the nonzeros per work
+ * tile (NULL = built-in choices).  spira_batch_spmm runs every tile in one
+ * persistent launch fed by a device work queue; each output is accumulated in
+ * FP32 with compensated (Dot2) summation.  The call can be captured once into
+ * a hipGraph (bound to the X and Y given) and replayed.  X and Y are device
+ * pointers; flags: SPIRA_NO_SYNC.  Destroy batches before their device. */
+typedef struct spira_batch spira_batch;
+int spira_batch_create(spira_dev *dev, uint32_t n_mats, spira_dcsr *const *mats, int storage, spira_batch **out);
+void spira_batch_destroy(spira_batch *b);
+int spira_batch_info(const spira_batch *b, uint64_t *n_rows, uint64_t *n_cols, uint64_t *nnz, uint32_t *n_tiles);
+int spira_batch_bases(const spira_batch *b, uint64_t *row_base, uint64_t *col_base); /* n_mats+1 each */
+int spira_batch_set_schedule(spira_batch *b, uint32_t k, const uint8_t *elog2, const uint32_t *tile_nnz);
+int spira_batch_spmm(spira_batch *b, const float *X, float *Y, uint32_t flags);
+int spira_batch_graph_capture(spira_batch *b, const float *X, float *Y);
+int spira_batch_graph_launch(spira_batch *b, uint32_t flags);
+
 #ifdef __cplusplus
 }
 #endif

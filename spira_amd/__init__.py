@@ -118,6 +118,8 @@ _shutdown = False
 @atexit.register
 def _release_all():
     global _shutdown
+    for B in list(globals().get("_live_batch", ())):
+        B.close()
     for A in list(_live_csr):
         A.close()
     for d in list(_live_dev):
@@ -157,6 +159,15 @@ def lib():
                                                        ctypes.POINTER(ctypes.c_int)]),
             "spira_spmv_mixed_f64": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_uint64, _vp, ctypes.c_uint64,
                                                     ctypes.c_int, ctypes.c_float, ctypes.c_uint32, _u64p]),
+            "spira_batch_create": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(_vp), ctypes.c_int,
+                                                  ctypes.POINTER(_vp)]),
+            "spira_batch_destroy": (None, [_vp]),
+            "spira_batch_info": (ctypes.c_int, [_vp, _u64p, _u64p, _u64p, _u32p]),
+            "spira_batch_bases": (ctypes.c_int, [_vp, _vp, _vp]),
+            "spira_batch_set_schedule": (ctypes.c_int, [_vp, ctypes.c_uint32, _vp, _vp]),
+            "spira_batch_spmm": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_uint32]),
+            "spira_batch_graph_capture": (ctypes.c_int, [_vp, _vp, _vp]),
+            "spira_batch_graph_launch": (ctypes.c_int, [_vp, ctypes.c_uint32]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -206,7 +217,10 @@ class Device:
 
     def close(self):
         if getattr(self, "_h", None) and not _shutdown:
-            for A in list(_live_csr):  # matrices of this device go first
+            for B in list(_live_batch):  # batches, then matrices of this device go first
+                if B.dev is self:
+                    B.close()
+            for A in list(_live_csr):
                 if A.dev is self:
                     A.close()
             lib().spira_dev_destroy(self._h)
@@ -356,6 +370,64 @@ class DeviceCSR:
     def close(self):
         if getattr(self, "_h", None) and not _shutdown:
             lib().spira_dcsr_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_live_batch = weakref.WeakSet()
+
+
+class Batch:
+    """spira_batch: locked matrices as one block-diagonal batch (reduced-
+    precision values), k right-hand sides, one persistent launch per call."""
+
+    def __init__(self, dev: Device, mats, storage: str = "f32"):
+        self.dev = dev
+        self.mats = list(mats)
+        arr = (_vp * max(1, len(self.mats)))(*[m.handle for m in self.mats])
+        h = _vp()
+        dev.check(lib().spira_batch_create(dev.handle, len(self.mats), arr, STORAGE[storage], ctypes.byref(h)))
+        self._h = h
+        _live_batch.add(self)
+        a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        t = ctypes.c_uint32()
+        lib().spira_batch_info(h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(t))
+        self.n_rows, self.n_cols, self.nnz = a.value, b.value, c.value
+        rb = np.zeros(len(self.mats) + 1, np.uint64)
+        cb = np.zeros(len(self.mats) + 1, np.uint64)
+        lib().spira_batch_bases(h, _ptr(rb), _ptr(cb))
+        self.row_base, self.col_base = rb, cb
+        self.k = 0
+
+    def n_tiles(self) -> int:
+        t = ctypes.c_uint32()
+        lib().spira_batch_info(self._h, None, None, None, ctypes.byref(t))
+        return t.value
+
+    def set_schedule(self, k: int, elog2=None, tile_nnz=None):
+        e = None if elog2 is None else np.ascontiguousarray(elog2, dtype=np.uint8)
+        t = None if tile_nnz is None else np.ascontiguousarray(tile_nnz, dtype=np.uint32)
+        self.dev.check(lib().spira_batch_set_schedule(self._h, int(k), _ptr(e), _ptr(t)))
+        self.k = int(k)
+
+    def spmm(self, X, Y, sync: bool = True):
+        self.dev.check(lib().spira_batch_spmm(self._h, _ptr(X), _ptr(Y), 0 if sync else NO_SYNC))
+        return Y
+
+    def capture(self, X, Y):
+        self.dev.check(lib().spira_batch_graph_capture(self._h, _ptr(X), _ptr(Y)))
+
+    def replay(self, sync: bool = True):
+        self.dev.check(lib().spira_batch_graph_launch(self._h, 0 if sync else NO_SYNC))
+
+    def close(self):
+        if getattr(self, "_h", None) and not _shutdown:
+            lib().spira_batch_destroy(self._h)
         self._h = None
 
     def __del__(self):

@@ -121,6 +121,42 @@ def powerlaw_rows(n_rows, s=1.8, clip=50000, local_frac=0.7, band=4096, seed=BAS
     return off, cols, vals
 
 
+BATCH_ZIPF_CLIP = 1024  # Zipf(1.6) row lengths are clipped to min(n_cols, 1024)
+
+
+def batch_matrix(seed, n_rows=None, min_rows=256, max_rows=65536, device="cpu", zipf_clip=BATCH_ZIPF_CLIP):
+    """One matrix of the batched workload: square, n_rows log-uniform in
+    [min_rows, max_rows] (unless given), nnz/row from a 50/50 mix of uniform
+    4..64 and Zipf(1.6) clipped to min(n_cols, zipf_clip), uniform columns (a
+    row's repeated draws are merged, so rows can come out slightly shorter),
+    values N(0, 1) rounded to FP32 (stored as FP64).  Generated with torch on
+    `device`; returns (offsets u64, cols i32, vals f64) tensors there."""
+    import torch
+    rng = np.random.default_rng(seed)
+    if n_rows is None:
+        n_rows = int(np.exp(rng.uniform(np.log(min_rows), np.log(max_rows + 1))))
+        n_rows = max(min_rows, min(max_rows, n_rows))
+    n = n_rows
+    uni = rng.integers(4, 65, size=n)
+    zpf = np.minimum(rng.zipf(1.6, size=n), min(n, zipf_clip))
+    lens = np.where(rng.random(n) < 0.5, uni, zpf).astype(np.int64)
+    lens = np.minimum(lens, n)
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed) & 0x7fffffffffffffff)
+    lt = torch.from_numpy(lens).to(device)
+    rowid = torch.repeat_interleave(torch.arange(n, device=device, dtype=torch.int64), lt)
+    cols = (torch.rand(rowid.numel(), generator=g, device=device, dtype=torch.float64) * n).long().clamp_(max=n - 1)
+    key, _ = torch.sort(rowid * n + cols)
+    key = torch.unique_consecutive(key)
+    r = key // n
+    c = (key % n).to(torch.int32)
+    counts = torch.bincount(r, minlength=n)
+    off = torch.zeros(n + 1, dtype=torch.int64, device=device)
+    off[1:] = torch.cumsum(counts, 0)
+    vals = torch.randn(c.numel(), generator=g, device=device, dtype=torch.float32).to(torch.float64)
+    return off, c, vals
+
+
 def laplacian_2d(nx, ny=None):
     """5-point Laplacian on an nx x ny grid, Dirichlet boundary: diagonal 4,
     neighbours -1, row-major grid numbering."""
