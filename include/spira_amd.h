@@ -178,6 +178,28 @@ int spira_batch_spmm(spira_batch *b, const float *X, float *Y, uint32_t flags);
 int spira_batch_graph_capture(spira_batch *b, const float *X, float *Y);
 int spira_batch_graph_launch(spira_batch *b, uint32_t flags);
 
+/* ---- learned schedule selector --------------------------------------------
+ * No counterpart in the reference: it replaces the reference's one-time CPUID
+ * pick of a kernel (dispatch.cpp:34-67) and the exhaustive per-matrix search
+ * of spira_spmv_autotune_f64 / batch schedules with a small MLP.
+ * spira_matrix_features writes SPIRA_FEATURES floats describing a locked
+ * matrix (row-length moments and histogram, bandwidth, column spread), computed
+ * on the device.  spira_mlp_* is an n_in -> hidden -> hidden -> n_out ReLU
+ * network in FP32 on the device: forward, masked mean-squared-error gradient
+ * (all parameters, flat: W1, b1, W2, b2, W3, b3; W row-major [out][in]) and an
+ * Adam step.  X, T, M, Y are device arrays (batch x n_in / batch x n_out,
+ * row major); mask_count is the number of M entries that are > 0. */
+#define SPIRA_FEATURES 18
+int spira_matrix_features(spira_dcsr *A, float *features_host);
+typedef struct spira_mlp spira_mlp;
+int spira_mlp_create(spira_dev *dev, uint32_t n_in, uint32_t hidden, uint32_t n_out, uint64_t seed, spira_mlp **out);
+void spira_mlp_destroy(spira_mlp *net);
+int spira_mlp_params(spira_mlp *net, float *get_host, const float *set_host, uint64_t *n_params);
+int spira_mlp_forward(spira_mlp *net, const float *X, uint32_t batch, float *Y);
+int spira_mlp_loss_grad(spira_mlp *net, const float *X, const float *T, const float *M, uint32_t batch,
+                        float mask_count, float *loss_host, float *grad_host);
+int spira_mlp_adam_step(spira_mlp *net, float lr, float beta1, float beta2, float eps);
+
 #ifdef __cplusplus
 }
 #endif

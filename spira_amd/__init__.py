@@ -118,7 +118,7 @@ _shutdown = False
 @atexit.register
 def _release_all():
     global _shutdown
-    for B in list(globals().get("_live_batch", ())):
+    for B in list(globals().get("_live_batch", ())) + list(globals().get("_live_mlp", ())):
         B.close()
     for A in list(_live_csr):
         A.close()
@@ -168,6 +168,15 @@ def lib():
             "spira_batch_spmm": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_uint32]),
             "spira_batch_graph_capture": (ctypes.c_int, [_vp, _vp, _vp]),
             "spira_batch_graph_launch": (ctypes.c_int, [_vp, ctypes.c_uint32]),
+            "spira_matrix_features": (ctypes.c_int, [_vp, _vp]),
+            "spira_mlp_create": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                                ctypes.c_uint64, ctypes.POINTER(_vp)]),
+            "spira_mlp_destroy": (None, [_vp]),
+            "spira_mlp_params": (ctypes.c_int, [_vp, _vp, _vp, _u64p]),
+            "spira_mlp_forward": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp]),
+            "spira_mlp_loss_grad": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_uint32, ctypes.c_float, _vp, _vp]),
+            "spira_mlp_adam_step": (ctypes.c_int, [_vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                                   ctypes.c_float]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -217,7 +226,7 @@ class Device:
 
     def close(self):
         if getattr(self, "_h", None) and not _shutdown:
-            for B in list(_live_batch):  # batches, then matrices of this device go first
+            for B in list(_live_batch) + list(_live_mlp):  # batches, nets, then matrices go first
                 if B.dev is self:
                     B.close()
             for A in list(_live_csr):
@@ -428,6 +437,71 @@ class Batch:
     def close(self):
         if getattr(self, "_h", None) and not _shutdown:
             lib().spira_batch_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+N_FEATURES = 18
+_live_mlp = weakref.WeakSet()
+
+
+def matrix_features(A: DeviceCSR) -> np.ndarray:
+    """SPIRA_FEATURES floats describing a locked matrix (computed on the device)."""
+    f = np.zeros(N_FEATURES, np.float32)
+    A.dev.check(lib().spira_matrix_features(A.handle, _ptr(f)))
+    return f
+
+
+class MLP:
+    """spira_mlp: n_in -> hidden -> hidden -> n_out ReLU network on the device
+    (hand-written HIP forward, backward and Adam)."""
+
+    def __init__(self, dev: Device, n_in: int, hidden: int, n_out: int, seed: int = 0):
+        self.dev = dev
+        h = _vp()
+        dev.check(lib().spira_mlp_create(dev.handle, n_in, hidden, n_out, seed, ctypes.byref(h)))
+        self._h = h
+        _live_mlp.add(self)
+        self.n_in, self.hidden, self.n_out = n_in, hidden, n_out
+        n = ctypes.c_uint64()
+        lib().spira_mlp_params(h, None, None, ctypes.byref(n))
+        self.n_params = int(n.value)
+
+    def get_params(self) -> np.ndarray:
+        p = np.empty(self.n_params, np.float32)
+        self.dev.check(lib().spira_mlp_params(self._h, _ptr(p), None, None))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.n_params
+        self.dev.check(lib().spira_mlp_params(self._h, None, _ptr(p), None))
+
+    def forward(self, X_dev, Y_dev):
+        """X_dev, Y_dev: float32 device tensors (batch x n_in, batch x n_out)."""
+        b = int(X_dev.shape[0])
+        self.dev.check(lib().spira_mlp_forward(self._h, _ptr(X_dev), b, _ptr(Y_dev)))
+        return Y_dev
+
+    def loss_grad(self, X_dev, T_dev, M_dev, mask_count: float, want_grad: bool = False):
+        b = int(X_dev.shape[0])
+        loss = np.zeros(1, np.float32)
+        g = np.empty(self.n_params, np.float32) if want_grad else None
+        self.dev.check(lib().spira_mlp_loss_grad(self._h, _ptr(X_dev), _ptr(T_dev), _ptr(M_dev), b,
+                                                 float(mask_count), _ptr(loss), _ptr(g)))
+        return float(loss[0]), g
+
+    def adam_step(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        self.dev.check(lib().spira_mlp_adam_step(self._h, lr, beta1, beta2, eps))
+
+    def close(self):
+        if getattr(self, "_h", None) and not _shutdown:
+            lib().spira_mlp_destroy(self._h)
         self._h = None
 
     def __del__(self):
