@@ -43,6 +43,18 @@ def parse():
     p.add_argument("--lock-only", action="store_true", help="only the lock() measurement (profiling)")
     p.add_argument("--calib-copy-mb", type=int, default=0,
                    help="also run a plain device copy of this many MiB (PMC traffic calibration)")
+    p.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision power-law section")
+    p.add_argument("--no-batch", action="store_true", help="skip the batched / learned-selector section")
+    p.add_argument("--mixed-rows", type=int, default=1 << 23)
+    p.add_argument("--mixed-reps", type=int, default=50)
+    p.add_argument("--recompute-ratio", type=float, default=16.0,
+                   help="cancellation-ratio threshold of the FP64 row recompute in the mixed section")
+    p.add_argument("--batch-mats", type=int, default=4096)
+    p.add_argument("--batch-seed", type=int, default=20_000_000,
+                   help="first seed of the batch (disjoint from the selector's training and held-out seeds)")
+    p.add_argument("--batch-reps", type=int, default=64)
+    p.add_argument("--no-batch-exhaustive", dest="batch_exhaustive", action="store_false",
+                   help="skip the exhaustive autotune of the whole batch (the selector's judge)")
     return p.parse_args()
 
 
@@ -97,6 +109,226 @@ def run_lock(args, np, torch, spira_amd, W, dev, stream):
         detail["ms_host_inputs_pcie_inclusive"] = round(1e3 * float(np.median(t_host[1:])), 3)
     L.close()
     return detail, (o3, c3, v3, sp3, sc3, sv3, n3, lbytes)
+
+
+def _time_calls(torch, stream, fn, reps):
+    """Mean ms per call of fn() over reps back-to-back calls on `stream`."""
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def _nrm_err(torch, y, y_ref):
+    d = torch.linalg.vector_norm((y.double() - y_ref.double()))
+    r = torch.linalg.vector_norm(y_ref.double())
+    return float(d / r) if float(r) > 0 else float(d)
+
+
+def run_mixed(args, np, torch, spira_amd, W, dev, stream):
+    """Mixed-precision SpMV on power-law rows: 8,388,608 rows, lengths
+    Zipf(1.8) clipped to [1, 50,000], 70% of columns within +-4096 of the
+    diagonal, values N(0, 1).  FP64 plain path vs FP32 / BF16 / split-BF16
+    value storage with compensated FP32 accumulation, each with its error
+    against the FP64 result."""
+    n = args.mixed_rows
+    t0 = time.time()
+    with torch.cuda.stream(stream):
+        off, col, val = W.powerlaw_rows_device(n, device="cuda")
+        A = spira_amd.DeviceCSR(dev, n, n)
+        A.upload(off, col, val, device=True)
+        x = torch.from_numpy(W.dense_x(n)).to("cuda")
+        y64 = torch.empty(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(y64)
+    stream.synchronize()
+    nnz = A.nnz
+    log(f"bench: mixed: {n} rows, {nnz} nnz generated and uploaded in {time.time() - t0:.1f}s")
+    reps = args.mixed_reps
+    best, at = A.autotune(x, y64, reps=3)
+    A.spmv(x, y64, device=True, sync=True)
+    t64 = _time_calls(torch, stream, lambda: A.spmv(x, y64, device=True, sync=False), reps)
+    b64 = W.spmv_bytes(n, n, nnz, s_val=8)
+    out = {"workload": f"power-law rows: {n} rows, Zipf(1.8) lengths clipped to [1, 50000], 70% of columns "
+                       f"within +-4096 of the diagonal, values N(0,1), x U(-1,1)",
+           "n_rows": n, "nnz": nnz, "mean_row": round(nnz / n, 2),
+           "fp64": {"variant": spira_amd.VARIANT_NAMES[best], "ms": round(t64, 4), "bytes_per_call": b64,
+                    "GB_s": round(b64 / t64 / 1e6, 1), "GFLOP_s": round(2 * nnz / t64 / 1e6, 1)},
+           "storage": {}}
+    sval = {"f32": 4, "bf16": 2, "split_bf16": 4}
+    for st in ("f32", "bf16", "split_bf16"):
+        # fastest of the mixed kernels for this storage
+        times = {}
+        for v in spira_amd.MIXED_VARIANTS:
+            A.spmv_mixed(x, y, storage=st, variant=v, device=True, sync=True, count=False)
+            times[v] = _time_calls(torch, stream, lambda: A.spmv_mixed(x, y, storage=st, variant=v, device=True,
+                                                                       sync=False, count=False), 5)
+        v = min(times, key=times.get)
+        bts = W.spmv_bytes(n, n, nnz, s_val=sval[st])
+        rec = {"variant": spira_amd.VARIANT_NAMES[v], "bytes_per_call": bts}
+        for ratio in (0.0, args.recompute_ratio):
+            _, nrec = A.spmv_mixed(x, y, storage=st, variant=v, recompute_ratio=ratio, device=True, sync=True)
+            err = _nrm_err(torch, y, y64)
+            t = _time_calls(torch, stream, lambda: A.spmv_mixed(x, y, storage=st, variant=v, recompute_ratio=ratio,
+                                                                device=True, sync=False, count=False), reps)
+            rec["recompute_off" if ratio == 0.0 else f"recompute_ratio_{ratio:g}"] = {
+                "ms": round(t, 4), "GB_s": round(bts / t / 1e6, 1), "GFLOP_s": round(2 * nnz / t / 1e6, 1),
+                "speedup_vs_fp64": round(t64 / t, 3), "rows_recomputed": nrec,
+                "normwise_rel_err_vs_fp64": err}
+        out["storage"][st] = rec
+        log(f"bench: mixed {st}: {rec}")
+    host = None
+    if not args.no_cpu:
+        host = (off.cpu().numpy().astype(np.uint64), col.cpu().numpy().astype(np.uint32), val.cpu().numpy(),
+                x.cpu().numpy())
+    del off, col, val
+    return out, (A, x, y64, y, host)
+
+
+def _batch_x(torch, n_cols, k, seed):
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    return (torch.rand((n_cols, k), generator=g, device="cuda", dtype=torch.float32) * 2 - 1).contiguous()
+
+
+def run_batch(args, np, torch, spira_amd, W, dev, stream):
+    """Batched SpMV with learned scheduling: args.batch_mats square matrices
+    (rows log-uniform in [256, 65536], nnz/row a 50/50 mix of uniform 4..64
+    and Zipf(1.6) clipped to min(n, 1024), FP32 values N(0, 1)), k = 1, 4, 16,
+    32 right-hand sides, one persistent launch per product replayed from a
+    hipGraph, against the plain FP64 path (the drop-in spira_spmv_f64 on the
+    same matrices as one block-diagonal FP64 CSR, one call per right-hand
+    side)."""
+    from spira_amd import selector as S
+    seed0 = args.batch_seed
+    t0 = time.time()
+    mats, offs, cols, vals = [], [], [], []
+    with torch.cuda.stream(stream):
+        for i in range(args.batch_mats):
+            o, c, v = W.batch_matrix(seed0 + i, device="cuda")
+            nr = o.numel() - 1
+            A = spira_amd.DeviceCSR(dev, nr, nr)
+            A.upload(o, c, v, device=True)
+            mats.append(A)
+            offs.append(o)
+            cols.append(c)
+            vals.append(v)
+        # block-diagonal FP64 copy for the plain path
+        rows = np.array([A.n_rows for A in mats], np.int64)
+        nnzs = np.array([A.nnz for A in mats], np.int64)
+        rb = np.concatenate([[0], np.cumsum(rows)])
+        nb = np.concatenate([[0], np.cumsum(nnzs)])
+        big_off = torch.cat([o[:-1] + int(nb[i]) for i, o in enumerate(offs)]
+                            + [torch.tensor([int(nb[-1])], dtype=torch.int64, device="cuda")])
+        big_col = torch.cat([c + int(rb[i]) for i, c in enumerate(cols)])
+        big_val = torch.cat(vals)
+        del offs, cols, vals
+        N = int(rb[-1])
+        P = spira_amd.DeviceCSR(dev, N, N)
+        P.upload(big_off, big_col, big_val, device=True)
+        del big_off, big_col, big_val
+        batch = spira_amd.Batch(dev, mats, "f32")
+    stream.synchronize()
+    nnz = int(batch.nnz)
+    n_cols = np.array([A.n_cols for A in mats])
+    log(f"bench: batch: {len(mats)} matrices, {N} rows, {nnz} nnz built in {time.time() - t0:.1f}s")
+    sel = None
+    if os.path.exists(S.MODEL_PATH):
+        try:
+            sel = S.Selector.load(dev)
+        except ValueError as e:
+            log(f"bench: selector model not used: {e}")
+    t1 = time.time()
+    feats = {k: np.stack([S.features_for(A, k) for A in mats]) for k in S.K_LIST}
+    choice = {k: sel.predict(feats[k], [k] * len(mats), n_cols) for k in S.K_LIST} if sel is not None else {}
+    t_select = time.time() - t1
+    out = {"workload": f"{len(mats)} square matrices (seeds {seed0}..{seed0 + len(mats) - 1}), rows log-uniform in "
+                       f"[256, 65536], nnz/row 50/50 uniform 4..64 / Zipf(1.6) clipped to min(n, 1024), "
+                       f"FP32 values N(0,1), X U(-1,1) FP32",
+           "n_matrices": len(mats), "rows": N, "nnz": nnz,
+           "plain_fp64_path": "spira_spmv_f64 (autotuned variant) on the block-diagonal FP64 CSR, one call per "
+                              "right-hand side",
+           "batched_path": "spira_batch: FP32 values, FP32 X/Y, Dot2 compensated accumulation, one persistent "
+                           "work-queue launch per product, replayed from a hipGraph",
+           "per_k": {}}
+    y64 = torch.empty(N, dtype=torch.float64, device="cuda")
+    x0 = torch.empty(N, dtype=torch.float64, device="cuda").uniform_(-1, 1)
+    best_v, _ = P.autotune(x0, y64, reps=2)
+    out["plain_fp64_variant"] = spira_amd.VARIANT_NAMES[best_v]
+    tot_flops, tot_t_batch, tot_t_plain = 0.0, 0.0, 0.0
+    worst_err = 0.0
+    for k in S.K_LIST:
+        X = _batch_x(torch, N, k, 1000 + k)
+        Y = torch.empty((N, k), dtype=torch.float32, device="cuda")
+        xs = [X[:, j].double().contiguous() for j in range(k)]
+        ys = [torch.empty(N, dtype=torch.float64, device="cuda") for _ in range(k)]
+        stream.synchronize()
+        reps = max(3, args.batch_reps // k)
+
+        def plain():
+            for j in range(k):
+                P.spmv(xs[j], ys[j], device=True, sync=False)
+
+        plain()
+        t_plain = _time_calls(torch, stream, plain, reps)
+        rec = {"plain_fp64_ms": round(t_plain, 4), "plain_fp64_GFLOP_s": round(2 * nnz * k / t_plain / 1e6, 1)}
+        scheds = {"heuristic": None}
+        if args.batch_exhaustive:
+            # exhaustive autotune of the whole batch (in-batch costs per matrix)
+            t1 = time.time()
+            T = S.exhaustive(batch, k, n_cols, X, Y, reps=1)
+            rec["exhaustive_autotune_s"] = round(time.time() - t1, 2)
+            scheds["exhaustive"] = np.nanargmin(T, axis=1)
+            if sel is not None:
+                rec["selector_cost_fraction_of_best"] = round(float(np.nansum(np.nanmin(T, 1)) / np.nansum(
+                    T[np.arange(len(mats)), choice[k]])), 4)
+        if sel is not None:
+            scheds["selector"] = choice[k]
+        for name, idx in scheds.items():
+            if idx is None:
+                batch.set_schedule(k)
+            else:
+                S.apply(batch, k, idx, n_cols)
+            batch.capture(X, Y)
+            batch.replay(sync=True)
+            t = _time_calls(torch, stream, lambda: batch.replay(sync=False), reps)
+            rec[f"{name}_ms"] = round(t, 4)
+            rec[f"{name}_GFLOP_s"] = round(2 * nnz * k / t / 1e6, 1)
+            rec[f"{name}_GB_s"] = round(W.spmv_bytes(N, N, nnz, s_val=4, s_x=4, s_y=4, k=k) / t / 1e6, 1)
+        # error of the last schedule's result against the FP64 path
+        batch.replay(sync=True)
+        plain()
+        stream.synchronize()
+        Y64 = torch.stack(ys, dim=1)
+        err = _nrm_err(torch, Y, Y64)
+        worst_err = max(worst_err, err)
+        t_b = rec.get("selector_ms", rec["heuristic_ms"])
+        rec["speedup_vs_plain_fp64"] = round(t_plain / t_b, 3)
+        if "exhaustive_ms" in rec:
+            for name in ("selector", "heuristic"):
+                if f"{name}_ms" in rec:
+                    rec[f"{name}_fraction_of_exhaustive"] = round(rec["exhaustive_ms"] / rec[f"{name}_ms"], 4)
+        rec["normwise_rel_err_vs_fp64"] = err
+        out["per_k"][str(k)] = rec
+        tot_flops += 2.0 * nnz * k
+        tot_t_batch += t_b
+        tot_t_plain += t_plain
+        log(f"bench: batch k={k}: {rec}")
+        del X, Y, xs, ys, Y64
+    out["GFLOP_s"] = round(tot_flops / tot_t_batch / 1e6, 1)
+    out["plain_fp64_GFLOP_s"] = round(tot_flops / tot_t_plain / 1e6, 1)
+    out["speedup_vs_plain_fp64"] = round(tot_t_plain / tot_t_batch, 3)
+    out["normwise_rel_err_vs_fp64_max"] = worst_err
+    out["schedule"] = "selector" if sel is not None else "heuristic"
+    if sel is not None:
+        out["selector_features_and_inference_s"] = round(t_select, 3)
+    batch.close()
+    P.close()
+    for A in mats:
+        A.close()
+    return out
 
 
 def main():
@@ -230,6 +462,8 @@ def main():
             detail[name]["autotune_ms"] = info[name]["autotune_ms"]
 
     lock_detail, lock_src = (None, None) if args.no_lock else run_lock(args, np, torch, spira_amd, W, dev, stream)
+    mixed_detail, mixed_src = (None, None) if args.no_mixed else run_mixed(args, np, torch, spira_amd, W, dev, stream)
+    batch_detail = None if args.no_batch else run_batch(args, np, torch, spira_amd, W, dev, stream)
 
     achieved = info[dominant]["bytes"] / (kern_ms * 1e-3) / 1e9
     traffic = profile_traffic(dominant + ":" + info[dominant]["variant"])
@@ -301,6 +535,27 @@ def main():
                 p.close()
             lock_detail["cpu_baseline_ms"] = round(1e3 * float(np.median(tl)), 2)
             lock_detail["cpu_baseline_threads"] = threads
+        if mixed_detail is not None:
+            # errors of every storage against the FP64 oracle and its
+            # double-double reference (checker use)
+            A, x, y64, y, (ho, hc, hv, hx) = mixed_src
+            y_or = oracle.spmv_f64(ho, hc, hv, hx)
+            dd_hi, dd_lo = oracle.spmv_dd(ho, hc, hv, hx)
+            A.spmv(x, y64, device=True, sync=True)
+            g64 = y64.cpu().numpy()
+            mixed_detail["fp64"]["normwise_rel_err_vs_fp64_oracle"] = oracle.normwise_rel_err(g64, y_or)
+            mixed_detail["fp64"]["normwise_rel_err_vs_double_double"] = oracle.normwise_rel_err(g64, dd_hi, dd_lo)
+            for st, rec in mixed_detail["storage"].items():
+                v = spira_amd.VARIANTS[rec["variant"]]
+                for key, ratio in (("recompute_off", 0.0), (f"recompute_ratio_{args.recompute_ratio:g}",
+                                                             args.recompute_ratio)):
+                    A.spmv_mixed(x, y, storage=st, variant=v, recompute_ratio=ratio, device=True, sync=True)
+                    got = y.cpu().numpy()
+                    rec[key]["normwise_rel_err_vs_fp64_oracle"] = oracle.normwise_rel_err(got, y_or)
+                    rec[key]["normwise_rel_err_vs_double_double"] = oracle.normwise_rel_err(got, dd_hi, dd_lo)
+            log("bench: mixed-precision errors checked against the FP64 oracle")
+    if mixed_src is not None:
+        mixed_src[0].close()
 
     out = {
         "metric": "SpMV GB/s & % HBM peak; batched mixed-prec GFLOP/s vs plain FP64 at set error",
@@ -324,6 +579,8 @@ def main():
         "cpu_baseline": cpu,
         "detail": detail,
         "lock": lock_detail,
+        "mixed": mixed_detail,
+        "batched": batch_detail,
     }
     print(json.dumps(out), flush=True)
 

@@ -161,20 +161,40 @@ int spira_spmv_mixed_f64(spira_dcsr *A, int storage, const double *x, uint64_t x
  * block-diagonal CSR with reduced-precision values (SPIRA_VAL_*): X is the
  * column-wise concatenation of every matrix's n_cols x k block (row major,
  * FP32), Y likewise with n_rows (FP32).  spira_batch_bases gives each matrix's
- * first row and column.  A schedule fixes k (1, 4, 16 or 32) and, per matrix,
- * the lanes per row (1 << elog2[m], at most 64 / k) and the nonzeros per work
- * tile (NULL = built-in choices).  spira_batch_spmm runs every tile in one
- * persistent launch fed by a device work queue; each output is accumulated in
- * FP32 with compensated (Dot2) summation.  The call can be captured once into
- * a hipGraph (bound to the X and Y given) and replayed.  X and Y are device
- * pointers; flags: SPIRA_NO_SYNC.  Destroy batches before their device. */
+ * first row and column.  Inside the batch each matrix's rows are stored
+ * sorted by length class (rows of 2^(c-1) .. 2^c - 1 nonzeros form class c),
+ * columns as u16 offsets inside their matrix when every matrix has at most
+ * 65536 columns.  A schedule fixes k (1, 4, 16 or 32) and, per matrix
+ * (spira_batch_sched; NULL = built-in choices): lanes per row of class c =
+ * 2^c >> plog2 (at least 1, at most 64 / k), the nonzeros per work item (the
+ * device queue's unit), and whether the matrix's X block (n_cols * k floats,
+ * at most SPIRA_BATCH_XLDS_FLOATS) is staged in LDS for its gathers.
+ * spira_batch_spmm runs every item in one persistent launch fed by a device
+ * work queue; each output is accumulated in FP32 with compensated (Dot2)
+ * summation.  The call can be captured once into a hipGraph (bound to the X
+ * and Y given) and replayed.  X and Y are device pointers; flags:
+ * SPIRA_NO_SYNC.  Destroy batches before their device. */
+#define SPIRA_BATCH_XLDS_FLOATS 36864
+#define SPIRA_BATCH_MAX_PLOG2 5
+typedef struct spira_batch_sched {
+    uint8_t plog2;     /* target nonzeros per lane: 1 << plog2 */
+    uint8_t xlds;      /* 1: stage the X block in LDS */
+    uint16_t reserved; /* 0 */
+    uint32_t item_nnz; /* work-item size (nonzeros + rows) */
+} spira_batch_sched;
 typedef struct spira_batch spira_batch;
 int spira_batch_create(spira_dev *dev, uint32_t n_mats, spira_dcsr *const *mats, int storage, spira_batch **out);
 void spira_batch_destroy(spira_batch *b);
 int spira_batch_info(const spira_batch *b, uint64_t *n_rows, uint64_t *n_cols, uint64_t *nnz, uint32_t *n_tiles);
 int spira_batch_bases(const spira_batch *b, uint64_t *row_base, uint64_t *col_base); /* n_mats+1 each */
-int spira_batch_set_schedule(spira_batch *b, uint32_t k, const uint8_t *elog2, const uint32_t *tile_nnz);
+int spira_batch_set_schedule(spira_batch *b, uint32_t k, const spira_batch_sched *per_matrix);
 int spira_batch_spmm(spira_batch *b, const float *X, float *Y, uint32_t flags);
+/* One spira_batch_spmm launch that also timestamps every work item on the
+ * device clock; matrix_cost_us[m] (n_mats entries) receives the sum of the
+ * durations of matrix m's items, scaled to CU-microseconds by the blocks
+ * resident per CU.  This is the exhaustive autotuner's measurement: one
+ * profiled launch per schedule gives every matrix's cost inside its batch. */
+int spira_batch_profile(spira_batch *b, const float *X, float *Y, double *matrix_cost_us);
 int spira_batch_graph_capture(spira_batch *b, const float *X, float *Y);
 int spira_batch_graph_launch(spira_batch *b, uint32_t flags);
 

@@ -61,6 +61,9 @@ VARIANT_NAMES = {v: k for k, v in VARIANTS.items()}
 STORAGE = {"f32": 1, "bf16": 2, "split_bf16": 3}
 MIXED_VARIANTS = [VARIANTS[k] for k in ("vec4", "vec8", "vec16", "vec32", "stream1k", "stream2k")]
 N_VARIANTS = 13
+BATCH_XLDS_FLOATS = 36864
+BATCH_MAX_PLOG2 = 5
+BATCH_SCHED_DTYPE = np.dtype([("plog2", "u1"), ("xlds", "u1"), ("reserved", "<u2"), ("item_nnz", "<u4")])
 LOCK_MAX_ROW_INSERTS = 8192
 
 
@@ -164,7 +167,8 @@ def lib():
             "spira_batch_destroy": (None, [_vp]),
             "spira_batch_info": (ctypes.c_int, [_vp, _u64p, _u64p, _u64p, _u32p]),
             "spira_batch_bases": (ctypes.c_int, [_vp, _vp, _vp]),
-            "spira_batch_set_schedule": (ctypes.c_int, [_vp, ctypes.c_uint32, _vp, _vp]),
+            "spira_batch_set_schedule": (ctypes.c_int, [_vp, ctypes.c_uint32, _vp]),
+            "spira_batch_profile": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
             "spira_batch_spmm": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_uint32]),
             "spira_batch_graph_capture": (ctypes.c_int, [_vp, _vp, _vp]),
             "spira_batch_graph_launch": (ctypes.c_int, [_vp, ctypes.c_uint32]),
@@ -418,15 +422,32 @@ class Batch:
         lib().spira_batch_info(self._h, None, None, None, ctypes.byref(t))
         return t.value
 
-    def set_schedule(self, k: int, elog2=None, tile_nnz=None):
-        e = None if elog2 is None else np.ascontiguousarray(elog2, dtype=np.uint8)
-        t = None if tile_nnz is None else np.ascontiguousarray(tile_nnz, dtype=np.uint32)
-        self.dev.check(lib().spira_batch_set_schedule(self._h, int(k), _ptr(e), _ptr(t)))
+    def set_schedule(self, k: int, plog2=None, xlds=None, item_nnz=None):
+        """Per-matrix schedule (spira_batch_sched): plog2 (target nonzeros
+        per lane 2^plog2), xlds (stage the X block in LDS), item_nnz (work-item
+        size).  All three None: the library's built-in choices."""
+        given = [a is not None for a in (plog2, xlds, item_nnz)]
+        if any(given) and not all(given):
+            raise InvalidArgument("set_schedule: give plog2, xlds and item_nnz together (or none)")
+        s = None
+        if all(given):
+            n = len(self.mats)
+            s = np.zeros(n, dtype=BATCH_SCHED_DTYPE)
+            s["plog2"] = np.broadcast_to(np.asarray(plog2, np.int64), n)
+            s["xlds"] = np.broadcast_to(np.asarray(xlds, np.int64), n)
+            s["item_nnz"] = np.broadcast_to(np.asarray(item_nnz, np.int64), n)
+        self.dev.check(lib().spira_batch_set_schedule(self._h, int(k), _ptr(s)))
         self.k = int(k)
 
     def spmm(self, X, Y, sync: bool = True):
         self.dev.check(lib().spira_batch_spmm(self._h, _ptr(X), _ptr(Y), 0 if sync else NO_SYNC))
         return Y
+
+    def profile(self, X, Y) -> np.ndarray:
+        """One profiled spmm: per-matrix cost in CU-microseconds."""
+        out = np.zeros(max(1, len(self.mats)), np.float64)
+        self.dev.check(lib().spira_batch_profile(self._h, _ptr(X), _ptr(Y), _ptr(out)))
+        return out[:len(self.mats)]
 
     def capture(self, X, Y):
         self.dev.check(lib().spira_batch_graph_capture(self._h, _ptr(X), _ptr(Y)))

@@ -2,17 +2,39 @@
 // or 32), one persistent launch fed by a device work queue; the repeated call
 // can be captured in a hipGraph.
 //
-// A batch is stored as one block-diagonal CSR: rows and columns of matrix m
-// are shifted by row_base[m] / col_base[m], so X (sum of n_cols x k, row
-// major) and Y (sum of n_rows x k, row major) are single arrays and the kernel
-// needs no per-matrix metadata.  The rows are cut into tiles of bounded work
-// (nnz * k); every tile carries the lanes-per-row of its matrix's schedule.
-// Tiles are queued heaviest first; each block of the persistent grid takes
-// the next tile with one atomic add until the queue is empty (every block
-// leaves once the counter passes the tile count).
+// Layout.  A batch is one block-diagonal CSR whose rows are stored, within
+// each matrix, sorted by length class (class c holds rows of 2^(c-1) ..
+// 2^c - 1 nonzeros; class 0 the empty rows), each class in ascending row
+// order; rowid[p] maps a stored row back to its row of the block-diagonal
+// system, so X (sum of n_cols x k, row major) and Y (sum of n_rows x k) keep
+// the caller's order.  Columns are stored as u16 offsets inside their matrix
+// when every matrix has at most 65536 columns (u32 batch columns otherwise);
+// values as FP32, BF16 or split BF16.
 //
-// Values are FP32 / BF16 / split BF16, X and Y are FP32, and each output is
-// accumulated with the compensated Dot2 scheme (Ogita, Rump, Oishi 2005).
+// Work.  A schedule gives every matrix a lanes-per-row rule (2^plog2 target
+// nonzeros per lane: lanes per row = 2^class / 2^plog2, at most 64 / k), a
+// work-item size and whether its X block is staged in LDS.  A tile is a run
+// of stored rows of one class (one lanes-per-row value: rows per wavefront =
+// 64 / (lanes * k)); rows long enough to give a wavefront's lane slot more
+// than 32 nonzeros (from 2048 / k nonzeros) are tiles of their own, worked on
+// by the whole block.  A work item is a run of consecutive stored
+// rows of one matrix (its tiles) of about item_nnz nonzeros + rows and is the
+// unit of the device queue: items are queued heaviest first and every block
+// of the persistent grid (one 1024-thread block per CU) takes the next one
+// with one atomic add until the queue is empty.
+//
+// LDS.  A block first copies what its item reads into LDS with 16-byte
+// loads, all in flight at once: the matrix's X block when the schedule
+// stages it (n_cols * k floats, up to SPIRA_BATCH_XLDS_FLOATS), and the
+// item's row pointers, row ids, columns and values when they fit beside it.
+// The rows are then computed from LDS, so the only global reads left in the
+// compute phase are the X gathers of unstaged X blocks.  Items that do not
+// fit (long rows, or an X block that fills the LDS) read their entries from
+// global memory with the next rows' bounds loaded a step ahead.
+//
+// Each output is accumulated with the compensated Dot2 scheme (Ogita, Rump,
+// Oishi 2005) in FP32; the lanes of a row merge their (sum, correction) pairs
+// in a fixed order, so a call is deterministic.
 #include "internal.hpp"
 
 #include <algorithm>
@@ -23,26 +45,52 @@ namespace spira_amd {
 
 namespace {
 
-constexpr int kThreads = 256;
+constexpr int kThreads = 1024;  // one block per CU
 constexpr int kWaves = kThreads / 64;
+constexpr uint32_t kLdsBytes = 155584;  // LDS for the X block and staged entries
+static_assert(SPIRA_BATCH_XLDS_FLOATS * 4 <= kLdsBytes, "X block must fit the LDS area");
+constexpr uint32_t kMinEntryLds = 65536;  // stage entries only with at least this much LDS left
+constexpr uint32_t kClasses = 34;         // bit length of a u32 row length, plus one
+constexpr uint32_t kTileLong = 0x80u;
+constexpr uint32_t kItemStaged = 1u;  // item flag: entries staged in LDS
+constexpr size_t kArrayPad = 64;      // bytes after each entry array (16-byte staging reads)
 
 __device__ __forceinline__ float bf16f(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
 
 struct tile_t {
-    uint32_t r0, r1;   // rows [r0, r1) of the block-diagonal CSR
-    uint32_t elog2;    // lanes per row = 1 << elog2 (per right-hand side)
+    uint32_t r0, r1;  // stored rows [r0, r1)
+    uint32_t meta;    // bits 0-6: log2 lanes per row per right-hand side; kTileLong: one whole-block row
     uint32_t pad;
 };
 
-struct bvals {
-    int storage;
-    const float *f32;
-    const uint16_t *hi, *lo;
-    __device__ __forceinline__ float operator()(uint64_t i) const
+struct item_t {
+    uint32_t t0, t1;  // tiles [t0, t1)
+    uint32_t cbase;   // first batch column of the item's matrix
+    uint32_t xcols;   // columns whose X rows are staged in LDS (0: gathers read global memory)
+    uint32_t r0, r1;  // stored rows of the item
+    uint32_t ebase;   // first entry of row r0
+    uint32_t flags;   // kItemStaged
+};
+
+inline __host__ __device__ uint32_t align16(uint32_t v) { return (v + 15u) & ~15u; }
+
+// Byte layout of an item's staged entries in LDS (after the X block).  An
+// entry array is copied as the 16-byte aligned span around its elements,
+// at most align16(n * size) + 32 bytes.
+template <int ST, typename CT>
+struct lds_layout {
+    static constexpr uint32_t n_val_arrays = ST == SPIRA_VAL_SPLIT_BF16 ? 2 : 1;
+    static constexpr uint32_t val_size = ST == SPIRA_VAL_F32 ? 4 : 2;
+    static __host__ __device__ uint32_t span(uint32_t n, uint32_t size) { return align16(n * size) + 32; }
+    uint32_t off_o, rid_o, col_o, v1_o, v2_o, end_o;
+    __host__ __device__ lds_layout(uint32_t xbytes, uint32_t rows, uint32_t nnz)
     {
-        if (storage == SPIRA_VAL_F32) return __builtin_nontemporal_load(f32 + i);
-        if (storage == SPIRA_VAL_BF16) return bf16f(__builtin_nontemporal_load(hi + i));
-        return bf16f(__builtin_nontemporal_load(hi + i)) + bf16f(__builtin_nontemporal_load(lo + i));
+        off_o = align16(xbytes);
+        rid_o = off_o + 4 * (rows + 1);
+        col_o = align16(rid_o + 4 * rows);
+        v1_o = col_o + span(nnz, (uint32_t)sizeof(CT));
+        v2_o = v1_o + span(nnz, val_size);
+        end_o = n_val_arrays == 2 ? v2_o + span(nnz, val_size) : v2_o;
     }
 };
 
@@ -68,64 +116,385 @@ struct acc2 {
 
 __global__ void k_reset_counter(uint32_t *counter) { *counter = 0u; }
 
-template <int K>
-__global__ __launch_bounds__(kThreads) void k_batch_spmm(const tile_t *__restrict__ tiles, uint32_t n_tiles,
-                                                          uint32_t *__restrict__ counter,
-                                                          const uint32_t *__restrict__ off,
-                                                          const uint32_t *__restrict__ col, bvals vals,
-                                                          const float *__restrict__ X, float *__restrict__ Y)
+// Source of entries (column, value): global arrays or the item's LDS copy,
+// one storage format each (template parameters, so the unrolled loads stay
+// straight-line code).  get() is masked: a lane with ok false reads entry
+// min(q, last) -- inside the arrays -- and gets column 0, value 0.
+template <int ST, typename CT, bool LDS>
+struct esrc {
+    const CT *col;
+    const float *f32;
+    const uint16_t *hi, *lo;
+    uint32_t last;
+    __device__ __forceinline__ void get(bool ok, uint32_t q, uint32_t &c, float &v) const
+    {
+        const uint32_t qc = q < last ? q : last;
+        uint32_t cc;
+        float vv;
+        if constexpr (LDS) {
+            cc = (uint32_t)col[qc];
+            if constexpr (ST == SPIRA_VAL_F32) vv = f32[qc];
+            else if constexpr (ST == SPIRA_VAL_BF16) vv = bf16f(hi[qc]);
+            else vv = bf16f(hi[qc]) + bf16f(lo[qc]);
+        } else {
+            cc = (uint32_t)__builtin_nontemporal_load(col + qc);
+            if constexpr (ST == SPIRA_VAL_F32) vv = __builtin_nontemporal_load(f32 + qc);
+            else if constexpr (ST == SPIRA_VAL_BF16) vv = bf16f(__builtin_nontemporal_load(hi + qc));
+            else vv = bf16f(__builtin_nontemporal_load(hi + qc)) + bf16f(__builtin_nontemporal_load(lo + qc));
+        }
+        c = ok ? cc : 0u;
+        v = ok ? vv : 0.0f;
+    }
+};
+
+// Masked X element of stored column c for right-hand side j, from the LDS
+// block (XS) or global X.  A masked lane reads element j of the block
+// (present whenever the batch has a nonzero) and returns 0.
+template <int K, typename CT, bool XS>
+struct xsrc {
+    const float *X;  // global X, or the LDS block
+    uint32_t cbase;
+    __device__ __forceinline__ float operator()(bool ok, uint32_t c, int j) const
+    {
+        const uint32_t lc = (sizeof(CT) == 2) ? c : c - cbase;  // column inside the matrix
+        float v;
+        if constexpr (XS) {
+            v = X[ok ? lc * K + j : (uint32_t)j];
+        } else {
+            const uint64_t gi = ((uint64_t)cbase + lc) * K + j;
+            v = X[ok ? gi : (uint64_t)j];
+        }
+        return ok ? v : 0.0f;
+    }
+};
+
+// One row's share of the lanes (entries beg + e, beg + e + E, ...), U
+// independent loads in flight per step.
+template <int K, int U, typename ES, typename XV>
+__device__ __forceinline__ void row_dot(acc2 &a, uint32_t beg, uint32_t end, uint32_t e, uint32_t E, const ES &es,
+                                        const XV &xv, int j)
 {
-    __shared__ uint32_t s_t;
-    __shared__ float s_part[2][kWaves][64];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int j = lane % K;  // right-hand side of this lane
-    for (;;) {
-        if (threadIdx.x == 0) s_t = atomicAdd(counter, 1u);
-        __syncthreads();
-        const uint32_t t = s_t;
-        __syncthreads();  // every lane has read s_t before it is overwritten
-        if (t >= n_tiles) break;  // block-uniform: the whole block leaves
-        const tile_t tl = tiles[t];
-        if (tl.r1 - tl.r0 == 1 && off[tl.r0 + 1] - off[tl.r0] > 256u) {
-            // one long row: all four waves stride over it, then combine
-            const uint32_t r = tl.r0;
+    for (uint32_t i = beg + e; i < end; i += U * E) {
+        uint32_t c[U];
+        float av[U], xx[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) es.get(i + u * E < end, i + u * E, c[u], av[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) xx[u] = xv(i + u * E < end, c[u], j);
+#pragma unroll
+        for (int u = 0; u < U; ++u) a.add(av[u], xx[u]);
+    }
+}
+
+// Two rows of one lane group at once: per step U column/value loads of each
+// row in flight, then U gathers of each.
+template <int K, int U, typename ES, typename XV>
+__device__ __forceinline__ void rows2_dot(acc2 &a0, acc2 &a1, uint32_t b0, uint32_t e0, uint32_t b1, uint32_t e1,
+                                          uint32_t e, uint32_t E, const ES &es, const XV &xv, int j)
+{
+    uint32_t i0 = b0 + e, i1 = b1 + e;
+    while (i0 < e0 || i1 < e1) {
+        uint32_t c0[U], c1[U];
+        float v0[U], v1[U], x0[U], x1[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            es.get(i0 + u * E < e0, i0 + u * E, c0[u], v0[u]);
+            es.get(i1 + u * E < e1, i1 + u * E, c1[u], v1[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            x0[u] = xv(i0 + u * E < e0, c0[u], j);
+            x1[u] = xv(i1 + u * E < e1, c1[u], j);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            a0.add(v0[u], x0[u]);
+            a1.add(v1[u], x1[u]);
+        }
+        i0 += U * E;
+        i1 += U * E;
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void merge_lanes(acc2 &a, uint32_t E)
+{
+    for (uint32_t o = K; o < K * E; o <<= 1) a.merge(__shfl_xor(a.s, (int)o, 64), __shfl_xor(a.c, (int)o, 64));
+}
+
+// Copy nbytes (a multiple of 16) from 16-byte aligned global src to LDS dst.
+__device__ __forceinline__ void stage16(unsigned char *dst, const unsigned char *src, uint32_t nbytes)
+{
+    const u32x4 *s = (const u32x4 *)src;
+    u32x4 *d = (u32x4 *)dst;
+    for (uint32_t i = threadIdx.x; i < nbytes / 16; i += kThreads) d[i] = __builtin_nontemporal_load(s + i);
+}
+
+// Stage one entry array: the 16-byte aligned span holding elements
+// [base, base + n) of arr; returns the LDS pointer to element base.
+template <typename T>
+__device__ __forceinline__ const T *stage_array(unsigned char *lds, uint32_t o, const T *arr, uint32_t base,
+                                                uint32_t n)
+{
+    const uintptr_t a = (uintptr_t)(arr + base);
+    const uintptr_t a0 = a & ~(uintptr_t)15;
+    const uintptr_t a1 = ((uintptr_t)(arr + base + n) + 15) & ~(uintptr_t)15;
+    stage16(lds + o, (const unsigned char *)a0, (uint32_t)(a1 - a0));
+    return (const T *)(lds + o + (uint32_t)(a - a0));
+}
+
+struct kparams {
+    const item_t *items;
+    uint32_t n_items;
+    uint32_t *counter;
+    const tile_t *tiles;
+    const uint32_t *off;
+    const uint32_t *rowid;
+    const void *col;
+    const float *f32;
+    const uint16_t *hi, *lo;
+    uint32_t last;  // index of the batch's last stored entry (0 when it has none)
+    const float *X;
+    float *Y;
+    uint64_t *prof;
+};
+
+// All tiles of one item; XS: X block in LDS, ESTAGED: entries in LDS.
+template <int K, int ST, typename CT, bool XS, bool ESTAGED>
+__device__ __forceinline__ void run_item(const kparams &kp, const item_t &it, unsigned char *lds,
+                                         float (*s_part)[kWaves][64], int lane, int w, int j)
+{
+    constexpr int U = K >= 16 ? 8 : 4;  // loads in flight per row and lane group
+    const xsrc<K, CT, XS> xv{XS ? (const float *)lds : kp.X, it.cbase};
+    const uint32_t *s_off = nullptr, *s_rid = nullptr;
+    esrc<ST, CT, ESTAGED> es{};
+    if constexpr (ESTAGED) {
+        const uint32_t rows = it.r1 - it.r0;
+        const uint32_t nnz = kp.off[it.r1] - it.ebase;
+        const lds_layout<ST, CT> L(XS ? it.xcols * K * 4 : 0u, rows, nnz);
+        s_off = (const uint32_t *)(lds + L.off_o);
+        s_rid = (const uint32_t *)(lds + L.rid_o);
+        // the entry arrays start at their 16-byte aligned span (see stage_array)
+        const uint32_t cs = (uint32_t)(((uintptr_t)((const CT *)kp.col + it.ebase)) & 15u);
+        es.col = (const CT *)(lds + L.col_o + cs);
+        if constexpr (ST == SPIRA_VAL_F32) {
+            es.f32 = (const float *)(lds + L.v1_o + (uint32_t)(((uintptr_t)(kp.f32 + it.ebase)) & 15u));
+        } else {
+            es.hi = (const uint16_t *)(lds + L.v1_o + (uint32_t)(((uintptr_t)(kp.hi + it.ebase)) & 15u));
+            if constexpr (ST == SPIRA_VAL_SPLIT_BF16)
+                es.lo = (const uint16_t *)(lds + L.v2_o + (uint32_t)(((uintptr_t)(kp.lo + it.ebase)) & 15u));
+        }
+        es.last = nnz ? nnz - 1 : 0u;
+    } else {
+        es.col = (const CT *)kp.col;
+        es.f32 = kp.f32;
+        es.hi = kp.hi;
+        es.lo = kp.lo;
+        es.last = kp.last;
+    }
+    for (uint32_t tt = it.t0; tt < it.t1; ++tt) {
+        const tile_t tl = kp.tiles[tt];
+        if (tl.meta & kTileLong) {
+            // one long row: every wave strides over it, then the waves merge
+            const uint32_t p = tl.r0;
+            uint32_t beg, end, id;
+            if constexpr (ESTAGED) {
+                beg = s_off[p - it.r0];
+                end = s_off[p - it.r0 + 1];
+                id = s_rid[p - it.r0];
+            } else {
+                beg = kp.off[p];
+                end = kp.off[p + 1];
+                id = kp.rowid[p];
+            }
             constexpr int slots = 64 / K;
-            const int e = lane / K;
+            const uint32_t e = (uint32_t)(lane / K) + (uint32_t)w * slots;
             acc2 a;
-            for (uint32_t i = off[r] + w * slots + e; i < off[r + 1]; i += kWaves * slots)
-                a.add(vals(i), X[(uint64_t)col[i] * K + j]);
+            row_dot<K, U>(a, beg, end, e, (uint32_t)(kWaves * slots), es, xv, j);
             for (int o = K; o < 64; o <<= 1) a.merge(__shfl_xor(a.s, o, 64), __shfl_xor(a.c, o, 64));
-            if (e == 0) {
+            if (lane < K) {
                 s_part[0][w][j] = a.s;
                 s_part[1][w][j] = a.c;
             }
             __syncthreads();
-            if (w == 0 && e == 0) {
+            if (w == 0 && lane < K) {
                 acc2 b;
                 for (int ww = 0; ww < kWaves; ++ww) b.merge(s_part[0][ww][j], s_part[1][ww][j]);
-                Y[(uint64_t)r * K + j] = b.s + b.c;
+                kp.Y[(uint64_t)id * K + j] = b.s + b.c;
             }
             __syncthreads();
             continue;
         }
-        // lanes = (row slot, entry slot e, rhs j): E entry lanes per row,
-        // 64 / (E * K) rows per wavefront step
-        const int E = 1 << tl.elog2;
-        const int e = (lane / K) & (E - 1);
-        const int rs = lane / (K * E);
-        const int rpw = 64 / (K * E);
-        // the lanes of one row (same rs) share the row and so the trip count:
-        // the cross-lane merge only pairs lanes of the same row
-        for (uint32_t r = tl.r0 + w * rpw + rs; r < tl.r1; r += kWaves * rpw) {
-            acc2 a;
-            for (uint32_t i = off[r] + e; i < off[r + 1]; i += E) a.add(vals(i), X[(uint64_t)col[i] * K + j]);
-            for (int o = K; o < K * E; o <<= 1) a.merge(__shfl_xor(a.s, o, 64), __shfl_xor(a.c, o, 64));
-            if (e == 0) Y[(uint64_t)r * K + j] = a.s + a.c;
+        // lanes = (row slot rs, entry slot e, rhs j): E entry lanes per row,
+        // 64 / (E * K) rows per wavefront step; the group takes rows p and
+        // p + S per step (S = rows per block sweep).  The lanes of a row share
+        // its row and so its activity: the merge pairs only them.
+        const uint32_t E = 1u << (tl.meta & 0x7fu);
+        const uint32_t e = (uint32_t)(lane / K) & (E - 1);
+        const uint32_t rs = (uint32_t)lane / (K * E);
+        const uint32_t rpw = 64u / (K * E);
+        const uint32_t S = kWaves * rpw;
+        const uint32_t r1 = tl.r1;
+        uint32_t p = tl.r0 + (uint32_t)w * rpw + rs;
+        if constexpr (ESTAGED) {
+            for (; p < r1; p += 2 * S) {
+                const uint32_t l0 = p - it.r0, l1 = l0 + S;
+                const bool has1 = p + S < r1;
+                const uint32_t b0 = s_off[l0], e0 = s_off[l0 + 1];
+                const uint32_t b1 = has1 ? s_off[l1] : 0u, e1 = has1 ? s_off[l1 + 1] : 0u;
+                acc2 a0, a1;
+                rows2_dot<K, U>(a0, a1, b0, e0, b1, e1, e, E, es, xv, j);
+                merge_lanes<K>(a0, E);
+                merge_lanes<K>(a1, E);
+                if (e == 0) {
+                    kp.Y[(uint64_t)s_rid[l0] * K + j] = a0.s + a0.c;
+                    if (has1) kp.Y[(uint64_t)s_rid[l1] * K + j] = a1.s + a1.c;
+                }
+            }
+        } else {
+            // bounds and row ids of the next pair are loaded a step ahead
+            uint32_t b0 = 0, e0 = 0, b1 = 0, e1 = 0, id0 = 0, id1 = 0;
+            if (p < r1) {
+                b0 = kp.off[p];
+                e0 = kp.off[p + 1];
+                id0 = kp.rowid[p];
+            }
+            if (p + S < r1) {
+                b1 = kp.off[p + S];
+                e1 = kp.off[p + S + 1];
+                id1 = kp.rowid[p + S];
+            }
+            while (p < r1) {
+                const uint32_t pn = p + 2 * S;
+                uint32_t nb0 = 0, ne0 = 0, nb1 = 0, ne1 = 0, nid0 = 0, nid1 = 0;
+                if (pn < r1) {
+                    nb0 = kp.off[pn];
+                    ne0 = kp.off[pn + 1];
+                    nid0 = kp.rowid[pn];
+                }
+                if (pn + S < r1) {
+                    nb1 = kp.off[pn + S];
+                    ne1 = kp.off[pn + S + 1];
+                    nid1 = kp.rowid[pn + S];
+                }
+                acc2 a0, a1;
+                rows2_dot<K, U>(a0, a1, b0, e0, b1, e1, e, E, es, xv, j);
+                merge_lanes<K>(a0, E);
+                merge_lanes<K>(a1, E);
+                if (e == 0) {
+                    kp.Y[(uint64_t)id0 * K + j] = a0.s + a0.c;
+                    if (p + S < r1) kp.Y[(uint64_t)id1 * K + j] = a1.s + a1.c;
+                }
+                p = pn;
+                b0 = nb0;
+                e0 = ne0;
+                b1 = nb1;
+                e1 = ne1;
+                id0 = nid0;
+                id1 = nid1;
+            }
         }
     }
 }
 
-// Concatenation of one matrix into the batch (grid-stride).
+// prof (profiling launches only, else null): prof[2 i] / prof[2 i + 1] get
+// the device wall clock when the block starts item i and when every wave of
+// the block has finished it.
+template <int K, int ST, typename CT>
+__global__ __launch_bounds__(kThreads) void k_batch_spmm(kparams kp)
+{
+    __shared__ __align__(16) unsigned char lds[kLdsBytes];
+    __shared__ float s_part[2][kWaves][64];
+    __shared__ uint32_t s_t;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int j = lane % K;  // right-hand side of this lane
+    uint32_t prev = 0xffffffffu;
+    for (;;) {
+        // The barrier after the atomic also orders the previous item's LDS
+        // reads before this item's staging writes.
+        if (threadIdx.x == 0) s_t = atomicAdd(kp.counter, 1u);
+        __syncthreads();
+        const uint32_t i = s_t;
+        __syncthreads();
+        if (kp.prof && threadIdx.x == 0) {
+            const uint64_t now = wall_clock64();
+            if (prev != 0xffffffffu) kp.prof[2 * (uint64_t)prev + 1] = now;
+            if (i < kp.n_items) kp.prof[2 * (uint64_t)i] = now;
+        }
+        prev = i;
+        if (i >= kp.n_items) break;  // block-uniform: the whole block leaves
+        const item_t it = kp.items[i];
+        const bool xs = it.xcols != 0, staged = (it.flags & kItemStaged) != 0;
+        const uint32_t xbytes = xs ? it.xcols * K * 4 : 0u;
+        if (xs) {
+            const float *src = kp.X + (uint64_t)it.cbase * K;
+            float *dst = (float *)lds;
+            for (uint32_t q = threadIdx.x; q < it.xcols * K; q += kThreads) dst[q] = src[q];
+        }
+        if (staged) {
+            const uint32_t rows = it.r1 - it.r0;
+            const uint32_t nnz = kp.off[it.r1] - it.ebase;
+            const lds_layout<ST, CT> L(xbytes, rows, nnz);
+            uint32_t *s_off = (uint32_t *)(lds + L.off_o);
+            uint32_t *s_rid = (uint32_t *)(lds + L.rid_o);
+            for (uint32_t q = threadIdx.x; q <= rows; q += kThreads) s_off[q] = kp.off[it.r0 + q] - it.ebase;
+            for (uint32_t q = threadIdx.x; q < rows; q += kThreads) s_rid[q] = kp.rowid[it.r0 + q];
+            stage_array(lds, L.col_o, (const CT *)kp.col, it.ebase, nnz);
+            if constexpr (ST == SPIRA_VAL_F32) {
+                stage_array(lds, L.v1_o, kp.f32, it.ebase, nnz);
+            } else {
+                stage_array(lds, L.v1_o, kp.hi, it.ebase, nnz);
+                if constexpr (ST == SPIRA_VAL_SPLIT_BF16) stage_array(lds, L.v2_o, kp.lo, it.ebase, nnz);
+            }
+        }
+        if (xs || staged) __syncthreads();
+        if (xs) {
+            if (staged) run_item<K, ST, CT, true, true>(kp, it, lds, s_part, lane, w, j);
+            else run_item<K, ST, CT, true, false>(kp, it, lds, s_part, lane, w, j);
+        } else {
+            if (staged) run_item<K, ST, CT, false, true>(kp, it, lds, s_part, lane, w, j);
+            else run_item<K, ST, CT, false, false>(kp, it, lds, s_part, lane, w, j);
+        }
+    }
+}
+
+// Copy matrix m's rows into the batch in stored order: stored row q (global)
+// takes source row perm[q] - row_base; 16 lanes per row.
+template <typename CT>
+__global__ void k_gather_rows(const uint64_t *__restrict__ src_off, const uint32_t *__restrict__ src_col,
+                              const double *__restrict__ src_val, uint32_t row_base, uint32_t n,
+                              uint32_t col_base, const uint32_t *__restrict__ perm,
+                              const uint32_t *__restrict__ dst_off, int storage, CT *__restrict__ ocol,
+                              float *__restrict__ of32, uint16_t *__restrict__ ohi, uint16_t *__restrict__ olo)
+{
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t q = (uint32_t)(g / 16);
+    const uint32_t l = (uint32_t)(g % 16);
+    if (q >= n) return;
+    const uint32_t r = perm[row_base + q] - row_base;
+    const uint64_t s0 = src_off[r], s1 = src_off[r + 1];
+    const uint32_t d0 = dst_off[row_base + q];
+    for (uint64_t i = s0 + l; i < s1; i += 16) {
+        const uint32_t d = d0 + (uint32_t)(i - s0);
+        ocol[d] = (CT)(sizeof(CT) == 2 ? src_col[i] : src_col[i] + col_base);
+        const float f = (float)src_val[i];
+        if (storage == SPIRA_VAL_F32) {
+            of32[d] = f;
+        } else {
+            const uint32_t u = __float_as_uint(f);
+            const uint32_t ru = u + 0x7fffu + ((u >> 16) & 1u);
+            const uint16_t h = (u & 0x7fffffffu) > 0x7f800000u ? (uint16_t)((u >> 16) | 0x40u) : (uint16_t)(ru >> 16);
+            ohi[d] = h;
+            if (storage == SPIRA_VAL_SPLIT_BF16) {
+                const float rest = (float)(src_val[i] - (double)bf16f(h));
+                const uint32_t v = __float_as_uint(rest);
+                const uint32_t rv = v + 0x7fffu + ((v >> 16) & 1u);
+                olo[d] = (v & 0x7fffffffu) > 0x7f800000u ? (uint16_t)((v >> 16) | 0x40u) : (uint16_t)(rv >> 16);
+            }
+        }
+    }
+}
+
 __global__ void k_cat_off(const uint64_t *__restrict__ off, uint64_t n, uint64_t nnz_base, uint64_t row_base,
                           uint64_t *__restrict__ out)
 {
@@ -133,35 +502,14 @@ __global__ void k_cat_off(const uint64_t *__restrict__ off, uint64_t n, uint64_t
         out[row_base + i] = off[i] + nnz_base;
 }
 
-__global__ void k_cat_entries(const uint32_t *__restrict__ col, const double *__restrict__ val, uint64_t nnz,
-                              uint64_t nnz_base, uint32_t col_base, int storage, uint32_t *__restrict__ ocol,
-                              float *__restrict__ of32, uint16_t *__restrict__ ohi, uint16_t *__restrict__ olo)
+inline uint32_t len_class(uint64_t len)
 {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        ocol[nnz_base + i] = col[i] + col_base;
-        const float f = (float)val[i];
-        if (storage == SPIRA_VAL_F32) {
-            of32[nnz_base + i] = f;
-        } else {
-            uint32_t u = __float_as_uint(f);
-            uint32_t r = u + 0x7fffu + ((u >> 16) & 1u);
-            const uint16_t h = (u & 0x7fffffffu) > 0x7f800000u ? (uint16_t)((u >> 16) | 0x40u) : (uint16_t)(r >> 16);
-            ohi[nnz_base + i] = h;
-            if (storage == SPIRA_VAL_SPLIT_BF16) {
-                const float rest = (float)(val[i] - (double)bf16f(h));
-                uint32_t v = __float_as_uint(rest);
-                uint32_t rv = v + 0x7fffu + ((v >> 16) & 1u);
-                olo[nnz_base + i] = (v & 0x7fffffffu) > 0x7f800000u ? (uint16_t)((v >> 16) | 0x40u) : (uint16_t)(rv >> 16);
-            }
-        }
+    uint32_t c = 0;
+    while (len) {
+        ++c;
+        len >>= 1;
     }
-}
-
-__global__ void k_u64_to_u32_b(const uint64_t *__restrict__ a, uint32_t *__restrict__ b, uint64_t n)
-{
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) b[i] = (uint32_t)a[i];
+    return c;
 }
 
 }  // namespace
@@ -174,20 +522,24 @@ struct spira_batch {
     spira_dev *dev = nullptr;
     uint32_t n_mats = 0;
     int storage = SPIRA_VAL_F32;
+    bool col16 = true;
     uint64_t n_rows = 0, n_cols = 0, nnz = 0;
     std::vector<uint64_t> row_base, col_base, nnz_base;  // n_mats+1 each
-    std::vector<uint64_t> h_off;                          // host copy of the offsets (tile building)
+    std::vector<uint32_t> h_off;                          // stored-order offsets (host copy, tile building)
+    std::vector<uint32_t> class_start;                    // n_mats * (kClasses+1): first stored row of each class
     uint32_t *off32 = nullptr;
-    uint32_t *col = nullptr;
+    uint32_t *rowid = nullptr;
+    void *col = nullptr;
     float *f32 = nullptr;
     uint16_t *hi = nullptr, *lo = nullptr;
     // schedule
     uint32_t k = 0;
-    std::vector<uint8_t> elog2;      // per matrix
-    std::vector<uint32_t> tile_nnz;  // per matrix
+    std::vector<spira_batch_sched> sched;
     tile_t *tiles = nullptr;
-    size_t tiles_bytes = 0;
-    uint32_t n_tiles = 0;
+    item_t *items = nullptr;
+    size_t tiles_bytes = 0, items_bytes = 0;
+    uint32_t n_tiles = 0, n_items = 0;
+    std::vector<uint32_t> item_mat;  // matrix of each queued item
     uint32_t *counter = nullptr;
     // captured graph
     hipGraph_t graph = nullptr;
@@ -203,58 +555,173 @@ struct spira_batch {
 
 namespace spira_amd {
 
-static void batch_build_tiles(spira_batch *b)
+static uint32_t max_elog2(uint32_t K)
 {
-    const uint32_t K = b->k;
-    std::vector<tile_t> tl;
-    std::vector<uint64_t> work;
-    for (uint32_t m = 0; m < b->n_mats; ++m) {
-        const uint64_t r0 = b->row_base[m], r1 = b->row_base[m + 1];
-        const uint64_t budget = std::max<uint64_t>(b->tile_nnz[m], 1);
-        const uint32_t el = b->elog2[m];
-        const uint64_t max_rows = 1024;
-        uint64_t r = r0;
-        while (r < r1) {
-            uint64_t e = r + 1, acc = b->h_off[r + 1] - b->h_off[r];
-            while (e < r1 && e - r < max_rows && acc + (b->h_off[e + 1] - b->h_off[e]) <= budget) {
-                acc += b->h_off[e + 1] - b->h_off[e];
-                ++e;
-            }
-            tl.push_back(tile_t{(uint32_t)r, (uint32_t)e, el, 0});
-            work.push_back(acc * K + (e - r) * K);
-            r = e;
-        }
-    }
-    // heaviest tiles first (longest-processing-time order for the queue)
-    std::vector<uint32_t> order(tl.size());
-    std::iota(order.begin(), order.end(), 0u);
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) { return work[a] > work[c]; });
-    std::vector<tile_t> sorted(tl.size());
-    for (size_t i = 0; i < order.size(); ++i) sorted[i] = tl[order[i]];
-    if (b->tiles) {
-        SPIRA_HIP(hipStreamSynchronize(b->dev->stream));
-        dev_release(b->dev, b->tiles, b->tiles_bytes);
-        b->tiles = nullptr;
-    }
-    b->tiles_bytes = std::max<size_t>(sorted.size(), 1) * sizeof(tile_t);
-    b->tiles = (tile_t *)dev_alloc(b->dev, b->tiles_bytes);
-    SPIRA_HIP(hipMemcpyAsync(b->tiles, sorted.data(), sorted.size() * sizeof(tile_t), hipMemcpyHostToDevice,
-                             b->dev->stream));
-    dev_sync(b->dev);
-    b->n_tiles = (uint32_t)sorted.size();
+    uint32_t e = 0;
+    while ((1u << (e + 1)) * K <= 64u) ++e;
+    return e;
 }
 
-static void batch_launch(spira_batch *b, const float *X, float *Y)
+template <int ST, typename CT>
+static uint32_t staged_bytes(uint32_t xbytes, uint32_t rows, uint32_t nnz)
 {
+    return lds_layout<ST, CT>(xbytes, rows, nnz).end_o;
+}
+
+static uint32_t staged_bytes_rt(int storage, bool col16, uint32_t xbytes, uint32_t rows, uint32_t nnz)
+{
+    if (col16) {
+        if (storage == SPIRA_VAL_F32) return staged_bytes<SPIRA_VAL_F32, uint16_t>(xbytes, rows, nnz);
+        if (storage == SPIRA_VAL_BF16) return staged_bytes<SPIRA_VAL_BF16, uint16_t>(xbytes, rows, nnz);
+        return staged_bytes<SPIRA_VAL_SPLIT_BF16, uint16_t>(xbytes, rows, nnz);
+    }
+    if (storage == SPIRA_VAL_F32) return staged_bytes<SPIRA_VAL_F32, uint32_t>(xbytes, rows, nnz);
+    if (storage == SPIRA_VAL_BF16) return staged_bytes<SPIRA_VAL_BF16, uint32_t>(xbytes, rows, nnz);
+    return staged_bytes<SPIRA_VAL_SPLIT_BF16, uint32_t>(xbytes, rows, nnz);
+}
+
+static void batch_build_work(spira_batch *b)
+{
+    const uint32_t K = b->k;
+    const uint32_t emax = max_elog2(K);
+    // first length class (rows of 2^(c-1) nonzeros and more) worked on by the
+    // whole block: 2^(c-1) >= 32 * (64 / k) nonzeros per wavefront lane slot
+    const uint32_t long_class = emax + 6;
+    std::vector<tile_t> tl;
+    struct pending {
+        item_t it;
+        uint64_t work;
+        uint32_t mat;
+    };
+    std::vector<pending> its;
+    const std::vector<uint32_t> &off = b->h_off;
+    for (uint32_t m = 0; m < b->n_mats; ++m) {
+        const spira_batch_sched &s = b->sched[m];
+        const uint64_t budget = std::max<uint32_t>(s.item_nnz, 1u);
+        const uint32_t xcols = s.xlds ? (uint32_t)(b->col_base[m + 1] - b->col_base[m]) : 0u;
+        const uint32_t xbytes = xcols * K * 4;
+        const bool stage = align16(xbytes) + kMinEntryLds <= kLdsBytes;
+        const uint32_t *cs = &b->class_start[(size_t)m * (kClasses + 1)];
+        // cost of a run of rows = nonzeros + rows (an empty row still writes k outputs)
+        item_t cur{(uint32_t)tl.size(), 0, (uint32_t)b->col_base[m], xcols, 0, 0, 0, 0};
+        uint32_t ir0 = 0;  // first stored row of the open item
+        bool open = false;
+        uint64_t cur_cost = 0;
+        // the open item's rows [ir0, r_end) fit in LDS beside the X block
+        auto fits = [&](uint32_t r_end) {
+            const uint32_t rows = r_end - ir0, nnz = off[r_end] - off[ir0];
+            if (rows > kLdsBytes / 8 || nnz > kLdsBytes / 2) return false;
+            return staged_bytes_rt(b->storage, b->col16, xbytes, rows, nnz) <= kLdsBytes;
+        };
+        auto close_item = [&](uint32_t r_end, bool may_stage) {
+            cur.t1 = (uint32_t)tl.size();
+            if (cur.t1 > cur.t0) {
+                cur.r0 = ir0;
+                cur.r1 = r_end;
+                cur.ebase = off[ir0];
+                cur.flags = (may_stage && fits(r_end)) ? kItemStaged : 0u;
+                its.push_back({cur, (cur_cost + xcols) * K, m});
+            }
+            cur.t0 = (uint32_t)tl.size();
+            cur_cost = 0;
+            open = false;
+        };
+        for (uint32_t c = 0; c < kClasses; ++c) {
+            const uint32_t a = cs[c], z = cs[c + 1];
+            if (a == z) continue;
+            // rows that would give a lane slot of one wavefront more than 32
+            // nonzeros are whole-block tiles of one row each
+            const bool lng = c >= long_class;
+            const uint32_t meta = lng ? kTileLong : std::min<uint32_t>(c > s.plog2 ? c - s.plog2 : 0u, emax);
+            uint32_t p = a;
+            while (p < z) {
+                if (!open) {
+                    ir0 = p;
+                    open = true;
+                }
+                // extend the tile while the item stays within its budget and,
+                // when staged, within the LDS
+                uint32_t q = p;
+                while (q < z && (!lng || q == p) && cur_cost + (off[q + 1] - off[p]) + (q + 1 - p) <= budget &&
+                       (!stage || fits(q + 1)))
+                    ++q;
+                if (q == p) {
+                    if (p > ir0) {  // the item is full: close it and retry the row in a new one
+                        close_item(p, stage);
+                        continue;
+                    }
+                    q = p + 1;  // a lone row over budget
+                }
+                tl.push_back(tile_t{p, q, meta, 0});
+                cur_cost += (off[q] - off[p]) + (q - p);
+                p = q;
+                if (cur_cost >= budget || (stage && p < z && !fits(p + 1))) close_item(p, stage);
+            }
+        }
+        if (open) close_item(cs[kClasses], stage);
+    }
+    // heaviest items first (longest-processing-time order for the queue)
+    std::stable_sort(its.begin(), its.end(), [](const pending &x, const pending &y) { return x.work > y.work; });
+    std::vector<item_t> items(its.size());
+    b->item_mat.resize(its.size());
+    for (size_t i = 0; i < its.size(); ++i) {
+        items[i] = its[i].it;
+        b->item_mat[i] = its[i].mat;
+    }
     hipStream_t st = b->dev->stream;
-    bvals v{b->storage, b->f32, b->hi, b->lo};
-    const unsigned grid = (unsigned)std::max<uint32_t>(1u, std::min<uint32_t>(b->n_tiles, (uint32_t)b->dev->n_cu * 8u));
-    k_reset_counter<<<1, 1, 0, st>>>(b->counter);
+    if (b->tiles || b->items) SPIRA_HIP(hipStreamSynchronize(st));
+    if (b->tiles) dev_release(b->dev, b->tiles, b->tiles_bytes);
+    if (b->items) dev_release(b->dev, b->items, b->items_bytes);
+    b->tiles = nullptr;
+    b->items = nullptr;
+    b->tiles_bytes = std::max<size_t>(tl.size(), 1) * sizeof(tile_t);
+    b->items_bytes = std::max<size_t>(items.size(), 1) * sizeof(item_t);
+    b->tiles = (tile_t *)dev_alloc(b->dev, b->tiles_bytes);
+    b->items = (item_t *)dev_alloc(b->dev, b->items_bytes);
+    SPIRA_HIP(hipMemcpyAsync(b->tiles, tl.data(), tl.size() * sizeof(tile_t), hipMemcpyHostToDevice, st));
+    SPIRA_HIP(hipMemcpyAsync(b->items, items.data(), items.size() * sizeof(item_t), hipMemcpyHostToDevice, st));
+    dev_sync(b->dev);
+    b->n_tiles = (uint32_t)tl.size();
+    b->n_items = (uint32_t)items.size();
+}
+
+// One 1024-thread block per CU (the LDS area takes most of a CU's LDS); the
+// grid is exactly the resident blocks, so every block is live from the start
+// and a block-microsecond is a CU-microsecond.
+template <int K, int ST, typename CT>
+static void launch_k(spira_batch *b, const float *X, float *Y, uint64_t *prof)
+{
+    kparams kp{b->items, b->n_items, b->counter, b->tiles, b->off32, b->rowid, b->col, b->f32, b->hi, b->lo,
+               b->nnz ? (uint32_t)(b->nnz - 1) : 0u, X, Y, prof};
+    const unsigned grid = std::max(1u, std::min(b->n_items, (uint32_t)b->dev->n_cu));
+    k_batch_spmm<K, ST, CT><<<grid, kThreads, 0, b->dev->stream>>>(kp);
+}
+
+template <int K, int ST>
+static void launch_col(spira_batch *b, const float *X, float *Y, uint64_t *prof)
+{
+    if (b->col16) launch_k<K, ST, uint16_t>(b, X, Y, prof);
+    else launch_k<K, ST, uint32_t>(b, X, Y, prof);
+}
+
+template <int K>
+static void launch_st(spira_batch *b, const float *X, float *Y, uint64_t *prof)
+{
+    switch (b->storage) {
+    case SPIRA_VAL_F32: launch_col<K, SPIRA_VAL_F32>(b, X, Y, prof); break;
+    case SPIRA_VAL_BF16: launch_col<K, SPIRA_VAL_BF16>(b, X, Y, prof); break;
+    default: launch_col<K, SPIRA_VAL_SPLIT_BF16>(b, X, Y, prof); break;
+    }
+}
+
+static void batch_launch(spira_batch *b, const float *X, float *Y, uint64_t *prof = nullptr)
+{
+    k_reset_counter<<<1, 1, 0, b->dev->stream>>>(b->counter);
     switch (b->k) {
-    case 1: k_batch_spmm<1><<<grid, kThreads, 0, st>>>(b->tiles, b->n_tiles, b->counter, b->off32, b->col, v, X, Y); break;
-    case 4: k_batch_spmm<4><<<grid, kThreads, 0, st>>>(b->tiles, b->n_tiles, b->counter, b->off32, b->col, v, X, Y); break;
-    case 16: k_batch_spmm<16><<<grid, kThreads, 0, st>>>(b->tiles, b->n_tiles, b->counter, b->off32, b->col, v, X, Y); break;
-    default: k_batch_spmm<32><<<grid, kThreads, 0, st>>>(b->tiles, b->n_tiles, b->counter, b->off32, b->col, v, X, Y); break;
+    case 1: launch_st<1>(b, X, Y, prof); break;
+    case 4: launch_st<4>(b, X, Y, prof); break;
+    case 16: launch_st<16>(b, X, Y, prof); break;
+    default: launch_st<32>(b, X, Y, prof); break;
     }
     SPIRA_HIP(hipGetLastError());
 }
@@ -267,13 +734,14 @@ static void batch_drop_graph(spira_batch *b)
     b->graph = nullptr;
 }
 
-static uint32_t default_elog2(double mean_len, uint32_t K)
+// Built-in schedule of one matrix (the method without a selector).
+static spira_batch_sched default_sched(uint64_t n_cols, uint32_t K)
 {
-    uint32_t maxe = 0;
-    while ((1u << (maxe + 1)) * K <= 64u) ++maxe;
-    uint32_t el = 0;
-    while (el < maxe && (double)(1u << el) * 2.0 < mean_len) ++el;
-    return el;
+    spira_batch_sched s{};
+    s.plog2 = 2;
+    s.xlds = (K == 1 && n_cols * K <= SPIRA_BATCH_XLDS_FLOATS) ? 1 : 0;
+    s.item_nnz = std::max<uint32_t>(4096u, 131072u / K);
+    return s;
 }
 
 }  // namespace spira_amd
@@ -325,35 +793,71 @@ int spira_batch_create(spira_dev *dev, uint32_t n_mats, spira_dcsr *const *mats,
             b->row_base[m + 1] = b->row_base[m] + A->n_rows;
             b->col_base[m + 1] = b->col_base[m] + A->n_cols;
             b->nnz_base[m + 1] = b->nnz_base[m] + A->nnz;
+            if (A->n_cols > 65536) b->col16 = false;
         }
         b->n_rows = b->row_base[n_mats];
         b->n_cols = b->col_base[n_mats];
         b->nnz = b->nnz_base[n_mats];
         if (b->nnz >= (1ull << 32) || b->n_rows >= (1ull << 32) || b->n_cols >= (1ull << 32))
             fail(SPIRA_ERR_UNSUPPORTED, "batch: totals must stay below 2^32 rows, columns and nonzeros");
-        uint64_t *off = (uint64_t *)dev->ws.get(WS_MISC2, (b->n_rows + 1) * sizeof(uint64_t));
-        b->col = (uint32_t *)b->alloc((b->nnz ? b->nnz : 1) * sizeof(uint32_t));
-        if (storage == SPIRA_VAL_F32) b->f32 = (float *)b->alloc((b->nnz ? b->nnz : 1) * sizeof(float));
-        else b->hi = (uint16_t *)b->alloc((b->nnz ? b->nnz : 1) * sizeof(uint16_t));
-        if (storage == SPIRA_VAL_SPLIT_BF16) b->lo = (uint16_t *)b->alloc((b->nnz ? b->nnz : 1) * sizeof(uint16_t));
+        hipStream_t st = dev->stream;
+        // source offsets, concatenated, to the host
+        const uint64_t nr = b->n_rows;
+        uint64_t *cat = (uint64_t *)dev->ws.get(WS_MISC2, (nr + 1) * sizeof(uint64_t));
         for (uint32_t m = 0; m < n_mats; ++m) {
             const spira_dcsr *A = mats[m];
-            k_cat_off<<<grid_for(A->n_rows + 1, 256, 1024), 256, 0, dev->stream>>>(A->off, A->n_rows, b->nnz_base[m],
-                                                                                 b->row_base[m], off);
-            if (A->nnz)
-                k_cat_entries<<<grid_for(A->nnz, 256, 2048), 256, 0, dev->stream>>>(
-                    A->col, A->val, A->nnz, b->nnz_base[m], (uint32_t)b->col_base[m], storage, b->col, b->f32, b->hi,
-                    b->lo);
+            k_cat_off<<<grid_for(A->n_rows + 1, 256, 1024), 256, 0, st>>>(A->off, A->n_rows, b->nnz_base[m],
+                                                                         b->row_base[m], cat);
         }
         SPIRA_HIP(hipGetLastError());
-        if (n_mats == 0) SPIRA_HIP(hipMemsetAsync(off, 0, sizeof(uint64_t), dev->stream));
-        b->off32 = (uint32_t *)b->alloc((b->n_rows + 1) * sizeof(uint32_t));
-        k_u64_to_u32_b<<<grid_for(b->n_rows + 1, 256), 256, 0, dev->stream>>>(off, b->off32, b->n_rows + 1);
-        b->h_off.resize(b->n_rows + 1);
-        SPIRA_HIP(hipMemcpyAsync(b->h_off.data(), off, (b->n_rows + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost,
-                                 dev->stream));
-        b->counter = (uint32_t *)b->alloc(64);
+        std::vector<uint64_t> src(nr + 1, 0);
+        if (n_mats) SPIRA_HIP(hipMemcpyAsync(src.data(), cat, (nr + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
         dev_sync(dev);
+        // stored order: within each matrix, rows by length class (stable)
+        std::vector<uint32_t> perm(nr);
+        b->h_off.assign(nr + 1, 0);
+        b->class_start.assign((size_t)n_mats * (kClasses + 1), 0);
+        std::vector<uint32_t> cnt(kClasses + 1);
+        for (uint32_t m = 0; m < n_mats; ++m) {
+            const uint64_t r0 = b->row_base[m], r1 = b->row_base[m + 1];
+            std::fill(cnt.begin(), cnt.end(), 0u);
+            for (uint64_t r = r0; r < r1; ++r) ++cnt[len_class(src[r + 1] - src[r]) + 1];
+            for (uint32_t c = 0; c < kClasses; ++c) cnt[c + 1] += cnt[c];
+            uint32_t *cs = &b->class_start[(size_t)m * (kClasses + 1)];
+            for (uint32_t c = 0; c <= kClasses; ++c) cs[c] = (uint32_t)(r0 + cnt[c]);
+            for (uint64_t r = r0; r < r1; ++r) perm[r0 + cnt[len_class(src[r + 1] - src[r])]++] = (uint32_t)r;
+        }
+        for (uint64_t q = 0; q < nr; ++q) b->h_off[q + 1] = b->h_off[q] + (uint32_t)(src[perm[q] + 1] - src[perm[q]]);
+        src.clear();
+        src.shrink_to_fit();
+        // device arrays (entry arrays padded: items copy them in 16-byte spans)
+        const uint64_t nz = b->nnz ? b->nnz : 1;
+        b->off32 = (uint32_t *)b->alloc((nr + 1) * sizeof(uint32_t));
+        b->rowid = (uint32_t *)b->alloc((nr ? nr : 1) * sizeof(uint32_t));
+        b->col = b->alloc(nz * (b->col16 ? sizeof(uint16_t) : sizeof(uint32_t)) + kArrayPad);
+        if (storage == SPIRA_VAL_F32) b->f32 = (float *)b->alloc(nz * sizeof(float) + kArrayPad);
+        else b->hi = (uint16_t *)b->alloc(nz * sizeof(uint16_t) + kArrayPad);
+        if (storage == SPIRA_VAL_SPLIT_BF16) b->lo = (uint16_t *)b->alloc(nz * sizeof(uint16_t) + kArrayPad);
+        SPIRA_HIP(hipMemcpyAsync(b->off32, b->h_off.data(), (nr + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        if (nr) SPIRA_HIP(hipMemcpyAsync(b->rowid, perm.data(), nr * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        for (uint32_t m = 0; m < n_mats; ++m) {
+            const spira_dcsr *A = mats[m];
+            if (!A->n_rows || !A->nnz) continue;
+            const unsigned grid = grid_for((uint64_t)A->n_rows * 16, 256);
+            if (b->col16)
+                k_gather_rows<uint16_t><<<grid, 256, 0, st>>>(A->off, A->col, A->val, (uint32_t)b->row_base[m],
+                                                             (uint32_t)A->n_rows, (uint32_t)b->col_base[m],
+                                                             b->rowid, b->off32, storage, (uint16_t *)b->col,
+                                                             b->f32, b->hi, b->lo);
+            else
+                k_gather_rows<uint32_t><<<grid, 256, 0, st>>>(A->off, A->col, A->val, (uint32_t)b->row_base[m],
+                                                             (uint32_t)A->n_rows, (uint32_t)b->col_base[m],
+                                                             b->rowid, b->off32, storage, (uint32_t *)b->col,
+                                                             b->f32, b->hi, b->lo);
+        }
+        SPIRA_HIP(hipGetLastError());
+        b->counter = (uint32_t *)b->alloc(64);
+        dev_sync(dev);  // perm (host) is read by the copies above until here
     });
     if (rc != SPIRA_OK) {
         for (auto &o : b->owned) dev_release(dev, o.first, o.second);
@@ -371,6 +875,7 @@ void spira_batch_destroy(spira_batch *b)
     (void)hipStreamSynchronize(b->dev->stream);
     batch_drop_graph(b);
     if (b->tiles) dev_release(b->dev, b->tiles, b->tiles_bytes);
+    if (b->items) dev_release(b->dev, b->items, b->items_bytes);
     for (auto &o : b->owned) dev_release(b->dev, o.first, o.second);
     delete b;
 }
@@ -393,28 +898,27 @@ int spira_batch_bases(const spira_batch *b, uint64_t *row_base, uint64_t *col_ba
     return SPIRA_OK;
 }
 
-int spira_batch_set_schedule(spira_batch *b, uint32_t k, const uint8_t *elog2, const uint32_t *tile_nnz)
+int spira_batch_set_schedule(spira_batch *b, uint32_t k, const spira_batch_sched *per_matrix)
 {
     if (!b) return SPIRA_ERR_INVALID_ARGUMENT;
     return bguard(b->dev, [&] {
         if (k != 1 && k != 4 && k != 16 && k != 32)
             fail(SPIRA_ERR_INVALID_ARGUMENT, "batch: k must be 1, 4, 16 or 32");
-        uint32_t maxe = 0;
-        while ((1u << (maxe + 1)) * k <= 64u) ++maxe;
-        b->k = k;
-        b->elog2.resize(b->n_mats);
-        b->tile_nnz.resize(b->n_mats);
+        std::vector<spira_batch_sched> s(b->n_mats);
         for (uint32_t m = 0; m < b->n_mats; ++m) {
-            const uint64_t rows = b->row_base[m + 1] - b->row_base[m];
-            const uint64_t nz = b->nnz_base[m + 1] - b->nnz_base[m];
-            const double mean = rows ? (double)nz / (double)rows : 0.0;
-            uint32_t el = elog2 ? elog2[m] : default_elog2(mean, k);
-            if (el > maxe) fail(SPIRA_ERR_INVALID_ARGUMENT, "batch: lanes per row exceed 64 / k");
-            b->elog2[m] = (uint8_t)el;
-            b->tile_nnz[m] = tile_nnz ? tile_nnz[m] : std::max<uint32_t>(256u, 16384u / k);
+            const uint64_t nc = b->col_base[m + 1] - b->col_base[m];
+            s[m] = per_matrix ? per_matrix[m] : default_sched(nc, k);
+            if (s[m].plog2 > SPIRA_BATCH_MAX_PLOG2)
+                fail(SPIRA_ERR_INVALID_ARGUMENT, "batch: plog2 must be at most SPIRA_BATCH_MAX_PLOG2");
+            if (s[m].xlds > 1) fail(SPIRA_ERR_INVALID_ARGUMENT, "batch: xlds must be 0 or 1");
+            if (s[m].xlds && nc * k > SPIRA_BATCH_XLDS_FLOATS)
+                fail(SPIRA_ERR_INVALID_ARGUMENT, "batch: X block of this matrix does not fit in LDS at this k");
+            if (s[m].item_nnz == 0) fail(SPIRA_ERR_INVALID_ARGUMENT, "batch: item_nnz must be positive");
         }
+        b->k = k;
+        b->sched.swap(s);
         batch_drop_graph(b);
-        batch_build_tiles(b);
+        batch_build_work(b);
     });
 }
 
@@ -429,11 +933,44 @@ int spira_batch_spmm(spira_batch *b, const float *X, float *Y, uint32_t flags)
     });
 }
 
+int spira_batch_profile(spira_batch *b, const float *X, float *Y, double *matrix_cost_us)
+{
+    if (!b) return SPIRA_ERR_INVALID_ARGUMENT;
+    return bguard(b->dev, [&] {
+        if (!b->k) fail(SPIRA_ERR_LOGIC, "batch: set a schedule first");
+        if ((b->n_cols && !X) || (b->n_rows && !Y)) fail(SPIRA_ERR_INVALID_ARGUMENT, "batch: X or Y is NULL");
+        if (!matrix_cost_us && b->n_mats) fail(SPIRA_ERR_INVALID_ARGUMENT, "batch: matrix_cost_us is NULL");
+        int rate_khz = 0;
+        SPIRA_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, b->dev->device));
+        if (rate_khz <= 0) fail(SPIRA_ERR_DEVICE, "batch: device wall clock rate unavailable");
+        const size_t pbytes = std::max<size_t>(2 * (size_t)b->n_items, 2) * sizeof(uint64_t);
+        uint64_t *prof = (uint64_t *)dev_alloc(b->dev, pbytes);
+        std::vector<uint64_t> h(2 * (size_t)b->n_items);
+        try {
+            batch_launch(b, X, Y, prof);
+            if (b->n_items)
+                SPIRA_HIP(hipMemcpyAsync(h.data(), prof, h.size() * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                         b->dev->stream));
+            dev_sync(b->dev);
+        } catch (...) {
+            (void)hipStreamSynchronize(b->dev->stream);
+            dev_release(b->dev, prof, pbytes);
+            throw;
+        }
+        dev_release(b->dev, prof, pbytes);
+        const double us_per_tick = 1e3 / (double)rate_khz;
+        std::fill(matrix_cost_us, matrix_cost_us + b->n_mats, 0.0);
+        for (uint32_t i = 0; i < b->n_items; ++i)
+            matrix_cost_us[b->item_mat[i]] += (double)(h[2 * i + 1] - h[2 * i]) * us_per_tick;
+    });
+}
+
 int spira_batch_graph_capture(spira_batch *b, const float *X, float *Y)
 {
     if (!b) return SPIRA_ERR_INVALID_ARGUMENT;
     return bguard(b->dev, [&] {
         if (!b->k) fail(SPIRA_ERR_LOGIC, "batch: set a schedule first");
+        if ((b->n_cols && !X) || (b->n_rows && !Y)) fail(SPIRA_ERR_INVALID_ARGUMENT, "batch: X or Y is NULL");
         batch_drop_graph(b);
         hipStream_t st = b->dev->stream;
         SPIRA_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));

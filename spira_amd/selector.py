@@ -1,17 +1,20 @@
 """Learned schedule selector for batched SpMV.
 
 For every matrix of a batch and every k (1, 4, 16, 32) a schedule is one of
-SCHEDULES: lanes per row (1 << elog2, at most 64 / k: this fixes the rows per
-wavefront, 64 / (lanes * k)) and the work-tile size (nonzeros per tile of the
-persistent kernel).  The exhaustive autotuner (`time_schedules`) times every
-schedule on the device; the selector is a small MLP (hand-written HIP
+SCHEDULES: plog2 (target nonzeros per lane, which sets the lanes per row of
+every row-length class and so the rows per wavefront, 64 / (lanes * k)),
+xlds (stage the matrix's X block in LDS: the LDS tile) and the work-item
+size class.  The exhaustive autotuner (`exhaustive`) runs every schedule on
+the whole batch with device timestamps on every work item
+(`Batch.profile`), which gives every matrix's cost inside its batch from
+one launch per schedule.  The selector is a small MLP (hand-written HIP
 forward/backward/Adam in libspira_amd) that maps the matrix features
 (`matrix_features`, computed on the device) plus log2(k) to the predicted
-log2 slowdown of every schedule relative to the best, and picks the argmin.
-It is trained on autotuner timings over a seeded corpus (`train`) and scored
-as the fraction of the best achievable throughput on held-out matrices.
-The built-in heuristic (`heuristic_schedule`, the batch default) is the
-method without it.
+log2 slowdown of every schedule relative to the best, and picks the argmin
+among the schedules valid for the matrix.  It is trained on autotuner costs
+over a seeded corpus (`train_selector`) and scored as the fraction of the
+best achievable throughput on held-out matrices.  The built-in schedule
+(`heuristic_schedule`, the batch default) is the method without it.
 """
 from __future__ import annotations
 
@@ -21,40 +24,36 @@ import os
 
 import numpy as np
 
-from . import Batch, Device, DeviceCSR, MLP, N_FEATURES, matrix_features
+from . import BATCH_XLDS_FLOATS, BATCH_MAX_PLOG2, Batch, Device, DeviceCSR, MLP, N_FEATURES, matrix_features
 
 K_LIST = (1, 4, 16, 32)
-ELOG2 = tuple(range(7))            # lanes per row 1..64
-TILES = ("small", "default")        # work-tile size class
-SCHEDULES = [(e, t) for e in ELOG2 for t in range(len(TILES))]
+PLOG2 = tuple(range(BATCH_MAX_PLOG2 + 1))
+ITEMS = ("default", "large")        # work-item size class
+SCHEDULES = [(p, x, t) for p in PLOG2 for x in (0, 1) for t in range(len(ITEMS))]
 N_OUT = len(SCHEDULES)
 HIDDEN = 64
 MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "selector_model.json")
 
 
-def max_elog2(k: int) -> int:
-    e = 0
-    while (1 << (e + 1)) * k <= 64:
-        e += 1
-    return e
+def item_nnz(k: int, t: int) -> int:
+    """Work-item size (nonzeros + rows): 'default' is the batch's built-in."""
+    return max(4096, 131072 // k) if t == 0 else max(16384, 524288 // k)
 
 
-def tile_nnz(k: int, t: int) -> int:
-    """Nonzeros per work tile: 'default' matches the batch's built-in choice."""
-    return max(64, 2048 // k) if t == 0 else max(256, 16384 // k)
+def xlds_fits(n_cols, k: int):
+    return np.asarray(n_cols) * k <= BATCH_XLDS_FLOATS
 
 
-def valid_mask(k: int) -> np.ndarray:
-    return np.array([e <= max_elog2(k) for e, t in SCHEDULES], dtype=bool)
+def valid_mask(n_cols: int, k: int) -> np.ndarray:
+    fits = bool(xlds_fits(n_cols, k))
+    return np.array([(x == 0) or fits for p, x, t in SCHEDULES], dtype=bool)
 
 
-def heuristic_schedule(mean_len: float, k: int) -> int:
-    """The batch's built-in default (lanes per row ~ half the mean row
-    length, default tile), as an index into SCHEDULES."""
-    e = 0
-    while e < max_elog2(k) and (1 << e) * 2.0 < mean_len:
-        e += 1
-    return SCHEDULES.index((e, 1))
+def heuristic_schedule(n_cols: int, k: int) -> int:
+    """The batch's built-in schedule (plog2 2, X block in LDS at k = 1 when
+    it fits, default items), as an index into SCHEDULES."""
+    x = 1 if (k == 1 and bool(xlds_fits(n_cols, k))) else 0
+    return SCHEDULES.index((2, x, 0))
 
 
 def features_for(A: DeviceCSR, k: int) -> np.ndarray:
@@ -63,25 +62,30 @@ def features_for(A: DeviceCSR, k: int) -> np.ndarray:
     return f
 
 
-def time_schedules(dev: Device, A: DeviceCSR, k: int, X, Y, stream, reps: int = 3) -> np.ndarray:
-    """Exhaustive autotune of one matrix at one k: mean ms per call for every
-    schedule (NaN where the schedule is not valid for k)."""
-    import torch
-    B = Batch(dev, [A], "f32")
-    out = np.full(N_OUT, np.nan)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    for i, (e, t) in enumerate(SCHEDULES):
-        if e > max_elog2(k):
+def apply(batch: Batch, k: int, choice, n_cols):
+    """Install per-matrix schedule indices into the batch."""
+    choice = np.asarray(choice)
+    p = np.array([SCHEDULES[i][0] for i in choice])
+    x = np.array([SCHEDULES[i][1] for i in choice]) * xlds_fits(n_cols, k)
+    t = np.array([item_nnz(k, SCHEDULES[i][2]) for i in choice])
+    batch.set_schedule(k, plog2=p, xlds=x.astype(np.int64), item_nnz=t)
+
+
+def exhaustive(batch: Batch, k: int, n_cols, X, Y, reps: int = 2) -> np.ndarray:
+    """Exhaustive autotune of a whole batch at one k: every matrix's mean
+    cost (CU-microseconds inside the batch) under every schedule; NaN where
+    the schedule is not valid for the matrix."""
+    n = len(batch.mats)
+    fits = xlds_fits(n_cols, k)
+    out = np.full((n, N_OUT), np.nan)
+    for i, (p, x, t) in enumerate(SCHEDULES):
+        if x and not fits.any():
             continue
-        B.set_schedule(k, elog2=[e], tile_nnz=[tile_nnz(k, t)])
-        B.spmm(X, Y, sync=False)  # warm
-        e0.record(stream)
-        for _ in range(reps):
-            B.spmm(X, Y, sync=False)
-        e1.record(stream)
-        e1.synchronize()
-        out[i] = e0.elapsed_time(e1) / reps
-    B.close()
+        xl = (fits if x else np.zeros(n, bool)).astype(np.int64)
+        batch.set_schedule(k, plog2=p, xlds=xl, item_nnz=item_nnz(k, t))
+        batch.spmm(X, Y, sync=True)  # warm
+        c = np.mean([batch.profile(X, Y) for _ in range(reps)], axis=0)
+        out[:, i] = np.where(fits | (x == 0), c, np.nan)
     return out
 
 
@@ -112,6 +116,8 @@ class Selector:
     def load(cls, dev: Device, path: str = MODEL_PATH):
         with open(path) as f:
             d = json.load(f)
+        if d.get("n_out") != N_OUT or [tuple(s) for s in d.get("schedules", [])] != SCHEDULES:
+            raise ValueError(f"{path}: model was trained for another schedule space")
         s = cls(dev, np.array(d["params"], np.float32))
         s.meta = {k: v for k, v in d.items() if k != "params"}
         return s
@@ -123,16 +129,17 @@ class Selector:
         with open(path, "w") as f:
             json.dump(d, f)
 
-    def predict(self, F: np.ndarray, ks) -> np.ndarray:
-        """Schedule index per row of F (features incl. the k slot) for k in ks."""
+    def predict(self, F: np.ndarray, ks, n_cols) -> np.ndarray:
+        """Schedule index per row of F (features incl. the k slot), for the
+        k and the column count of each row's matrix."""
         import torch
         X = torch.from_numpy(np.ascontiguousarray(F, np.float32)).cuda()
         Y = torch.empty((F.shape[0], N_OUT), dtype=torch.float32, device="cuda")
         torch.cuda.synchronize()
         self.net.forward(X, Y)
         P = Y.cpu().numpy()
-        for i, k in enumerate(ks):
-            P[i, ~valid_mask(k)] = np.inf
+        for i, (k, nc) in enumerate(zip(ks, n_cols)):
+            P[i, ~valid_mask(nc, k)] = np.inf
         return np.argmin(P, axis=1)
 
     def fit(self, F: np.ndarray, T: np.ndarray, epochs: int = 2000, lr: float = 3e-3, log=None):
@@ -154,49 +161,47 @@ class Selector:
                     log(f"selector: epoch {ep} loss {loss:.4f}")
         return hist
 
-    def schedule_batch(self, batch: Batch, mats, k: int):
-        F = np.stack([features_for(A, k) for A in mats])
-        idx = self.predict(F, [k] * len(mats))
-        el = [SCHEDULES[i][0] for i in idx]
-        tl = [tile_nnz(k, SCHEDULES[i][1]) for i in idx]
-        batch.set_schedule(k, elog2=el, tile_nnz=tl)
+    def schedule_batch(self, batch: Batch, mats, k: int, feats=None):
+        """Predict and install a schedule for every matrix of the batch."""
+        F = np.stack([features_for(A, k) for A in mats]) if feats is None else feats
+        n_cols = [A.n_cols for A in mats]
+        idx = self.predict(F, [k] * len(mats), n_cols)
+        apply(batch, k, idx, n_cols)
         return idx
-
-
-def heuristic_batch_schedule(batch: Batch, mats, k: int):
-    batch.set_schedule(k)
 
 
 def build_corpus(dev: Device, n: int, seed0: int, min_rows=256, max_rows=65536):
     """Seeded matrices of the batched workload, uploaded as locked DeviceCSRs."""
     import torch
     from . import workloads as W
-    mats, means = [], []
+    mats = []
     for i in range(n):
         off, col, val = W.batch_matrix(seed0 + i, min_rows=min_rows, max_rows=max_rows, device="cuda")
         nr = off.numel() - 1
         A = DeviceCSR(dev, nr, nr)
         A.upload(off, col, val, device=True)
         mats.append(A)
-        means.append(float(col.numel()) / max(nr, 1))
         del off, col, val
     torch.cuda.synchronize()
-    return mats, np.array(means)
+    return mats
 
 
-def collect(dev: Device, mats, stream, reps: int = 3, log=None):
-    """Features and exhaustive timings for every (matrix, k)."""
+def collect(dev: Device, mats, reps: int = 2, log=None, ks=K_LIST):
+    """Features and exhaustive in-batch costs for every (matrix, k), the
+    matrices forming one FP32 batch.  Returns F, T, K, MI, n_cols."""
     import torch
-    nmax = max(A.n_rows for A in mats)
-    X = torch.randn((nmax * 32,), dtype=torch.float32, device="cuda")
-    Y = torch.empty((nmax * 32,), dtype=torch.float32, device="cuda")
+    B = Batch(dev, mats, "f32")
+    X = torch.randn((B.n_cols * max(ks),), dtype=torch.float32, device="cuda")
+    Y = torch.empty((B.n_rows * max(ks),), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    n_cols = np.array([A.n_cols for A in mats])
     F, T, KS, MI = [], [], [], []
-    for i, A in enumerate(mats):
-        for k in K_LIST:
-            F.append(features_for(A, k))
-            T.append(time_schedules(dev, A, k, X, Y, stream, reps))
-            KS.append(k)
-            MI.append(i)
-        if log and i % 200 == 0:
-            log(f"selector: timed {i}/{len(mats)} matrices")
-    return np.stack(F), np.stack(T), np.array(KS), np.array(MI)
+    for k in ks:
+        F.append(np.stack([features_for(A, k) for A in mats]))
+        T.append(exhaustive(B, k, n_cols, X, Y, reps))
+        KS.append(np.full(len(mats), k))
+        MI.append(np.arange(len(mats)))
+        if log:
+            log(f"selector: k={k} exhaustive costs collected for {len(mats)} matrices")
+    B.close()
+    return np.concatenate(F), np.concatenate(T), np.concatenate(KS), np.concatenate(MI), n_cols

@@ -121,6 +121,48 @@ def powerlaw_rows(n_rows, s=1.8, clip=50000, local_frac=0.7, band=4096, seed=BAS
     return off, cols, vals
 
 
+def powerlaw_rows_device(n_rows, s=1.8, clip=50000, local_frac=0.7, band=4096, seed=BASE_SEED ^ 0xDEADBEEF,
+                         device="cuda"):
+    """The power-law matrix of `powerlaw_rows` drawn with torch on `device`
+    (seconds instead of minutes at 8M rows): same row-length law (numpy's
+    Zipf(s) clipped to [1, clip]) and column law (local_frac uniform in
+    [r - band, r + band] clipped to the matrix, the rest uniform), values
+    N(0, 1).  A row's repeated column draws are merged instead of redrawn,
+    so rows can come out slightly shorter.  Returns (offsets i64, cols i32,
+    vals f64) tensors on `device`."""
+    import torch
+    rng = np.random.default_rng(seed)
+    n = n_rows
+    lens = np.minimum(rng.zipf(s, size=n), min(clip, n)).astype(np.int64)
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed) & 0x7fffffffffffffff)
+    rowid = torch.repeat_interleave(torch.arange(n, device=device, dtype=torch.int64),
+                                    torch.from_numpy(lens).to(device))
+    m = rowid.numel()
+    lo = (rowid - band).clamp_(min=0)
+    hi = (rowid + band + 1).clamp_(max=n)
+    u = torch.rand(m, generator=g, device=device, dtype=torch.float64)
+    local = lo + (u * (hi - lo).double()).long()
+    del lo, hi
+    u = torch.rand(m, generator=g, device=device, dtype=torch.float64)
+    glob = (u * n).long()
+    del u
+    pick = torch.rand(m, generator=g, device=device) < local_frac
+    cols = torch.where(pick, local, glob).clamp_(max=n - 1)
+    del local, glob, pick
+    key, _ = torch.sort(rowid * n + cols)
+    del rowid, cols
+    key = torch.unique_consecutive(key)
+    r = key // n
+    c = (key % n).to(torch.int32)
+    counts = torch.bincount(r, minlength=n)
+    del r, key
+    off = torch.zeros(n + 1, dtype=torch.int64, device=device)
+    off[1:] = torch.cumsum(counts, 0)
+    vals = torch.randn(c.numel(), generator=g, device=device, dtype=torch.float64)
+    return off, c, vals
+
+
 BATCH_ZIPF_CLIP = 1024  # Zipf(1.6) row lengths are clipped to min(n_cols, 1024)
 
 

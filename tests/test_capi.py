@@ -63,6 +63,20 @@ def test_constants_match_python_mirror():
     for name, v in spira_amd.VARIANTS.items():
         key = "SPIRA_SPMV_" + name.upper().replace("STREAM1K", "STREAM1K")
         assert d[key] == v, key
+    assert d["SPIRA_BATCH_XLDS_FLOATS"] == spira_amd.BATCH_XLDS_FLOATS
+    assert d["SPIRA_BATCH_MAX_PLOG2"] == spira_amd.BATCH_MAX_PLOG2
+
+
+def test_batch_schedule_struct_layout():
+    """spira_batch_sched is {u8 plog2, u8 xlds, u16 reserved, u32 item_nnz}:
+    8 bytes, matched by the Python record type."""
+    src = open(HEADER).read()
+    body = re.search(r"typedef struct spira_batch_sched \{(.*?)\} spira_batch_sched;", src, re.S).group(1)
+    fields = re.findall(r"(uint\d+_t)\s+(\w+);", body)
+    assert fields == [("uint8_t", "plog2"), ("uint8_t", "xlds"), ("uint16_t", "reserved"), ("uint32_t", "item_nnz")]
+    dt = spira_amd.BATCH_SCHED_DTYPE
+    assert dt.itemsize == 8 and list(dt.names) == [f for _, f in fields]
+    assert [dt.fields[n][1] for n in dt.names] == [0, 1, 2, 4]
 
 
 def test_null_arguments_are_rejected_without_a_device():

@@ -84,22 +84,73 @@ def test_batch_spmm_matches_oracle(batch, k):
     assert oracle.normwise_rel_err(y.ravel(), ref.ravel()) <= 1e-6
 
 
-@pytest.mark.parametrize("k", [1, 4, 32])
+def _xlds_ok(host, k):
+    return np.array([(len(off) - 1) * k <= spira_amd.BATCH_XLDS_FLOATS for off, _, _ in host])
+
+
+@pytest.mark.parametrize("k", [1, 4, 16, 32])
 def test_every_schedule_gives_the_same_answer(batch, k):
     B, host = batch
     X = np.random.default_rng(7).standard_normal((B.n_cols, k)).astype(np.float32)
     Xt = torch.from_numpy(X).cuda()
     ref = _reference(host, X, k, B.col_base)
-    maxe = int(np.log2(64 // k))
-    nm = len(host)
-    for el in range(maxe + 1):
-        for tile in (64, 4096):
-            B.set_schedule(k, elog2=np.full(nm, el), tile_nnz=np.full(nm, tile))
-            Yt = torch.zeros((B.n_rows, k), dtype=torch.float32, device="cuda")
+    fits = _xlds_ok(host, k)
+    for plog2 in range(spira_amd.BATCH_MAX_PLOG2 + 1):
+        for xl in (0, 1):
+            for item in (64, 4096, 1 << 20):
+                B.set_schedule(k, plog2=plog2, xlds=fits.astype(int) * xl, item_nnz=item)
+                Yt = torch.full((B.n_rows, k), float("nan"), dtype=torch.float32, device="cuda")
+                torch.cuda.synchronize()
+                B.spmm(Xt, Yt)
+                y = Yt.cpu().numpy().astype(np.float64)
+                assert np.isfinite(y).all(), (plog2, xl, item)
+                assert oracle.normwise_rel_err(y.ravel(), ref.ravel()) <= 1e-6, (plog2, xl, item)
+
+
+def test_wide_matrices_use_batch_columns(dev):
+    """A matrix with more than 65536 columns switches the batch to u32
+    columns; empty matrices and empty rows still produce zeros."""
+    rng = np.random.default_rng(4)
+    n = 70_000
+    lens = rng.integers(0, 6, size=n)
+    lens[::7] = 0
+    off = np.zeros(n + 1, np.uint64)
+    np.cumsum(lens, out=off[1:])
+    col = np.concatenate([np.sort(rng.choice(n, size=int(L), replace=False)) for L in lens]).astype(np.uint32)
+    val = rng.standard_normal(col.size).astype(np.float32).astype(np.float64)
+    empty = (np.zeros(11, np.uint64), np.zeros(0, np.uint32), np.zeros(0))
+    mats, host = _make(dev, [400, 401], extra=(off, col, val))
+    for o, c, v in [empty]:
+        A = spira_amd.DeviceCSR(dev, 10, 10)
+        A.upload(o, c, v)
+        mats.append(A)
+        host.append((o, c, v))
+    B = spira_amd.Batch(dev, mats, "f32")
+    for k in (1, 4, 32):
+        X = rng.standard_normal((B.n_cols, k)).astype(np.float32)
+        ref = _reference(host, X, k, B.col_base)
+        for xl in (0, 1):
+            B.set_schedule(k, plog2=2, xlds=_xlds_ok(host, k).astype(int) * xl, item_nnz=2048)
+            Yt = torch.full((B.n_rows, k), float("nan"), dtype=torch.float32, device="cuda")
             torch.cuda.synchronize()
-            B.spmm(Xt, Yt)
+            B.spmm(torch.from_numpy(X).cuda(), Yt)
             y = Yt.cpu().numpy().astype(np.float64)
-            assert oracle.normwise_rel_err(y.ravel(), ref.ravel()) <= 1e-6, (el, tile)
+            assert np.isfinite(y).all()
+            assert oracle.normwise_rel_err(y.ravel(), ref.ravel()) <= 1e-6, (k, xl)
+            assert (y[-10:] == 0).all()
+
+
+def test_schedule_arguments_are_checked(batch):
+    B, host = batch
+    n = len(host)
+    with pytest.raises(spira_amd.InvalidArgument):
+        B.set_schedule(1, plog2=spira_amd.BATCH_MAX_PLOG2 + 1, xlds=0, item_nnz=1024)
+    with pytest.raises(spira_amd.InvalidArgument):
+        B.set_schedule(32, plog2=0, xlds=np.ones(n, int), item_nnz=1024)  # 6000 x 32 floats do not fit in LDS
+    with pytest.raises(spira_amd.InvalidArgument):
+        B.set_schedule(1, plog2=0, xlds=0, item_nnz=0)
+    with pytest.raises(spira_amd.InvalidArgument):
+        B.set_schedule(1, plog2=0)
 
 
 def test_graph_replay_equals_eager(batch):
@@ -146,8 +197,6 @@ def test_batch_errors(dev):
         B.spmm(X, Y)
     with pytest.raises(spira_amd.InvalidArgument):
         B.set_schedule(3)
-    with pytest.raises(spira_amd.InvalidArgument):
-        B.set_schedule(32, elog2=np.full(2, 2))  # 4 lanes x 32 rhs > 64
     unlocked = spira_amd.DeviceCSR(dev, 5, 5)
     with pytest.raises(spira_amd.LogicError):
         spira_amd.Batch(dev, [mats[0], unlocked])
