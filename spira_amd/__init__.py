@@ -201,12 +201,15 @@ def _ptr(a) -> int:
     return int(a.data_ptr())
 
 
-def _producers_done(*arrays):
-    """Device inputs are read on the library's stream: wait for torch's
-    current stream, where the given tensors were produced, first."""
+def _producers_done(dev, *arrays):
+    """Device inputs are read on the library's stream: unless torch's current
+    stream (where the given tensors were produced) is that stream, wait for
+    it first."""
     if any(hasattr(a, "is_cuda") and a.is_cuda for a in arrays):
         import torch
-        torch.cuda.current_stream().synchronize()
+        cur = torch.cuda.current_stream()
+        if cur.cuda_stream != dev.stream:
+            cur.synchronize()
 
 
 class Device:
@@ -307,7 +310,7 @@ class DeviceCSR:
                 raise InvalidArgument("load_csr: cols and vals differ in length")
         else:
             nnz = int(cols.numel())
-            _producers_done(offsets, cols, vals)
+            _producers_done(self.dev, offsets, cols, vals)
         self.dev.check(lib().spira_dcsr_upload_f64(self._h, nnz, _ptr(offsets), _ptr(cols), _ptr(vals),
                                                    IN_DEVICE if device else 0))
 
@@ -334,7 +337,7 @@ class DeviceCSR:
                 if dirty.size != self.n_rows:
                     raise InvalidArgument("lock: dirty must have n_rows entries")
         else:
-            _producers_done(stage_ptr, stage_col, stage_val, dirty)
+            _producers_done(self.dev, stage_ptr, stage_col, stage_val, dirty)
         self.dev.check(lib().spira_lock_f64(self._h, _ptr(stage_ptr), _ptr(stage_col), _ptr(stage_val),
                                             _ptr(dirty), IN_DEVICE if device else 0, ctypes.byref(out)))
         return int(out.value)

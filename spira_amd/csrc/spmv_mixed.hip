@@ -8,8 +8,9 @@
 // Comput. 26(6), 2005): p = a*x, e = fma(a, x, -p) (the exact product error),
 // (s, q) = TwoSum(s, p) (Knuth), c += q + e, result s + c.  The same pass sums
 // |a*x|; a row whose cancellation ratio sum|a*x| / |sum a*x| exceeds the
-// caller's threshold (or whose result is not finite) is queued and recomputed
-// in FP64 from A's FP64 values and the FP64 x by a second kernel.
+// caller's threshold (or whose result is not finite) is flagged (one byte per
+// row, no atomics) and recomputed in FP64 from A's FP64 values and the FP64 x
+// by a second kernel.
 #include "internal.hpp"
 
 #include <cmath>
@@ -85,22 +86,22 @@ struct dot2 {
 };
 
 __device__ __forceinline__ void finish_row(const dot2 &d, uint64_t row, double *y, float ratio,
-                                           uint32_t *queue, uint32_t *qcount)
+                                           uint8_t *flags, uint32_t *count)
 {
     const double v = (double)d.s + (double)d.c;
     y[row] = v;
     if (ratio > 0.0f) {
         const bool bad = !isfinite(v) || (double)d.a > (double)ratio * fabs(v);
-        if (bad) queue[atomicAdd(qcount, 1u)] = (uint32_t)row;
+        flags[row] = bad ? 1u : 0u;
     }
 }
 
 // ---- kernels ---------------------------------------------------------------
-__global__ void k_x_to_f32(const double *__restrict__ x, float *__restrict__ x32, uint64_t n, uint32_t *qcount)
+__global__ void k_x_to_f32(const double *__restrict__ x, float *__restrict__ x32, uint64_t n, uint32_t *count)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) x32[i] = (float)x[i];
-    if (i == 0 && qcount) *qcount = 0u;  // reset the recompute queue (no memset: graph-safe)
+    if (i == 0 && count) *count = 0u;  // reset the recompute count (no memset: graph-safe)
 }
 
 // VEC<G>: G lanes per row.
@@ -108,7 +109,7 @@ template <int G, class Vals>
 __global__ __launch_bounds__(kThreads) void k_mixed_vec(uint64_t n_rows, const uint32_t *__restrict__ off,
                                                          const uint32_t *__restrict__ col, Vals vals,
                                                          const float *__restrict__ x, double *__restrict__ y,
-                                                         float ratio, uint32_t *queue, uint32_t *qcount)
+                                                         float ratio, uint8_t *flags, uint32_t *count)
 {
     const uint64_t gid = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
     const uint64_t row = gid / G;
@@ -118,7 +119,7 @@ __global__ __launch_bounds__(kThreads) void k_mixed_vec(uint64_t n_rows, const u
     dot2 d;
     for (uint32_t k = b + lane; k < e; k += G) d.add(vals(k), x[ld_stream(col + k)]);
     d.group_reduce(G);
-    if (lane == 0) finish_row(d, row, y, ratio, queue, qcount);
+    if (lane == 0) finish_row(d, row, y, ratio, flags, count);
 }
 
 // STREAM<C>: LDS-staged row blocks (same plan as the FP64 kernel); the LDS
@@ -128,7 +129,7 @@ __global__ __launch_bounds__(kThreads) void k_mixed_stream(const uint32_t *__res
                                                             const uint32_t *__restrict__ off,
                                                             const uint32_t *__restrict__ col, Vals vals,
                                                             const float *__restrict__ x, double *__restrict__ y,
-                                                            float ratio, uint32_t *queue, uint32_t *qcount)
+                                                            float ratio, uint8_t *flags, uint32_t *count)
 {
     __shared__ float s_p[2 * C];
     __shared__ float s_e[2 * C];
@@ -151,7 +152,7 @@ __global__ __launch_bounds__(kThreads) void k_mixed_stream(const uint32_t *__res
         if (threadIdx.x == 0) {
             dot2 t;
             for (int i = 0; i < kThreads / 64; ++i) t.merge(s_red[0][i], s_red[1][i], s_red[2][i]);
-            finish_row(t, r0, y, ratio, queue, qcount);
+            finish_row(t, r0, y, ratio, flags, count);
         }
         return;
     }
@@ -174,28 +175,70 @@ __global__ __launch_bounds__(kThreads) void k_mixed_stream(const uint32_t *__res
         dot2 d;
         for (uint32_t k = a + lane; k < e; k += T) d.add_prod(s_p[k], s_e[k]);
         d.group_reduce(T);
-        if (lane == 0) finish_row(d, r0 + r, y, ratio, queue, qcount);
+        if (lane == 0) finish_row(d, r0 + r, y, ratio, flags, count);
     }
 }
 
-// FP64 recompute of queued rows: one wavefront per row.
-__global__ __launch_bounds__(kThreads) void k_recompute_f64(const uint32_t *__restrict__ queue,
-                                                             const uint32_t *__restrict__ qcount,
+// FP64 recompute of flagged rows.  A wavefront takes 64 rows at a time,
+// finds the flagged ones with a ballot and recomputes them eight at a time
+// (8 lanes per row); *count (reset by k_x_to_f32) gets the number of
+// recomputed rows with one atomic per block.
+__global__ __launch_bounds__(kThreads) void k_recompute_f64(const uint8_t *__restrict__ flags, uint64_t n_rows,
+                                                             uint32_t *__restrict__ count,
                                                              const uint64_t *__restrict__ off,
                                                              const uint32_t *__restrict__ col,
                                                              const double *__restrict__ val,
                                                              const double *__restrict__ x, double *__restrict__ y)
 {
-    const uint32_t count = *qcount;
-    const uint32_t waves = gridDim.x * (kThreads / 64);
+    __shared__ uint32_t s_cnt;
+    if (threadIdx.x == 0) s_cnt = 0u;
+    __syncthreads();
+    const uint64_t waves = (uint64_t)gridDim.x * (kThreads / 64);
     const int lane = threadIdx.x & 63;
-    for (uint32_t w = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); w < count; w += waves) {
-        const uint32_t row = queue[w];
-        double acc = 0.0;
-        for (uint64_t k = off[row] + lane; k < off[row + 1]; k += 64) acc += val[k] * x[col[k]];
-        acc = group_sum<64>(acc);
-        if (lane == 0) y[row] = acc;
+    const int g = lane >> 3, gl = lane & 7;
+    uint32_t mine = 0;
+    for (uint64_t c = (uint64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); c * 64 < n_rows; c += waves) {
+        const uint64_t r = c * 64 + (uint64_t)lane;
+        const bool f = r < n_rows && flags[r] != 0;
+        const uint64_t rb = f ? off[r] : 0, re = f ? off[r + 1] : 0;
+        uint64_t m = __ballot(f);
+        mine += (uint32_t)__popcll(m);
+        // rows of more than 64 nonzeros: the whole wavefront, one at a time
+        uint64_t ml = __ballot(f && re - rb > 64);
+        m &= ~ml;
+        while (ml) {
+            const int bit = __ffsll((long long)ml) - 1;
+            ml &= ml - 1;
+            const uint64_t row = c * 64 + (uint64_t)bit;
+            const uint64_t b = __shfl(rb, bit, 64), e = __shfl(re, bit, 64);
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+            uint64_t q = b + (uint64_t)lane;
+            for (; q + 192 < e; q += 256) {
+                a0 += val[q] * x[col[q]];
+                a1 += val[q + 64] * x[col[q + 64]];
+                a2 += val[q + 128] * x[col[q + 128]];
+                a3 += val[q + 192] * x[col[q + 192]];
+            }
+            for (; q < e; q += 64) a0 += val[q] * x[col[q]];
+            const double acc = group_sum<64>((a0 + a1) + (a2 + a3));
+            if (lane == 0) y[row] = acc;
+        }
+        // the others: lane group g takes flagged rows g, g + 8, g + 16, ... of
+        // the chunk (in bit order), so the eight groups work concurrently
+        for (int s = 0; s < g && m; ++s) m &= m - 1;
+        while (m) {  // divergent across groups: bounds come from memory, not a shuffle
+            const uint64_t row = c * 64 + (uint64_t)(__ffsll((long long)m) - 1);
+            const uint64_t b = off[row], e = off[row + 1];
+            double acc = 0.0;
+            for (uint64_t q = b + gl; q < e; q += 8) acc += val[q] * x[col[q]];
+            acc = group_sum<8>(acc);
+            if (gl == 0) y[row] = acc;
+            for (int s = 0; s < 8 && m; ++s) m &= m - 1;
+        }
     }
+    if (lane == 0 && mine) atomicAdd(&s_cnt, mine);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_cnt) atomicAdd(count, s_cnt);
 }
 
 // ---- storage conversion ------------------------------------------------------
@@ -243,8 +286,8 @@ void ensure_storage(spira_dcsr *A, int storage)
 }
 
 template <class Vals>
-void launch_mixed(spira_dcsr *A, int variant, Vals vals, const float *x32, double *y, float ratio, uint32_t *queue,
-                  uint32_t *qcount, const uint32_t *blk, uint32_t nblk)
+void launch_mixed(spira_dcsr *A, int variant, Vals vals, const float *x32, double *y, float ratio, uint8_t *flags,
+                  uint32_t *count, const uint32_t *blk, uint32_t nblk)
 {
     hipStream_t st = A->dev->stream;
     const uint32_t *off = A->plan.off32;
@@ -252,25 +295,25 @@ void launch_mixed(spira_dcsr *A, int variant, Vals vals, const float *x32, doubl
     switch (variant) {
     case SPIRA_SPMV_VEC4:
         k_mixed_vec<4, Vals><<<grid_for(n * 4, kThreads), kThreads, 0, st>>>(n, off, A->col, vals, x32, y, ratio,
-                                                                            queue, qcount);
+                                                                            flags, count);
         break;
     case SPIRA_SPMV_VEC8:
         k_mixed_vec<8, Vals><<<grid_for(n * 8, kThreads), kThreads, 0, st>>>(n, off, A->col, vals, x32, y, ratio,
-                                                                            queue, qcount);
+                                                                            flags, count);
         break;
     case SPIRA_SPMV_VEC16:
         k_mixed_vec<16, Vals><<<grid_for(n * 16, kThreads), kThreads, 0, st>>>(n, off, A->col, vals, x32, y,
-                                                                              ratio, queue, qcount);
+                                                                              ratio, flags, count);
         break;
     case SPIRA_SPMV_VEC32:
         k_mixed_vec<32, Vals><<<grid_for(n * 32, kThreads), kThreads, 0, st>>>(n, off, A->col, vals, x32, y,
-                                                                              ratio, queue, qcount);
+                                                                              ratio, flags, count);
         break;
     case SPIRA_SPMV_STREAM1K:
-        k_mixed_stream<1024, Vals><<<nblk, kThreads, 0, st>>>(blk, off, A->col, vals, x32, y, ratio, queue, qcount);
+        k_mixed_stream<1024, Vals><<<nblk, kThreads, 0, st>>>(blk, off, A->col, vals, x32, y, ratio, flags, count);
         break;
     default:
-        k_mixed_stream<2048, Vals><<<nblk, kThreads, 0, st>>>(blk, off, A->col, vals, x32, y, ratio, queue, qcount);
+        k_mixed_stream<2048, Vals><<<nblk, kThreads, 0, st>>>(blk, off, A->col, vals, x32, y, ratio, flags, count);
         break;
     }
     SPIRA_HIP(hipGetLastError());
@@ -284,9 +327,15 @@ bool mixed_variant_supported(int variant)
            variant == SPIRA_SPMV_VEC32 || variant == SPIRA_SPMV_STREAM1K || variant == SPIRA_SPMV_STREAM2K;
 }
 
+// WS_MIXQ: one flag byte per row, then (16-byte aligned) the recompute count.
+static uint8_t *mixed_flags(spira_dcsr *A)
+{
+    return (uint8_t *)A->dev->ws.get(WS_MIXQ, ((A->n_rows + 15) & ~(uint64_t)15) + 16);
+}
+
 uint32_t *mixed_queue_count(spira_dcsr *A)
 {
-    return (uint32_t *)A->dev->ws.get(WS_MIXQ, (A->n_rows + 16) * sizeof(uint32_t)) + A->n_rows + 8;
+    return (uint32_t *)(mixed_flags(A) + ((A->n_rows + 15) & ~(uint64_t)15));
 }
 
 void spmv_mixed_launch(spira_dcsr *A, int storage, const double *x, double *y, int variant, float ratio)
@@ -309,21 +358,21 @@ void spmv_mixed_launch(spira_dcsr *A, int storage, const double *x, double *y, i
         const int which = variant == SPIRA_SPMV_STREAM1K ? 0 : 1;
         spmv_stream_blocks(A, which, &blk, &nblk);
     }
-    uint32_t *queue = (uint32_t *)dev->ws.get(WS_MIXQ, (A->n_rows + 16) * sizeof(uint32_t));
-    uint32_t *qcount = queue + A->n_rows + 8;
+    uint8_t *flags = mixed_flags(A);
+    uint32_t *count = mixed_queue_count(A);
     float *x32 = (float *)dev->ws.get(WS_X32, (A->n_cols ? A->n_cols : 1) * sizeof(float));
-    k_x_to_f32<<<grid_for(A->n_cols ? A->n_cols : 1, kThreads), kThreads, 0, st>>>(x, x32, A->n_cols, qcount);
+    k_x_to_f32<<<grid_for(A->n_cols ? A->n_cols : 1, kThreads), kThreads, 0, st>>>(x, x32, A->n_cols, count);
     SPIRA_HIP(hipGetLastError());
     if (storage == SPIRA_VAL_F32)
-        launch_mixed(A, variant, vals_f32{A->val_f32}, x32, y, ratio, queue, qcount, blk, nblk);
+        launch_mixed(A, variant, vals_f32{A->val_f32}, x32, y, ratio, flags, count, blk, nblk);
     else if (storage == SPIRA_VAL_BF16)
-        launch_mixed(A, variant, vals_bf16{A->val_bf16}, x32, y, ratio, queue, qcount, blk, nblk);
+        launch_mixed(A, variant, vals_bf16{A->val_bf16}, x32, y, ratio, flags, count, blk, nblk);
     else
-        launch_mixed(A, variant, vals_split{A->val_split_hi, A->val_bf16_lo}, x32, y, ratio, queue, qcount, blk,
+        launch_mixed(A, variant, vals_split{A->val_split_hi, A->val_bf16_lo}, x32, y, ratio, flags, count, blk,
                      nblk);
     if (ratio > 0.0f) {
         const unsigned g = std::min<uint64_t>(grid_for(A->n_rows, kThreads / 64), 2048u);
-        k_recompute_f64<<<g, kThreads, 0, st>>>(queue, qcount, A->off, A->col, A->val, x, y);
+        k_recompute_f64<<<g, kThreads, 0, st>>>(flags, A->n_rows, count, A->off, A->col, A->val, x, y);
         SPIRA_HIP(hipGetLastError());
     }
 }
