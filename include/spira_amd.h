@@ -136,6 +136,15 @@ int spira_spmv_set_variant(spira_dcsr *A, int variant);
  * AUTO choice and returns it in *best (either output may be NULL). */
 int spira_spmv_autotune_f64(spira_dcsr *A, const double *x_dev, double *y_dev, int reps, float *ms, int *best);
 
+/* ---- partitions -------------------------------------------------------------
+ * compute_partition_boundaries (partition.hpp:78-126) as used by
+ * parallel_matrix::rebalance() (parallel_matrix.hpp:434-497), on the device
+ * from A's offsets (the per-row nnz prefix): boundaries[n_threads+1] (host
+ * array) receives nnz-balanced row boundaries, equal to the reference's.
+ * SPIRA_ERR_INVALID_ARGUMENT for n_threads == 0, SPIRA_ERR_LOGIC if A is not
+ * built ("rebalance: matrix must be locked"). */
+int spira_partition_boundaries(spira_dcsr *A, uint64_t n_threads, uint64_t *boundaries);
+
 /* ---- mixed precision ------------------------------------------------------
  * No counterpart in the reference (its float instantiation,
  * serial/spmv.hpp:90-135, stores and accumulates in FP32).  y = A*x with A's

@@ -141,6 +141,13 @@ def lib():
         if not os.path.exists(_LIB_PATH):
             raise ImportError(
                 f"{_LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+        # torch ships its own HIP runtime: load it first so that the library
+        # binds to the same one (one runtime per process, whichever of the two
+        # touches the GPU first)
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(_LIB_PATH)
         sig = {
             "spira_dev_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _vp]),
@@ -158,6 +165,7 @@ def lib():
                                               ctypes.c_uint32]),
             "spira_spmv_choose": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
             "spira_spmv_set_variant": (ctypes.c_int, [_vp, ctypes.c_int]),
+            "spira_partition_boundaries": (ctypes.c_int, [_vp, ctypes.c_uint64, _vp]),
             "spira_spmv_autotune_f64": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
                                                        ctypes.POINTER(ctypes.c_int)]),
             "spira_spmv_mixed_f64": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_uint64, _vp, ctypes.c_uint64,
@@ -376,6 +384,12 @@ class DeviceCSR:
                                                   float(recompute_ratio), flags,
                                                   ctypes.byref(nrec) if count else None))
         return y, (int(nrec.value) if count else None)
+
+    def partition_boundaries(self, n_threads: int) -> np.ndarray:
+        """compute_partition_boundaries (partition.hpp:78-126) on the device."""
+        b = np.zeros(int(n_threads) + 1, np.uint64)
+        self.dev.check(lib().spira_partition_boundaries(self._h, int(n_threads), _ptr(b)))
+        return b
 
     def choose_variant(self) -> int:
         v = ctypes.c_int()
@@ -801,6 +815,14 @@ class ParallelMatrix(Matrix):
 
     def partition_bounds(self):
         return list(self._bounds)
+
+    def rebalance(self):
+        """parallel_matrix::rebalance() (parallel_matrix.hpp:434-497): move the
+        partition boundaries to the nnz-balanced ones (computed on the device);
+        the device CSR is one concatenated array, so no entry moves."""
+        if not self._locked:
+            raise LogicError("rebalance: matrix must be locked")
+        self._bounds = [int(v) for v in self._csr.partition_boundaries(self._n_threads)]
 
     def partition_at(self, t):
         if t < 0 or t >= self._n_threads:

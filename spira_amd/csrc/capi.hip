@@ -32,6 +32,42 @@ int guarded(spira_dev *dev, F &&f)
     }
 }
 
+// compute_partition_boundaries (partition.hpp:78-126) on a locked CSR: the
+// per-row nnz prefix is the offsets array itself.  Boundary t is the first
+// row whose prefix reaches max(1, t * total / n_threads); the targets grow
+// with t, so searching the whole prefix gives the reference's search from
+// boundary t-1.  total == 0 gives the uniform split t * n_rows / n_threads.
+__global__ void k_partition_bounds(const uint64_t *__restrict__ off, uint64_t n_rows, uint64_t n_threads,
+                                   uint64_t *__restrict__ b)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > n_threads) return;
+    if (t == 0) {
+        b[0] = 0;
+        return;
+    }
+    if (t == n_threads || n_threads <= 1 || n_rows == 0) {
+        b[t] = n_rows;
+        return;
+    }
+    const uint64_t total = off[n_rows];
+    if (total == 0) {
+        b[t] = t * n_rows / n_threads;
+        return;
+    }
+    // t * total / n_threads without overflowing 64 bits
+    const uint64_t q = total / n_threads, r = total % n_threads;
+    uint64_t target = t * q + (t * r) / n_threads;
+    if (target < 1) target = 1;
+    uint64_t lo = 0, hi = n_rows + 1;
+    while (lo < hi) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if (off[mid] < target) lo = mid + 1;
+        else hi = mid;
+    }
+    b[t] = lo < n_rows ? lo : n_rows;
+}
+
 void copy_in(spira_dev *dev, void *dst, const void *src, size_t bytes, bool src_device)
 {
     if (bytes == 0) return;
@@ -397,6 +433,29 @@ int spira_spmv_set_variant(spira_dcsr *A, int variant)
         if (variant < SPIRA_SPMV_AUTO || variant >= SPIRA_SPMV_N_VARIANTS)
             fail(SPIRA_ERR_INVALID_ARGUMENT, "spmv: unknown kernel variant");
         A->plan.variant = variant;
+    });
+}
+
+int spira_partition_boundaries(spira_dcsr *A, uint64_t n_threads, uint64_t *boundaries)
+{
+    if (!A || !boundaries) return SPIRA_ERR_INVALID_ARGUMENT;
+    spira_dev *dev = A->dev;
+    return guarded(dev, [&] {
+        if (n_threads == 0) fail(SPIRA_ERR_INVALID_ARGUMENT, "parallel_matrix requires at least one thread");
+        if (!A->built) fail(SPIRA_ERR_LOGIC, "rebalance: matrix must be locked");
+        const size_t bytes = (n_threads + 1) * sizeof(uint64_t);
+        uint64_t *b = (uint64_t *)dev_alloc(dev, bytes);
+        try {
+            k_partition_bounds<<<grid_for(n_threads + 1, 256), 256, 0, dev->stream>>>(A->off, A->n_rows, n_threads, b);
+            SPIRA_HIP(hipGetLastError());
+            SPIRA_HIP(hipMemcpyAsync(boundaries, b, bytes, hipMemcpyDeviceToHost, dev->stream));
+            dev_sync(dev);
+        } catch (...) {
+            (void)hipStreamSynchronize(dev->stream);
+            dev_release(dev, b, bytes);
+            throw;
+        }
+        dev_release(dev, b, bytes);
     });
 }
 

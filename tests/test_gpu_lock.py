@@ -264,3 +264,48 @@ def test_parallel_matrix_partitions_are_slices(dev):
         assert np.array_equal(pc, col[int(off[a]):int(off[b])])
     with pytest.raises(spira_amd.OutOfRange):
         pm.partition_at(3)
+
+
+@pytest.mark.parametrize("case", KATS["partition_balanced"], ids=lambda c: c["source"])
+def test_device_partition_boundaries_known_answers(dev, case):
+    """compute_partition_boundaries on the device (from a locked CSR's
+    offsets) equals the reference's known answers (partition_tests.cpp)."""
+    row_nnz = case["row_nnz"]
+    n = len(row_nnz)
+    if n == 0:
+        pytest.skip("no rows")
+    off = np.zeros(n + 1, np.uint64)
+    np.cumsum(row_nnz, out=off[1:])
+    width = max(max(row_nnz), 1)
+    col = np.concatenate([np.arange(k, dtype=np.uint32) for k in row_nnz]) if off[-1] else np.zeros(0, np.uint32)
+    A = spira_amd.DeviceCSR(dev, n, width)
+    A.upload(off, col, np.ones(col.size))
+    b = A.partition_boundaries(case["n_threads"]).tolist()
+    if "bounds" in case:
+        assert b == case["bounds"]
+    for k, v in case.get("bounds_at", {}).items():
+        assert b[int(k)] == v
+
+
+def test_device_partition_boundaries_random_and_rebalance(dev):
+    rng = np.random.default_rng(8)
+    for trial in range(10):
+        n = int(rng.integers(1, 5000))
+        lens = rng.integers(0, 50, n) * (rng.random(n) < 0.7)
+        off = np.zeros(n + 1, np.uint64)
+        np.cumsum(lens, out=off[1:])
+        col = np.concatenate([np.arange(k, dtype=np.uint32) for k in lens]) if off[-1] else np.zeros(0, np.uint32)
+        A = spira_amd.DeviceCSR(dev, n, 64)
+        A.upload(off, col, np.ones(col.size))
+        for T in (1, 2, 7, 64, n + 3):
+            assert np.array_equal(A.partition_boundaries(T), oracle.partition_bounds(lens, T)), (trial, T)
+    pm = spira_amd.ParallelMatrix(1000, 100, 4, dev)
+    with pytest.raises(spira_amd.LogicError):
+        pm.rebalance()
+    for r in range(1000):
+        for c in range(1 + r // 20):
+            pm.insert(r, c, 1.0)
+    pm.lock()
+    pm.rebalance()
+    off = pm.csr()[0]
+    assert pm.partition_bounds() == oracle.partition_bounds(np.diff(off.astype(np.int64)), 4).tolist()
