@@ -177,13 +177,16 @@ int spira_spmv_mixed_f64(spira_dcsr *A, int storage, const double *x, uint64_t x
  * (spira_batch_sched; NULL = built-in choices): lanes per row of class c =
  * 2^c >> plog2 (at least 1, at most 64 / k), the nonzeros per work item (the
  * device queue's unit), and whether the matrix's X block (n_cols * k floats,
- * at most SPIRA_BATCH_XLDS_FLOATS) is staged in LDS for its gathers.
+ * at most SPIRA_BATCH_XLDS_FLOATS; at k = 1 also up to
+ * SPIRA_BATCH_XLDS_TWO_PASS_COLS columns, staged in two halves) is staged in
+ * LDS for its gathers.
  * spira_batch_spmm runs every item in one persistent launch fed by a device
  * work queue; each output is accumulated in FP32 with compensated (Dot2)
  * summation.  The call can be captured once into a hipGraph (bound to the X
  * and Y given) and replayed.  X and Y are device pointers; flags:
  * SPIRA_NO_SYNC.  Destroy batches before their device. */
 #define SPIRA_BATCH_XLDS_FLOATS 36864
+#define SPIRA_BATCH_XLDS_TWO_PASS_COLS 65536
 #define SPIRA_BATCH_MAX_PLOG2 5
 typedef struct spira_batch_sched {
     uint8_t plog2;     /* target nonzeros per lane: 1 << plog2 */

@@ -62,6 +62,7 @@ STORAGE = {"f32": 1, "bf16": 2, "split_bf16": 3}
 MIXED_VARIANTS = [VARIANTS[k] for k in ("vec4", "vec8", "vec16", "vec32", "stream1k", "stream2k")]
 N_VARIANTS = 13
 BATCH_XLDS_FLOATS = 36864
+BATCH_XLDS_TWO_PASS_COLS = 65536
 BATCH_MAX_PLOG2 = 5
 BATCH_SCHED_DTYPE = np.dtype([("plog2", "u1"), ("xlds", "u1"), ("reserved", "<u2"), ("item_nnz", "<u4")])
 LOCK_MAX_ROW_INSERTS = 8192

@@ -52,7 +52,13 @@ static_assert(SPIRA_BATCH_XLDS_FLOATS * 4 <= kLdsBytes, "X block must fit the LD
 constexpr uint32_t kMinEntryLds = 65536;  // stage entries only with at least this much LDS left
 constexpr uint32_t kClasses = 34;         // bit length of a u32 row length, plus one
 constexpr uint32_t kTileLong = 0x80u;
-constexpr uint32_t kItemStaged = 1u;  // item flag: entries staged in LDS
+constexpr uint32_t kItemStaged = 1u;   // item flag: entries staged in LDS
+constexpr uint32_t kItemTwoPass = 2u;  // item flag: k = 1, X block staged in two column halves
+// Two-pass items: each half of the X block (at most kHalfFloats floats) sits
+// at the start of the LDS area, followed by one (sum, correction, cursor)
+// record per row of the item.
+constexpr uint32_t kHalfFloats = 32768;
+constexpr uint32_t kTwoPassRows = (kLdsBytes - kHalfFloats * 4) / 12;
 constexpr size_t kArrayPad = 64;      // bytes after each entry array (16-byte staging reads)
 
 __device__ __forceinline__ float bf16f(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
@@ -69,7 +75,7 @@ struct item_t {
     uint32_t xcols;   // columns whose X rows are staged in LDS (0: gathers read global memory)
     uint32_t r0, r1;  // stored rows of the item
     uint32_t ebase;   // first entry of row r0
-    uint32_t flags;   // kItemStaged
+    uint32_t flags;   // kItemStaged, kItemTwoPass
 };
 
 inline __host__ __device__ uint32_t align16(uint32_t v) { return (v + 15u) & ~15u; }
@@ -397,6 +403,88 @@ __device__ __forceinline__ void run_item(const kparams &kp, const item_t &it, un
     }
 }
 
+// k = 1 item whose X block is staged in two column halves.  Pass h stages
+// columns [h W, min(n, (h + 1) W)) of the matrix (W = ceil(n / 2)); each row's
+// lanes consume its entries from the row's cursor while their column is in the
+// half (columns are sorted, so a row's entries of half 0 precede those of half
+// 1), merge into the row's (sum, correction) record and advance the cursor.
+// Entries are read from global memory; the rows carry no whole-block tiles.
+template <int ST, typename CT>
+__device__ __forceinline__ void run_item_two_pass(const kparams &kp, const item_t &it, unsigned char *lds, int lane,
+                                                  int w)
+{
+    constexpr int U = 4;
+    const uint32_t n = it.xcols, W = (n + 1) / 2;
+    const uint32_t rows = it.r1 - it.r0;
+    float *xs = (float *)lds;
+    float *s_sum = (float *)(lds + kHalfFloats * 4);
+    float *s_cor = s_sum + kTwoPassRows;
+    uint32_t *s_pos = (uint32_t *)(s_cor + kTwoPassRows);
+    const esrc<ST, CT, false> es{(const CT *)kp.col, kp.f32, kp.hi, kp.lo, kp.last};
+    for (uint32_t q = threadIdx.x; q < rows; q += kThreads) {
+        s_sum[q] = 0.0f;
+        s_cor[q] = 0.0f;
+        s_pos[q] = kp.off[it.r0 + q];
+    }
+    for (uint32_t h = 0; h < 2; ++h) {
+        const uint32_t c0 = h * W, c1 = min(n, c0 + W);
+        __syncthreads();  // the previous pass is done with the half and the records
+        const float *src = kp.X + (uint64_t)it.cbase + c0;
+        for (uint32_t q = threadIdx.x; q < c1 - c0; q += kThreads) xs[q] = src[q];
+        __syncthreads();
+        for (uint32_t tt = it.t0; tt < it.t1; ++tt) {
+            const tile_t tl = kp.tiles[tt];
+            const uint32_t E = 1u << (tl.meta & 0x7fu);
+            const uint32_t e = (uint32_t)lane & (E - 1);
+            const uint32_t rs = (uint32_t)lane / E;
+            const uint32_t rpw = 64u / E;
+            for (uint32_t p = tl.r0 + (uint32_t)w * rpw + rs; p < tl.r1; p += kWaves * rpw) {
+                const uint32_t lq = p - it.r0;
+                const uint32_t beg = s_pos[lq], end = kp.off[p + 1];
+                acc2 a;
+                uint32_t upto = beg;  // one past the last entry this lane consumed
+                for (uint32_t i = beg + e; i < end; i += U * E) {
+                    uint32_t c[U];
+                    float v[U], xx[U];
+                    bool ok[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        es.get(i + u * E < end, i + u * E, c[u], v[u]);
+                        const uint32_t lc = (sizeof(CT) == 2) ? c[u] : c[u] - it.cbase;
+                        ok[u] = i + u * E < end && lc < c1;
+                        xx[u] = xs[ok[u] ? lc - c0 : 0u];
+                        if (!ok[u]) {
+                            v[u] = 0.0f;
+                            xx[u] = 0.0f;
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        a.add(v[u], xx[u]);
+                        if (ok[u]) upto = i + u * E + 1;
+                    }
+                    if (!ok[U - 1]) break;  // sorted columns: the rest of the row is in the next half
+                }
+                for (uint32_t o = 1; o < E; o <<= 1) {
+                    a.merge(__shfl_xor(a.s, (int)o, 64), __shfl_xor(a.c, (int)o, 64));
+                    upto = max(upto, (uint32_t)__shfl_xor((int)upto, (int)o, 64));
+                }
+                if (e == 0) {
+                    acc2 r;
+                    r.s = s_sum[lq];
+                    r.c = s_cor[lq];
+                    r.merge(a.s, a.c);
+                    s_sum[lq] = r.s;
+                    s_cor[lq] = r.c;
+                    s_pos[lq] = upto;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < rows; q += kThreads) kp.Y[kp.rowid[it.r0 + q]] = s_sum[q] + s_cor[q];
+}
+
 // prof (profiling launches only, else null): prof[2 i] / prof[2 i + 1] get
 // the device wall clock when the block starts item i and when every wave of
 // the block has finished it.
@@ -424,6 +512,12 @@ __global__ __launch_bounds__(kThreads) void k_batch_spmm(kparams kp)
         prev = i;
         if (i >= kp.n_items) break;  // block-uniform: the whole block leaves
         const item_t it = kp.items[i];
+        if constexpr (K == 1) {
+            if (it.flags & kItemTwoPass) {
+                run_item_two_pass<ST, CT>(kp, it, lds, lane, w);
+                continue;
+            }
+        }
         const bool xs = it.xcols != 0, staged = (it.flags & kItemStaged) != 0;
         const uint32_t xbytes = xs ? it.xcols * K * 4 : 0u;
         if (xs) {
@@ -598,14 +692,18 @@ static void batch_build_work(spira_batch *b)
     for (uint32_t m = 0; m < b->n_mats; ++m) {
         const spira_batch_sched &s = b->sched[m];
         const uint64_t budget = std::max<uint32_t>(s.item_nnz, 1u);
-        const uint32_t xcols = s.xlds ? (uint32_t)(b->col_base[m + 1] - b->col_base[m]) : 0u;
+        const uint32_t ncols = (uint32_t)(b->col_base[m + 1] - b->col_base[m]);
+        // k = 1 X blocks too wide for one LDS load go in two halves (rows of
+        // whole-block tiles excepted: those read X from global memory)
+        const bool two_pass = s.xlds && K == 1 && ncols > SPIRA_BATCH_XLDS_FLOATS;
+        const uint32_t xcols = (s.xlds && !two_pass) ? ncols : 0u;  // single-load X block
         const uint32_t xbytes = xcols * K * 4;
-        const bool stage = align16(xbytes) + kMinEntryLds <= kLdsBytes;
+        const bool stage = !two_pass && align16(xbytes) + kMinEntryLds <= kLdsBytes;
         const uint32_t *cs = &b->class_start[(size_t)m * (kClasses + 1)];
         // cost of a run of rows = nonzeros + rows (an empty row still writes k outputs)
         item_t cur{(uint32_t)tl.size(), 0, (uint32_t)b->col_base[m], xcols, 0, 0, 0, 0};
         uint32_t ir0 = 0;  // first stored row of the open item
-        bool open = false;
+        bool open = false, item_tp = false;
         uint64_t cur_cost = 0;
         // the open item's rows [ir0, r_end) fit in LDS beside the X block
         auto fits = [&](uint32_t r_end) {
@@ -619,8 +717,9 @@ static void batch_build_work(spira_batch *b)
                 cur.r0 = ir0;
                 cur.r1 = r_end;
                 cur.ebase = off[ir0];
-                cur.flags = (may_stage && fits(r_end)) ? kItemStaged : 0u;
-                its.push_back({cur, (cur_cost + xcols) * K, m});
+                cur.xcols = item_tp ? ncols : xcols;
+                cur.flags = item_tp ? kItemTwoPass : ((may_stage && fits(r_end)) ? kItemStaged : 0u);
+                its.push_back({cur, (cur_cost + cur.xcols) * K, m});
             }
             cur.t0 = (uint32_t)tl.size();
             cur_cost = 0;
@@ -633,17 +732,20 @@ static void batch_build_work(spira_batch *b)
             // nonzeros are whole-block tiles of one row each
             const bool lng = c >= long_class;
             const uint32_t meta = lng ? kTileLong : std::min<uint32_t>(c > s.plog2 ? c - s.plog2 : 0u, emax);
+            if (open && item_tp && lng) close_item(a, false);  // two-pass items hold no whole-block rows
             uint32_t p = a;
             while (p < z) {
                 if (!open) {
                     ir0 = p;
                     open = true;
+                    item_tp = two_pass && !lng;
                 }
                 // extend the tile while the item stays within its budget and,
-                // when staged, within the LDS
+                // when staged, within the LDS (two-pass items: within the rows
+                // their LDS records hold)
                 uint32_t q = p;
                 while (q < z && (!lng || q == p) && cur_cost + (off[q + 1] - off[p]) + (q + 1 - p) <= budget &&
-                       (!stage || fits(q + 1)))
+                       (!stage || fits(q + 1)) && (!item_tp || q + 1 - ir0 <= kTwoPassRows))
                     ++q;
                 if (q == p) {
                     if (p > ir0) {  // the item is full: close it and retry the row in a new one
@@ -655,7 +757,8 @@ static void batch_build_work(spira_batch *b)
                 tl.push_back(tile_t{p, q, meta, 0});
                 cur_cost += (off[q] - off[p]) + (q - p);
                 p = q;
-                if (cur_cost >= budget || (stage && p < z && !fits(p + 1))) close_item(p, stage);
+                if (cur_cost >= budget || (stage && p < z && !fits(p + 1)) || (item_tp && p - ir0 >= kTwoPassRows))
+                    close_item(p, stage);
             }
         }
         if (open) close_item(cs[kClasses], stage);
@@ -739,7 +842,7 @@ static spira_batch_sched default_sched(uint64_t n_cols, uint32_t K)
 {
     spira_batch_sched s{};
     s.plog2 = 2;
-    s.xlds = (K == 1 && n_cols * K <= SPIRA_BATCH_XLDS_FLOATS) ? 1 : 0;
+    s.xlds = (K == 1 && n_cols <= SPIRA_BATCH_XLDS_TWO_PASS_COLS) ? 1 : 0;
     s.item_nnz = std::max<uint32_t>(4096u, 131072u / K);
     return s;
 }
@@ -911,7 +1014,7 @@ int spira_batch_set_schedule(spira_batch *b, uint32_t k, const spira_batch_sched
             if (s[m].plog2 > SPIRA_BATCH_MAX_PLOG2)
                 fail(SPIRA_ERR_INVALID_ARGUMENT, "batch: plog2 must be at most SPIRA_BATCH_MAX_PLOG2");
             if (s[m].xlds > 1) fail(SPIRA_ERR_INVALID_ARGUMENT, "batch: xlds must be 0 or 1");
-            if (s[m].xlds && nc * k > SPIRA_BATCH_XLDS_FLOATS)
+            if (s[m].xlds && nc * k > SPIRA_BATCH_XLDS_FLOATS && !(k == 1 && nc <= SPIRA_BATCH_XLDS_TWO_PASS_COLS))
                 fail(SPIRA_ERR_INVALID_ARGUMENT, "batch: X block of this matrix does not fit in LDS at this k");
             if (s[m].item_nnz == 0) fail(SPIRA_ERR_INVALID_ARGUMENT, "batch: item_nnz must be positive");
         }

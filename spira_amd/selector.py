@@ -24,7 +24,8 @@ import os
 
 import numpy as np
 
-from . import BATCH_XLDS_FLOATS, BATCH_MAX_PLOG2, Batch, Device, DeviceCSR, MLP, N_FEATURES, matrix_features
+from . import (BATCH_MAX_PLOG2, BATCH_XLDS_FLOATS, BATCH_XLDS_TWO_PASS_COLS, Batch, Device, DeviceCSR, MLP,
+               N_FEATURES, matrix_features)
 
 K_LIST = (1, 4, 16, 32)
 PLOG2 = tuple(range(BATCH_MAX_PLOG2 + 1))
@@ -41,7 +42,9 @@ def item_nnz(k: int, t: int) -> int:
 
 
 def xlds_fits(n_cols, k: int):
-    return np.asarray(n_cols) * k <= BATCH_XLDS_FLOATS
+    """The X block can be staged in LDS: in one load, or at k = 1 in two halves."""
+    n_cols = np.asarray(n_cols)
+    return (n_cols * k <= BATCH_XLDS_FLOATS) | ((k == 1) & (n_cols <= BATCH_XLDS_TWO_PASS_COLS))
 
 
 def valid_mask(n_cols: int, k: int) -> np.ndarray:
@@ -51,7 +54,7 @@ def valid_mask(n_cols: int, k: int) -> np.ndarray:
 
 def heuristic_schedule(n_cols: int, k: int) -> int:
     """The batch's built-in schedule (plog2 2, X block in LDS at k = 1 when
-    it fits, default items), as an index into SCHEDULES."""
+    it fits in one or two loads, default items), as an index into SCHEDULES."""
     x = 1 if (k == 1 and bool(xlds_fits(n_cols, k))) else 0
     return SCHEDULES.index((2, x, 0))
 

@@ -64,6 +64,7 @@ def test_constants_match_python_mirror():
         key = "SPIRA_SPMV_" + name.upper().replace("STREAM1K", "STREAM1K")
         assert d[key] == v, key
     assert d["SPIRA_BATCH_XLDS_FLOATS"] == spira_amd.BATCH_XLDS_FLOATS
+    assert d["SPIRA_BATCH_XLDS_TWO_PASS_COLS"] == spira_amd.BATCH_XLDS_TWO_PASS_COLS
     assert d["SPIRA_BATCH_MAX_PLOG2"] == spira_amd.BATCH_MAX_PLOG2
 
 

@@ -85,7 +85,54 @@ def test_batch_spmm_matches_oracle(batch, k):
 
 
 def _xlds_ok(host, k):
-    return np.array([(len(off) - 1) * k <= spira_amd.BATCH_XLDS_FLOATS for off, _, _ in host])
+    n = np.array([len(off) - 1 for off, _, _ in host])
+    return (n * k <= spira_amd.BATCH_XLDS_FLOATS) | ((k == 1) & (n <= spira_amd.BATCH_XLDS_TWO_PASS_COLS))
+
+
+def test_k1_two_pass_x_blocks(dev):
+    """k = 1 matrices with 36865..65536 columns stage X in two halves: rows
+    whose entries straddle the halves, rows entirely in one half, empty rows,
+    long rows (whole-block tiles, read from global X) and small items."""
+    rng = np.random.default_rng(21)
+    host = []
+    for n in (40000, 65536):
+        lens = rng.integers(0, 60, size=n)
+        lens[::11] = 0
+        lens[5::97] = 3000  # whole-block rows
+        off = np.zeros(n + 1, np.uint64)
+        np.cumsum(lens, out=off[1:])
+        cols = []
+        for r, L in enumerate(lens):
+            if r % 3 == 0:  # only in the first half
+                cols.append(np.sort(rng.choice(n // 2, size=min(int(L), n // 2), replace=False)))
+            elif r % 3 == 1:  # only in the second half
+                cols.append(np.sort(rng.choice(n - n // 2, size=int(L), replace=False) + n // 2))
+            else:
+                cols.append(np.sort(rng.choice(n, size=int(L), replace=False)))
+        lens = np.array([c.size for c in cols])
+        np.cumsum(lens, out=off[1:])
+        col = np.concatenate(cols).astype(np.uint32)
+        val = rng.standard_normal(col.size).astype(np.float32).astype(np.float64)
+        host.append((off, col, val))
+    mats = []
+    for off, col, val in host:
+        A = spira_amd.DeviceCSR(dev, len(off) - 1, len(off) - 1)
+        A.upload(off, col, val)
+        mats.append(A)
+    B = spira_amd.Batch(dev, mats, "f32")
+    X = rng.standard_normal((B.n_cols, 1)).astype(np.float32)
+    ref = _reference(host, X, 1, B.col_base)
+    for plog2 in (0, 2, 5):
+        for item in (4096, 1 << 20):
+            B.set_schedule(1, plog2=plog2, xlds=1, item_nnz=item)
+            Yt = torch.full((B.n_rows, 1), float("nan"), dtype=torch.float32, device="cuda")
+            torch.cuda.synchronize()
+            B.spmm(torch.from_numpy(X).cuda(), Yt)
+            y = Yt.cpu().numpy().astype(np.float64)
+            assert np.isfinite(y).all(), (plog2, item)
+            assert oracle.normwise_rel_err(y.ravel(), ref.ravel()) <= 1e-6, (plog2, item)
+    with pytest.raises(spira_amd.InvalidArgument):  # two halves are a k = 1 mode only
+        B.set_schedule(4, plog2=2, xlds=1, item_nnz=4096)
 
 
 @pytest.mark.parametrize("k", [1, 4, 16, 32])
