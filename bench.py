@@ -242,7 +242,7 @@ def run_batch(args, np, torch, spira_amd, W, dev, stream):
         except ValueError as e:
             log(f"bench: selector model not used: {e}")
     t1 = time.time()
-    feats = {k: np.stack([S.features_for(A, k) for A in mats]) for k in S.K_LIST}
+    feats = S.features_by_k(mats)
     choice = {k: sel.predict(feats[k], [k] * len(mats), n_cols) for k in S.K_LIST} if sel is not None else {}
     t_select = time.time() - t1
     out = {"workload": f"{len(mats)} square matrices (seeds {seed0}..{seed0 + len(mats) - 1}), rows log-uniform in "

@@ -12,6 +12,9 @@
 //             coalesced, stages products in LDS, then reduces each row with a
 //             power-of-two team of lanes sized to the rows in the block.  Rows
 //             longer than C get a block of their own reduced by all 256 lanes.
+//   XSLICE<G> the columns cut into 8 slices, one per XCD (slice-major copy of
+//             A), G lanes per row; partial y per slice, summed in slice order.
+//   XSTREAM   the same slices, each run with the STREAM row-block kernel.
 //
 // Values and columns are read with non-temporal loads so the x gathers keep
 // L2 and the MALL.  Blocks are renumbered so that each XCD (8 on MI355X, which

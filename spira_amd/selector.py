@@ -65,6 +65,18 @@ def features_for(A: DeviceCSR, k: int) -> np.ndarray:
     return f
 
 
+def features_by_k(mats, ks=K_LIST) -> dict:
+    """{k: features of every matrix at k}: one device feature pass per matrix,
+    the k slot filled per k."""
+    base = np.stack([matrix_features(A) for A in mats])
+    out = {}
+    for k in ks:
+        f = base.copy()
+        f[:, N_FEATURES - 1] = math.log2(k) / 5.0
+        out[k] = f
+    return out
+
+
 def apply(batch: Batch, k: int, choice, n_cols):
     """Install per-matrix schedule indices into the batch."""
     choice = np.asarray(choice)
@@ -198,9 +210,10 @@ def collect(dev: Device, mats, reps: int = 2, log=None, ks=K_LIST):
     Y = torch.empty((B.n_rows * max(ks),), dtype=torch.float32, device="cuda")
     torch.cuda.synchronize()
     n_cols = np.array([A.n_cols for A in mats])
+    feats = features_by_k(mats, ks)
     F, T, KS, MI = [], [], [], []
     for k in ks:
-        F.append(np.stack([features_for(A, k) for A in mats]))
+        F.append(feats[k])
         T.append(exhaustive(B, k, n_cols, X, Y, reps))
         KS.append(np.full(len(mats), k))
         MI.append(np.arange(len(mats)))
