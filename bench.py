@@ -100,16 +100,48 @@ def run_lock(args, np, torch, spira_amd, W, dev, stream):
         t = time.perf_counter()
         L.lock(sp3, sc3, sv3)
         t_host.append(time.perf_counter() - t)
+    ref_csr = L.download()
+    # the same inserts as a device COO stream: append + sort by row + lock
+    st = spira_amd.Stage(dev, n3, n3)
+    with torch.cuda.stream(stream):
+        d_r = torch.from_numpy(r3.astype(np.int32)).to("cuda")
+        d_c = torch.from_numpy(ic3.astype(np.int32)).to("cuda")
+        d_v = torch.from_numpy(iv3).to("cuda")
+    stream.synchronize()
+    t_stg = []
+    for rep in range(6):
+        L.upload(o3, c3, v3)
+        dev.synchronize()
+        with torch.cuda.stream(stream):
+            t = time.perf_counter()
+            st.append(d_r, d_c, d_v, device=True)
+            L.lock_staged(st)
+            t_stg.append(time.perf_counter() - t)
+    staged_same = all(np.array_equal(a.view(np.uint8), b.view(np.uint8)) for a, b in zip(L.download(), ref_csr))
+    st.close()
     lbytes = W.lock_bytes(n3, int(c3.size), int(sc3.size), nnz_new)
     tdev = float(np.median(t_dev[1:]))
     detail = {"workload": "re-lock merge 1,048,576^2: 16 random nnz/row, then 2 inserts/row, 25% overwrite",
               "nnz_old": int(c3.size), "staged": int(sc3.size), "nnz_new": int(nnz_new),
               "bytes_per_lock": lbytes, "ms_device_inputs": round(1e3 * tdev, 3),
-              "GB_s": round(lbytes / tdev / 1e9, 1)}
+              "GB_s": round(lbytes / tdev / 1e9, 1),
+              "ms_device_coo_stream": round(1e3 * float(np.median(t_stg[1:])), 3),
+              "device_coo_stream_equals_segments": bool(staged_same)}
     if t_host:
         detail["ms_host_inputs_pcie_inclusive"] = round(1e3 * float(np.median(t_host[1:])), 3)
     L.close()
     return detail, (o3, c3, v3, sp3, sc3, sv3, n3, lbytes)
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def _time_calls(torch, stream, fn, reps):
@@ -516,8 +548,24 @@ def main():
                "sample": f"{reps} warm steps (both matrices, full size) in {t_cpu:.1f}s; oracle/ threaded "
                          f"parallel_matrix restatement, uniform row split, u64 offsets, scalar loop -O3 "
                          f"-march=x86-64-v3",
-               "ms_per_step": round(1e3 * t_cpu / reps, 3)}
+               "ms_per_step": round(1e3 * t_cpu / reps, 3),
+               "host": {"cpu_model": _cpu_model(), "nproc": os.cpu_count(), "affinity": cores}}
         for p in pools.values():
+            p.close()
+        # one thread, same matrices (a bounded sample of steps)
+        p1 = {name: oracle.Pool(1, info[name]["n_rows"], info[name]["n_rows"]) for name in names}
+        for name in names:
+            p1[name].load_csr(*mats[name])
+        reps1, t1 = 0, 0.0
+        tstart = time.perf_counter()
+        while t1 < args.cpu_seconds / 3 and reps1 < 50:
+            for name in names:
+                p1[name].spmv(info[name]["x_host"], ybuf[name])
+            reps1 += 1
+            t1 = time.perf_counter() - tstart
+        cpu["one_thread"] = {"value": round(cpu_bytes * reps1 / t1 / 1e9, 2), "unit": "GB/s",
+                             "ms_per_step": round(1e3 * t1 / reps1, 3), "steps": reps1}
+        for p in p1.values():
             p.close()
         if lock_detail is not None:
             o3, c3, v3, sp3, sc3, sv3, n3, lbytes = lock_src

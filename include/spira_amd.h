@@ -112,6 +112,28 @@ int spira_lock_f64(spira_dcsr *A, const uint64_t *stage_ptr, const uint32_t *sta
                    const double *stage_val, const uint8_t *dirty, uint32_t flags,
                    uint64_t *out_nnz);
 
+/* ---- open-mode staging on the device --------------------------------------
+ * The reference's open-mode inserts (matrix::insert, matrix.hpp:453-462;
+ * parallel_matrix::insert and its staging arrays, parallel_matrix.hpp:581-602,
+ * insert_staging.hpp:58-96) as one device COO append buffer kept in insertion
+ * order.  spira_stage_append copies n (row, col, value) triplets (host arrays,
+ * or device arrays with SPIRA_IN_DEVICE) to its end; if any row >= n_rows or
+ * col >= n_cols the whole append is rejected with SPIRA_ERR_OUT_OF_RANGE
+ * (matrix.hpp:161-171 messages).  spira_lock_staged_f64 is spira_lock_f64 on
+ * the buffer's contents: a stable radix sort by row on the device forms the
+ * per-row segments (each row's inserts stay in insertion order), then the same
+ * dedup + build/merge runs; the buffer is emptied on success.  dirty as in
+ * spira_lock_f64 (host array, or device array with SPIRA_IN_DEVICE).  Fewer
+ * than 2^32 staged inserts; n_rows < 2^32. */
+typedef struct spira_stage spira_stage;
+int spira_stage_create(spira_dev *dev, uint64_t n_rows, uint64_t n_cols, spira_stage **out);
+void spira_stage_destroy(spira_stage *s);
+int spira_stage_append(spira_stage *s, const uint32_t *rows, const uint32_t *cols, const double *vals, uint64_t n,
+                       uint32_t flags);
+int spira_stage_size(const spira_stage *s, uint64_t *n);
+int spira_stage_clear(spira_stage *s);
+int spira_lock_staged_f64(spira_dcsr *A, spira_stage *s, const uint8_t *dirty, uint32_t flags, uint64_t *out_nnz);
+
 /* ---- SpMV -----------------------------------------------------------------
  * serial::algorithms::spmv / parallel::algorithms::spmv for
  * <soa_tag, uint32_t, double> (serial/spmv.hpp:142-186,

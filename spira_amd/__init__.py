@@ -122,7 +122,8 @@ _shutdown = False
 @atexit.register
 def _release_all():
     global _shutdown
-    for B in list(globals().get("_live_batch", ())) + list(globals().get("_live_mlp", ())):
+    for B in (list(globals().get("_live_batch", ())) + list(globals().get("_live_mlp", ()))
+              + list(globals().get("_live_stage", ()))):
         B.close()
     for A in list(_live_csr):
         A.close()
@@ -167,6 +168,12 @@ def lib():
             "spira_spmv_choose": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
             "spira_spmv_set_variant": (ctypes.c_int, [_vp, ctypes.c_int]),
             "spira_partition_boundaries": (ctypes.c_int, [_vp, ctypes.c_uint64, _vp]),
+            "spira_stage_create": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(_vp)]),
+            "spira_stage_destroy": (None, [_vp]),
+            "spira_stage_append": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_uint64, ctypes.c_uint32]),
+            "spira_stage_size": (ctypes.c_int, [_vp, _u64p]),
+            "spira_stage_clear": (ctypes.c_int, [_vp]),
+            "spira_lock_staged_f64": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_uint32, _u64p]),
             "spira_spmv_autotune_f64": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
                                                        ctypes.POINTER(ctypes.c_int)]),
             "spira_spmv_mixed_f64": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_uint64, _vp, ctypes.c_uint64,
@@ -250,7 +257,7 @@ class Device:
 
     def close(self):
         if getattr(self, "_h", None) and not _shutdown:
-            for B in list(_live_batch) + list(_live_mlp):  # batches, nets, then matrices go first
+            for B in list(_live_batch) + list(_live_mlp) + list(_live_stage):  # these, then matrices, go first
                 if B.dev is self:
                     B.close()
             for A in list(_live_csr):
@@ -386,6 +393,19 @@ class DeviceCSR:
                                                   ctypes.byref(nrec) if count else None))
         return y, (int(nrec.value) if count else None)
 
+    def lock_staged(self, stage: "Stage", dirty=None, device: bool = False) -> int:
+        """lock() on a device Stage's inserts (sorted by row on the device);
+        the stage is emptied.  dirty: optional u8[n_rows] (host, or device
+        with device=True)."""
+        out = ctypes.c_uint64()
+        if dirty is not None and not device:
+            dirty = np.ascontiguousarray(dirty, dtype=np.uint8)
+            if dirty.size != self.n_rows:
+                raise InvalidArgument("lock: dirty must have n_rows entries")
+        self.dev.check(lib().spira_lock_staged_f64(self._h, stage.handle, _ptr(dirty), IN_DEVICE if device else 0,
+                                                   ctypes.byref(out)))
+        return int(out.value)
+
     def partition_boundaries(self, n_threads: int) -> np.ndarray:
         """compute_partition_boundaries (partition.hpp:78-126) on the device."""
         b = np.zeros(int(n_threads) + 1, np.uint64)
@@ -412,6 +432,70 @@ class DeviceCSR:
     def close(self):
         if getattr(self, "_h", None) and not _shutdown:
             lib().spira_dcsr_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_live_stage = weakref.WeakSet()
+
+
+class Stage:
+    """spira_stage: open-mode inserts as a device COO buffer in insertion order
+    (matrix::insert / parallel_matrix::insert staging), consumed by
+    DeviceCSR.lock_staged()."""
+
+    def __init__(self, dev: Device, n_rows: int, n_cols: int):
+        self.dev = dev
+        h = _vp()
+        dev.check(lib().spira_stage_create(dev.handle, int(n_rows), int(n_cols), ctypes.byref(h)))
+        self._h = h
+        self.n_rows, self.n_cols = int(n_rows), int(n_cols)
+        _live_stage.add(self)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def append(self, rows, cols, vals, device: bool = False):
+        """Append triplets in order (host arrays, or device tensors with
+        device=True: rows and cols int32/uint32, vals float64)."""
+        if device:
+            n = int(cols.numel())
+            if int(rows.numel()) != n or int(vals.numel()) != n:
+                raise InvalidArgument("stage: rows, cols and vals differ in length")
+            _producers_done(self.dev, rows, cols, vals)
+        else:
+            rows = np.asarray(rows)
+            cols = np.asarray(cols)
+            if rows.size and (rows.min() < 0 or rows.max() >= self.n_rows):
+                raise OutOfRange("spira::matrix: row_index out of range")
+            if cols.size and (cols.min() < 0 or cols.max() >= self.n_cols):
+                raise OutOfRange("spira::matrix: col_index out of range")
+            rows = np.ascontiguousarray(rows, dtype=np.uint32)
+            cols = np.ascontiguousarray(cols, dtype=np.uint32)
+            vals = np.ascontiguousarray(vals, dtype=np.float64)
+            n = int(cols.size)
+            if rows.size != n or vals.size != n:
+                raise InvalidArgument("stage: rows, cols and vals differ in length")
+        self.dev.check(lib().spira_stage_append(self._h, _ptr(rows), _ptr(cols), _ptr(vals), n,
+                                                IN_DEVICE if device else 0))
+
+    def __len__(self):
+        n = ctypes.c_uint64()
+        self.dev.check(lib().spira_stage_size(self._h, ctypes.byref(n)))
+        return int(n.value)
+
+    def clear(self):
+        self.dev.check(lib().spira_stage_clear(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None) and not _shutdown:
+            lib().spira_stage_destroy(self._h)
         self._h = None
 
     def __del__(self):

@@ -229,8 +229,9 @@ enum ws_slot : int {
     WS_BIG = 13,     // lock: rows with many staged inserts
     WS_CNT = 14,     // lock: new row lengths
     WS_PZ = 15,      // lock: prefix of dropped old zeros
-    WS_MIXQ = 16,    // mixed spmv: queue of rows to recompute in FP64 + its count
+    WS_MIXQ = 16,    // mixed spmv: flag byte per row for the FP64 recompute + its count
     WS_X32 = 17,     // mixed spmv: x rounded to FP32
+    WS_SORT = 18,    // device staging: radix sort temporary storage
 };
 
 }  // namespace spira_amd
