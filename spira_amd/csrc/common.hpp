@@ -98,6 +98,25 @@ __device__ __forceinline__ uint64_t row_of(const OffT *ptr, uint64_t lo, uint64_
     return lo;
 }
 
+// row_of with the 64 lanes of a wavefront probing together: each step narrows
+// [lo, hi) 64-fold, so a million rows cost 4 dependent loads instead of 20.
+// Every lane of the wavefront must be active and pass the same arguments, and
+// ptr[lo] <= k must hold.
+template <class OffT>
+__device__ __forceinline__ uint64_t wave_row_of(const OffT *ptr, uint64_t lo, uint64_t hi, uint64_t k)
+{
+    const uint64_t lane = threadIdx.x & (kWave - 1);
+    while (hi - lo > 1) {
+        const uint64_t step = (hi - lo + kWave - 1) / kWave;
+        const uint64_t p = lo + lane * step;
+        const bool ok = p < hi && (uint64_t)ptr[p] <= k;  // true on a prefix of the lanes, lane 0 included
+        const int cnt = __popcll(__ballot(ok));
+        lo += (uint64_t)(cnt - 1) * step;
+        hi = min(hi, lo + step);
+    }
+    return lo;
+}
+
 // First index in [lo, hi) with a[i] >= key (a strictly increasing).
 __device__ __forceinline__ uint64_t lower_bound_u32(const uint32_t *a, uint64_t lo, uint64_t hi, uint32_t key)
 {
@@ -232,6 +251,7 @@ enum ws_slot : int {
     WS_MIXQ = 16,    // mixed spmv: flag byte per row for the FP64 recompute + its count
     WS_X32 = 17,     // mixed spmv: x rounded to FP32
     WS_SORT = 18,    // device staging: radix sort temporary storage
+    WS_BLK = 19,     // lock: row tables of the element-parallel passes
 };
 
 }  // namespace spira_amd

@@ -33,6 +33,9 @@ constexpr int kThreads = 256;
 constexpr uint32_t kSmallSeg = 32;
 constexpr uint32_t kMaxSeg = SPIRA_LOCK_MAX_ROW_INSERTS;  // 8192, LDS bitonic limit
 constexpr int kItems = 4;                                   // elements per thread in scatter passes
+constexpr uint32_t kPerBlock = kThreads * kItems;           // elements per block of the staged-slot passes
+constexpr int kItemsOld = 8;                                // old entries per thread in the old-entry scatter
+constexpr uint32_t kPerBlockOld = kThreads * kItemsOld;
 
 enum err_bits : uint32_t {
     ERR_OUT_OF_RANGE = 1u,
@@ -45,6 +48,19 @@ __device__ __forceinline__ bool row_dirty(const uint8_t *dirty, const uint64_t *
 {
     if (dirty) return dirty[r] != 0;
     return sptr[r + 1] != sptr[r];
+}
+
+// Row table of an element-parallel pass whose block b covers the elements
+// [b*per, (b+1)*per) of a CSR-like pointer array: tab[b] = row owning element
+// b*per, tab[nb] = row owning the last element (nb = ceil(total / per)), so a
+// block reads its row range [tab[b], tab[b+1]] instead of searching for it.
+// Called once for every row r (elements [e0, e1)) by a row-parallel kernel.
+__device__ __forceinline__ void mark_block_rows(uint64_t r, uint64_t e0, uint64_t e1, uint64_t total, uint32_t per,
+                                                uint64_t *__restrict__ tab)
+{
+    if (e0 == e1) return;
+    for (uint64_t b = (e0 + per - 1) / per; b * per < e1; ++b) tab[b] = r;
+    if (e1 == total) tab[(total + per - 1) / per] = r;  // the last non-empty row owns the last element
 }
 
 // ---------------------------------------------------------------------------
@@ -88,12 +104,18 @@ __global__ __launch_bounds__(kThreads) void k_dedup_small(uint64_t n, const uint
                                                            const uint32_t *__restrict__ scol,
                                                            const double *__restrict__ sval,
                                                            uint32_t *__restrict__ dcol, double *__restrict__ dval,
-                                                           uint32_t *__restrict__ dcnt)
+                                                           uint32_t *__restrict__ dcnt, uint64_t *__restrict__ stab,
+                                                           const uint64_t *__restrict__ old_off, uint64_t nnz_old,
+                                                           uint64_t *__restrict__ otab)
 {
     const uint64_t r = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
     if (r >= n) return;
     const uint64_t s0 = sptr[r];
-    const uint32_t k = (uint32_t)(sptr[r + 1] - s0);
+    const uint64_t s1 = sptr[r + 1];
+    // row tables of the later element-parallel passes (staged slots; old entries)
+    mark_block_rows(r, s0, s1, sptr[n], kPerBlock, stab);
+    if (otab) mark_block_rows(r, old_off[r], old_off[r + 1], nnz_old, kPerBlockOld, otab);
+    const uint32_t k = (uint32_t)(s1 - s0);
     if (k > kSmallSeg) return;  // handled by the block kernel
     const uint32_t *c = scol + s0;
     uint32_t keep = 0;
@@ -205,12 +227,14 @@ struct row_range {
     uint64_t lo, hi;  // rows [lo, hi] own the block's elements
 };
 
+// Called by every thread of the block (whole wavefronts): each wavefront runs
+// the two searches 64 lanes wide.
 __device__ __forceinline__ row_range block_rows(const uint64_t *ptr, uint64_t n_rows, uint64_t first,
                                                 uint64_t last)
 {
     row_range rr;
-    rr.lo = row_of(ptr, 0, n_rows, first);
-    rr.hi = row_of(ptr, rr.lo, n_rows, last);
+    rr.lo = wave_row_of(ptr, 0, n_rows, first);
+    rr.hi = wave_row_of(ptr, rr.lo, n_rows, last);
     return rr;
 }
 
@@ -238,11 +262,9 @@ struct row_cache {
     __device__ __forceinline__ uint64_t at(uint64_t r) const { return cached ? s[r - lo] : g[r]; }  // r in [lo, hi+1]
 };
 
-__device__ __forceinline__ row_cache make_row_cache(const uint64_t *ptr, uint64_t n_rows, uint64_t first,
-                                                    uint64_t last, uint64_t *s_buf)
+__device__ __forceinline__ row_cache make_row_cache(const row_range rr, const uint64_t *ptr, uint64_t *s_buf)
 {
     row_cache rc;
-    const row_range rr = block_rows(ptr, n_rows, first, last);
     rc.lo = rr.lo;
     rc.hi = rr.hi;
     rc.g = ptr;
@@ -252,6 +274,12 @@ __device__ __forceinline__ row_cache make_row_cache(const uint64_t *ptr, uint64_
         for (uint64_t j = threadIdx.x; j <= rr.hi - rr.lo + 1; j += blockDim.x) s_buf[j] = ptr[rr.lo + j];
     __syncthreads();
     return rc;
+}
+
+// The row range of this block from a table built by mark_block_rows.
+__device__ __forceinline__ row_range table_rows(const uint64_t *__restrict__ tab)
+{
+    return row_range{tab[blockIdx.x], tab[blockIdx.x + 1]};
 }
 
 // ---------------------------------------------------------------------------
@@ -264,13 +292,22 @@ __global__ __launch_bounds__(kThreads) void k_ded_flags(uint64_t n, uint64_t m, 
                                                          const uint8_t *__restrict__ dirty, int merge,
                                                          const uint64_t *__restrict__ old_off,
                                                          const uint32_t *__restrict__ old_col,
-                                                         uint64_t *__restrict__ flags, uint32_t *__restrict__ qpos)
+                                                         uint64_t *__restrict__ flags, uint32_t *__restrict__ qpos,
+                                                         const uint64_t *__restrict__ stab)
 {
     __shared__ uint64_t s_rows[kRowCache + 1];
-    const uint64_t first = (uint64_t)blockIdx.x * kThreads * kItems;
+    const uint64_t first = (uint64_t)blockIdx.x * kPerBlock;
     if (first >= m) return;
-    const uint64_t last = min(first + kThreads * kItems, m) - 1;
-    const row_cache rc = make_row_cache(sptr, n, first, last, s_rows);
+    // slot data first: these loads are in flight while the row cache fills
+    uint32_t cs[kItems];
+    double vs[kItems];
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+        const uint64_t s = first + (uint64_t)it * kThreads + threadIdx.x;
+        cs[it] = s < m ? dcol[s] : 0u;
+        vs[it] = s < m ? dval[s] : 0.0;
+    }
+    const row_cache rc = make_row_cache(table_rows(stab), sptr, s_rows);
 #pragma unroll
     for (int it = 0; it < kItems; ++it) {
         const uint64_t s = first + (uint64_t)it * kThreads + threadIdx.x;
@@ -281,8 +318,8 @@ __global__ __launch_bounds__(kThreads) void k_ded_flags(uint64_t n, uint64_t m, 
         uint32_t q = 0;
         if (j < dcnt[r]) {
             const bool d = merge ? row_dirty(dirty, sptr, r) : true;
-            const uint32_t c = dcol[s];
-            const bool nz = d && !is_zero_f64(dval[s]);
+            const uint32_t c = cs[it];
+            const bool nz = d && !is_zero_f64(vs[it]);
             bool match = false;
             if (merge && d) {
                 const uint64_t ob = old_off[r], oe = old_off[r + 1];
@@ -313,7 +350,7 @@ __global__ __launch_bounds__(kThreads) void k_old_zero_flags(uint64_t n, uint64_
     const uint64_t first = (uint64_t)blockIdx.x * kThreads * kItems;
     if (first >= nnz_old) return;
     const uint64_t last = min(first + kThreads * kItems, nnz_old) - 1;
-    const row_cache rc = make_row_cache(old_off, n, first, last, s_rows);
+    const row_cache rc = make_row_cache(block_rows(old_off, n, first, last), old_off, s_rows);
 #pragma unroll
     for (int it = 0; it < kItems; ++it) {
         const uint64_t k = first + (uint64_t)it * kThreads + threadIdx.x;
@@ -359,6 +396,10 @@ __global__ void k_row_counts(uint64_t n, const uint64_t *__restrict__ sptr, cons
 // ---------------------------------------------------------------------------
 // 5. scatter passes
 // ---------------------------------------------------------------------------
+constexpr uint32_t kSoRows = 512;   // rows of one old-entry scatter block staged in LDS
+constexpr uint32_t kSoIns = 1024;   // deduped inserts of those rows staged in LDS (< 2^16: 16-bit counts)
+constexpr uint32_t kCleanRow = 0xffffffffu;
+
 __global__ __launch_bounds__(kThreads) void k_scatter_old(uint64_t n, uint64_t nnz_old,
                                                            const uint64_t *__restrict__ old_off,
                                                            const uint32_t *__restrict__ old_col,
@@ -371,85 +412,119 @@ __global__ __launch_bounds__(kThreads) void k_scatter_old(uint64_t n, uint64_t n
                                                            const uint8_t *__restrict__ dirty,
                                                            const uint64_t *__restrict__ new_off,
                                                            uint32_t *__restrict__ new_col, double *__restrict__ new_val,
-                                                           uint32_t *__restrict__ err)
+                                                           uint32_t *__restrict__ err, const uint64_t *__restrict__ otab)
 {
-    __shared__ uint64_t s_rows[kRowCache + 1];
-    const uint64_t first = (uint64_t)blockIdx.x * kThreads * kItems;
+    // The block's rows (old_off, new_off, dirty bit, deduped-insert range) and
+    // their deduped inserts (columns, prefix counts relative to the first) are
+    // loaded into LDS once, in parallel; every entry then resolves its slot
+    // from LDS alone.  Blocks whose rows or inserts do not fit read the same
+    // quantities from global memory per entry.
+    __shared__ uint64_t s_oo[kSoRows + 1];  // old_off[lo .. hi+1]
+    __shared__ uint64_t s_no[kSoRows + 1];  // new_off[lo .. hi+1]
+    __shared__ uint32_t s_db[kSoRows];      // sptr[r] - ibase, or kCleanRow
+    __shared__ uint16_t s_dn[kSoRows];      // dcnt[r]
+    __shared__ uint32_t s_dcol[kSoIns];     // dcol[ibase ..)
+    __shared__ uint32_t s_P[kSoIns + 1];    // (matches << 16 | nonzeros) of P[ibase + t] - P[ibase]
+    const uint64_t first = (uint64_t)blockIdx.x * kPerBlockOld;
     if (first >= nnz_old) return;
-    const uint64_t last = min(first + kThreads * kItems, nnz_old) - 1;
-    const row_cache rc = make_row_cache(old_off, n, first, last, s_rows);
-    // Each thread owns kItems consecutive entries: one row search, then a walk.
-    const uint64_t k0 = first + (uint64_t)threadIdx.x * kItems;
-    if (k0 >= nnz_old) return;
-    uint32_t c[kItems];
-    double v[kItems];
-    if (k0 + kItems <= nnz_old) {
-        static_assert(kItems == 4, "vector loads assume 4 entries per thread");
-        const u32x4 c4 = ld_stream(reinterpret_cast<const u32x4 *>(old_col + k0));
-        const f64x2 v01 = ld_stream(reinterpret_cast<const f64x2 *>(old_val + k0));
-        const f64x2 v23 = ld_stream(reinterpret_cast<const f64x2 *>(old_val + k0 + 2));
-        c[0] = c4[0];
-        c[1] = c4[1];
-        c[2] = c4[2];
-        c[3] = c4[3];
-        v[0] = v01[0];
-        v[1] = v01[1];
-        v[2] = v23[0];
-        v[3] = v23[1];
-    } else {
+    const uint64_t last = min(first + kPerBlockOld, nnz_old) - 1;
+    // Entry it*kThreads + tid of the block: loads and stores stay coalesced.
+    // The entries are loaded first and are in flight while the rows stage.
+    uint32_t cs[kItemsOld];
+    double vs[kItemsOld];
 #pragma unroll
-        for (int it = 0; it < kItems; ++it) {
-            c[it] = k0 + it < nnz_old ? old_col[k0 + it] : 0u;
-            v[it] = k0 + it < nnz_old ? old_val[k0 + it] : 0.0;
+    for (int it = 0; it < kItemsOld; ++it) {
+        const uint64_t k = first + (uint64_t)it * kThreads + threadIdx.x;
+        cs[it] = k <= last ? ld_stream(old_col + k) : 0u;
+        vs[it] = k <= last ? ld_stream(old_val + k) : 0.0;
+    }
+    const row_range rr = table_rows(otab);
+    const uint64_t nr = rr.hi - rr.lo + 1;
+    const uint64_t ibase = sptr[rr.lo];
+    const uint64_t ni = sptr[rr.hi] + dcnt[rr.hi] - ibase;  // deduped inserts of rows lo..hi lie in [ibase, ibase+ni)
+    const bool staged = nr <= kSoRows && ni <= kSoIns;
+    if (staged) {
+        for (uint32_t j = threadIdx.x; j <= nr; j += kThreads) {
+            const uint64_t r = rr.lo + j;
+            s_oo[j] = old_off[r];
+            s_no[j] = new_off[r];
+            if (j < nr) {
+                s_db[j] = row_dirty(dirty, sptr, r) ? (uint32_t)(sptr[r] - ibase) : kCleanRow;
+                s_dn[j] = (uint16_t)dcnt[r];
+            }
+        }
+        const uint64_t p0 = P[ibase];
+        for (uint32_t t = threadIdx.x; t <= ni; t += kThreads) {
+            const uint64_t pt = P[ibase + t];
+            const uint32_t dm = (uint32_t)(pt >> 32) - (uint32_t)(p0 >> 32);
+            const uint32_t dz = (uint32_t)pt - (uint32_t)p0;
+            s_P[t] = (dm << 16) | dz;
+            if (t < ni) s_dcol[t] = dcol[ibase + t];
         }
     }
-    uint64_t r = rc.find(k0);
-    uint64_t ob = rc.at(r), oe = rc.at(r + 1);
-    // per-row state, refreshed when the walk enters a new row
-    uint64_t no = 0, nlen = 0, db = 0, de = 0, pa = 0;
-    bool dirty_r = false, have = false;
+    __syncthreads();
 #pragma unroll
-    for (int it = 0; it < kItems; ++it) {
-        const uint64_t k = k0 + it;
-        if (k >= nnz_old) break;
-        while (k >= oe) {  // skip to the row owning k (empty rows included)
-            ++r;
-            ob = oe;
-            oe = rc.at(r + 1);
-            have = false;
-        }
-        if (!have) {
+    for (int it = 0; it < kItemsOld; ++it) {
+        const uint64_t k = first + (uint64_t)it * kThreads + threadIdx.x;
+        if (k > last) break;
+        const uint32_t c = cs[it];
+        const double v = vs[it];
+        uint64_t no, nlen, out;
+        if (staged) {
+            uint32_t a = 0, b = (uint32_t)nr;  // largest a < nr with s_oo[a] <= k
+            while (b - a > 1) {
+                const uint32_t mid = (a + b) >> 1;
+                if (s_oo[mid] <= k) a = mid;
+                else b = mid;
+            }
+            const uint64_t ob = s_oo[a];
+            no = s_no[a];
+            nlen = s_no[a + 1] - no;
+            const uint32_t db = s_db[a];
+            if (db == kCleanRow) {
+                out = no + (k - ob);  // clean row: verbatim (csr_build.hpp:362-377)
+            } else {
+                const uint32_t de = db + s_dn[a];
+                uint32_t p = db, q = de;  // lower bound of c in s_dcol[db, de)
+                while (p < q) {
+                    const uint32_t mid = (p + q) >> 1;
+                    if (s_dcol[mid] < c) p = mid + 1;
+                    else q = mid;
+                }
+                if (p < de && s_dcol[p] == c) continue;  // buffer wins (:430-436)
+                if (PZ && is_zero_f64(v)) continue;      // old zero dropped (:420-421)
+                const uint32_t pa = s_P[db], pp = s_P[p];
+                const uint64_t before_m = (pp >> 16) - (pa >> 16);
+                const uint64_t before_nz = (pp & 0xffffu) - (pa & 0xffffu);
+                const uint64_t before_z = PZ ? (PZ[k] - PZ[ob]) : 0;
+                out = no + (k - ob) - before_m - before_z + before_nz;
+            }
+        } else {
+            const uint64_t r = row_of(old_off, rr.lo, rr.hi + 1, k);
+            const uint64_t ob = old_off[r];
             no = new_off[r];
             nlen = new_off[r + 1] - no;
-            dirty_r = row_dirty(dirty, sptr, r);
-            if (dirty_r) {
-                db = sptr[r];
-                de = db + dcnt[r];
-                pa = P[db];
+            if (!row_dirty(dirty, sptr, r)) {
+                out = no + (k - ob);
+            } else {
+                const uint64_t db = sptr[r], de = db + dcnt[r];
+                const uint64_t p = lower_bound_u32(dcol, db, de, c);
+                if (p < de && dcol[p] == c) continue;
+                if (PZ && is_zero_f64(v)) continue;
+                const uint64_t pa = P[db], pp = P[p];
+                const uint64_t before_m = (pp >> 32) - (pa >> 32);
+                const uint64_t before_nz = (pp & 0xffffffffull) - (pa & 0xffffffffull);
+                const uint64_t before_z = PZ ? (PZ[k] - PZ[ob]) : 0;
+                out = no + (k - ob) - before_m - before_z + before_nz;
             }
-            have = true;
-        }
-        const uint64_t i = k - ob;
-        uint64_t out;
-        if (!dirty_r) {
-            out = no + i;  // clean row: verbatim (csr_build.hpp:362-377)
-        } else {
-            const uint64_t p = de > db ? lower_bound_u32(dcol, db, de, c[it]) : db;
-            if (p < de && dcol[p] == c[it]) continue;  // buffer wins (:430-436)
-            if (PZ && is_zero_f64(v[it])) continue;    // old zero dropped (:420-421)
-            const uint64_t pp = p == db ? pa : P[p];
-            const uint64_t before_m = (pp >> 32) - (pa >> 32);
-            const uint64_t before_nz = (pp & 0xffffffffull) - (pa & 0xffffffffull);
-            const uint64_t before_z = PZ ? (PZ[k] - PZ[ob]) : 0;
-            out = no + i - before_m - before_z + before_nz;
         }
         // Every slot must fall inside the row's new range; never write outside.
         if (out - no >= nlen) {
             atomicOr(err, ERR_INTERNAL);
             continue;
         }
-        new_col[out] = c[it];
-        new_val[out] = v[it];
+        new_col[out] = c;
+        new_val[out] = v;
     }
 }
 
@@ -463,19 +538,26 @@ __global__ __launch_bounds__(kThreads) void k_scatter_ded(uint64_t n, uint64_t m
                                                            const uint64_t *__restrict__ PZ, int merge,
                                                            const uint64_t *__restrict__ new_off,
                                                            uint32_t *__restrict__ new_col, double *__restrict__ new_val,
-                                                           uint32_t *__restrict__ err)
+                                                           uint32_t *__restrict__ err, const uint64_t *__restrict__ stab)
 {
     __shared__ uint64_t s_rows[kRowCache + 1];
-    const uint64_t first = (uint64_t)blockIdx.x * kThreads * kItems;
+    const uint64_t first = (uint64_t)blockIdx.x * kPerBlock;
     if (first >= m) return;
-    const uint64_t last = min(first + kThreads * kItems, m) - 1;
-    const row_cache rc = make_row_cache(sptr, n, first, last, s_rows);
+    // prefix words first: these loads are in flight while the row cache fills
+    uint64_t pw[kItems], pw1[kItems];
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+        const uint64_t s = first + (uint64_t)it * kThreads + threadIdx.x;
+        pw[it] = s < m ? P[s] : 0;
+        pw1[it] = s < m ? P[s + 1] : 0;  // P holds m + 1 words
+    }
+    const row_cache rc = make_row_cache(table_rows(stab), sptr, s_rows);
 #pragma unroll
     for (int it = 0; it < kItems; ++it) {
         const uint64_t s = first + (uint64_t)it * kThreads + threadIdx.x;
         if (s >= m) break;
-        const uint64_t ps = P[s];
-        const uint64_t ps1 = P[s + 1];
+        const uint64_t ps = pw[it];
+        const uint64_t ps1 = pw1[it];
         if (((ps1 - ps) & 0xffffffffull) == 0) continue;  // not a surviving entry
         const uint64_t r = rc.find(s);
         const uint64_t db = rc.at(r);
@@ -519,22 +601,34 @@ __global__ __launch_bounds__(kThreads) void k_validate_entries(uint64_t n, uint6
                                                                 unsigned long long *__restrict__ first_bad,
                                                                 uint32_t *__restrict__ zero_seen)
 {
+    __shared__ uint64_t s_rows[kRowCache + 1];
     const uint64_t first = (uint64_t)blockIdx.x * kThreads * kItems;
     if (first >= nnz) return;
     const uint64_t last = min(first + kThreads * kItems, nnz) - 1;
-    const row_range rr = block_rows(off, n, first, last);
+    // Entry data first (in flight during the row search).  Only an entry that
+    // is out of range or not above its predecessor needs its row: to tell a
+    // row start from an order violation, and for the failure key.
+    uint32_t cur[kItems], prev[kItems];
     bool z = false;
 #pragma unroll
     for (int it = 0; it < kItems; ++it) {
         const uint64_t k = first + (uint64_t)it * kThreads + threadIdx.x;
+        cur[it] = k < nnz ? col[k] : 0u;
+        prev[it] = k < nnz && k > 0 ? col[k - 1] : 0u;
+        z |= k < nnz && is_zero_f64(val[k]);
+    }
+    const row_cache rc = make_row_cache(block_rows(off, n, first, last), off, s_rows);
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+        const uint64_t k = first + (uint64_t)it * kThreads + threadIdx.x;
         if (k >= nnz) break;
-        const uint64_t r = row_of(off, rr.lo, rr.hi + 1, k);
-        const uint64_t t = k - off[r];
+        const bool oob = (uint64_t)cur[it] >= n_cols;
+        if (!oob && (k == 0 || cur[it] > prev[it])) continue;
+        const uint64_t r = rc.find(k);
+        const uint64_t t = k - rc.at(r);
         const uint64_t tt = t < 0x7fffffffull ? t : 0x7fffffffull;
-        if ((uint64_t)col[k] >= n_cols) atomicMin(first_bad, (unsigned long long)((r << 32) | (1 + 2 * tt)));
-        else if (t > 0 && col[k] <= col[k - 1])
-            atomicMin(first_bad, (unsigned long long)((r << 32) | (2 + 2 * tt)));
-        z |= is_zero_f64(val[k]);
+        if (oob) atomicMin(first_bad, (unsigned long long)((r << 32) | (1 + 2 * tt)));
+        else if (t > 0) atomicMin(first_bad, (unsigned long long)((r << 32) | (2 + 2 * tt)));
     }
     if (z) atomicOr(zero_seen, 1u);
 }
@@ -618,8 +712,16 @@ uint64_t lock_device_f64(spira_dcsr *A, const uint64_t *sptr, const uint32_t *sc
     uint32_t *dcol = (uint32_t *)dev->ws.get(WS_DED_COL, m * sizeof(uint32_t));
     double *dval = (double *)dev->ws.get(WS_DED_VAL, m * sizeof(double));
     uint32_t *dcnt = (uint32_t *)dev->ws.get(WS_ROWS, (n + 1) * sizeof(uint32_t) + 8);
+    // Row tables (mark_block_rows) of the staged-slot passes and of the
+    // old-entry scatter, filled by the dedup pass that reads both pointers.
+    const bool scatter_old = merge && A->nnz > 0;
+    const uint64_t nb_s = (m + kPerBlock - 1) / kPerBlock + 1;
+    const uint64_t nb_o = scatter_old ? (A->nnz + kPerBlockOld - 1) / kPerBlockOld + 1 : 0;
+    uint64_t *stab = (uint64_t *)dev->ws.get(WS_BLK, (nb_s + nb_o) * sizeof(uint64_t));
+    uint64_t *otab = scatter_old ? stab + nb_s : nullptr;
     if (n > 0) {
-        k_dedup_small<<<grid_for(n, kThreads), kThreads, 0, st>>>(n, sptr, scol, sval, dcol, dval, dcnt);
+        k_dedup_small<<<grid_for(n, kThreads), kThreads, 0, st>>>(n, sptr, scol, sval, dcol, dval, dcnt, stab,
+                                                                   A->off, scatter_old ? A->nnz : 0, otab);
         SPIRA_HIP(hipGetLastError());
     }
     if (n_big > 0) {
@@ -633,8 +735,8 @@ uint64_t lock_device_f64(spira_dcsr *A, const uint64_t *sptr, const uint32_t *sc
     uint64_t *P = (uint64_t *)dev->ws.get(WS_FLAGS, (m + 1) * sizeof(uint64_t));
     uint32_t *qpos = merge ? (uint32_t *)dev->ws.get(WS_QPOS, m * sizeof(uint32_t)) : nullptr;
     if (m > 0) {
-        k_ded_flags<<<grid_for(m, kThreads * kItems), kThreads, 0, st>>>(n, m, sptr, dcnt, dcol, dval, dirty,
-                                                                         merge ? 1 : 0, A->off, A->col, P, qpos);
+        k_ded_flags<<<grid_for(m, kPerBlock), kThreads, 0, st>>>(n, m, sptr, dcnt, dcol, dval, dirty, merge ? 1 : 0,
+                                                                 A->off, A->col, P, qpos, stab);
         SPIRA_HIP(hipGetLastError());
     }
     scan_excl_u64(dev, P, P, m);
@@ -671,14 +773,16 @@ uint64_t lock_device_f64(spira_dcsr *A, const uint64_t *sptr, const uint32_t *sc
         scan_excl_u64(dev, cnt, new_off, n);
 
         // ---- 5. scatter ------------------------------------------------------
-        if (merge && A->nnz > 0) {
-            k_scatter_old<<<grid_for(A->nnz, kThreads * kItems), kThreads, 0, st>>>(
-                n, A->nnz, A->off, A->col, A->val, sptr, dcnt, dcol, P, PZ, dirty, new_off, new_col, new_val, ctl);
+        if (scatter_old) {
+            k_scatter_old<<<grid_for(A->nnz, kPerBlockOld), kThreads, 0, st>>>(
+                n, A->nnz, A->off, A->col, A->val, sptr, dcnt, dcol, P, PZ, dirty, new_off, new_col, new_val, ctl,
+                otab);
             SPIRA_HIP(hipGetLastError());
         }
         if (m > 0) {
-            k_scatter_ded<<<grid_for(m, kThreads * kItems), kThreads, 0, st>>>(
-                n, m, sptr, dcnt, dcol, dval, P, qpos, A->off, PZ, merge ? 1 : 0, new_off, new_col, new_val, ctl);
+            k_scatter_ded<<<grid_for(m, kPerBlock), kThreads, 0, st>>>(n, m, sptr, dcnt, dcol, dval, P, qpos, A->off,
+                                                                       PZ, merge ? 1 : 0, new_off, new_col, new_val,
+                                                                       ctl, stab);
             SPIRA_HIP(hipGetLastError());
         }
         uint32_t herr = 0;
