@@ -552,17 +552,33 @@ class Batch:
         self.dev.check(lib().spira_batch_set_schedule(self._h, int(k), _ptr(s)))
         self.k = int(k)
 
+    def _check_xy(self, X, Y):
+        """The C entry points take bare device pointers to X (n_cols x k) and
+        Y (n_rows x k) FP32 values, row-major, so their sizes are checked here.
+        Before a schedule is set (k == 0) the library reports the mode error."""
+        if self.k == 0:
+            return
+        import torch
+        for name, T, rows in (("X", X, self.n_cols), ("Y", Y, self.n_rows)):
+            if not (isinstance(T, torch.Tensor) and T.is_cuda and T.dtype == torch.float32 and T.is_contiguous()):
+                raise InvalidArgument(f"batch: {name} must be a contiguous float32 device tensor")
+            if T.numel() < rows * self.k:
+                raise InvalidArgument(f"batch: {name} holds {T.numel()} values, needs {rows} x k = {rows * self.k}")
+
     def spmm(self, X, Y, sync: bool = True):
+        self._check_xy(X, Y)
         self.dev.check(lib().spira_batch_spmm(self._h, _ptr(X), _ptr(Y), 0 if sync else NO_SYNC))
         return Y
 
     def profile(self, X, Y) -> np.ndarray:
         """One profiled spmm: per-matrix cost in CU-microseconds."""
+        self._check_xy(X, Y)
         out = np.zeros(max(1, len(self.mats)), np.float64)
         self.dev.check(lib().spira_batch_profile(self._h, _ptr(X), _ptr(Y), _ptr(out)))
         return out[:len(self.mats)]
 
     def capture(self, X, Y):
+        self._check_xy(X, Y)
         self.dev.check(lib().spira_batch_graph_capture(self._h, _ptr(X), _ptr(Y)))
 
     def replay(self, sync: bool = True):

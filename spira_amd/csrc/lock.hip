@@ -17,9 +17,12 @@
 //                 explicit zeros); exclusive scan -> new 64-bit offsets.
 //   5. scatter    every old entry and every surviving new entry computes its
 //                 own output slot from the prefix sums (merge by ranking), so
-//                 the work is element-parallel and independent of row lengths.
-// The output arrays are exactly sized (capacity == nnz), like build_csr's
-// csr_storage(n_rows, total_nnz).
+//                 the work is element-parallel and independent of row lengths;
+//                 a re-lock does both in one pass over "row r's old entries,
+//                 then its staged slots" so each row is written by one block.
+// The host waits once, at the end, for the error bits and the new nnz; a
+// failed check makes every later kernel return at once.  The new arrays hold
+// nnz_old + m entries (merge_csr's total_ub); the matrix reports nnz.
 #include "internal.hpp"
 
 #include <algorithm>
@@ -34,8 +37,8 @@ constexpr uint32_t kSmallSeg = 32;
 constexpr uint32_t kMaxSeg = SPIRA_LOCK_MAX_ROW_INSERTS;  // 8192, LDS bitonic limit
 constexpr int kItems = 4;                                   // elements per thread in scatter passes
 constexpr uint32_t kPerBlock = kThreads * kItems;           // elements per block of the staged-slot passes
-constexpr int kItemsOld = 8;                                // old entries per thread in the old-entry scatter
-constexpr uint32_t kPerBlockOld = kThreads * kItemsOld;
+constexpr int kItemsMerge = 8;                              // positions per thread in the re-lock scatter
+constexpr uint32_t kPerBlockMerge = kThreads * kItemsMerge;
 
 enum err_bits : uint32_t {
     ERR_OUT_OF_RANGE = 1u,
@@ -105,16 +108,18 @@ __global__ __launch_bounds__(kThreads) void k_dedup_small(uint64_t n, const uint
                                                            const double *__restrict__ sval,
                                                            uint32_t *__restrict__ dcol, double *__restrict__ dval,
                                                            uint32_t *__restrict__ dcnt, uint64_t *__restrict__ stab,
-                                                           const uint64_t *__restrict__ old_off, uint64_t nnz_old,
-                                                           uint64_t *__restrict__ otab)
+                                                           const uint64_t *__restrict__ old_off,
+                                                           uint64_t *__restrict__ mtab, const uint32_t *__restrict__ ctl)
 {
+    if (ctl[0]) return;  // the checks failed: nothing below may trust the staged arrays
     const uint64_t r = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
     if (r >= n) return;
     const uint64_t s0 = sptr[r];
     const uint64_t s1 = sptr[r + 1];
-    // row tables of the later element-parallel passes (staged slots; old entries)
+    // row tables of the later element-parallel passes: staged slots, and the
+    // re-lock scatter's combined index space (old entries, then staged slots)
     mark_block_rows(r, s0, s1, sptr[n], kPerBlock, stab);
-    if (otab) mark_block_rows(r, old_off[r], old_off[r + 1], nnz_old, kPerBlockOld, otab);
+    if (mtab) mark_block_rows(r, old_off[r] + s0, old_off[r + 1] + s1, old_off[n] + sptr[n], kPerBlockMerge, mtab);
     const uint32_t k = (uint32_t)(s1 - s0);
     if (k > kSmallSeg) return;  // handled by the block kernel
     const uint32_t *c = scol + s0;
@@ -164,7 +169,7 @@ __device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t *to
 // the order of the reference's stable sort; the last key of each column run is
 // the last write.
 __global__ __launch_bounds__(kThreads) void k_dedup_big(const uint32_t *__restrict__ rows,
-                                                         const uint32_t *__restrict__ n_rows_list,
+                                                         const uint32_t *__restrict__ ctl,  // [error bits, rows listed]
                                                          const uint64_t *__restrict__ sptr,
                                                          const uint32_t *__restrict__ scol,
                                                          const double *__restrict__ sval,
@@ -172,7 +177,8 @@ __global__ __launch_bounds__(kThreads) void k_dedup_big(const uint32_t *__restri
                                                          uint32_t *__restrict__ dcnt)
 {
     extern __shared__ uint64_t s_key[];  // kMaxSeg entries
-    const uint32_t count = *n_rows_list;
+    if (ctl[0]) return;
+    const uint32_t count = ctl[1];
     for (uint32_t li = blockIdx.x; li < count; li += gridDim.x) {
         const uint32_t r = rows[li];
         const uint64_t s0 = sptr[r];
@@ -293,11 +299,12 @@ __global__ __launch_bounds__(kThreads) void k_ded_flags(uint64_t n, uint64_t m, 
                                                          const uint64_t *__restrict__ old_off,
                                                          const uint32_t *__restrict__ old_col,
                                                          uint64_t *__restrict__ flags, uint32_t *__restrict__ qpos,
-                                                         const uint64_t *__restrict__ stab)
+                                                         const uint64_t *__restrict__ stab,
+                                                         const uint32_t *__restrict__ ctl)
 {
     __shared__ uint64_t s_rows[kRowCache + 1];
     const uint64_t first = (uint64_t)blockIdx.x * kPerBlock;
-    if (first >= m) return;
+    if (first >= m || ctl[0]) return;
     // slot data first: these loads are in flight while the row cache fills
     uint32_t cs[kItems];
     double vs[kItems];
@@ -344,11 +351,12 @@ __global__ __launch_bounds__(kThreads) void k_old_zero_flags(uint64_t n, uint64_
                                                               const uint32_t *__restrict__ dcnt,
                                                               const uint32_t *__restrict__ dcol,
                                                               const uint8_t *__restrict__ dirty,
-                                                              uint64_t *__restrict__ zf)
+                                                              uint64_t *__restrict__ zf,
+                                                              const uint32_t *__restrict__ ctl)
 {
     __shared__ uint64_t s_rows[kRowCache + 1];
     const uint64_t first = (uint64_t)blockIdx.x * kThreads * kItems;
-    if (first >= nnz_old) return;
+    if (first >= nnz_old || ctl[0]) return;
     const uint64_t last = min(first + kThreads * kItems, nnz_old) - 1;
     const row_cache rc = make_row_cache(block_rows(old_off, n, first, last), old_off, s_rows);
 #pragma unroll
@@ -373,10 +381,11 @@ __global__ __launch_bounds__(kThreads) void k_old_zero_flags(uint64_t n, uint64_
 // ---------------------------------------------------------------------------
 __global__ void k_row_counts(uint64_t n, const uint64_t *__restrict__ sptr, const uint64_t *__restrict__ P,
                              const uint8_t *__restrict__ dirty, int merge, const uint64_t *__restrict__ old_off,
-                             const uint64_t *__restrict__ PZ, uint64_t *__restrict__ cnt)
+                             const uint64_t *__restrict__ PZ, uint64_t *__restrict__ cnt,
+                             const uint32_t *__restrict__ ctl)
 {
     const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
+    if (r >= n || ctl[0]) return;
     const uint64_t pa = P[sptr[r]], pb = P[sptr[r + 1]];
     const uint64_t nz = (pb & 0xffffffffull) - (pa & 0xffffffffull);
     const uint64_t mt = (pb >> 32) - (pa >> 32);
@@ -396,49 +405,61 @@ __global__ void k_row_counts(uint64_t n, const uint64_t *__restrict__ sptr, cons
 // ---------------------------------------------------------------------------
 // 5. scatter passes
 // ---------------------------------------------------------------------------
-constexpr uint32_t kSoRows = 512;   // rows of one old-entry scatter block staged in LDS
+constexpr uint32_t kSoRows = 384;   // rows of one re-lock scatter block staged in LDS
 constexpr uint32_t kSoIns = 1024;   // deduped inserts of those rows staged in LDS (< 2^16: 16-bit counts)
 constexpr uint32_t kCleanRow = 0xffffffffu;
 
-__global__ __launch_bounds__(kThreads) void k_scatter_old(uint64_t n, uint64_t nnz_old,
-                                                           const uint64_t *__restrict__ old_off,
-                                                           const uint32_t *__restrict__ old_col,
-                                                           const double *__restrict__ old_val,
-                                                           const uint64_t *__restrict__ sptr,
-                                                           const uint32_t *__restrict__ dcnt,
-                                                           const uint32_t *__restrict__ dcol,
-                                                           const uint64_t *__restrict__ P,
-                                                           const uint64_t *__restrict__ PZ,
-                                                           const uint8_t *__restrict__ dirty,
-                                                           const uint64_t *__restrict__ new_off,
-                                                           uint32_t *__restrict__ new_col, double *__restrict__ new_val,
-                                                           uint32_t *__restrict__ err, const uint64_t *__restrict__ otab)
+// Largest r in [lo, hi) with old_off[r] + sptr[r] <= p (both non-decreasing).
+__device__ __forceinline__ uint64_t row_of_key(const uint64_t *old_off, const uint64_t *sptr, uint64_t lo, uint64_t hi,
+                                               uint64_t p)
 {
-    // The block's rows (old_off, new_off, dirty bit, deduped-insert range) and
-    // their deduped inserts (columns, prefix counts relative to the first) are
-    // loaded into LDS once, in parallel; every entry then resolves its slot
-    // from LDS alone.  Blocks whose rows or inserts do not fit read the same
-    // quantities from global memory per entry.
-    __shared__ uint64_t s_oo[kSoRows + 1];  // old_off[lo .. hi+1]
-    __shared__ uint64_t s_no[kSoRows + 1];  // new_off[lo .. hi+1]
-    __shared__ uint32_t s_db[kSoRows];      // sptr[r] - ibase, or kCleanRow
-    __shared__ uint16_t s_dn[kSoRows];      // dcnt[r]
-    __shared__ uint32_t s_dcol[kSoIns];     // dcol[ibase ..)
-    __shared__ uint32_t s_P[kSoIns + 1];    // (matches << 16 | nonzeros) of P[ibase + t] - P[ibase]
-    const uint64_t first = (uint64_t)blockIdx.x * kPerBlockOld;
-    if (first >= nnz_old) return;
-    const uint64_t last = min(first + kPerBlockOld, nnz_old) - 1;
-    // Entry it*kThreads + tid of the block: loads and stores stay coalesced.
-    // The entries are loaded first and are in flight while the rows stage.
-    uint32_t cs[kItemsOld];
-    double vs[kItemsOld];
-#pragma unroll
-    for (int it = 0; it < kItemsOld; ++it) {
-        const uint64_t k = first + (uint64_t)it * kThreads + threadIdx.x;
-        cs[it] = k <= last ? ld_stream(old_col + k) : 0u;
-        vs[it] = k <= last ? ld_stream(old_val + k) : 0.0;
+    while (hi - lo > 1) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if (old_off[mid] + sptr[mid] <= p) lo = mid;
+        else hi = mid;
     }
-    const row_range rr = table_rows(otab);
+    return lo;
+}
+
+// Re-lock scatter (merge_csr, csr_build.hpp:126-488) as one element-parallel
+// pass over a combined index space in which row r holds its old entries and
+// then its staged slots, starting at key(r) = old_off[r] + sptr[r].  Every old
+// entry and every surviving deduped insert computes its own output slot from
+// the prefix sums, and both parts of a row's new range are written by the
+// same block close together in time, so its cache lines leave L2 whole.
+// The block's rows (key, old_off, new_off, dirty bit, deduped-insert range)
+// and their deduped inserts (columns, prefix counts relative to the first)
+// are loaded into LDS once, in parallel; blocks whose rows or inserts do not
+// fit read the same quantities from global memory.
+__global__ __launch_bounds__(kThreads) void k_scatter_merge(uint64_t total, const uint64_t *__restrict__ old_off,
+                                                             const uint32_t *__restrict__ old_col,
+                                                             const double *__restrict__ old_val,
+                                                             const uint64_t *__restrict__ sptr,
+                                                             const uint32_t *__restrict__ dcnt,
+                                                             const uint32_t *__restrict__ dcol,
+                                                             const double *__restrict__ dval,
+                                                             const uint64_t *__restrict__ P,
+                                                             const uint32_t *__restrict__ qpos,
+                                                             const uint64_t *__restrict__ PZ,
+                                                             const uint8_t *__restrict__ dirty,
+                                                             const uint64_t *__restrict__ new_off,
+                                                             uint32_t *__restrict__ new_col,
+                                                             double *__restrict__ new_val, uint32_t *__restrict__ err,
+                                                             const uint64_t *__restrict__ mtab)
+{
+    __shared__ uint64_t s_key[kSoRows + 1];  // key(lo .. hi+1)
+    __shared__ uint64_t s_oo[kSoRows + 1];   // old_off[lo .. hi+1]
+    __shared__ uint64_t s_no[kSoRows + 1];   // new_off[lo .. hi+1]
+    __shared__ uint32_t s_db[kSoRows];       // sptr[r] - ibase, or kCleanRow
+    __shared__ uint16_t s_dn[kSoRows];       // dcnt[r]
+    __shared__ uint32_t s_dcol[kSoIns];      // dcol[ibase ..)
+    __shared__ uint32_t s_P[kSoIns + 1];     // (matches << 16 | nonzeros) of P[ibase + t] - P[ibase]
+    __shared__ uint16_t s_rid[kPerBlockMerge];  // local row of each position of the block
+    static_assert(kSoRows <= 65536, "local rows are 16-bit");
+    const uint64_t first = (uint64_t)blockIdx.x * kPerBlockMerge;
+    if (first >= total || *err) return;
+    const uint64_t last = min(first + kPerBlockMerge, total) - 1;
+    const row_range rr = table_rows(mtab);
     const uint64_t nr = rr.hi - rr.lo + 1;
     const uint64_t ibase = sptr[rr.lo];
     const uint64_t ni = sptr[rr.hi] + dcnt[rr.hi] - ibase;  // deduped inserts of rows lo..hi lie in [ibase, ibase+ni)
@@ -446,11 +467,16 @@ __global__ __launch_bounds__(kThreads) void k_scatter_old(uint64_t n, uint64_t n
     if (staged) {
         for (uint32_t j = threadIdx.x; j <= nr; j += kThreads) {
             const uint64_t r = rr.lo + j;
-            s_oo[j] = old_off[r];
+            const uint64_t oo = old_off[r], sp = sptr[r];
+            s_key[j] = oo + sp;
+            s_oo[j] = oo;
             s_no[j] = new_off[r];
             if (j < nr) {
-                s_db[j] = row_dirty(dirty, sptr, r) ? (uint32_t)(sptr[r] - ibase) : kCleanRow;
+                s_db[j] = row_dirty(dirty, sptr, r) ? (uint32_t)(sp - ibase) : kCleanRow;
                 s_dn[j] = (uint16_t)dcnt[r];
+                // the block's positions that row j owns
+                const uint64_t a = max(oo + sp, first), b = min(old_off[r + 1] + sptr[r + 1], last + 1);
+                for (uint64_t p = a; p < b; ++p) s_rid[p - first] = (uint16_t)j;
             }
         }
         const uint64_t p0 = P[ibase];
@@ -463,61 +489,120 @@ __global__ __launch_bounds__(kThreads) void k_scatter_old(uint64_t n, uint64_t n
         }
     }
     __syncthreads();
+
+    // Row of position p and what the scatter needs of it, from LDS or memory.
+    struct row_view {
+        uint64_t key, ob, olen, no, nlen, db, dn;  // db: absolute index of the row's first staged slot
+        bool dirty;
+    };
+    auto view = [&](uint64_t p) {
+        row_view w;
+        if (staged) {
+            const uint32_t a = s_rid[p - first];
+            w.key = s_key[a];
+            w.ob = s_oo[a];
+            w.olen = s_oo[a + 1] - w.ob;
+            w.no = s_no[a];
+            w.nlen = s_no[a + 1] - w.no;
+            w.dirty = s_db[a] != kCleanRow;
+            w.db = ibase + (w.dirty ? s_db[a] : 0u);
+            w.dn = s_dn[a];
+        } else {
+            const uint64_t r = row_of_key(old_off, sptr, rr.lo, rr.hi + 1, p);
+            w.ob = old_off[r];
+            w.db = sptr[r];
+            w.key = w.ob + w.db;
+            w.olen = old_off[r + 1] - w.ob;
+            w.no = new_off[r];
+            w.nlen = new_off[r + 1] - w.no;
+            w.dirty = row_dirty(dirty, sptr, r);
+            w.dn = dcnt[r];
+        }
+        return w;
+    };
+    // (matches, nonzeros) before deduped slot t, relative to any other slot of
+    // the block's rows: only differences are used.
+    auto prefix = [&](uint64_t t, uint32_t &pm, uint32_t &pz) {
+        if (staged) {
+            const uint32_t w = s_P[t - ibase];
+            pm = w >> 16;
+            pz = w & 0xffffu;
+        } else {
+            const uint64_t w = P[t];
+            pm = (uint32_t)(w >> 32);
+            pz = (uint32_t)w;
+        }
+    };
+    auto col_of = [&](uint64_t t) -> uint32_t { return staged ? s_dcol[t - ibase] : dcol[t]; };
+
+    // Pass 1: classify every position and issue all of its loads.
+    uint32_t cs[kItemsMerge], qs[kItemsMerge];
+    double vs[kItemsMerge];
+    uint8_t kind[kItemsMerge];  // 0 nothing, 1 old entry, 2 deduped insert
 #pragma unroll
-    for (int it = 0; it < kItemsOld; ++it) {
-        const uint64_t k = first + (uint64_t)it * kThreads + threadIdx.x;
-        if (k > last) break;
+    for (int it = 0; it < kItemsMerge; ++it) {
+        const uint64_t p = first + (uint64_t)it * kThreads + threadIdx.x;
+        kind[it] = 0;
+        cs[it] = 0;
+        qs[it] = 0;
+        vs[it] = 0.0;
+        if (p > last) continue;
+        const row_view w = view(p);
+        const uint64_t i = p - w.key;
+        if (i < w.olen) {
+            kind[it] = 1;
+            cs[it] = ld_stream(old_col + w.ob + i);
+            vs[it] = ld_stream(old_val + w.ob + i);
+        } else if (w.dirty && i - w.olen < w.dn) {  // clean rows keep no insert; slots past the run hold nothing
+            const uint64_t s = w.db + (i - w.olen);
+            kind[it] = 2;
+            cs[it] = col_of(s);
+            vs[it] = dval[s];
+            qs[it] = qpos[s];
+        }
+    }
+    // Pass 2: output slots and stores.
+#pragma unroll
+    for (int it = 0; it < kItemsMerge; ++it) {
+        if (kind[it] == 0) continue;
+        const uint64_t p = first + (uint64_t)it * kThreads + threadIdx.x;
+        const row_view w = view(p);
+        const uint64_t i = p - w.key;
         const uint32_t c = cs[it];
         const double v = vs[it];
-        uint64_t no, nlen, out;
-        if (staged) {
-            uint32_t a = 0, b = (uint32_t)nr;  // largest a < nr with s_oo[a] <= k
-            while (b - a > 1) {
-                const uint32_t mid = (a + b) >> 1;
-                if (s_oo[mid] <= k) a = mid;
-                else b = mid;
-            }
-            const uint64_t ob = s_oo[a];
-            no = s_no[a];
-            nlen = s_no[a + 1] - no;
-            const uint32_t db = s_db[a];
-            if (db == kCleanRow) {
-                out = no + (k - ob);  // clean row: verbatim (csr_build.hpp:362-377)
+        uint64_t out;
+        if (kind[it] == 1) {
+            const uint64_t k = w.ob + i;
+            if (!w.dirty) {
+                out = w.no + i;  // clean row: verbatim (csr_build.hpp:362-377)
             } else {
-                const uint32_t de = db + s_dn[a];
-                uint32_t p = db, q = de;  // lower bound of c in s_dcol[db, de)
-                while (p < q) {
-                    const uint32_t mid = (p + q) >> 1;
-                    if (s_dcol[mid] < c) p = mid + 1;
-                    else q = mid;
+                const uint64_t de = w.db + w.dn;
+                uint64_t lo = w.db, hi = de;  // lower bound of c in the row's deduped columns
+                while (lo < hi) {
+                    const uint64_t mid = (lo + hi) >> 1;
+                    if (col_of(mid) < c) lo = mid + 1;
+                    else hi = mid;
                 }
-                if (p < de && s_dcol[p] == c) continue;  // buffer wins (:430-436)
-                if (PZ && is_zero_f64(v)) continue;      // old zero dropped (:420-421)
-                const uint32_t pa = s_P[db], pp = s_P[p];
-                const uint64_t before_m = (pp >> 16) - (pa >> 16);
-                const uint64_t before_nz = (pp & 0xffffu) - (pa & 0xffffu);
-                const uint64_t before_z = PZ ? (PZ[k] - PZ[ob]) : 0;
-                out = no + (k - ob) - before_m - before_z + before_nz;
+                if (lo < de && col_of(lo) == c) continue;  // buffer wins (:430-436)
+                if (PZ && is_zero_f64(v)) continue;        // old zero dropped (:420-421)
+                uint32_t am, az, pm, pz;
+                prefix(w.db, am, az);
+                prefix(lo, pm, pz);
+                const uint64_t before_z = PZ ? (PZ[k] - PZ[w.ob]) : 0;
+                out = w.no + i - (uint32_t)(pm - am) - before_z + (uint32_t)(pz - az);
             }
         } else {
-            const uint64_t r = row_of(old_off, rr.lo, rr.hi + 1, k);
-            const uint64_t ob = old_off[r];
-            no = new_off[r];
-            nlen = new_off[r + 1] - no;
-            if (!row_dirty(dirty, sptr, r)) {
-                out = no + (k - ob);
-            } else {
-                const uint64_t db = sptr[r], de = db + dcnt[r];
-                const uint64_t p = lower_bound_u32(dcol, db, de, c);
-                if (p < de && dcol[p] == c) continue;
-                if (PZ && is_zero_f64(v)) continue;
-                const uint64_t pa = P[db], pp = P[p];
-                const uint64_t before_m = (pp >> 32) - (pa >> 32);
-                const uint64_t before_nz = (pp & 0xffffffffull) - (pa & 0xffffffffull);
-                const uint64_t before_z = PZ ? (PZ[k] - PZ[ob]) : 0;
-                out = no + (k - ob) - before_m - before_z + before_nz;
-            }
+            const uint64_t s = w.db + (i - w.olen);
+            uint32_t am, az, sm, sz, tm, tz;
+            prefix(s, sm, sz);
+            prefix(s + 1, tm, tz);
+            if (tz == sz) continue;  // zero insert: a delete marker, not an entry
+            prefix(w.db, am, az);
+            const uint64_t q = qs[it];  // old entries of the row before this column
+            const uint64_t before_z = PZ ? (PZ[w.ob + q] - PZ[w.ob]) : 0;
+            out = w.no + (uint32_t)(sz - az) + q - (uint32_t)(sm - am) - before_z;
         }
+        const uint64_t no = w.no, nlen = w.nlen;
         // Every slot must fall inside the row's new range; never write outside.
         if (out - no >= nlen) {
             atomicOr(err, ERR_INTERNAL);
@@ -528,21 +613,19 @@ __global__ __launch_bounds__(kThreads) void k_scatter_old(uint64_t n, uint64_t n
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_scatter_ded(uint64_t n, uint64_t m, const uint64_t *__restrict__ sptr,
-                                                           const uint32_t *__restrict__ dcnt,
+// First lock (build_csr, csr_build.hpp:32-98): the surviving deduped inserts
+// are the whole matrix; each finds its slot from the nonzero prefix.
+__global__ __launch_bounds__(kThreads) void k_scatter_ded(uint64_t m, const uint64_t *__restrict__ sptr,
                                                            const uint32_t *__restrict__ dcol,
                                                            const double *__restrict__ dval,
                                                            const uint64_t *__restrict__ P,
-                                                           const uint32_t *__restrict__ qpos,
-                                                           const uint64_t *__restrict__ old_off,
-                                                           const uint64_t *__restrict__ PZ, int merge,
                                                            const uint64_t *__restrict__ new_off,
                                                            uint32_t *__restrict__ new_col, double *__restrict__ new_val,
                                                            uint32_t *__restrict__ err, const uint64_t *__restrict__ stab)
 {
     __shared__ uint64_t s_rows[kRowCache + 1];
     const uint64_t first = (uint64_t)blockIdx.x * kPerBlock;
-    if (first >= m) return;
+    if (first >= m || *err) return;
     // prefix words first: these loads are in flight while the row cache fills
     uint64_t pw[kItems], pw1[kItems];
 #pragma unroll
@@ -563,14 +646,7 @@ __global__ __launch_bounds__(kThreads) void k_scatter_ded(uint64_t n, uint64_t m
         const uint64_t db = rc.at(r);
         const uint64_t pa = P[db];
         const uint64_t before_nz = (ps & 0xffffffffull) - (pa & 0xffffffffull);
-        uint64_t out = new_off[r] + before_nz;
-        if (merge) {
-            const uint64_t q = qpos[s];
-            const uint64_t before_m = (ps >> 32) - (pa >> 32);
-            const uint64_t ob = old_off[r];
-            const uint64_t before_z = PZ ? (PZ[ob + q] - PZ[ob]) : 0;
-            out += q - before_m - before_z;
-        }
+        const uint64_t out = new_off[r] + before_nz;
         if (out - new_off[r] >= new_off[r + 1] - new_off[r]) {
             atomicOr(err, ERR_INTERNAL);
             continue;
@@ -578,6 +654,13 @@ __global__ __launch_bounds__(kThreads) void k_scatter_ded(uint64_t n, uint64_t m
         new_col[out] = dcol[s];
         new_val[out] = dval[s];
     }
+}
+
+// The new nnz next to the error bits, so one copy brings both back.
+__global__ void k_finish(const uint64_t *__restrict__ nnz_new, uint32_t *__restrict__ ctl)
+{
+    ctl[2] = (uint32_t)*nnz_new;
+    ctl[3] = (uint32_t)(*nnz_new >> 32);
 }
 
 // ---------------------------------------------------------------------------
@@ -689,7 +772,11 @@ uint64_t lock_device_f64(spira_dcsr *A, const uint64_t *sptr, const uint32_t *sc
     if (m >= (1ull << 32)) fail(SPIRA_ERR_UNSUPPORTED, "lock: more than 2^32-1 staged inserts in one call");
 
     // ---- 1. checks + binning -------------------------------------------------
-    uint32_t *ctl = (uint32_t *)dev->ws.get(WS_MISC, 4 * sizeof(uint32_t));  // [err, n_big]
+    // ctl = [error bits, rows with many inserts, new nnz (u64)].  There is no
+    // host round trip here: every later kernel returns at once when an error
+    // bit is set, and the host reads the bits with the new nnz at the end, so
+    // a failed check still leaves the matrix untouched.
+    uint32_t *ctl = (uint32_t *)dev->ws.get(WS_MISC, 4 * sizeof(uint32_t));
     uint32_t *big = (uint32_t *)dev->ws.get(WS_BIG, (n + 1) * sizeof(uint32_t));
     SPIRA_HIP(hipMemsetAsync(ctl, 0, 4 * sizeof(uint32_t), st));
     k_check_ptr<<<grid_for(n + 1, kThreads), kThreads, 0, st>>>(sptr, n, m, ctl, big, ctl + 1);
@@ -698,15 +785,6 @@ uint64_t lock_device_f64(spira_dcsr *A, const uint64_t *sptr, const uint32_t *sc
         k_check_cols<<<grid_for(m, kThreads), kThreads, 0, st>>>(scol, m, A->n_cols, ctl);
         SPIRA_HIP(hipGetLastError());
     }
-    uint32_t hctl[2];
-    SPIRA_HIP(hipMemcpyAsync(hctl, ctl, sizeof(hctl), hipMemcpyDeviceToHost, st));
-    dev_sync(dev);
-    if (hctl[0] & ERR_BAD_PTR)
-        fail(SPIRA_ERR_INVALID_ARGUMENT, "lock: stage_ptr must start at 0, be non-decreasing and end at m");
-    if (hctl[0] & ERR_OUT_OF_RANGE) fail(SPIRA_ERR_OUT_OF_RANGE, "spira::matrix: col_index out of range");
-    if (hctl[0] & ERR_TOO_LONG)
-        fail(SPIRA_ERR_UNSUPPORTED, "lock: a row has more staged inserts than SPIRA_LOCK_MAX_ROW_INSERTS");
-    const uint32_t n_big = hctl[1];
 
     // ---- 2. dedup ----------------------------------------------------------
     uint32_t *dcol = (uint32_t *)dev->ws.get(WS_DED_COL, m * sizeof(uint32_t));
@@ -714,20 +792,19 @@ uint64_t lock_device_f64(spira_dcsr *A, const uint64_t *sptr, const uint32_t *sc
     uint32_t *dcnt = (uint32_t *)dev->ws.get(WS_ROWS, (n + 1) * sizeof(uint32_t) + 8);
     // Row tables (mark_block_rows) of the staged-slot passes and of the
     // old-entry scatter, filled by the dedup pass that reads both pointers.
-    const bool scatter_old = merge && A->nnz > 0;
+    const uint64_t total_merge = merge ? A->nnz + m : 0;  // re-lock scatter positions: old entries + staged slots
     const uint64_t nb_s = (m + kPerBlock - 1) / kPerBlock + 1;
-    const uint64_t nb_o = scatter_old ? (A->nnz + kPerBlockOld - 1) / kPerBlockOld + 1 : 0;
-    uint64_t *stab = (uint64_t *)dev->ws.get(WS_BLK, (nb_s + nb_o) * sizeof(uint64_t));
-    uint64_t *otab = scatter_old ? stab + nb_s : nullptr;
+    const uint64_t nb_m = total_merge ? (total_merge + kPerBlockMerge - 1) / kPerBlockMerge + 1 : 0;
+    uint64_t *stab = (uint64_t *)dev->ws.get(WS_BLK, (nb_s + nb_m) * sizeof(uint64_t));
+    uint64_t *mtab = total_merge ? stab + nb_s : nullptr;
     if (n > 0) {
         k_dedup_small<<<grid_for(n, kThreads), kThreads, 0, st>>>(n, sptr, scol, sval, dcol, dval, dcnt, stab,
-                                                                   A->off, scatter_old ? A->nnz : 0, otab);
+                                                                   A->off, mtab, ctl);
         SPIRA_HIP(hipGetLastError());
     }
-    if (n_big > 0) {
-        const unsigned g = std::min<uint32_t>(n_big, 4096u);
-        k_dedup_big<<<g, kThreads, kMaxSeg * sizeof(uint64_t), st>>>(big, ctl + 1, sptr, scol, sval, dcol, dval,
-                                                                     dcnt);
+    if (n > 0) {  // the number of listed rows is on the device: blocks stride over it
+        const unsigned g = (unsigned)std::min<uint64_t>(n, 2048u);
+        k_dedup_big<<<g, kThreads, kMaxSeg * sizeof(uint64_t), st>>>(big, ctl, sptr, scol, sval, dcol, dval, dcnt);
         SPIRA_HIP(hipGetLastError());
     }
 
@@ -736,7 +813,7 @@ uint64_t lock_device_f64(spira_dcsr *A, const uint64_t *sptr, const uint32_t *sc
     uint32_t *qpos = merge ? (uint32_t *)dev->ws.get(WS_QPOS, m * sizeof(uint32_t)) : nullptr;
     if (m > 0) {
         k_ded_flags<<<grid_for(m, kPerBlock), kThreads, 0, st>>>(n, m, sptr, dcnt, dcol, dval, dirty, merge ? 1 : 0,
-                                                                 A->off, A->col, P, qpos, stab);
+                                                                 A->off, A->col, P, qpos, stab, ctl);
         SPIRA_HIP(hipGetLastError());
     }
     scan_excl_u64(dev, P, P, m);
@@ -746,7 +823,7 @@ uint64_t lock_device_f64(spira_dcsr *A, const uint64_t *sptr, const uint32_t *sc
     if (merge && A->has_zeros && A->nnz > 0) {
         PZ = (uint64_t *)dev->ws.get(WS_PZ, (A->nnz + 1) * sizeof(uint64_t));
         k_old_zero_flags<<<grid_for(A->nnz, kThreads * kItems), kThreads, 0, st>>>(n, A->nnz, A->off, A->col, A->val,
-                                                                                   sptr, dcnt, dcol, dirty, PZ);
+                                                                                   sptr, dcnt, dcol, dirty, PZ, ctl);
         SPIRA_HIP(hipGetLastError());
         scan_excl_u64(dev, PZ, PZ, A->nnz);
     }
@@ -767,30 +844,36 @@ uint64_t lock_device_f64(spira_dcsr *A, const uint64_t *sptr, const uint32_t *sc
         new_val = (double *)dev_alloc(dev, vb);
         if (n > 0) {
             k_row_counts<<<grid_for(n, kThreads), kThreads, 0, st>>>(n, sptr, P, dirty, merge ? 1 : 0, A->off, PZ,
-                                                                     cnt);
+                                                                     cnt, ctl);
             SPIRA_HIP(hipGetLastError());
         }
         scan_excl_u64(dev, cnt, new_off, n);
 
         // ---- 5. scatter ------------------------------------------------------
-        if (scatter_old) {
-            k_scatter_old<<<grid_for(A->nnz, kPerBlockOld), kThreads, 0, st>>>(
-                n, A->nnz, A->off, A->col, A->val, sptr, dcnt, dcol, P, PZ, dirty, new_off, new_col, new_val, ctl,
-                otab);
+        if (total_merge > 0) {
+            k_scatter_merge<<<grid_for(total_merge, kPerBlockMerge), kThreads, 0, st>>>(
+                total_merge, A->off, A->col, A->val, sptr, dcnt, dcol, dval, P, qpos, PZ, dirty, new_off, new_col,
+                new_val, ctl, mtab);
+            SPIRA_HIP(hipGetLastError());
+        } else if (!merge && m > 0) {
+            k_scatter_ded<<<grid_for(m, kPerBlock), kThreads, 0, st>>>(m, sptr, dcol, dval, P, new_off, new_col,
+                                                                       new_val, ctl, stab);
             SPIRA_HIP(hipGetLastError());
         }
-        if (m > 0) {
-            k_scatter_ded<<<grid_for(m, kPerBlock), kThreads, 0, st>>>(n, m, sptr, dcnt, dcol, dval, P, qpos, A->off,
-                                                                       PZ, merge ? 1 : 0, new_off, new_col, new_val,
-                                                                       ctl, stab);
-            SPIRA_HIP(hipGetLastError());
-        }
-        uint32_t herr = 0;
-        uint64_t nnz_new = 0;
-        SPIRA_HIP(hipMemcpyAsync(&herr, ctl, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-        SPIRA_HIP(hipMemcpyAsync(&nnz_new, new_off + n, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+        k_finish<<<1, 1, 0, st>>>(new_off + n, ctl);
+        SPIRA_HIP(hipGetLastError());
+        uint32_t hctl[4];
+        SPIRA_HIP(hipMemcpyAsync(hctl, ctl, sizeof(hctl), hipMemcpyDeviceToHost, st));
         dev_sync(dev);
-        if (herr & ERR_INTERNAL) fail(SPIRA_ERR_DEVICE, "lock: internal error, a merged entry fell outside its row");
+        if (hctl[0] & ERR_BAD_PTR)
+            fail(SPIRA_ERR_INVALID_ARGUMENT, "lock: stage_ptr must start at 0, be non-decreasing and end at m");
+        if (hctl[0] & ERR_OUT_OF_RANGE) fail(SPIRA_ERR_OUT_OF_RANGE, "spira::matrix: col_index out of range");
+        if (hctl[0] & ERR_TOO_LONG)
+            fail(SPIRA_ERR_UNSUPPORTED, "lock: a row has more staged inserts than SPIRA_LOCK_MAX_ROW_INSERTS");
+        if (hctl[0] & ERR_INTERNAL)
+            fail(SPIRA_ERR_DEVICE, "lock: internal error, a merged entry fell outside its row");
+        uint64_t nnz_new;
+        std::memcpy(&nnz_new, hctl + 2, sizeof(nnz_new));
         // install: the old arrays go back to the cache (stream order keeps
         // them intact for anything still queued)
         free_dcsr_arrays(A);

@@ -244,6 +244,11 @@ def test_batch_errors(dev):
         B.spmm(X, Y)
     with pytest.raises(spira_amd.InvalidArgument):
         B.set_schedule(3)
+    B.set_schedule(4)
+    with pytest.raises(spira_amd.InvalidArgument):  # Y sized for k = 1: rejected before any launch
+        B.spmm(X, Y[:, :1].contiguous())
+    with pytest.raises(spira_amd.InvalidArgument):
+        B.spmm(X.double(), Y)
     unlocked = spira_amd.DeviceCSR(dev, 5, 5)
     with pytest.raises(spira_amd.LogicError):
         spira_amd.Batch(dev, [mats[0], unlocked])
