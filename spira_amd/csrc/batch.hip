@@ -48,18 +48,33 @@ namespace {
 constexpr int kThreads = 1024;  // one block per CU
 constexpr int kWaves = kThreads / 64;
 constexpr uint32_t kLdsBytes = 155584;  // LDS for the X block and staged entries
-static_assert(SPIRA_BATCH_XLDS_FLOATS * 4 <= kLdsBytes, "X block must fit the LDS area");
+static_assert(SPIRA_BATCH_XLDS_FLOATS * 4 + 32 <= kLdsBytes, "X block must fit the LDS area");
 constexpr uint32_t kMinEntryLds = 65536;  // stage entries only with at least this much LDS left
 constexpr uint32_t kClasses = 34;         // bit length of a u32 row length, plus one
 constexpr uint32_t kTileLong = 0x80u;
 constexpr uint32_t kItemStaged = 1u;   // item flag: entries staged in LDS
 constexpr uint32_t kItemTwoPass = 2u;  // item flag: k = 1, X block staged in two column halves
+constexpr size_t kArrayPad = 64;      // bytes after each entry array (16-byte staging reads)
+// 16-byte loads in flight per lane when a block stages its item: deep at
+// k = 1, where staging is most of the item's time; at k >= 4 the deeper loop
+// changes how the compiler schedules the compute phase and costs more than
+// it saves (measured).
+template <int K>
+constexpr uint32_t stage_depth() { return K == 1 ? 4u : 1u; }
+
+inline __host__ __device__ uint32_t align16(uint32_t v) { return (v + 15u) & ~15u; }
+
+// LDS bytes of an X block of n floats: it is copied as the 16-byte aligned
+// span around its elements (see stage_array), so it may start up to 12 bytes
+// into its area and needs up to 32 bytes more than n * 4.
+inline __host__ __device__ uint32_t xblock_bytes(uint32_t n) { return n ? align16(n * 4) + 32 : 0u; }
+
 // Two-pass items: each half of the X block (at most kHalfFloats floats) sits
 // at the start of the LDS area, followed by one (sum, correction, cursor)
 // record per row of the item.
 constexpr uint32_t kHalfFloats = 32768;
-constexpr uint32_t kTwoPassRows = (kLdsBytes - kHalfFloats * 4) / 12;
-constexpr size_t kArrayPad = 64;      // bytes after each entry array (16-byte staging reads)
+constexpr uint32_t kHalfBytes = kHalfFloats * 4 + 32;
+constexpr uint32_t kTwoPassRows = (kLdsBytes - kHalfBytes) / 12;
 
 __device__ __forceinline__ float bf16f(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
 
@@ -77,8 +92,6 @@ struct item_t {
     uint32_t ebase;   // first entry of row r0
     uint32_t flags;   // kItemStaged, kItemTwoPass
 };
-
-inline __host__ __device__ uint32_t align16(uint32_t v) { return (v + 15u) & ~15u; }
 
 // Byte layout of an item's staged entries in LDS (after the X block).  An
 // entry array is copied as the 16-byte aligned span around its elements,
@@ -228,25 +241,45 @@ __device__ __forceinline__ void merge_lanes(acc2 &a, uint32_t E)
     for (uint32_t o = K; o < K * E; o <<= 1) a.merge(__shfl_xor(a.s, (int)o, 64), __shfl_xor(a.c, (int)o, 64));
 }
 
-// Copy nbytes (a multiple of 16) from 16-byte aligned global src to LDS dst.
+// Copy nbytes (a multiple of 16) from 16-byte aligned global src to LDS dst
+// with D 16-byte loads in flight per lane.
+template <uint32_t D>
 __device__ __forceinline__ void stage16(unsigned char *dst, const unsigned char *src, uint32_t nbytes)
 {
     const u32x4 *s = (const u32x4 *)src;
     u32x4 *d = (u32x4 *)dst;
-    for (uint32_t i = threadIdx.x; i < nbytes / 16; i += kThreads) d[i] = __builtin_nontemporal_load(s + i);
+    const uint32_t n = nbytes / 16;
+    uint32_t i = threadIdx.x;
+    if constexpr (D > 1) {
+        for (; i + (D - 1) * kThreads < n; i += D * kThreads) {
+            u32x4 v[D];
+#pragma unroll
+            for (uint32_t u = 0; u < D; ++u) v[u] = __builtin_nontemporal_load(s + i + u * kThreads);
+#pragma unroll
+            for (uint32_t u = 0; u < D; ++u) d[i + u * kThreads] = v[u];
+        }
+    }
+    for (; i < n; i += kThreads) d[i] = __builtin_nontemporal_load(s + i);
 }
 
-// Stage one entry array: the 16-byte aligned span holding elements
-// [base, base + n) of arr; returns the LDS pointer to element base.
-template <typename T>
-__device__ __forceinline__ const T *stage_array(unsigned char *lds, uint32_t o, const T *arr, uint32_t base,
+// Stage one array: the 16-byte aligned span holding elements [base, base + n)
+// of arr, D loads in flight per lane; returns the LDS pointer to element base.
+template <uint32_t D, typename T>
+__device__ __forceinline__ const T *stage_array(unsigned char *lds, uint32_t o, const T *arr, uint64_t base,
                                                 uint32_t n)
 {
     const uintptr_t a = (uintptr_t)(arr + base);
     const uintptr_t a0 = a & ~(uintptr_t)15;
     const uintptr_t a1 = ((uintptr_t)(arr + base + n) + 15) & ~(uintptr_t)15;
-    stage16(lds + o, (const unsigned char *)a0, (uint32_t)(a1 - a0));
+    stage16<D>(lds + o, (const unsigned char *)a0, (uint32_t)(a1 - a0));
     return (const T *)(lds + o + (uint32_t)(a - a0));
+}
+
+// Where stage_array put element base of arr in the LDS area at offset o.
+template <typename T>
+__device__ __forceinline__ const T *staged_at(const unsigned char *lds, uint32_t o, const T *arr, uint64_t base)
+{
+    return (const T *)(lds + o + (uint32_t)(((uintptr_t)(arr + base)) & 15u));
 }
 
 struct kparams {
@@ -271,13 +304,13 @@ __device__ __forceinline__ void run_item(const kparams &kp, const item_t &it, un
                                          float (*s_part)[kWaves][64], int lane, int w, int j)
 {
     constexpr int U = K >= 16 ? 8 : 4;  // loads in flight per row and lane group
-    const xsrc<K, CT, XS> xv{XS ? (const float *)lds : kp.X, it.cbase};
+    const xsrc<K, CT, XS> xv{XS ? staged_at(lds, 0, kp.X, (uint64_t)it.cbase * K) : kp.X, it.cbase};
     const uint32_t *s_off = nullptr, *s_rid = nullptr;
     esrc<ST, CT, ESTAGED> es{};
     if constexpr (ESTAGED) {
         const uint32_t rows = it.r1 - it.r0;
         const uint32_t nnz = kp.off[it.r1] - it.ebase;
-        const lds_layout<ST, CT> L(XS ? it.xcols * K * 4 : 0u, rows, nnz);
+        const lds_layout<ST, CT> L(XS ? xblock_bytes(it.xcols * K) : 0u, rows, nnz);
         s_off = (const uint32_t *)(lds + L.off_o);
         s_rid = (const uint32_t *)(lds + L.rid_o);
         // the entry arrays start at their 16-byte aligned span (see stage_array)
@@ -416,8 +449,7 @@ __device__ __forceinline__ void run_item_two_pass(const kparams &kp, const item_
     constexpr int U = 4;
     const uint32_t n = it.xcols, W = (n + 1) / 2;
     const uint32_t rows = it.r1 - it.r0;
-    float *xs = (float *)lds;
-    float *s_sum = (float *)(lds + kHalfFloats * 4);
+    float *s_sum = (float *)(lds + kHalfBytes);
     float *s_cor = s_sum + kTwoPassRows;
     uint32_t *s_pos = (uint32_t *)(s_cor + kTwoPassRows);
     const esrc<ST, CT, false> es{(const CT *)kp.col, kp.f32, kp.hi, kp.lo, kp.last};
@@ -429,8 +461,7 @@ __device__ __forceinline__ void run_item_two_pass(const kparams &kp, const item_
     for (uint32_t h = 0; h < 2; ++h) {
         const uint32_t c0 = h * W, c1 = min(n, c0 + W);
         __syncthreads();  // the previous pass is done with the half and the records
-        const float *src = kp.X + (uint64_t)it.cbase + c0;
-        for (uint32_t q = threadIdx.x; q < c1 - c0; q += kThreads) xs[q] = src[q];
+        const float *xs = stage_array<stage_depth<1>()>(lds, 0, kp.X, (uint64_t)it.cbase + c0, c1 - c0);
         __syncthreads();
         for (uint32_t tt = it.t0; tt < it.t1; ++tt) {
             const tile_t tl = kp.tiles[tt];
@@ -519,12 +550,8 @@ __global__ __launch_bounds__(kThreads) void k_batch_spmm(kparams kp)
             }
         }
         const bool xs = it.xcols != 0, staged = (it.flags & kItemStaged) != 0;
-        const uint32_t xbytes = xs ? it.xcols * K * 4 : 0u;
-        if (xs) {
-            const float *src = kp.X + (uint64_t)it.cbase * K;
-            float *dst = (float *)lds;
-            for (uint32_t q = threadIdx.x; q < it.xcols * K; q += kThreads) dst[q] = src[q];
-        }
+        const uint32_t xbytes = xs ? xblock_bytes(it.xcols * K) : 0u;
+        if (xs) stage_array<stage_depth<K>()>(lds, 0, kp.X, (uint64_t)it.cbase * K, it.xcols * K);
         if (staged) {
             const uint32_t rows = it.r1 - it.r0;
             const uint32_t nnz = kp.off[it.r1] - it.ebase;
@@ -533,12 +560,13 @@ __global__ __launch_bounds__(kThreads) void k_batch_spmm(kparams kp)
             uint32_t *s_rid = (uint32_t *)(lds + L.rid_o);
             for (uint32_t q = threadIdx.x; q <= rows; q += kThreads) s_off[q] = kp.off[it.r0 + q] - it.ebase;
             for (uint32_t q = threadIdx.x; q < rows; q += kThreads) s_rid[q] = kp.rowid[it.r0 + q];
-            stage_array(lds, L.col_o, (const CT *)kp.col, it.ebase, nnz);
+            stage_array<stage_depth<K>()>(lds, L.col_o, (const CT *)kp.col, it.ebase, nnz);
             if constexpr (ST == SPIRA_VAL_F32) {
-                stage_array(lds, L.v1_o, kp.f32, it.ebase, nnz);
+                stage_array<stage_depth<K>()>(lds, L.v1_o, kp.f32, it.ebase, nnz);
             } else {
-                stage_array(lds, L.v1_o, kp.hi, it.ebase, nnz);
-                if constexpr (ST == SPIRA_VAL_SPLIT_BF16) stage_array(lds, L.v2_o, kp.lo, it.ebase, nnz);
+                stage_array<stage_depth<K>()>(lds, L.v1_o, kp.hi, it.ebase, nnz);
+                if constexpr (ST == SPIRA_VAL_SPLIT_BF16)
+                    stage_array<stage_depth<K>()>(lds, L.v2_o, kp.lo, it.ebase, nnz);
             }
         }
         if (xs || staged) __syncthreads();
@@ -697,8 +725,8 @@ static void batch_build_work(spira_batch *b)
         // whole-block tiles excepted: those read X from global memory)
         const bool two_pass = s.xlds && K == 1 && ncols > SPIRA_BATCH_XLDS_FLOATS;
         const uint32_t xcols = (s.xlds && !two_pass) ? ncols : 0u;  // single-load X block
-        const uint32_t xbytes = xcols * K * 4;
-        const bool stage = !two_pass && align16(xbytes) + kMinEntryLds <= kLdsBytes;
+        const uint32_t xbytes = xblock_bytes(xcols * K);
+        const bool stage = !two_pass && xbytes + kMinEntryLds <= kLdsBytes;
         const uint32_t *cs = &b->class_start[(size_t)m * (kClasses + 1)];
         // cost of a run of rows = nonzeros + rows (an empty row still writes k outputs)
         item_t cur{(uint32_t)tl.size(), 0, (uint32_t)b->col_base[m], xcols, 0, 0, 0, 0};
