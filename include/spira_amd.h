@@ -58,7 +58,9 @@ typedef struct spira_dcsr spira_dcsr; /* device CSR: u64 offsets, u32 cols, f64 
 #define SPIRA_SPMV_XSLICE2 10     /* ... 2 lanes per row */
 #define SPIRA_SPMV_XSLICE4 11     /* ... 4 lanes per row */
 #define SPIRA_SPMV_XSTREAM 12     /* column slice per XCD, LDS row blocks per slice */
-#define SPIRA_SPMV_N_VARIANTS 13
+#define SPIRA_SPMV_XPHASE 13      /* row blocks with y in LDS, x slices in L2-sized phases per XCD */
+#define SPIRA_SPMV_XPHASE16 14    /* ... 16 entries per thread per step instead of 4 */
+#define SPIRA_SPMV_N_VARIANTS 15
 
 /* ---- device handle --------------------------------------------------------
  * Replaces the reference's CPU dispatch role (dispatch.cpp:34-67,
@@ -154,8 +156,9 @@ int spira_spmv_set_variant(spira_dcsr *A, int variant);
 /* Exhaustive autotuner (replaces the reference's one-time CPUID kernel pick,
  * dispatch.cpp:34-67, with a per-matrix timed pick): runs every variant `reps`
  * times on device vectors x (n_cols) and y (n_rows), writes the mean time per
- * call in milliseconds to ms[SPIRA_SPMV_N_VARIANTS], pins the fastest as A's
- * AUTO choice and returns it in *best (either output may be NULL). */
+ * call in milliseconds to ms[SPIRA_SPMV_N_VARIANTS] (-1 for a variant whose
+ * size limits A exceeds), pins the fastest as A's AUTO choice and returns it
+ * in *best (either output may be NULL). */
 int spira_spmv_autotune_f64(spira_dcsr *A, const double *x_dev, double *y_dev, int reps, float *ms, int *best);
 
 /* ---- partitions -------------------------------------------------------------

@@ -55,12 +55,13 @@ NO_SYNC = 16
 SPMV_AUTO = -1
 VARIANTS = {
     "vec1": 0, "vec2": 1, "vec4": 2, "vec8": 3, "vec16": 4, "vec32": 5, "vec64": 6,
-    "stream1k": 7, "stream2k": 8, "xslice1": 9, "xslice2": 10, "xslice4": 11, "xstream": 12,
+    "stream1k": 7, "stream2k": 8, "xslice1": 9, "xslice2": 10, "xslice4": 11, "xstream": 12, "xphase": 13,
+    "xphase16": 14,
 }
 VARIANT_NAMES = {v: k for k, v in VARIANTS.items()}
 STORAGE = {"f32": 1, "bf16": 2, "split_bf16": 3}
 MIXED_VARIANTS = [VARIANTS[k] for k in ("vec4", "vec8", "vec16", "vec32", "stream1k", "stream2k")]
-N_VARIANTS = 13
+N_VARIANTS = 15
 BATCH_XLDS_FLOATS = 36864
 BATCH_XLDS_TWO_PASS_COLS = 65536
 BATCH_MAX_PLOG2 = 5

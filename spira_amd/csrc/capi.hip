@@ -396,11 +396,17 @@ int spira_spmv_autotune_f64(spira_dcsr *A, const double *x_dev, double *y_dev, i
         hipEvent_t e0, e1;
         SPIRA_HIP(hipEventCreate(&e0));
         SPIRA_HIP(hipEventCreate(&e1));
-        int best_v = 0;
+        int best_v = -1;
         float best_ms = 0.0f;
         try {
             for (int v = 0; v < SPIRA_SPMV_N_VARIANTS; ++v) {
-                spmv_launch_f64(A, x_dev, y_dev, v);  // warm-up (builds the plan parts)
+                try {
+                    spmv_launch_f64(A, x_dev, y_dev, v);  // warm-up (builds the plan parts)
+                } catch (const status_error &e) {
+                    if (e.code != SPIRA_ERR_UNSUPPORTED) throw;
+                    if (ms) ms[v] = -1.0f;  // this variant cannot run this matrix (size limits)
+                    continue;
+                }
                 SPIRA_HIP(hipEventRecord(e0, dev->stream));
                 for (int r = 0; r < reps; ++r) spmv_launch_f64(A, x_dev, y_dev, v);
                 SPIRA_HIP(hipEventRecord(e1, dev->stream));
@@ -409,7 +415,7 @@ int spira_spmv_autotune_f64(spira_dcsr *A, const double *x_dev, double *y_dev, i
                 SPIRA_HIP(hipEventElapsedTime(&t, e0, e1));
                 t /= (float)reps;
                 if (ms) ms[v] = t;
-                if (v == 0 || t < best_ms) {
+                if (best_v < 0 || t < best_ms) {
                     best_ms = t;
                     best_v = v;
                 }

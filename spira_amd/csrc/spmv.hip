@@ -15,6 +15,9 @@
 //   XSLICE<G> the columns cut into 8 slices, one per XCD (slice-major copy of
 //             A), G lanes per row; partial y per slice, summed in slice order.
 //   XSTREAM   the same slices, each run with the STREAM row-block kernel.
+//   XPHASE    row blocks that keep their y in LDS; the blocks of one XCD walk
+//             L2-sized column slices in the same order (entries grouped by
+//             block and slice), so x gathers hit L2 and y is written once.
 //
 // Values and columns are read with non-temporal loads so the x gathers keep
 // L2 and the MALL.  Blocks are renumbered so that each XCD (8 on MI355X, which
@@ -22,6 +25,7 @@
 // keeping a row neighbourhood's x lines in one L2.
 #include "internal.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -280,6 +284,127 @@ __global__ void k_u64_to_u32(const uint64_t *__restrict__ a, uint32_t *__restric
 }
 
 // ---------------------------------------------------------------------------
+// XPHASE: a block owns a run of RB <= kPhaseRows rows and keeps their y in
+// LDS for the whole call.  The columns are cut into S slices of at most
+// kPhaseSliceBytes of x; the plan groups the entries by (block, slice), rows
+// in order, each entry carrying its row inside the block (u16).  A block walks
+// the S slices in an order that depends only on its XCD (hardware block b runs
+// on XCD b % 8), so the blocks of one XCD gather from the same slice at the
+// same time and that slice stays in the XCD's L2 while A streams past it.
+// Products are added into LDS with native FP64 atomics (the order of the
+// additions within a row can differ from call to call; the rounding of the
+// row sum is that of any summation order), and y is written once at the end:
+// no partial-y vectors and no second kernel.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kPhaseRows = 16384;                 // rows per block: 128 KiB of FP64 y in LDS
+constexpr uint64_t kPhaseSliceBytes = 1ull << 20;       // x bytes per slice: a quarter of an XCD's L2
+constexpr uint32_t kPhaseMaxSlices = 256;
+constexpr int kPhaseThreads = 1024;
+// Entries per thread per step: XPHASE 4 (banded matrices: few entries per
+// block and slice), XPHASE16 16 (scattered columns: more gathers in flight).
+
+template <int U>
+__global__ __launch_bounds__(kPhaseThreads) void k_spmv_xphase(uint64_t n_rows, uint32_t RB, uint32_t S,
+                                                                const uint64_t *__restrict__ goff,
+                                                                const uint16_t *__restrict__ prow,
+                                                                const uint32_t *__restrict__ pcol,
+                                                                const double *__restrict__ pval,
+                                                                const double *__restrict__ x, double *__restrict__ y)
+{
+    __shared__ double s_y[kPhaseRows];
+    __shared__ uint64_t s_goff[kPhaseMaxSlices + 1];  // this block's group starts
+    const uint32_t b = blockIdx.x;
+    const uint64_t r0 = (uint64_t)b * RB;
+    const uint32_t nr = (uint32_t)min((uint64_t)RB, n_rows - r0);
+    for (uint32_t i = threadIdx.x; i < nr; i += kPhaseThreads) s_y[i] = 0.0;
+    for (uint32_t i = threadIdx.x; i <= S; i += kPhaseThreads) s_goff[i] = goff[(uint64_t)b * S + i];
+    __syncthreads();
+    const uint32_t per = S / 8;  // S is a multiple of 8; XCD c starts at slice c * S / 8
+    const uint32_t first = (b % 8) * per;
+    for (uint32_t t = 0; t < S; ++t) {
+        const uint32_t s = first + t < S ? first + t : first + t - S;
+        const uint64_t e0 = s_goff[s], e1 = s_goff[s + 1];
+        for (uint64_t k = e0 + threadIdx.x; k < e1; k += (uint64_t)U * kPhaseThreads) {
+            uint32_t rr[U], cc[U];
+            double vv[U], xx[U];
+            bool ok[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint64_t q = k + (uint64_t)u * kPhaseThreads;
+                ok[u] = q < e1;
+                rr[u] = ok[u] ? (uint32_t)ld_stream(prow + q) : 0u;
+                cc[u] = ok[u] ? ld_stream(pcol + q) : 0u;  // a masked lane reads x[0] and adds nothing
+                vv[u] = ok[u] ? ld_stream(pval + q) : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) xx[u] = x[cc[u]];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (ok[u])
+                    __hip_atomic_fetch_add(&s_y[rr[u]], vv[u] * xx[u], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nr; i += kPhaseThreads) y[r0 + i] = s_y[i];
+}
+
+// Plan: entries of row r per slice, stored at index (b * S + s) * RB + (r - b RB)
+// so that one exclusive scan gives every row's first position in every group.
+__global__ __launch_bounds__(kThreads) void k_xphase_count(uint64_t n_rows, uint64_t n_slot_rows, uint32_t RB,
+                                                            uint32_t S, uint64_t width,
+                                                            const uint64_t *__restrict__ off,
+                                                            const uint32_t *__restrict__ col,
+                                                            uint64_t *__restrict__ cnt)
+{
+    // one thread per row slot r < n_slot_rows = nb * RB: the last block's
+    // slots past n_rows get zero counts
+    const uint64_t r = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (r >= n_slot_rows) return;
+    const uint64_t b = r / RB, rl = r % RB;
+    uint64_t *c = cnt + b * S * RB + rl;
+    for (uint32_t s = 0; s < S; ++s) c[(uint64_t)s * RB] = 0;
+    if (r >= n_rows) return;
+    for (uint64_t k = off[r]; k < off[r + 1]; ++k) c[(uint64_t)(col[k] / width) * RB] += 1;
+}
+
+__global__ __launch_bounds__(kThreads) void k_xphase_scatter(uint64_t n_rows, uint32_t RB, uint32_t S,
+                                                              uint64_t width, const uint64_t *__restrict__ off,
+                                                              const uint32_t *__restrict__ col,
+                                                              const double *__restrict__ val,
+                                                              const uint64_t *__restrict__ pos,
+                                                              uint16_t *__restrict__ prow, uint32_t *__restrict__ pcol,
+                                                              double *__restrict__ pval)
+{
+    const uint64_t r = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (r >= n_rows) return;
+    const uint64_t b = r / RB, rl = r % RB;
+    const uint64_t *p = pos + b * S * RB + rl;
+    uint32_t cur = 0xffffffffu;
+    uint64_t w = 0;
+    for (uint64_t k = off[r]; k < off[r + 1]; ++k) {  // columns ascend: slices ascend along the row
+        const uint32_t s = (uint32_t)(col[k] / width);
+        if (s != cur) {
+            cur = s;
+            w = p[(uint64_t)s * RB];
+        }
+        prow[w] = (uint16_t)rl;
+        pcol[w] = col[k];
+        pval[w] = val[k];
+        ++w;
+    }
+}
+
+// Group starts: group g begins where its first row slot begins.
+__global__ void k_xphase_goff(uint64_t groups, uint32_t RB, const uint64_t *__restrict__ pos, uint64_t total,
+                              uint64_t *__restrict__ goff)
+{
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < groups) goff[g] = pos[g * RB];
+    if (g == groups) goff[g] = total;
+}
+
+// ---------------------------------------------------------------------------
 // Plan kernels
 // ---------------------------------------------------------------------------
 __global__ void k_off32(const uint64_t *__restrict__ off, uint32_t *__restrict__ o32, uint64_t n)
@@ -513,6 +638,54 @@ void ensure_xslice(spira_dcsr *A)
     P.xs_off = xs_off;  // the slice plan is complete
 }
 
+// (block, slice)-grouped copy of A for XPHASE (one-time per locked matrix).
+void ensure_xphase(spira_dcsr *A)
+{
+    spira_plan &P = A->plan;
+    if (P.ph_goff) return;
+    spira_dev *dev = A->dev;
+    const uint64_t n = A->n_rows;
+    // rows per block: all CUs busy on mid-sized matrices, at most kPhaseRows
+    const uint64_t rb = std::min<uint64_t>(kPhaseRows, std::max<uint64_t>(1, (n + 255) / 256));
+    const uint64_t nb = (n + rb - 1) / rb;
+    const uint64_t xbytes = A->n_cols * sizeof(double);
+    const uint64_t S = std::min<uint64_t>(
+        kPhaseMaxSlices, 8 * std::max<uint64_t>(1, (xbytes + 8 * kPhaseSliceBytes - 1) / (8 * kPhaseSliceBytes)));
+    const uint64_t width = std::max<uint64_t>(1, (A->n_cols + S - 1) / S);
+    const uint64_t N = nb * S * rb;  // count slots
+    if (N >= (1ull << 32) || nb >= (1ull << 31))
+        fail(SPIRA_ERR_UNSUPPORTED, "spmv: matrix too large for XPHASE");
+    uint64_t *pos = (uint64_t *)dev->ws.get(WS_MISC2, (N + 1) * sizeof(uint64_t));
+    k_xphase_count<<<grid_for(nb * rb, kThreads), kThreads, 0, dev->stream>>>(n, nb * rb, (uint32_t)rb, (uint32_t)S,
+                                                                              width, A->off, A->col, pos);
+    SPIRA_HIP(hipGetLastError());
+    scan_excl_u64(dev, pos, pos, N);
+    const uint64_t nz = A->nnz ? A->nnz : 1;
+    P.ph_row = (uint16_t *)plan_alloc(A, nz * sizeof(uint16_t));
+    P.ph_col = (uint32_t *)plan_alloc(A, nz * sizeof(uint32_t));
+    P.ph_val = (double *)plan_alloc(A, nz * sizeof(double));
+    uint64_t *goff = (uint64_t *)plan_alloc(A, (nb * S + 1) * sizeof(uint64_t));
+    k_xphase_scatter<<<grid_for(n, kThreads), kThreads, 0, dev->stream>>>(
+        n, (uint32_t)rb, (uint32_t)S, width, A->off, A->col, A->val, pos, P.ph_row, P.ph_col, P.ph_val);
+    SPIRA_HIP(hipGetLastError());
+    k_xphase_goff<<<grid_for(nb * S + 1, kThreads), kThreads, 0, dev->stream>>>(nb * S, (uint32_t)rb, pos, A->nnz,
+                                                                                goff);
+    SPIRA_HIP(hipGetLastError());
+    dev_sync(dev);  // pos lives in the workspace: finish before it can be reused
+    P.ph_S = (uint32_t)S;
+    P.ph_RB = (uint32_t)rb;
+    P.ph_nb = (uint32_t)nb;
+    P.ph_goff = goff;  // the plan is complete
+}
+
+template <int U>
+void launch_xphase(spira_dcsr *A, const double *x, double *y)
+{
+    const spira_plan &P = A->plan;
+    k_spmv_xphase<U><<<P.ph_nb, kPhaseThreads, 0, A->dev->stream>>>(A->n_rows, P.ph_RB, P.ph_S, P.ph_goff, P.ph_row,
+                                                                      P.ph_col, P.ph_val, x, y);
+}
+
 void launch_xstream(spira_dcsr *A, const double *x, double *y)
 {
     const spira_plan &P = A->plan;
@@ -605,6 +778,10 @@ void spmv_launch_f64(spira_dcsr *A, const double *x, double *y, int variant)
     } else if (variant == SPIRA_SPMV_XSTREAM) {
         ensure_xslice(A);
         launch_xstream(A, x, y);
+    } else if (variant == SPIRA_SPMV_XPHASE || variant == SPIRA_SPMV_XPHASE16) {
+        ensure_xphase(A);
+        if (variant == SPIRA_SPMV_XPHASE) launch_xphase<4>(A, x, y);
+        else launch_xphase<16>(A, x, y);
     } else {
         fail(SPIRA_ERR_INVALID_ARGUMENT, "spmv: unknown kernel variant");
     }

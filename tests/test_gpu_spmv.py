@@ -127,6 +127,20 @@ def test_spmv_auto_and_device_pointers(dev):
     assert 0 <= v < spira_amd.N_VARIANTS
 
 
+def test_xphase_many_slices_and_blocks(dev):
+    """XPHASE with 24 column slices (x of 24 MB, 1 MiB slices) and 256 row
+    blocks of 782 rows: every block walks all slices in its XCD's order and
+    adds into its LDS rows; both launch widths share the plan, and a plan is
+    reused by a later call."""
+    n_rows, n_cols = 200000, 3000000
+    off, col, val = W.random_uniform_rows(n_rows, n_cols, 8, 32, seed=21)
+    A = _upload(dev, off, col, val, n_cols)
+    rng = np.random.default_rng(22)
+    for name in ("xphase", "xphase16", "xphase"):
+        x = rng.uniform(-1, 1, n_cols)
+        _check(A.spmv(x, variant=spira_amd.VARIANTS[name]), off, col, val, x)
+
+
 def test_spmv_laplacian_full_size_exact_row_sums(dev):
     """2048^2 5-point Laplacian (4,194,304 rows): A @ 1 = 4 - (#neighbours),
     exactly, for every variant; and linearity on random vectors."""
