@@ -278,6 +278,7 @@ struct spira_plan {
     uint16_t *ph_row = nullptr;   // row inside its block
     uint32_t *ph_col = nullptr;
     double *ph_val = nullptr;
+    uint32_t *ph_sync = nullptr;  // phase-pacing counters, one per XCD
     uint32_t ph_S = 0, ph_RB = 0, ph_nb = 0;
     int32_t variant = -1;        // pinned AUTO choice (-1 = heuristic)
     std::vector<std::pair<void *, size_t>> owned;  // every plan allocation

@@ -18,6 +18,7 @@
 //   XPHASE    row blocks that keep their y in LDS; the blocks of one XCD walk
 //             L2-sized column slices in the same order (entries grouped by
 //             block and slice), so x gathers hit L2 and y is written once.
+//             XPHASE16 also paces the blocks of an XCD to within two slices.
 //
 // Values and columns are read with non-temporal loads so the x gathers keep
 // L2 and the MALL.  Blocks are renumbered so that each XCD (8 on MI355X, which
@@ -300,8 +301,36 @@ constexpr uint32_t kPhaseRows = 16384;                 // rows per block: 128 Ki
 constexpr uint64_t kPhaseSliceBytes = 1ull << 20;       // x bytes per slice: a quarter of an XCD's L2
 constexpr uint32_t kPhaseMaxSlices = 256;
 constexpr int kPhaseThreads = 1024;
-// Entries per thread per step: XPHASE 4 (banded matrices: few entries per
-// block and slice), XPHASE16 16 (scattered columns: more gathers in flight).
+// XPHASE: 4 entries per thread per step, phases not paced (banded matrices:
+// a block meets few slices).  XPHASE16: 16 entries per thread per step and
+// paced phases (scattered columns: every block meets every slice).
+
+// Phase pacing.  A block may start phase t only after every block of its XCD
+// has finished phase t - 2, so an XCD's blocks gather from at most two slices
+// at a time (they drift apart otherwise, and the XCD's L2 then holds many
+// slices' worth of x).  sync[c * kPhaseSyncStride] counts finished (block,
+// phase) pairs of XCD c; it is zeroed before the launch.  The wait only paces
+// and never decides a result: it gives up after a bounded number of sleeps,
+// and the host passes sync only when every block of the grid can be resident
+// at once (one block per CU).
+constexpr uint32_t kPhaseSyncStride = 16;  // counters 64 bytes apart
+constexpr uint32_t kPhaseSpinMax = 1u << 14;
+
+__device__ __forceinline__ void phase_wait(const uint32_t *ctr, uint32_t target)
+{
+    if (threadIdx.x == 0) {
+        for (uint32_t i = 0; i < kPhaseSpinMax; ++i) {
+            if (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
+            __builtin_amdgcn_s_sleep(4);
+        }
+    }
+    __syncthreads();
+}
+
+__global__ void k_phase_sync_zero(uint32_t *sync)
+{
+    if (threadIdx.x < 8 * kPhaseSyncStride) sync[threadIdx.x] = 0u;
+}
 
 template <int U>
 __global__ __launch_bounds__(kPhaseThreads) void k_spmv_xphase(uint64_t n_rows, uint32_t RB, uint32_t S,
@@ -309,7 +338,8 @@ __global__ __launch_bounds__(kPhaseThreads) void k_spmv_xphase(uint64_t n_rows, 
                                                                 const uint16_t *__restrict__ prow,
                                                                 const uint32_t *__restrict__ pcol,
                                                                 const double *__restrict__ pval,
-                                                                const double *__restrict__ x, double *__restrict__ y)
+                                                                const double *__restrict__ x, double *__restrict__ y,
+                                                                uint32_t *__restrict__ sync)
 {
     __shared__ double s_y[kPhaseRows];
     __shared__ uint64_t s_goff[kPhaseMaxSlices + 1];  // this block's group starts
@@ -319,9 +349,13 @@ __global__ __launch_bounds__(kPhaseThreads) void k_spmv_xphase(uint64_t n_rows, 
     for (uint32_t i = threadIdx.x; i < nr; i += kPhaseThreads) s_y[i] = 0.0;
     for (uint32_t i = threadIdx.x; i <= S; i += kPhaseThreads) s_goff[i] = goff[(uint64_t)b * S + i];
     __syncthreads();
+    const uint32_t xcd = b % 8;
     const uint32_t per = S / 8;  // S is a multiple of 8; XCD c starts at slice c * S / 8
-    const uint32_t first = (b % 8) * per;
+    const uint32_t first = xcd * per;
+    const uint32_t nbx = (gridDim.x - xcd + 7) / 8;  // blocks of this XCD
+    uint32_t *ctr = sync ? sync + xcd * kPhaseSyncStride : nullptr;
     for (uint32_t t = 0; t < S; ++t) {
+        if (ctr && t >= 2) phase_wait(ctr, nbx * (t - 1));
         const uint32_t s = first + t < S ? first + t : first + t - S;
         const uint64_t e0 = s_goff[s], e1 = s_goff[s + 1];
         for (uint64_t k = e0 + threadIdx.x; k < e1; k += (uint64_t)U * kPhaseThreads) {
@@ -343,6 +377,10 @@ __global__ __launch_bounds__(kPhaseThreads) void k_spmv_xphase(uint64_t n_rows, 
                 if (ok[u])
                     __hip_atomic_fetch_add(&s_y[rr[u]], vv[u] * xx[u], __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        if (ctr) {
+            __syncthreads();  // the whole block is done with phase t
+            if (threadIdx.x == 0) __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
     __syncthreads();
@@ -665,6 +703,7 @@ void ensure_xphase(spira_dcsr *A)
     P.ph_col = (uint32_t *)plan_alloc(A, nz * sizeof(uint32_t));
     P.ph_val = (double *)plan_alloc(A, nz * sizeof(double));
     uint64_t *goff = (uint64_t *)plan_alloc(A, (nb * S + 1) * sizeof(uint64_t));
+    P.ph_sync = (uint32_t *)plan_alloc(A, 8 * kPhaseSyncStride * sizeof(uint32_t));
     k_xphase_scatter<<<grid_for(n, kThreads), kThreads, 0, dev->stream>>>(
         n, (uint32_t)rb, (uint32_t)S, width, A->off, A->col, A->val, pos, P.ph_row, P.ph_col, P.ph_val);
     SPIRA_HIP(hipGetLastError());
@@ -678,12 +717,21 @@ void ensure_xphase(spira_dcsr *A)
     P.ph_goff = goff;  // the plan is complete
 }
 
-template <int U>
+// PACE: pace the phases (worth it when every block has entries in every
+// slice; a banded matrix gives each block one or two slices, and pacing would
+// then make the blocks of an XCD take turns).  Only when the whole grid is
+// resident at once (one block per CU).
+template <int U, bool PACE>
 void launch_xphase(spira_dcsr *A, const double *x, double *y)
 {
     const spira_plan &P = A->plan;
+    uint32_t *sync = nullptr;
+    if (PACE && A->dev->n_cu > 0 && P.ph_nb <= (uint32_t)A->dev->n_cu) {
+        sync = P.ph_sync;
+        k_phase_sync_zero<<<1, 8 * kPhaseSyncStride, 0, A->dev->stream>>>(sync);
+    }
     k_spmv_xphase<U><<<P.ph_nb, kPhaseThreads, 0, A->dev->stream>>>(A->n_rows, P.ph_RB, P.ph_S, P.ph_goff, P.ph_row,
-                                                                      P.ph_col, P.ph_val, x, y);
+                                                                      P.ph_col, P.ph_val, x, y, sync);
 }
 
 void launch_xstream(spira_dcsr *A, const double *x, double *y)
@@ -780,8 +828,8 @@ void spmv_launch_f64(spira_dcsr *A, const double *x, double *y, int variant)
         launch_xstream(A, x, y);
     } else if (variant == SPIRA_SPMV_XPHASE || variant == SPIRA_SPMV_XPHASE16) {
         ensure_xphase(A);
-        if (variant == SPIRA_SPMV_XPHASE) launch_xphase<4>(A, x, y);
-        else launch_xphase<16>(A, x, y);
+        if (variant == SPIRA_SPMV_XPHASE) launch_xphase<4, false>(A, x, y);
+        else launch_xphase<16, true>(A, x, y);
     } else {
         fail(SPIRA_ERR_INVALID_ARGUMENT, "spmv: unknown kernel variant");
     }
