@@ -275,11 +275,10 @@ struct spira_plan {
     uint32_t xs_nblk_max = 0;
     // XPHASE: entries grouped by (row block, column slice), rows in order
     uint64_t *ph_goff = nullptr;  // ph_nb*ph_S+1 group starts
-    uint16_t *ph_row = nullptr;   // row inside its block
-    uint32_t *ph_col = nullptr;
+    uint32_t *ph_rc = nullptr;    // row inside its block << 18 | column inside its slice
     double *ph_val = nullptr;
     uint32_t *ph_sync = nullptr;  // phase-pacing counters, one per XCD
-    uint32_t ph_S = 0, ph_RB = 0, ph_nb = 0;
+    uint32_t ph_S = 0, ph_RB = 0, ph_nb = 0, ph_width = 0;
     int32_t variant = -1;        // pinned AUTO choice (-1 = heuristic)
     std::vector<std::pair<void *, size_t>> owned;  // every plan allocation
 };

@@ -288,8 +288,9 @@ __global__ void k_u64_to_u32(const uint64_t *__restrict__ a, uint32_t *__restric
 // XPHASE: a block owns a run of RB <= kPhaseRows rows and keeps their y in
 // LDS for the whole call.  The columns are cut into S slices of at most
 // kPhaseSliceBytes of x; the plan groups the entries by (block, slice), rows
-// in order, each entry carrying its row inside the block (u16).  A block walks
-// the S slices in an order that depends only on its XCD (hardware block b runs
+// in order, each entry carrying its row inside the block and its column inside
+// the slice in one u32 (14 + 18 bits): 12 bytes per entry, as in CSR.  A block
+// walks the S slices in an order that depends only on its XCD (block b runs
 // on XCD b % 8), so the blocks of one XCD gather from the same slice at the
 // same time and that slice stays in the XCD's L2 while A streams past it.
 // Products are added into LDS with native FP64 atomics (the order of the
@@ -300,6 +301,9 @@ __global__ void k_u64_to_u32(const uint64_t *__restrict__ a, uint32_t *__restric
 constexpr uint32_t kPhaseRows = 16384;                 // rows per block: 128 KiB of FP64 y in LDS
 constexpr uint64_t kPhaseSliceBytes = 1ull << 20;       // x bytes per slice: a quarter of an XCD's L2
 constexpr uint32_t kPhaseMaxSlices = 256;
+// An entry's row inside its block and column inside its slice share one u32.
+constexpr uint32_t kPhaseColBits = 18;
+static_assert(kPhaseRows <= (1u << (32 - kPhaseColBits)), "local rows must fit the high bits");
 constexpr int kPhaseThreads = 1024;
 // XPHASE: 4 entries per thread per step, phases not paced (banded matrices:
 // a block meets few slices).  XPHASE16: 16 entries per thread per step and
@@ -335,8 +339,7 @@ __global__ void k_phase_sync_zero(uint32_t *sync)
 template <int U>
 __global__ __launch_bounds__(kPhaseThreads) void k_spmv_xphase(uint64_t n_rows, uint32_t RB, uint32_t S,
                                                                 const uint64_t *__restrict__ goff,
-                                                                const uint16_t *__restrict__ prow,
-                                                                const uint32_t *__restrict__ pcol,
+                                                                uint32_t width, const uint32_t *__restrict__ prc,
                                                                 const double *__restrict__ pval,
                                                                 const double *__restrict__ x, double *__restrict__ y,
                                                                 uint32_t *__restrict__ sync)
@@ -358,6 +361,7 @@ __global__ __launch_bounds__(kPhaseThreads) void k_spmv_xphase(uint64_t n_rows, 
         if (ctr && t >= 2) phase_wait(ctr, nbx * (t - 1));
         const uint32_t s = first + t < S ? first + t : first + t - S;
         const uint64_t e0 = s_goff[s], e1 = s_goff[s + 1];
+        const uint32_t cbase = s * width;
         for (uint64_t k = e0 + threadIdx.x; k < e1; k += (uint64_t)U * kPhaseThreads) {
             uint32_t rr[U], cc[U];
             double vv[U], xx[U];
@@ -366,8 +370,9 @@ __global__ __launch_bounds__(kPhaseThreads) void k_spmv_xphase(uint64_t n_rows, 
             for (int u = 0; u < U; ++u) {
                 const uint64_t q = k + (uint64_t)u * kPhaseThreads;
                 ok[u] = q < e1;
-                rr[u] = ok[u] ? (uint32_t)ld_stream(prow + q) : 0u;
-                cc[u] = ok[u] ? ld_stream(pcol + q) : 0u;  // a masked lane reads x[0] and adds nothing
+                const uint32_t rc = ok[u] ? ld_stream(prc + q) : 0u;  // a masked lane reads x[cbase], adds nothing
+                rr[u] = rc >> kPhaseColBits;
+                cc[u] = cbase + (rc & ((1u << kPhaseColBits) - 1u));
                 vv[u] = ok[u] ? ld_stream(pval + q) : 0.0;
             }
 #pragma unroll
@@ -411,7 +416,7 @@ __global__ __launch_bounds__(kThreads) void k_xphase_scatter(uint64_t n_rows, ui
                                                               const uint32_t *__restrict__ col,
                                                               const double *__restrict__ val,
                                                               const uint64_t *__restrict__ pos,
-                                                              uint16_t *__restrict__ prow, uint32_t *__restrict__ pcol,
+                                                              uint32_t *__restrict__ prc,
                                                               double *__restrict__ pval)
 {
     const uint64_t r = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
@@ -426,8 +431,7 @@ __global__ __launch_bounds__(kThreads) void k_xphase_scatter(uint64_t n_rows, ui
             cur = s;
             w = p[(uint64_t)s * RB];
         }
-        prow[w] = (uint16_t)rl;
-        pcol[w] = col[k];
+        prc[w] = ((uint32_t)rl << kPhaseColBits) | (uint32_t)(col[k] - (uint64_t)s * width);
         pval[w] = val[k];
         ++w;
     }
@@ -690,6 +694,7 @@ void ensure_xphase(spira_dcsr *A)
     const uint64_t S = std::min<uint64_t>(
         kPhaseMaxSlices, 8 * std::max<uint64_t>(1, (xbytes + 8 * kPhaseSliceBytes - 1) / (8 * kPhaseSliceBytes)));
     const uint64_t width = std::max<uint64_t>(1, (A->n_cols + S - 1) / S);
+    if (width > (1ull << kPhaseColBits)) fail(SPIRA_ERR_UNSUPPORTED, "spmv: too many columns for XPHASE");
     const uint64_t N = nb * S * rb;  // count slots
     if (N >= (1ull << 32) || nb >= (1ull << 31))
         fail(SPIRA_ERR_UNSUPPORTED, "spmv: matrix too large for XPHASE");
@@ -699,19 +704,19 @@ void ensure_xphase(spira_dcsr *A)
     SPIRA_HIP(hipGetLastError());
     scan_excl_u64(dev, pos, pos, N);
     const uint64_t nz = A->nnz ? A->nnz : 1;
-    P.ph_row = (uint16_t *)plan_alloc(A, nz * sizeof(uint16_t));
-    P.ph_col = (uint32_t *)plan_alloc(A, nz * sizeof(uint32_t));
+    P.ph_rc = (uint32_t *)plan_alloc(A, nz * sizeof(uint32_t));
     P.ph_val = (double *)plan_alloc(A, nz * sizeof(double));
     uint64_t *goff = (uint64_t *)plan_alloc(A, (nb * S + 1) * sizeof(uint64_t));
     P.ph_sync = (uint32_t *)plan_alloc(A, 8 * kPhaseSyncStride * sizeof(uint32_t));
     k_xphase_scatter<<<grid_for(n, kThreads), kThreads, 0, dev->stream>>>(
-        n, (uint32_t)rb, (uint32_t)S, width, A->off, A->col, A->val, pos, P.ph_row, P.ph_col, P.ph_val);
+        n, (uint32_t)rb, (uint32_t)S, width, A->off, A->col, A->val, pos, P.ph_rc, P.ph_val);
     SPIRA_HIP(hipGetLastError());
     k_xphase_goff<<<grid_for(nb * S + 1, kThreads), kThreads, 0, dev->stream>>>(nb * S, (uint32_t)rb, pos, A->nnz,
                                                                                 goff);
     SPIRA_HIP(hipGetLastError());
     dev_sync(dev);  // pos lives in the workspace: finish before it can be reused
     P.ph_S = (uint32_t)S;
+    P.ph_width = (uint32_t)width;
     P.ph_RB = (uint32_t)rb;
     P.ph_nb = (uint32_t)nb;
     P.ph_goff = goff;  // the plan is complete
@@ -730,8 +735,8 @@ void launch_xphase(spira_dcsr *A, const double *x, double *y)
         sync = P.ph_sync;
         k_phase_sync_zero<<<1, 8 * kPhaseSyncStride, 0, A->dev->stream>>>(sync);
     }
-    k_spmv_xphase<U><<<P.ph_nb, kPhaseThreads, 0, A->dev->stream>>>(A->n_rows, P.ph_RB, P.ph_S, P.ph_goff, P.ph_row,
-                                                                      P.ph_col, P.ph_val, x, y, sync);
+    k_spmv_xphase<U><<<P.ph_nb, kPhaseThreads, 0, A->dev->stream>>>(A->n_rows, P.ph_RB, P.ph_S, P.ph_goff, P.ph_width,
+                                                                      P.ph_rc, P.ph_val, x, y, sync);
 }
 
 void launch_xstream(spira_dcsr *A, const double *x, double *y)
