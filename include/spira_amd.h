@@ -170,6 +170,19 @@ int spira_spmv_autotune_f64(spira_dcsr *A, const double *x_dev, double *y_dev, i
  * built ("rebalance: matrix must be locked"). */
 int spira_partition_boundaries(spira_dcsr *A, uint64_t n_threads, uint64_t *boundaries);
 
+/* ---- transpose ----------------------------------------------------------------
+ * Replaces serial::algorithms::transpose (serial/transpose.hpp:35-157): out
+ * (a handle of A's shape with rows and columns swapped, in any state; A itself
+ * when A is square) receives A^T, built: every row lists its columns ascending,
+ * values copied as stored (explicit zeros kept).  With
+ * SPIRA_TRANSPOSE_DROP_ZEROS it replaces parallel::algorithms::transpose
+ * (parallel/algorithms/transpose.hpp:37-69), whose final lock() removes
+ * explicit zeros.  SPIRA_ERR_LOGIC if A is not built ("transpose: matrix must
+ * be locked"), SPIRA_ERR_INVALID_ARGUMENT for a wrong shape or device;
+ * nnz, rows and columns below 2^32. */
+#define SPIRA_TRANSPOSE_DROP_ZEROS 1u
+int spira_transpose_f64(spira_dcsr *A, spira_dcsr *out, uint32_t flags);
+
 /* ---- mixed precision ------------------------------------------------------
  * No counterpart in the reference (its float instantiation,
  * serial/spmv.hpp:90-135, stores and accumulates in FP32).  y = A*x with A's

@@ -282,6 +282,21 @@ public:
     // the device objects
     [[nodiscard]] spira_dcsr *device_handle() const noexcept { return csr_->get(); }
     [[nodiscard]] const device &dev() const noexcept { return *dev_; }
+    [[nodiscard]] std::shared_ptr<device> device_ptr() const noexcept { return dev_; }
+
+    // A^T into out (a matrix of the swapped shape, or *this when square) on
+    // the device; out ends locked with the transposed CSR committed and no
+    // pending inserts.  flags: SPIRA_TRANSPOSE_DROP_ZEROS for the parallel form.
+    void transpose_into(matrix &out, uint32_t flags) const
+    {
+        check(spira_transpose_f64(csr_->get(), out.csr_->get(), flags));
+        for (auto &b : out.buf_) b.clear();
+        std::fill(out.dirty_.begin(), out.dirty_.end(), uint8_t{0});
+        out.mode_ = matrix_mode::locked;
+        out.csr_built_ = true;
+        out.committed_nnz_ = out.csr_->nnz();
+        out.host_.reset();
+    }
 
     // checks in serial/spmv.hpp:150-158 order, then the device SpMV
     void spmv_into(const std::vector<value_type> &x, std::vector<value_type> &y, const char *x_msg,
@@ -456,6 +471,27 @@ inline void spmv(const matrix &mat, const std::vector<double> &x, std::vector<do
     mat.spmv_into(x, y, "The size of the input vector x does not match the number of columns of the matrix",
                   "The size of the output vector y does not match the number of rows of the matrix");
 }
+
+// serial/transpose.hpp:35-157: a new locked matrix holding A^T (built on the
+// device; rows sorted, values as stored).
+inline matrix transpose(const matrix &mat)
+{
+    if (!mat.is_locked()) throw std::logic_error("transpose: matrix must be locked");
+    matrix out(mat.n_cols(), mat.n_rows(), mat.device_ptr());
+    mat.transpose_into(out, 0);
+    return out;
+}
+
+// serial/transpose.hpp:158-173: open, square; lock, transpose, open again.
+inline void transpose_itself(matrix &mat)
+{
+    if (!mat.is_open()) throw std::logic_error("transpose_itself: matrix must be open");
+    const auto [r, c] = mat.shape();
+    if (r != c) throw std::logic_error("in-place transpose requires square matrix");
+    mat.lock();
+    mat.transpose_into(mat, 0);
+    mat.open();
+}
 }  // namespace serial::algorithms
 
 namespace parallel::algorithms {
@@ -465,6 +501,17 @@ inline void spmv(const parallel_matrix &mat, const std::vector<double> &x, std::
 {
     mat.spmv_into(x, y, "spmv: x size does not match matrix column count",
                   "spmv: y size does not match matrix row count");
+}
+
+// parallel/algorithms/transpose.hpp:37-69: the output (same thread count,
+// uniform partitions) ends locked; its lock() would drop explicit zeros, and
+// so does the device transpose here.
+inline parallel_matrix transpose(parallel_matrix &mat)
+{
+    if (!mat.is_locked()) throw std::logic_error("transpose: matrix must be locked");
+    parallel_matrix out(mat.n_cols(), mat.n_rows(), mat.n_threads(), mat.device_ptr());
+    mat.transpose_into(out, SPIRA_TRANSPOSE_DROP_ZEROS);
+    return out;
 }
 }  // namespace parallel::algorithms
 

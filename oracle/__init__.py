@@ -18,6 +18,8 @@ and cites the reference code it restates:
 ``Pool``             parallel_matrix.hpp:289-405, thread_pool.hpp:102-134,
                      parallel/algorithms/spmv.hpp:127-161
 ``partition_bounds`` partition.hpp:78-126
+``transpose``        serial/transpose.hpp:35-157; with drop_zeros,
+                     parallel/algorithms/transpose.hpp:37-69 (insert + lock)
 ===================  ===========================================================
 """
 from __future__ import annotations
@@ -75,6 +77,8 @@ def lib():
             "spira_or_pool_nnz": (ctypes.c_uint64, [ctypes.c_void_p]),
             "spira_or_pool_export_csr": (ctypes.c_int, [ctypes.c_void_p, _u64p, _u32p, _f64p]),
             "spira_or_partition_boundaries": (None, [ctypes.c_uint64, _u64p, ctypes.c_uint64, _u64p]),
+            "spira_or_transpose": (ctypes.c_uint64, [ctypes.c_uint64, ctypes.c_uint64, _u64p, _u32p, _f64p,
+                                                     ctypes.c_int, _u64p, _u32p, _f64p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -201,6 +205,19 @@ def partition_bounds(row_nnz, n_threads):
     b = np.zeros(n_threads + 1, np.uint64)
     lib().spira_or_partition_boundaries(rn.size, _p(rn, _u64p), n_threads, _p(b, _u64p))
     return b
+
+
+def transpose(off, col, val, n_cols, drop_zeros=False):
+    """CSR of A^T (n_cols rows): columns ascending in every row, values as stored
+    (serial); drop_zeros=True also removes explicit zeros (parallel)."""
+    o, c, v = _u64(off), _u32(col), _f64(val)
+    n = o.size - 1
+    t_off = np.zeros(int(n_cols) + 1, np.uint64)
+    t_col = np.zeros(max(c.size, 1), np.uint32)
+    t_val = np.zeros(max(c.size, 1), np.float64)
+    nz = lib().spira_or_transpose(n, int(n_cols), _p(o, _u64p), _p(c, _u32p), _p(v, _f64p), 1 if drop_zeros else 0,
+                                  _p(t_off, _u64p), _p(t_col, _u32p), _p(t_val, _f64p))
+    return t_off, t_col[:nz].copy(), t_val[:nz].copy()
 
 
 def normwise_rel_err(y, y_ref_hi, y_ref_lo=None):

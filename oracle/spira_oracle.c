@@ -292,6 +292,31 @@ void spira_or_partition_boundaries(uint64_t n_rows, const uint64_t *row_nnz,
     free(prefix);
 }
 
+uint64_t spira_or_transpose(uint64_t n_rows, uint64_t n_cols, const uint64_t *off,
+                            const uint32_t *col, const double *val, int drop_zeros,
+                            uint64_t *t_off, uint32_t *t_col, double *t_val)
+{
+    /* pass 1: histogram of the column indices */
+    for (uint64_t j = 0; j <= n_cols; ++j) t_off[j] = 0;
+    const uint64_t nnz = off[n_rows];
+    for (uint64_t k = 0; k < nnz; ++k)
+        if (!drop_zeros || !spira_or_is_zero(val[k])) t_off[(uint64_t)col[k] + 1] += 1;
+    for (uint64_t j = 0; j < n_cols; ++j) t_off[j + 1] += t_off[j];
+    /* pass 2: fill row by row through a cursor per output row */
+    uint64_t *cursor = (uint64_t *)malloc((n_cols ? n_cols : 1) * sizeof(uint64_t));
+    for (uint64_t j = 0; j < n_cols; ++j) cursor[j] = t_off[j];
+    for (uint64_t i = 0; i < n_rows; ++i) {
+        for (uint64_t k = off[i]; k < off[i + 1]; ++k) {
+            if (drop_zeros && spira_or_is_zero(val[k])) continue;
+            const uint64_t pos = cursor[col[k]]++;
+            t_col[pos] = (uint32_t)i;
+            t_val[pos] = val[k];
+        }
+    }
+    free(cursor);
+    return t_off[n_cols];
+}
+
 /* ------------------------------------------------------------------------ */
 /* Threaded baseline                                                         */
 /* ------------------------------------------------------------------------ */

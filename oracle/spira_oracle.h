@@ -113,6 +113,15 @@ int spira_or_pool_export_csr(const spira_or_pool *p, uint64_t *off,
 void spira_or_partition_boundaries(uint64_t n_rows, const uint64_t *row_nnz,
                                    uint64_t n_threads, uint64_t *boundaries);
 
+/* Transpose of a CSR (serial/transpose.hpp:35-157: column histogram, offsets,
+ * then a fill in ascending input-row order, so output rows are sorted).  With
+ * drop_zeros, parallel/algorithms/transpose.hpp:37-69: the same entries
+ * inserted into a new matrix whose lock() removes explicit zeros.  t_off holds
+ * n_cols + 1 entries, t_col / t_val the input's nnz; returns the output nnz. */
+uint64_t spira_or_transpose(uint64_t n_rows, uint64_t n_cols, const uint64_t *off,
+                            const uint32_t *col, const double *val, int drop_zeros,
+                            uint64_t *t_off, uint32_t *t_col, double *t_val);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,7 +31,7 @@ import numpy as np
 __all__ = [
     "Device", "DeviceCSR", "Matrix", "ParallelMatrix", "spmv", "lib", "lib_path",
     "InvalidArgument", "LogicError", "OutOfRange", "UnsupportedInput", "DeviceError",
-    "VARIANTS", "SPMV_AUTO",
+    "VARIANTS", "SPMV_AUTO", "transpose", "transpose_itself",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -67,6 +67,7 @@ BATCH_XLDS_TWO_PASS_COLS = 65536
 BATCH_MAX_PLOG2 = 5
 BATCH_SCHED_DTYPE = np.dtype([("plog2", "u1"), ("xlds", "u1"), ("reserved", "<u2"), ("item_nnz", "<u4")])
 LOCK_MAX_ROW_INSERTS = 8192
+TRANSPOSE_DROP_ZEROS = 1
 
 
 class SpiraError(Exception):
@@ -169,6 +170,7 @@ def lib():
             "spira_spmv_choose": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
             "spira_spmv_set_variant": (ctypes.c_int, [_vp, ctypes.c_int]),
             "spira_partition_boundaries": (ctypes.c_int, [_vp, ctypes.c_uint64, _vp]),
+            "spira_transpose_f64": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32]),
             "spira_stage_create": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(_vp)]),
             "spira_stage_destroy": (None, [_vp]),
             "spira_stage_append": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_uint64, ctypes.c_uint32]),
@@ -412,6 +414,15 @@ class DeviceCSR:
         b = np.zeros(int(n_threads) + 1, np.uint64)
         self.dev.check(lib().spira_partition_boundaries(self._h, int(n_threads), _ptr(b)))
         return b
+
+    def transpose(self, out: "DeviceCSR" = None, drop_zeros: bool = False) -> "DeviceCSR":
+        """A^T on the device (serial/transpose.hpp:35-157; drop_zeros: the
+        parallel form, parallel/algorithms/transpose.hpp:37-69).  out: a handle
+        of the swapped shape (default: a new one); self for square in place."""
+        if out is None:
+            out = DeviceCSR(self.dev, self.n_cols, self.n_rows)
+        self.dev.check(lib().spira_transpose_f64(self._h, out._h, TRANSPOSE_DROP_ZEROS if drop_zeros else 0))
+        return out
 
     def choose_variant(self) -> int:
         v = ctypes.c_int()
@@ -936,6 +947,36 @@ class ParallelMatrix(Matrix):
         off, col, val = h
         base = int(off[a])
         return (off[a:b + 1] - np.uint64(base), col[base:int(off[b])], val[base:int(off[b])])
+
+
+def transpose(mat: Matrix) -> Matrix:
+    """serial::algorithms::transpose (serial/transpose.hpp:35-157) for a Matrix,
+    parallel::algorithms::transpose (parallel/algorithms/transpose.hpp:37-69:
+    explicit zeros dropped by its lock()) for a ParallelMatrix: a new locked
+    matrix of the swapped shape, built on the device."""
+    if not mat.is_locked():
+        raise LogicError("transpose: matrix must be locked")
+    if isinstance(mat, ParallelMatrix):
+        out = ParallelMatrix(mat.n_cols(), mat.n_rows(), mat.n_threads(), mat.dev)
+    else:
+        out = Matrix(mat.n_cols(), mat.n_rows(), mat.dev)
+    mat.device_csr().transpose(out.device_csr(), drop_zeros=isinstance(mat, ParallelMatrix))
+    out._locked = True
+    return out
+
+
+def transpose_itself(mat: Matrix):
+    """serial/transpose.hpp:158-173: an open square matrix is locked,
+    transposed in place on the device and opened again."""
+    if not mat.is_open():
+        raise LogicError("transpose_itself: matrix must be open")
+    r, c = mat.shape()
+    if r != c:
+        raise LogicError("in-place transpose requires square matrix")
+    mat.lock()
+    mat.device_csr().transpose(mat.device_csr())
+    mat._host = None
+    mat.open()
 
 
 def spmv(mat: Matrix, x, y, variant: int = SPMV_AUTO):

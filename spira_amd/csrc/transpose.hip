@@ -1,0 +1,182 @@
+// transpose.hip -- A^T of a locked device CSR, bit-identical to the
+// reference's serial transpose (serial/transpose.hpp:35-157: column histogram,
+// offsets, then a fill in ascending input-row order, so every output row lists
+// its columns ascending; values copied as stored, explicit zeros included) or,
+// with SPIRA_TRANSPOSE_DROP_ZEROS, to the parallel one
+// (parallel/algorithms/transpose.hpp:37-69: the same entries inserted into a
+// new matrix whose lock() removes explicit zeros).
+//
+//   1. k_transpose_keys: per entry its input row (the block's row range from a
+//      wavefront-wide search, then a search inside it), its key = column (or
+//      the sentinel n_cols for a dropped zero) and its index.
+//   2. A stable LSD radix sort (rocPRIM) of (key, index): entries come out
+//      grouped by output row and, inside a row, in input-entry order, which is
+//      input-row order -- the reference's fill order.
+//   3. k_transpose_offsets: output offsets = lower bounds of each row in the
+//      sorted keys (no atomics, no second pass over A).
+//   4. k_transpose_gather: column = input row, value = input value.
+#include "internal.hpp"
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+namespace spira_amd {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kItems = 4;  // entries per thread in the key pass
+
+__global__ __launch_bounds__(kThreads) void k_transpose_keys(uint64_t n_rows, uint64_t nnz, uint64_t n_cols,
+                                                              const uint64_t *__restrict__ off,
+                                                              const uint32_t *__restrict__ col,
+                                                              const double *__restrict__ val, int drop_zeros,
+                                                              uint32_t *__restrict__ rid, uint32_t *__restrict__ key,
+                                                              uint32_t *__restrict__ idx)
+{
+    const uint64_t first = (uint64_t)blockIdx.x * kThreads * kItems;
+    if (first >= nnz) return;  // whole blocks leave together: the searches below need every lane
+    const uint64_t last = min(first + (uint64_t)kThreads * kItems, nnz) - 1;
+    const uint64_t lo = wave_row_of(off, 0, n_rows, first);
+    const uint64_t hi = wave_row_of(off, lo, n_rows, last);
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+        const uint64_t k = first + (uint64_t)it * kThreads + threadIdx.x;
+        if (k >= nnz) break;
+        rid[k] = (uint32_t)row_of(off, lo, hi + 1, k);
+        key[k] = (drop_zeros && is_zero_f64(val[k])) ? (uint32_t)n_cols : col[k];
+        idx[k] = (uint32_t)k;
+    }
+}
+
+// t_off[r] = number of sorted keys below r, r = 0 .. n_cols (dropped zeros
+// carry the key n_cols and sort last, so t_off[n_cols] is the output nnz).
+__global__ void k_transpose_offsets(const uint32_t *__restrict__ skey, uint64_t nnz, uint64_t n_cols,
+                                    uint64_t *__restrict__ t_off)
+{
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r <= n_cols;
+         r += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t a = 0, b = nnz;
+        while (a < b) {
+            const uint64_t mid = a + (b - a) / 2;
+            if ((uint64_t)skey[mid] < r) a = mid + 1;
+            else b = mid;
+        }
+        t_off[r] = a;
+    }
+}
+
+__global__ void k_transpose_gather(const uint32_t *__restrict__ sidx, const uint64_t *__restrict__ t_off,
+                                   uint64_t n_cols, const uint32_t *__restrict__ rid,
+                                   const double *__restrict__ val, uint32_t *__restrict__ t_col,
+                                   double *__restrict__ t_val)
+{
+    const uint64_t nz = t_off[n_cols];
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nz;
+         p += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t k = sidx[p];
+        t_col[p] = rid[k];
+        t_val[p] = val[k];
+    }
+}
+
+unsigned key_bits(uint64_t max_key)
+{
+    unsigned b = 1;
+    while (b < 32 && (1ull << b) <= max_key) ++b;
+    return b;
+}
+
+}  // namespace
+
+}  // namespace spira_amd
+
+using namespace spira_amd;
+
+extern "C" int spira_transpose_f64(spira_dcsr *A, spira_dcsr *out, uint32_t flags)
+{
+    if (!A || !out) return SPIRA_ERR_INVALID_ARGUMENT;
+    spira_dev *dev = A->dev;
+    try {
+        SPIRA_HIP(hipSetDevice(dev->device));
+        if (out->dev != dev) fail(SPIRA_ERR_INVALID_ARGUMENT, "transpose: A and out belong to different devices");
+        if (!A->built) fail(SPIRA_ERR_LOGIC, "transpose: matrix must be locked");
+        if (out->n_rows != A->n_cols || out->n_cols != A->n_rows)
+            fail(SPIRA_ERR_INVALID_ARGUMENT, "transpose: out must have A's shape with rows and columns swapped");
+        if (flags & ~(uint32_t)SPIRA_TRANSPOSE_DROP_ZEROS) fail(SPIRA_ERR_INVALID_ARGUMENT, "transpose: unknown flags");
+        if (A->nnz >= (1ull << 32) || A->n_rows >= (1ull << 32) || A->n_cols >= (1ull << 32) - 1)
+            fail(SPIRA_ERR_UNSUPPORTED, "transpose: needs nnz, rows and columns below 2^32");
+        const bool drop = (flags & SPIRA_TRANSPOSE_DROP_ZEROS) != 0;
+        hipStream_t st = dev->stream;
+        const uint64_t n = A->n_rows, nc = A->n_cols, nnz = A->nnz;
+        const size_t ob = (nc + 1) * sizeof(uint64_t), cb = (nnz ? nnz : 1) * sizeof(uint32_t),
+                     vb = (nnz ? nnz : 1) * sizeof(double), kb = (nnz ? nnz : 1) * sizeof(uint32_t);
+        uint64_t *t_off = (uint64_t *)dev_alloc(dev, ob);
+        uint32_t *t_col = nullptr, *rid = nullptr, *key = nullptr, *skey = nullptr, *idx = nullptr, *sidx = nullptr;
+        double *t_val = nullptr;
+        auto release_tmp = [&] {
+            if (rid) dev_release(dev, rid, kb);
+            if (key) dev_release(dev, key, kb);
+            if (skey) dev_release(dev, skey, kb);
+            if (idx) dev_release(dev, idx, kb);
+            if (sidx) dev_release(dev, sidx, kb);
+            rid = key = skey = idx = sidx = nullptr;
+        };
+        try {
+            t_col = (uint32_t *)dev_alloc(dev, cb);
+            t_val = (double *)dev_alloc(dev, vb);
+            rid = (uint32_t *)dev_alloc(dev, kb);
+            key = (uint32_t *)dev_alloc(dev, kb);
+            skey = (uint32_t *)dev_alloc(dev, kb);
+            idx = (uint32_t *)dev_alloc(dev, kb);
+            sidx = (uint32_t *)dev_alloc(dev, kb);
+            if (nnz > 0) {
+                k_transpose_keys<<<grid_for(nnz, kThreads * kItems), kThreads, 0, st>>>(
+                    n, nnz, nc, A->off, A->col, A->val, drop ? 1 : 0, rid, key, idx);
+                SPIRA_HIP(hipGetLastError());
+                const unsigned bits = key_bits(nc);  // keys 0 .. n_cols (the sentinel included)
+                size_t tmp_bytes = 0;
+                SPIRA_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, key, skey, idx, sidx, (unsigned)nnz, 0, bits,
+                                                    st));
+                void *tmp = dev->ws.get(WS_SORT, tmp_bytes ? tmp_bytes : 8);
+                SPIRA_HIP(rocprim::radix_sort_pairs(tmp, tmp_bytes, key, skey, idx, sidx, (unsigned)nnz, 0, bits, st));
+            }
+            k_transpose_offsets<<<grid_for(nc + 1, kThreads, 8192), kThreads, 0, st>>>(skey, nnz, nc, t_off);
+            SPIRA_HIP(hipGetLastError());
+            if (nnz > 0) {
+                k_transpose_gather<<<grid_for(nnz, kThreads, 16384), kThreads, 0, st>>>(sidx, t_off, nc, rid, A->val,
+                                                                                        t_col, t_val);
+                SPIRA_HIP(hipGetLastError());
+            }
+            const uint64_t nnz_out = read_scalar(dev, t_off + nc);
+            release_tmp();
+            // install (out may be A itself for a square matrix: everything above
+            // has been read already, and stream order protects the old arrays)
+            const bool zeros = drop ? false : A->has_zeros;
+            free_dcsr_arrays(out);
+            out->off = t_off;
+            out->col = t_col;
+            out->val = t_val;
+            out->nnz = nnz_out;
+            out->capacity = nnz;
+            out->built = true;
+            out->has_zeros = zeros;
+        } catch (...) {
+            (void)hipStreamSynchronize(st);
+            release_tmp();
+            dev_release(dev, t_off, ob);
+            if (t_col) dev_release(dev, t_col, cb);
+            if (t_val) dev_release(dev, t_val, vb);
+            throw;
+        }
+        return SPIRA_OK;
+    } catch (const status_error &e) {
+        dev->last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        dev->last_error = "host allocation failed";
+        return SPIRA_ERR_NOMEM;
+    } catch (const std::exception &e) {
+        dev->last_error = e.what();
+        return SPIRA_ERR_DEVICE;
+    }
+}

@@ -13,6 +13,10 @@
 //   partition t                              -> part start end ; offs ; cols ; vals
 //   bounds                                   -> bounds b...
 //   load_csr n nnz offs... cols... vals...   -> ok
+//   transpose                                -> ok  (the matrix becomes its transpose:
+//                                               serial::algorithms::transpose, or
+//                                               parallel::algorithms::transpose)
+//   transpose_itself                         -> ok
 // Any exception -> "error <kind> <message>".
 #include "spira_amd.hpp"
 
@@ -174,6 +178,18 @@ std::string run(const std::string &line)
             v = parse_double(s);
         }
         mat().load_csr(std::move(h));
+        out << "ok";
+    } else if (cmd == "transpose") {
+        if (g_par) {
+            auto p = std::make_unique<spira_amd::parallel_matrix>(spira_amd::parallel::algorithms::transpose(*g_par));
+            g_par = p.get();
+            g_mat = std::move(p);
+        } else {
+            g_mat = std::make_unique<spira_amd::matrix>(spira_amd::serial::algorithms::transpose(mat()));
+        }
+        out << "ok";
+    } else if (cmd == "transpose_itself") {
+        spira_amd::serial::algorithms::transpose_itself(mat());
         out << "ok";
     } else {
         out << "error driver unknown command " << cmd;

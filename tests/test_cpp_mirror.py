@@ -301,3 +301,94 @@ def test_load_csr_validates_and_locks(drv):
         m.insert(0, 0, 1.0)  # locked
     with pytest.raises(IndexError):  # (the driver holds one matrix: this replaces m)
         CppMatrix(drv, 2, 3).get(2, 0)
+
+
+def _cpp_from_case(drv, case):
+    m = CppMatrix(drv, case["n_rows"], case["n_cols"],
+                  threads=case.get("n_threads") if case.get("mode", "serial") != "serial" else None)
+    for r, c, v in case["inserts"]:
+        m.insert(r, c, v)
+    return m
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", KATS["transpose"], ids=lambda c: c["source"])
+def test_transpose_known_answers_on_cpp_mirror(drv, case):
+    """serial::algorithms::transpose / parallel::algorithms::transpose through
+    the C++ mirror: the reference's known answers and the oracle's CSR."""
+    m = _cpp_from_case(drv, case)
+    m.lock()
+    before = m.csr()
+    drv.cmd("transpose")
+    shape = (case["n_cols"], case["n_rows"])
+    if case.get("twice"):
+        drv.cmd("transpose")
+        shape = (case["n_rows"], case["n_cols"])
+    assert m.is_locked()
+    for r, c, v in case["expect"]:
+        assert m.get(r, c) == v
+    for r, c in case.get("expect_zero", []):
+        assert not m.contains(r, c)
+    if "nnz" in case:
+        assert m.nnz() == case["nnz"]
+    got = m.csr()
+    assert len(got[0]) == shape[0] + 1
+    if not case.get("twice"):
+        exp = oracle.transpose(*before, case["n_cols"], drop_zeros=case["mode"] != "serial")
+        assert all(np.array_equal(a, b) for a, b in zip(got, exp))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", KATS["transpose_itself"], ids=lambda c: c["source"])
+def test_transpose_itself_known_answers_on_cpp_mirror(drv, case):
+    m = _cpp_from_case(drv, case)
+    if case.get("error"):
+        with pytest.raises(RuntimeError) as e:  # std::logic_error
+            drv.cmd("transpose_itself")
+        assert str(e.value) == "in-place transpose requires square matrix"
+        return
+    drv.cmd("transpose_itself")
+    assert not m.is_locked()
+    if case.get("twice"):
+        drv.cmd("transpose_itself")
+    for r, c, v in case["expect"]:
+        assert m.get(r, c) == v
+    for r, c in case.get("expect_zero", []):
+        assert not m.contains(r, c)
+
+
+@pytest.mark.gpu
+def test_transpose_checks_and_zeros_on_cpp_mirror(drv):
+    m = CppMatrix(drv, 3, 4)
+    with pytest.raises(RuntimeError) as e:
+        drv.cmd("transpose")
+    assert str(e.value) == "transpose: matrix must be locked"
+    m.lock()
+    with pytest.raises(RuntimeError) as e:
+        drv.cmd("transpose_itself")
+    assert str(e.value) == "transpose_itself: matrix must be open"
+    # load_csr keeps an explicit zero; the serial transpose keeps it too
+    m = CppMatrix(drv, 2, 3)
+    m.load_csr([0, 2, 3], [0, 2, 1], [0.0, 4.0, -1.5])
+    drv.cmd("transpose")
+    off, col, val = m.csr()
+    assert off.tolist() == [0, 1, 2, 3] and col.tolist() == [0, 1, 0] and val.tolist() == [0.0, -1.5, 4.0]
+    assert m.contains(0, 0) and m.get(2, 0) == 4.0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("parallel", [False, True])
+def test_transpose_random_matches_oracle_on_cpp_mirror(drv, parallel):
+    rng = np.random.default_rng(17)
+    n_rows, n_cols = 700, 450
+    m = CppMatrix(drv, n_rows, n_cols, threads=4 if parallel else None)
+    for _ in range(8000):
+        m.insert(int(rng.integers(0, n_rows)), int(rng.integers(0, n_cols)), float(rng.uniform(-1, 1)))
+    m.lock()
+    before = m.csr()
+    drv.cmd("transpose")
+    got = m.csr()
+    exp = oracle.transpose(*before, n_cols, drop_zeros=parallel)
+    assert all(np.array_equal(a, b) for a, b in zip(got, exp))
+    if parallel:
+        assert m.bounds() == [t * n_cols // 4 for t in range(5)]

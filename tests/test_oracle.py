@@ -315,3 +315,107 @@ def test_planted_transposed_spmv_is_caught():
             dense[r, col[k]] = val[k]
     xt = np.array(case["x"][:3])
     assert (dense.T @ xt)[:3].tolist() != case["y"]
+
+
+# ---- transpose (serial/transpose.hpp:35-157, parallel/algorithms/transpose.hpp:37-69) ----
+
+def _kat_csr(case):
+    ins = case["inserts"]
+    ptr, c, v = stage(case["n_rows"], [t[0] for t in ins], [t[1] for t in ins], [t[2] for t in ins])
+    return oracle.lock(ptr, c, v)
+
+
+def _dense_get(off, col, val, r, c):
+    lo, hi = int(off[r]), int(off[r + 1])
+    hit = np.nonzero(col[lo:hi] == c)[0]
+    return float(val[lo + hit[0]]) if hit.size else 0.0
+
+
+@pytest.mark.parametrize("case", KATS["transpose"], ids=lambda c: c["source"])
+def test_transpose_golden(case):
+    off, col, val = _kat_csr(case)
+    drop = case["mode"] != "serial"
+    t = oracle.transpose(off, col, val, case["n_cols"], drop_zeros=drop)
+    shape = (case["n_cols"], case["n_rows"])
+    if case.get("twice"):
+        t = oracle.transpose(*t, case["n_rows"], drop_zeros=drop)
+        shape = (case["n_rows"], case["n_cols"])
+    assert len(t[0]) == shape[0] + 1
+    for r, c, v in case["expect"]:
+        assert _dense_get(*t, r, c) == v
+    for r, c in case.get("expect_zero", []):
+        assert _dense_get(*t, r, c) == 0.0
+    if "nnz" in case:
+        assert int(t[0][-1]) == case["nnz"]
+    for r in range(shape[0]):  # output rows are sorted (and hold no duplicates)
+        seg = t[1][int(t[0][r]):int(t[0][r + 1])]
+        assert np.all(np.diff(seg.astype(np.int64)) > 0)
+    if case["mode"] == "parallel_vs_serial":
+        s = oracle.transpose(off, col, val, case["n_cols"], drop_zeros=False)
+        assert csr_equal(t, s)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_transpose_matches_scipy(seed):
+    """scipy.sparse's CSR transpose (library routine) on random shapes, with
+    empty rows and columns and explicit zeros kept by the serial form."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(900 + seed)
+    n_rows, n_cols = int(rng.integers(1, 300)), int(rng.integers(1, 300))
+    dense = rng.standard_normal((n_rows, n_cols)) * (rng.random((n_rows, n_cols)) < rng.uniform(0.0, 0.3))
+    dense[rng.random((n_rows, n_cols)) < 0.02] = 0.0
+    m = sp.csr_matrix(dense)
+    off, col, val = m.indptr.astype(np.uint64), m.indices.astype(np.uint32), m.data.astype(np.float64)
+    if val.size:  # explicit zeros (as load_csr can bring in)
+        val[rng.random(val.size) < 0.1] = 0.0
+    ref = sp.csr_matrix((val, col, off), shape=(n_rows, n_cols)).T.tocsr()
+    ref.sort_indices()
+    t = oracle.transpose(off, col, val, n_cols)
+    assert np.array_equal(t[0], ref.indptr.astype(np.uint64))
+    assert np.array_equal(t[1], ref.indices.astype(np.uint32))
+    assert np.array_equal(t[2].view(np.uint64), ref.data.astype(np.float64).view(np.uint64))
+    # parallel form: the same with explicit zeros removed
+    tz = oracle.transpose(off, col, val, n_cols, drop_zeros=True)
+    keep = ~((t[2].view(np.uint64) & np.uint64(0x7fffffffffffffff)) == 0)
+    rows = np.repeat(np.arange(n_cols), np.diff(t[0].astype(np.int64)))
+    exp_off = np.concatenate([[0], np.cumsum(np.bincount(rows[keep], minlength=n_cols))]).astype(np.uint64)
+    assert np.array_equal(tz[0], exp_off) and np.array_equal(tz[1], t[1][keep])
+    assert np.array_equal(tz[2], t[2][keep])
+    # transposing twice gives the input back
+    assert csr_equal(oracle.transpose(*t, n_rows), (off, col, val))
+
+
+def _transpose_py(off, col, val, n_cols, descending_rows=False):
+    """Pure-Python two-pass transpose; descending_rows plants the mistake of
+    filling from the last input row (output rows come out reversed)."""
+    n = len(off) - 1
+    cnt = np.zeros(n_cols + 1, np.int64)
+    for k in range(int(off[-1])):
+        cnt[int(col[k]) + 1] += 1
+    t_off = np.cumsum(cnt).astype(np.uint64)
+    cur = t_off[:-1].astype(np.int64).copy()
+    t_col = np.zeros(int(off[-1]), np.uint32)
+    t_val = np.zeros(int(off[-1]))
+    for i in (range(n - 1, -1, -1) if descending_rows else range(n)):
+        for k in range(int(off[i]), int(off[i + 1])):
+            j = int(col[k])
+            t_col[cur[j]] = i
+            t_val[cur[j]] = val[k]
+            cur[j] += 1
+    return t_off, t_col, t_val
+
+
+def test_planted_transpose_unstable_fill_is_caught():
+    """The restatement agrees with the oracle; the same code with the fill
+    walking the input rows in descending order fails the scipy check."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(5)
+    dense = rng.standard_normal((40, 30)) * (rng.random((40, 30)) < 0.3)
+    m = sp.csr_matrix(dense)
+    off, col, val = m.indptr.astype(np.uint64), m.indices.astype(np.uint32), m.data
+    ref = m.T.tocsr()
+    ref.sort_indices()
+    good = _transpose_py(off, col, val, 30)
+    assert csr_equal(good, oracle.transpose(off, col, val, 30))
+    bad = _transpose_py(off, col, val, 30, descending_rows=True)
+    assert not np.array_equal(bad[1], ref.indices.astype(np.uint32))
