@@ -41,6 +41,8 @@ def parse():
     p.add_argument("--matrices", default="laplacian_2048,random_4M_8to32",
                    help="comma list of the SpMV matrices to run (profiling runs pick one)")
     p.add_argument("--lock-only", action="store_true", help="only the lock() measurement (profiling)")
+    p.add_argument("--no-transpose", action="store_true", help="skip the transpose measurement")
+    p.add_argument("--transpose-only", action="store_true", help="only the transpose measurement (profiling)")
     p.add_argument("--calib-copy-mb", type=int, default=0,
                    help="also run a plain device copy of this many MiB (PMC traffic calibration)")
     p.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision power-law section")
@@ -131,6 +133,36 @@ def run_lock(args, np, torch, spira_amd, W, dev, stream):
         detail["ms_host_inputs_pcie_inclusive"] = round(1e3 * float(np.median(t_host[1:])), 3)
     L.close()
     return detail, (o3, c3, v3, sp3, sc3, sv3, n3, lbytes)
+
+
+def run_transpose(args, np, torch, spira_amd, W, dev, stream, src=None):
+    """transpose (serial/transpose.hpp:35-157) of the 1,048,576^2 matrix with 16
+    random nnz/row, resident in HBM; ms per call from HIP events on the
+    library's stream around 10 back-to-back calls (each call builds a new A^T,
+    allocation included, and ends with one readback of its nnz)."""
+    if src is None:
+        (o, c, v), _ = W.relock_workload(1 << 20)
+    else:
+        o, c, v = src
+    n = len(o) - 1
+    A = spira_amd.DeviceCSR(dev, n, n)
+    A.upload(o, c, v)
+    T = spira_amd.DeviceCSR(dev, n, n)
+    dev.synchronize()
+    detail = {"workload": "transpose of 1,048,576^2 with 16 random nnz/row (serial form: explicit zeros kept)",
+              "nnz": int(c.size), "bytes_per_call": W.transpose_bytes(n, n, int(c.size))}
+    for key, drop in (("serial", False), ("drop_zeros", True)):
+        for _ in range(2):
+            A.transpose(T, drop_zeros=drop)
+        dev.synchronize()
+        ms = _time_calls(torch, stream, lambda: A.transpose(T, drop_zeros=drop), 10)
+        detail[f"ms_{key}"] = round(ms, 3)
+    detail["GB_s"] = round(detail["bytes_per_call"] / (detail["ms_serial"] * 1e-3) / 1e9, 1)
+    A.transpose(T)
+    got = T.download()
+    A.close()
+    T.close()
+    return detail, (o, c, v, got)
 
 
 def _cpu_model():
@@ -401,6 +433,10 @@ def main():
         out = run_lock(args, np, torch, spira_amd, W, dev, stream)
         print(json.dumps({"lock": out[0]}), flush=True)
         return
+    if args.transpose_only:
+        out = run_transpose(args, np, torch, spira_amd, W, dev, stream)
+        print(json.dumps({"transpose": out[0]}), flush=True)
+        return
 
     generators = {
         "laplacian_2048": lambda: W.laplacian_2d(2048),
@@ -495,6 +531,10 @@ def main():
             detail[name]["autotune_ms"] = info[name]["autotune_ms"]
 
     lock_detail, lock_src = (None, None) if args.no_lock else run_lock(args, np, torch, spira_amd, W, dev, stream)
+    tr_detail, tr_src = (None, None) if args.no_transpose else run_transpose(
+        args, np, torch, spira_amd, W, dev, stream, None if lock_src is None else lock_src[:3])
+    if tr_detail is not None:
+        log(f"bench: transpose {tr_detail['ms_serial']} ms ({tr_detail['GB_s']} GB/s)")
     mixed_detail, mixed_src = (None, None) if args.no_mixed else run_mixed(args, np, torch, spira_amd, W, dev, stream)
     batch_detail = None if args.no_batch else run_batch(args, np, torch, spira_amd, W, dev, stream)
 
@@ -584,6 +624,17 @@ def main():
                 p.close()
             lock_detail["cpu_baseline_ms"] = round(1e3 * float(np.median(tl)), 2)
             lock_detail["cpu_baseline_threads"] = threads
+        if tr_detail is not None:
+            o, c, v, got = tr_src
+            tt = []
+            for rep in range(3):
+                t = time.perf_counter()
+                exp = oracle.transpose(o, c, v, len(o) - 1)
+                tt.append(time.perf_counter() - t)
+            tr_detail["bit_exact_vs_oracle"] = bool(
+                all(np.array_equal(a.view(np.uint8), b.view(np.uint8)) for a, b in zip(got, exp)))
+            tr_detail["cpu_baseline_ms"] = round(1e3 * float(np.median(tt)), 2)
+            tr_detail["cpu_baseline_threads"] = 1
         if mixed_detail is not None:
             # errors of every storage against the FP64 oracle and its
             # double-double reference (checker use)
@@ -628,6 +679,7 @@ def main():
         "cpu_baseline": cpu,
         "detail": detail,
         "lock": lock_detail,
+        "transpose": tr_detail,
         "mixed": mixed_detail,
         "batched": batch_detail,
     }

@@ -6,15 +6,16 @@
 // (parallel/algorithms/transpose.hpp:37-69: the same entries inserted into a
 // new matrix whose lock() removes explicit zeros).
 //
-//   1. k_transpose_keys: per entry its input row (the block's row range from a
-//      wavefront-wide search, then a search inside it), its key = column (or
-//      the sentinel n_cols for a dropped zero) and its index.
-//   2. A stable LSD radix sort (rocPRIM) of (key, index): entries come out
-//      grouped by output row and, inside a row, in input-entry order, which is
-//      input-row order -- the reference's fill order.
+//   1. k_transpose_keys: per entry a 64-bit key, (column << 32) | input row,
+//      with the sentinel column n_cols for a dropped zero; the input row comes
+//      from the block's row range (a wavefront-wide search) and a search inside.
+//   2. A stable LSD radix sort (rocPRIM) of (key, value) over the column bits
+//      only: entries come out grouped by output row and, inside a row, in
+//      input-entry order, which is input-row order -- the reference's fill
+//      order.  The values travel with the keys, so nothing is gathered later.
 //   3. k_transpose_offsets: output offsets = lower bounds of each row in the
 //      sorted keys (no atomics, no second pass over A).
-//   4. k_transpose_gather: column = input row, value = input value.
+//   4. k_transpose_cols: output column = the low half of the sorted key.
 #include "internal.hpp"
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -24,33 +25,55 @@ namespace spira_amd {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kItems = 4;  // entries per thread in the key pass
+constexpr int kItems = 8;         // entries per thread in the key pass
+constexpr int kRowCache = 2048;   // row offsets a key block keeps in LDS
 
 __global__ __launch_bounds__(kThreads) void k_transpose_keys(uint64_t n_rows, uint64_t nnz, uint64_t n_cols,
                                                               const uint64_t *__restrict__ off,
                                                               const uint32_t *__restrict__ col,
                                                               const double *__restrict__ val, int drop_zeros,
-                                                              uint32_t *__restrict__ rid, uint32_t *__restrict__ key,
-                                                              uint32_t *__restrict__ idx)
+                                                              uint64_t *__restrict__ key)
 {
+    __shared__ uint64_t s_off[kRowCache + 1];
     const uint64_t first = (uint64_t)blockIdx.x * kThreads * kItems;
     if (first >= nnz) return;  // whole blocks leave together: the searches below need every lane
     const uint64_t last = min(first + (uint64_t)kThreads * kItems, nnz) - 1;
     const uint64_t lo = wave_row_of(off, 0, n_rows, first);
     const uint64_t hi = wave_row_of(off, lo, n_rows, last);
+    const uint64_t nr = hi - lo + 1;  // rows the block's entries lie in (the same in every wavefront)
+    if (nr <= kRowCache) {
+        for (uint64_t i = threadIdx.x; i <= nr; i += kThreads) s_off[i] = off[lo + i];  // off[lo .. hi + 1]
+        __syncthreads();
+        uint32_t r = 0;  // a thread's entries ascend, so each search starts at the previous row
 #pragma unroll
-    for (int it = 0; it < kItems; ++it) {
-        const uint64_t k = first + (uint64_t)it * kThreads + threadIdx.x;
-        if (k >= nnz) break;
-        rid[k] = (uint32_t)row_of(off, lo, hi + 1, k);
-        key[k] = (drop_zeros && is_zero_f64(val[k])) ? (uint32_t)n_cols : col[k];
-        idx[k] = (uint32_t)k;
+        for (int it = 0; it < kItems; ++it) {
+            const uint64_t k = first + (uint64_t)it * kThreads + threadIdx.x;
+            if (k >= nnz) break;
+            uint32_t b = (uint32_t)nr;
+            while (b - r > 1) {
+                const uint32_t mid = r + (b - r) / 2;
+                if (s_off[mid] <= k) r = mid;
+                else b = mid;
+            }
+            const uint64_t c = (drop_zeros && is_zero_f64(val[k])) ? n_cols : (uint64_t)col[k];
+            key[k] = (c << 32) | (lo + r);
+        }
+    } else {  // many empty rows inside the block's span: search the offsets in memory
+#pragma unroll
+        for (int it = 0; it < kItems; ++it) {
+            const uint64_t k = first + (uint64_t)it * kThreads + threadIdx.x;
+            if (k >= nnz) break;
+            const uint64_t r = row_of(off, lo, hi + 1, k);
+            const uint64_t c = (drop_zeros && is_zero_f64(val[k])) ? n_cols : (uint64_t)col[k];
+            key[k] = (c << 32) | r;
+        }
     }
 }
 
-// t_off[r] = number of sorted keys below r, r = 0 .. n_cols (dropped zeros
-// carry the key n_cols and sort last, so t_off[n_cols] is the output nnz).
-__global__ void k_transpose_offsets(const uint32_t *__restrict__ skey, uint64_t nnz, uint64_t n_cols,
+// t_off[r] = number of sorted keys whose column is below r, r = 0 .. n_cols
+// (dropped zeros carry the column n_cols and sort last, so t_off[n_cols] is
+// the output nnz).
+__global__ void k_transpose_offsets(const uint64_t *__restrict__ skey, uint64_t nnz, uint64_t n_cols,
                                     uint64_t *__restrict__ t_off)
 {
     for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r <= n_cols;
@@ -58,25 +81,20 @@ __global__ void k_transpose_offsets(const uint32_t *__restrict__ skey, uint64_t 
         uint64_t a = 0, b = nnz;
         while (a < b) {
             const uint64_t mid = a + (b - a) / 2;
-            if ((uint64_t)skey[mid] < r) a = mid + 1;
+            if ((skey[mid] >> 32) < r) a = mid + 1;
             else b = mid;
         }
         t_off[r] = a;
     }
 }
 
-__global__ void k_transpose_gather(const uint32_t *__restrict__ sidx, const uint64_t *__restrict__ t_off,
-                                   uint64_t n_cols, const uint32_t *__restrict__ rid,
-                                   const double *__restrict__ val, uint32_t *__restrict__ t_col,
-                                   double *__restrict__ t_val)
+__global__ void k_transpose_cols(const uint64_t *__restrict__ skey, const uint64_t *__restrict__ t_off,
+                                 uint64_t n_cols, uint32_t *__restrict__ t_col)
 {
     const uint64_t nz = t_off[n_cols];
     for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nz;
-         p += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t k = sidx[p];
-        t_col[p] = rid[k];
-        t_val[p] = val[k];
-    }
+         p += (uint64_t)gridDim.x * blockDim.x)
+        t_col[p] = (uint32_t)skey[p];
 }
 
 unsigned key_bits(uint64_t max_key)
@@ -109,42 +127,39 @@ extern "C" int spira_transpose_f64(spira_dcsr *A, spira_dcsr *out, uint32_t flag
         hipStream_t st = dev->stream;
         const uint64_t n = A->n_rows, nc = A->n_cols, nnz = A->nnz;
         const size_t ob = (nc + 1) * sizeof(uint64_t), cb = (nnz ? nnz : 1) * sizeof(uint32_t),
-                     vb = (nnz ? nnz : 1) * sizeof(double), kb = (nnz ? nnz : 1) * sizeof(uint32_t);
+                     vb = (nnz ? nnz : 1) * sizeof(double), kb = (nnz ? nnz : 1) * sizeof(uint64_t);
         uint64_t *t_off = (uint64_t *)dev_alloc(dev, ob);
-        uint32_t *t_col = nullptr, *rid = nullptr, *key = nullptr, *skey = nullptr, *idx = nullptr, *sidx = nullptr;
+        uint32_t *t_col = nullptr;
+        uint64_t *key = nullptr, *skey = nullptr;
         double *t_val = nullptr;
         auto release_tmp = [&] {
-            if (rid) dev_release(dev, rid, kb);
             if (key) dev_release(dev, key, kb);
             if (skey) dev_release(dev, skey, kb);
-            if (idx) dev_release(dev, idx, kb);
-            if (sidx) dev_release(dev, sidx, kb);
-            rid = key = skey = idx = sidx = nullptr;
+            key = skey = nullptr;
         };
         try {
             t_col = (uint32_t *)dev_alloc(dev, cb);
             t_val = (double *)dev_alloc(dev, vb);
-            rid = (uint32_t *)dev_alloc(dev, kb);
-            key = (uint32_t *)dev_alloc(dev, kb);
-            skey = (uint32_t *)dev_alloc(dev, kb);
-            idx = (uint32_t *)dev_alloc(dev, kb);
-            sidx = (uint32_t *)dev_alloc(dev, kb);
+            key = (uint64_t *)dev_alloc(dev, kb);
+            skey = (uint64_t *)dev_alloc(dev, kb);
             if (nnz > 0) {
                 k_transpose_keys<<<grid_for(nnz, kThreads * kItems), kThreads, 0, st>>>(
-                    n, nnz, nc, A->off, A->col, A->val, drop ? 1 : 0, rid, key, idx);
+                    n, nnz, nc, A->off, A->col, A->val, drop ? 1 : 0, key);
                 SPIRA_HIP(hipGetLastError());
-                const unsigned bits = key_bits(nc);  // keys 0 .. n_cols (the sentinel included)
+                // columns 0 .. n_cols - 1, and the sentinel n_cols when zeros
+                // are dropped, in bits 32 and up
+                const unsigned bits = key_bits(drop ? nc : nc - 1);
                 size_t tmp_bytes = 0;
-                SPIRA_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, key, skey, idx, sidx, (unsigned)nnz, 0, bits,
-                                                    st));
+                SPIRA_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, key, skey, A->val, t_val, (unsigned)nnz, 32,
+                                                    32 + bits, st));
                 void *tmp = dev->ws.get(WS_SORT, tmp_bytes ? tmp_bytes : 8);
-                SPIRA_HIP(rocprim::radix_sort_pairs(tmp, tmp_bytes, key, skey, idx, sidx, (unsigned)nnz, 0, bits, st));
+                SPIRA_HIP(rocprim::radix_sort_pairs(tmp, tmp_bytes, key, skey, A->val, t_val, (unsigned)nnz, 32,
+                                                    32 + bits, st));
             }
             k_transpose_offsets<<<grid_for(nc + 1, kThreads, 8192), kThreads, 0, st>>>(skey, nnz, nc, t_off);
             SPIRA_HIP(hipGetLastError());
             if (nnz > 0) {
-                k_transpose_gather<<<grid_for(nnz, kThreads, 16384), kThreads, 0, st>>>(sidx, t_off, nc, rid, A->val,
-                                                                                        t_col, t_val);
+                k_transpose_cols<<<grid_for(nnz, kThreads, 16384), kThreads, 0, st>>>(skey, t_off, nc, t_col);
                 SPIRA_HIP(hipGetLastError());
             }
             const uint64_t nnz_out = read_scalar(dev, t_off + nc);

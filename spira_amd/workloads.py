@@ -271,6 +271,12 @@ def relock_workload(n, nnz_per_row=16, inserts_per_row=2, overwrite_frac=0.25, s
     return (off, cols, vals), (ins_rows, ins_cols.reshape(-1), ins_vals)
 
 
+def transpose_bytes(n_rows, n_cols, nnz, s_col=4, s_val=8):
+    """Algorithmic HBM bytes of one transpose: A's offsets, columns and values
+    read once, A^T's written once."""
+    return 8 * (n_rows + 1) + nnz * (s_col + s_val) + 8 * (n_cols + 1) + nnz * (s_col + s_val)
+
+
 def dense_x(n, seed=BASE_SEED ^ 0xC0FFEE):
     """x ~ U(-1, 1) with the reference's x sub-seed (bench/spira_bench.cpp:268)."""
     return np.random.default_rng(seed).uniform(-1.0, 1.0, size=n)

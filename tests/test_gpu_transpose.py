@@ -144,6 +144,18 @@ def _long_row_and_column(n):
     return off, col, val
 
 
+def _mostly_empty_rows(n, n_cols, frac, seed):
+    """About frac of the rows hold one entry: a key block's 2048 entries span
+    far more rows than its LDS row cache, so the in-memory search runs."""
+    rng = np.random.default_rng(seed)
+    lens = (rng.random(n) < frac).astype(np.int64)
+    off = np.zeros(n + 1, np.uint64)
+    np.cumsum(lens, out=off[1:])
+    col = rng.integers(0, n_cols, int(off[-1])).astype(np.uint32)
+    val = rng.standard_normal(col.size)
+    return off, col, val
+
+
 CASES = [
     # (name, builder)
     ("empty_1x1", lambda: (np.zeros(2, np.uint64), np.zeros(0, np.uint32), np.zeros(0), 1)),
@@ -153,6 +165,7 @@ CASES = [
     ("ragged_empty_rows", lambda: (*W.random_uniform_rows(30000, 20000, 0, 12, seed=33), 20000)),
     ("uniform_2b_like", lambda: (*W.random_uniform_rows(20000, 70000, 8, 32, seed=34), 70000)),
     ("long_row_long_col", lambda: (*_long_row_and_column(40000), 40000)),
+    ("mostly_empty_rows", lambda: (*_mostly_empty_rows(400000, 5000, 0.02, 37), 5000)),
     ("explicit_zeros", lambda: (*_with_zeros(*W.random_uniform_rows(9000, 11000, 0, 20, seed=35), 36), 11000)),
 ]
 
