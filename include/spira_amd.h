@@ -183,6 +183,35 @@ int spira_partition_boundaries(spira_dcsr *A, uint64_t n_threads, uint64_t *boun
 #define SPIRA_TRANSPOSE_DROP_ZEROS 1u
 int spira_transpose_f64(spira_dcsr *A, spira_dcsr *out, uint32_t flags);
 
+/* ---- matrix addition -------------------------------------------------------
+ * Replaces serial::algorithms::MatrixAddition (serial/matrix_addition.hpp:
+ * 88-112, with addRows :34-86), parallel::algorithms::MatrixAddition
+ * (parallel/algorithms/matrix_addition.hpp:24-99) and matrix::operator+ /
+ * operator+= (matrix_operators.hpp:21-50).  C (a handle of A's shape; may be
+ * A or B) receives A + B, built: each row is the column-ordered union of A's
+ * and B's row, a + b where both hold the column, the lone value elsewhere,
+ * every zero dropped (the output's first lock()).  With SPIRA_ADD_SUBTRACT,
+ * A + B * -1: operator- / operator-= (matrix_operators.hpp:29-62 with
+ * scalars.hpp:23-35).  Checks in the reference's order: shapes
+ * (SPIRA_ERR_INVALID_ARGUMENT, "Matrices aren't the same size"), then A and B
+ * built (SPIRA_ERR_LOGIC, "MatrixAddition: A must be locked", "... B ..."). */
+#define SPIRA_ADD_SUBTRACT 1u
+int spira_add_f64(spira_dcsr *A, spira_dcsr *B, spira_dcsr *C, uint32_t flags);
+
+/* ---- SpGEMM ------------------------------------------------------------------
+ * Replaces serial::algorithms::spgemm (serial/spgemm.hpp:12-50),
+ * parallel::algorithms::spgemm (parallel/algorithms/spgemm.hpp:24-69) and
+ * matrix::operator* / operator*= of two matrices (matrix_operators.hpp:
+ * 67-83).  C (A's rows, B's columns; may be A or B) receives A * B, built:
+ * for each output entry the products a_ik * b_kj are rounded, then added from
+ * 0.0 in the order of k and, inside B's row, of j -- the reference's hash
+ * accumulator order -- and entries whose sum is 0 are dropped.  Checks:
+ * A.n_cols != B.n_rows (SPIRA_ERR_INVALID_ARGUMENT, "A.cols must equal
+ * B.rows"), then A and B built (SPIRA_ERR_LOGIC); fewer than 2^32 - 1 rows,
+ * and fewer than 2^32 products in the rows with more than 1024 products or
+ * A entries (SPIRA_ERR_UNSUPPORTED). */
+int spira_spgemm_f64(spira_dcsr *A, spira_dcsr *B, spira_dcsr *C);
+
 /* ---- mixed precision ------------------------------------------------------
  * No counterpart in the reference (its float instantiation,
  * serial/spmv.hpp:90-135, stores and accumulates in FP32).  y = A*x with A's

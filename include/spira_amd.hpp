@@ -290,12 +290,72 @@ public:
     void transpose_into(matrix &out, uint32_t flags) const
     {
         check(spira_transpose_f64(csr_->get(), out.csr_->get(), flags));
-        for (auto &b : out.buf_) b.clear();
-        std::fill(out.dirty_.begin(), out.dirty_.end(), uint8_t{0});
-        out.mode_ = matrix_mode::locked;
-        out.csr_built_ = true;
-        out.committed_nnz_ = out.csr_->nnz();
-        out.host_.reset();
+        out.adopt_device_result();
+    }
+
+    // out = *this + B (flags: SPIRA_ADD_SUBTRACT for *this - B) on the device;
+    // out (this shape; may be *this or B) ends locked with no pending inserts.
+    void add_into(const matrix &B, matrix &out, uint32_t flags) const
+    {
+        check(spira_add_f64(csr_->get(), B.csr_->get(), out.csr_->get(), flags));
+        out.adopt_device_result();
+    }
+
+    // out = *this * B on the device; out has this matrix's rows and B's columns.
+    void spgemm_into(const matrix &B, matrix &out) const
+    {
+        check(spira_spgemm_f64(csr_->get(), B.csr_->get(), out.csr_->get()));
+        out.adopt_device_result();
+    }
+
+    // operator+ / operator+= (matrix_operators.hpp:21-47): MatrixAddition.
+    matrix operator+(const matrix &other) const { return add_checked(other, 0, "Matrices aren't the same size"); }
+    matrix &operator+=(const matrix &other)
+    {
+        *this = add_checked(other, 0, "Matrices aren't the same size");
+        return *this;
+    }
+    // operator- / operator-= (matrix_operators.hpp:29-62): the shape check,
+    // then MatrixAddition(*this, other * -1).
+    matrix operator-(const matrix &other) const
+    {
+        return add_checked(other, SPIRA_ADD_SUBTRACT, "operator-: matrix shapes must match");
+    }
+    matrix &operator-=(const matrix &other)
+    {
+        *this = add_checked(other, SPIRA_ADD_SUBTRACT, "operator-=: matrix shapes must match");
+        return *this;
+    }
+    // operator* / operator*= of two matrices (matrix_operators.hpp:67-83): spgemm.
+    matrix operator*(const matrix &other) const { return spgemm_checked(other, "A.cols must equal B.rows"); }
+    matrix &operator*=(const matrix &other)
+    {
+        *this = spgemm_checked(other, "A.cols must equal B.rows");
+        return *this;
+    }
+
+    // the checks of serial::algorithms::MatrixAddition (serial/
+    // matrix_addition.hpp:88-101; A - B checks that B is locked in its scaled
+    // copy first, scalars.hpp:26-27) and of spgemm (serial/spgemm.hpp:14-23).
+    matrix add_checked(const matrix &other, uint32_t flags, const char *shape_msg) const
+    {
+        if (shape() != other.shape()) throw std::invalid_argument(shape_msg);
+        if ((flags & SPIRA_ADD_SUBTRACT) && !other.is_locked())
+            throw std::logic_error("multiplication_scaler: input matrix must be locked");
+        if (!is_locked()) throw std::logic_error("MatrixAddition: A must be locked");
+        if (!other.is_locked()) throw std::logic_error("MatrixAddition: B must be locked");
+        matrix out(rows_, cols_, dev_);
+        add_into(other, out, flags);
+        return out;
+    }
+    matrix spgemm_checked(const matrix &other, const char *shape_msg) const
+    {
+        if (cols_ != other.rows_) throw std::invalid_argument(shape_msg);
+        if (!is_locked()) throw std::logic_error("spgemm: A must be locked");
+        if (!other.is_locked()) throw std::logic_error("spgemm: B must be locked");
+        matrix out(rows_, other.cols_, dev_);
+        spgemm_into(other, out);
+        return out;
     }
 
     // checks in serial/spmv.hpp:150-158 order, then the device SpMV
@@ -327,6 +387,17 @@ protected:
 
     // Stage the row buffers (insertion order kept) and lock on the device.
     // dirty == nullptr is the parallel rule: a row is dirty iff it has inserts.
+    // The device wrote a new committed CSR into this matrix's handle.
+    void adopt_device_result()
+    {
+        for (auto &b : buf_) b.clear();
+        std::fill(dirty_.begin(), dirty_.end(), uint8_t{0});
+        mode_ = matrix_mode::locked;
+        csr_built_ = true;
+        committed_nnz_ = csr_->nnz();
+        host_.reset();
+    }
+
     void lock_with(const uint8_t *dirty)
     {
         if (mode_ == matrix_mode::locked) return;
@@ -483,6 +554,17 @@ inline matrix transpose(const matrix &mat)
 }
 
 // serial/transpose.hpp:158-173: open, square; lock, transpose, open again.
+inline void transpose_itself(matrix &mat);
+
+// serial/matrix_addition.hpp:88-112
+inline matrix MatrixAddition(const matrix &A, const matrix &B)
+{
+    return A.add_checked(B, 0, "Matrices aren't the same size");
+}
+
+// serial/spgemm.hpp:12-50
+inline matrix spgemm(const matrix &A, const matrix &B) { return A.spgemm_checked(B, "A.cols must equal B.rows"); }
+
 inline void transpose_itself(matrix &mat)
 {
     if (!mat.is_open()) throw std::logic_error("transpose_itself: matrix must be open");
@@ -512,6 +594,32 @@ inline parallel_matrix transpose(parallel_matrix &mat)
     parallel_matrix out(mat.n_cols(), mat.n_rows(), mat.n_threads(), mat.device_ptr());
     mat.transpose_into(out, SPIRA_TRANSPOSE_DROP_ZEROS);
     return out;
+}
+
+// parallel/algorithms/matrix_addition.hpp:24-99: same shape and thread count;
+// C has A's thread count (uniform partitions) and ends locked.
+inline parallel_matrix MatrixAddition(parallel_matrix &A, parallel_matrix &B)
+{
+    if (A.shape() != B.shape()) throw std::invalid_argument("MatrixAddition: matrices must have the same shape");
+    if (A.n_threads() != B.n_threads())
+        throw std::invalid_argument("MatrixAddition: matrices must have the same thread count");
+    if (!A.is_locked()) throw std::logic_error("MatrixAddition: A must be locked");
+    if (!B.is_locked()) throw std::logic_error("MatrixAddition: B must be locked");
+    parallel_matrix C(A.n_rows(), A.n_cols(), A.n_threads(), A.device_ptr());
+    A.add_into(B, C, 0);
+    return C;
+}
+
+// parallel/algorithms/spgemm.hpp:24-69: C is A.n_rows() x B.n_cols() with A's
+// thread count.
+inline parallel_matrix spgemm(parallel_matrix &A, parallel_matrix &B)
+{
+    if (A.n_cols() != B.n_rows()) throw std::invalid_argument("spgemm: A.n_cols() must equal B.n_rows()");
+    if (!A.is_locked()) throw std::logic_error("spgemm: A must be locked");
+    if (!B.is_locked()) throw std::logic_error("spgemm: B must be locked");
+    parallel_matrix C(A.n_rows(), B.n_cols(), A.n_threads(), A.device_ptr());
+    A.spgemm_into(B, C);
+    return C;
 }
 }  // namespace parallel::algorithms
 

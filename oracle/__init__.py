@@ -20,6 +20,9 @@ and cites the reference code it restates:
 ``partition_bounds`` partition.hpp:78-126
 ``transpose``        serial/transpose.hpp:35-157; with drop_zeros,
                      parallel/algorithms/transpose.hpp:37-69 (insert + lock)
+``add``              serial/matrix_addition.hpp:34-112 (subtract:
+                     matrix_operators.hpp:32-41 with scalars.hpp:23-35)
+``spgemm``           serial/spgemm.hpp:12-50
 ===================  ===========================================================
 """
 from __future__ import annotations
@@ -77,6 +80,10 @@ def lib():
             "spira_or_pool_nnz": (ctypes.c_uint64, [ctypes.c_void_p]),
             "spira_or_pool_export_csr": (ctypes.c_int, [ctypes.c_void_p, _u64p, _u32p, _f64p]),
             "spira_or_partition_boundaries": (None, [ctypes.c_uint64, _u64p, ctypes.c_uint64, _u64p]),
+            "spira_or_add": (ctypes.c_uint64, [ctypes.c_uint64, _u64p, _u32p, _f64p, _u64p, _u32p, _f64p,
+                                               ctypes.c_int, _u64p, _u32p, _f64p]),
+            "spira_or_spgemm": (ctypes.c_int64, [ctypes.c_uint64, ctypes.c_uint64, _u64p, _u32p, _f64p, _u64p,
+                                                 _u32p, _f64p, _u64p, _u32p, _f64p, ctypes.c_uint64]),
             "spira_or_transpose": (ctypes.c_uint64, [ctypes.c_uint64, ctypes.c_uint64, _u64p, _u32p, _f64p,
                                                      ctypes.c_int, _u64p, _u32p, _f64p]),
         }
@@ -218,6 +225,42 @@ def transpose(off, col, val, n_cols, drop_zeros=False):
     nz = lib().spira_or_transpose(n, int(n_cols), _p(o, _u64p), _p(c, _u32p), _p(v, _f64p), 1 if drop_zeros else 0,
                                   _p(t_off, _u64p), _p(t_col, _u32p), _p(t_val, _f64p))
     return t_off, t_col[:nz].copy(), t_val[:nz].copy()
+
+
+def add(a, b, subtract=False):
+    """C = A + B (A - B): a and b are (off, col, val) of the same row count."""
+    ao, ac, av = _u64(a[0]), _u32(a[1]), _f64(a[2])
+    bo, bc, bv = _u64(b[0]), _u32(b[1]), _f64(b[2])
+    n = ao.size - 1
+    cap = max(ac.size + bc.size, 1)
+    co = np.zeros(n + 1, np.uint64)
+    cc = np.zeros(cap, np.uint32)
+    cv = np.zeros(cap, np.float64)
+    nz = lib().spira_or_add(n, _p(ao, _u64p), _p(ac, _u32p), _p(av, _f64p), _p(bo, _u64p), _p(bc, _u32p),
+                            _p(bv, _f64p), 1 if subtract else 0, _p(co, _u64p), _p(cc, _u32p), _p(cv, _f64p))
+    return co, cc[:nz].copy(), cv[:nz].copy()
+
+
+def spgemm_products(a, b):
+    """Number of products a * b of C = A * B (the work of the row-wise form)."""
+    ao, ac = _u64(a[0]), _u32(a[1])
+    blen = np.diff(_u64(b[0]).astype(np.int64))
+    return int(blen[ac].sum()) if ac.size else 0
+
+
+def spgemm(a, b, n_cols):
+    """C = A * B: a = (off, col, val) of A, b of B, n_cols = B's columns."""
+    ao, ac, av = _u64(a[0]), _u32(a[1]), _f64(a[2])
+    bo, bc, bv = _u64(b[0]), _u32(b[1]), _f64(b[2])
+    n = ao.size - 1
+    cap = max(min(spgemm_products(a, b), n * max(int(n_cols), 1)), 1)
+    co = np.zeros(n + 1, np.uint64)
+    cc = np.zeros(cap, np.uint32)
+    cv = np.zeros(cap, np.float64)
+    nz = lib().spira_or_spgemm(n, int(n_cols), _p(ao, _u64p), _p(ac, _u32p), _p(av, _f64p), _p(bo, _u64p),
+                               _p(bc, _u32p), _p(bv, _f64p), _p(co, _u64p), _p(cc, _u32p), _p(cv, _f64p), cap)
+    assert nz >= 0
+    return co, cc[:nz].copy(), cv[:nz].copy()
 
 
 def normwise_rel_err(y, y_ref_hi, y_ref_lo=None):

@@ -317,6 +317,106 @@ uint64_t spira_or_transpose(uint64_t n_rows, uint64_t n_cols, const uint64_t *of
     return t_off[n_cols];
 }
 
+static int cmp_u32(const void *x, const void *y)
+{
+    const uint32_t a = *(const uint32_t *)x, b = *(const uint32_t *)y;
+    return (a > b) - (a < b);
+}
+
+uint64_t spira_or_add(uint64_t n_rows, const uint64_t *a_off, const uint32_t *a_col, const double *a_val,
+                      const uint64_t *b_off, const uint32_t *b_col, const double *b_val, int subtract,
+                      uint64_t *c_off, uint32_t *c_col, double *c_val)
+{
+    /* serial/matrix_addition.hpp:34-86 addRows: a two-pointer merge per row;
+     * a column in both rows inserts a + b unless that sum is zero, a column in
+     * one row inserts its value; the output's first lock() then drops every
+     * zero (csr_build.hpp:45-95).  A - B adds B scaled by -1
+     * (matrix_operators.hpp:32-41, scalars.hpp:23-35: the scaled copy's lock
+     * drops B's zeros, which the sum drops as well). */
+    uint64_t nz = 0;
+    c_off[0] = 0;
+    for (uint64_t i = 0; i < n_rows; ++i) {
+        uint64_t ka = a_off[i], kb = b_off[i];
+        const uint64_t ea = a_off[i + 1], eb = b_off[i + 1];
+        while (ka < ea || kb < eb) {
+            uint32_t c;
+            double v;
+            if (kb >= eb || (ka < ea && a_col[ka] < b_col[kb])) {
+                c = a_col[ka];
+                v = a_val[ka++];
+            } else {
+                const double bv = subtract ? b_val[kb] * -1.0 : b_val[kb];
+                if (ka >= ea || b_col[kb] < a_col[ka]) {
+                    c = b_col[kb++];
+                    v = bv;
+                } else {
+                    c = a_col[ka];
+                    v = a_val[ka++] + bv;
+                    ++kb;
+                }
+            }
+            if (spira_or_is_zero(v)) continue;
+            c_col[nz] = c;
+            c_val[nz++] = v;
+        }
+        c_off[i + 1] = nz;
+    }
+    return nz;
+}
+
+int64_t spira_or_spgemm(uint64_t n_rows, uint64_t n_cols, const uint64_t *a_off, const uint32_t *a_col,
+                        const double *a_val, const uint64_t *b_off, const uint32_t *b_col, const double *b_val,
+                        uint64_t *c_off, uint32_t *c_col, double *c_val, uint64_t capacity)
+{
+    /* serial/spgemm.hpp:12-50 (Gustavson): per row of A, for its entries in
+     * column order and then B's row entries in column order, acc[j] += a * b
+     * with acc[j] starting at 0.0 (product rounded, then the sum); entries
+     * with acc[j] != 0 are inserted and the output's lock() sorts the row.
+     * The reference's hash map only fixes which j are present, not the order
+     * of the additions into one j, so a dense accumulator with a touched list
+     * gives the same bits. */
+    double *acc = (double *)malloc((n_cols ? n_cols : 1) * sizeof(double));
+    uint64_t *mark = (uint64_t *)calloc(n_cols ? n_cols : 1, sizeof(uint64_t));
+    uint32_t *touched = (uint32_t *)malloc((n_cols ? n_cols : 1) * sizeof(uint32_t));
+    uint64_t nz = 0;
+    c_off[0] = 0;
+    for (uint64_t i = 0; i < n_rows; ++i) {
+        uint64_t nt = 0;
+        for (uint64_t ka = a_off[i]; ka < a_off[i + 1]; ++ka) {
+            const uint32_t k = a_col[ka];
+            const double a = a_val[ka];
+            for (uint64_t kb = b_off[k]; kb < b_off[k + 1]; ++kb) {
+                const uint32_t j = b_col[kb];
+                if (mark[j] != i + 1) {
+                    mark[j] = i + 1;
+                    acc[j] = 0.0;
+                    touched[nt++] = j;
+                }
+                const double p = a * b_val[kb];
+                acc[j] = acc[j] + p;
+            }
+        }
+        qsort(touched, nt, sizeof(uint32_t), cmp_u32);
+        for (uint64_t t = 0; t < nt; ++t) {
+            const double v = acc[touched[t]];
+            if (!(v != 0.0)) continue;
+            if (nz >= capacity) {
+                free(acc);
+                free(mark);
+                free(touched);
+                return -1;
+            }
+            c_col[nz] = touched[t];
+            c_val[nz++] = v;
+        }
+        c_off[i + 1] = nz;
+    }
+    free(acc);
+    free(mark);
+    free(touched);
+    return (int64_t)nz;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Threaded baseline                                                         */
 /* ------------------------------------------------------------------------ */

@@ -122,6 +122,21 @@ uint64_t spira_or_transpose(uint64_t n_rows, uint64_t n_cols, const uint64_t *of
                             const uint32_t *col, const double *val, int drop_zeros,
                             uint64_t *t_off, uint32_t *t_col, double *t_val);
 
+/* C = A + B, or A - B with subtract (serial/matrix_addition.hpp:34-112,
+ * matrix_operators.hpp:21-41): rows merged by column, sums and single values
+ * kept unless zero.  c_* hold at least nnz(A) + nnz(B) entries.  Returns
+ * nnz(C). */
+uint64_t spira_or_add(uint64_t n_rows, const uint64_t *a_off, const uint32_t *a_col, const double *a_val,
+                      const uint64_t *b_off, const uint32_t *b_col, const double *b_val, int subtract,
+                      uint64_t *c_off, uint32_t *c_col, double *c_val);
+
+/* C = A * B (serial/spgemm.hpp:12-50): Gustavson rows, acc[j] += a * b in
+ * A's then B's column order, entries != 0 kept, rows sorted.  n_cols = B's
+ * columns.  Returns nnz(C), or -1 when it exceeds capacity. */
+int64_t spira_or_spgemm(uint64_t n_rows, uint64_t n_cols, const uint64_t *a_off, const uint32_t *a_col,
+                        const double *a_val, const uint64_t *b_off, const uint32_t *b_col, const double *b_val,
+                        uint64_t *c_off, uint32_t *c_col, double *c_val, uint64_t capacity);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,7 +31,7 @@ import numpy as np
 __all__ = [
     "Device", "DeviceCSR", "Matrix", "ParallelMatrix", "spmv", "lib", "lib_path",
     "InvalidArgument", "LogicError", "OutOfRange", "UnsupportedInput", "DeviceError",
-    "VARIANTS", "SPMV_AUTO", "transpose", "transpose_itself",
+    "VARIANTS", "SPMV_AUTO", "transpose", "transpose_itself", "MatrixAddition", "spgemm",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -68,6 +68,7 @@ BATCH_MAX_PLOG2 = 5
 BATCH_SCHED_DTYPE = np.dtype([("plog2", "u1"), ("xlds", "u1"), ("reserved", "<u2"), ("item_nnz", "<u4")])
 LOCK_MAX_ROW_INSERTS = 8192
 TRANSPOSE_DROP_ZEROS = 1
+ADD_SUBTRACT = 1
 
 
 class SpiraError(Exception):
@@ -171,6 +172,8 @@ def lib():
             "spira_spmv_set_variant": (ctypes.c_int, [_vp, ctypes.c_int]),
             "spira_partition_boundaries": (ctypes.c_int, [_vp, ctypes.c_uint64, _vp]),
             "spira_transpose_f64": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32]),
+            "spira_add_f64": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_uint32]),
+            "spira_spgemm_f64": (ctypes.c_int, [_vp, _vp, _vp]),
             "spira_stage_create": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(_vp)]),
             "spira_stage_destroy": (None, [_vp]),
             "spira_stage_append": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_uint64, ctypes.c_uint32]),
@@ -422,6 +425,23 @@ class DeviceCSR:
         if out is None:
             out = DeviceCSR(self.dev, self.n_cols, self.n_rows)
         self.dev.check(lib().spira_transpose_f64(self._h, out._h, TRANSPOSE_DROP_ZEROS if drop_zeros else 0))
+        return out
+
+    def add(self, B: "DeviceCSR", out: "DeviceCSR" = None, subtract: bool = False) -> "DeviceCSR":
+        """C = self + B (subtract: self - B) on the device
+        (serial/matrix_addition.hpp:88-112).  out: a handle of the same shape
+        (default: a new one); may be self or B."""
+        if out is None:
+            out = DeviceCSR(self.dev, self.n_rows, self.n_cols)
+        self.dev.check(lib().spira_add_f64(self._h, B._h, out._h, ADD_SUBTRACT if subtract else 0))
+        return out
+
+    def spgemm(self, B: "DeviceCSR", out: "DeviceCSR" = None) -> "DeviceCSR":
+        """C = self * B on the device (serial/spgemm.hpp:12-50).  out: a handle
+        with self's rows and B's columns (default: a new one)."""
+        if out is None:
+            out = DeviceCSR(self.dev, self.n_rows, B.n_cols)
+        self.dev.check(lib().spira_spgemm_f64(self._h, B._h, out._h))
         return out
 
     def choose_variant(self) -> int:
@@ -892,9 +912,49 @@ class Matrix:
         self._host = None
 
     def __matmul__(self, x):
+        """A @ B: spgemm for a matrix (matrix_operators.hpp:67-74), else y = A x."""
+        if isinstance(x, Matrix):
+            return spgemm(self, x)
         y = np.empty(self._n_rows, np.float64)
         spmv(self, x, y)
         return y
+
+    def __mul__(self, other):
+        """operator* of two matrices (matrix_operators.hpp:67-74)."""
+        if not isinstance(other, Matrix):
+            return NotImplemented
+        return spgemm(self, other)
+
+    def __imul__(self, other):
+        """operator*= (matrix_operators.hpp:76-83)."""
+        if not isinstance(other, Matrix):
+            return NotImplemented
+        return spgemm(self, other)
+
+    def __add__(self, other):
+        """operator+ (matrix_operators.hpp:21-27)."""
+        return MatrixAddition(self, other)
+
+    def __iadd__(self, other):
+        """operator+= (matrix_operators.hpp:41-47)."""
+        return MatrixAddition(self, other)
+
+    def __sub__(self, other):
+        """operator- (matrix_operators.hpp:29-39): the shape check first, then
+        A + B * -1."""
+        if self.shape() != other.shape():
+            raise InvalidArgument("operator-: matrix shapes must match")
+        if not other.is_locked():
+            raise LogicError("multiplication_scaler: input matrix must be locked")
+        return _add(self, other, subtract=True)
+
+    def __isub__(self, other):
+        """operator-= (matrix_operators.hpp:50-62)."""
+        if self.shape() != other.shape():
+            raise InvalidArgument("operator-=: matrix shapes must match")
+        if not other.is_locked():
+            raise LogicError("multiplication_scaler: input matrix must be locked")
+        return _add(self, other, subtract=True)
 
 
 class ParallelMatrix(Matrix):
@@ -977,6 +1037,56 @@ def transpose_itself(mat: Matrix):
     mat.device_csr().transpose(mat.device_csr())
     mat._host = None
     mat.open()
+
+
+def _like(mat: Matrix, n_rows: int, n_cols: int) -> Matrix:
+    if isinstance(mat, ParallelMatrix):
+        return ParallelMatrix(n_rows, n_cols, mat.n_threads(), mat.dev)
+    return Matrix(n_rows, n_cols, mat.dev)
+
+
+def _add(A: Matrix, B: Matrix, subtract: bool) -> Matrix:
+    if not A.is_locked():
+        raise LogicError("MatrixAddition: A must be locked")
+    if not B.is_locked():
+        raise LogicError("MatrixAddition: B must be locked")
+    C = _like(A, A.n_rows(), A.n_cols())
+    A.device_csr().add(B.device_csr(), C.device_csr(), subtract=subtract)
+    C._locked = True
+    return C
+
+
+def MatrixAddition(A: Matrix, B: Matrix) -> Matrix:
+    """serial::algorithms::MatrixAddition (serial/matrix_addition.hpp:88-112)
+    or, for ParallelMatrix inputs, parallel::algorithms::MatrixAddition
+    (parallel/algorithms/matrix_addition.hpp:24-99; same thread count
+    required): a new locked matrix, added on the device."""
+    if isinstance(A, ParallelMatrix) and isinstance(B, ParallelMatrix):
+        if A.shape() != B.shape():
+            raise InvalidArgument("MatrixAddition: matrices must have the same shape")
+        if A.n_threads() != B.n_threads():
+            raise InvalidArgument("MatrixAddition: matrices must have the same thread count")
+    elif A.shape() != B.shape():
+        raise InvalidArgument("Matrices aren't the same size")
+    return _add(A, B, subtract=False)
+
+
+def spgemm(A: Matrix, B: Matrix) -> Matrix:
+    """serial::algorithms::spgemm (serial/spgemm.hpp:12-50) or, for
+    ParallelMatrix inputs, parallel::algorithms::spgemm
+    (parallel/algorithms/spgemm.hpp:24-69; C takes A's thread count): a new
+    locked matrix A * B, multiplied on the device."""
+    par = isinstance(A, ParallelMatrix) and isinstance(B, ParallelMatrix)
+    if A.n_cols() != B.n_rows():
+        raise InvalidArgument("spgemm: A.n_cols() must equal B.n_rows()" if par else "A.cols must equal B.rows")
+    if not A.is_locked():
+        raise LogicError("spgemm: A must be locked")
+    if not B.is_locked():
+        raise LogicError("spgemm: B must be locked")
+    C = _like(A, A.n_rows(), B.n_cols())
+    A.device_csr().spgemm(B.device_csr(), C.device_csr())
+    C._locked = True
+    return C
 
 
 def spmv(mat: Matrix, x, y, variant: int = SPMV_AUTO):

@@ -1,0 +1,659 @@
+// algebra.hip -- C = A + B, C = A - B and C = A * B on locked device CSRs,
+// bit-identical to the reference's serial and parallel algorithms.
+//
+// Addition (serial/matrix_addition.hpp:34-112, parallel/algorithms/
+// matrix_addition.hpp:24-99): every row is the column-ordered union of A's
+// and B's row; a shared column holds a + b, a lone one its value; the output's
+// first lock() drops every zero.  A - B adds B scaled by -1
+// (matrix_operators.hpp:32-41, scalars.hpp:23-35).  Merge by ranking, no
+// per-row loop:
+//   1. k_add_flags: per entry of A and of B, look its column up in the other
+//      row (binary search); keep = the value it contributes is not zero (a B
+//      entry whose column A also holds contributes through A's entry).
+//   2. exclusive scans of both keep arrays; C's offsets are
+//      keepA-prefix[A.off[r]] + keepB-prefix[B.off[r]].
+//   3. k_add_scatter: an entry's output slot is its row's offset + the kept
+//      entries before it in its own row + the kept entries of the other row
+//      with smaller columns (one more binary search).
+//
+// SpGEMM (serial/spgemm.hpp:12-50, parallel/algorithms/spgemm.hpp:24-69):
+// Gustavson rows.  For output (i, j) the reference adds the products
+// a_ik * b_kj into acc[j] = 0.0 in the order of k (A's row) and, inside B's
+// row, of j; each product is rounded, then added (no fused multiply-add);
+// entries with acc[j] != 0 are kept.  The device keeps that order exactly:
+//   * k_spgemm_classify: per row of A its product count (sum of the B row
+//     lengths its columns select) and a size class;
+//   * k_spgemm_rows<T, CAP>: one workgroup per row with at most CAP products
+//     (and CAP entries of A): products expanded into LDS in the reference's
+//     order with their position as a tie-break key, a bitonic sort on
+//     (column, position), then one lane per column adds its products from
+//     0.0 in position order.  Run once to count the kept entries per row
+//     and, after a scan gives C's offsets, again to write them;
+//   * rows with more products (expand-sort-compress in HBM): products keyed
+//     by (row, column) in the reference's order, a stable radix sort, then
+//     one thread per (row, column) run adds it in order.
+// Contraction into fused multiply-adds is switched off in the kernels that
+// round a product or a sum.
+#include "internal.hpp"
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include <algorithm>
+#include <functional>
+
+namespace spira_amd {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kItems = 8;        // entries per thread in the addition passes
+constexpr int kRowCache = 2048;  // row offsets a block keeps in LDS
+
+constexpr uint32_t kSmallCap = 256;   // products (and A entries) of a 64-lane row workgroup
+constexpr uint32_t kLargeCap = 1024;  // ... of a 256-lane row workgroup
+
+// Rows of the entries k = first + it * kThreads + threadIdx.x (it < kItems)
+// of a CSR: the block's row range from two wavefront-wide searches, then a
+// search inside it -- over a copy of the range's offsets in LDS when the range
+// has at most kRowCache rows.  Every thread of the block must call it.
+__device__ __forceinline__ void entry_rows(const uint64_t *__restrict__ off, uint64_t n_rows, uint64_t nnz,
+                                           uint64_t first, uint64_t *s_off, uint64_t (&rows)[kItems])
+{
+    const uint64_t last = min(first + (uint64_t)kThreads * kItems, nnz) - 1;
+    const uint64_t lo = wave_row_of(off, 0, n_rows, first);
+    const uint64_t hi = wave_row_of(off, lo, n_rows, last);
+    const uint64_t nr = hi - lo + 1;  // the same in every wavefront
+    if (nr <= kRowCache) {
+        for (uint64_t i = threadIdx.x; i <= nr; i += kThreads) s_off[i] = off[lo + i];  // off[lo .. hi + 1]
+        __syncthreads();
+        uint32_t r = 0;  // a thread's entries ascend: each search starts at the previous row
+#pragma unroll
+        for (int it = 0; it < kItems; ++it) {
+            const uint64_t k = first + (uint64_t)it * kThreads + threadIdx.x;
+            if (k < nnz) {
+                uint32_t b = (uint32_t)nr;
+                while (b - r > 1) {
+                    const uint32_t mid = r + (b - r) / 2;
+                    if (s_off[mid] <= k) r = mid;
+                    else b = mid;
+                }
+            }
+            rows[it] = lo + r;
+        }
+    } else {
+#pragma unroll
+        for (int it = 0; it < kItems; ++it) {
+            const uint64_t k = first + (uint64_t)it * kThreads + threadIdx.x;
+            rows[it] = k < nnz ? row_of(off, lo, hi + 1, k) : lo;
+        }
+    }
+}
+
+__device__ __forceinline__ double scale_b(double b, int subtract)
+{
+#pragma clang fp contract(off)
+    return subtract ? b * -1.0 : b;
+}
+
+// keep[k] = 1 when entry k of A (blocks [0, a_blocks)) or of B (the rest)
+// contributes a nonzero value to C.
+__global__ __launch_bounds__(kThreads) void k_add_flags(uint64_t n_rows, const uint64_t *__restrict__ a_off,
+                                                         const uint32_t *__restrict__ a_col,
+                                                         const double *__restrict__ a_val, uint64_t a_nnz,
+                                                         uint32_t a_blocks, const uint64_t *__restrict__ b_off,
+                                                         const uint32_t *__restrict__ b_col,
+                                                         const double *__restrict__ b_val, uint64_t b_nnz,
+                                                         int subtract, uint64_t *__restrict__ keep_a,
+                                                         uint64_t *__restrict__ keep_b)
+{
+#pragma clang fp contract(off)
+    __shared__ uint64_t s_off[kRowCache + 1];
+    const bool side_b = blockIdx.x >= a_blocks;
+    const uint64_t blk = side_b ? blockIdx.x - a_blocks : blockIdx.x;
+    const uint64_t *off = side_b ? b_off : a_off;
+    const uint32_t *col = side_b ? b_col : a_col;
+    const double *val = side_b ? b_val : a_val;
+    const uint64_t *o_off = side_b ? a_off : b_off;
+    const uint32_t *o_col = side_b ? a_col : b_col;
+    const double *o_val = side_b ? a_val : b_val;
+    const uint64_t nnz = side_b ? b_nnz : a_nnz;
+    uint64_t *keep = side_b ? keep_b : keep_a;
+    const uint64_t first = blk * kThreads * kItems;
+    if (first >= nnz) return;  // whole blocks leave together
+    uint64_t rows[kItems];
+    entry_rows(off, n_rows, nnz, first, s_off, rows);
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+        const uint64_t k = first + (uint64_t)it * kThreads + threadIdx.x;
+        if (k >= nnz) break;
+        const uint64_t r = rows[it];
+        const uint32_t c = col[k];
+        const uint64_t lo = o_off[r], hi = o_off[r + 1];
+        const uint64_t p = lower_bound_u32(o_col, lo, hi, c);
+        const bool matched = p < hi && o_col[p] == c;
+        bool kp;
+        if (!side_b) {
+            const double v = matched ? val[k] + scale_b(o_val[p], subtract) : val[k];
+            kp = !is_zero_f64(v);
+        } else {
+            kp = !matched && !is_zero_f64(val[k]);  // b * -1 is zero exactly when b is
+        }
+        keep[k] = kp ? 1 : 0;
+    }
+}
+
+// c_off[r] = kept A entries before row r + kept B entries before row r.
+__global__ void k_add_offsets(uint64_t n_rows, const uint64_t *__restrict__ a_off, const uint64_t *__restrict__ sa,
+                              const uint64_t *__restrict__ b_off, const uint64_t *__restrict__ sb,
+                              uint64_t *__restrict__ c_off)
+{
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r <= n_rows;
+         r += (uint64_t)gridDim.x * blockDim.x)
+        c_off[r] = sa[a_off[r]] + sb[b_off[r]];
+}
+
+__global__ __launch_bounds__(kThreads) void k_add_scatter(uint64_t n_rows, const uint64_t *__restrict__ a_off,
+                                                           const uint32_t *__restrict__ a_col,
+                                                           const double *__restrict__ a_val, uint64_t a_nnz,
+                                                           uint32_t a_blocks, const uint64_t *__restrict__ b_off,
+                                                           const uint32_t *__restrict__ b_col,
+                                                           const double *__restrict__ b_val, uint64_t b_nnz,
+                                                           int subtract, const uint64_t *__restrict__ sa,
+                                                           const uint64_t *__restrict__ sb,
+                                                           const uint64_t *__restrict__ c_off,
+                                                           uint32_t *__restrict__ c_col, double *__restrict__ c_val)
+{
+#pragma clang fp contract(off)
+    __shared__ uint64_t s_off[kRowCache + 1];
+    const bool side_b = blockIdx.x >= a_blocks;
+    const uint64_t blk = side_b ? blockIdx.x - a_blocks : blockIdx.x;
+    const uint64_t *off = side_b ? b_off : a_off;
+    const uint32_t *col = side_b ? b_col : a_col;
+    const double *val = side_b ? b_val : a_val;
+    const uint64_t *o_off = side_b ? a_off : b_off;
+    const uint32_t *o_col = side_b ? a_col : b_col;
+    const double *o_val = side_b ? a_val : b_val;
+    const uint64_t nnz = side_b ? b_nnz : a_nnz;
+    const uint64_t *s_own = side_b ? sb : sa;
+    const uint64_t *s_oth = side_b ? sa : sb;
+    const uint64_t first = blk * kThreads * kItems;
+    if (first >= nnz) return;
+    uint64_t rows[kItems];
+    entry_rows(off, n_rows, nnz, first, s_off, rows);
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+        const uint64_t k = first + (uint64_t)it * kThreads + threadIdx.x;
+        if (k >= nnz) break;
+        if (s_own[k + 1] == s_own[k]) continue;  // not kept
+        const uint64_t r = rows[it];
+        const uint32_t c = col[k];
+        const uint64_t lo = o_off[r], hi = o_off[r + 1];
+        const uint64_t p = lower_bound_u32(o_col, lo, hi, c);
+        const uint64_t pos = c_off[r] + (s_own[k] - s_own[off[r]]) + (s_oth[p] - s_oth[lo]);
+        double v;
+        if (!side_b) v = (p < hi && o_col[p] == c) ? val[k] + scale_b(o_val[p], subtract) : val[k];
+        else v = scale_b(val[k], subtract);
+        c_col[pos] = c;
+        c_val[pos] = v;
+    }
+}
+
+// ---- SpGEMM ----------------------------------------------------------------
+
+// Per row of A: its product count ub[i] and a size class; rows with products
+// are appended to lists[cls * n_rows + ...] (order inside a list is free: rows
+// are independent).  cnt[i] = 0 for rows without products.
+__global__ void k_spgemm_classify(uint64_t n_rows, const uint64_t *__restrict__ a_off,
+                                  const uint32_t *__restrict__ a_col, const uint64_t *__restrict__ b_off,
+                                  uint64_t *__restrict__ ub, uint64_t *__restrict__ cnt, uint32_t *__restrict__ lists,
+                                  uint32_t *__restrict__ list_n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int cls = -1;
+    if (i < n_rows) {
+        const uint64_t a0 = a_off[i], na = a_off[i + 1] - a0;
+        uint64_t u = 0;
+        for (uint64_t e = 0; e < na; ++e) {
+            const uint32_t k = a_col[a0 + e];
+            u += b_off[k + 1] - b_off[k];
+        }
+        ub[i] = u;
+        cnt[i] = 0;
+        if (u > 0) cls = (na <= kSmallCap && u <= kSmallCap) ? 0 : (na <= kLargeCap && u <= kLargeCap) ? 1 : 2;
+    }
+    // one atomic per wavefront and class
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    for (int c = 0; c < 3; ++c) {
+        const uint64_t m = __ballot(cls == c);
+        if (!m) continue;
+        const uint32_t leader = __ffsll((unsigned long long)m) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&list_n[c], (uint32_t)__popcll(m));
+        base = __shfl(base, leader, kWave);
+        if (cls == c) lists[(uint64_t)c * n_rows + base + __popcll(m & ((1ull << lane) - 1))] = (uint32_t)i;
+    }
+}
+
+// Inclusive scan of v over a workgroup of T threads; total = the sum.
+template <int T>
+__device__ __forceinline__ uint32_t block_incl_scan(uint32_t v, uint32_t *s_w, uint32_t &total)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const uint32_t u = __shfl_up(v, d, kWave);
+        if (lane >= d) v += u;
+    }
+    if constexpr (T == kWave) {
+        total = __shfl(v, kWave - 1, kWave);
+        return v;
+    } else {
+        const int w = threadIdx.x / kWave;
+        if (lane == kWave - 1) s_w[w] = v;
+        __syncthreads();
+        uint32_t pre = 0, tot = 0;
+#pragma unroll
+        for (int i = 0; i < T / kWave; ++i) {
+            const uint32_t x = s_w[i];
+            pre += i < w ? x : 0;
+            tot += x;
+        }
+        __syncthreads();  // s_w free again
+        total = tot;
+        return v + pre;
+    }
+}
+
+template <int T, uint32_t CAP, bool WRITE>
+__global__ __launch_bounds__(T) void k_spgemm_rows(const uint32_t *__restrict__ list, uint32_t n_list,
+                                                    const uint64_t *__restrict__ a_off,
+                                                    const uint32_t *__restrict__ a_col,
+                                                    const double *__restrict__ a_val,
+                                                    const uint64_t *__restrict__ b_off,
+                                                    const uint32_t *__restrict__ b_col,
+                                                    const double *__restrict__ b_val, uint64_t *__restrict__ cnt,
+                                                    const uint64_t *__restrict__ c_off, uint32_t *__restrict__ c_col,
+                                                    double *__restrict__ c_val)
+{
+#pragma clang fp contract(off)
+    __shared__ uint64_t s_key[CAP];    // (column << 32) | position in the reference's order
+    __shared__ double s_prod[CAP];     // products by position
+    __shared__ uint32_t s_pre[CAP + 1];  // first position of each A entry's products
+    __shared__ uint64_t s_start[CAP];  // B row start of each A entry
+    __shared__ double s_a[CAP];        // A values
+    __shared__ uint32_t s_w[T / kWave];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t li = blockIdx.x; li < n_list; li += gridDim.x) {
+        const uint32_t i = list[li];
+        const uint64_t a0 = a_off[i];
+        const uint32_t na = (uint32_t)(a_off[i + 1] - a0);  // <= CAP (classified)
+        // 1. A's entries and the prefix of their B row lengths
+        uint32_t carry = 0;
+        for (uint32_t base = 0; base < na; base += T) {
+            const uint32_t e = base + tid;
+            uint32_t len = 0;
+            if (e < na) {
+                const uint32_t k = a_col[a0 + e];
+                const uint64_t bs = b_off[k];
+                len = (uint32_t)(b_off[k + 1] - bs);
+                s_start[e] = bs;
+                s_a[e] = a_val[a0 + e];
+            }
+            uint32_t tot;
+            const uint32_t incl = block_incl_scan<T>(len, s_w, tot);
+            if (e < na) s_pre[e] = carry + incl - len;
+            carry += tot;
+        }
+        const uint32_t ub = carry;  // 1 .. CAP
+        if (tid == 0) s_pre[na] = ub;
+        uint32_t p2 = 2;
+        while (p2 < ub) p2 <<= 1;
+        __syncthreads();
+        // 2. expand: position q -> A entry e (last e with s_pre[e] <= q), B entry t
+        for (uint32_t q = tid; q < p2; q += T) {
+            if (q < ub) {
+                uint32_t lo = 0, hi = na;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) / 2;
+                    if (s_pre[mid] <= q) lo = mid;
+                    else hi = mid;
+                }
+                const uint64_t p = s_start[lo] + (q - s_pre[lo]);
+                s_prod[q] = s_a[lo] * b_val[p];
+                s_key[q] = ((uint64_t)b_col[p] << 32) | q;
+            } else {
+                s_key[q] = ~0ull;
+            }
+        }
+        __syncthreads();
+        // 3. bitonic sort of s_key[0, p2)
+        for (uint32_t size = 2; size <= p2; size <<= 1) {
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                for (uint32_t t = tid; t < p2 / 2; t += T) {
+                    const uint32_t lo = ((t & ~(stride - 1)) << 1) | (t & (stride - 1));
+                    const uint32_t hi = lo + stride;
+                    const uint64_t x = s_key[lo], y = s_key[hi];
+                    if ((x > y) == ((lo & size) == 0)) {
+                        s_key[lo] = y;
+                        s_key[hi] = x;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // 4. one lane per column run: sum from 0.0 in position order
+        uint32_t kept = 0;
+        for (uint32_t base = 0; base < ub; base += T) {
+            const uint32_t s = base + tid;
+            uint32_t keep = 0, c = 0;
+            double sum = 0.0;
+            if (s < ub) {
+                c = (uint32_t)(s_key[s] >> 32);
+                if (s == 0 || (uint32_t)(s_key[s - 1] >> 32) != c) {
+                    for (uint32_t u = s; u < ub && (uint32_t)(s_key[u] >> 32) == c; ++u)
+                        sum = sum + s_prod[(uint32_t)s_key[u]];
+                    keep = sum != 0.0;
+                }
+            }
+            uint32_t tot;
+            const uint32_t incl = block_incl_scan<T>(keep, s_w, tot);
+            if (WRITE && keep) {
+                const uint64_t pos = c_off[i] + kept + incl - 1;
+                c_col[pos] = c;
+                c_val[pos] = sum;
+            }
+            kept += tot;
+        }
+        if (!WRITE && tid == 0) cnt[i] = kept;
+        __syncthreads();  // LDS reused by the next row
+    }
+}
+
+// ---- rows with many products: expand, sort, compress in HBM ----------------
+
+__global__ void k_spgemm_big_ub(const uint32_t *__restrict__ list, uint32_t n, const uint64_t *__restrict__ ub,
+                                uint64_t *__restrict__ u)
+{
+    const uint64_t l = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < n) u[l] = ub[list[l]];
+}
+
+__global__ __launch_bounds__(kThreads) void k_spgemm_big_expand(
+    const uint32_t *__restrict__ list, uint32_t n, const uint64_t *__restrict__ u, unsigned cb,
+    const uint64_t *__restrict__ a_off, const uint32_t *__restrict__ a_col, const double *__restrict__ a_val,
+    const uint64_t *__restrict__ b_off, const uint32_t *__restrict__ b_col, const double *__restrict__ b_val,
+    uint64_t *__restrict__ key, double *__restrict__ prod)
+{
+#pragma clang fp contract(off)
+    for (uint32_t l = blockIdx.x; l < n; l += gridDim.x) {
+        const uint32_t i = list[l];
+        uint64_t pos = u[l];
+        for (uint64_t ka = a_off[i]; ka < a_off[i + 1]; ++ka) {
+            const uint32_t k = a_col[ka];
+            const double a = a_val[ka];
+            const uint64_t bs = b_off[k], len = b_off[k + 1] - bs;
+            for (uint64_t t = threadIdx.x; t < len; t += kThreads) {
+                key[pos + t] = ((uint64_t)l << cb) | b_col[bs + t];
+                prod[pos + t] = a * b_val[bs + t];
+            }
+            pos += len;
+        }
+    }
+}
+
+// One thread per run of equal keys (a (row, column) pair): the sum from 0.0 in
+// the sorted order, which the stable sort left in the reference's order.
+__global__ void k_spgemm_big_runs(const uint64_t *__restrict__ skey, const double *__restrict__ sprod, uint64_t P,
+                                  double *__restrict__ sum_out, uint64_t *__restrict__ keep)
+{
+#pragma clang fp contract(off)
+    for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < P;
+         s += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = skey[s];
+        uint64_t kp = 0;
+        if (s == 0 || skey[s - 1] != k) {
+            double sum = 0.0;
+            for (uint64_t v = s; v < P && skey[v] == k; ++v) sum = sum + sprod[v];
+            sum_out[s] = sum;
+            kp = sum != 0.0;
+        }
+        keep[s] = kp;
+    }
+}
+
+__global__ void k_spgemm_big_counts(const uint32_t *__restrict__ list, uint32_t n, const uint64_t *__restrict__ u,
+                                    const uint64_t *__restrict__ sk, uint64_t *__restrict__ cnt)
+{
+    const uint64_t l = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < n) cnt[list[l]] = sk[u[l + 1]] - sk[u[l]];
+}
+
+__global__ void k_spgemm_big_write(const uint64_t *__restrict__ skey, const double *__restrict__ sums, uint64_t P,
+                                   unsigned cb, const uint32_t *__restrict__ list, const uint64_t *__restrict__ u,
+                                   const uint64_t *__restrict__ sk, const uint64_t *__restrict__ c_off,
+                                   uint32_t *__restrict__ c_col, double *__restrict__ c_val)
+{
+    for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < P;
+         s += (uint64_t)gridDim.x * blockDim.x) {
+        if (sk[s + 1] == sk[s]) continue;
+        const uint64_t k = skey[s];
+        const uint32_t l = (uint32_t)(k >> cb);
+        const uint64_t pos = c_off[list[l]] + (sk[s] - sk[u[l]]);
+        c_col[pos] = (uint32_t)(k & ((1ull << cb) - 1));
+        c_val[pos] = sums[s];
+    }
+}
+
+unsigned bits_for(uint64_t max_value)
+{
+    unsigned b = 1;
+    while (b < 64 && (1ull << b) <= max_value) ++b;
+    return b;
+}
+
+// Frees device blocks on every exit path.
+struct scratch_set {
+    spira_dev *dev;
+    std::vector<std::pair<void *, size_t>> blocks;
+    void *get(size_t bytes)
+    {
+        void *p = dev_alloc(dev, bytes ? bytes : 8);
+        blocks.push_back({p, bytes ? bytes : 8});
+        return p;
+    }
+    void keep(void *p)
+    {
+        for (auto &b : blocks)
+            if (b.first == p) b.first = nullptr;
+    }
+    ~scratch_set()
+    {
+        for (auto &b : blocks)
+            if (b.first) dev_release(dev, b.first, b.second);
+    }
+};
+
+void install(spira_dcsr *C, uint64_t *off, uint32_t *col, double *val, uint64_t nnz)
+{
+    free_dcsr_arrays(C);
+    C->off = off;
+    C->col = col;
+    C->val = val;
+    C->nnz = nnz;
+    C->capacity = nnz;
+    C->built = true;
+    C->has_zeros = false;
+}
+
+int run_guarded(spira_dev *dev, const std::function<void()> &body)
+{
+    try {
+        SPIRA_HIP(hipSetDevice(dev->device));
+        body();
+        return SPIRA_OK;
+    } catch (const status_error &e) {
+        (void)hipStreamSynchronize(dev->stream);
+        dev->last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        (void)hipStreamSynchronize(dev->stream);
+        dev->last_error = "host allocation failed";
+        return SPIRA_ERR_NOMEM;
+    } catch (const std::exception &e) {
+        (void)hipStreamSynchronize(dev->stream);
+        dev->last_error = e.what();
+        return SPIRA_ERR_DEVICE;
+    }
+}
+
+}  // namespace
+
+}  // namespace spira_amd
+
+using namespace spira_amd;
+
+extern "C" int spira_add_f64(spira_dcsr *A, spira_dcsr *B, spira_dcsr *C, uint32_t flags)
+{
+    if (!A || !B || !C) return SPIRA_ERR_INVALID_ARGUMENT;
+    spira_dev *dev = A->dev;
+    return run_guarded(dev, [&] {
+        if (B->dev != dev || C->dev != dev) fail(SPIRA_ERR_INVALID_ARGUMENT, "add: A, B and C belong to different devices");
+        if (A->n_rows != B->n_rows || A->n_cols != B->n_cols)
+            fail(SPIRA_ERR_INVALID_ARGUMENT, "Matrices aren't the same size");
+        if (!A->built) fail(SPIRA_ERR_LOGIC, "MatrixAddition: A must be locked");
+        if (!B->built) fail(SPIRA_ERR_LOGIC, "MatrixAddition: B must be locked");
+        if (C->n_rows != A->n_rows || C->n_cols != A->n_cols)
+            fail(SPIRA_ERR_INVALID_ARGUMENT, "add: C must have the shape of A and B");
+        if (flags & ~(uint32_t)SPIRA_ADD_SUBTRACT) fail(SPIRA_ERR_INVALID_ARGUMENT, "add: unknown flags");
+        const uint64_t n = A->n_rows, na = A->nnz, nb = B->nnz;
+        const uint64_t blocks_a = (na + (uint64_t)kThreads * kItems - 1) / ((uint64_t)kThreads * kItems);
+        const uint64_t blocks_b = (nb + (uint64_t)kThreads * kItems - 1) / ((uint64_t)kThreads * kItems);
+        if (blocks_a + blocks_b >= (1ull << 31)) fail(SPIRA_ERR_UNSUPPORTED, "add: too many entries");
+        const int sub = (flags & SPIRA_ADD_SUBTRACT) ? 1 : 0;
+        hipStream_t st = dev->stream;
+        scratch_set tmp{dev, {}};
+        uint64_t *sa = (uint64_t *)tmp.get((na + 1) * sizeof(uint64_t));
+        uint64_t *sb = (uint64_t *)tmp.get((nb + 1) * sizeof(uint64_t));
+        uint64_t *c_off = (uint64_t *)tmp.get((n + 1) * sizeof(uint64_t));
+        if (blocks_a + blocks_b > 0) {
+            k_add_flags<<<(unsigned)(blocks_a + blocks_b), kThreads, 0, st>>>(
+                n, A->off, A->col, A->val, na, (uint32_t)blocks_a, B->off, B->col, B->val, nb, sub, sa, sb);
+            SPIRA_HIP(hipGetLastError());
+        }
+        scan_excl_u64(dev, sa, sa, na);
+        scan_excl_u64(dev, sb, sb, nb);
+        k_add_offsets<<<grid_for(n + 1, kThreads, 8192), kThreads, 0, st>>>(n, A->off, sa, B->off, sb, c_off);
+        SPIRA_HIP(hipGetLastError());
+        const uint64_t nnz = read_scalar(dev, c_off + n);
+        uint32_t *c_col = (uint32_t *)tmp.get(nnz * sizeof(uint32_t));
+        double *c_val = (double *)tmp.get(nnz * sizeof(double));
+        if (blocks_a + blocks_b > 0 && nnz > 0) {
+            k_add_scatter<<<(unsigned)(blocks_a + blocks_b), kThreads, 0, st>>>(
+                n, A->off, A->col, A->val, na, (uint32_t)blocks_a, B->off, B->col, B->val, nb, sub, sa, sb, c_off,
+                c_col, c_val);
+            SPIRA_HIP(hipGetLastError());
+        }
+        tmp.keep(c_off);
+        tmp.keep(c_col);
+        tmp.keep(c_val);
+        install(C, c_off, c_col, c_val, nnz);  // C may be A or B: stream order protects their arrays
+    });
+}
+
+extern "C" int spira_spgemm_f64(spira_dcsr *A, spira_dcsr *B, spira_dcsr *C)
+{
+    if (!A || !B || !C) return SPIRA_ERR_INVALID_ARGUMENT;
+    spira_dev *dev = A->dev;
+    return run_guarded(dev, [&] {
+        if (B->dev != dev || C->dev != dev)
+            fail(SPIRA_ERR_INVALID_ARGUMENT, "spgemm: A, B and C belong to different devices");
+        if (A->n_cols != B->n_rows) fail(SPIRA_ERR_INVALID_ARGUMENT, "A.cols must equal B.rows");
+        if (!A->built) fail(SPIRA_ERR_LOGIC, "spgemm: A must be locked");
+        if (!B->built) fail(SPIRA_ERR_LOGIC, "spgemm: B must be locked");
+        if (C->n_rows != A->n_rows || C->n_cols != B->n_cols)
+            fail(SPIRA_ERR_INVALID_ARGUMENT, "spgemm: C must have A's rows and B's columns");
+        const uint64_t n = A->n_rows, m = B->n_cols;
+        if (n >= (1ull << 32) - 1) fail(SPIRA_ERR_UNSUPPORTED, "spgemm: needs fewer than 2^32 - 1 rows");
+        hipStream_t st = dev->stream;
+        scratch_set tmp{dev, {}};
+        uint64_t *cnt = (uint64_t *)tmp.get((n + 1) * sizeof(uint64_t));  // row counts, then C's offsets
+        uint64_t *ub = (uint64_t *)tmp.get((n ? n : 1) * sizeof(uint64_t));
+        uint32_t *lists = (uint32_t *)tmp.get(3 * (n ? n : 1) * sizeof(uint32_t));
+        uint32_t *list_n = (uint32_t *)dev->ws.get(WS_MISC, 4 * sizeof(uint32_t));
+        SPIRA_HIP(hipMemsetAsync(list_n, 0, 4 * sizeof(uint32_t), st));
+        uint32_t ln[4] = {0, 0, 0, 0};
+        if (n > 0) {
+            k_spgemm_classify<<<grid_for(n, kThreads), kThreads, 0, st>>>(n, A->off, A->col, B->off, ub, cnt, lists,
+                                                                          list_n);
+            SPIRA_HIP(hipGetLastError());
+            SPIRA_HIP(hipMemcpyAsync(ln, list_n, sizeof(ln), hipMemcpyDeviceToHost, st));
+            dev_sync(dev);
+        }
+        const uint32_t n0 = ln[0], n1 = ln[1], n2 = ln[2];
+        const uint32_t *l0 = lists, *l1 = lists + n, *l2 = lists + 2 * n;
+        const unsigned g0 = (unsigned)std::min<uint64_t>(n0, 65536), g1 = (unsigned)std::min<uint64_t>(n1, 32768);
+        if (n0)
+            k_spgemm_rows<64, kSmallCap, false><<<g0, 64, 0, st>>>(l0, n0, A->off, A->col, A->val, B->off, B->col,
+                                                                     B->val, cnt, nullptr, nullptr, nullptr);
+        if (n1)
+            k_spgemm_rows<256, kLargeCap, false><<<g1, 256, 0, st>>>(l1, n1, A->off, A->col, A->val, B->off, B->col,
+                                                                       B->val, cnt, nullptr, nullptr, nullptr);
+        SPIRA_HIP(hipGetLastError());
+        // rows with many products
+        uint64_t P = 0;
+        unsigned cb = 0;
+        uint64_t *u = nullptr, *skey = nullptr, *keep = nullptr;
+        double *sums = nullptr;
+        if (n2) {
+            u = (uint64_t *)tmp.get(((uint64_t)n2 + 1) * sizeof(uint64_t));
+            k_spgemm_big_ub<<<grid_for(n2, kThreads), kThreads, 0, st>>>(l2, n2, ub, u);
+            SPIRA_HIP(hipGetLastError());
+            scan_excl_u64(dev, u, u, n2);
+            P = read_scalar(dev, u + n2);
+            if (P >= (1ull << 32)) fail(SPIRA_ERR_UNSUPPORTED, "spgemm: more than 2^32 products in long rows");
+            cb = bits_for(m - 1);
+            const unsigned rb = bits_for(n2 - 1);
+            uint64_t *key = (uint64_t *)tmp.get(P * sizeof(uint64_t));
+            double *prod = (double *)tmp.get(P * sizeof(double));
+            skey = (uint64_t *)tmp.get(P * sizeof(uint64_t));
+            double *sprod = (double *)tmp.get(P * sizeof(double));
+            keep = (uint64_t *)tmp.get((P + 1) * sizeof(uint64_t));
+            k_spgemm_big_expand<<<std::min<uint32_t>(n2, 8192), kThreads, 0, st>>>(
+                l2, n2, u, cb, A->off, A->col, A->val, B->off, B->col, B->val, key, prod);
+            SPIRA_HIP(hipGetLastError());
+            size_t tb = 0;
+            SPIRA_HIP(rocprim::radix_sort_pairs(nullptr, tb, key, skey, prod, sprod, (unsigned)P, 0, cb + rb, st));
+            void *t = dev->ws.get(WS_SORT, tb ? tb : 8);
+            SPIRA_HIP(rocprim::radix_sort_pairs(t, tb, key, skey, prod, sprod, (unsigned)P, 0, cb + rb, st));
+            sums = prod;  // the unsorted products are no longer needed
+            k_spgemm_big_runs<<<grid_for(P, kThreads, 16384), kThreads, 0, st>>>(skey, sprod, P, sums, keep);
+            SPIRA_HIP(hipGetLastError());
+            scan_excl_u64(dev, keep, keep, P);
+            k_spgemm_big_counts<<<grid_for(n2, kThreads), kThreads, 0, st>>>(l2, n2, u, keep, cnt);
+            SPIRA_HIP(hipGetLastError());
+        }
+        scan_excl_u64(dev, cnt, cnt, n);
+        const uint64_t nnz = read_scalar(dev, cnt + n);
+        uint32_t *c_col = (uint32_t *)tmp.get(nnz * sizeof(uint32_t));
+        double *c_val = (double *)tmp.get(nnz * sizeof(double));
+        if (nnz > 0) {
+            if (n0)
+                k_spgemm_rows<64, kSmallCap, true><<<g0, 64, 0, st>>>(l0, n0, A->off, A->col, A->val, B->off,
+                                                                        B->col, B->val, nullptr, cnt, c_col, c_val);
+            if (n1)
+                k_spgemm_rows<256, kLargeCap, true><<<g1, 256, 0, st>>>(l1, n1, A->off, A->col, A->val, B->off,
+                                                                          B->col, B->val, nullptr, cnt, c_col, c_val);
+            SPIRA_HIP(hipGetLastError());
+            if (n2) {
+                k_spgemm_big_write<<<grid_for(P, kThreads, 16384), kThreads, 0, st>>>(skey, sums, P, cb, l2, u, keep,
+                                                                                     cnt, c_col, c_val);
+                SPIRA_HIP(hipGetLastError());
+            }
+        }
+        tmp.keep(cnt);
+        tmp.keep(c_col);
+        tmp.keep(c_val);
+        install(C, cnt, c_col, c_val, nnz);
+        dev_sync(dev);  // temporaries go back to the cache after the kernels that use them
+    });
+}

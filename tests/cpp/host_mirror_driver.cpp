@@ -17,6 +17,12 @@
 //                                               serial::algorithms::transpose, or
 //                                               parallel::algorithms::transpose)
 //   transpose_itself                         -> ok
+//   push                                     -> ok  (the matrix moves onto a stack; no current matrix)
+//   add | sub | mul | iadd | isub | imul     -> ok  (left = popped stack top, right = the current
+//                                               matrix; the result becomes the current matrix:
+//                                               MatrixAddition / operator- / spgemm, the parallel
+//                                               algorithms when both are parallel_matrix, and the
+//                                               in-place operators +=, -=, *= on the left)
 // Any exception -> "error <kind> <message>".
 #include "spira_amd.hpp"
 
@@ -32,6 +38,7 @@ namespace {
 
 std::unique_ptr<spira_amd::matrix> g_mat;
 spira_amd::parallel_matrix *g_par = nullptr;
+std::vector<std::unique_ptr<spira_amd::matrix>> g_stack;
 
 std::string hexf(double v)
 {
@@ -186,6 +193,38 @@ std::string run(const std::string &line)
             g_mat = std::move(p);
         } else {
             g_mat = std::make_unique<spira_amd::matrix>(spira_amd::serial::algorithms::transpose(mat()));
+        }
+        out << "ok";
+    } else if (cmd == "push") {
+        g_stack.push_back(std::move(g_mat));
+        g_par = nullptr;
+        out << "ok";
+    } else if (cmd == "add" || cmd == "sub" || cmd == "mul" || cmd == "iadd" || cmd == "isub" || cmd == "imul") {
+        if (g_stack.empty() || !g_stack.back()) throw std::logic_error("driver: empty stack");
+        std::unique_ptr<spira_amd::matrix> left = std::move(g_stack.back());
+        g_stack.pop_back();
+        auto *lp = dynamic_cast<spira_amd::parallel_matrix *>(left.get());
+        if (lp && g_par && (cmd == "add" || cmd == "mul")) {
+            auto p = std::make_unique<spira_amd::parallel_matrix>(
+                cmd == "add" ? spira_amd::parallel::algorithms::MatrixAddition(*lp, *g_par)
+                             : spira_amd::parallel::algorithms::spgemm(*lp, *g_par));
+            g_par = p.get();
+            g_mat = std::move(p);
+        } else {
+            const spira_amd::matrix &right = mat();
+            std::unique_ptr<spira_amd::matrix> res;
+            if (cmd == "add") res = std::make_unique<spira_amd::matrix>(spira_amd::serial::algorithms::MatrixAddition(*left, right));
+            else if (cmd == "sub") res = std::make_unique<spira_amd::matrix>(*left - right);
+            else if (cmd == "mul") res = std::make_unique<spira_amd::matrix>(spira_amd::serial::algorithms::spgemm(*left, right));
+            else {
+                spira_amd::matrix l2 = std::move(*left);
+                if (cmd == "iadd") l2 += right;
+                else if (cmd == "isub") l2 -= right;
+                else l2 *= right;
+                res = std::make_unique<spira_amd::matrix>(std::move(l2));
+            }
+            g_par = nullptr;
+            g_mat = std::move(res);
         }
         out << "ok";
     } else if (cmd == "transpose_itself") {

@@ -392,3 +392,101 @@ def test_transpose_random_matches_oracle_on_cpp_mirror(drv, parallel):
     assert all(np.array_equal(a, b) for a, b in zip(got, exp))
     if parallel:
         assert m.bounds() == [t * n_cols // 4 for t in range(5)]
+
+
+def _cpp_pair(drv, case, inner=False):
+    """Left operand pushed on the driver's stack, right operand current."""
+    threads = case.get("n_threads") if case["mode"] == "parallel" else None
+    lock = case.get("lock", True)
+    n_cols_a = case["n_inner"] if inner else case["n_cols"]
+    A = CppMatrix(drv, case["n_rows"], n_cols_a, threads=threads)
+    for r, c, v in case["a"]:
+        A.insert(r, c, v)
+    if lock:
+        A.lock()
+    a_csr = A.csr()
+    drv.cmd("push")
+    br, bc = case.get("b_shape", (case["n_inner"], case["n_cols"]) if inner else (case["n_rows"], case["n_cols"]))
+    B = CppMatrix(drv, br, bc, threads=threads)
+    for r, c, v in case["b"]:
+        B.insert(r, c, v)
+    if lock:
+        B.lock()
+    return B, a_csr, B.csr()
+
+
+def _cpp_check(m, case):
+    off, col, val = m.csr()
+    if "expect_rows" in case:
+        for r, exp in enumerate(case["expect_rows"]):
+            assert [[int(col[k]), float(val[k])] for k in range(int(off[r]), int(off[r + 1]))] == exp
+    for r, c, v in case.get("expect", []):
+        assert m.get(r, c) == v
+    for r, c in case.get("expect_zero", []):
+        assert not m.contains(r, c)
+    if "nnz" in case:
+        assert m.nnz() == case["nnz"]
+    assert not np.any(val == 0.0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", KATS["add"], ids=lambda c: f'{c["source"]}:{c.get("op", "")}')
+def test_add_known_answers_on_cpp_mirror(drv, case):
+    """serial / parallel MatrixAddition and operator+ / operator- through the C++ mirror."""
+    m, a, b = _cpp_pair(drv, case)
+    cmd = "sub" if case.get("op") == "-" else "add"
+    if "error" in case:
+        with pytest.raises(ValueError) as e:
+            drv.cmd(cmd)
+        assert str(e.value) == case["message"]
+        return
+    drv.cmd(cmd)
+    assert m.is_locked()
+    _cpp_check(m, case)
+    assert all(np.array_equal(x, y) for x, y in zip(m.csr(), oracle.add(a, b, subtract=cmd == "sub")))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", KATS["spgemm"], ids=lambda c: c["source"])
+def test_spgemm_known_answers_on_cpp_mirror(drv, case):
+    m, a, b = _cpp_pair(drv, case, inner=True)
+    if "error" in case:
+        with pytest.raises(ValueError) as e:
+            drv.cmd("mul")
+        assert str(e.value) == case["message"]
+        return
+    drv.cmd("mul")
+    _cpp_check(m, case)
+    exp = oracle.spgemm(a, b, case["n_cols"])
+    got = m.csr()
+    assert all(np.array_equal(x, y) for x, y in zip(got, exp))
+
+
+@pytest.mark.gpu
+def test_in_place_operators_on_cpp_mirror(drv):
+    rng = np.random.default_rng(23)
+    ins_a = [(int(rng.integers(0, 40)), int(rng.integers(0, 40)), float(rng.integers(-3, 4))) for _ in range(300)]
+    ins_b = [(int(rng.integers(0, 40)), int(rng.integers(0, 40)), float(rng.integers(-3, 4))) for _ in range(300)]
+    for op in ("iadd", "isub", "imul"):
+        A = CppMatrix(drv, 40, 40)
+        for t in ins_a:
+            A.insert(*t)
+        A.lock()
+        a = A.csr()
+        drv.cmd("push")
+        B = CppMatrix(drv, 40, 40)
+        for t in ins_b:
+            B.insert(*t)
+        B.lock()
+        b = B.csr()
+        drv.cmd(op)
+        exp = oracle.spgemm(a, b, 40) if op == "imul" else oracle.add(a, b, subtract=op == "isub")
+        assert all(np.array_equal(x, y) for x, y in zip(B.csr(), exp)), op
+    # operator- refuses an open right operand first (its scaled copy needs it locked)
+    A = CppMatrix(drv, 3, 3)
+    A.lock()
+    drv.cmd("push")
+    CppMatrix(drv, 3, 3)
+    with pytest.raises(RuntimeError) as e:
+        drv.cmd("sub")
+    assert str(e.value) == "multiplication_scaler: input matrix must be locked"

@@ -419,3 +419,138 @@ def test_planted_transpose_unstable_fill_is_caught():
     assert csr_equal(good, oracle.transpose(off, col, val, 30))
     bad = _transpose_py(off, col, val, 30, descending_rows=True)
     assert not np.array_equal(bad[1], ref.indices.astype(np.uint32))
+
+
+def _trip_csr(n_rows, ins):
+    """Locked CSR of an insert list (the reference's test helpers insert and lock)."""
+    ptr, c, v = stage(n_rows, [t[0] for t in ins], [t[1] for t in ins], [t[2] for t in ins])
+    return oracle.lock(ptr, c, v)
+
+
+def _dense_of(csr, n_cols):
+    off, col, val = csr
+    d = np.zeros((len(off) - 1, n_cols))
+    for r in range(len(off) - 1):
+        for k in range(int(off[r]), int(off[r + 1])):
+            d[r, col[k]] = val[k]
+    return d
+
+
+def _check_rows(csr, case, n_cols):
+    if "expect_rows" in case:
+        off, col, val = csr
+        for r, exp in enumerate(case["expect_rows"]):
+            seg = [[int(col[k]), float(val[k])] for k in range(int(off[r]), int(off[r + 1]))]
+            assert seg == exp
+    for r, c, v in case.get("expect", []):
+        assert _dense_get(*csr, r, c) == v
+    for r, c in case.get("expect_zero", []):
+        lo, hi = int(csr[0][r]), int(csr[0][r + 1])
+        assert c not in csr[1][lo:hi]
+    if "nnz" in case:
+        assert int(csr[0][-1]) == case["nnz"]
+    if "expect_dense" in case:
+        assert np.array_equal(_dense_of(csr, n_cols), np.array(case["expect_dense"]))
+    assert not np.any(csr[2] == 0.0)  # zeros never stored
+
+
+@pytest.mark.parametrize("case", [c for c in KATS["add"] if "error" not in c], ids=lambda c: c["source"])
+def test_add_golden(case):
+    a = _trip_csr(case["n_rows"], case["a"])
+    b = _trip_csr(case["n_rows"], case["b"])
+    c = oracle.add(a, b, subtract=case.get("op") == "-")
+    _check_rows(c, case, case["n_cols"])
+
+
+@pytest.mark.parametrize("case", [c for c in KATS["spgemm"] if "error" not in c], ids=lambda c: c["source"])
+def test_spgemm_golden(case):
+    a = _trip_csr(case["n_rows"], case["a"])
+    b = _trip_csr(case["n_inner"], case["b"])
+    c = oracle.spgemm(a, b, case["n_cols"])
+    _check_rows(c, case, case["n_cols"])
+    if case.get("swap_differs"):
+        assert not np.array_equal(_dense_of(oracle.spgemm(b, a, case["n_cols"]), case["n_cols"]),
+                                  _dense_of(c, case["n_cols"]))
+
+
+def _scipy_csr(M):
+    M = M.tocsr()
+    M.sort_indices()
+    return M.indptr.astype(np.uint64), M.indices.astype(np.uint32), M.data.astype(np.float64)
+
+
+def _rand_sp(n, m, density, seed, zeros=0.0):
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    M = sp.random(n, m, density=density, random_state=rng, format="csr")
+    M.data = rng.standard_normal(M.data.size)
+    if zeros:
+        M.data[rng.random(M.data.size) < zeros] = 0.0  # explicit zeros, as load_csr may store
+    M.sort_indices()
+    return M
+
+
+def _same_bits(x, y):
+    return (np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1])
+            and np.array_equal(np.asarray(x[2]).view(np.uint64), np.asarray(y[2]).view(np.uint64)))
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_add_and_spgemm_match_scipy(seed):
+    """scipy.sparse's CSR kernels (library routines) are bit-identical here:
+    csr_binop_csr merges sorted rows and drops results equal to zero;
+    csr_matmat accumulates sums[j] += a * b from 0 in A's then B's column
+    order and drops sums equal to zero -- the reference's semantics."""
+    rng = np.random.default_rng(seed)
+    n, k, m = (int(v) for v in rng.integers(1, 400, 3))
+    A = _rand_sp(n, k, 0.04, 10 + seed, zeros=0.05)
+    B = _rand_sp(k, m, 0.04, 20 + seed)
+    A2 = _rand_sp(n, k, 0.04, 30 + seed)
+    a, b, a2 = _scipy_csr(A), _scipy_csr(B), _scipy_csr(A2)
+    P = (A @ B).tocsr()
+    P.eliminate_zeros()
+    assert _same_bits(oracle.spgemm(a, b, m), _scipy_csr(P))
+    for sub in (False, True):
+        S = (A - A2 if sub else A + A2).tocsr()
+        S.eliminate_zeros()
+        assert _same_bits(oracle.add(a, a2, subtract=sub), _scipy_csr(S))
+
+
+def test_add_cancellation_and_identities():
+    """A + (-A) is empty, A - A is empty, A + 0 = A without its zeros."""
+    A = _rand_sp(200, 150, 0.05, 5, zeros=0.1)
+    a = _scipy_csr(A)
+    neg = (a[0], a[1], -a[2])
+    assert int(oracle.add(a, neg)[0][-1]) == 0
+    assert int(oracle.add(a, a, subtract=True)[0][-1]) == 0
+    z = (np.zeros(201, np.uint64), np.zeros(0, np.uint32), np.zeros(0))
+    A.eliminate_zeros()
+    assert _same_bits(oracle.add(a, z), _scipy_csr(A))
+
+
+def _spgemm_reversed_k(a, b, n_cols):
+    """A planted mistake: products summed in descending k order."""
+    ao, ac, av = a
+    bo, bc, bv = b
+    rows = []
+    for i in range(len(ao) - 1):
+        acc = {}
+        for ka in reversed(range(int(ao[i]), int(ao[i + 1]))):
+            k = int(ac[ka])
+            for kb in range(int(bo[k]), int(bo[k + 1])):
+                j = int(bc[kb])
+                acc[j] = acc.get(j, 0.0) + av[ka] * bv[kb]
+        rows.append(sorted((j, v) for j, v in acc.items() if v != 0.0))
+    off = np.zeros(len(rows) + 1, np.uint64)
+    np.cumsum([len(r) for r in rows], out=off[1:])
+    return off, np.array([j for r in rows for j, _ in r], np.uint32), np.array([v for r in rows for _, v in r])
+
+
+def test_planted_spgemm_order_mistake_is_caught():
+    A = _rand_sp(60, 80, 0.3, 7)
+    B = _rand_sp(80, 50, 0.3, 8)
+    a, b = _scipy_csr(A), _scipy_csr(B)
+    P = (A @ B).tocsr()
+    P.eliminate_zeros()
+    assert _same_bits(oracle.spgemm(a, b, 50), _scipy_csr(P))
+    assert not _same_bits(_spgemm_reversed_k(a, b, 50), _scipy_csr(P))
