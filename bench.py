@@ -43,6 +43,8 @@ def parse():
     p.add_argument("--lock-only", action="store_true", help="only the lock() measurement (profiling)")
     p.add_argument("--no-transpose", action="store_true", help="skip the transpose measurement")
     p.add_argument("--transpose-only", action="store_true", help="only the transpose measurement (profiling)")
+    p.add_argument("--no-algebra", action="store_true", help="skip the addition / SpGEMM measurement")
+    p.add_argument("--algebra-only", action="store_true", help="only the addition / SpGEMM measurement (profiling)")
     p.add_argument("--calib-copy-mb", type=int, default=0,
                    help="also run a plain device copy of this many MiB (PMC traffic calibration)")
     p.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision power-law section")
@@ -163,6 +165,51 @@ def run_transpose(args, np, torch, spira_amd, W, dev, stream, src=None):
     A.close()
     T.close()
     return detail, (o, c, v, got)
+
+
+def run_algebra(args, np, torch, spira_amd, W, dev, stream, base=None):
+    """Matrix addition (serial/matrix_addition.hpp:88-112) and SpGEMM
+    (serial/spgemm.hpp:12-50) on device-resident matrices: A + B for two
+    1,048,576^2 matrices with 16 random nnz/row, A * A for the same A (268M
+    products) and for the 2048^2 5-point Laplacian.  ms per call from HIP
+    events around back-to-back calls (allocation and the readbacks of the
+    output size included)."""
+    if base is None:
+        (base, _) = W.relock_workload(1 << 20)
+    n = len(base[0]) - 1
+    other, _ = W.relock_workload(n, seed=W.BASE_SEED + 7)
+    lap = W.laplacian_2d(2048)
+    A = spira_amd.DeviceCSR(dev, n, n)
+    A.upload(*base)
+    B = spira_amd.DeviceCSR(dev, n, n)
+    B.upload(*other)
+    L = spira_amd.DeviceCSR(dev, len(lap[0]) - 1, len(lap[0]) - 1)
+    L.upload(*lap)
+    C = spira_amd.DeviceCSR(dev, n, n)
+    CL = spira_amd.DeviceCSR(dev, L.n_rows, L.n_cols)
+    dev.synchronize()
+    out = {}
+    A.add(B, C)
+    dev.synchronize()
+    ms = _time_calls(torch, stream, lambda: A.add(B, C), 10)
+    nnz_c = C.nnz
+    byt = 2 * 8 * (n + 1) + 12 * (int(base[1].size) + int(other[1].size)) + 8 * (n + 1) + 12 * nnz_c
+    out["add"] = {"workload": "A + B, 1,048,576^2, 16 random nnz/row each", "nnz_a": int(base[1].size),
+                  "nnz_b": int(other[1].size), "nnz_c": int(nnz_c), "ms": round(ms, 3), "bytes_per_call": byt,
+                  "GB_s": round(byt / (ms * 1e-3) / 1e9, 1)}
+    for key, M, Cm, src in (("spgemm_random_1m", A, C, base), ("spgemm_laplacian_2048", L, CL, lap)):
+        M.spgemm(M, Cm)
+        dev.synchronize()
+        ms = _time_calls(torch, stream, lambda: M.spgemm(M, Cm), 3)
+        blen = np.diff(src[0].astype(np.int64))
+        prods = int(blen[src[1]].sum())
+        out[key] = {"workload": ("A * A, 1,048,576^2 with 16 random nnz/row" if key == "spgemm_random_1m"
+                                 else "A * A, 2048^2 5-point Laplacian"),
+                    "products": prods, "nnz_c": int(Cm.nnz), "ms": round(ms, 3),
+                    "GFLOP_s": round(2 * prods / (ms * 1e-3) / 1e9, 1)}
+    for M in (A, B, L, C, CL):
+        M.close()
+    return out, (base, other, lap)
 
 
 def _cpu_model():
@@ -437,6 +484,10 @@ def main():
         out = run_transpose(args, np, torch, spira_amd, W, dev, stream)
         print(json.dumps({"transpose": out[0]}), flush=True)
         return
+    if args.algebra_only:
+        out = run_algebra(args, np, torch, spira_amd, W, dev, stream)
+        print(json.dumps({"algebra": out[0]}), flush=True)
+        return
 
     generators = {
         "laplacian_2048": lambda: W.laplacian_2d(2048),
@@ -535,6 +586,10 @@ def main():
         args, np, torch, spira_amd, W, dev, stream, None if lock_src is None else lock_src[:3])
     if tr_detail is not None:
         log(f"bench: transpose {tr_detail['ms_serial']} ms ({tr_detail['GB_s']} GB/s)")
+    alg_detail, alg_src = (None, None) if args.no_algebra else run_algebra(
+        args, np, torch, spira_amd, W, dev, stream, None if lock_src is None else lock_src[:3])
+    if alg_detail is not None:
+        log(f"bench: add {alg_detail['add']['ms']} ms, spgemm {alg_detail['spgemm_random_1m']['ms']} ms")
     mixed_detail, mixed_src = (None, None) if args.no_mixed else run_mixed(args, np, torch, spira_amd, W, dev, stream)
     batch_detail = None if args.no_batch else run_batch(args, np, torch, spira_amd, W, dev, stream)
 
@@ -635,6 +690,24 @@ def main():
                 all(np.array_equal(a.view(np.uint8), b.view(np.uint8)) for a, b in zip(got, exp)))
             tr_detail["cpu_baseline_ms"] = round(1e3 * float(np.median(tt)), 2)
             tr_detail["cpu_baseline_threads"] = 1
+        if alg_detail is not None:
+            base, other, lap = alg_src
+            t = time.perf_counter()
+            oracle.add(base, other)
+            alg_detail["add"]["cpu_baseline_ms"] = round(1e3 * (time.perf_counter() - t), 2)
+            # SpGEMM on the CPU: a bounded sample of rows (the first 65,536 rows
+            # of A times all of A), scaled per product
+            rows = 65536
+            for key, src in (("spgemm_random_1m", base), ("spgemm_laplacian_2048", lap)):
+                off = src[0][:rows + 1]
+                sub = (off, src[1][:int(off[-1])], src[2][:int(off[-1])])
+                nprod = oracle.spgemm_products(sub, src)
+                t = time.perf_counter()
+                oracle.spgemm(sub, src, len(src[0]) - 1)
+                dt = time.perf_counter() - t
+                alg_detail[key]["cpu_baseline_GFLOP_s"] = round(2 * nprod / dt / 1e9, 3)
+                alg_detail[key]["cpu_baseline_sample"] = f"first {rows} rows of A times A ({nprod} products)"
+            alg_detail["cpu_baseline_threads"] = 1
         if mixed_detail is not None:
             # errors of every storage against the FP64 oracle and its
             # double-double reference (checker use)
@@ -680,6 +753,7 @@ def main():
         "detail": detail,
         "lock": lock_detail,
         "transpose": tr_detail,
+        "algebra": alg_detail,
         "mixed": mixed_detail,
         "batched": batch_detail,
     }

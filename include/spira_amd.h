@@ -208,8 +208,9 @@ int spira_add_f64(spira_dcsr *A, spira_dcsr *B, spira_dcsr *C, uint32_t flags);
  * accumulator order -- and entries whose sum is 0 are dropped.  Checks:
  * A.n_cols != B.n_rows (SPIRA_ERR_INVALID_ARGUMENT, "A.cols must equal
  * B.rows"), then A and B built (SPIRA_ERR_LOGIC); fewer than 2^32 - 1 rows,
- * and fewer than 2^32 products in the rows with more than 1024 products or
- * A entries (SPIRA_ERR_UNSUPPORTED). */
+ * at most 2^32 products in all (a scratch of 12 bytes per product) and fewer
+ * than 2^32 in the rows with more than 1024 products or A entries
+ * (SPIRA_ERR_UNSUPPORTED). */
 int spira_spgemm_f64(spira_dcsr *A, spira_dcsr *B, spira_dcsr *C);
 
 /* ---- mixed precision ------------------------------------------------------

@@ -21,17 +21,21 @@
 // a_ik * b_kj into acc[j] = 0.0 in the order of k (A's row) and, inside B's
 // row, of j; each product is rounded, then added (no fused multiply-add);
 // entries with acc[j] != 0 are kept.  The device keeps that order exactly:
-//   * k_spgemm_classify: per row of A its product count (sum of the B row
-//     lengths its columns select) and a size class;
-//   * k_spgemm_rows<T, CAP>: one workgroup per row with at most CAP products
-//     (and CAP entries of A): products expanded into LDS in the reference's
-//     order with their position as a tie-break key, a bitonic sort on
-//     (column, position), then one lane per column adds its products from
-//     0.0 in position order.  Run once to count the kept entries per row
-//     and, after a scan gives C's offsets, again to write them;
-//   * rows with more products (expand-sort-compress in HBM): products keyed
-//     by (row, column) in the reference's order, a stable radix sort, then
-//     one thread per (row, column) run adds it in order.
+//   * k_spgemm_classify: per row of A its product count (the B row lengths
+//     its columns select) and a size class; a scan of the counts gives each
+//     row a scratch area with room for one entry per product;
+//   * k_spgemm_lanes<G> (rows of <= 16, 32, 64 products): 64 / G rows per
+//     wavefront, all in registers -- lane g owns product g in the reference's
+//     order, a shuffle bitonic network sorts (column, g), and the first lane
+//     of each column run adds the run from 0.0 in order;
+//   * k_spgemm_rows<T, CAP> (<= 256, 1024 products): one workgroup per row,
+//     the same steps through LDS;
+//   * longer rows, expand-sort-compress in HBM: products keyed by (row,
+//     column) in the reference's order, a stable radix sort, one thread per
+//     (row, column) run;
+//   * every path writes a row's kept entries, sorted, to its scratch area and
+//     its count; a scan of the counts gives C's offsets and k_spgemm_gather
+//     moves the entries into C (one pass over the products in all).
 // Contraction into fused multiply-adds is switched off in the kernels that
 // round a product or a sum.
 #include "internal.hpp"
@@ -200,9 +204,16 @@ __global__ __launch_bounds__(kThreads) void k_add_scatter(uint64_t n_rows, const
 
 // ---- SpGEMM ----------------------------------------------------------------
 
-// Per row of A: its product count ub[i] and a size class; rows with products
-// are appended to lists[cls * n_rows + ...] (order inside a list is free: rows
-// are independent).  cnt[i] = 0 for rows without products.
+// Size classes of a row of C by its product count ub and A entries na:
+//   0..2: ub, na <= 16 / 32 / 64: k_spgemm_lanes<G>, 64 / G rows per wavefront;
+//   3, 4: ub, na <= 256 / 1024: k_spgemm_rows (LDS);
+//   5: the rest, expand-sort-compress in HBM.
+constexpr int kClasses = 6;
+constexpr uint32_t kClassCap[kClasses - 1] = {16, 32, 64, 256, 1024};
+
+// Per row of A: its product count ub[i] and its size class; rows with
+// products are appended to lists[cls * n_rows + ...] (order inside a list is
+// free: rows are independent).  cnt[i] = 0 for rows without products.
 __global__ void k_spgemm_classify(uint64_t n_rows, const uint64_t *__restrict__ a_off,
                                   const uint32_t *__restrict__ a_col, const uint64_t *__restrict__ b_off,
                                   uint64_t *__restrict__ ub, uint64_t *__restrict__ cnt, uint32_t *__restrict__ lists,
@@ -219,11 +230,17 @@ __global__ void k_spgemm_classify(uint64_t n_rows, const uint64_t *__restrict__ 
         }
         ub[i] = u;
         cnt[i] = 0;
-        if (u > 0) cls = (na <= kSmallCap && u <= kSmallCap) ? 0 : (na <= kLargeCap && u <= kLargeCap) ? 1 : 2;
+        if (u > 0) {
+            cls = kClasses - 1;
+            for (int c = kClasses - 2; c >= 0; --c)
+                if (na <= kClassCap[c] && u <= kClassCap[c]) cls = c;
+        }
+    } else if (i == n_rows) {
+        ub[i] = 0;  // the scan of ub reads n_rows + 1 words
     }
     // one atomic per wavefront and class
     const uint32_t lane = threadIdx.x & (kWave - 1);
-    for (int c = 0; c < 3; ++c) {
+    for (int c = 0; c < kClasses; ++c) {
         const uint64_t m = __ballot(cls == c);
         if (!m) continue;
         const uint32_t leader = __ffsll((unsigned long long)m) - 1;
@@ -231,6 +248,115 @@ __global__ void k_spgemm_classify(uint64_t n_rows, const uint64_t *__restrict__ 
         if (lane == leader) base = atomicAdd(&list_n[c], (uint32_t)__popcll(m));
         base = __shfl(base, leader, kWave);
         if (cls == c) lists[(uint64_t)c * n_rows + base + __popcll(m & ((1ull << lane) - 1))] = (uint32_t)i;
+    }
+}
+
+// Rows with at most G products and G entries of A, 64 / G rows per
+// wavefront, every step in registers: lane g of a row's G-lane group owns
+// product g (in the reference's order), the group sorts (column, g) with a
+// shuffle bitonic network, and the first lane of each column run adds the
+// run's products from 0.0 in order.  The row's kept entries go to
+// t_col/t_val at base[i]; cnt[i] = their number.
+template <int G>
+__global__ __launch_bounds__(256) void k_spgemm_lanes(const uint32_t *__restrict__ list, uint32_t n_list,
+                                                       const uint64_t *__restrict__ a_off,
+                                                       const uint32_t *__restrict__ a_col,
+                                                       const double *__restrict__ a_val,
+                                                       const uint64_t *__restrict__ b_off,
+                                                       const uint32_t *__restrict__ b_col,
+                                                       const double *__restrict__ b_val, const uint64_t *__restrict__ base,
+                                                       uint64_t *__restrict__ cnt, uint32_t *__restrict__ t_col,
+                                                       double *__restrict__ t_val)
+{
+#pragma clang fp contract(off)
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t g = lane & (G - 1);
+    const uint32_t groups = blockDim.x / G;
+    const uint64_t gmask = G == kWave ? ~0ull : (((1ull << G) - 1) << (lane & ~(uint32_t)(G - 1)));
+    for (uint64_t l0 = (uint64_t)blockIdx.x * groups; l0 < n_list; l0 += (uint64_t)gridDim.x * groups) {
+        const uint64_t li = l0 + threadIdx.x / G;
+        const bool active = li < n_list;
+        const uint32_t i = active ? list[li] : 0;
+        uint32_t na = 0;
+        uint64_t a0 = 0;
+        if (active) {
+            a0 = a_off[i];
+            na = (uint32_t)(a_off[i + 1] - a0);  // <= G (classified)
+        }
+        uint32_t len = 0;
+        uint64_t bs = 0;
+        double av = 0.0;
+        if (g < na) {
+            const uint32_t k = a_col[a0 + g];
+            bs = b_off[k];
+            len = (uint32_t)(b_off[k + 1] - bs);
+            av = a_val[a0 + g];
+        }
+        uint32_t incl = len;  // inclusive prefix of the B row lengths over the group
+#pragma unroll
+        for (int d = 1; d < G; d <<= 1) {
+            const uint32_t u = __shfl_up(incl, d, G);
+            if (g >= (uint32_t)d) incl += u;
+        }
+        const uint32_t pre = incl - len;
+        const uint32_t ub = __shfl(incl, G - 1, G);
+        // product g belongs to A entry e = the last e < na with pre[e] <= g
+        uint32_t na_w = na;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) na_w = max(na_w, (uint32_t)__shfl_xor(na_w, d, kWave));
+        uint32_t e = 0;
+        for (uint32_t t = 1; t < na_w; ++t) {
+            const uint32_t pt = __shfl(pre, t, G);
+            if (t < na && pt <= g) e = t;
+        }
+        const uint64_t bse = __shfl(bs, e, G);
+        const double ae = __shfl(av, e, G);
+        const uint32_t pe = __shfl(pre, e, G);
+        uint64_t key = ~0ull;
+        double prod = 0.0;
+        if (g < ub) {
+            const uint64_t p = bse + (g - pe);
+            prod = ae * b_val[p];
+            key = ((uint64_t)b_col[p] << 32) | g;
+        }
+        // bitonic sort of (key, prod) over the group
+#pragma unroll
+        for (int size = 2; size <= G; size <<= 1) {
+#pragma unroll
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                const uint64_t ok = __shfl_xor(key, stride, G);
+                const double op = __shfl_xor(prod, stride, G);
+                const bool up = (g & size) == 0, lower = (g & stride) == 0;
+                if (lower == up ? ok < key : ok > key) {
+                    key = ok;
+                    prod = op;
+                }
+            }
+        }
+        // column runs: the head adds its run from 0.0 in order
+        const uint32_t c = (uint32_t)(key >> 32);
+        const uint32_t cprev = __shfl_up(c, 1, G);
+        const bool head = g < ub && (g == 0 || cprev != c);
+        double sum = 0.0;
+        if (head) sum = sum + prod;
+        bool open = head;
+        for (int d = 1; d < G; ++d) {
+            if (!__any(open)) break;
+            const uint32_t cd = __shfl_down(c, d, G);
+            const double pd = __shfl_down(prod, d, G);
+            if (open) {
+                if (g + d < ub && cd == c) sum = sum + pd;
+                else open = false;
+            }
+        }
+        const bool keep = head && sum != 0.0;
+        const uint64_t bal = __ballot(keep) & gmask;
+        if (keep) {
+            const uint64_t pos = base[i] + __popcll(bal & ((1ull << lane) - 1));
+            t_col[pos] = c;
+            t_val[pos] = sum;
+        }
+        if (active && g == 0) cnt[i] = __popcll(bal);
     }
 }
 
@@ -264,23 +390,27 @@ __device__ __forceinline__ uint32_t block_incl_scan(uint32_t v, uint32_t *s_w, u
     }
 }
 
-template <int T, uint32_t CAP, bool WRITE>
+// Rows with at most CAP products and CAP entries of A: one workgroup per row,
+// products expanded into LDS, a bitonic sort on (column, position), one lane
+// per column run adding from 0.0 in position order.  Kept entries go to
+// t_col/t_val at base[i]; cnt[i] = their number.
+template <int T, uint32_t CAP>
 __global__ __launch_bounds__(T) void k_spgemm_rows(const uint32_t *__restrict__ list, uint32_t n_list,
                                                     const uint64_t *__restrict__ a_off,
                                                     const uint32_t *__restrict__ a_col,
                                                     const double *__restrict__ a_val,
                                                     const uint64_t *__restrict__ b_off,
                                                     const uint32_t *__restrict__ b_col,
-                                                    const double *__restrict__ b_val, uint64_t *__restrict__ cnt,
-                                                    const uint64_t *__restrict__ c_off, uint32_t *__restrict__ c_col,
-                                                    double *__restrict__ c_val)
+                                                    const double *__restrict__ b_val, const uint64_t *__restrict__ base,
+                                                    uint64_t *__restrict__ cnt, uint32_t *__restrict__ t_col,
+                                                    double *__restrict__ t_val)
 {
 #pragma clang fp contract(off)
-    __shared__ uint64_t s_key[CAP];    // (column << 32) | position in the reference's order
-    __shared__ double s_prod[CAP];     // products by position
+    __shared__ uint64_t s_key[CAP];      // (column << 32) | position in the reference's order
+    __shared__ double s_prod[CAP];       // products by position
     __shared__ uint32_t s_pre[CAP + 1];  // first position of each A entry's products
-    __shared__ uint64_t s_start[CAP];  // B row start of each A entry
-    __shared__ double s_a[CAP];        // A values
+    __shared__ uint64_t s_start[CAP];    // B row start of each A entry
+    __shared__ double s_a[CAP];          // A values
     __shared__ uint32_t s_w[T / kWave];
     const uint32_t tid = threadIdx.x;
     for (uint32_t li = blockIdx.x; li < n_list; li += gridDim.x) {
@@ -289,8 +419,8 @@ __global__ __launch_bounds__(T) void k_spgemm_rows(const uint32_t *__restrict__ 
         const uint32_t na = (uint32_t)(a_off[i + 1] - a0);  // <= CAP (classified)
         // 1. A's entries and the prefix of their B row lengths
         uint32_t carry = 0;
-        for (uint32_t base = 0; base < na; base += T) {
-            const uint32_t e = base + tid;
+        for (uint32_t b0 = 0; b0 < na; b0 += T) {
+            const uint32_t e = b0 + tid;
             uint32_t len = 0;
             if (e < na) {
                 const uint32_t k = a_col[a0 + e];
@@ -309,7 +439,7 @@ __global__ __launch_bounds__(T) void k_spgemm_rows(const uint32_t *__restrict__ 
         uint32_t p2 = 2;
         while (p2 < ub) p2 <<= 1;
         __syncthreads();
-        // 2. expand: position q -> A entry e (last e with s_pre[e] <= q), B entry t
+        // 2. expand: position q -> A entry e (last e with s_pre[e] <= q), B entry
         for (uint32_t q = tid; q < p2; q += T) {
             if (q < ub) {
                 uint32_t lo = 0, hi = na;
@@ -343,8 +473,9 @@ __global__ __launch_bounds__(T) void k_spgemm_rows(const uint32_t *__restrict__ 
         }
         // 4. one lane per column run: sum from 0.0 in position order
         uint32_t kept = 0;
-        for (uint32_t base = 0; base < ub; base += T) {
-            const uint32_t s = base + tid;
+        const uint64_t dst = base[i];
+        for (uint32_t b0 = 0; b0 < ub; b0 += T) {
+            const uint32_t s = b0 + tid;
             uint32_t keep = 0, c = 0;
             double sum = 0.0;
             if (s < ub) {
@@ -357,25 +488,26 @@ __global__ __launch_bounds__(T) void k_spgemm_rows(const uint32_t *__restrict__ 
             }
             uint32_t tot;
             const uint32_t incl = block_incl_scan<T>(keep, s_w, tot);
-            if (WRITE && keep) {
-                const uint64_t pos = c_off[i] + kept + incl - 1;
-                c_col[pos] = c;
-                c_val[pos] = sum;
+            if (keep) {
+                const uint64_t pos = dst + kept + incl - 1;
+                t_col[pos] = c;
+                t_val[pos] = sum;
             }
             kept += tot;
         }
-        if (!WRITE && tid == 0) cnt[i] = kept;
+        if (tid == 0) cnt[i] = kept;
         __syncthreads();  // LDS reused by the next row
     }
 }
 
 // ---- rows with many products: expand, sort, compress in HBM ----------------
 
-__global__ void k_spgemm_big_ub(const uint32_t *__restrict__ list, uint32_t n, const uint64_t *__restrict__ ub,
+// u[l] = product count of long row l, from the scanned counts.
+__global__ void k_spgemm_big_ub(const uint32_t *__restrict__ list, uint32_t n, const uint64_t *__restrict__ base,
                                 uint64_t *__restrict__ u)
 {
     const uint64_t l = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (l < n) u[l] = ub[list[l]];
+    if (l < n) u[l] = base[list[l] + 1] - base[list[l]];
 }
 
 __global__ __launch_bounds__(kThreads) void k_spgemm_big_expand(
@@ -428,19 +560,44 @@ __global__ void k_spgemm_big_counts(const uint32_t *__restrict__ list, uint32_t 
     if (l < n) cnt[list[l]] = sk[u[l + 1]] - sk[u[l]];
 }
 
+// Kept runs of the long rows into the row's scratch area at base[row].
 __global__ void k_spgemm_big_write(const uint64_t *__restrict__ skey, const double *__restrict__ sums, uint64_t P,
                                    unsigned cb, const uint32_t *__restrict__ list, const uint64_t *__restrict__ u,
-                                   const uint64_t *__restrict__ sk, const uint64_t *__restrict__ c_off,
-                                   uint32_t *__restrict__ c_col, double *__restrict__ c_val)
+                                   const uint64_t *__restrict__ sk, const uint64_t *__restrict__ base,
+                                   uint32_t *__restrict__ t_col, double *__restrict__ t_val)
 {
     for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < P;
          s += (uint64_t)gridDim.x * blockDim.x) {
         if (sk[s + 1] == sk[s]) continue;
         const uint64_t k = skey[s];
         const uint32_t l = (uint32_t)(k >> cb);
-        const uint64_t pos = c_off[list[l]] + (sk[s] - sk[u[l]]);
-        c_col[pos] = (uint32_t)(k & ((1ull << cb) - 1));
-        c_val[pos] = sums[s];
+        const uint64_t pos = base[list[l]] + (sk[s] - sk[u[l]]);
+        t_col[pos] = (uint32_t)(k & ((1ull << cb) - 1));
+        t_val[pos] = sums[s];
+    }
+}
+
+// C's entry p (row r) = scratch entry base[r] + (p - c_off[r]).
+__global__ __launch_bounds__(kThreads) void k_spgemm_gather(uint64_t n_rows, uint64_t nnz,
+                                                             const uint64_t *__restrict__ c_off,
+                                                             const uint64_t *__restrict__ base,
+                                                             const uint32_t *__restrict__ t_col,
+                                                             const double *__restrict__ t_val,
+                                                             uint32_t *__restrict__ c_col, double *__restrict__ c_val)
+{
+    __shared__ uint64_t s_off[kRowCache + 1];
+    const uint64_t first = (uint64_t)blockIdx.x * kThreads * kItems;
+    if (first >= nnz) return;
+    uint64_t rows[kItems];
+    entry_rows(c_off, n_rows, nnz, first, s_off, rows);
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+        const uint64_t p = first + (uint64_t)it * kThreads + threadIdx.x;
+        if (p >= nnz) break;
+        const uint64_t r = rows[it];
+        const uint64_t src = base[r] + (p - c_off[r]);
+        c_col[p] = t_col[src];
+        c_val[p] = t_val[src];
     }
 }
 
@@ -576,60 +733,74 @@ extern "C" int spira_spgemm_f64(spira_dcsr *A, spira_dcsr *B, spira_dcsr *C)
         if (n >= (1ull << 32) - 1) fail(SPIRA_ERR_UNSUPPORTED, "spgemm: needs fewer than 2^32 - 1 rows");
         hipStream_t st = dev->stream;
         scratch_set tmp{dev, {}};
-        uint64_t *cnt = (uint64_t *)tmp.get((n + 1) * sizeof(uint64_t));  // row counts, then C's offsets
-        uint64_t *ub = (uint64_t *)tmp.get((n ? n : 1) * sizeof(uint64_t));
-        uint32_t *lists = (uint32_t *)tmp.get(3 * (n ? n : 1) * sizeof(uint32_t));
-        uint32_t *list_n = (uint32_t *)dev->ws.get(WS_MISC, 4 * sizeof(uint32_t));
-        SPIRA_HIP(hipMemsetAsync(list_n, 0, 4 * sizeof(uint32_t), st));
-        uint32_t ln[4] = {0, 0, 0, 0};
-        if (n > 0) {
-            k_spgemm_classify<<<grid_for(n, kThreads), kThreads, 0, st>>>(n, A->off, A->col, B->off, ub, cnt, lists,
-                                                                          list_n);
-            SPIRA_HIP(hipGetLastError());
-            SPIRA_HIP(hipMemcpyAsync(ln, list_n, sizeof(ln), hipMemcpyDeviceToHost, st));
-            dev_sync(dev);
-        }
-        const uint32_t n0 = ln[0], n1 = ln[1], n2 = ln[2];
-        const uint32_t *l0 = lists, *l1 = lists + n, *l2 = lists + 2 * n;
-        const unsigned g0 = (unsigned)std::min<uint64_t>(n0, 65536), g1 = (unsigned)std::min<uint64_t>(n1, 32768);
-        if (n0)
-            k_spgemm_rows<64, kSmallCap, false><<<g0, 64, 0, st>>>(l0, n0, A->off, A->col, A->val, B->off, B->col,
-                                                                     B->val, cnt, nullptr, nullptr, nullptr);
-        if (n1)
-            k_spgemm_rows<256, kLargeCap, false><<<g1, 256, 0, st>>>(l1, n1, A->off, A->col, A->val, B->off, B->col,
-                                                                       B->val, cnt, nullptr, nullptr, nullptr);
+        uint64_t *cnt = (uint64_t *)tmp.get((n + 1) * sizeof(uint64_t));   // row counts, then C's offsets
+        uint64_t *base = (uint64_t *)tmp.get((n + 1) * sizeof(uint64_t));  // product counts, then scratch bases
+        uint32_t *lists = (uint32_t *)tmp.get((uint64_t)kClasses * (n ? n : 1) * sizeof(uint32_t));
+        uint32_t *list_n = (uint32_t *)dev->ws.get(WS_MISC, 8 * sizeof(uint32_t));
+        SPIRA_HIP(hipMemsetAsync(list_n, 0, 8 * sizeof(uint32_t), st));
+        k_spgemm_classify<<<grid_for(n + 1, kThreads), kThreads, 0, st>>>(n, A->off, A->col, B->off, base, cnt,
+                                                                          lists, list_n);
         SPIRA_HIP(hipGetLastError());
-        // rows with many products
-        uint64_t P = 0;
-        unsigned cb = 0;
-        uint64_t *u = nullptr, *skey = nullptr, *keep = nullptr;
-        double *sums = nullptr;
-        if (n2) {
-            u = (uint64_t *)tmp.get(((uint64_t)n2 + 1) * sizeof(uint64_t));
-            k_spgemm_big_ub<<<grid_for(n2, kThreads), kThreads, 0, st>>>(l2, n2, ub, u);
+        scan_excl_u64(dev, base, base, n);  // base[i] = products before row i: row i's scratch area
+        uint32_t ln[8] = {};
+        SPIRA_HIP(hipMemcpyAsync(ln, list_n, sizeof(ln), hipMemcpyDeviceToHost, st));
+        const uint64_t total = read_scalar(dev, base + n);  // synchronises ln too
+        // scratch for the rows' kept entries (at most one per product)
+        if (total > (uint64_t(48) << 30) / 12)
+            fail(SPIRA_ERR_UNSUPPORTED, "spgemm: more than 4G products (scratch of 12 bytes per product)");
+        uint32_t *t_col = (uint32_t *)tmp.get(total * sizeof(uint32_t));
+        double *t_val = (double *)tmp.get(total * sizeof(double));
+        const uint32_t *l[kClasses];
+        for (int c = 0; c < kClasses; ++c) l[c] = lists + (uint64_t)c * n;
+        auto blocks = [](uint32_t rows, uint32_t per_block, uint32_t cap) {
+            return (unsigned)std::min<uint64_t>((rows + per_block - 1) / per_block, cap);
+        };
+        if (ln[0])
+            k_spgemm_lanes<16><<<blocks(ln[0], 16, 65536), 256, 0, st>>>(l[0], ln[0], A->off, A->col, A->val, B->off,
+                                                                        B->col, B->val, base, cnt, t_col, t_val);
+        if (ln[1])
+            k_spgemm_lanes<32><<<blocks(ln[1], 8, 65536), 256, 0, st>>>(l[1], ln[1], A->off, A->col, A->val, B->off,
+                                                                       B->col, B->val, base, cnt, t_col, t_val);
+        if (ln[2])
+            k_spgemm_lanes<64><<<blocks(ln[2], 4, 65536), 256, 0, st>>>(l[2], ln[2], A->off, A->col, A->val, B->off,
+                                                                       B->col, B->val, base, cnt, t_col, t_val);
+        if (ln[3])
+            k_spgemm_rows<64, kClassCap[3]><<<blocks(ln[3], 1, 65536), 64, 0, st>>>(
+                l[3], ln[3], A->off, A->col, A->val, B->off, B->col, B->val, base, cnt, t_col, t_val);
+        if (ln[4])
+            k_spgemm_rows<256, kClassCap[4]><<<blocks(ln[4], 1, 32768), 256, 0, st>>>(
+                l[4], ln[4], A->off, A->col, A->val, B->off, B->col, B->val, base, cnt, t_col, t_val);
+        SPIRA_HIP(hipGetLastError());
+        const uint32_t n5 = ln[5];
+        if (n5) {  // rows with many products
+            const uint32_t *l5 = l[5];
+            uint64_t *u = (uint64_t *)tmp.get(((uint64_t)n5 + 1) * sizeof(uint64_t));
+            k_spgemm_big_ub<<<grid_for(n5, kThreads), kThreads, 0, st>>>(l5, n5, base, u);
             SPIRA_HIP(hipGetLastError());
-            scan_excl_u64(dev, u, u, n2);
-            P = read_scalar(dev, u + n2);
+            scan_excl_u64(dev, u, u, n5);
+            const uint64_t P = read_scalar(dev, u + n5);
             if (P >= (1ull << 32)) fail(SPIRA_ERR_UNSUPPORTED, "spgemm: more than 2^32 products in long rows");
-            cb = bits_for(m - 1);
-            const unsigned rb = bits_for(n2 - 1);
+            const unsigned cb = bits_for(m - 1), rb = bits_for(n5 - 1);
             uint64_t *key = (uint64_t *)tmp.get(P * sizeof(uint64_t));
             double *prod = (double *)tmp.get(P * sizeof(double));
-            skey = (uint64_t *)tmp.get(P * sizeof(uint64_t));
+            uint64_t *skey = (uint64_t *)tmp.get(P * sizeof(uint64_t));
             double *sprod = (double *)tmp.get(P * sizeof(double));
-            keep = (uint64_t *)tmp.get((P + 1) * sizeof(uint64_t));
-            k_spgemm_big_expand<<<std::min<uint32_t>(n2, 8192), kThreads, 0, st>>>(
-                l2, n2, u, cb, A->off, A->col, A->val, B->off, B->col, B->val, key, prod);
+            uint64_t *keep = (uint64_t *)tmp.get((P + 1) * sizeof(uint64_t));
+            k_spgemm_big_expand<<<std::min<uint32_t>(n5, 8192), kThreads, 0, st>>>(
+                l5, n5, u, cb, A->off, A->col, A->val, B->off, B->col, B->val, key, prod);
             SPIRA_HIP(hipGetLastError());
             size_t tb = 0;
             SPIRA_HIP(rocprim::radix_sort_pairs(nullptr, tb, key, skey, prod, sprod, (unsigned)P, 0, cb + rb, st));
             void *t = dev->ws.get(WS_SORT, tb ? tb : 8);
             SPIRA_HIP(rocprim::radix_sort_pairs(t, tb, key, skey, prod, sprod, (unsigned)P, 0, cb + rb, st));
-            sums = prod;  // the unsorted products are no longer needed
+            double *sums = prod;  // the unsorted products are no longer needed
             k_spgemm_big_runs<<<grid_for(P, kThreads, 16384), kThreads, 0, st>>>(skey, sprod, P, sums, keep);
             SPIRA_HIP(hipGetLastError());
             scan_excl_u64(dev, keep, keep, P);
-            k_spgemm_big_counts<<<grid_for(n2, kThreads), kThreads, 0, st>>>(l2, n2, u, keep, cnt);
+            k_spgemm_big_counts<<<grid_for(n5, kThreads), kThreads, 0, st>>>(l5, n5, u, keep, cnt);
+            SPIRA_HIP(hipGetLastError());
+            k_spgemm_big_write<<<grid_for(P, kThreads, 16384), kThreads, 0, st>>>(skey, sums, P, cb, l5, u, keep,
+                                                                                base, t_col, t_val);
             SPIRA_HIP(hipGetLastError());
         }
         scan_excl_u64(dev, cnt, cnt, n);
@@ -637,23 +808,14 @@ extern "C" int spira_spgemm_f64(spira_dcsr *A, spira_dcsr *B, spira_dcsr *C)
         uint32_t *c_col = (uint32_t *)tmp.get(nnz * sizeof(uint32_t));
         double *c_val = (double *)tmp.get(nnz * sizeof(double));
         if (nnz > 0) {
-            if (n0)
-                k_spgemm_rows<64, kSmallCap, true><<<g0, 64, 0, st>>>(l0, n0, A->off, A->col, A->val, B->off,
-                                                                        B->col, B->val, nullptr, cnt, c_col, c_val);
-            if (n1)
-                k_spgemm_rows<256, kLargeCap, true><<<g1, 256, 0, st>>>(l1, n1, A->off, A->col, A->val, B->off,
-                                                                          B->col, B->val, nullptr, cnt, c_col, c_val);
+            const uint64_t nb = (nnz + (uint64_t)kThreads * kItems - 1) / ((uint64_t)kThreads * kItems);
+            if (nb >= (1ull << 31)) fail(SPIRA_ERR_UNSUPPORTED, "spgemm: too many output entries");
+            k_spgemm_gather<<<(unsigned)nb, kThreads, 0, st>>>(n, nnz, cnt, base, t_col, t_val, c_col, c_val);
             SPIRA_HIP(hipGetLastError());
-            if (n2) {
-                k_spgemm_big_write<<<grid_for(P, kThreads, 16384), kThreads, 0, st>>>(skey, sums, P, cb, l2, u, keep,
-                                                                                     cnt, c_col, c_val);
-                SPIRA_HIP(hipGetLastError());
-            }
         }
         tmp.keep(cnt);
         tmp.keep(c_col);
         tmp.keep(c_val);
         install(C, cnt, c_col, c_val, nnz);
-        dev_sync(dev);  // temporaries go back to the cache after the kernels that use them
     });
 }
