@@ -55,12 +55,22 @@ constexpr uint32_t kTileLong = 0x80u;
 constexpr uint32_t kItemStaged = 1u;   // item flag: entries staged in LDS
 constexpr uint32_t kItemTwoPass = 2u;  // item flag: k = 1, X block staged in two column halves
 constexpr size_t kArrayPad = 64;      // bytes after each entry array (16-byte staging reads)
-// 16-byte loads in flight per lane when a block stages its item: deep at
-// k = 1, where staging is most of the item's time; at k >= 4 the deeper loop
-// changes how the compiler schedules the compute phase and costs more than
-// it saves (measured).
+// How a block stages its item, as 16-byte loads in flight per lane: 0 =
+// straight into LDS (global_load_lds_dwordx4: no VGPRs, every chunk of the
+// item in flight at once) at k = 1, where staging is most of the item's time;
+// one register-staged load at k >= 4, where a deeper staging loop changes how
+// the compiler schedules the compute phase and costs more than it saves
+// (measured).
 template <int K>
-constexpr uint32_t stage_depth() { return K == 1 ? 4u : 1u; }
+constexpr uint32_t stage_depth() { return K == 1 ? 0u : 1u; }
+
+// Before the barrier that publishes staged data: direct-to-LDS loads complete
+// only when the issuing wavefront has waited for them.
+template <uint32_t D>
+__device__ __forceinline__ void stage_wait()
+{
+    if constexpr (D == 0) __builtin_amdgcn_s_waitcnt(0);
+}
 
 inline __host__ __device__ uint32_t align16(uint32_t v) { return (v + 15u) & ~15u; }
 
@@ -242,13 +252,24 @@ __device__ __forceinline__ void merge_lanes(acc2 &a, uint32_t E)
 }
 
 // Copy nbytes (a multiple of 16) from 16-byte aligned global src to LDS dst
-// with D 16-byte loads in flight per lane.
+// with D 16-byte loads in flight per lane, or (D = 0) straight into LDS: one
+// wavefront instruction moves 64 consecutive chunks, lane l's to dst + 16 l
+// (lanes past the end masked off); the caller waits with stage_wait<0>().
 template <uint32_t D>
 __device__ __forceinline__ void stage16(unsigned char *dst, const unsigned char *src, uint32_t nbytes)
 {
     const u32x4 *s = (const u32x4 *)src;
     u32x4 *d = (u32x4 *)dst;
     const uint32_t n = nbytes / 16;
+    if constexpr (D == 0) {
+        const uint32_t lane = threadIdx.x & (kWave - 1);
+        for (uint32_t c0 = threadIdx.x & ~(uint32_t)(kWave - 1); c0 < n; c0 += kThreads)
+            if (c0 + lane < n)
+                __builtin_amdgcn_global_load_lds((const void *)(s + c0 + lane),
+                                                 (__attribute__((address_space(3))) void *)(d + c0), 16, 0,
+                                                 2 /* non-temporal */);
+        return;
+    }
     uint32_t i = threadIdx.x;
     if constexpr (D > 1) {
         for (; i + (D - 1) * kThreads < n; i += D * kThreads) {
@@ -462,6 +483,7 @@ __device__ __forceinline__ void run_item_two_pass(const kparams &kp, const item_
         const uint32_t c0 = h * W, c1 = min(n, c0 + W);
         __syncthreads();  // the previous pass is done with the half and the records
         const float *xs = stage_array<stage_depth<1>()>(lds, 0, kp.X, (uint64_t)it.cbase + c0, c1 - c0);
+        stage_wait<stage_depth<1>()>();
         __syncthreads();
         for (uint32_t tt = it.t0; tt < it.t1; ++tt) {
             const tile_t tl = kp.tiles[tt];
@@ -569,7 +591,10 @@ __global__ __launch_bounds__(kThreads) void k_batch_spmm(kparams kp)
                     stage_array<stage_depth<K>()>(lds, L.v2_o, kp.lo, it.ebase, nnz);
             }
         }
-        if (xs || staged) __syncthreads();
+        if (xs || staged) {
+            stage_wait<stage_depth<K>()>();
+            __syncthreads();
+        }
         if (xs) {
             if (staged) run_item<K, ST, CT, true, true>(kp, it, lds, s_part, lane, w, j);
             else run_item<K, ST, CT, true, false>(kp, it, lds, s_part, lane, w, j);
