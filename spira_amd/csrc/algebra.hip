@@ -21,9 +21,9 @@
 // a_ik * b_kj into acc[j] = 0.0 in the order of k (A's row) and, inside B's
 // row, of j; each product is rounded, then added (no fused multiply-add);
 // entries with acc[j] != 0 are kept.  The device keeps that order exactly:
-//   * k_spgemm_classify: per row of A its product count (the B row lengths
-//     its columns select) and a size class; a scan of the counts gives each
-//     row a scratch area with room for one entry per product;
+//   * k_spgemm_entry_len + a scan: products before each entry of A, so each
+//     row has a product count and a scratch area with room for one entry per
+//     product; k_spgemm_classify gives the row a size class;
 //   * k_spgemm_lanes<G> (rows of <= 16, 32, 64 products): 64 / G rows per
 //     wavefront, all in registers -- lane g owns product g in the reference's
 //     order, a shuffle bitonic network sorts (column, g), and the first lane
@@ -211,43 +211,83 @@ __global__ __launch_bounds__(kThreads) void k_add_scatter(uint64_t n_rows, const
 constexpr int kClasses = 6;
 constexpr uint32_t kClassCap[kClasses - 1] = {16, 32, 64, 256, 1024};
 
-// Per row of A: its product count ub[i] and its size class; rows with
-// products are appended to lists[cls * n_rows + ...] (order inside a list is
-// free: rows are independent).  cnt[i] = 0 for rows without products.
-__global__ void k_spgemm_classify(uint64_t n_rows, const uint64_t *__restrict__ a_off,
-                                  const uint32_t *__restrict__ a_col, const uint64_t *__restrict__ b_off,
-                                  uint64_t *__restrict__ ub, uint64_t *__restrict__ cnt, uint32_t *__restrict__ lists,
-                                  uint32_t *__restrict__ list_n)
+// len[k] = length of the B row that entry k of A selects (its products).
+__global__ void k_spgemm_entry_len(uint64_t nnz_a, const uint32_t *__restrict__ a_col,
+                                   const uint64_t *__restrict__ b_off, uint64_t *__restrict__ len)
 {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int cls = -1;
-    if (i < n_rows) {
-        const uint64_t a0 = a_off[i], na = a_off[i + 1] - a0;
-        uint64_t u = 0;
-        for (uint64_t e = 0; e < na; ++e) {
-            const uint32_t k = a_col[a0 + e];
-            u += b_off[k + 1] - b_off[k];
-        }
-        ub[i] = u;
-        cnt[i] = 0;
-        if (u > 0) {
-            cls = kClasses - 1;
-            for (int c = kClasses - 2; c >= 0; --c)
-                if (na <= kClassCap[c] && u <= kClassCap[c]) cls = c;
-        }
-    } else if (i == n_rows) {
-        ub[i] = 0;  // the scan of ub reads n_rows + 1 words
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz_a;
+         k += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t c = a_col[k];
+        len[k] = b_off[c + 1] - b_off[c];
     }
-    // one atomic per wavefront and class
+}
+
+// Per row of A, from the prefix s of the per-entry product counts: base[i] =
+// products before row i (the start of its scratch area; base[n_rows] = all),
+// its size class; rows with products are appended to lists[cls * n_rows + ...]
+// (order inside a list is free: rows are independent).  cnt[i] = 0 for rows
+// without products.  A block takes kClassifyRows rows, counts its classes in
+// LDS, reserves its ranges with one global atomic per class, then writes.
+constexpr int kClassifyItems = 16;
+constexpr uint64_t kClassifyRows = (uint64_t)kThreads * kClassifyItems;
+
+__global__ __launch_bounds__(kThreads) void k_spgemm_classify(uint64_t n_rows, const uint64_t *__restrict__ a_off,
+                                                               const uint64_t *__restrict__ s,
+                                                               uint64_t *__restrict__ base,
+                                                               uint64_t *__restrict__ cnt,
+                                                               uint32_t *__restrict__ lists,
+                                                               uint32_t *__restrict__ list_n)
+{
+    __shared__ uint32_t s_n[kClasses], s_b[kClasses];
+    if (threadIdx.x < kClasses) s_n[threadIdx.x] = 0;
+    __syncthreads();
     const uint32_t lane = threadIdx.x & (kWave - 1);
-    for (int c = 0; c < kClasses; ++c) {
-        const uint64_t m = __ballot(cls == c);
-        if (!m) continue;
-        const uint32_t leader = __ffsll((unsigned long long)m) - 1;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&list_n[c], (uint32_t)__popcll(m));
-        base = __shfl(base, leader, kWave);
-        if (cls == c) lists[(uint64_t)c * n_rows + base + __popcll(m & ((1ull << lane) - 1))] = (uint32_t)i;
+    const uint64_t r0 = (uint64_t)blockIdx.x * kClassifyRows;
+    int cls[kClassifyItems];
+#pragma unroll
+    for (int it = 0; it < kClassifyItems; ++it) {
+        const uint64_t i = r0 + (uint64_t)it * kThreads + threadIdx.x;
+        int c = -1;
+        if (i < n_rows) {
+            const uint64_t a0 = a_off[i], a1 = a_off[i + 1], na = a1 - a0;
+            const uint64_t s0 = s[a0], u = s[a1] - s0;
+            base[i] = s0;
+            cnt[i] = 0;
+            if (u > 0) {
+                c = kClasses - 1;
+                for (int q = kClasses - 2; q >= 0; --q)
+                    if (na <= kClassCap[q] && u <= kClassCap[q]) c = q;
+            }
+        } else if (i == n_rows) {
+            base[i] = s[a_off[i]];
+        }
+        cls[it] = c;
+        for (int q = 0; q < kClasses; ++q) {  // one LDS atomic per wavefront and class
+            const uint64_t m = __ballot(c == q);
+            if (m && lane == (uint32_t)(__ffsll((unsigned long long)m) - 1)) atomicAdd(&s_n[q], (uint32_t)__popcll(m));
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < kClasses) {
+        const uint32_t q = threadIdx.x;
+        s_b[q] = s_n[q] ? atomicAdd(&list_n[q], s_n[q]) : 0u;
+        s_n[q] = 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < kClassifyItems; ++it) {
+        const uint64_t i = r0 + (uint64_t)it * kThreads + threadIdx.x;
+        const int c = cls[it];
+        for (int q = 0; q < kClasses; ++q) {
+            const uint64_t m = __ballot(c == q);
+            if (!m) continue;
+            const uint32_t leader = __ffsll((unsigned long long)m) - 1;
+            uint32_t b0 = 0;
+            if (lane == leader) b0 = atomicAdd(&s_n[q], (uint32_t)__popcll(m));
+            b0 = __shfl(b0, leader, kWave);
+            if (c == q)
+                lists[(uint64_t)q * n_rows + s_b[q] + b0 + __popcll(m & ((1ull << lane) - 1))] = (uint32_t)i;
+        }
     }
 }
 
@@ -440,7 +480,7 @@ __global__ __launch_bounds__(T) void k_spgemm_rows(const uint32_t *__restrict__ 
         while (p2 < ub) p2 <<= 1;
         __syncthreads();
         // 2. expand: position q -> A entry e (last e with s_pre[e] <= q), B entry
-        for (uint32_t q = tid; q < p2; q += T) {
+        auto expand = [&](uint32_t q, uint64_t &key) {
             if (q < ub) {
                 uint32_t lo = 0, hi = na;
                 while (hi - lo > 1) {
@@ -450,10 +490,15 @@ __global__ __launch_bounds__(T) void k_spgemm_rows(const uint32_t *__restrict__ 
                 }
                 const uint64_t p = s_start[lo] + (q - s_pre[lo]);
                 s_prod[q] = s_a[lo] * b_val[p];
-                s_key[q] = ((uint64_t)b_col[p] << 32) | q;
+                key = ((uint64_t)b_col[p] << 32) | q;
             } else {
-                s_key[q] = ~0ull;
+                key = ~0ull;
             }
+        };
+        for (uint32_t q = tid; q < p2; q += T) {
+            uint64_t key;
+            expand(q, key);
+            s_key[q] = key;
         }
         __syncthreads();
         // 3. bitonic sort of s_key[0, p2)
@@ -734,14 +779,20 @@ extern "C" int spira_spgemm_f64(spira_dcsr *A, spira_dcsr *B, spira_dcsr *C)
         hipStream_t st = dev->stream;
         scratch_set tmp{dev, {}};
         uint64_t *cnt = (uint64_t *)tmp.get((n + 1) * sizeof(uint64_t));   // row counts, then C's offsets
-        uint64_t *base = (uint64_t *)tmp.get((n + 1) * sizeof(uint64_t));  // product counts, then scratch bases
+        uint64_t *base = (uint64_t *)tmp.get((n + 1) * sizeof(uint64_t));  // scratch area of each row
         uint32_t *lists = (uint32_t *)tmp.get((uint64_t)kClasses * (n ? n : 1) * sizeof(uint32_t));
         uint32_t *list_n = (uint32_t *)dev->ws.get(WS_MISC, 8 * sizeof(uint32_t));
         SPIRA_HIP(hipMemsetAsync(list_n, 0, 8 * sizeof(uint32_t), st));
-        k_spgemm_classify<<<grid_for(n + 1, kThreads), kThreads, 0, st>>>(n, A->off, A->col, B->off, base, cnt,
-                                                                          lists, list_n);
+        const uint64_t nnz_a = A->nnz;
+        uint64_t *len = (uint64_t *)tmp.get((nnz_a + 1) * sizeof(uint64_t));
+        if (nnz_a) {
+            k_spgemm_entry_len<<<grid_for(nnz_a, kThreads, 16384), kThreads, 0, st>>>(nnz_a, A->col, B->off, len);
+            SPIRA_HIP(hipGetLastError());
+        }
+        scan_excl_u64(dev, len, len, nnz_a);  // products before each entry of A
+        k_spgemm_classify<<<(unsigned)((n + kClassifyRows) / kClassifyRows), kThreads, 0, st>>>(n, A->off, len, base,
+                                                                                             cnt, lists, list_n);
         SPIRA_HIP(hipGetLastError());
-        scan_excl_u64(dev, base, base, n);  // base[i] = products before row i: row i's scratch area
         uint32_t ln[8] = {};
         SPIRA_HIP(hipMemcpyAsync(ln, list_n, sizeof(ln), hipMemcpyDeviceToHost, st));
         const uint64_t total = read_scalar(dev, base + n);  // synchronises ln too
