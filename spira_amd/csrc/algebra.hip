@@ -291,6 +291,42 @@ __global__ __launch_bounds__(kThreads) void k_spgemm_classify(uint64_t n_rows, c
     }
 }
 
+// v of lane (lane ^ m): masks 1, 2, 4, 8 stay inside a 16-lane row and go
+// through DPP (quad_perm, row_shl / row_shr: VALU moves), larger ones through
+// the LDS crossbar (ds_bpermute).  Every lane of the wavefront must be active.
+__device__ __forceinline__ uint32_t xor_lane(uint32_t v, uint32_t m)
+{
+    const int x = (int)v;
+    switch (m) {
+    case 1: return (uint32_t)__builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+    case 2: return (uint32_t)__builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    case 4: {
+        const int up = __builtin_amdgcn_update_dpp(0, x, 0x104, 0xF, 0xF, false);  // row_shl:4, lane i <- i + 4
+        const int dn = __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, false);  // row_shr:4, lane i <- i - 4
+        return (uint32_t)((threadIdx.x & 4) ? dn : up);
+    }
+    case 8: {
+        const int up = __builtin_amdgcn_update_dpp(0, x, 0x108, 0xF, 0xF, false);
+        const int dn = __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, false);
+        return (uint32_t)((threadIdx.x & 8) ? dn : up);
+    }
+    default: return (uint32_t)__shfl_xor(x, (int)m, kWave);
+    }
+}
+
+__device__ __forceinline__ uint64_t xor_lane64(uint64_t v, uint32_t m)
+{
+    return ((uint64_t)xor_lane((uint32_t)(v >> 32), m) << 32) | xor_lane((uint32_t)v, m);
+}
+
+// v of lane + 1 (wave_shl1: a DPP move across the wavefront; lane 63 reads 0).
+__device__ __forceinline__ uint64_t next_lane64(uint64_t v)
+{
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(v >> 32), 0x130, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, 0x130, 0xF, 0xF, false);
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
 // Rows with at most G products and G entries of A, 64 / G rows per
 // wavefront, every step in registers: lane g of a row's G-lane group owns
 // product g (in the reference's order), the group sorts (column, g) with a
@@ -364,8 +400,8 @@ __global__ __launch_bounds__(256) void k_spgemm_lanes(const uint32_t *__restrict
         for (int size = 2; size <= G; size <<= 1) {
 #pragma unroll
             for (int stride = size >> 1; stride > 0; stride >>= 1) {
-                const uint64_t ok = __shfl_xor(key, stride, G);
-                const double op = __shfl_xor(prod, stride, G);
+                const uint64_t ok = xor_lane64(key, (uint32_t)stride);
+                const double op = __builtin_bit_cast(double, xor_lane64(__builtin_bit_cast(uint64_t, prod), (uint32_t)stride));
                 const bool up = (g & size) == 0, lower = (g & stride) == 0;
                 if (lower == up ? ok < key : ok > key) {
                     key = ok;
@@ -380,12 +416,13 @@ __global__ __launch_bounds__(256) void k_spgemm_lanes(const uint32_t *__restrict
         double sum = 0.0;
         if (head) sum = sum + prod;
         bool open = head;
+        uint64_t kd = key, pdb = __builtin_bit_cast(uint64_t, prod);  // key and product of lane + d
         for (int d = 1; d < G; ++d) {
             if (!__any(open)) break;
-            const uint32_t cd = __shfl_down(c, d, G);
-            const double pd = __shfl_down(prod, d, G);
+            kd = next_lane64(kd);
+            pdb = next_lane64(pdb);
             if (open) {
-                if (g + d < ub && cd == c) sum = sum + pd;
+                if (g + d < ub && (uint32_t)(kd >> 32) == c) sum = sum + __builtin_bit_cast(double, pdb);
                 else open = false;
             }
         }
