@@ -209,7 +209,7 @@ int spira_add_f64(spira_dcsr *A, spira_dcsr *B, spira_dcsr *C, uint32_t flags);
  * A.n_cols != B.n_rows (SPIRA_ERR_INVALID_ARGUMENT, "A.cols must equal
  * B.rows"), then A and B built (SPIRA_ERR_LOGIC); fewer than 2^32 - 1 rows,
  * at most 2^32 products in all (a scratch of 12 bytes per product) and fewer
- * than 2^32 in the rows with more than 1024 products or A entries
+ * than 2^32 in the rows with more than 1024 products or 256 A entries
  * (SPIRA_ERR_UNSUPPORTED). */
 int spira_spgemm_f64(spira_dcsr *A, spira_dcsr *B, spira_dcsr *C);
 

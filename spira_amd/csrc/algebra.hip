@@ -28,8 +28,10 @@
 //     wavefront, all in registers -- lane g owns product g in the reference's
 //     order, a shuffle bitonic network sorts (column, g), and the first lane
 //     of each column run adds the run from 0.0 in order;
-//   * k_spgemm_rows<T, CAP> (<= 256, 1024 products): one workgroup per row,
-//     the same steps through LDS;
+//   * k_spgemm_rows<T, CAP, NA> (<= 256, 1024 products): one workgroup per row
+//     through LDS; the B rows are already sorted runs, so log2(A entries)
+//     merge levels (a binary search per key into the partner run) replace
+//     the sort;
 //   * longer rows, expand-sort-compress in HBM: products keyed by (row,
 //     column) in the reference's order, a stable radix sort, one thread per
 //     (row, column) run;
@@ -209,7 +211,8 @@ __global__ __launch_bounds__(kThreads) void k_add_scatter(uint64_t n_rows, const
 //   3, 4: ub, na <= 256 / 1024: k_spgemm_rows (LDS);
 //   5: the rest, expand-sort-compress in HBM.
 constexpr int kClasses = 6;
-constexpr uint32_t kClassCap[kClasses - 1] = {16, 32, 64, 256, 1024};
+constexpr uint32_t kClassCap[kClasses - 1] = {16, 32, 64, 256, 1024};  // products
+constexpr uint32_t kClassNa[kClasses - 1] = {16, 32, 64, 64, 256};     // entries of A
 
 // len[k] = length of the B row that entry k of A selects (its products).
 __global__ void k_spgemm_entry_len(uint64_t nnz_a, const uint32_t *__restrict__ a_col,
@@ -256,7 +259,7 @@ __global__ __launch_bounds__(kThreads) void k_spgemm_classify(uint64_t n_rows, c
             if (u > 0) {
                 c = kClasses - 1;
                 for (int q = kClasses - 2; q >= 0; --q)
-                    if (na <= kClassCap[q] && u <= kClassCap[q]) c = q;
+                    if (na <= kClassNa[q] && u <= kClassCap[q]) c = q;
             }
         } else if (i == n_rows) {
             base[i] = s[a_off[i]];
@@ -467,11 +470,11 @@ __device__ __forceinline__ uint32_t block_incl_scan(uint32_t v, uint32_t *s_w, u
     }
 }
 
-// Rows with at most CAP products and CAP entries of A: one workgroup per row,
-// products expanded into LDS, a bitonic sort on (column, position), one lane
-// per column run adding from 0.0 in position order.  Kept entries go to
-// t_col/t_val at base[i]; cnt[i] = their number.
-template <int T, uint32_t CAP>
+// Rows with at most CAP products and NA entries of A: one workgroup per row,
+// products expanded into LDS, merged by (column, position) from the sorted
+// runs the B rows give, one lane per column run adding from 0.0 in position
+// order.  Kept entries go to t_col/t_val at base[i]; cnt[i] = their number.
+template <int T, uint32_t CAP, uint32_t NA>
 __global__ __launch_bounds__(T) void k_spgemm_rows(const uint32_t *__restrict__ list, uint32_t n_list,
                                                     const uint64_t *__restrict__ a_off,
                                                     const uint32_t *__restrict__ a_col,
@@ -483,17 +486,23 @@ __global__ __launch_bounds__(T) void k_spgemm_rows(const uint32_t *__restrict__ 
                                                     double *__restrict__ t_val)
 {
 #pragma clang fp contract(off)
-    __shared__ uint64_t s_key[CAP];      // (column << 32) | position in the reference's order
+    // key = column << 32 | A entry << 16 | position in the reference's order
+    // LDS is what limits how many rows a CU holds: the A-entry arrays are
+    // needed only until the products are expanded, so the merge buffer reuses them
+    constexpr uint32_t kAux = CAP > 2 * NA ? CAP : 2 * NA;
+    __shared__ uint64_t s_key[CAP];
+    __shared__ uint64_t s_aux[kAux];     // B row start and A value of each A entry; then the merge buffer
     __shared__ double s_prod[CAP];       // products by position
-    __shared__ uint32_t s_pre[CAP + 1];  // first position of each A entry's products
-    __shared__ uint64_t s_start[CAP];    // B row start of each A entry
-    __shared__ double s_a[CAP];          // A values
+    __shared__ uint32_t s_pre[NA + 1];   // first position of each A entry's products
     __shared__ uint32_t s_w[T / kWave];
+    uint64_t *const s_start = s_aux;
+    double *const s_a = (double *)(s_aux + NA);
+    uint64_t *const s_key2 = s_aux;
     const uint32_t tid = threadIdx.x;
     for (uint32_t li = blockIdx.x; li < n_list; li += gridDim.x) {
         const uint32_t i = list[li];
         const uint64_t a0 = a_off[i];
-        const uint32_t na = (uint32_t)(a_off[i + 1] - a0);  // <= CAP (classified)
+        const uint32_t na = (uint32_t)(a_off[i + 1] - a0);  // <= NA (classified)
         // 1. A's entries and the prefix of their B row lengths
         uint32_t carry = 0;
         for (uint32_t b0 = 0; b0 < na; b0 += T) {
@@ -513,45 +522,73 @@ __global__ __launch_bounds__(T) void k_spgemm_rows(const uint32_t *__restrict__ 
         }
         const uint32_t ub = carry;  // 1 .. CAP
         if (tid == 0) s_pre[na] = ub;
-        uint32_t p2 = 2;
-        while (p2 < ub) p2 <<= 1;
         __syncthreads();
         // 2. expand: position q -> A entry e (last e with s_pre[e] <= q), B entry
-        auto expand = [&](uint32_t q, uint64_t &key) {
-            if (q < ub) {
+        {
+            // every position's A entry first, then all B loads in flight at once
+            constexpr int EU = CAP / T;
+            uint64_t pp[EU];
+            uint32_t ee[EU];
+#pragma unroll
+            for (int j = 0; j < EU; ++j) {
+                const uint32_t q = tid + j * T;
                 uint32_t lo = 0, hi = na;
-                while (hi - lo > 1) {
-                    const uint32_t mid = (lo + hi) / 2;
-                    if (s_pre[mid] <= q) lo = mid;
-                    else hi = mid;
+                if (q < ub) {
+                    while (hi - lo > 1) {
+                        const uint32_t mid = (lo + hi) / 2;
+                        if (s_pre[mid] <= q) lo = mid;
+                        else hi = mid;
+                    }
+                    pp[j] = s_start[lo] + (q - s_pre[lo]);
+                } else {
+                    pp[j] = 0;
                 }
-                const uint64_t p = s_start[lo] + (q - s_pre[lo]);
-                s_prod[q] = s_a[lo] * b_val[p];
-                key = ((uint64_t)b_col[p] << 32) | q;
-            } else {
-                key = ~0ull;
+                ee[j] = lo;
             }
-        };
-        for (uint32_t q = tid; q < p2; q += T) {
-            uint64_t key;
-            expand(q, key);
-            s_key[q] = key;
+            uint32_t bc[EU];
+            double bv[EU];
+#pragma unroll
+            for (int j = 0; j < EU; ++j) {
+                if (tid + j * T < ub) {
+                    bc[j] = b_col[pp[j]];
+                    bv[j] = b_val[pp[j]];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < EU; ++j) {
+                const uint32_t q = tid + j * T;
+                if (q < ub) {
+                    s_prod[q] = s_a[ee[j]] * bv[j];
+                    s_key[q] = ((uint64_t)bc[j] << 32) | (ee[j] << 16) | q;
+                }
+            }
         }
         __syncthreads();
-        // 3. bitonic sort of s_key[0, p2)
-        for (uint32_t size = 2; size <= p2; size <<= 1) {
-            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-                for (uint32_t t = tid; t < p2 / 2; t += T) {
-                    const uint32_t lo = ((t & ~(stride - 1)) << 1) | (t & (stride - 1));
-                    const uint32_t hi = lo + stride;
-                    const uint64_t x = s_key[lo], y = s_key[hi];
-                    if ((x > y) == ((lo & size) == 0)) {
-                        s_key[lo] = y;
-                        s_key[hi] = x;
-                    }
+        // 3. merge sort: each A entry's products arrive sorted by column (a B
+        // row), and in position order; runs of w lists merge pairwise, each key
+        // going to (its index in its run) + (keys of the partner run below it)
+        // -- keys are distinct, so the merge keeps position order for equal
+        // columns
+        uint64_t *kin = s_key, *kout = s_key2;
+        for (uint32_t w = 1; w < na; w <<= 1) {
+            for (uint32_t x = tid; x < ub; x += T) {
+                const uint64_t key = kin[x];
+                const uint32_t run = ((uint32_t)(key >> 16) & 0xffffu) / w;
+                const uint32_t rs = s_pre[min(run * w, na)];
+                const uint32_t pr = run ^ 1u;
+                const uint32_t ps = s_pre[min(pr * w, na)], pe = s_pre[min((pr + 1) * w, na)];
+                uint32_t lo = ps, hi = pe;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (kin[mid] < key) lo = mid + 1;
+                    else hi = mid;
                 }
-                __syncthreads();
+                kout[min(rs, ps) + (x - rs) + (lo - ps)] = key;
             }
+            __syncthreads();
+            uint64_t *t = kin;
+            kin = kout;
+            kout = t;
         }
         // 4. one lane per column run: sum from 0.0 in position order
         uint32_t kept = 0;
@@ -561,10 +598,10 @@ __global__ __launch_bounds__(T) void k_spgemm_rows(const uint32_t *__restrict__ 
             uint32_t keep = 0, c = 0;
             double sum = 0.0;
             if (s < ub) {
-                c = (uint32_t)(s_key[s] >> 32);
-                if (s == 0 || (uint32_t)(s_key[s - 1] >> 32) != c) {
-                    for (uint32_t u = s; u < ub && (uint32_t)(s_key[u] >> 32) == c; ++u)
-                        sum = sum + s_prod[(uint32_t)s_key[u]];
+                c = (uint32_t)(kin[s] >> 32);
+                if (s == 0 || (uint32_t)(kin[s - 1] >> 32) != c) {
+                    for (uint32_t u = s; u < ub && (uint32_t)(kin[u] >> 32) == c; ++u)
+                        sum = sum + s_prod[(uint32_t)kin[u] & 0xffffu];
                     keep = sum != 0.0;
                 }
             }
@@ -853,10 +890,10 @@ extern "C" int spira_spgemm_f64(spira_dcsr *A, spira_dcsr *B, spira_dcsr *C)
             k_spgemm_lanes<64><<<blocks(ln[2], 4, 65536), 256, 0, st>>>(l[2], ln[2], A->off, A->col, A->val, B->off,
                                                                        B->col, B->val, base, cnt, t_col, t_val);
         if (ln[3])
-            k_spgemm_rows<64, kClassCap[3]><<<blocks(ln[3], 1, 65536), 64, 0, st>>>(
+            k_spgemm_rows<64, kClassCap[3], kClassNa[3]><<<blocks(ln[3], 1, 65536), 64, 0, st>>>(
                 l[3], ln[3], A->off, A->col, A->val, B->off, B->col, B->val, base, cnt, t_col, t_val);
         if (ln[4])
-            k_spgemm_rows<256, kClassCap[4]><<<blocks(ln[4], 1, 32768), 256, 0, st>>>(
+            k_spgemm_rows<256, kClassCap[4], kClassNa[4]><<<blocks(ln[4], 1, 32768), 256, 0, st>>>(
                 l[4], ln[4], A->off, A->col, A->val, B->off, B->col, B->val, base, cnt, t_col, t_val);
         SPIRA_HIP(hipGetLastError());
         const uint32_t n5 = ln[5];
