@@ -580,8 +580,8 @@ __global__ __launch_bounds__(kThreads) void k_batch_spmm(kparams kp)
             const lds_layout<ST, CT> L(xbytes, rows, nnz);
             uint32_t *s_off = (uint32_t *)(lds + L.off_o);
             uint32_t *s_rid = (uint32_t *)(lds + L.rid_o);
-            for (uint32_t q = threadIdx.x; q <= rows; q += kThreads) s_off[q] = kp.off[it.r0 + q] - it.ebase;
-            for (uint32_t q = threadIdx.x; q < rows; q += kThreads) s_rid[q] = kp.rowid[it.r0 + q];
+            // the entry copies first: the row arrays go through registers, and
+            // waiting for their loads waits for every copy issued before them
             stage_array<stage_depth<K>()>(lds, L.col_o, (const CT *)kp.col, it.ebase, nnz);
             if constexpr (ST == SPIRA_VAL_F32) {
                 stage_array<stage_depth<K>()>(lds, L.v1_o, kp.f32, it.ebase, nnz);
@@ -590,6 +590,8 @@ __global__ __launch_bounds__(kThreads) void k_batch_spmm(kparams kp)
                 if constexpr (ST == SPIRA_VAL_SPLIT_BF16)
                     stage_array<stage_depth<K>()>(lds, L.v2_o, kp.lo, it.ebase, nnz);
             }
+            for (uint32_t q = threadIdx.x; q <= rows; q += kThreads) s_off[q] = kp.off[it.r0 + q] - it.ebase;
+            for (uint32_t q = threadIdx.x; q < rows; q += kThreads) s_rid[q] = kp.rowid[it.r0 + q];
         }
         if (xs || staged) {
             stage_wait<stage_depth<K>()>();
