@@ -5,16 +5,12 @@
 // matrix_addition.hpp:24-99): every row is the column-ordered union of A's
 // and B's row; a shared column holds a + b, a lone one its value; the output's
 // first lock() drops every zero.  A - B adds B scaled by -1
-// (matrix_operators.hpp:32-41, scalars.hpp:23-35).  Merge by ranking, no
-// per-row loop:
-//   1. k_add_flags: per entry of A and of B, look its column up in the other
-//      row (binary search); keep = the value it contributes is not zero (a B
-//      entry whose column A also holds contributes through A's entry).
-//   2. exclusive scans of both keep arrays; C's offsets are
-//      keepA-prefix[A.off[r]] + keepB-prefix[B.off[r]].
-//   3. k_add_scatter: an entry's output slot is its row's offset + the kept
-//      entries before it in its own row + the kept entries of the other row
-//      with smaller columns (one more binary search).
+// (matrix_operators.hpp:32-41, scalars.hpp:23-35).  Rows are classified by
+// la + lb: up to 64 entries are merged in registers (k_add_lanes<G>, a
+// bitonic network on column << 1 | source, the shared column's two entries
+// then neighbours), longer rows along the merge path (k_add_long); both write
+// a row's kept entries to its scratch area (A.off[r] + B.off[r]), and a scan
+// of the counts and a gather build C.
 //
 // SpGEMM (serial/spgemm.hpp:12-50, parallel/algorithms/spgemm.hpp:24-69):
 // Gustavson rows.  For output (i, j) the reference adds the products
@@ -55,8 +51,6 @@ constexpr int kThreads = 256;
 constexpr int kItems = 8;        // entries per thread in the addition passes
 constexpr int kRowCache = 2048;  // row offsets a block keeps in LDS
 
-constexpr uint32_t kSmallCap = 256;   // products (and A entries) of a 64-lane row workgroup
-constexpr uint32_t kLargeCap = 1024;  // ... of a 256-lane row workgroup
 
 // Rows of the entries k = first + it * kThreads + threadIdx.x (it < kItems)
 // of a CSR: the block's row range from two wavefront-wide searches, then a
@@ -101,109 +95,6 @@ __device__ __forceinline__ double scale_b(double b, int subtract)
     return subtract ? b * -1.0 : b;
 }
 
-// keep[k] = 1 when entry k of A (blocks [0, a_blocks)) or of B (the rest)
-// contributes a nonzero value to C.
-__global__ __launch_bounds__(kThreads) void k_add_flags(uint64_t n_rows, const uint64_t *__restrict__ a_off,
-                                                         const uint32_t *__restrict__ a_col,
-                                                         const double *__restrict__ a_val, uint64_t a_nnz,
-                                                         uint32_t a_blocks, const uint64_t *__restrict__ b_off,
-                                                         const uint32_t *__restrict__ b_col,
-                                                         const double *__restrict__ b_val, uint64_t b_nnz,
-                                                         int subtract, uint64_t *__restrict__ keep_a,
-                                                         uint64_t *__restrict__ keep_b)
-{
-#pragma clang fp contract(off)
-    __shared__ uint64_t s_off[kRowCache + 1];
-    const bool side_b = blockIdx.x >= a_blocks;
-    const uint64_t blk = side_b ? blockIdx.x - a_blocks : blockIdx.x;
-    const uint64_t *off = side_b ? b_off : a_off;
-    const uint32_t *col = side_b ? b_col : a_col;
-    const double *val = side_b ? b_val : a_val;
-    const uint64_t *o_off = side_b ? a_off : b_off;
-    const uint32_t *o_col = side_b ? a_col : b_col;
-    const double *o_val = side_b ? a_val : b_val;
-    const uint64_t nnz = side_b ? b_nnz : a_nnz;
-    uint64_t *keep = side_b ? keep_b : keep_a;
-    const uint64_t first = blk * kThreads * kItems;
-    if (first >= nnz) return;  // whole blocks leave together
-    uint64_t rows[kItems];
-    entry_rows(off, n_rows, nnz, first, s_off, rows);
-#pragma unroll
-    for (int it = 0; it < kItems; ++it) {
-        const uint64_t k = first + (uint64_t)it * kThreads + threadIdx.x;
-        if (k >= nnz) break;
-        const uint64_t r = rows[it];
-        const uint32_t c = col[k];
-        const uint64_t lo = o_off[r], hi = o_off[r + 1];
-        const uint64_t p = lower_bound_u32(o_col, lo, hi, c);
-        const bool matched = p < hi && o_col[p] == c;
-        bool kp;
-        if (!side_b) {
-            const double v = matched ? val[k] + scale_b(o_val[p], subtract) : val[k];
-            kp = !is_zero_f64(v);
-        } else {
-            kp = !matched && !is_zero_f64(val[k]);  // b * -1 is zero exactly when b is
-        }
-        keep[k] = kp ? 1 : 0;
-    }
-}
-
-// c_off[r] = kept A entries before row r + kept B entries before row r.
-__global__ void k_add_offsets(uint64_t n_rows, const uint64_t *__restrict__ a_off, const uint64_t *__restrict__ sa,
-                              const uint64_t *__restrict__ b_off, const uint64_t *__restrict__ sb,
-                              uint64_t *__restrict__ c_off)
-{
-    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r <= n_rows;
-         r += (uint64_t)gridDim.x * blockDim.x)
-        c_off[r] = sa[a_off[r]] + sb[b_off[r]];
-}
-
-__global__ __launch_bounds__(kThreads) void k_add_scatter(uint64_t n_rows, const uint64_t *__restrict__ a_off,
-                                                           const uint32_t *__restrict__ a_col,
-                                                           const double *__restrict__ a_val, uint64_t a_nnz,
-                                                           uint32_t a_blocks, const uint64_t *__restrict__ b_off,
-                                                           const uint32_t *__restrict__ b_col,
-                                                           const double *__restrict__ b_val, uint64_t b_nnz,
-                                                           int subtract, const uint64_t *__restrict__ sa,
-                                                           const uint64_t *__restrict__ sb,
-                                                           const uint64_t *__restrict__ c_off,
-                                                           uint32_t *__restrict__ c_col, double *__restrict__ c_val)
-{
-#pragma clang fp contract(off)
-    __shared__ uint64_t s_off[kRowCache + 1];
-    const bool side_b = blockIdx.x >= a_blocks;
-    const uint64_t blk = side_b ? blockIdx.x - a_blocks : blockIdx.x;
-    const uint64_t *off = side_b ? b_off : a_off;
-    const uint32_t *col = side_b ? b_col : a_col;
-    const double *val = side_b ? b_val : a_val;
-    const uint64_t *o_off = side_b ? a_off : b_off;
-    const uint32_t *o_col = side_b ? a_col : b_col;
-    const double *o_val = side_b ? a_val : b_val;
-    const uint64_t nnz = side_b ? b_nnz : a_nnz;
-    const uint64_t *s_own = side_b ? sb : sa;
-    const uint64_t *s_oth = side_b ? sa : sb;
-    const uint64_t first = blk * kThreads * kItems;
-    if (first >= nnz) return;
-    uint64_t rows[kItems];
-    entry_rows(off, n_rows, nnz, first, s_off, rows);
-#pragma unroll
-    for (int it = 0; it < kItems; ++it) {
-        const uint64_t k = first + (uint64_t)it * kThreads + threadIdx.x;
-        if (k >= nnz) break;
-        if (s_own[k + 1] == s_own[k]) continue;  // not kept
-        const uint64_t r = rows[it];
-        const uint32_t c = col[k];
-        const uint64_t lo = o_off[r], hi = o_off[r + 1];
-        const uint64_t p = lower_bound_u32(o_col, lo, hi, c);
-        const uint64_t pos = c_off[r] + (s_own[k] - s_own[off[r]]) + (s_oth[p] - s_oth[lo]);
-        double v;
-        if (!side_b) v = (p < hi && o_col[p] == c) ? val[k] + scale_b(o_val[p], subtract) : val[k];
-        else v = scale_b(val[k], subtract);
-        c_col[pos] = c;
-        c_val[pos] = v;
-    }
-}
-
 // ---- SpGEMM ----------------------------------------------------------------
 
 // Size classes of a row of C by its product count ub and A entries na:
@@ -234,6 +125,49 @@ __global__ void k_spgemm_entry_len(uint64_t nnz_a, const uint32_t *__restrict__ 
 constexpr int kClassifyItems = 16;
 constexpr uint64_t kClassifyRows = (uint64_t)kThreads * kClassifyItems;
 
+// Appends the rows r0 + it * kThreads + threadIdx.x with class cls[it] >= 0 to
+// lists[class * n_rows + ...]: counts in LDS (one LDS atomic per wavefront and
+// class), one global atomic per class reserves the block's ranges, then the
+// rows are written (order inside a list is free: rows are independent).
+template <int NC>
+__device__ __forceinline__ void append_to_lists(const int (&cls)[kClassifyItems], uint64_t r0, uint64_t n_rows,
+                                                uint32_t *__restrict__ lists, uint32_t *__restrict__ list_n)
+{
+    __shared__ uint32_t s_n[NC], s_b[NC];
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    if (threadIdx.x < NC) s_n[threadIdx.x] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < kClassifyItems; ++it) {
+        for (int q = 0; q < NC; ++q) {
+            const uint64_t m = __ballot(cls[it] == q);
+            if (m && lane == (uint32_t)(__ffsll((unsigned long long)m) - 1)) atomicAdd(&s_n[q], (uint32_t)__popcll(m));
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < NC) {
+        const uint32_t q = threadIdx.x;
+        s_b[q] = s_n[q] ? atomicAdd(&list_n[q], s_n[q]) : 0u;
+        s_n[q] = 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < kClassifyItems; ++it) {
+        const uint64_t i = r0 + (uint64_t)it * kThreads + threadIdx.x;
+        const int c = cls[it];
+        for (int q = 0; q < NC; ++q) {
+            const uint64_t m = __ballot(c == q);
+            if (!m) continue;
+            const uint32_t leader = __ffsll((unsigned long long)m) - 1;
+            uint32_t b0 = 0;
+            if (lane == leader) b0 = atomicAdd(&s_n[q], (uint32_t)__popcll(m));
+            b0 = __shfl(b0, leader, kWave);
+            if (c == q)
+                lists[(uint64_t)q * n_rows + s_b[q] + b0 + __popcll(m & ((1ull << lane) - 1))] = (uint32_t)i;
+        }
+    }
+}
+
 __global__ __launch_bounds__(kThreads) void k_spgemm_classify(uint64_t n_rows, const uint64_t *__restrict__ a_off,
                                                                const uint64_t *__restrict__ s,
                                                                uint64_t *__restrict__ base,
@@ -241,10 +175,6 @@ __global__ __launch_bounds__(kThreads) void k_spgemm_classify(uint64_t n_rows, c
                                                                uint32_t *__restrict__ lists,
                                                                uint32_t *__restrict__ list_n)
 {
-    __shared__ uint32_t s_n[kClasses], s_b[kClasses];
-    if (threadIdx.x < kClasses) s_n[threadIdx.x] = 0;
-    __syncthreads();
-    const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint64_t r0 = (uint64_t)blockIdx.x * kClassifyRows;
     int cls[kClassifyItems];
 #pragma unroll
@@ -265,33 +195,8 @@ __global__ __launch_bounds__(kThreads) void k_spgemm_classify(uint64_t n_rows, c
             base[i] = s[a_off[i]];
         }
         cls[it] = c;
-        for (int q = 0; q < kClasses; ++q) {  // one LDS atomic per wavefront and class
-            const uint64_t m = __ballot(c == q);
-            if (m && lane == (uint32_t)(__ffsll((unsigned long long)m) - 1)) atomicAdd(&s_n[q], (uint32_t)__popcll(m));
-        }
     }
-    __syncthreads();
-    if (threadIdx.x < kClasses) {
-        const uint32_t q = threadIdx.x;
-        s_b[q] = s_n[q] ? atomicAdd(&list_n[q], s_n[q]) : 0u;
-        s_n[q] = 0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int it = 0; it < kClassifyItems; ++it) {
-        const uint64_t i = r0 + (uint64_t)it * kThreads + threadIdx.x;
-        const int c = cls[it];
-        for (int q = 0; q < kClasses; ++q) {
-            const uint64_t m = __ballot(c == q);
-            if (!m) continue;
-            const uint32_t leader = __ffsll((unsigned long long)m) - 1;
-            uint32_t b0 = 0;
-            if (lane == leader) b0 = atomicAdd(&s_n[q], (uint32_t)__popcll(m));
-            b0 = __shfl(b0, leader, kWave);
-            if (c == q)
-                lists[(uint64_t)q * n_rows + s_b[q] + b0 + __popcll(m & ((1ull << lane) - 1))] = (uint32_t)i;
-        }
-    }
+    append_to_lists<kClasses>(cls, r0, n_rows, lists, list_n);
 }
 
 // v of lane (lane ^ m): masks 1, 2, 4, 8 stay inside a 16-lane row and go
@@ -327,6 +232,14 @@ __device__ __forceinline__ uint64_t next_lane64(uint64_t v)
 {
     const int hi = __builtin_amdgcn_update_dpp(0, (int)(v >> 32), 0x130, 0xF, 0xF, false);
     const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, 0x130, 0xF, 0xF, false);
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+// v of lane - 1 (wave_shr1; lane 0 reads 0).
+__device__ __forceinline__ uint64_t prev_lane64(uint64_t v)
+{
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(v >> 32), 0x138, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, 0x138, 0xF, 0xF, false);
     return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
 }
 
@@ -720,6 +633,200 @@ __global__ __launch_bounds__(kThreads) void k_spgemm_gather(uint64_t n_rows, uin
     }
 }
 
+// ---- addition ----------------------------------------------------------------
+
+// Size classes of an output row by its input entries la + lb: at most 16 / 32
+// / 64 (k_add_lanes<G>), or longer (k_add_long).
+constexpr int kAddClasses = 4;
+constexpr uint32_t kAddCap[kAddClasses - 1] = {16, 32, 64};
+
+// base[i] = A.off[i] + B.off[i]: row i's scratch area (room for la + lb
+// entries); cnt[i] = 0 for empty rows; rows with entries go to their class list.
+__global__ __launch_bounds__(kThreads) void k_add_classify(uint64_t n_rows, const uint64_t *__restrict__ a_off,
+                                                            const uint64_t *__restrict__ b_off,
+                                                            uint64_t *__restrict__ base, uint64_t *__restrict__ cnt,
+                                                            uint32_t *__restrict__ lists,
+                                                            uint32_t *__restrict__ list_n)
+{
+    const uint64_t r0 = (uint64_t)blockIdx.x * kClassifyRows;
+    int cls[kClassifyItems];
+#pragma unroll
+    for (int it = 0; it < kClassifyItems; ++it) {
+        const uint64_t i = r0 + (uint64_t)it * kThreads + threadIdx.x;
+        int c = -1;
+        if (i < n_rows) {
+            const uint64_t a0 = a_off[i], b0 = b_off[i];
+            const uint64_t t = (a_off[i + 1] - a0) + (b_off[i + 1] - b0);
+            base[i] = a0 + b0;
+            cnt[i] = 0;
+            if (t > 0) {
+                c = kAddClasses - 1;
+                for (int q = kAddClasses - 2; q >= 0; --q)
+                    if (t <= kAddCap[q]) c = q;
+            }
+        } else if (i == n_rows) {
+            base[i] = a_off[i] + b_off[i];
+        }
+        cls[it] = c;
+    }
+    append_to_lists<kAddClasses>(cls, r0, n_rows, lists, list_n);
+}
+
+// Rows with at most G entries in A's and B's row together, 64 / G rows per
+// wavefront, in registers: lane g holds A's entry g or B's entry g - la, keyed
+// column << 1 | (0 for A, 1 for B); a bitonic network sorts the keys, so a
+// column both rows hold sits on two neighbouring lanes, A's first: that lane
+// writes a + b (b * -1 for A - B), the next writes nothing; zeros are dropped.
+template <int G>
+__global__ __launch_bounds__(256) void k_add_lanes(const uint32_t *__restrict__ list, uint32_t n_list,
+                                                    const uint64_t *__restrict__ a_off,
+                                                    const uint32_t *__restrict__ a_col,
+                                                    const double *__restrict__ a_val,
+                                                    const uint64_t *__restrict__ b_off,
+                                                    const uint32_t *__restrict__ b_col,
+                                                    const double *__restrict__ b_val, int subtract,
+                                                    const uint64_t *__restrict__ base, uint64_t *__restrict__ cnt,
+                                                    uint32_t *__restrict__ t_col, double *__restrict__ t_val)
+{
+#pragma clang fp contract(off)
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t g = lane & (G - 1);
+    const uint32_t groups = blockDim.x / G;
+    const uint64_t gmask = G == kWave ? ~0ull : (((1ull << G) - 1) << (lane & ~(uint32_t)(G - 1)));
+    for (uint64_t l0 = (uint64_t)blockIdx.x * groups; l0 < n_list; l0 += (uint64_t)gridDim.x * groups) {
+        const uint64_t li = l0 + threadIdx.x / G;
+        const bool active = li < n_list;
+        const uint32_t i = active ? list[li] : 0;
+        uint64_t a0 = 0, b0 = 0;
+        uint32_t la = 0, lb = 0;
+        if (active) {
+            a0 = a_off[i];
+            la = (uint32_t)(a_off[i + 1] - a0);
+            b0 = b_off[i];
+            lb = (uint32_t)(b_off[i + 1] - b0);  // la + lb <= G (classified)
+        }
+        uint64_t key = ~0ull;
+        double v = 0.0;
+        if (g < la) {
+            key = (uint64_t)a_col[a0 + g] << 1;
+            v = a_val[a0 + g];
+        } else if (g < la + lb) {
+            const uint64_t k = b0 + (g - la);
+            key = ((uint64_t)b_col[k] << 1) | 1u;
+            v = scale_b(b_val[k], subtract);
+        }
+#pragma unroll
+        for (int size = 2; size <= G; size <<= 1) {
+#pragma unroll
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                const uint64_t ok = xor_lane64(key, (uint32_t)stride);
+                const double ov = __builtin_bit_cast(double, xor_lane64(__builtin_bit_cast(uint64_t, v), (uint32_t)stride));
+                const bool up = (g & size) == 0, lower = (g & stride) == 0;
+                if (lower == up ? ok < key : ok > key) {
+                    key = ok;
+                    v = ov;
+                }
+            }
+        }
+        const uint32_t n = la + lb;
+        const uint64_t nk = next_lane64(key), pk = prev_lane64(key);
+        const double nv = __builtin_bit_cast(double, next_lane64(__builtin_bit_cast(uint64_t, v)));
+        const bool valid = g < n;
+        const bool dup = (key & 1u) && g > 0 && (pk >> 1) == (key >> 1);  // B's entry of a shared column
+        double out = v;
+        if (!(key & 1u) && g + 1 < n && (nk >> 1) == (key >> 1)) out = v + nv;
+        const bool keep = valid && !dup && !is_zero_f64(out);
+        const uint64_t bal = __ballot(keep) & gmask;
+        if (keep) {
+            const uint64_t pos = base[i] + __popcll(bal & ((1ull << lane) - 1));
+            t_col[pos] = (uint32_t)(key >> 1);
+            t_val[pos] = out;
+        }
+        if (active && g == 0) cnt[i] = __popcll(bal);
+    }
+}
+
+// Longer rows: one workgroup per row walks the merged order (A's entry before
+// B's on a shared column) in chunks of kThreads * kAddItems positions; each
+// thread finds its start on the merge path by a binary search, merges
+// kAddItems positions (a shared column: a + b at A's entry, nothing at B's),
+// and a workgroup scan of the kept counts places the entries.
+constexpr int kAddItems = 8;
+
+__global__ __launch_bounds__(kThreads) void k_add_long(const uint32_t *__restrict__ list, uint32_t n_list,
+                                                        const uint64_t *__restrict__ a_off,
+                                                        const uint32_t *__restrict__ a_col,
+                                                        const double *__restrict__ a_val,
+                                                        const uint64_t *__restrict__ b_off,
+                                                        const uint32_t *__restrict__ b_col,
+                                                        const double *__restrict__ b_val, int subtract,
+                                                        const uint64_t *__restrict__ base, uint64_t *__restrict__ cnt,
+                                                        uint32_t *__restrict__ t_col, double *__restrict__ t_val)
+{
+#pragma clang fp contract(off)
+    __shared__ uint32_t s_w[kThreads / kWave];
+    for (uint32_t li = blockIdx.x; li < n_list; li += gridDim.x) {
+        const uint32_t i = list[li];
+        const uint64_t a0 = a_off[i], la = a_off[i + 1] - a0, b0 = b_off[i], lb = b_off[i + 1] - b0;
+        const uint64_t n = la + lb, dst = base[i];
+        const uint32_t *ac = a_col + a0, *bc = b_col + b0;
+        const double *av = a_val + a0, *bv = b_val + b0;
+        uint64_t carry = 0;
+        for (uint64_t c0 = 0; c0 < n; c0 += (uint64_t)kThreads * kAddItems) {
+            const uint64_t d = min(n, c0 + (uint64_t)threadIdx.x * kAddItems);
+            // A entries among the first d merged: A's key col << 1, B's col << 1 | 1
+            uint64_t lo = d > lb ? d - lb : 0, hi = min(d, la);
+            while (lo < hi) {
+                const uint64_t mid = (lo + hi) >> 1;
+                if (((uint64_t)ac[mid] << 1) < (((uint64_t)bc[d - mid - 1] << 1) | 1u)) lo = mid + 1;
+                else hi = mid;
+            }
+            uint64_t ia = lo, ib = d - lo;
+            uint32_t oc[kAddItems];
+            double ov[kAddItems];
+            bool kp[kAddItems];
+            uint32_t nk = 0;
+#pragma unroll
+            for (int t = 0; t < kAddItems; ++t) {
+                kp[t] = false;
+                oc[t] = 0;
+                ov[t] = 0.0;
+                if (d + t >= n) continue;
+                if (ib >= lb || (ia < la && ac[ia] <= bc[ib])) {  // A's entry (first on a shared column)
+                    const uint32_t c = ac[ia];
+                    double v = av[ia];
+                    if (ib < lb && bc[ib] == c) v = v + scale_b(bv[ib], subtract);
+                    ++ia;
+                    oc[t] = c;
+                    ov[t] = v;
+                    kp[t] = !is_zero_f64(v);
+                } else {
+                    const uint32_t c = bc[ib];
+                    const double v = scale_b(bv[ib], subtract);
+                    ++ib;
+                    oc[t] = c;
+                    ov[t] = v;
+                    kp[t] = !(ia > 0 && ac[ia - 1] == c) && !is_zero_f64(v);
+                }
+                nk += kp[t] ? 1u : 0u;
+            }
+            uint32_t tot;
+            const uint32_t incl = block_incl_scan<kThreads>(nk, s_w, tot);
+            uint64_t pos = dst + carry + (incl - nk);
+#pragma unroll
+            for (int t = 0; t < kAddItems; ++t) {
+                if (kp[t]) {
+                    t_col[pos] = oc[t];
+                    t_val[pos] = ov[t];
+                    ++pos;
+                }
+            }
+            carry += tot;
+        }
+        if (threadIdx.x == 0) cnt[i] = carry;
+    }
+}
+
 unsigned bits_for(uint64_t max_value)
 {
     unsigned b = 1;
@@ -801,38 +908,56 @@ extern "C" int spira_add_f64(spira_dcsr *A, spira_dcsr *B, spira_dcsr *C, uint32
         if (C->n_rows != A->n_rows || C->n_cols != A->n_cols)
             fail(SPIRA_ERR_INVALID_ARGUMENT, "add: C must have the shape of A and B");
         if (flags & ~(uint32_t)SPIRA_ADD_SUBTRACT) fail(SPIRA_ERR_INVALID_ARGUMENT, "add: unknown flags");
-        const uint64_t n = A->n_rows, na = A->nnz, nb = B->nnz;
-        const uint64_t blocks_a = (na + (uint64_t)kThreads * kItems - 1) / ((uint64_t)kThreads * kItems);
-        const uint64_t blocks_b = (nb + (uint64_t)kThreads * kItems - 1) / ((uint64_t)kThreads * kItems);
-        if (blocks_a + blocks_b >= (1ull << 31)) fail(SPIRA_ERR_UNSUPPORTED, "add: too many entries");
+        const uint64_t n = A->n_rows;
+        if (n >= (1ull << 32) - 1) fail(SPIRA_ERR_UNSUPPORTED, "add: needs fewer than 2^32 - 1 rows");
         const int sub = (flags & SPIRA_ADD_SUBTRACT) ? 1 : 0;
         hipStream_t st = dev->stream;
         scratch_set tmp{dev, {}};
-        uint64_t *sa = (uint64_t *)tmp.get((na + 1) * sizeof(uint64_t));
-        uint64_t *sb = (uint64_t *)tmp.get((nb + 1) * sizeof(uint64_t));
-        uint64_t *c_off = (uint64_t *)tmp.get((n + 1) * sizeof(uint64_t));
-        if (blocks_a + blocks_b > 0) {
-            k_add_flags<<<(unsigned)(blocks_a + blocks_b), kThreads, 0, st>>>(
-                n, A->off, A->col, A->val, na, (uint32_t)blocks_a, B->off, B->col, B->val, nb, sub, sa, sb);
-            SPIRA_HIP(hipGetLastError());
-        }
-        scan_excl_u64(dev, sa, sa, na);
-        scan_excl_u64(dev, sb, sb, nb);
-        k_add_offsets<<<grid_for(n + 1, kThreads, 8192), kThreads, 0, st>>>(n, A->off, sa, B->off, sb, c_off);
+        uint64_t *cnt = (uint64_t *)tmp.get((n + 1) * sizeof(uint64_t));   // row counts, then C's offsets
+        uint64_t *base = (uint64_t *)tmp.get((n + 1) * sizeof(uint64_t));  // scratch area of each row
+        uint32_t *lists = (uint32_t *)tmp.get((uint64_t)kAddClasses * (n ? n : 1) * sizeof(uint32_t));
+        uint32_t *list_n = (uint32_t *)dev->ws.get(WS_MISC, 8 * sizeof(uint32_t));
+        SPIRA_HIP(hipMemsetAsync(list_n, 0, 8 * sizeof(uint32_t), st));
+        k_add_classify<<<(unsigned)((n + kClassifyRows) / kClassifyRows), kThreads, 0, st>>>(n, A->off, B->off, base,
+                                                                                         cnt, lists, list_n);
         SPIRA_HIP(hipGetLastError());
-        const uint64_t nnz = read_scalar(dev, c_off + n);
+        uint32_t ln[8] = {};
+        SPIRA_HIP(hipMemcpyAsync(ln, list_n, sizeof(ln), hipMemcpyDeviceToHost, st));
+        const uint64_t total = read_scalar(dev, base + n);  // nnz(A) + nnz(B); synchronises ln too
+        uint32_t *t_col = (uint32_t *)tmp.get(total * sizeof(uint32_t));
+        double *t_val = (double *)tmp.get(total * sizeof(double));
+        const uint32_t *l[kAddClasses];
+        for (int c = 0; c < kAddClasses; ++c) l[c] = lists + (uint64_t)c * n;
+        auto blocks = [](uint32_t rows, uint32_t per_block, uint32_t cap) {
+            return (unsigned)std::min<uint64_t>((rows + per_block - 1) / per_block, cap);
+        };
+        if (ln[0])
+            k_add_lanes<16><<<blocks(ln[0], 16, 65536), 256, 0, st>>>(l[0], ln[0], A->off, A->col, A->val, B->off,
+                                                                     B->col, B->val, sub, base, cnt, t_col, t_val);
+        if (ln[1])
+            k_add_lanes<32><<<blocks(ln[1], 8, 65536), 256, 0, st>>>(l[1], ln[1], A->off, A->col, A->val, B->off,
+                                                                    B->col, B->val, sub, base, cnt, t_col, t_val);
+        if (ln[2])
+            k_add_lanes<64><<<blocks(ln[2], 4, 65536), 256, 0, st>>>(l[2], ln[2], A->off, A->col, A->val, B->off,
+                                                                    B->col, B->val, sub, base, cnt, t_col, t_val);
+        if (ln[3])
+            k_add_long<<<blocks(ln[3], 1, 16384), kThreads, 0, st>>>(l[3], ln[3], A->off, A->col, A->val, B->off,
+                                                                    B->col, B->val, sub, base, cnt, t_col, t_val);
+        SPIRA_HIP(hipGetLastError());
+        scan_excl_u64(dev, cnt, cnt, n);
+        const uint64_t nnz = read_scalar(dev, cnt + n);
         uint32_t *c_col = (uint32_t *)tmp.get(nnz * sizeof(uint32_t));
         double *c_val = (double *)tmp.get(nnz * sizeof(double));
-        if (blocks_a + blocks_b > 0 && nnz > 0) {
-            k_add_scatter<<<(unsigned)(blocks_a + blocks_b), kThreads, 0, st>>>(
-                n, A->off, A->col, A->val, na, (uint32_t)blocks_a, B->off, B->col, B->val, nb, sub, sa, sb, c_off,
-                c_col, c_val);
+        if (nnz > 0) {
+            const uint64_t nb = (nnz + (uint64_t)kThreads * kItems - 1) / ((uint64_t)kThreads * kItems);
+            if (nb >= (1ull << 31)) fail(SPIRA_ERR_UNSUPPORTED, "add: too many output entries");
+            k_spgemm_gather<<<(unsigned)nb, kThreads, 0, st>>>(n, nnz, cnt, base, t_col, t_val, c_col, c_val);
             SPIRA_HIP(hipGetLastError());
         }
-        tmp.keep(c_off);
+        tmp.keep(cnt);
         tmp.keep(c_col);
         tmp.keep(c_val);
-        install(C, c_off, c_col, c_val, nnz);  // C may be A or B: stream order protects their arrays
+        install(C, cnt, c_col, c_val, nnz);  // C may be A or B: stream order protects their arrays
     });
 }
 

@@ -203,6 +203,10 @@ ADD_CASES = [
     ("long_rows", lambda: (W.random_uniform_rows(40, 200000, 2000, 9000, seed=69),
                            W.random_uniform_rows(40, 200000, 0, 9000, seed=70), 200000)),
     ("mostly_empty_rows", lambda: (_mostly_empty(400000, 3000, 0.02, 71), _mostly_empty(400000, 3000, 0.03, 72), 3000)),
+    ("mixed_lengths", lambda: (W.random_uniform_rows(20000, 6000, 0, 90, seed=73),
+                               W.random_uniform_rows(20000, 6000, 0, 90, seed=74), 6000)),
+    ("one_row_dense_other_sparse", lambda: (W.random_uniform_rows(3, 50000, 20000, 30000, seed=75),
+                                            W.random_uniform_rows(3, 50000, 0, 40, seed=76), 50000)),
 ]
 
 
