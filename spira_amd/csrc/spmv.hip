@@ -26,6 +26,8 @@
 // keeping a row neighbourhood's x lines in one L2.
 #include "internal.hpp"
 
+#include <rocprim/device/device_segmented_radix_sort.hpp>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -708,12 +710,35 @@ void ensure_xphase(spira_dcsr *A)
     P.ph_val = (double *)plan_alloc(A, nz * sizeof(double));
     uint64_t *goff = (uint64_t *)plan_alloc(A, (nb * S + 1) * sizeof(uint64_t));
     P.ph_sync = (uint32_t *)plan_alloc(A, 8 * kPhaseSyncStride * sizeof(uint32_t));
+    // entries grouped in row order first, then every group sorted by its
+    // column bits: the 64 entries of a wavefront load then sweep x in order,
+    // so lanes on one x line share a request and a line a group touches
+    // twice is fetched once
+    struct temp {  // released on every exit (stream order protects queued work)
+        spira_dev *d;
+        void *p;
+        size_t bytes;
+        ~temp() { dev_release(d, p, bytes); }
+    } t_rc{dev, dev_alloc(dev, nz * sizeof(uint32_t)), nz * sizeof(uint32_t)},
+        t_val{dev, dev_alloc(dev, nz * sizeof(double)), nz * sizeof(double)};
+    uint32_t *rc0 = (uint32_t *)t_rc.p;
+    double *val0 = (double *)t_val.p;
     k_xphase_scatter<<<grid_for(n, kThreads), kThreads, 0, dev->stream>>>(
-        n, (uint32_t)rb, (uint32_t)S, width, A->off, A->col, A->val, pos, P.ph_rc, P.ph_val);
+        n, (uint32_t)rb, (uint32_t)S, width, A->off, A->col, A->val, pos, rc0, val0);
     SPIRA_HIP(hipGetLastError());
     k_xphase_goff<<<grid_for(nb * S + 1, kThreads), kThreads, 0, dev->stream>>>(nb * S, (uint32_t)rb, pos, A->nnz,
                                                                                 goff);
     SPIRA_HIP(hipGetLastError());
+    if (A->nnz > 0) {
+        size_t tb = 0;
+        SPIRA_HIP(rocprim::segmented_radix_sort_pairs(nullptr, tb, rc0, P.ph_rc, val0, P.ph_val, (unsigned)A->nnz,
+                                                      (unsigned)(nb * S), goff, goff + 1, 0, kPhaseColBits,
+                                                      dev->stream));
+        void *t = dev->ws.get(WS_SORT, tb ? tb : 8);
+        SPIRA_HIP(rocprim::segmented_radix_sort_pairs(t, tb, rc0, P.ph_rc, val0, P.ph_val, (unsigned)A->nnz,
+                                                      (unsigned)(nb * S), goff, goff + 1, 0, kPhaseColBits,
+                                                      dev->stream));
+    }
     dev_sync(dev);  // pos lives in the workspace: finish before it can be reused
     P.ph_S = (uint32_t)S;
     P.ph_width = (uint32_t)width;
