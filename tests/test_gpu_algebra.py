@@ -311,3 +311,40 @@ def test_spgemm_full_size_random_1m(dev):
     y1 = C.spmv(x)
     y2 = A.spmv(A.spmv(x))
     assert oracle.normwise_rel_err(y1, y2) <= 1e-12
+
+
+def _empty(n_rows):
+    return np.zeros(n_rows + 1, np.uint64), np.zeros(0, np.uint32), np.zeros(0)
+
+
+@pytest.mark.parametrize("shape", [(0, 5), (5, 0), (0, 0)], ids=lambda s: f"{s[0]}x{s[1]}")
+def test_add_degenerate_shapes(dev, shape):
+    n_rows, n_cols = shape
+    A, B = _upload(dev, _empty(n_rows), n_cols), _upload(dev, _empty(n_rows), n_cols)
+    for sub in (False, True):
+        C = A.add(B, subtract=sub)
+        assert C.nnz == 0 and _bits_equal(C.download(), _empty(n_rows))
+
+
+@pytest.mark.parametrize("dims", [(0, 5, 3), (4, 0, 3), (4, 5, 0), (3, 6, 2)], ids=lambda d: "x".join(map(str, d)))
+def test_spgemm_degenerate_shapes(dev, dims):
+    """Empty products: no rows, no inner dimension, no columns, or B without entries."""
+    n, k, m = dims
+    a = W.random_uniform_rows(n, k, 0, min(k, 3), seed=110) if n and k else _empty(n)
+    A, B = _upload(dev, a, k), _upload(dev, _empty(k), m)
+    C = A.spgemm(B)
+    assert C.n_rows == n and C.n_cols == m and C.nnz == 0
+    assert _bits_equal(C.download(), oracle.spgemm(a, _empty(k), m))
+
+
+def test_add_and_spgemm_aliasing_the_right_operand(dev):
+    """C = A + B into B's handle and C = A * B into B's handle (square)."""
+    a = W.random_uniform_rows(3000, 3000, 0, 12, seed=111)
+    b = W.random_uniform_rows(3000, 3000, 0, 12, seed=112)
+    A = _upload(dev, a, 3000)
+    B = _upload(dev, b, 3000)
+    A.add(B, B, subtract=True)
+    assert _bits_equal(B.download(), oracle.add(a, b, subtract=True))
+    B2 = _upload(dev, b, 3000)
+    A.spgemm(B2, B2)
+    assert _bits_equal(B2.download(), oracle.spgemm(a, b, 3000))

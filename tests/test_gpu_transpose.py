@@ -235,3 +235,15 @@ def test_transpose_full_size_1m(dev):
     T = A.transpose()
     assert _bits_equal(T.download(), oracle.transpose(off, col, val, n))
     assert _bits_equal(T.transpose().download(), (off, col, val))
+
+
+@pytest.mark.parametrize("shape", [(0, 5), (5, 0), (0, 0)], ids=lambda s: f"{s[0]}x{s[1]}")
+@pytest.mark.parametrize("drop", [False, True], ids=["serial", "drop_zeros"])
+def test_transpose_degenerate_shapes(dev, shape, drop):
+    n_rows, n_cols = shape
+    A = spira_amd.DeviceCSR(dev, n_rows, n_cols)
+    A.upload(np.zeros(n_rows + 1, np.uint64), np.zeros(0, np.uint32), np.zeros(0))
+    T = A.transpose(drop_zeros=drop)
+    assert T.n_rows == n_cols and T.n_cols == n_rows and T.nnz == 0
+    off, col, val = T.download()
+    assert off.tolist() == [0] * (n_cols + 1) and col.size == 0
