@@ -301,7 +301,7 @@ __global__ void k_u64_to_u32(const uint64_t *__restrict__ a, uint32_t *__restric
 // no partial-y vectors and no second kernel.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kPhaseRows = 16384;                 // rows per block: 128 KiB of FP64 y in LDS
-constexpr uint64_t kPhaseSliceBytes = 1ull << 20;       // x bytes per slice: a quarter of an XCD's L2
+constexpr uint64_t kPhaseSliceBytes = 2ull << 20;       // x bytes per slice: half of an XCD's L2
 constexpr uint32_t kPhaseMaxSlices = 256;
 // An entry's row inside its block and column inside its slice share one u32.
 constexpr uint32_t kPhaseColBits = 18;

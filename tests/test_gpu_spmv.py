@@ -128,7 +128,7 @@ def test_spmv_auto_and_device_pointers(dev):
 
 
 def test_xphase_many_slices_and_blocks(dev):
-    """XPHASE with 24 column slices (x of 24 MB, 1 MiB slices) and 256 row
+    """XPHASE with 16 column slices (x of 24 MB, slices of at most 2 MiB) and 256 row
     blocks of 782 rows: every block walks all slices in its XCD's order and
     adds into its LDS rows; both launch widths share the plan, and a plan is
     reused by a later call."""
