@@ -56,7 +56,7 @@ SPMV_AUTO = -1
 VARIANTS = {
     "vec1": 0, "vec2": 1, "vec4": 2, "vec8": 3, "vec16": 4, "vec32": 5, "vec64": 6,
     "stream1k": 7, "stream2k": 8, "xslice1": 9, "xslice2": 10, "xslice4": 11, "xstream": 12, "xphase": 13,
-    "xphase16": 14,
+    "xphase_paced": 14,
 }
 VARIANT_NAMES = {v: k for k, v in VARIANTS.items()}
 STORAGE = {"f32": 1, "bf16": 2, "split_bf16": 3}

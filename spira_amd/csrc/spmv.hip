@@ -18,7 +18,8 @@
 //   XPHASE    row blocks that keep their y in LDS; the blocks of one XCD walk
 //             L2-sized column slices in the same order (entries grouped by
 //             block and slice), so x gathers hit L2 and y is written once.
-//             XPHASE16 also paces the blocks of an XCD to within two slices.
+//             XPHASE_PACED also paces the blocks of an XCD to within two
+//             slices.  Each (block, slice) group is sorted by column.
 //
 // Values and columns are read with non-temporal loads so the x gathers keep
 // L2 and the MALL.  Blocks are renumbered so that each XCD (8 on MI355X, which
@@ -308,8 +309,9 @@ constexpr uint32_t kPhaseColBits = 18;
 static_assert(kPhaseRows <= (1u << (32 - kPhaseColBits)), "local rows must fit the high bits");
 constexpr int kPhaseThreads = 1024;
 // XPHASE: 4 entries per thread per step, phases not paced (banded matrices:
-// a block meets few slices).  XPHASE16: 16 entries per thread per step and
-// paced phases (scattered columns: every block meets every slice).
+// a block meets few slices).  XPHASE_PACED: 20 entries per thread per step
+// (a 2 MiB slice of a scattered matrix is one step for most blocks) and paced
+// phases (scattered columns: every block meets every slice).
 
 // Phase pacing.  A block may start phase t only after every block of its XCD
 // has finished phase t - 2, so an XCD's blocks gather from at most two slices
@@ -856,10 +858,10 @@ void spmv_launch_f64(spira_dcsr *A, const double *x, double *y, int variant)
     } else if (variant == SPIRA_SPMV_XSTREAM) {
         ensure_xslice(A);
         launch_xstream(A, x, y);
-    } else if (variant == SPIRA_SPMV_XPHASE || variant == SPIRA_SPMV_XPHASE16) {
+    } else if (variant == SPIRA_SPMV_XPHASE || variant == SPIRA_SPMV_XPHASE_PACED) {
         ensure_xphase(A);
         if (variant == SPIRA_SPMV_XPHASE) launch_xphase<4, false>(A, x, y);
-        else launch_xphase<16, true>(A, x, y);
+        else launch_xphase<20, true>(A, x, y);
     } else {
         fail(SPIRA_ERR_INVALID_ARGUMENT, "spmv: unknown kernel variant");
     }

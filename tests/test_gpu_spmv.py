@@ -136,7 +136,7 @@ def test_xphase_many_slices_and_blocks(dev):
     off, col, val = W.random_uniform_rows(n_rows, n_cols, 8, 32, seed=21)
     A = _upload(dev, off, col, val, n_cols)
     rng = np.random.default_rng(22)
-    for name in ("xphase", "xphase16", "xphase"):
+    for name in ("xphase", "xphase_paced", "xphase"):
         x = rng.uniform(-1, 1, n_cols)
         _check(A.spmv(x, variant=spira_amd.VARIANTS[name]), off, col, val, x)
 
