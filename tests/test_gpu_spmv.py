@@ -141,6 +141,19 @@ def test_xphase_many_slices_and_blocks(dev):
         _check(A.spmv(x, variant=spira_amd.VARIANTS[name]), off, col, val, x)
 
 
+def test_xphase_more_row_blocks_than_cus(dev):
+    """4,294,304 rows (more than 256 blocks of 16,384 rows): the grid is not
+    resident at once, so XPHASE_PACED runs its phases unpaced; both widths must
+    still match the oracle, the last block holding a partial row range."""
+    n_rows, n_cols = (1 << 22) + 100000, 2500000
+    off, col, val = W.random_uniform_rows(n_rows, n_cols, 0, 4, seed=23)
+    A = _upload(dev, off, col, val, n_cols)
+    rng = np.random.default_rng(24)
+    for name in ("xphase_paced", "xphase"):
+        x = rng.uniform(-1, 1, n_cols)
+        _check(A.spmv(x, variant=spira_amd.VARIANTS[name]), off, col, val, x)
+
+
 def test_spmv_laplacian_full_size_exact_row_sums(dev):
     """2048^2 5-point Laplacian (4,194,304 rows): A @ 1 = 4 - (#neighbours),
     exactly, for every variant; and linearity on random vectors."""
