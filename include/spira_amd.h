@@ -59,7 +59,7 @@ typedef struct spira_dcsr spira_dcsr; /* device CSR: u64 offsets, u32 cols, f64 
 #define SPIRA_SPMV_XSLICE4 11     /* ... 4 lanes per row */
 #define SPIRA_SPMV_XSTREAM 12     /* column slice per XCD, LDS row blocks per slice */
 #define SPIRA_SPMV_XPHASE 13      /* row blocks with y in LDS, x slices in L2-sized phases per XCD */
-#define SPIRA_SPMV_XPHASE_PACED 14 /* ... 20 entries per thread per step, an XCD's blocks paced to within 2 slices */
+#define SPIRA_SPMV_XPHASE_PACED 14 /* ... 20 entries per thread per step, an XCD's blocks paced to within 3 slices */
 #define SPIRA_SPMV_N_VARIANTS 15
 
 /* ---- device handle --------------------------------------------------------

@@ -314,9 +314,9 @@ constexpr int kPhaseThreads = 1024;
 // phases (scattered columns: every block meets every slice).
 
 // Phase pacing.  A block may start phase t only after every block of its XCD
-// has finished phase t - 2, so an XCD's blocks gather from at most two slices
-// at a time (they drift apart otherwise, and the XCD's L2 then holds many
-// slices' worth of x).  sync[c * kPhaseSyncStride] counts finished (block,
+// has finished phase t - 3, so an XCD's blocks gather from at most three
+// slices at a time (they drift apart otherwise, and the XCD's L2 then holds
+// many slices' worth of x).  sync[c * kPhaseSyncStride] counts finished (block,
 // phase) pairs of XCD c; it is zeroed before the launch.  The wait only paces
 // and never decides a result: it gives up after a bounded number of sleeps,
 // and the host passes sync only when every block of the grid can be resident
@@ -362,7 +362,7 @@ __global__ __launch_bounds__(kPhaseThreads) void k_spmv_xphase(uint64_t n_rows, 
     const uint32_t nbx = (gridDim.x - xcd + 7) / 8;  // blocks of this XCD
     uint32_t *ctr = sync ? sync + xcd * kPhaseSyncStride : nullptr;
     for (uint32_t t = 0; t < S; ++t) {
-        if (ctr && t >= 2) phase_wait(ctr, nbx * (t - 1));
+        if (ctr && t >= 3) phase_wait(ctr, nbx * (t - 2));
         const uint32_t s = first + t < S ? first + t : first + t - S;
         const uint64_t e0 = s_goff[s], e1 = s_goff[s + 1];
         const uint32_t cbase = s * width;
