@@ -308,7 +308,7 @@ constexpr uint32_t kPhaseMaxSlices = 256;
 constexpr uint32_t kPhaseColBits = 18;
 static_assert(kPhaseRows <= (1u << (32 - kPhaseColBits)), "local rows must fit the high bits");
 constexpr int kPhaseThreads = 1024;
-// XPHASE: 4 entries per thread per step, phases not paced (banded matrices:
+// XPHASE: 6 entries per thread per step, phases not paced (banded matrices:
 // a block meets few slices).  XPHASE_PACED: 20 entries per thread per step
 // (a 2 MiB slice of a scattered matrix is one step for most blocks) and paced
 // phases (scattered columns: every block meets every slice).
@@ -860,7 +860,7 @@ void spmv_launch_f64(spira_dcsr *A, const double *x, double *y, int variant)
         launch_xstream(A, x, y);
     } else if (variant == SPIRA_SPMV_XPHASE || variant == SPIRA_SPMV_XPHASE_PACED) {
         ensure_xphase(A);
-        if (variant == SPIRA_SPMV_XPHASE) launch_xphase<4, false>(A, x, y);
+        if (variant == SPIRA_SPMV_XPHASE) launch_xphase<6, false>(A, x, y);
         else launch_xphase<20, true>(A, x, y);
     } else {
         fail(SPIRA_ERR_INVALID_ARGUMENT, "spmv: unknown kernel variant");
