@@ -106,10 +106,9 @@ int spira_dcsr_download_f64(const spira_dcsr *A, uint64_t *offsets, uint32_t *co
  * First call on an unbuilt A = build_csr; later calls = merge_csr.  Result is
  * bit-identical to the reference's offsets/cols/vals.  *out_nnz (may be NULL)
  * receives the new nnz.  Column indices >= n_cols give SPIRA_ERR_OUT_OF_RANGE
- * (matrix.hpp:167-171).  A row with more than SPIRA_LOCK_MAX_ROW_INSERTS
- * staged inserts in one call gives SPIRA_ERR_UNSUPPORTED.
+ * (matrix.hpp:167-171).  Rows may hold any number of staged inserts
+ * (fewer than 2^32 in one call, as for the whole call).
  * flags: SPIRA_IN_DEVICE. */
-#define SPIRA_LOCK_MAX_ROW_INSERTS 8192u
 int spira_lock_f64(spira_dcsr *A, const uint64_t *stage_ptr, const uint32_t *stage_col,
                    const double *stage_val, const uint8_t *dirty, uint32_t flags,
                    uint64_t *out_nnz);

@@ -66,7 +66,6 @@ BATCH_XLDS_FLOATS = 36864
 BATCH_XLDS_TWO_PASS_COLS = 65536
 BATCH_MAX_PLOG2 = 5
 BATCH_SCHED_DTYPE = np.dtype([("plog2", "u1"), ("xlds", "u1"), ("reserved", "<u2"), ("item_nnz", "<u4")])
-LOCK_MAX_ROW_INSERTS = 8192
 TRANSPOSE_DROP_ZEROS = 1
 ADD_SUBTRACT = 1
 
@@ -234,10 +233,39 @@ def _producers_done(dev, *arrays):
             cur.synchronize()
 
 
+def check_device_vector(t, n: int, name: str, ordinal: int, dtype: str = "float64", what: str = "spmv"):
+    """The C entry points take bare device pointers, so a device vector is
+    checked here before any HIP call: a contiguous CUDA tensor of `dtype` on
+    GPU `ordinal` with exactly n elements (n = -1: at least one check of the
+    element count is left to the caller).  Raises InvalidArgument otherwise."""
+    import torch
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise InvalidArgument(f"{what}: {name} must be a {dtype} device tensor (got {type(t).__name__}"
+                              + ("" if not isinstance(t, torch.Tensor) else f" on {t.device}") + ")")
+    if t.dtype != getattr(torch, dtype):
+        raise InvalidArgument(f"{what}: {name} must be {dtype}, got {str(t.dtype).replace('torch.', '')}")
+    if not t.is_contiguous():
+        raise InvalidArgument(f"{what}: {name} must be contiguous")
+    if t.device.index != ordinal:
+        raise InvalidArgument(f"{what}: {name} is on cuda:{t.device.index}, the handle on cuda:{ordinal}")
+    if n >= 0 and t.numel() != n:
+        raise InvalidArgument(f"{what}: {name} holds {t.numel()} elements, needs {n}")
+
+
+def check_host_vector(a, n: int, name: str, dtype=np.float64, what: str = "spmv"):
+    """A host output vector the library writes through a bare pointer: a
+    contiguous numpy array of `dtype` with exactly n elements."""
+    if not (isinstance(a, np.ndarray) and a.dtype == dtype and a.flags.c_contiguous):
+        raise InvalidArgument(f"{what}: {name} must be a contiguous {np.dtype(dtype).name} array")
+    if a.size != n:
+        raise InvalidArgument(f"{what}: {name} holds {a.size} elements, needs {n}")
+
+
 class Device:
     """spira_dev: one GPU and one HIP stream (``stream`` = raw hipStream_t or None)."""
 
     def __init__(self, ordinal: int = 0, stream: int | None = None):
+        self.ordinal = int(ordinal)
         h = _vp()
         rc = lib().spira_dev_create(ctypes.byref(h), ordinal, stream or None)
         if rc != OK:
@@ -364,17 +392,36 @@ class DeviceCSR:
                                             _ptr(dirty), IN_DEVICE if device else 0, ctypes.byref(out)))
         return int(out.value)
 
-    def spmv(self, x, y=None, variant: int = SPMV_AUTO, device: bool = False, sync: bool = True):
-        """y = A*x.  Host numpy arrays, or device tensors with device=True."""
+    def _vectors(self, x, y, device, what="spmv"):
+        """Checked (x, y, x_len, y_len, flags) for an FP64 SpMV call.  Sizes
+        must match the matrix (the reference's checks, in its order); device
+        vectors must be float64 CUDA tensors on this handle's GPU, and the
+        library's stream waits for torch's current stream first."""
         if not device:
             x = np.ascontiguousarray(x, dtype=np.float64)
             if y is None:
                 y = np.empty(self.n_rows, np.float64)
-            xl, yl = x.size, y.size
-            flags = 0
-        else:
-            xl, yl = int(x.numel()), int(y.numel())
-            flags = X_DEVICE | Y_DEVICE | (0 if sync else NO_SYNC)
+            if x.size != self.n_cols:
+                raise InvalidArgument("The size of the input vector x does not match the number of columns of the matrix")
+            if not (isinstance(y, np.ndarray) and y.size == self.n_rows):
+                raise InvalidArgument("The size of the output vector y does not match the number of rows of the matrix")
+            check_host_vector(y, self.n_rows, "y", what=what)
+            return x, y, x.size, y.size, 0
+        import torch
+        if isinstance(x, torch.Tensor) and x.numel() != self.n_cols:
+            raise InvalidArgument("The size of the input vector x does not match the number of columns of the matrix")
+        if isinstance(y, torch.Tensor) and y.numel() != self.n_rows:
+            raise InvalidArgument("The size of the output vector y does not match the number of rows of the matrix")
+        check_device_vector(x, self.n_cols, "x", self.dev.ordinal, what=what)
+        check_device_vector(y, self.n_rows, "y", self.dev.ordinal, what=what)
+        _producers_done(self.dev, x, y)
+        return x, y, x.numel(), y.numel(), X_DEVICE | Y_DEVICE
+
+    def spmv(self, x, y=None, variant: int = SPMV_AUTO, device: bool = False, sync: bool = True):
+        """y = A*x.  Host numpy arrays, or device tensors with device=True."""
+        x, y, xl, yl, flags = self._vectors(x, y, device)
+        if device and not sync:
+            flags |= NO_SYNC
         self.dev.check(lib().spira_spmv_f64(self._h, _ptr(x), xl, _ptr(y), yl, int(variant), flags))
         return y
 
@@ -384,15 +431,9 @@ class DeviceCSR:
         compensated FP32 accumulation, FP64 recompute of rows whose
         cancellation ratio exceeds recompute_ratio).  Returns (y, n_recomputed)."""
         st = STORAGE[storage]
-        if not device:
-            x = np.ascontiguousarray(x, dtype=np.float64)
-            if y is None:
-                y = np.empty(self.n_rows, np.float64)
-            xl, yl = x.size, y.size
-            flags = 0
-        else:
-            xl, yl = int(x.numel()), int(y.numel())
-            flags = X_DEVICE | Y_DEVICE | (0 if sync else NO_SYNC)
+        x, y, xl, yl, flags = self._vectors(x, y, device, what="mixed spmv")
+        if device and not sync:
+            flags |= NO_SYNC
         nrec = ctypes.c_uint64()
         self.dev.check(lib().spira_spmv_mixed_f64(self._h, st, _ptr(x), xl, _ptr(y), yl, int(variant),
                                                   float(recompute_ratio), flags,
@@ -592,25 +633,27 @@ class Batch:
             return
         import torch
         for name, T, rows in (("X", X, self.n_cols), ("Y", Y, self.n_rows)):
-            if not (isinstance(T, torch.Tensor) and T.is_cuda and T.dtype == torch.float32 and T.is_contiguous()):
-                raise InvalidArgument(f"batch: {name} must be a contiguous float32 device tensor")
+            check_device_vector(T, -1, name, self.dev.ordinal, dtype="float32", what="batch")
             if T.numel() < rows * self.k:
                 raise InvalidArgument(f"batch: {name} holds {T.numel()} values, needs {rows} x k = {rows * self.k}")
 
     def spmm(self, X, Y, sync: bool = True):
         self._check_xy(X, Y)
+        _producers_done(self.dev, X, Y)
         self.dev.check(lib().spira_batch_spmm(self._h, _ptr(X), _ptr(Y), 0 if sync else NO_SYNC))
         return Y
 
     def profile(self, X, Y) -> np.ndarray:
         """One profiled spmm: per-matrix cost in CU-microseconds."""
         self._check_xy(X, Y)
+        _producers_done(self.dev, X, Y)
         out = np.zeros(max(1, len(self.mats)), np.float64)
         self.dev.check(lib().spira_batch_profile(self._h, _ptr(X), _ptr(Y), _ptr(out)))
         return out[:len(self.mats)]
 
     def capture(self, X, Y):
         self._check_xy(X, Y)
+        _producers_done(self.dev, X, Y)
         self.dev.check(lib().spira_batch_graph_capture(self._h, _ptr(X), _ptr(Y)))
 
     def replay(self, sync: bool = True):
@@ -1093,9 +1136,9 @@ def spmv(mat: Matrix, x, y, variant: int = SPMV_AUTO):
     """serial::algorithms::spmv / parallel::algorithms::spmv (FP64, device).
 
     x and y are numpy float64 arrays (host) or torch float64 CUDA tensors."""
-    is_dev = not isinstance(x, np.ndarray)
-    xl = int(x.numel()) if is_dev else int(np.asarray(x).size)
-    yl = int(y.numel()) if is_dev else int(np.asarray(y).size)
+    is_dev = hasattr(x, "numel") or hasattr(y, "numel")  # torch tensors: checked as device vectors below
+    xl = int(x.numel()) if hasattr(x, "numel") else int(np.asarray(x).size)
+    yl = int(y.numel()) if hasattr(y, "numel") else int(np.asarray(y).size)
     if xl != mat.n_cols():
         raise InvalidArgument("The size of the input vector x does not match the number of columns of the matrix")
     if yl != mat.n_rows():

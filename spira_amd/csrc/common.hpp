@@ -143,7 +143,7 @@ inline unsigned grid_for(uint64_t items, unsigned per_block, unsigned cap = 0x7f
 // Growable device scratch owned by a handle.  Slot i keeps its own buffer so
 // one operation can hold several at once.
 struct workspace {
-    static constexpr int kSlots = 20;
+    static constexpr int kSlots = 25;
     void *ptr[kSlots] = {};
     size_t cap[kSlots] = {};
     hipStream_t stream = nullptr;  // work that may still read a slot runs here
@@ -252,6 +252,11 @@ enum ws_slot : int {
     WS_X32 = 17,     // mixed spmv: x rounded to FP32
     WS_SORT = 18,    // device staging: radix sort temporary storage
     WS_BLK = 19,     // lock: row tables of the element-parallel passes
+    WS_LONG_ROWS = 20,  // lock: rows with more than 8192 staged inserts
+    WS_LONG_OFF = 21,   // lock: their inserts' offsets, back to back
+    WS_LONG_KEY = 22,   // lock: (list index, column) keys, unsorted then sorted
+    WS_LONG_IDX = 23,   // lock: insertion index of each key, unsorted then sorted
+    WS_LONG_P = 24,     // lock: prefix of the kept keys
 };
 
 }  // namespace spira_amd

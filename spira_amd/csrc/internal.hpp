@@ -27,7 +27,7 @@ bool mixed_variant_supported(int variant);
 uint64_t lock_device_f64(spira_dcsr *A, const uint64_t *d_stage_ptr, const uint32_t *d_stage_col,
                          const double *d_stage_val, uint64_t m, const uint8_t *d_dirty);
 
-// validate.hip: load_csr checks on device arrays; throws status_error.
+// lock.hip: load_csr checks on device arrays; throws status_error.
 void validate_csr_device(spira_dcsr *A, uint64_t nnz, const uint64_t *d_off, const uint32_t *d_col,
                          const double *d_val, bool *has_zeros);
 

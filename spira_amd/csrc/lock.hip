@@ -4,11 +4,13 @@
 //
 // Pipeline (all on the handle's stream):
 //   1. check      stage_ptr well formed, columns < n_cols, rows binned by the
-//                 number of staged inserts (<= 32, <= 8192, larger = error).
+//                 number of staged inserts (<= 32, <= 8192, longer).
 //   2. dedup      per row: last write wins, ascending columns, zeros kept.
 //                 <= 32 inserts: one lane per row, O(k^2) ranking in registers
 //                 and L1.  <= 8192: one block per row, bitonic sort of
-//                 (column, insertion index) keys in LDS.
+//                 (column, insertion index) keys in LDS.  Longer rows (any
+//                 length): one stable radix sort in HBM of all of them by
+//                 (row in their list, column), run only when such rows exist.
 //   3. flags      per deduped entry: nz = "value is not zero and the row is
 //                 dirty", match = "column already present in the old row"
 //                 (binary search); packed (match << 32 | nz) and scanned.
@@ -21,9 +23,15 @@
 //                 a re-lock does both in one pass over "row r's old entries,
 //                 then its staged slots" so each row is written by one block.
 // The host waits once, at the end, for the error bits and the new nnz; a
-// failed check makes every later kernel return at once.  The new arrays hold
-// nnz_old + m entries (merge_csr's total_ub); the matrix reports nnz.
+// failed check makes every later kernel return at once.  Rows longer than the
+// LDS sort holds set a retry bit that acts like an error in that first pass;
+// the host, which then knows how many such rows there are and how many
+// inserts they hold, sorts them and runs steps 2-5 again (so matrices without
+// such rows pay nothing for them).  The new arrays hold nnz_old + m entries
+// (merge_csr's total_ub); the matrix reports nnz.
 #include "internal.hpp"
+
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <cstring>
@@ -34,7 +42,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr uint32_t kSmallSeg = 32;
-constexpr uint32_t kMaxSeg = SPIRA_LOCK_MAX_ROW_INSERTS;  // 8192, LDS bitonic limit
+constexpr uint32_t kMaxSeg = 8192;                          // LDS bitonic limit of k_dedup_big
 constexpr int kItems = 4;                                   // elements per thread in scatter passes
 constexpr uint32_t kPerBlock = kThreads * kItems;           // elements per block of the staged-slot passes
 constexpr int kItemsMerge = 8;                              // positions per thread in the re-lock scatter
@@ -43,8 +51,19 @@ constexpr uint32_t kPerBlockMerge = kThreads * kItemsMerge;
 enum err_bits : uint32_t {
     ERR_OUT_OF_RANGE = 1u,
     ERR_BAD_PTR = 2u,
-    ERR_TOO_LONG = 4u,
-    ERR_INTERNAL = 8u,  // a scatter slot fell outside its row (never expected)
+    ERR_LONG_ROWS = 4u,  // not an error: rows with > kMaxSeg inserts are listed, the host sorts them and reruns
+    ERR_INTERNAL = 8u,   // a scatter slot fell outside its row (never expected)
+};
+
+// Control words of one lock, read back by the host in one copy at the end.
+enum ctl_word : int {
+    CTL_ERR = 0,       // err_bits
+    CTL_NBIG = 1,      // rows listed for k_dedup_big
+    CTL_NNZ_LO = 2,    // new nnz (k_finish)
+    CTL_NNZ_HI = 3,
+    CTL_NLONG = 4,     // rows with more than kMaxSeg staged inserts
+    CTL_LONG_INS = 5,  // staged inserts in those rows (< m < 2^32)
+    CTL_WORDS = 8,
 };
 
 __device__ __forceinline__ bool row_dirty(const uint8_t *dirty, const uint64_t *sptr, uint64_t r)
@@ -69,9 +88,10 @@ __device__ __forceinline__ void mark_block_rows(uint64_t r, uint64_t e0, uint64_
 // ---------------------------------------------------------------------------
 // 1. checks and binning
 // ---------------------------------------------------------------------------
-__global__ void k_check_ptr(const uint64_t *__restrict__ sptr, uint64_t n, uint64_t m, uint32_t *err,
-                            uint32_t *big_rows, uint32_t *n_big)
+__global__ void k_check_ptr(const uint64_t *__restrict__ sptr, uint64_t n, uint64_t m, uint32_t *ctl,
+                            uint32_t *big_rows, uint32_t *long_rows)
 {
+    uint32_t *err = ctl + CTL_ERR;
     const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r > n) return;
     if (r == 0 && sptr[0] != 0) atomicOr(err, ERR_BAD_PTR);
@@ -85,8 +105,13 @@ __global__ void k_check_ptr(const uint64_t *__restrict__ sptr, uint64_t n, uint6
         return;
     }
     const uint64_t k = b - a;
-    if (k > kMaxSeg) atomicOr(err, ERR_TOO_LONG);
-    else if (k > kSmallSeg) big_rows[atomicAdd(n_big, 1u)] = (uint32_t)r;
+    if (k > kMaxSeg) {
+        atomicOr(err, ERR_LONG_ROWS);
+        long_rows[atomicAdd(ctl + CTL_NLONG, 1u)] = (uint32_t)r;
+        atomicAdd(ctl + CTL_LONG_INS, (uint32_t)k);  // sums to at most m < 2^32
+    } else if (k > kSmallSeg) {
+        big_rows[atomicAdd(ctl + CTL_NBIG, 1u)] = (uint32_t)r;
+    }
 }
 
 __global__ void k_check_cols(const uint32_t *__restrict__ scol, uint64_t m, uint64_t n_cols, uint32_t *err)
@@ -111,7 +136,7 @@ __global__ __launch_bounds__(kThreads) void k_dedup_small(uint64_t n, const uint
                                                            const uint64_t *__restrict__ old_off,
                                                            uint64_t *__restrict__ mtab, const uint32_t *__restrict__ ctl)
 {
-    if (ctl[0]) return;  // the checks failed: nothing below may trust the staged arrays
+    if (ctl[CTL_ERR]) return;  // the checks failed: nothing below may trust the staged arrays
     const uint64_t r = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
     if (r >= n) return;
     const uint64_t s0 = sptr[r];
@@ -177,8 +202,8 @@ __global__ __launch_bounds__(kThreads) void k_dedup_big(const uint32_t *__restri
                                                          uint32_t *__restrict__ dcnt)
 {
     extern __shared__ uint64_t s_key[];  // kMaxSeg entries
-    if (ctl[0]) return;
-    const uint32_t count = ctl[1];
+    if (ctl[CTL_ERR]) return;
+    const uint32_t count = ctl[CTL_NBIG];
     for (uint32_t li = blockIdx.x; li < count; li += gridDim.x) {
         const uint32_t r = rows[li];
         const uint64_t s0 = sptr[r];
@@ -223,6 +248,70 @@ __global__ __launch_bounds__(kThreads) void k_dedup_big(const uint32_t *__restri
         if (threadIdx.x == 0) dcnt[r] = carry;
         __syncthreads();
     }
+}
+
+// Rows with more than kMaxSeg staged inserts (any length).  Their number h
+// and their insert count H are known on the host when these run.  Every such
+// row's inserts are laid out back to back (list order), keyed
+// (list index << cb | column) with the insertion index inside the row as the
+// payload; one stable LSD radix sort orders them by row, then column, with
+// equal columns in insertion order (the order the reference's reverse +
+// stable_sort + keep-first walks, aos_array_buffer.hpp:81-96), so the last
+// key of each run is the last write.
+__global__ void k_long_lens(const uint32_t *__restrict__ rows, uint32_t h, const uint64_t *__restrict__ sptr,
+                            uint64_t *__restrict__ lens)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < h) lens[i] = sptr[rows[i] + 1] - sptr[rows[i]];
+}
+
+__global__ __launch_bounds__(kThreads) void k_long_keys(uint64_t H, uint32_t h, const uint64_t *__restrict__ hoff,
+                                                         const uint32_t *__restrict__ rows,
+                                                         const uint64_t *__restrict__ sptr,
+                                                         const uint32_t *__restrict__ scol, uint32_t cb,
+                                                         uint64_t *__restrict__ key, uint32_t *__restrict__ idx)
+{
+    const uint64_t p = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (p >= H) return;
+    const uint64_t i = row_of(hoff, 0, h, p);  // every listed row is non-empty
+    const uint64_t j = p - hoff[i];
+    key[p] = (i << cb) | scol[sptr[rows[i]] + j];
+    idx[p] = (uint32_t)j;
+}
+
+__global__ __launch_bounds__(kThreads) void k_long_keep(uint64_t H, const uint64_t *__restrict__ key,
+                                                         uint64_t *__restrict__ keep)
+{
+    const uint64_t p = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (p < H) keep[p] = (p + 1 == H || key[p + 1] != key[p]) ? 1u : 0u;
+}
+
+// Kept entry p goes to its row's staged base + (kept entries of the row
+// before it); P is the exclusive prefix of the keep flags (H + 1 words).
+__global__ __launch_bounds__(kThreads) void k_long_scatter(uint64_t H, const uint64_t *__restrict__ key,
+                                                            const uint32_t *__restrict__ idx,
+                                                            const uint64_t *__restrict__ P,
+                                                            const uint64_t *__restrict__ hoff,
+                                                            const uint32_t *__restrict__ rows,
+                                                            const uint64_t *__restrict__ sptr,
+                                                            const double *__restrict__ sval, uint32_t cb,
+                                                            uint32_t *__restrict__ dcol, double *__restrict__ dval)
+{
+    const uint64_t p = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (p >= H || P[p + 1] == P[p]) return;
+    const uint64_t k = key[p];
+    const uint64_t i = k >> cb;
+    const uint64_t s0 = sptr[rows[i]];
+    const uint64_t rank = P[p] - P[hoff[i]];
+    dcol[s0 + rank] = (uint32_t)(k & ((1ull << cb) - 1));
+    dval[s0 + rank] = sval[s0 + idx[p]];
+}
+
+__global__ void k_long_counts(uint32_t h, const uint32_t *__restrict__ rows, const uint64_t *__restrict__ hoff,
+                              const uint64_t *__restrict__ P, uint32_t *__restrict__ dcnt)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < h) dcnt[rows[i]] = (uint32_t)(P[hoff[i + 1]] - P[hoff[i]]);
 }
 
 // ---------------------------------------------------------------------------
@@ -304,7 +393,7 @@ __global__ __launch_bounds__(kThreads) void k_ded_flags(uint64_t n, uint64_t m, 
 {
     __shared__ uint64_t s_rows[kRowCache + 1];
     const uint64_t first = (uint64_t)blockIdx.x * kPerBlock;
-    if (first >= m || ctl[0]) return;
+    if (first >= m || ctl[CTL_ERR]) return;
     // slot data first: these loads are in flight while the row cache fills
     uint32_t cs[kItems];
     double vs[kItems];
@@ -356,7 +445,7 @@ __global__ __launch_bounds__(kThreads) void k_old_zero_flags(uint64_t n, uint64_
 {
     __shared__ uint64_t s_rows[kRowCache + 1];
     const uint64_t first = (uint64_t)blockIdx.x * kThreads * kItems;
-    if (first >= nnz_old || ctl[0]) return;
+    if (first >= nnz_old || ctl[CTL_ERR]) return;
     const uint64_t last = min(first + kThreads * kItems, nnz_old) - 1;
     const row_cache rc = make_row_cache(block_rows(old_off, n, first, last), old_off, s_rows);
 #pragma unroll
@@ -385,7 +474,7 @@ __global__ void k_row_counts(uint64_t n, const uint64_t *__restrict__ sptr, cons
                              const uint32_t *__restrict__ ctl)
 {
     const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n || ctl[0]) return;
+    if (r >= n || ctl[CTL_ERR]) return;
     const uint64_t pa = P[sptr[r]], pb = P[sptr[r + 1]];
     const uint64_t nz = (pb & 0xffffffffull) - (pa & 0xffffffffull);
     const uint64_t mt = (pb >> 32) - (pa >> 32);
@@ -659,8 +748,8 @@ __global__ __launch_bounds__(kThreads) void k_scatter_ded(uint64_t m, const uint
 // The new nnz next to the error bits, so one copy brings both back.
 __global__ void k_finish(const uint64_t *__restrict__ nnz_new, uint32_t *__restrict__ ctl)
 {
-    ctl[2] = (uint32_t)*nnz_new;
-    ctl[3] = (uint32_t)(*nnz_new >> 32);
+    ctl[CTL_NNZ_LO] = (uint32_t)*nnz_new;
+    ctl[CTL_NNZ_HI] = (uint32_t)(*nnz_new >> 32);
 }
 
 // ---------------------------------------------------------------------------
@@ -762,6 +851,50 @@ void validate_csr_device(spira_dcsr *A, uint64_t nnz, const uint64_t *d_off, con
     *has_zeros = zflag != 0;
 }
 
+namespace {
+
+uint32_t bits_for(uint64_t v)  // bits needed to hold every value in [0, v]
+{
+    uint32_t b = 0;
+    while (b < 64 && (v >> b) != 0) ++b;
+    return b;
+}
+
+// Dedup of the h listed rows with more than kMaxSeg staged inserts (H in all):
+// deduped (column, value) pairs at each row's staged base and dcnt[row].
+void dedup_long_rows(spira_dev *dev, uint64_t n_cols, const uint32_t *rows, uint32_t h, uint64_t H,
+                     const uint64_t *sptr, const uint32_t *scol, const double *sval, uint32_t *dcol, double *dval,
+                     uint32_t *dcnt)
+{
+    hipStream_t st = dev->stream;
+    const uint32_t cb = std::max<uint32_t>(1, bits_for(n_cols ? n_cols - 1 : 0));
+    const uint32_t kb = cb + bits_for(h - 1);  // key bits: list index, then column
+    if (kb > 64) fail(SPIRA_ERR_UNSUPPORTED, "lock: too many long rows for the long-row sort");
+    uint64_t *hoff = (uint64_t *)dev->ws.get(WS_LONG_OFF, (h + 1) * sizeof(uint64_t));
+    uint64_t *key = (uint64_t *)dev->ws.get(WS_LONG_KEY, 2 * H * sizeof(uint64_t));
+    uint32_t *idx = (uint32_t *)dev->ws.get(WS_LONG_IDX, 2 * H * sizeof(uint32_t));
+    uint64_t *P = (uint64_t *)dev->ws.get(WS_LONG_P, (H + 1) * sizeof(uint64_t));
+    k_long_lens<<<grid_for(h, kThreads), kThreads, 0, st>>>(rows, h, sptr, hoff);
+    SPIRA_HIP(hipGetLastError());
+    scan_excl_u64(dev, hoff, hoff, h);
+    k_long_keys<<<grid_for(H, kThreads), kThreads, 0, st>>>(H, h, hoff, rows, sptr, scol, cb, key, idx);
+    SPIRA_HIP(hipGetLastError());
+    size_t tb = 0;
+    SPIRA_HIP(rocprim::radix_sort_pairs(nullptr, tb, key, key + H, idx, idx + H, (unsigned)H, 0, kb, st));
+    void *tmp = dev->ws.get(WS_SORT, tb ? tb : 8);
+    SPIRA_HIP(rocprim::radix_sort_pairs(tmp, tb, key, key + H, idx, idx + H, (unsigned)H, 0, kb, st));
+    k_long_keep<<<grid_for(H, kThreads), kThreads, 0, st>>>(H, key + H, P);
+    SPIRA_HIP(hipGetLastError());
+    scan_excl_u64(dev, P, P, H);
+    k_long_scatter<<<grid_for(H, kThreads), kThreads, 0, st>>>(H, key + H, idx + H, P, hoff, rows, sptr, sval, cb,
+                                                               dcol, dval);
+    SPIRA_HIP(hipGetLastError());
+    k_long_counts<<<grid_for(h, kThreads), kThreads, 0, st>>>(h, rows, hoff, P, dcnt);
+    SPIRA_HIP(hipGetLastError());
+}
+
+}  // namespace
+
 uint64_t lock_device_f64(spira_dcsr *A, const uint64_t *sptr, const uint32_t *scol, const double *sval,
                          uint64_t m, const uint8_t *dirty)
 {
@@ -772,21 +905,21 @@ uint64_t lock_device_f64(spira_dcsr *A, const uint64_t *sptr, const uint32_t *sc
     if (m >= (1ull << 32)) fail(SPIRA_ERR_UNSUPPORTED, "lock: more than 2^32-1 staged inserts in one call");
 
     // ---- 1. checks + binning -------------------------------------------------
-    // ctl = [error bits, rows with many inserts, new nnz (u64)].  There is no
-    // host round trip here: every later kernel returns at once when an error
-    // bit is set, and the host reads the bits with the new nnz at the end, so
-    // a failed check still leaves the matrix untouched.
-    uint32_t *ctl = (uint32_t *)dev->ws.get(WS_MISC, 4 * sizeof(uint32_t));
+    // ctl (ctl_word): error bits, listed rows, new nnz.  There is no host round
+    // trip here: every later kernel returns at once when a bit is set, and the
+    // host reads the words with the new nnz at the end, so a failed check
+    // still leaves the matrix untouched.
+    uint32_t *ctl = (uint32_t *)dev->ws.get(WS_MISC, CTL_WORDS * sizeof(uint32_t));
     uint32_t *big = (uint32_t *)dev->ws.get(WS_BIG, (n + 1) * sizeof(uint32_t));
-    SPIRA_HIP(hipMemsetAsync(ctl, 0, 4 * sizeof(uint32_t), st));
-    k_check_ptr<<<grid_for(n + 1, kThreads), kThreads, 0, st>>>(sptr, n, m, ctl, big, ctl + 1);
+    uint32_t *long_rows = (uint32_t *)dev->ws.get(WS_LONG_ROWS, (m / (kMaxSeg + 1) + 1) * sizeof(uint32_t));
+    SPIRA_HIP(hipMemsetAsync(ctl, 0, CTL_WORDS * sizeof(uint32_t), st));
+    k_check_ptr<<<grid_for(n + 1, kThreads), kThreads, 0, st>>>(sptr, n, m, ctl, big, long_rows);
     SPIRA_HIP(hipGetLastError());
     if (m > 0) {
         k_check_cols<<<grid_for(m, kThreads), kThreads, 0, st>>>(scol, m, A->n_cols, ctl);
         SPIRA_HIP(hipGetLastError());
     }
 
-    // ---- 2. dedup ----------------------------------------------------------
     uint32_t *dcol = (uint32_t *)dev->ws.get(WS_DED_COL, m * sizeof(uint32_t));
     double *dval = (double *)dev->ws.get(WS_DED_VAL, m * sizeof(double));
     uint32_t *dcnt = (uint32_t *)dev->ws.get(WS_ROWS, (n + 1) * sizeof(uint32_t) + 8);
@@ -797,42 +930,15 @@ uint64_t lock_device_f64(spira_dcsr *A, const uint64_t *sptr, const uint32_t *sc
     const uint64_t nb_m = total_merge ? (total_merge + kPerBlockMerge - 1) / kPerBlockMerge + 1 : 0;
     uint64_t *stab = (uint64_t *)dev->ws.get(WS_BLK, (nb_s + nb_m) * sizeof(uint64_t));
     uint64_t *mtab = total_merge ? stab + nb_s : nullptr;
-    if (n > 0) {
-        k_dedup_small<<<grid_for(n, kThreads), kThreads, 0, st>>>(n, sptr, scol, sval, dcol, dval, dcnt, stab,
-                                                                   A->off, mtab, ctl);
-        SPIRA_HIP(hipGetLastError());
-    }
-    if (n > 0) {  // the number of listed rows is on the device: blocks stride over it
-        const unsigned g = (unsigned)std::min<uint64_t>(n, 2048u);
-        k_dedup_big<<<g, kThreads, kMaxSeg * sizeof(uint64_t), st>>>(big, ctl, sptr, scol, sval, dcol, dval, dcnt);
-        SPIRA_HIP(hipGetLastError());
-    }
-
-    // ---- 3. flags + scan ----------------------------------------------------
     uint64_t *P = (uint64_t *)dev->ws.get(WS_FLAGS, (m + 1) * sizeof(uint64_t));
     uint32_t *qpos = merge ? (uint32_t *)dev->ws.get(WS_QPOS, m * sizeof(uint32_t)) : nullptr;
-    if (m > 0) {
-        k_ded_flags<<<grid_for(m, kPerBlock), kThreads, 0, st>>>(n, m, sptr, dcnt, dcol, dval, dirty, merge ? 1 : 0,
-                                                                 A->off, A->col, P, qpos, stab, ctl);
-        SPIRA_HIP(hipGetLastError());
-    }
-    scan_excl_u64(dev, P, P, m);
+    const bool zeros = merge && A->has_zeros && A->nnz > 0;
+    uint64_t *PZ = zeros ? (uint64_t *)dev->ws.get(WS_PZ, (A->nnz + 1) * sizeof(uint64_t)) : nullptr;
+    uint64_t *cnt = (uint64_t *)dev->ws.get(WS_CNT, (n + 1) * sizeof(uint64_t));
 
-    // Old explicit zeros (only possible after load_csr with zeros).
-    uint64_t *PZ = nullptr;
-    if (merge && A->has_zeros && A->nnz > 0) {
-        PZ = (uint64_t *)dev->ws.get(WS_PZ, (A->nnz + 1) * sizeof(uint64_t));
-        k_old_zero_flags<<<grid_for(A->nnz, kThreads * kItems), kThreads, 0, st>>>(n, A->nnz, A->off, A->col, A->val,
-                                                                                   sptr, dcnt, dcol, dirty, PZ, ctl);
-        SPIRA_HIP(hipGetLastError());
-        scan_excl_u64(dev, PZ, PZ, A->nnz);
-    }
-
-    // ---- 4. counts + offsets ----------------------------------------------
     // The new arrays are sized by the upper bound nnz_old + m (as merge_csr's
     // total_ub, csr_build.hpp:139-145) so no host round trip is needed before
     // the scatter; the exact nnz comes back with the final status word.
-    uint64_t *cnt = (uint64_t *)dev->ws.get(WS_CNT, (n + 1) * sizeof(uint64_t));
     const uint64_t ub = (merge ? A->nnz : 0) + m;
     const size_t ob = (n + 1) * sizeof(uint64_t), cb = (ub ? ub : 1) * sizeof(uint32_t),
                  vb = (ub ? ub : 1) * sizeof(double);
@@ -842,38 +948,75 @@ uint64_t lock_device_f64(spira_dcsr *A, const uint64_t *sptr, const uint32_t *sc
     try {
         new_col = (uint32_t *)dev_alloc(dev, cb);
         new_val = (double *)dev_alloc(dev, vb);
-        if (n > 0) {
-            k_row_counts<<<grid_for(n, kThreads), kThreads, 0, st>>>(n, sptr, P, dirty, merge ? 1 : 0, A->off, PZ,
-                                                                     cnt, ctl);
-            SPIRA_HIP(hipGetLastError());
-        }
-        scan_excl_u64(dev, cnt, new_off, n);
 
-        // ---- 5. scatter ------------------------------------------------------
-        if (total_merge > 0) {
-            k_scatter_merge<<<grid_for(total_merge, kPerBlockMerge), kThreads, 0, st>>>(
-                total_merge, A->off, A->col, A->val, sptr, dcnt, dcol, dval, P, qpos, PZ, dirty, new_off, new_col,
-                new_val, ctl, mtab);
+        // Steps 2-5; every kernel returns at once while an error (or the
+        // long-row retry) bit is set.
+        uint32_t hctl[CTL_WORDS];
+        auto steps = [&]() {
+            // ---- 2. dedup -------------------------------------------------
+            if (n > 0) {
+                k_dedup_small<<<grid_for(n, kThreads), kThreads, 0, st>>>(n, sptr, scol, sval, dcol, dval, dcnt,
+                                                                           stab, A->off, mtab, ctl);
+                SPIRA_HIP(hipGetLastError());
+                // the number of listed rows is on the device: blocks stride over it
+                const unsigned g = (unsigned)std::min<uint64_t>(n, 2048u);
+                k_dedup_big<<<g, kThreads, kMaxSeg * sizeof(uint64_t), st>>>(big, ctl, sptr, scol, sval, dcol,
+                                                                              dval, dcnt);
+                SPIRA_HIP(hipGetLastError());
+            }
+            // ---- 3. flags + scan ------------------------------------------
+            if (m > 0) {
+                k_ded_flags<<<grid_for(m, kPerBlock), kThreads, 0, st>>>(n, m, sptr, dcnt, dcol, dval, dirty,
+                                                                         merge ? 1 : 0, A->off, A->col, P, qpos,
+                                                                         stab, ctl);
+                SPIRA_HIP(hipGetLastError());
+            }
+            scan_excl_u64(dev, P, P, m);
+            // Old explicit zeros (only possible after load_csr with zeros).
+            if (zeros) {
+                k_old_zero_flags<<<grid_for(A->nnz, kThreads * kItems), kThreads, 0, st>>>(
+                    n, A->nnz, A->off, A->col, A->val, sptr, dcnt, dcol, dirty, PZ, ctl);
+                SPIRA_HIP(hipGetLastError());
+                scan_excl_u64(dev, PZ, PZ, A->nnz);
+            }
+            // ---- 4. counts + offsets --------------------------------------
+            if (n > 0) {
+                k_row_counts<<<grid_for(n, kThreads), kThreads, 0, st>>>(n, sptr, P, dirty, merge ? 1 : 0, A->off,
+                                                                         PZ, cnt, ctl);
+                SPIRA_HIP(hipGetLastError());
+            }
+            scan_excl_u64(dev, cnt, new_off, n);
+            // ---- 5. scatter ------------------------------------------------
+            if (total_merge > 0) {
+                k_scatter_merge<<<grid_for(total_merge, kPerBlockMerge), kThreads, 0, st>>>(
+                    total_merge, A->off, A->col, A->val, sptr, dcnt, dcol, dval, P, qpos, PZ, dirty, new_off,
+                    new_col, new_val, ctl, mtab);
+                SPIRA_HIP(hipGetLastError());
+            } else if (!merge && m > 0) {
+                k_scatter_ded<<<grid_for(m, kPerBlock), kThreads, 0, st>>>(m, sptr, dcol, dval, P, new_off,
+                                                                           new_col, new_val, ctl, stab);
+                SPIRA_HIP(hipGetLastError());
+            }
+            k_finish<<<1, 1, 0, st>>>(new_off + n, ctl);
             SPIRA_HIP(hipGetLastError());
-        } else if (!merge && m > 0) {
-            k_scatter_ded<<<grid_for(m, kPerBlock), kThreads, 0, st>>>(m, sptr, dcol, dval, P, new_off, new_col,
-                                                                       new_val, ctl, stab);
-            SPIRA_HIP(hipGetLastError());
+            SPIRA_HIP(hipMemcpyAsync(hctl, ctl, sizeof(hctl), hipMemcpyDeviceToHost, st));
+            dev_sync(dev);
+        };
+        steps();
+        if (hctl[CTL_ERR] == ERR_LONG_ROWS) {
+            // Rows longer than the LDS sort: sort them, clear the bit, rerun.
+            dedup_long_rows(dev, A->n_cols, long_rows, hctl[CTL_NLONG], hctl[CTL_LONG_INS], sptr, scol, sval, dcol,
+                            dval, dcnt);
+            SPIRA_HIP(hipMemsetAsync(ctl + CTL_ERR, 0, sizeof(uint32_t), st));
+            steps();
         }
-        k_finish<<<1, 1, 0, st>>>(new_off + n, ctl);
-        SPIRA_HIP(hipGetLastError());
-        uint32_t hctl[4];
-        SPIRA_HIP(hipMemcpyAsync(hctl, ctl, sizeof(hctl), hipMemcpyDeviceToHost, st));
-        dev_sync(dev);
-        if (hctl[0] & ERR_BAD_PTR)
+        if (hctl[CTL_ERR] & ERR_BAD_PTR)
             fail(SPIRA_ERR_INVALID_ARGUMENT, "lock: stage_ptr must start at 0, be non-decreasing and end at m");
-        if (hctl[0] & ERR_OUT_OF_RANGE) fail(SPIRA_ERR_OUT_OF_RANGE, "spira::matrix: col_index out of range");
-        if (hctl[0] & ERR_TOO_LONG)
-            fail(SPIRA_ERR_UNSUPPORTED, "lock: a row has more staged inserts than SPIRA_LOCK_MAX_ROW_INSERTS");
-        if (hctl[0] & ERR_INTERNAL)
+        if (hctl[CTL_ERR] & ERR_OUT_OF_RANGE) fail(SPIRA_ERR_OUT_OF_RANGE, "spira::matrix: col_index out of range");
+        if (hctl[CTL_ERR] & ERR_INTERNAL)
             fail(SPIRA_ERR_DEVICE, "lock: internal error, a merged entry fell outside its row");
         uint64_t nnz_new;
-        std::memcpy(&nnz_new, hctl + 2, sizeof(nnz_new));
+        std::memcpy(&nnz_new, hctl + CTL_NNZ_LO, sizeof(nnz_new));
         // install: the old arrays go back to the cache (stream order keeps
         // them intact for anything still queued)
         free_dcsr_arrays(A);

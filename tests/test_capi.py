@@ -59,7 +59,6 @@ def test_constants_match_python_mirror():
     assert d["SPIRA_X_DEVICE"] == spira_amd.X_DEVICE and d["SPIRA_Y_DEVICE"] == spira_amd.Y_DEVICE
     assert d["SPIRA_IN_DEVICE"] == spira_amd.IN_DEVICE and d["SPIRA_NO_SYNC"] == spira_amd.NO_SYNC
     assert d["SPIRA_SPMV_N_VARIANTS"] == spira_amd.N_VARIANTS
-    assert d["SPIRA_LOCK_MAX_ROW_INSERTS"] == spira_amd.LOCK_MAX_ROW_INSERTS
     for name, v in spira_amd.VARIANTS.items():
         key = "SPIRA_SPMV_" + name.upper().replace("STREAM1K", "STREAM1K")
         assert d[key] == v, key

@@ -123,6 +123,106 @@ def test_lock_rows_with_many_staged_inserts(dev, k):
     assert csr_equal(A.download(), oracle.lock(ptr2, c2, v2, old=exp))
 
 
+def _long_row_stream(rng, n_rows, lens, col_pool):
+    """Interleaved inserts: row r gets lens[r] of them, columns from
+    [0, col_pool) (so a long row repeats columns: last write wins), 10 % zeros."""
+    rows = np.concatenate([np.full(k, r, np.int64) for r, k in lens.items()])
+    rng.shuffle(rows)
+    cols = rng.integers(0, col_pool, rows.size)
+    vals = rng.uniform(-1, 1, rows.size)
+    vals[rng.random(rows.size) < 0.1] = 0.0
+    return rows, cols, vals
+
+
+@pytest.mark.parametrize("k", [8193, 50000, 1 << 20])
+def test_lock_rows_longer_than_the_lds_sort(dev, k):
+    """Rows with more than 8192 staged inserts (the LDS sort's size) take the
+    HBM radix sort: first lock and re-lock (with a dirty mask), through both
+    entry points (per-row segments and the device COO stage), bit-exact
+    against the oracle, next to short and LDS-sorted rows."""
+    rng = np.random.default_rng(k)
+    n_rows, n_cols = 7, max(2 * k, 4096)
+    print(f"long rows: k={k}", flush=True)
+    rows, cols, vals = _long_row_stream(rng, n_rows, {1: k, 3: 7, 4: 9000, 5: k // 3 + 40}, k // 2 + 1)
+    ptr, c, v = stage(n_rows, rows, cols, vals)
+    exp = oracle.lock(ptr, c, v)
+    A = spira_amd.DeviceCSR(dev, n_rows, n_cols)
+    assert A.lock(ptr, c, v) == exp[1].size
+    assert csr_equal(A.download(), exp)
+    B = spira_amd.DeviceCSR(dev, n_rows, n_cols)
+    st = spira_amd.Stage(dev, n_rows, n_cols)
+    st.append(rows, cols, vals)
+    B.lock_staged(st)
+    assert csr_equal(B.download(), exp)
+    # re-lock: row 1 again with k inserts, row 0 new and long, row 6 short;
+    # row 2 marked dirty without inserts
+    rows2, cols2, vals2 = _long_row_stream(rng, n_rows, {0: k, 1: k, 6: 3}, n_cols)
+    ptr2, c2, v2 = stage(n_rows, rows2, cols2, vals2)
+    dirty = np.zeros(n_rows, np.uint8)
+    dirty[[0, 1, 2, 6]] = 1
+    exp2 = oracle.lock(ptr2, c2, v2, old=exp, dirty=dirty)
+    assert A.lock(ptr2, c2, v2, dirty=dirty) == exp2[1].size
+    assert csr_equal(A.download(), exp2)
+    st.append(rows2, cols2, vals2)
+    B.lock_staged(st, dirty=dirty)
+    assert csr_equal(B.download(), exp2)
+    st.close()
+
+
+def test_lock_many_long_rows_device_inputs(dev):
+    """Hundreds of rows past the LDS sort among short rows, device-resident
+    staged arrays, first lock and re-lock, bit-exact against the oracle."""
+    import torch
+    rng = np.random.default_rng(51)
+    n_rows, n_cols = 3000, 40000
+    lens = {int(r): int(rng.integers(8193, 12000)) for r in rng.choice(n_rows, 300, replace=False)}
+    for r in range(0, n_rows, 7):
+        lens.setdefault(r, int(rng.integers(0, 40)))
+    A = spira_amd.DeviceCSR(dev, n_rows, n_cols)
+    old = None
+    for cycle in range(2):
+        rows, cols, vals = _long_row_stream(rng, n_rows, lens, n_cols // 2)
+        ptr, c, v = stage(n_rows, rows, cols, vals)
+        exp = oracle.lock(ptr, c, v, old=old)
+        d = [torch.from_numpy(a).cuda() for a in (ptr.astype(np.int64), c.astype(np.int32), v)]
+        assert A.lock(*d, device=True) == exp[1].size
+        assert csr_equal(A.download(), exp), cycle
+        old = exp
+
+
+def test_powerlaw_8M_built_by_insert_and_lock(dev):
+    """The power-law matrix (8,388,608 rows, Zipf(1.8) lengths clipped to
+    50,000, so thousands of rows past the LDS sort) built through
+    Matrix.insert_many + lock(): every row's entries inserted in reverse
+    column order after a first write of 5 % of them with other values, so the
+    locked CSR must equal the generator's (last write wins, sorted columns)."""
+    import torch
+    n = 1 << 23
+    off_d, col_d, val_d = W.powerlaw_rows_device(n)
+    off = off_d.cpu().numpy().astype(np.uint64)
+    col = col_d.cpu().numpy().astype(np.uint32)
+    val = val_d.cpu().numpy()
+    del off_d, col_d, val_d
+    torch.cuda.empty_cache()
+    lens = np.diff(off.astype(np.int64))
+    print(f"power-law: nnz {col.size}, longest row {lens.max()}, rows > 8192: {(lens > 8192).sum()}", flush=True)
+    rows = np.repeat(np.arange(n, dtype=np.int64), lens)
+    rev = (np.repeat(off[1:].astype(np.int64), lens) - 1) - (np.arange(col.size) - np.repeat(off[:-1].astype(np.int64), lens))
+    first = np.nonzero(np.random.default_rng(5).random(col.size) < 0.05)[0]
+    m = spira_amd.Matrix(n, n, dev)
+    m.insert_many(rows[first], col[first], -val[first] - 1.0)  # overwritten below
+    m.insert_many(rows[rev], col[rev], val[rev])
+    del rows, rev, first
+    m.lock()
+    keep = val != 0.0  # an exact zero would be a delete marker
+    exp_off = off
+    if not keep.all():
+        exp_off = np.zeros(n + 1, np.uint64)
+        np.cumsum(np.bincount(np.repeat(np.arange(n), lens)[keep], minlength=n), out=exp_off[1:])
+    assert m.nnz() == int(keep.sum())
+    assert csr_equal(m.csr(), (exp_off, col[keep], val[keep]))
+
+
 def test_lock_after_load_csr_with_explicit_zeros(dev):
     rng = np.random.default_rng(5)
     n, nc = 500, 400
@@ -170,9 +270,11 @@ def test_lock_errors(dev):
     bad_ptr = np.array([0, 2, 1, 2], np.uint64)
     with pytest.raises(spira_amd.InvalidArgument):
         A.lock(bad_ptr, np.array([0, 1], np.uint32), np.array([1.0, 2.0]))
-    k = spira_amd.LOCK_MAX_ROW_INSERTS + 1
-    ptr, c, v = stage(3, np.zeros(k, np.int64), np.zeros(k, np.uint32), np.ones(k))
-    with pytest.raises(spira_amd.UnsupportedInput):
+    # a long row (past the LDS sort) together with a bad column: still the
+    # reference's exception, and the handle is untouched
+    k = 9000
+    ptr, c, v = stage(3, np.zeros(k + 1, np.int64), np.r_[np.zeros(k, np.uint32), 4], np.ones(k + 1))
+    with pytest.raises(spira_amd.OutOfRange):
         A.lock(ptr, c, v)
     assert not A.built  # failed locks leave the handle untouched
     m = spira_amd.Matrix(2, 3, dev)

@@ -202,3 +202,27 @@ def test_spmv_config1_parity(dev):
     y = np.empty(10000)
     spira_amd.spmv(m, x, y)
     _check(y, *exp, x)
+
+
+def test_device_vectors_from_torch_stream_without_sync(dev):
+    """x written by a kernel on torch's current stream, then handed to the
+    library (whose handle owns another stream) with no synchronize: the call
+    waits for the producer.  Wrong tensors are refused before any launch."""
+    import torch
+    n = 200_000
+    off, col, val = W.random_uniform_rows(n, n, 8, 32, seed=11)
+    A = _upload(dev, off, col, val, n)
+    torch.cuda.synchronize()
+    y = torch.empty(n, dtype=torch.float64, device="cuda")
+    for rep in range(3):
+        x = (torch.arange(n, device="cuda", dtype=torch.float64) * (0.37 + rep)).sin_()
+        x = x * x + torch.cos(x)  # several kernels on torch's stream
+        A.spmv(x, y, device=True)
+        _check(y.cpu().numpy(), off, col, val, x.cpu().numpy())
+    for bad_x, msg in ((x.float(), "float64"), (x.cpu(), "device tensor"), (x[::2], "columns")):
+        with pytest.raises(spira_amd.InvalidArgument, match=msg):
+            A.spmv(bad_x, y, device=True)
+    with pytest.raises(spira_amd.InvalidArgument, match="float64"):
+        A.spmv(x, y.float(), device=True)
+    with pytest.raises(spira_amd.InvalidArgument, match="float64"):
+        A.spmv_mixed(x.float(), y, device=True)
