@@ -147,6 +147,25 @@ int spira_lock_staged_f64(spira_dcsr *A, spira_stage *s, const uint8_t *dirty, u
 int spira_spmv_f64(spira_dcsr *A, const double *x, uint64_t x_len, double *y,
                    uint64_t y_len, int variant, uint32_t flags);
 
+/* ---- allocation-free SpMV ----------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  spira_spmv_f64 with the same checks, arguments and
+ * results, for device x and y only (flags must hold SPIRA_X_DEVICE |
+ * SPIRA_Y_DEVICE, and may add SPIRA_NO_SYNC), calling no allocator: A's SpMV
+ * plan for the variant must already exist (one spira_spmv_f64 call with the
+ * same variant after the last lock or upload builds it; SPIRA_ERR_LOGIC
+ * otherwise), and the scratch its kernels use -- the XPHASE_PACED pacing
+ * counters (512 bytes) or the XSLICE/XSTREAM partial sums (8 * n_rows
+ * doubles) -- comes from one fixed array of SPIRA_NOALLOC_SCRATCH_BYTES per
+ * GPU, held until the process ends.  A variant needing more scratch gives
+ * SPIRA_ERR_UNSUPPORTED (XSLICE/XSTREAM above 524,288 rows).  Every call on
+ * one GPU shares that array: calls that overlap on one GPU (from several
+ * handles or streams) race and must be serialised by the caller. */
+#define SPIRA_NOALLOC_SCRATCH_BYTES 33554432u /* 32 MiB */
+int spira_spmv_f64_noalloc(spira_dcsr *A, const double *x, uint64_t x_len, double *y, uint64_t y_len, int variant,
+                           uint32_t flags);
+
 /* Variant the AUTO path would use now for A (after plan build). */
 int spira_spmv_choose(spira_dcsr *A, int *variant);
 /* Pin the AUTO choice for A (autotuner / selector output). */

@@ -65,6 +65,7 @@ N_VARIANTS = 15
 BATCH_XLDS_FLOATS = 36864
 BATCH_XLDS_TWO_PASS_COLS = 65536
 BATCH_MAX_PLOG2 = 5
+NOALLOC_SCRATCH_BYTES = 32 << 20
 BATCH_SCHED_DTYPE = np.dtype([("plog2", "u1"), ("xlds", "u1"), ("reserved", "<u2"), ("item_nnz", "<u4")])
 TRANSPOSE_DROP_ZEROS = 1
 ADD_SUBTRACT = 1
@@ -167,6 +168,8 @@ def lib():
             "spira_lock_f64": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_uint32, _u64p]),
             "spira_spmv_f64": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64, _vp, ctypes.c_uint64, ctypes.c_int,
                                               ctypes.c_uint32]),
+            "spira_spmv_f64_noalloc": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64, _vp, ctypes.c_uint64,
+                                                      ctypes.c_int, ctypes.c_uint32]),
             "spira_spmv_choose": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
             "spira_spmv_set_variant": (ctypes.c_int, [_vp, ctypes.c_int]),
             "spira_partition_boundaries": (ctypes.c_int, [_vp, ctypes.c_uint64, _vp]),
@@ -423,6 +426,16 @@ class DeviceCSR:
         if device and not sync:
             flags |= NO_SYNC
         self.dev.check(lib().spira_spmv_f64(self._h, _ptr(x), xl, _ptr(y), yl, int(variant), flags))
+        return y
+
+    def spmv_noalloc(self, x, y, variant: int = SPMV_AUTO, sync: bool = True):
+        """y = A*x through spira_spmv_f64_noalloc: device tensors only, no
+        allocation (the plan for the variant must exist: one spmv() call with
+        the variant after the last lock builds it)."""
+        x, y, xl, yl, flags = self._vectors(x, y, True)
+        if not sync:
+            flags |= NO_SYNC
+        self.dev.check(lib().spira_spmv_f64_noalloc(self._h, _ptr(x), xl, _ptr(y), yl, int(variant), flags))
         return y
 
     def spmv_mixed(self, x, y=None, storage: str = "f32", variant: int = SPMV_AUTO, recompute_ratio: float = 0.0,

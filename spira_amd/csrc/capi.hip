@@ -4,6 +4,7 @@
 #include "internal.hpp"
 
 #include <cstring>
+#include <mutex>
 #include <new>
 
 using namespace spira_amd;
@@ -11,6 +12,31 @@ using namespace spira_amd;
 namespace {
 
 thread_local std::string g_last_error;  // for calls without a usable handle
+
+// Fixed scratch of spira_spmv_f64_noalloc: the XPHASE_PACED pacing counters
+// or the XSLICE/XSTREAM partial sums.  Every GPU holds its own copy until the
+// process ends.  On the host `scratch` only names the symbol: kernels get the
+// device address from hipGetSymbolAddress, queried once per device with that
+// device current (the first spira_dev_create on it) and kept below.
+constexpr size_t kNoAllocScratchBytes = SPIRA_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's copy of `scratch` (the caller has made `device` current).
+void *scratch_of_device(int device)
+{
+    if (device < 0 || device >= kMaxDevices) fail(SPIRA_ERR_UNSUPPORTED, "device ordinal beyond this build's table");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        SPIRA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[device] = p;
+    }
+    return g_scratch_ptr[device];
+}
 
 template <class F>
 int guarded(spira_dev *dev, F &&f)
@@ -125,6 +151,7 @@ int spira_dev_create(spira_dev **out, int device_ordinal, void *stream)
         dev->arch_major = prop.major;
         dev->arch_minor = prop.minor;
         dev->n_cu = prop.multiProcessorCount;
+        dev->noalloc_scratch = scratch_of_device(device_ordinal);
         if (stream) {
             dev->stream = (hipStream_t)stream;
         } else {
@@ -329,6 +356,45 @@ int spira_spmv_f64(spira_dcsr *A, const double *x, uint64_t x_len, double *y, ui
         if (!yd && y_len)
             SPIRA_HIP(hipMemcpyAsync(y, dy, y_len * sizeof(double), hipMemcpyDeviceToHost, dev->stream));
         if (!(flags & SPIRA_NO_SYNC) || !xd || !yd) dev_sync(dev);
+    });
+}
+
+int spira_spmv_f64_noalloc(spira_dcsr *A, const double *x, uint64_t x_len, double *y, uint64_t y_len, int variant,
+                           uint32_t flags)
+{
+    if (!A) return SPIRA_ERR_INVALID_ARGUMENT;
+    spira_dev *dev = A->dev;
+    return guarded(dev, [&] {
+        // the checks of spira_spmv_f64, in the same order
+        if (x_len != A->n_cols)
+            fail(SPIRA_ERR_INVALID_ARGUMENT,
+                 "The size of the input vector x does not match the number of columns of the matrix");
+        if (y_len != A->n_rows)
+            fail(SPIRA_ERR_INVALID_ARGUMENT,
+                 "The size of the output vector y does not match the number of rows of the matrix");
+        if (!A->built) fail(SPIRA_ERR_LOGIC, "spmv: matrix must be locked");
+        if (variant < SPIRA_SPMV_AUTO || variant >= SPIRA_SPMV_N_VARIANTS)
+            fail(SPIRA_ERR_INVALID_ARGUMENT, "spmv: unknown kernel variant");
+        if ((x_len && !x) || (y_len && !y)) fail(SPIRA_ERR_INVALID_ARGUMENT, "spmv: x or y is NULL");
+        if ((flags & (SPIRA_X_DEVICE | SPIRA_Y_DEVICE)) != (SPIRA_X_DEVICE | SPIRA_Y_DEVICE))
+            fail(SPIRA_ERR_INVALID_ARGUMENT,
+                 "spmv_noalloc: x and y must be device pointers (SPIRA_X_DEVICE | SPIRA_Y_DEVICE)");
+        if (A->n_rows == 0) return;
+        if (A->n_rows >= (1ull << 32) - 1) fail(SPIRA_ERR_UNSUPPORTED, "spmv: n_rows must be < 2^32 - 1");
+        if (!dev->noalloc_scratch) fail(SPIRA_ERR_DEVICE, "spmv_noalloc: no scratch address for this device");
+        const char *unprepared =
+            "spmv_noalloc: the matrix has no SpMV plan for this variant (one spira_spmv_f64 call with the "
+            "variant after the last lock or upload builds it)";
+        if (variant < 0 && !A->plan.stats_valid) fail(SPIRA_ERR_LOGIC, unprepared);
+        const int v = variant < 0 ? spmv_heuristic_variant(A) : variant;
+        const uint64_t need = spmv_scratch_bytes(A, v);
+        if (need > kNoAllocScratchBytes)
+            fail(SPIRA_ERR_UNSUPPORTED, "spmv_noalloc: the variant needs " + std::to_string(need) +
+                                            " bytes of scratch, more than SPIRA_NOALLOC_SCRATCH_BYTES (" +
+                                            std::to_string(kNoAllocScratchBytes) + ")");
+        if (!spmv_plan_ready(A, v)) fail(SPIRA_ERR_LOGIC, unprepared);
+        spmv_launch_planned(A, x, y, v, dev->noalloc_scratch);
+        if (!(flags & SPIRA_NO_SYNC)) dev_sync(dev);
     });
 }
 

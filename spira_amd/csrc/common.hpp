@@ -313,4 +313,5 @@ struct spira_dev {
     std::string last_error;
     int arch_major = 0, arch_minor = 0;
     int n_cu = 0;
+    void *noalloc_scratch = nullptr;  // this GPU's copy of the fixed scratch array (capi.hip)
 };

@@ -14,6 +14,13 @@ void spmv_build_plan(spira_dcsr *A);
 void spmv_free_plan(spira_dcsr *A);
 int spmv_heuristic_variant(const spira_dcsr *A);
 void spmv_launch_f64(spira_dcsr *A, const double *x, double *y, int variant);
+// The allocation-free launch: true when A's plan already holds what `variant`
+// (AUTO resolved to the pinned or heuristic choice) needs; the bytes of
+// scratch it takes (XPHASE_PACED counters, XSLICE/XSTREAM partial sums); and
+// the launch with that scratch passed in.  None of them allocates.
+bool spmv_plan_ready(const spira_dcsr *A, int variant);
+uint64_t spmv_scratch_bytes(const spira_dcsr *A, int variant);
+void spmv_launch_planned(spira_dcsr *A, const double *x, double *y, int variant, void *scratch);
 // Row-block list of the STREAM kernels (which: 0 = chunk 1024, 1 = 2048).
 void spmv_stream_blocks(spira_dcsr *A, int which, const uint32_t **blk, uint32_t *nblk);
 

@@ -753,40 +753,42 @@ void ensure_xphase(spira_dcsr *A)
 // slice; a banded matrix gives each block one or two slices, and pacing would
 // then make the blocks of an XCD take turns).  Only when the whole grid is
 // resident at once (one block per CU).
+// ctr: the 8 * kPhaseSyncStride pacing counters (zeroed here by a kernel).
 template <int U, bool PACE>
-void launch_xphase(spira_dcsr *A, const double *x, double *y)
+void launch_xphase(spira_dcsr *A, const double *x, double *y, uint32_t *ctr)
 {
     const spira_plan &P = A->plan;
     uint32_t *sync = nullptr;
     if (PACE && A->dev->n_cu > 0 && P.ph_nb <= (uint32_t)A->dev->n_cu) {
-        sync = P.ph_sync;
+        sync = ctr;
         k_phase_sync_zero<<<1, 8 * kPhaseSyncStride, 0, A->dev->stream>>>(sync);
     }
     k_spmv_xphase<U><<<P.ph_nb, kPhaseThreads, 0, A->dev->stream>>>(A->n_rows, P.ph_RB, P.ph_S, P.ph_goff, P.ph_width,
                                                                       P.ph_rc, P.ph_val, x, y, sync);
 }
 
-void launch_xstream(spira_dcsr *A, const double *x, double *y)
+// part: kSlices * n_rows partial sums (every one written before it is read).
+void launch_xstream(spira_dcsr *A, const double *x, double *y, double *part)
 {
     const spira_plan &P = A->plan;
     const uint64_t grid = (uint64_t)P.xs_nblk_max * kSlices;
     if (grid > 0x7fffffffull) fail(SPIRA_ERR_UNSUPPORTED, "spmv: matrix too large for XSTREAM");
     if (grid > 0)
         k_spmv_xstream<1024><<<(unsigned)grid, kThreads, 0, A->dev->stream>>>(
-            P.xs_blk, P.xs_blk_start, A->n_rows, P.xs_off, P.xs_col, P.xs_val, x, P.xs_part);
-    k_xslice_sum<<<grid_for(A->n_rows, kThreads), kThreads, 0, A->dev->stream>>>(A->n_rows, P.xs_part, y);
+            P.xs_blk, P.xs_blk_start, A->n_rows, P.xs_off, P.xs_col, P.xs_val, x, part);
+    k_xslice_sum<<<grid_for(A->n_rows, kThreads), kThreads, 0, A->dev->stream>>>(A->n_rows, part, y);
 }
 
 template <int G>
-void launch_xslice(spira_dcsr *A, const double *x, double *y)
+void launch_xslice(spira_dcsr *A, const double *x, double *y, double *part)
 {
     const spira_plan &P = A->plan;
     const uint64_t nrb = (A->n_rows * (uint64_t)G + kThreads - 1) / kThreads;
     const uint64_t grid = nrb * kSlices;
     if (grid > 0x7fffffffull) fail(SPIRA_ERR_UNSUPPORTED, "spmv: matrix too large for XSLICE");
     k_spmv_xslice<G><<<(unsigned)grid, kThreads, 0, A->dev->stream>>>(A->n_rows, P.xs_off, P.xs_col, P.xs_val, x,
-                                                                       P.xs_part);
-    k_xslice_sum<<<grid_for(A->n_rows, kThreads), kThreads, 0, A->dev->stream>>>(A->n_rows, P.xs_part, y);
+                                                                       part);
+    k_xslice_sum<<<grid_for(A->n_rows, kThreads), kThreads, 0, A->dev->stream>>>(A->n_rows, part, y);
 }
 
 }  // namespace
@@ -827,12 +829,13 @@ int spmv_heuristic_variant(const spira_dcsr *A)
     }
 }
 
-void spmv_launch_f64(spira_dcsr *A, const double *x, double *y, int variant)
+namespace {
+
+// Launch `variant` on a plan whose parts it needs are in place; the scratch
+// of the XPHASE_PACED counters and the XSLICE/XSTREAM partial sums is passed
+// in (plan memory, or the caller's fixed array).
+void launch_planned(spira_dcsr *A, const double *x, double *y, int variant, uint32_t *ctr, double *part)
 {
-    if (A->n_rows == 0) return;
-    if (A->n_rows >= (1ull << 32) - 1) fail(SPIRA_ERR_UNSUPPORTED, "spmv: n_rows must be < 2^32 - 1");
-    ensure_stats(A);
-    if (variant < 0) variant = spmv_heuristic_variant(A);
     const bool o32 = A->plan.off32 != nullptr;
     if (variant <= SPIRA_SPMV_VEC64) {
         const int g = 1 << variant;
@@ -840,7 +843,6 @@ void spmv_launch_f64(spira_dcsr *A, const double *x, double *y, int variant)
         else launch_vec_g<uint64_t>(g, A, A->off, x, y);
     } else if (variant == SPIRA_SPMV_STREAM1K || variant == SPIRA_SPMV_STREAM2K) {
         const int which = variant == SPIRA_SPMV_STREAM1K ? 0 : 1;
-        ensure_blocks(A, which);
         const uint32_t *blk = A->plan.blk[which];
         const uint32_t nb = A->plan.nblk[which];
         if (which == 0) {
@@ -850,22 +852,65 @@ void spmv_launch_f64(spira_dcsr *A, const double *x, double *y, int variant)
             if (o32) launch_stream<2048, uint32_t>(A, blk, nb, A->plan.off32, x, y);
             else launch_stream<2048, uint64_t>(A, blk, nb, A->off, x, y);
         }
-    } else if (variant >= SPIRA_SPMV_XSLICE1 && variant <= SPIRA_SPMV_XSLICE4) {
-        ensure_xslice(A);
-        if (variant == SPIRA_SPMV_XSLICE1) launch_xslice<1>(A, x, y);
-        else if (variant == SPIRA_SPMV_XSLICE2) launch_xslice<2>(A, x, y);
-        else launch_xslice<4>(A, x, y);
+    } else if (variant == SPIRA_SPMV_XSLICE1) {
+        launch_xslice<1>(A, x, y, part);
+    } else if (variant == SPIRA_SPMV_XSLICE2) {
+        launch_xslice<2>(A, x, y, part);
+    } else if (variant == SPIRA_SPMV_XSLICE4) {
+        launch_xslice<4>(A, x, y, part);
     } else if (variant == SPIRA_SPMV_XSTREAM) {
-        ensure_xslice(A);
-        launch_xstream(A, x, y);
-    } else if (variant == SPIRA_SPMV_XPHASE || variant == SPIRA_SPMV_XPHASE_PACED) {
-        ensure_xphase(A);
-        if (variant == SPIRA_SPMV_XPHASE) launch_xphase<6, false>(A, x, y);
-        else launch_xphase<20, true>(A, x, y);
+        launch_xstream(A, x, y, part);
+    } else if (variant == SPIRA_SPMV_XPHASE) {
+        launch_xphase<6, false>(A, x, y, ctr);
+    } else if (variant == SPIRA_SPMV_XPHASE_PACED) {
+        launch_xphase<20, true>(A, x, y, ctr);
     } else {
         fail(SPIRA_ERR_INVALID_ARGUMENT, "spmv: unknown kernel variant");
     }
     SPIRA_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+bool spmv_plan_ready(const spira_dcsr *A, int variant)
+{
+    const spira_plan &P = A->plan;
+    if (!P.stats_valid || (A->nnz < (1ull << 32) && !P.off32)) return false;
+    if (variant < 0) variant = spmv_heuristic_variant(A);
+    if (variant <= SPIRA_SPMV_VEC64) return true;
+    if (variant == SPIRA_SPMV_STREAM1K) return P.blk[0] != nullptr;
+    if (variant == SPIRA_SPMV_STREAM2K) return P.blk[1] != nullptr;
+    if (variant >= SPIRA_SPMV_XSLICE1 && variant <= SPIRA_SPMV_XSTREAM) return P.xs_off != nullptr;
+    return P.ph_goff != nullptr;
+}
+
+uint64_t spmv_scratch_bytes(const spira_dcsr *A, int variant)
+{
+    if (variant < 0) variant = spmv_heuristic_variant(A);
+    if (variant == SPIRA_SPMV_XPHASE_PACED) return 8 * kPhaseSyncStride * sizeof(uint32_t);
+    if (variant >= SPIRA_SPMV_XSLICE1 && variant <= SPIRA_SPMV_XSTREAM) return kSlices * A->n_rows * sizeof(double);
+    return 0;
+}
+
+void spmv_launch_planned(spira_dcsr *A, const double *x, double *y, int variant, void *scratch)
+{
+    if (A->n_rows == 0) return;
+    if (variant < 0) variant = spmv_heuristic_variant(A);
+    launch_planned(A, x, y, variant, (uint32_t *)scratch, (double *)scratch);
+}
+
+void spmv_launch_f64(spira_dcsr *A, const double *x, double *y, int variant)
+{
+    if (A->n_rows == 0) return;
+    if (A->n_rows >= (1ull << 32) - 1) fail(SPIRA_ERR_UNSUPPORTED, "spmv: n_rows must be < 2^32 - 1");
+    ensure_stats(A);
+    if (variant < 0) variant = spmv_heuristic_variant(A);
+    if (variant == SPIRA_SPMV_STREAM1K || variant == SPIRA_SPMV_STREAM2K)
+        ensure_blocks(A, variant == SPIRA_SPMV_STREAM1K ? 0 : 1);
+    else if (variant >= SPIRA_SPMV_XSLICE1 && variant <= SPIRA_SPMV_XSTREAM) ensure_xslice(A);
+    else if (variant == SPIRA_SPMV_XPHASE || variant == SPIRA_SPMV_XPHASE_PACED) ensure_xphase(A);
+    else if (variant > SPIRA_SPMV_VEC64) fail(SPIRA_ERR_INVALID_ARGUMENT, "spmv: unknown kernel variant");
+    launch_planned(A, x, y, variant, A->plan.ph_sync, A->plan.xs_part);
 }
 
 }  // namespace spira_amd

@@ -59,6 +59,7 @@ def test_constants_match_python_mirror():
     assert d["SPIRA_X_DEVICE"] == spira_amd.X_DEVICE and d["SPIRA_Y_DEVICE"] == spira_amd.Y_DEVICE
     assert d["SPIRA_IN_DEVICE"] == spira_amd.IN_DEVICE and d["SPIRA_NO_SYNC"] == spira_amd.NO_SYNC
     assert d["SPIRA_SPMV_N_VARIANTS"] == spira_amd.N_VARIANTS
+    assert d["SPIRA_NOALLOC_SCRATCH_BYTES"] == spira_amd.NOALLOC_SCRATCH_BYTES
     for name, v in spira_amd.VARIANTS.items():
         key = "SPIRA_SPMV_" + name.upper().replace("STREAM1K", "STREAM1K")
         assert d[key] == v, key
