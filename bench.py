@@ -53,6 +53,9 @@ def parse():
     p.add_argument("--mixed-reps", type=int, default=50)
     p.add_argument("--recompute-ratio", type=float, default=16.0,
                    help="cancellation-ratio threshold of the FP64 row recompute in the mixed section")
+    p.add_argument("--no-auto", action="store_true", help="skip the AUTO kernel-selector section")
+    p.add_argument("--auto-seeds", type=int, default=100)
+    p.add_argument("--auto-seed", type=int, default=960_000)
     p.add_argument("--batch-mats", type=int, default=4096)
     p.add_argument("--batch-seed", type=int, default=20_000_000,
                    help="first seed of the batch (disjoint from the selector's training and held-out seeds)")
@@ -259,6 +262,7 @@ def run_mixed(args, np, torch, spira_amd, W, dev, stream):
     nnz = A.nnz
     log(f"bench: mixed: {n} rows, {nnz} nnz generated and uploaded in {time.time() - t0:.1f}s")
     reps = args.mixed_reps
+    auto_v = A.choose_variant()  # AUTO (learned selector), before the autotuner pins a variant
     best, at = A.autotune(x, y64, reps=3)
     A.spmv(x, y64, device=True, sync=True)
     t64 = _time_calls(torch, stream, lambda: A.spmv(x, y64, device=True, sync=False), reps)
@@ -267,7 +271,9 @@ def run_mixed(args, np, torch, spira_amd, W, dev, stream):
                        f"within +-4096 of the diagonal, values N(0,1), x U(-1,1)",
            "n_rows": n, "nnz": nnz, "mean_row": round(nnz / n, 2),
            "fp64": {"variant": spira_amd.VARIANT_NAMES[best], "ms": round(t64, 4), "bytes_per_call": b64,
-                    "GB_s": round(b64 / t64 / 1e6, 1), "GFLOP_s": round(2 * nnz / t64 / 1e6, 1)},
+                    "GB_s": round(b64 / t64 / 1e6, 1), "GFLOP_s": round(2 * nnz / t64 / 1e6, 1),
+                    "auto_variant": spira_amd.VARIANT_NAMES[auto_v],
+                    "auto_fraction_of_autotuned": round(at[best] / at[auto_v], 4)},
            "storage": {}}
     sval = {"f32": 4, "bf16": 2, "split_bf16": 4}
     for st in ("f32", "bf16", "split_bf16"):
@@ -297,6 +303,51 @@ def run_mixed(args, np, torch, spira_amd, W, dev, stream):
                 x.cpu().numpy())
     del off, col, val
     return out, (A, x, y64, y, host)
+
+
+def run_auto_selector(args, np, torch, spira_amd, dev, stream):
+    """AUTO (the learned kernel selector, no autotune call) against the
+    exhaustive autotuner on fresh corpus seeds (disjoint from the training
+    and held-out seeds), next to the row-length rule it replaces; plus the
+    500-seed held-out score recorded when the model was trained."""
+    from spira_amd import spmv_selector as S
+    from spira_amd import workloads as W
+    auto, rule, T, fams = [], [], [], []
+    t0 = time.time()
+    with torch.cuda.stream(stream):
+        for seed in range(args.auto_seed, args.auto_seed + args.auto_seeds):
+            off, col, val, n_cols, fam = W.spmv_corpus_matrix(seed, "cuda", 1 << 23)
+            n = off.numel() - 1
+            A = spira_amd.DeviceCSR(dev, n, n_cols)
+            A.upload(off, col, val, device=True)
+            rule.append(S.rule_variant(*S.row_stats(off)))
+            del off, col, val
+            auto.append(A.choose_variant())
+            x = torch.rand(n_cols, dtype=torch.float64, device="cuda")
+            y = torch.empty(n, dtype=torch.float64, device="cuda")
+            _, ms = A.autotune(x, y, reps=3)
+            T.append([ms[v] if ms[v] >= 0 else np.nan for v in range(spira_amd.N_VARIANTS)])
+            fams.append(fam)
+            A.close()
+    T = np.array(T)
+    rows = np.arange(len(auto))
+    best = np.nanmin(T, axis=1)
+    f_auto, f_rule = S.fraction_of_best(T, auto), S.fraction_of_best(T, rule)
+    with open(S.MODEL_PATH) as f:
+        trained = json.load(f)
+    out = {"workload": f"{len(auto)} corpus matrices (seeds {args.auto_seed}..{args.auto_seed + len(auto) - 1}; "
+                       f"uniform, banded, stencil, power-law, block-diagonal, mostly-local; up to 2^23 rows)",
+           "auto_mean_fraction_of_best": round(float(f_auto.mean()), 4),
+           "auto_total_time_fraction": round(float(best.sum() / T[rows, auto].sum()), 4),
+           "rule_mean_fraction_of_best": round(float(f_rule.mean()), 4),
+           "rule_total_time_fraction": round(float(best.sum() / T[rows, rule].sum()), 4),
+           "auto_fraction_below_0.9": round(float((f_auto < 0.9).mean()), 4),
+           "per_family_auto": {fm: round(float(f_auto[[f == fm for f in fams]].mean()), 4) for fm in sorted(set(fams))},
+           "heldout_500_at_training": {k: trained[k] for k in ("heldout_mean_fraction_of_best",
+                                                              "heldout_total_time_fraction", "corpus")},
+           "seconds": round(time.time() - t0, 1)}
+    log(f"bench: auto selector: {out}")
+    return out
 
 
 def _batch_x(torch, n_cols, k, seed):
@@ -514,8 +565,23 @@ def main():
                       "bytes": W.spmv_bytes(n, n, nnz, s_val=8, s_col=4, s_ptr=4, s_x=8, s_y=8),
                       "x_host": x_host}
 
+    # XPHASE plan build (once per locked matrix, on the first XPHASE call):
+    # the first call's time minus a steady call's, row statistics built before
+    for name, A in dA.items():
+        A.spmv(xs[name], ys[name], variant=spira_amd.VARIANTS["vec32"], device=True, sync=True)
+        t = time.perf_counter()
+        A.spmv(xs[name], ys[name], variant=spira_amd.VARIANTS["xphase_paced"], device=True, sync=True)
+        first = time.perf_counter() - t
+        t = time.perf_counter()
+        for _ in range(5):
+            A.spmv(xs[name], ys[name], variant=spira_amd.VARIANTS["xphase_paced"], device=True, sync=True)
+        steady = (time.perf_counter() - t) / 5
+        info[name]["plan_build_ms"] = round(1e3 * (first - steady), 3)
+        log(f"bench: {name}: XPHASE plan built in {info[name]['plan_build_ms']} ms")
+
     # variant choice: exhaustive autotune (default), heuristic, or fixed
     for name, A in dA.items():
+        info[name]["auto_variant"] = A.choose_variant()  # AUTO (learned selector), before any pin
         if args.variant == "autotune":
             best, ms = A.autotune(xs[name], ys[name], reps=5)
             info[name]["autotune_ms"] = {spira_amd.VARIANT_NAMES[v]: round(t, 4) for v, t in ms.items()}
@@ -580,6 +646,13 @@ def main():
                         "GFLOP_s": round(2 * info[name]["nnz"] / (t * 1e-3) / 1e9, 1)}
         if "autotune_ms" in info[name]:
             detail[name]["autotune_ms"] = info[name]["autotune_ms"]
+        detail[name]["plan_build_ms"] = info[name]["plan_build_ms"]
+        av = info[name]["auto_variant"]
+        detail[name]["auto_variant"] = spira_amd.VARIANT_NAMES[av]
+        if "autotune_ms" in info[name]:
+            at = info[name]["autotune_ms"]
+            detail[name]["auto_fraction_of_autotuned"] = round(
+                min(t for t in at.values() if t >= 0) / at[spira_amd.VARIANT_NAMES[av]], 4)
 
     lock_detail, lock_src = (None, None) if args.no_lock else run_lock(args, np, torch, spira_amd, W, dev, stream)
     tr_detail, tr_src = (None, None) if args.no_transpose else run_transpose(
@@ -592,6 +665,7 @@ def main():
         log(f"bench: add {alg_detail['add']['ms']} ms, spgemm {alg_detail['spgemm_random_1m']['ms']} ms")
     mixed_detail, mixed_src = (None, None) if args.no_mixed else run_mixed(args, np, torch, spira_amd, W, dev, stream)
     batch_detail = None if args.no_batch else run_batch(args, np, torch, spira_amd, W, dev, stream)
+    auto_detail = None if args.no_auto else run_auto_selector(args, np, torch, spira_amd, dev, stream)
 
     achieved = info[dominant]["bytes"] / (kern_ms * 1e-3) / 1e9
     traffic = profile_traffic(dominant + ":" + info[dominant]["variant"])
@@ -756,6 +830,7 @@ def main():
         "algebra": alg_detail,
         "mixed": mixed_detail,
         "batched": batch_detail,
+        "auto_selector": auto_detail,
     }
     print(json.dumps(out), flush=True)
 

@@ -385,8 +385,8 @@ int spira_spmv_f64_noalloc(spira_dcsr *A, const double *x, uint64_t x_len, doubl
         const char *unprepared =
             "spmv_noalloc: the matrix has no SpMV plan for this variant (one spira_spmv_f64 call with the "
             "variant after the last lock or upload builds it)";
-        if (variant < 0 && !A->plan.stats_valid) fail(SPIRA_ERR_LOGIC, unprepared);
-        const int v = variant < 0 ? spmv_heuristic_variant(A) : variant;
+        const int v = variant < 0 ? spmv_auto_choice(A) : variant;
+        if (v < 0 || !A->plan.stats_valid) fail(SPIRA_ERR_LOGIC, unprepared);
         const uint64_t need = spmv_scratch_bytes(A, v);
         if (need > kNoAllocScratchBytes)
             fail(SPIRA_ERR_UNSUPPORTED, "spmv_noalloc: the variant needs " + std::to_string(need) +
@@ -445,8 +445,7 @@ int spira_spmv_choose(spira_dcsr *A, int *variant)
     if (!A || !variant) return SPIRA_ERR_INVALID_ARGUMENT;
     return guarded(A->dev, [&] {
         if (!A->built) fail(SPIRA_ERR_LOGIC, "spmv: matrix must be locked");
-        spmv_build_plan(A);
-        *variant = spmv_heuristic_variant(A);
+        *variant = A->n_rows ? spmv_resolve_auto(A) : 0;
     });
 }
 

@@ -143,7 +143,7 @@ inline unsigned grid_for(uint64_t items, unsigned per_block, unsigned cap = 0x7f
 // Growable device scratch owned by a handle.  Slot i keeps its own buffer so
 // one operation can hold several at once.
 struct workspace {
-    static constexpr int kSlots = 25;
+    static constexpr int kSlots = 26;
     void *ptr[kSlots] = {};
     size_t cap[kSlots] = {};
     hipStream_t stream = nullptr;  // work that may still read a slot runs here
@@ -257,6 +257,7 @@ enum ws_slot : int {
     WS_LONG_KEY = 22,   // lock: (list index, column) keys, unsorted then sorted
     WS_LONG_IDX = 23,   // lock: insertion index of each key, unsorted then sorted
     WS_LONG_P = 24,     // lock: prefix of the kept keys
+    WS_SEL = 25,        // spmv AUTO: selector parameters and activations
 };
 
 }  // namespace spira_amd
@@ -284,7 +285,8 @@ struct spira_plan {
     double *ph_val = nullptr;
     uint32_t *ph_sync = nullptr;  // phase-pacing counters, one per XCD
     uint32_t ph_S = 0, ph_RB = 0, ph_nb = 0, ph_width = 0;
-    int32_t variant = -1;        // pinned AUTO choice (-1 = heuristic)
+    int32_t variant = -1;        // pinned AUTO choice (-1 = none)
+    int32_t auto_variant = -1;   // AUTO's own choice once resolved (learned selector, else the rule)
     std::vector<std::pair<void *, size_t>> owned;  // every plan allocation
 };
 

@@ -246,6 +246,87 @@ void forward(spira_mlp *N, const float *X, uint32_t B)
 }
 }  // namespace
 
+namespace spira_amd {
+
+// The features of a locked CSR (the list at the top of this file), computed
+// on the device; slot 17 is left 0.
+void matrix_features_host(spira_dcsr *A, float *features)
+{
+    spira_dev *dev = A->dev;
+    double *st = (double *)dev->ws.get(WS_MISC, kStat * sizeof(double));
+    SPIRA_HIP(hipMemsetAsync(st, 0, kStat * sizeof(double), dev->stream));
+    double h[kStat] = {};
+    if (A->n_rows > 0) {
+        const unsigned g = grid_for(A->n_rows, kThreads, 1024);
+        k_feature_stats<<<g, kThreads, 0, dev->stream>>>(A->off, A->col, A->n_rows, st);
+        SPIRA_HIP(hipGetLastError());
+        SPIRA_HIP(hipMemcpyAsync(h, st, sizeof(h), hipMemcpyDeviceToHost, dev->stream));
+    }
+    dev_sync(dev);
+    const double n = (double)(A->n_rows ? A->n_rows : 1), nc = (double)(A->n_cols ? A->n_cols : 1);
+    const double mean = h[0] / n;
+    const double var = h[1] / n - mean * mean;
+    const double cv = mean > 0 ? std::sqrt(var > 0 ? var : 0.0) / mean : 0.0;
+    const double nonempty = n - h[3] > 0 ? n - h[3] : 1.0;
+    float f[kFeat] = {};
+    f[0] = (float)(std::log2((double)A->n_rows + 1) / 20.0);
+    f[1] = (float)(std::log2((double)A->nnz + 1) / 30.0);
+    f[2] = (float)(std::log2(mean + 1) / 10.0);
+    f[3] = (float)(std::min(cv, 4.0) / 4.0);
+    f[4] = (float)(std::log2(h[2] + 1) / 16.0);
+    for (int b = 0; b < 8; ++b) f[5 + b] = (float)(h[3 + b] / n);
+    f[13] = (float)(h[11] / nonempty / nc);
+    f[14] = (float)(h[12] / nonempty / nc);
+    f[15] = (float)(h[13] / nc);
+    f[16] = (float)(std::log2((double)A->n_cols + 1) / 20.0);
+    f[17] = 0.0f;
+    std::copy(f, f + kFeat, features);
+}
+
+namespace {
+#include "spmv_selector_weights.inc"
+}  // namespace
+
+// AUTO's kernel variant for a single matrix: the trained network's outputs
+// (predicted log2 slowdown of every variant against the best) for A's
+// features, lowest among the variants whose size limits A fits.  The forward
+// pass runs on the device (k_dense_fwd); the parameters are compiled in from
+// the trained model.  -1 when no trained model was compiled in.
+int spmv_learned_variant(spira_dcsr *A)
+{
+    if (!kSpmvSelTrained || A->n_rows == 0) return -1;
+    static_assert(kSpmvSelIn == kFeat && kSpmvSelOut == SPIRA_SPMV_N_VARIANTS, "selector model shape");
+    spira_dev *dev = A->dev;
+    hipStream_t st = dev->stream;
+    float f[kFeat];
+    matrix_features_host(A, f);
+    constexpr uint32_t dims[4] = {kSpmvSelIn, kSpmvSelHidden, kSpmvSelHidden, kSpmvSelOut};
+    constexpr size_t n_params = sizeof(kSpmvSelParams) / sizeof(float);
+    const size_t n_act = dims[0] + dims[1] + dims[2] + dims[3];
+    float *buf = (float *)dev->ws.get(WS_SEL, (n_params + n_act) * sizeof(float));
+    float *act = buf + n_params;
+    SPIRA_HIP(hipMemcpyAsync(buf, kSpmvSelParams, n_params * sizeof(float), hipMemcpyHostToDevice, st));
+    SPIRA_HIP(hipMemcpyAsync(act, f, kFeat * sizeof(float), hipMemcpyHostToDevice, st));
+    size_t po = 0, ao = 0;
+    for (int l = 0; l < 3; ++l) {
+        const uint32_t din = dims[l], dout = dims[l + 1];
+        const float *W = buf + po, *b = buf + po + (size_t)din * dout;
+        k_dense_fwd<<<1, kThreads, 0, st>>>(act + ao, W, b, act + ao + din, 1, din, dout, l < 2 ? 1 : 0);
+        SPIRA_HIP(hipGetLastError());
+        po += (size_t)din * dout + dout;
+        ao += din;
+    }
+    float out[kSpmvSelOut];
+    SPIRA_HIP(hipMemcpyAsync(out, act + ao, sizeof(out), hipMemcpyDeviceToHost, st));
+    dev_sync(dev);
+    int best = -1;
+    for (int v = 0; v < (int)kSpmvSelOut; ++v)
+        if (spmv_variant_fits(A, v) && (best < 0 || out[v] < out[best])) best = v;
+    return best;
+}
+
+}  // namespace spira_amd
+
 extern "C" {
 
 int spira_matrix_features(spira_dcsr *A, float *features)
@@ -253,35 +334,7 @@ int spira_matrix_features(spira_dcsr *A, float *features)
     if (!A || !features) return SPIRA_ERR_INVALID_ARGUMENT;
     return sguard(A->dev, [&] {
         if (!A->built) fail(SPIRA_ERR_LOGIC, "features: matrix must be locked");
-        spira_dev *dev = A->dev;
-        double *st = (double *)dev->ws.get(WS_MISC, kStat * sizeof(double));
-        SPIRA_HIP(hipMemsetAsync(st, 0, kStat * sizeof(double), dev->stream));
-        double h[kStat] = {};
-        if (A->n_rows > 0) {
-            const unsigned g = grid_for(A->n_rows, kThreads, 1024);
-            k_feature_stats<<<g, kThreads, 0, dev->stream>>>(A->off, A->col, A->n_rows, st);
-            SPIRA_HIP(hipGetLastError());
-            SPIRA_HIP(hipMemcpyAsync(h, st, sizeof(h), hipMemcpyDeviceToHost, dev->stream));
-        }
-        dev_sync(dev);
-        const double n = (double)(A->n_rows ? A->n_rows : 1), nc = (double)(A->n_cols ? A->n_cols : 1);
-        const double mean = h[0] / n;
-        const double var = h[1] / n - mean * mean;
-        const double cv = mean > 0 ? std::sqrt(var > 0 ? var : 0.0) / mean : 0.0;
-        const double nonempty = n - h[3] > 0 ? n - h[3] : 1.0;
-        float f[kFeat] = {};
-        f[0] = (float)(std::log2((double)A->n_rows + 1) / 20.0);
-        f[1] = (float)(std::log2((double)A->nnz + 1) / 30.0);
-        f[2] = (float)(std::log2(mean + 1) / 10.0);
-        f[3] = (float)(std::min(cv, 4.0) / 4.0);
-        f[4] = (float)(std::log2(h[2] + 1) / 16.0);
-        for (int b = 0; b < 8; ++b) f[5 + b] = (float)(h[3 + b] / n);
-        f[13] = (float)(h[11] / nonempty / nc);
-        f[14] = (float)(h[12] / nonempty / nc);
-        f[15] = (float)(h[13] / nc);
-        f[16] = (float)(std::log2((double)A->n_cols + 1) / 20.0);
-        f[17] = 0.0f;
-        std::copy(f, f + kFeat, features);
+        matrix_features_host(A, features);
     });
 }
 

@@ -27,8 +27,6 @@
 // keeping a row neighbourhood's x lines in one L2.
 #include "internal.hpp"
 
-#include <rocprim/device/device_segmented_radix_sort.hpp>
-
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -396,58 +394,292 @@ __global__ __launch_bounds__(kPhaseThreads) void k_spmv_xphase(uint64_t n_rows, 
     for (uint32_t i = threadIdx.x; i < nr; i += kPhaseThreads) y[r0 + i] = s_y[i];
 }
 
-// Plan: entries of row r per slice, stored at index (b * S + s) * RB + (r - b RB)
-// so that one exclusive scan gives every row's first position in every group.
-__global__ __launch_bounds__(kThreads) void k_xphase_count(uint64_t n_rows, uint64_t n_slot_rows, uint32_t RB,
-                                                            uint32_t S, uint64_t width,
-                                                            const uint64_t *__restrict__ off,
-                                                            const uint32_t *__restrict__ col,
-                                                            uint64_t *__restrict__ cnt)
+// XPHASE plan, built element-parallel in three passes (no per-row state, so
+// its cost and memory grow with nnz alone).  Entries are binned into
+// sub-groups (row block, column slice, sub-slice of 2^16 columns); a slice's
+// sub-groups are adjacent, so a (block, slice) group is their union.
+//   bin<count>    a workgroup takes 8192 consecutive entries, finds each
+//                 one's row from an LDS copy of its rows' offsets, counts its
+//                 sub-groups in LDS and adds them to the sub-group counters;
+//                 an exclusive scan gives the sub-group starts.
+//   bin<scatter>  the same chunks reserve their share of each sub-group with
+//                 one atomic per sub-group and write (row in block << 18 |
+//                 column in slice, value) there.
+//   lines         one workgroup per sub-group orders its entries by x line
+//                 (16 columns, 128 bytes): line counts and their prefix in
+//                 LDS, then windows of consecutive lines holding at most
+//                 kLineStage entries are staged in LDS in line order and
+//                 written out whole, so every store is coalesced.  The 64
+//                 lanes of a wavefront then sweep x in order and lanes on one
+//                 line share its request.  (The order inside a line is not
+//                 fixed; the FP64 LDS adds are not order-fixed either.)
+constexpr int kPlanThreads = 1024;
+constexpr int kPlanItems = 8;
+constexpr uint32_t kPlanChunk = kPlanThreads * kPlanItems;
+constexpr uint32_t kPlanRowCache = 2048;
+constexpr uint32_t kPlanLocalGroups = 4096;
+constexpr uint32_t kLineCols = 16;
+constexpr uint32_t kSubBits = 16;                              // columns per sub-slice: 2^16
+constexpr uint32_t kSubLines = (1u << kSubBits) / kLineCols;  // 4096 lines
+constexpr uint32_t kWinLines = kSubLines;
+constexpr uint32_t kLineStage = 4096;                          // entries staged in LDS per window
+constexpr int kLineItems = 8;                                  // entries per thread held in registers
+
+// Row table of the bin passes: tab[b] = row owning entry b * kPlanChunk, and
+// tab[chunks] = row owning the last entry, so chunk b's rows are
+// [tab[b], tab[b + 1]] without a search.  One thread per row.
+__global__ void k_xphase_chunk_rows(const uint64_t *__restrict__ off, uint64_t n_rows, uint64_t nnz,
+                                    uint64_t *__restrict__ tab)
 {
-    // one thread per row slot r < n_slot_rows = nb * RB: the last block's
-    // slots past n_rows get zero counts
-    const uint64_t r = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (r >= n_slot_rows) return;
-    const uint64_t b = r / RB, rl = r % RB;
-    uint64_t *c = cnt + b * S * RB + rl;
-    for (uint32_t s = 0; s < S; ++s) c[(uint64_t)s * RB] = 0;
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n_rows) return;
-    for (uint64_t k = off[r]; k < off[r + 1]; ++k) c[(uint64_t)(col[k] / width) * RB] += 1;
+    const uint64_t e0 = off[r], e1 = off[r + 1];
+    if (e0 == e1) return;
+    for (uint64_t b = (e0 + kPlanChunk - 1) / kPlanChunk; b * kPlanChunk < e1; ++b) tab[b] = r;
+    if (e1 == nnz) tab[(nnz + kPlanChunk - 1) / kPlanChunk] = r;
 }
 
-__global__ __launch_bounds__(kThreads) void k_xphase_scatter(uint64_t n_rows, uint32_t RB, uint32_t S,
-                                                              uint64_t width, const uint64_t *__restrict__ off,
+template <bool SCATTER>
+__global__ __launch_bounds__(kPlanThreads) void k_xphase_bin(uint64_t nnz, uint64_t n_rows, uint32_t RB, uint32_t S,
+                                                              uint32_t SUB, uint32_t width,
+                                                              const uint64_t *__restrict__ off,
                                                               const uint32_t *__restrict__ col,
                                                               const double *__restrict__ val,
-                                                              const uint64_t *__restrict__ pos,
-                                                              uint32_t *__restrict__ prc,
-                                                              double *__restrict__ pval)
+                                                              const uint64_t *__restrict__ tab,
+                                                              unsigned long long *__restrict__ gctr,
+                                                              uint32_t *__restrict__ trc, double *__restrict__ tval)
 {
-    const uint64_t r = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (r >= n_rows) return;
-    const uint64_t b = r / RB, rl = r % RB;
-    const uint64_t *p = pos + b * S * RB + rl;
-    uint32_t cur = 0xffffffffu;
-    uint64_t w = 0;
-    for (uint64_t k = off[r]; k < off[r + 1]; ++k) {  // columns ascend: slices ascend along the row
-        const uint32_t s = (uint32_t)(col[k] / width);
-        if (s != cur) {
-            cur = s;
-            w = p[(uint64_t)s * RB];
+    __shared__ uint64_t s_off[kPlanRowCache + 1];
+    __shared__ uint32_t s_cnt[kPlanLocalGroups];
+    __shared__ uint64_t s_base[SCATTER ? kPlanLocalGroups : 1];
+    const uint64_t first = (uint64_t)blockIdx.x * kPlanChunk;
+    if (first >= nnz) return;
+    const uint64_t last = min(first + kPlanChunk, nnz) - 1;
+    // the entries first: their loads are in flight while the row cache fills
+    uint32_t cc[kPlanItems];
+    double v[kPlanItems];
+#pragma unroll
+    for (int it = 0; it < kPlanItems; ++it) {
+        const uint64_t k = first + (uint64_t)it * kPlanThreads + threadIdx.x;
+        cc[it] = k <= last ? ld_stream(col + k) : 0u;
+        if (SCATTER) v[it] = k <= last ? ld_stream(val + k) : 0.0;
+    }
+    const uint64_t rlo = tab[blockIdx.x], rhi = tab[blockIdx.x + 1];
+    const bool cached = rhi - rlo + 1 <= kPlanRowCache;
+    if (cached)
+        for (uint64_t j = threadIdx.x; j <= rhi - rlo + 1; j += kPlanThreads) s_off[j] = off[rlo + j];
+    const uint64_t gps = (uint64_t)S * SUB;  // sub-groups per block
+    const uint64_t b0 = rlo / RB;
+    const uint64_t nlg = (rhi / RB - b0 + 1) * gps;  // sub-groups this chunk can touch
+    const bool local = nlg <= kPlanLocalGroups;
+    if (local)
+        for (uint32_t j = threadIdx.x; j < nlg; j += kPlanThreads) s_cnt[j] = 0;
+    __syncthreads();
+
+    uint32_t lg[kPlanItems], rc[kPlanItems];
+#pragma unroll
+    for (int it = 0; it < kPlanItems; ++it) {
+        const uint64_t k = first + (uint64_t)it * kPlanThreads + threadIdx.x;
+        lg[it] = 0xffffffffu;
+        if (k > last) continue;
+        const uint32_t c = cc[it];
+        uint64_t r;
+        if (cached) {
+            uint32_t a = 0, e = (uint32_t)(rhi - rlo + 1);
+            while (e - a > 1) {
+                const uint32_t mid = (a + e) >> 1;
+                if (s_off[mid] <= k) a = mid;
+                else e = mid;
+            }
+            r = rlo + a;
+        } else {
+            r = row_of(off, rlo, rhi + 1, k);
         }
-        prc[w] = ((uint32_t)rl << kPhaseColBits) | (uint32_t)(col[k] - (uint64_t)s * width);
-        pval[w] = val[k];
-        ++w;
+        const uint64_t b = r / RB;
+        const uint32_t sl = c / width;
+        const uint32_t cl = c - sl * width;
+        rc[it] = ((uint32_t)(r - b * RB) << kPhaseColBits) | cl;
+        lg[it] = (uint32_t)((b - b0) * gps + (uint64_t)sl * SUB + (cl >> kSubBits));
+        if (!local) {
+            const uint64_t g = b0 * gps + lg[it];
+            if (SCATTER) {
+                const uint64_t p = atomicAdd(gctr + g, 1ull);
+                trc[p] = rc[it];
+                tval[p] = v[it];
+            } else {
+                atomicAdd(gctr + g, 1ull);
+            }
+        } else {
+            atomicAdd(&s_cnt[lg[it]], 1u);
+        }
+    }
+    if (!local) return;
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < nlg; j += kPlanThreads) {
+        const uint32_t n = s_cnt[j];
+        if (n == 0) continue;
+        if (SCATTER) {
+            s_base[j] = atomicAdd(gctr + b0 * gps + j, (unsigned long long)n);
+            s_cnt[j] = 0;  // now the running rank inside the reserved run
+        } else {
+            atomicAdd(gctr + b0 * gps + j, (unsigned long long)n);
+        }
+    }
+    if (!SCATTER) return;
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < kPlanItems; ++it) {
+        if (lg[it] == 0xffffffffu) continue;
+        const uint64_t p = s_base[lg[it]] + atomicAdd(&s_cnt[lg[it]], 1u);
+        trc[p] = rc[it];
+        tval[p] = v[it];
     }
 }
 
-// Group starts: group g begins where its first row slot begins.
-__global__ void k_xphase_goff(uint64_t groups, uint32_t RB, const uint64_t *__restrict__ pos, uint64_t total,
+// In-place exclusive scan of c[0..n) (n <= kWinLines) by a 1024-thread block
+// (each thread owns a run of 4); c[n] receives the total.
+__device__ __forceinline__ void block_scan_lines(uint32_t *c, uint32_t n)
+{
+    __shared__ uint32_t s_w[kPlanThreads / kWave];
+    constexpr uint32_t per = kWinLines / kPlanThreads;
+    const uint32_t a = threadIdx.x * per;
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < per; ++i) sum += (a + i < n) ? c[a + i] : 0u;
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x >> 6;
+    uint32_t inc = sum;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const uint32_t t = __shfl_up(inc, d, kWave);
+        if (lane >= d) inc += t;
+    }
+    if (lane == kWave - 1) s_w[wid] = inc;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+    for (int w = 0; w < kPlanThreads / kWave; ++w) {
+        if (w < wid) base += s_w[w];
+        tot += s_w[w];
+    }
+    uint32_t run = base + inc - sum;
+#pragma unroll
+    for (uint32_t i = 0; i < per; ++i) {
+        if (a + i < n) {
+            const uint32_t t = c[a + i];
+            c[a + i] = run;
+            run += t;
+        }
+    }
+    if (threadIdx.x == 0) c[n] = tot;
+    __syncthreads();
+}
+
+// One workgroup per sub-group at a time.  Sub-groups of up to kLineItems
+// entries per thread are held in registers: counted per line, then placed in
+// windows of consecutive lines of at most kLineStage entries into LDS in line
+// order and written out whole.  Larger ones read their entries again for each
+// window; a single line holding more than a window is written in place (its
+// own range; the order inside a line is free).
+__global__ __launch_bounds__(kPlanThreads) void k_xphase_lines(const uint64_t *__restrict__ gsub, uint64_t n_sub,
+                                                                uint32_t SUB, uint32_t width,
+                                                                const uint32_t *__restrict__ trc,
+                                                                const double *__restrict__ tval,
+                                                                uint32_t *__restrict__ prc, double *__restrict__ pval)
+{
+    __shared__ uint32_t s_pre[kSubLines + 1];  // line counts, then their exclusive prefix
+    __shared__ uint32_t s_cur[kSubLines];      // cursors of the current window's lines
+    __shared__ uint32_t s_rc[kLineStage];
+    __shared__ double s_val[kLineStage];
+    constexpr uint32_t kColMask = (1u << kPhaseColBits) - 1u;
+    for (uint64_t g = blockIdx.x; g < n_sub; g += gridDim.x) {
+        const uint64_t e0 = gsub[g], e1 = gsub[g + 1];
+        if (e0 == e1) continue;  // the same for every thread of the block
+        const uint32_t n = (uint32_t)(e1 - e0);  // < 2^32 (checked by the host)
+        const uint32_t c0 = (uint32_t)(g % SUB) << kSubBits;  // first column of the sub-slice in its slice
+        const uint32_t nl = (min(width, c0 + (1u << kSubBits)) - c0 + kLineCols - 1) / kLineCols;
+        const bool reg = n <= kLineItems * kPlanThreads;
+        uint32_t rw[kLineItems];
+        double rv[kLineItems];
+        if (reg) {
+#pragma unroll
+            for (int it = 0; it < kLineItems; ++it) {
+                const uint32_t i = threadIdx.x + it * kPlanThreads;
+                rw[it] = i < n ? ld_stream(trc + e0 + i) : 0u;
+                rv[it] = i < n ? ld_stream(tval + e0 + i) : 0.0;
+            }
+        }
+        for (uint32_t j = threadIdx.x; j < nl; j += kPlanThreads) s_pre[j] = 0;
+        __syncthreads();
+        if (reg) {
+#pragma unroll
+            for (int it = 0; it < kLineItems; ++it)
+                if (threadIdx.x + it * kPlanThreads < n) atomicAdd(&s_pre[((rw[it] & kColMask) - c0) / kLineCols], 1u);
+        } else {
+            for (uint32_t i = threadIdx.x; i < n; i += kPlanThreads)
+                atomicAdd(&s_pre[((trc[e0 + i] & kColMask) - c0) / kLineCols], 1u);
+        }
+        __syncthreads();
+        block_scan_lines(s_pre, nl);
+        uint32_t ws = 0;  // windows of consecutive lines; identical in every thread
+        while (ws < nl) {
+            uint32_t lo = ws + 1, hi = nl;  // last we in (ws, nl] with s_pre[we] - s_pre[ws] <= kLineStage
+            const uint32_t lim = s_pre[ws] + kLineStage;
+            if (s_pre[lo] > lim) {
+                hi = lo;  // one line holds more than a window
+            } else {
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi + 1) >> 1;
+                    if (s_pre[mid] <= lim) lo = mid;
+                    else hi = mid - 1;
+                }
+                hi = lo;
+            }
+            const uint32_t we = hi;
+            const uint32_t wbase = s_pre[ws], wn = s_pre[we] - wbase;
+            const bool staged = wn <= kLineStage;
+            for (uint32_t l = ws + threadIdx.x; l < we; l += kPlanThreads) s_cur[l - ws] = s_pre[l] - wbase;
+            __syncthreads();
+            auto place = [&](uint32_t w, double v) {
+                const uint32_t l = ((w & kColMask) - c0) / kLineCols;
+                if (l < ws || l >= we) return;
+                const uint32_t d = atomicAdd(&s_cur[l - ws], 1u);
+                if (staged) {
+                    s_rc[d] = w;
+                    s_val[d] = v;
+                } else {
+                    prc[e0 + wbase + d] = w;
+                    pval[e0 + wbase + d] = v;
+                }
+            };
+            if (reg) {
+#pragma unroll
+                for (int it = 0; it < kLineItems; ++it)
+                    if (threadIdx.x + it * kPlanThreads < n) place(rw[it], rv[it]);
+            } else {
+                for (uint32_t i = threadIdx.x; i < n; i += kPlanThreads) {
+                    const uint32_t w = trc[e0 + i];
+                    const uint32_t l = ((w & kColMask) - c0) / kLineCols;
+                    if (l >= ws && l < we) place(w, tval[e0 + i]);
+                }
+            }
+            __syncthreads();
+            if (staged) {
+                for (uint32_t d = threadIdx.x; d < wn; d += kPlanThreads) {
+                    prc[e0 + wbase + d] = s_rc[d];
+                    pval[e0 + wbase + d] = s_val[d];
+                }
+                __syncthreads();
+            }
+            ws = we;
+        }
+    }
+}
+
+// Slice-group starts of the kernel: every SUB-th sub-group start.
+__global__ void k_xphase_goff(const uint64_t *__restrict__ gsub, uint64_t groups, uint32_t SUB,
                               uint64_t *__restrict__ goff)
 {
     const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g < groups) goff[g] = pos[g * RB];
-    if (g == groups) goff[g] = total;
+    if (g <= groups) goff[g] = gsub[g * SUB];
 }
 
 // ---------------------------------------------------------------------------
@@ -543,6 +775,20 @@ __global__ void k_plan_scatter(const uint64_t *__restrict__ off, uint64_t n_rows
     const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r < n_rows && stream_block_start(off, r, C)) blk[pos[r]] = (uint32_t)r;
     if (r == 0) blk[pos[n_rows]] = (uint32_t)n_rows;
+}
+
+// XPHASE's shape for A: rows per block (all CUs busy on mid-sized matrices,
+// at most kPhaseRows), blocks, column slices (a multiple of 8, each at most
+// kPhaseSliceBytes of x, at most kPhaseMaxSlices) and columns per slice.
+void xphase_geometry(const spira_dcsr *A, uint64_t *rb, uint64_t *nb, uint64_t *S, uint64_t *width)
+{
+    const uint64_t n = A->n_rows;
+    *rb = std::min<uint64_t>(kPhaseRows, std::max<uint64_t>(1, (n + 255) / 256));
+    *nb = (n + *rb - 1) / *rb;
+    const uint64_t xbytes = A->n_cols * sizeof(double);
+    *S = std::min<uint64_t>(kPhaseMaxSlices,
+                            8 * std::max<uint64_t>(1, (xbytes + 8 * kPhaseSliceBytes - 1) / (8 * kPhaseSliceBytes)));
+    *width = std::max<uint64_t>(1, (A->n_cols + *S - 1) / *S);
 }
 
 template <int G, class OffT>
@@ -691,31 +937,23 @@ void ensure_xphase(spira_dcsr *A)
     if (P.ph_goff) return;
     spira_dev *dev = A->dev;
     const uint64_t n = A->n_rows;
-    // rows per block: all CUs busy on mid-sized matrices, at most kPhaseRows
-    const uint64_t rb = std::min<uint64_t>(kPhaseRows, std::max<uint64_t>(1, (n + 255) / 256));
-    const uint64_t nb = (n + rb - 1) / rb;
-    const uint64_t xbytes = A->n_cols * sizeof(double);
-    const uint64_t S = std::min<uint64_t>(
-        kPhaseMaxSlices, 8 * std::max<uint64_t>(1, (xbytes + 8 * kPhaseSliceBytes - 1) / (8 * kPhaseSliceBytes)));
-    const uint64_t width = std::max<uint64_t>(1, (A->n_cols + S - 1) / S);
+    uint64_t rb, nb, S, width;
+    xphase_geometry(A, &rb, &nb, &S, &width);
     if (width > (1ull << kPhaseColBits)) fail(SPIRA_ERR_UNSUPPORTED, "spmv: too many columns for XPHASE");
-    const uint64_t N = nb * S * rb;  // count slots
-    if (N >= (1ull << 32) || nb >= (1ull << 31))
-        fail(SPIRA_ERR_UNSUPPORTED, "spmv: matrix too large for XPHASE");
-    uint64_t *pos = (uint64_t *)dev->ws.get(WS_MISC2, (N + 1) * sizeof(uint64_t));
-    k_xphase_count<<<grid_for(nb * rb, kThreads), kThreads, 0, dev->stream>>>(n, nb * rb, (uint32_t)rb, (uint32_t)S,
-                                                                              width, A->off, A->col, pos);
-    SPIRA_HIP(hipGetLastError());
-    scan_excl_u64(dev, pos, pos, N);
+    if (nb >= (1ull << 31)) fail(SPIRA_ERR_UNSUPPORTED, "spmv: matrix too large for XPHASE");
+    const uint64_t G = nb * S;
+    const uint64_t SUB = (width + (1ull << kSubBits) - 1) >> kSubBits;
+    const uint64_t GS = G * SUB;  // sub-groups
     const uint64_t nz = A->nnz ? A->nnz : 1;
+    const uint64_t chunks = (A->nnz + kPlanChunk - 1) / kPlanChunk;
+    if (chunks >= (1ull << 31)) fail(SPIRA_ERR_UNSUPPORTED, "spmv: matrix too large for XPHASE");
     P.ph_rc = (uint32_t *)plan_alloc(A, nz * sizeof(uint32_t));
     P.ph_val = (double *)plan_alloc(A, nz * sizeof(double));
-    uint64_t *goff = (uint64_t *)plan_alloc(A, (nb * S + 1) * sizeof(uint64_t));
+    uint64_t *goff = (uint64_t *)plan_alloc(A, (G + 1) * sizeof(uint64_t));
     P.ph_sync = (uint32_t *)plan_alloc(A, 8 * kPhaseSyncStride * sizeof(uint32_t));
-    // entries grouped in row order first, then every group sorted by its
-    // column bits: the 64 entries of a wavefront load then sweep x in order,
-    // so lanes on one x line share a request and a line a group touches
-    // twice is fetched once
+    // sub-group counters, then starts (GS + 1), then cursors (GS)
+    uint64_t *gsub = (uint64_t *)dev->ws.get(WS_MISC2, (2 * GS + 1) * sizeof(uint64_t));
+    unsigned long long *cur = (unsigned long long *)(gsub + GS + 1);
     struct temp {  // released on every exit (stream order protects queued work)
         spira_dev *d;
         void *p;
@@ -723,25 +961,43 @@ void ensure_xphase(spira_dcsr *A)
         ~temp() { dev_release(d, p, bytes); }
     } t_rc{dev, dev_alloc(dev, nz * sizeof(uint32_t)), nz * sizeof(uint32_t)},
         t_val{dev, dev_alloc(dev, nz * sizeof(double)), nz * sizeof(double)};
-    uint32_t *rc0 = (uint32_t *)t_rc.p;
-    double *val0 = (double *)t_val.p;
-    k_xphase_scatter<<<grid_for(n, kThreads), kThreads, 0, dev->stream>>>(
-        n, (uint32_t)rb, (uint32_t)S, width, A->off, A->col, A->val, pos, rc0, val0);
-    SPIRA_HIP(hipGetLastError());
-    k_xphase_goff<<<grid_for(nb * S + 1, kThreads), kThreads, 0, dev->stream>>>(nb * S, (uint32_t)rb, pos, A->nnz,
-                                                                                goff);
-    SPIRA_HIP(hipGetLastError());
-    if (A->nnz > 0) {
-        size_t tb = 0;
-        SPIRA_HIP(rocprim::segmented_radix_sort_pairs(nullptr, tb, rc0, P.ph_rc, val0, P.ph_val, (unsigned)A->nnz,
-                                                      (unsigned)(nb * S), goff, goff + 1, 0, kPhaseColBits,
-                                                      dev->stream));
-        void *t = dev->ws.get(WS_SORT, tb ? tb : 8);
-        SPIRA_HIP(rocprim::segmented_radix_sort_pairs(t, tb, rc0, P.ph_rc, val0, P.ph_val, (unsigned)A->nnz,
-                                                      (unsigned)(nb * S), goff, goff + 1, 0, kPhaseColBits,
-                                                      dev->stream));
+    uint32_t *trc = (uint32_t *)t_rc.p;
+    double *tval = (double *)t_val.p;
+    hipStream_t st = dev->stream;
+    SPIRA_HIP(hipMemsetAsync(gsub, 0, (GS + 1) * sizeof(uint64_t), st));
+    uint64_t *tab = (uint64_t *)dev->ws.get(WS_CNT, (chunks + 1) * sizeof(uint64_t));
+    if (chunks > 0) {
+        k_xphase_chunk_rows<<<grid_for(n, kThreads), kThreads, 0, st>>>(A->off, n, A->nnz, tab);
+        SPIRA_HIP(hipGetLastError());
+        k_xphase_bin<false><<<(unsigned)chunks, kPlanThreads, 0, st>>>(
+            A->nnz, n, (uint32_t)rb, (uint32_t)S, (uint32_t)SUB, (uint32_t)width, A->off, A->col, A->val, tab,
+            (unsigned long long *)gsub, nullptr, nullptr);
+        SPIRA_HIP(hipGetLastError());
     }
-    dev_sync(dev);  // pos lives in the workspace: finish before it can be reused
+    scan_excl_u64(dev, gsub, gsub, GS);  // sub-group starts; gsub[GS] = nnz
+    k_xphase_goff<<<grid_for(G + 1, kThreads), kThreads, 0, st>>>(gsub, G, (uint32_t)SUB, goff);
+    SPIRA_HIP(hipGetLastError());
+    if (chunks > 0) {
+        SPIRA_HIP(hipMemcpyAsync(cur, gsub, GS * sizeof(uint64_t), hipMemcpyDeviceToDevice, st));
+        k_xphase_bin<true><<<(unsigned)chunks, kPlanThreads, 0, st>>>(A->nnz, n, (uint32_t)rb, (uint32_t)S,
+                                                                       (uint32_t)SUB, (uint32_t)width, A->off,
+                                                                       A->col, A->val, tab, cur, trc, tval);
+        SPIRA_HIP(hipGetLastError());
+        if (A->nnz >= (1ull << 32)) {
+            // a sub-group's entry count is held in 32 bits by the line pass
+            uint64_t mx = 0;
+            std::vector<uint64_t> h(GS + 1);
+            SPIRA_HIP(hipMemcpyAsync(h.data(), gsub, (GS + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+            dev_sync(dev);
+            for (uint64_t g = 0; g < GS; ++g) mx = std::max(mx, h[g + 1] - h[g]);
+            if (mx >= (1ull << 32)) fail(SPIRA_ERR_UNSUPPORTED, "spmv: an XPHASE group holds 2^32 entries or more");
+        }
+        const uint64_t grid = std::min<uint64_t>(GS, 2u * (uint64_t)std::max(dev->n_cu, 1));
+        k_xphase_lines<<<(unsigned)grid, kPlanThreads, 0, st>>>(gsub, GS, (uint32_t)SUB, (uint32_t)width, trc, tval,
+                                                                P.ph_rc, P.ph_val);
+        SPIRA_HIP(hipGetLastError());
+    }
+    dev_sync(dev);  // the cursors live in the workspace: finish before they can be reused
     P.ph_S = (uint32_t)S;
     P.ph_width = (uint32_t)width;
     P.ph_RB = (uint32_t)rb;
@@ -809,10 +1065,43 @@ void spmv_stream_blocks(spira_dcsr *A, int which, const uint32_t **blk, uint32_t
     *nblk = A->plan.nblk[which];
 }
 
-int spmv_heuristic_variant(const spira_dcsr *A)
+int spmv_auto_choice(const spira_dcsr *A)
+{
+    return A->plan.variant >= 0 ? A->plan.variant : A->plan.auto_variant;
+}
+
+int spmv_resolve_auto(spira_dcsr *A)
+{
+    spira_plan &P = A->plan;
+    ensure_stats(A);
+    if (P.variant >= 0) return P.variant;
+    if (P.auto_variant < 0) {
+        const int learned = spmv_learned_variant(A);
+        P.auto_variant = learned >= 0 ? learned : spmv_rule_variant(A);
+    }
+    return P.auto_variant;
+}
+
+bool spmv_variant_fits(const spira_dcsr *A, int v)
+{
+    if (v < 0 || v >= SPIRA_SPMV_N_VARIANTS) return false;
+    if (v >= SPIRA_SPMV_XSLICE1 && v <= SPIRA_SPMV_XSTREAM) {
+        if (A->nnz >= (1ull << 32)) return false;
+        const uint64_t G = v == SPIRA_SPMV_XSLICE1 ? 1 : v == SPIRA_SPMV_XSLICE2 ? 2 : 4;
+        return (A->n_rows * G + kThreads - 1) / kThreads * kSlices <= 0x7fffffffull;
+    }
+    if (v == SPIRA_SPMV_XPHASE || v == SPIRA_SPMV_XPHASE_PACED) {
+        uint64_t rb, nb, S, width;
+        xphase_geometry(A, &rb, &nb, &S, &width);
+        return width <= (1ull << kPhaseColBits) && nb < (1ull << 31) &&
+               (A->nnz + kPlanChunk - 1) / kPlanChunk < (1ull << 31);
+    }
+    return true;
+}
+
+int spmv_rule_variant(const spira_dcsr *A)
 {
     const spira_plan &P = A->plan;
-    if (P.variant >= 0) return P.variant;
     const double mean = P.mean_len;
     // Irregular lengths (power-law rows) or very short rows: LDS row blocks.
     if ((double)P.max_len > 4.0 * (mean > 8.0 ? mean : 8.0) || P.cv_len > 0.5) return SPIRA_SPMV_STREAM2K;
@@ -876,7 +1165,8 @@ bool spmv_plan_ready(const spira_dcsr *A, int variant)
 {
     const spira_plan &P = A->plan;
     if (!P.stats_valid || (A->nnz < (1ull << 32) && !P.off32)) return false;
-    if (variant < 0) variant = spmv_heuristic_variant(A);
+    if (variant < 0) variant = spmv_auto_choice(A);
+    if (variant < 0) return false;
     if (variant <= SPIRA_SPMV_VEC64) return true;
     if (variant == SPIRA_SPMV_STREAM1K) return P.blk[0] != nullptr;
     if (variant == SPIRA_SPMV_STREAM2K) return P.blk[1] != nullptr;
@@ -886,7 +1176,7 @@ bool spmv_plan_ready(const spira_dcsr *A, int variant)
 
 uint64_t spmv_scratch_bytes(const spira_dcsr *A, int variant)
 {
-    if (variant < 0) variant = spmv_heuristic_variant(A);
+    if (variant < 0) variant = spmv_auto_choice(A);
     if (variant == SPIRA_SPMV_XPHASE_PACED) return 8 * kPhaseSyncStride * sizeof(uint32_t);
     if (variant >= SPIRA_SPMV_XSLICE1 && variant <= SPIRA_SPMV_XSTREAM) return kSlices * A->n_rows * sizeof(double);
     return 0;
@@ -895,7 +1185,8 @@ uint64_t spmv_scratch_bytes(const spira_dcsr *A, int variant)
 void spmv_launch_planned(spira_dcsr *A, const double *x, double *y, int variant, void *scratch)
 {
     if (A->n_rows == 0) return;
-    if (variant < 0) variant = spmv_heuristic_variant(A);
+    if (variant < 0) variant = spmv_auto_choice(A);
+    if (variant < 0) fail(SPIRA_ERR_LOGIC, "spmv: AUTO's choice is not resolved for this matrix");
     launch_planned(A, x, y, variant, (uint32_t *)scratch, (double *)scratch);
 }
 
@@ -904,7 +1195,7 @@ void spmv_launch_f64(spira_dcsr *A, const double *x, double *y, int variant)
     if (A->n_rows == 0) return;
     if (A->n_rows >= (1ull << 32) - 1) fail(SPIRA_ERR_UNSUPPORTED, "spmv: n_rows must be < 2^32 - 1");
     ensure_stats(A);
-    if (variant < 0) variant = spmv_heuristic_variant(A);
+    if (variant < 0) variant = spmv_resolve_auto(A);
     if (variant == SPIRA_SPMV_STREAM1K || variant == SPIRA_SPMV_STREAM2K)
         ensure_blocks(A, variant == SPIRA_SPMV_STREAM1K ? 0 : 1);
     else if (variant >= SPIRA_SPMV_XSLICE1 && variant <= SPIRA_SPMV_XSTREAM) ensure_xslice(A);

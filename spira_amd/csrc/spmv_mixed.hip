@@ -346,7 +346,7 @@ void spmv_mixed_launch(spira_dcsr *A, int storage, const double *x, double *y, i
     spira_dev *dev = A->dev;
     hipStream_t st = dev->stream;
     if (variant < 0) {
-        variant = spmv_heuristic_variant(A);
+        variant = A->plan.variant >= 0 ? A->plan.variant : spmv_rule_variant(A);
         if (!mixed_variant_supported(variant)) variant = SPIRA_SPMV_STREAM2K;
     }
     if (!mixed_variant_supported(variant))

@@ -293,3 +293,135 @@ def lock_bytes(n_rows, nnz_old, n_staged, nnz_new, s_col=4, s_val=8):
     """Algorithmic bytes of one lock(): read old CSR + read staged + write new."""
     return (nnz_old * (s_col + s_val) + (n_rows + 1) * 8 + n_staged * (4 + 8) + (n_rows + 1) * 8
             + nnz_new * (s_col + s_val) + (n_rows + 1) * 8)
+
+
+# ---- seeded corpus of the single-matrix SpMV kernel selector ----------------
+SPMV_CORPUS_FAMILIES = ("uniform", "banded", "stencil", "powerlaw", "blockdiag", "local_mix")
+SPMV_CORPUS_MAX_NNZ = 120_000_000
+
+
+def _csr_from_entries(rowid, cols, n_rows, n_cols, g, device):
+    """Sorted, duplicate-free CSR from (row, column) draws (torch tensors)."""
+    import torch
+    key, _ = torch.sort(rowid * n_cols + cols)
+    del rowid, cols
+    key = torch.unique_consecutive(key)
+    r = key // n_cols
+    c = (key % n_cols).to(torch.int32)
+    counts = torch.bincount(r, minlength=n_rows)
+    del r, key
+    off = torch.zeros(n_rows + 1, dtype=torch.int64, device=device)
+    off[1:] = torch.cumsum(counts, 0)
+    vals = torch.rand(c.numel(), generator=g, device=device, dtype=torch.float64) * 2.0 - 1.0
+    return off, c, vals
+
+
+def spmv_corpus_matrix(seed, device="cuda", max_rows=1 << 23):
+    """One matrix of the kernel selector's corpus (family = seed % 6): uniform
+    random rows; banded random rows; 2-D 5/9-point and 3-D 7-point stencils;
+    power-law rows (Zipf lengths, a local/global column mix); block-diagonal
+    blocks with uniform/Zipf rows (the batched workload's matrices as one CSR);
+    mostly-local rows with a uniform fraction.  Sizes are log-uniform up to
+    max_rows rows and SPMV_CORPUS_MAX_NNZ entries.  Returns
+    (off, cols, vals, n_cols, family) with torch tensors on `device`."""
+    import torch
+    rng = np.random.default_rng(seed)
+    fam = SPMV_CORPUS_FAMILIES[seed % len(SPMV_CORPUS_FAMILIES)]
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed) & 0x7fffffffffffffff)
+    logn = lambda lo, hi: int(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+
+    def row_ids(lens):
+        lt = torch.from_numpy(lens.astype(np.int64)).to(device)
+        return torch.repeat_interleave(torch.arange(lens.size, device=device, dtype=torch.int64), lt)
+
+    def urand(m):
+        return torch.rand(m, generator=g, device=device, dtype=torch.float64)
+
+    if fam == "stencil":
+        kind = rng.integers(0, 3)
+        if kind < 2:  # 2-D, 5 or 9 points
+            nx = logn(64, int(np.sqrt(max_rows)))
+            ny = max(1, min(logn(64, int(np.sqrt(max_rows))), max_rows // nx))
+            n = nx * ny
+            i = torch.arange(n, device=device, dtype=torch.int64)
+            gx, gy = i % nx, i // nx
+            offs = [(0, 0), (-1, 0), (1, 0), (0, -1), (0, 1)] + ([(-1, -1), (1, -1), (-1, 1), (1, 1)] if kind else [])
+            rs, cs = [], []
+            for dx, dy in offs:
+                m = (gx + dx >= 0) & (gx + dx < nx) & (gy + dy >= 0) & (gy + dy < ny)
+                rs.append(i[m])
+                cs.append(i[m] + dx + dy * nx)
+        else:  # 3-D 7 points
+            side = logn(16, int(round(max_rows ** (1 / 3))))
+            n = side ** 3
+            i = torch.arange(n, device=device, dtype=torch.int64)
+            gx, gy, gz = i % side, (i // side) % side, i // (side * side)
+            rs, cs = [], []
+            for dx, dy, dz in ((0, 0, 0), (-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1)):
+                m = ((gx + dx >= 0) & (gx + dx < side) & (gy + dy >= 0) & (gy + dy < side) & (gz + dz >= 0)
+                     & (gz + dz < side))
+                rs.append(i[m])
+                cs.append(i[m] + dx + dy * side + dz * side * side)
+        off, c, v = _csr_from_entries(torch.cat(rs), torch.cat(cs), n, n, g, device)
+        return off, c, v, n, fam
+
+    n = logn(4096, max_rows)
+    if fam == "uniform":
+        n_cols = max(1, int(n * 2.0 ** rng.uniform(-1.0, 1.0)))
+        mean = 2.0 ** rng.uniform(0.0, 6.0)
+        lens = rng.integers(int(mean // 2), int(mean * 1.5) + 1, size=n)
+        lens = np.minimum(lens, min(n_cols, max(1, SPMV_CORPUS_MAX_NNZ // n)))
+        rid = row_ids(lens)
+        cols = (urand(rid.numel()) * n_cols).long().clamp_(max=n_cols - 1)
+    elif fam == "banded":
+        n_cols = n
+        band = logn(1, max(2, n // 4))
+        mean = 2.0 ** rng.uniform(1.0, 5.5)
+        lens = np.minimum(rng.integers(1, int(2 * mean) + 1, size=n), 2 * band + 1)
+        lens = np.minimum(lens, max(1, SPMV_CORPUS_MAX_NNZ // n))
+        rid = row_ids(lens)
+        cols = (rid + ((urand(rid.numel()) * (2 * band + 1)).long() - band)).clamp_(0, n_cols - 1)
+    elif fam in ("powerlaw", "local_mix"):
+        n_cols = n
+        band = logn(64, max(128, n // 8))
+        if fam == "powerlaw":
+            s = rng.uniform(1.5, 2.5)
+            lens = np.minimum(rng.zipf(s, size=n), min(n, 50000))
+            local = rng.uniform(0.0, 1.0)
+        else:
+            mean = 2.0 ** rng.uniform(1.0, 5.5)
+            lens = rng.integers(1, int(2 * mean) + 1, size=n)
+            local = rng.uniform(0.5, 1.0)
+        if lens.sum() > SPMV_CORPUS_MAX_NNZ:
+            lens = np.maximum(1, (lens * (SPMV_CORPUS_MAX_NNZ / lens.sum())).astype(np.int64))
+        rid = row_ids(lens)
+        lo = (rid - band).clamp_(min=0)
+        hi = (rid + band + 1).clamp_(max=n_cols)
+        loc = lo + (urand(rid.numel()) * (hi - lo).double()).long()
+        glob = (urand(rid.numel()) * n_cols).long()
+        cols = torch.where(urand(rid.numel()) < local, loc, glob).clamp_(max=n_cols - 1)
+        del lo, hi, loc, glob
+    else:  # blockdiag
+        sizes = []
+        total = 0
+        while total < n:
+            b = logn(256, 65536)
+            sizes.append(min(b, n - total))
+            total += sizes[-1]
+        n_cols = n
+        starts = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+        bstart = np.repeat(starts, sizes)
+        bsize = np.repeat(sizes, sizes)
+        uni = rng.integers(4, 65, size=n)
+        zpf = np.minimum(rng.zipf(1.6, size=n), np.minimum(bsize, 1024))
+        lens = np.minimum(np.where(rng.random(n) < 0.5, uni, zpf), bsize)
+        if lens.sum() > SPMV_CORPUS_MAX_NNZ:
+            lens = np.maximum(1, (lens * (SPMV_CORPUS_MAX_NNZ / lens.sum())).astype(np.int64))
+        rid = row_ids(lens)
+        bs = torch.from_numpy(bstart).to(device)[rid]
+        bz = torch.from_numpy(bsize.astype(np.int64)).to(device)[rid]
+        cols = bs + (urand(rid.numel()) * bz.double()).long().clamp_(max=bz - 1)
+        del bs, bz
+    off, c, v = _csr_from_entries(rid, cols, n, n_cols, g, device)
+    return off, c, v, n_cols, fam

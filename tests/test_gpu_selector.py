@@ -145,8 +145,9 @@ def _selector_or_skip(dev):
 def test_trained_selector_beats_heuristic_on_fresh_matrices(dev):
     """Fresh seeds (not in the training or held-out corpus), as one batch:
     the selector's schedules reach >= 90% of the exhaustive autotuner's best
-    in-batch throughput (costs summed over the batch) and do at least as well
-    as the built-in schedule."""
+    in-batch throughput (costs summed over the batch) and beat the built-in
+    schedule by at least 3 points of that fraction (what the network gains
+    over the method without it)."""
     sel = _selector_or_skip(dev)
     mats = S.build_corpus(dev, 64, 5_000_000)
     F, T, K, MI, nc = S.collect(dev, mats, reps=2)
@@ -158,4 +159,4 @@ def test_trained_selector_beats_heuristic_on_fresh_matrices(dev):
     f_heur = best.sum() / T[rows, heur].sum()
     print(f"selector {f_sel:.3f} built-in {f_heur:.3f} (fraction of the exhaustive best)")
     assert f_sel >= 0.9
-    assert f_sel >= f_heur - 0.02
+    assert f_sel >= f_heur + 0.03

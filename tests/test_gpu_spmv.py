@@ -226,3 +226,48 @@ def test_device_vectors_from_torch_stream_without_sync(dev):
         A.spmv(x, y.float(), device=True)
     with pytest.raises(spira_amd.InvalidArgument, match="float64"):
         A.spmv_mixed(x.float(), y, device=True)
+
+
+def test_spmv_random_4M_full_size(dev):
+    """The HBM-scale random matrix at full size (4,194,304 rows, 8-32 uniform
+    distinct columns per row, seed 43): the XPHASE variants -- XPHASE_PACED
+    with its whole grid resident and its phases paced -- and AUTO, against the
+    oracle at 1e-12 normwise plus the per-row bound."""
+    n = 1 << 22
+    off, col, val = W.random_uniform_rows(n, n, 8, 32, seed=43)
+    A = _upload(dev, off, col, val, n)
+    x = W.dense_x(n)
+    for variant in (spira_amd.VARIANTS["xphase_paced"], spira_amd.VARIANTS["xphase"], spira_amd.SPMV_AUTO):
+        print(f"random 4M: variant {variant}", flush=True)
+        _check(A.spmv(x, variant=variant), off, col, val, x)
+
+
+def test_autotune_pins_the_fastest_and_skips_what_cannot_run(dev):
+    """spira_spmv_autotune_f64: every variant is timed, the fastest is pinned
+    and AUTO runs it afterwards; a variant whose size limits the matrix
+    exceeds (XPHASE slices hold at most 2^18 columns, 256 slices: 67,108,864
+    columns) is reported as -1 and never chosen."""
+    import torch
+    for n_rows, n_cols, seed in ((300_000, 300_000, 31), (3000, 70_000_000, 32)):
+        print(f"autotune: {n_rows} x {n_cols}", flush=True)
+        off, col, val = W.random_uniform_rows(n_rows, n_cols, 4, 24, seed=seed)
+        A = _upload(dev, off, col, val, n_cols)
+        xh = np.random.default_rng(seed).uniform(-1, 1, n_cols)
+        x = torch.from_numpy(xh).cuda()
+        y = torch.empty(n_rows, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        best, ms = A.autotune(x, y, reps=3)
+        ran = {v: t for v, t in ms.items() if t >= 0}
+        assert best == min(ran, key=ran.get)
+        skipped = {v for v, t in ms.items() if t < 0}
+        if n_cols > (1 << 26):
+            assert skipped == {spira_amd.VARIANTS["xphase"], spira_amd.VARIANTS["xphase_paced"]}
+        else:
+            assert not skipped
+        assert A.choose_variant() == best
+        y_auto = A.spmv(x, torch.empty_like(y), device=True).cpu().numpy()
+        y_best = A.spmv(x, torch.empty_like(y), variant=best, device=True).cpu().numpy()
+        if best not in (spira_amd.VARIANTS["xphase"], spira_amd.VARIANTS["xphase_paced"]):
+            assert np.array_equal(y_auto.view(np.uint64), y_best.view(np.uint64))
+        _check(y_auto, off, col, val, xh)
+        A.close()
