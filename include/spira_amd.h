@@ -42,6 +42,9 @@ typedef struct spira_dcsr spira_dcsr; /* device CSR: u64 offsets, u32 cols, f64 
 #define SPIRA_OUT_DEVICE 8u /* download destinations are device pointers */
 #define SPIRA_NO_SYNC 16u   /* return without synchronising the stream
                                (device pointers only) */
+#define SPIRA_REPRODUCIBLE 32u /* SpMV: bit-identical y on every call with the
+                                  same A and x (AUTO keeps to variants whose
+                                  summation order is fixed) */
 
 /* ---- SpMV kernel variants ------------------------------------------------ */
 #define SPIRA_SPMV_AUTO (-1)      /* cached choice, else the built-in heuristic */
@@ -142,8 +145,17 @@ int spira_lock_staged_f64(spira_dcsr *A, spira_stage *s, const uint8_t *dirty, u
  * give +0.0.  Checks in the reference's order: x_len != n_cols and
  * y_len != n_rows give SPIRA_ERR_INVALID_ARGUMENT, an unbuilt A gives
  * SPIRA_ERR_LOGIC.  FP64 throughout; normwise relative error vs. the
- * sequential FP64 loop <= 1e-12.  variant: SPIRA_SPMV_*.
- * flags: SPIRA_X_DEVICE | SPIRA_Y_DEVICE | SPIRA_NO_SYNC. */
+ * sequential FP64 loop <= 1e-12.  variant: SPIRA_SPMV_*; SPIRA_SPMV_AUTO is
+ * the pinned variant (spira_spmv_set_variant, the autotuner) or else the
+ * learned selector's choice for A (computed on the device once per locked
+ * matrix).  Every variant except XPHASE and XPHASE_PACED adds each row's
+ * products in a fixed order, so y is the same bits on every call; the XPHASE
+ * variants add into LDS with atomics, whose order (and so the last bits of
+ * y) can change between calls.  With SPIRA_REPRODUCIBLE, AUTO keeps to the
+ * fixed-order variants (the fastest of them by the autotuner's times, else
+ * by the selector) and an explicit XPHASE variant is refused
+ * (SPIRA_ERR_INVALID_ARGUMENT).
+ * flags: SPIRA_X_DEVICE | SPIRA_Y_DEVICE | SPIRA_NO_SYNC | SPIRA_REPRODUCIBLE. */
 int spira_spmv_f64(spira_dcsr *A, const double *x, uint64_t x_len, double *y,
                    uint64_t y_len, int variant, uint32_t flags);
 

@@ -369,8 +369,15 @@ public:
             std::fill(y.begin(), y.end(), 0.0);
             return;
         }
-        check(spira_spmv_f64(csr_->get(), x.data(), x.size(), y.data(), y.size(), variant, SPIRA_PTR_HOST));
+        check(spira_spmv_f64(csr_->get(), x.data(), x.size(), y.data(), y.size(), variant,
+                             reproducible_ ? SPIRA_REPRODUCIBLE : SPIRA_PTR_HOST));
     }
+
+    // Bit-identical y on every SpMV with the same matrix and x (the kernel
+    // choice keeps to variants whose summation order is fixed), as the
+    // reference's per-row loop gives; off by default (the fastest variant).
+    void set_reproducible(bool on) { reproducible_ = on; }
+    bool reproducible() const { return reproducible_; }
 
 protected:
     void check(int rc) const { dev_->check(rc); }
@@ -449,6 +456,7 @@ protected:
     std::vector<uint8_t> dirty_;                                       // matrix::dirty_
     std::unique_ptr<device_csr> csr_;
     bool csr_built_ = false;
+    bool reproducible_ = false;
     size_type committed_nnz_ = 0;
     mutable std::optional<csr_storage> host_;  // downloaded copy of the committed CSR
 };

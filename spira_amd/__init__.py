@@ -51,6 +51,7 @@ Y_DEVICE = 2
 IN_DEVICE = 4
 OUT_DEVICE = 8
 NO_SYNC = 16
+REPRODUCIBLE = 32
 
 SPMV_AUTO = -1
 VARIANTS = {
@@ -420,21 +421,28 @@ class DeviceCSR:
         _producers_done(self.dev, x, y)
         return x, y, x.numel(), y.numel(), X_DEVICE | Y_DEVICE
 
-    def spmv(self, x, y=None, variant: int = SPMV_AUTO, device: bool = False, sync: bool = True):
-        """y = A*x.  Host numpy arrays, or device tensors with device=True."""
+    def spmv(self, x, y=None, variant: int = SPMV_AUTO, device: bool = False, sync: bool = True,
+             reproducible: bool = False):
+        """y = A*x.  Host numpy arrays, or device tensors with device=True.
+        reproducible: bit-identical y on every call (AUTO keeps to variants
+        whose summation order is fixed; an XPHASE variant is refused)."""
         x, y, xl, yl, flags = self._vectors(x, y, device)
         if device and not sync:
             flags |= NO_SYNC
+        if reproducible:
+            flags |= REPRODUCIBLE
         self.dev.check(lib().spira_spmv_f64(self._h, _ptr(x), xl, _ptr(y), yl, int(variant), flags))
         return y
 
-    def spmv_noalloc(self, x, y, variant: int = SPMV_AUTO, sync: bool = True):
+    def spmv_noalloc(self, x, y, variant: int = SPMV_AUTO, sync: bool = True, reproducible: bool = False):
         """y = A*x through spira_spmv_f64_noalloc: device tensors only, no
         allocation (the plan for the variant must exist: one spmv() call with
         the variant after the last lock builds it)."""
         x, y, xl, yl, flags = self._vectors(x, y, True)
         if not sync:
             flags |= NO_SYNC
+        if reproducible:
+            flags |= REPRODUCIBLE
         self.dev.check(lib().spira_spmv_f64_noalloc(self._h, _ptr(x), xl, _ptr(y), yl, int(variant), flags))
         return y
 
@@ -1145,10 +1153,11 @@ def spgemm(A: Matrix, B: Matrix) -> Matrix:
     return C
 
 
-def spmv(mat: Matrix, x, y, variant: int = SPMV_AUTO):
+def spmv(mat: Matrix, x, y, variant: int = SPMV_AUTO, reproducible: bool = False):
     """serial::algorithms::spmv / parallel::algorithms::spmv (FP64, device).
 
-    x and y are numpy float64 arrays (host) or torch float64 CUDA tensors."""
+    x and y are numpy float64 arrays (host) or torch float64 CUDA tensors.
+    reproducible: bit-identical y on every call with the same matrix and x."""
     is_dev = hasattr(x, "numel") or hasattr(y, "numel")  # torch tensors: checked as device vectors below
     xl = int(x.numel()) if hasattr(x, "numel") else int(np.asarray(x).size)
     yl = int(y.numel()) if hasattr(y, "numel") else int(np.asarray(y).size)
@@ -1159,9 +1168,10 @@ def spmv(mat: Matrix, x, y, variant: int = SPMV_AUTO):
     if not mat.is_locked():
         raise LogicError("spmv: matrix must be locked")
     if is_dev:
-        mat.device_csr().spmv(x, y, variant=variant, device=True)
+        mat.device_csr().spmv(x, y, variant=variant, device=True, reproducible=reproducible)
     else:
         if not (isinstance(y, np.ndarray) and y.dtype == np.float64 and y.flags.c_contiguous):
             raise InvalidArgument("spmv: y must be a contiguous float64 array")
-        mat.device_csr().spmv(np.ascontiguousarray(x, dtype=np.float64), y, variant=variant)
+        mat.device_csr().spmv(np.ascontiguousarray(x, dtype=np.float64), y, variant=variant,
+                              reproducible=reproducible)
     return y

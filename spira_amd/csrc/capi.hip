@@ -352,7 +352,7 @@ int spira_spmv_f64(spira_dcsr *A, const double *x, uint64_t x_len, double *y, ui
             dx = t;
         }
         if (!yd) dy = (double *)dev->ws.get(WS_Y, y_len * sizeof(double));
-        spmv_launch_f64(A, dx, dy, variant);
+        spmv_launch_f64(A, dx, dy, variant, (flags & SPIRA_REPRODUCIBLE) != 0);
         if (!yd && y_len)
             SPIRA_HIP(hipMemcpyAsync(y, dy, y_len * sizeof(double), hipMemcpyDeviceToHost, dev->stream));
         if (!(flags & SPIRA_NO_SYNC) || !xd || !yd) dev_sync(dev);
@@ -385,7 +385,11 @@ int spira_spmv_f64_noalloc(spira_dcsr *A, const double *x, uint64_t x_len, doubl
         const char *unprepared =
             "spmv_noalloc: the matrix has no SpMV plan for this variant (one spira_spmv_f64 call with the "
             "variant after the last lock or upload builds it)";
-        const int v = variant < 0 ? spmv_auto_choice(A) : variant;
+        const bool repro = (flags & SPIRA_REPRODUCIBLE) != 0;
+        if (repro && variant >= 0 && !spmv_order_fixed(variant))
+            fail(SPIRA_ERR_INVALID_ARGUMENT,
+                 "spmv: XPHASE variants add in a varying order; drop SPIRA_REPRODUCIBLE or choose another variant");
+        const int v = variant < 0 ? spmv_auto_choice(A, repro) : variant;
         if (v < 0 || !A->plan.stats_valid) fail(SPIRA_ERR_LOGIC, unprepared);
         const uint64_t need = spmv_scratch_bytes(A, v);
         if (need > kNoAllocScratchBytes)
@@ -463,8 +467,10 @@ int spira_spmv_autotune_f64(spira_dcsr *A, const double *x_dev, double *y_dev, i
         SPIRA_HIP(hipEventCreate(&e1));
         int best_v = -1;
         float best_ms = 0.0f;
+        float t_ms[SPIRA_SPMV_N_VARIANTS];
         try {
             for (int v = 0; v < SPIRA_SPMV_N_VARIANTS; ++v) {
+                t_ms[v] = -1.0f;
                 try {
                     spmv_launch_f64(A, x_dev, y_dev, v);  // warm-up (builds the plan parts)
                 } catch (const status_error &e) {
@@ -480,6 +486,7 @@ int spira_spmv_autotune_f64(spira_dcsr *A, const double *x_dev, double *y_dev, i
                 SPIRA_HIP(hipEventElapsedTime(&t, e0, e1));
                 t /= (float)reps;
                 if (ms) ms[v] = t;
+                t_ms[v] = t;
                 if (best_v < 0 || t < best_ms) {
                     best_ms = t;
                     best_v = v;
@@ -493,6 +500,10 @@ int spira_spmv_autotune_f64(spira_dcsr *A, const double *x_dev, double *y_dev, i
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         A->plan.variant = best_v;
+        A->plan.tuned = true;
+        for (int v = 0; v < SPIRA_SPMV_N_VARIANTS; ++v) A->plan.tuned_ms[v] = t_ms[v];
+        A->plan.auto_variant = -1;  // AUTO's own choices now follow the measured times
+        A->plan.auto_repro = -1;
         if (best) *best = best_v;
     });
 }

@@ -287,6 +287,11 @@ struct spira_plan {
     uint32_t ph_S = 0, ph_RB = 0, ph_nb = 0, ph_width = 0;
     int32_t variant = -1;        // pinned AUTO choice (-1 = none)
     int32_t auto_variant = -1;   // AUTO's own choice once resolved (learned selector, else the rule)
+    int32_t auto_repro = -1;     // ... among the variants with a fixed summation order
+    bool scored = false;         // sel_score holds the selector's outputs for this matrix
+    float sel_score[SPIRA_SPMV_N_VARIANTS] = {};
+    bool tuned = false;          // tuned_ms holds the autotuner's times (-1: did not run)
+    float tuned_ms[SPIRA_SPMV_N_VARIANTS] = {};
     std::vector<std::pair<void *, size_t>> owned;  // every plan allocation
 };
 

@@ -15,18 +15,23 @@ void spmv_free_plan(spira_dcsr *A);
 // AUTO's choice: the pinned variant, else the resolved learned/rule choice
 // (-1 while unresolved; no device work).  spmv_resolve_auto resolves it
 // (features + the selector network on the device, once per locked matrix).
-int spmv_auto_choice(const spira_dcsr *A);
-int spmv_resolve_auto(spira_dcsr *A);
+// With `repro`, only variants whose summation order is fixed (not XPHASE*).
+int spmv_auto_choice(const spira_dcsr *A, bool repro = false);
+int spmv_resolve_auto(spira_dcsr *A, bool repro = false);
+inline bool spmv_order_fixed(int variant)
+{
+    return variant != SPIRA_SPMV_XPHASE && variant != SPIRA_SPMV_XPHASE_PACED;
+}
 // The rule used when no trained selector is compiled in (row-length stats).
 int spmv_rule_variant(const spira_dcsr *A);
 // Whether A is within `variant`'s size limits (no SPIRA_ERR_UNSUPPORTED).
 bool spmv_variant_fits(const spira_dcsr *A, int variant);
 
-// selector.hip: the features of a locked CSR; the learned AUTO variant (-1
-// when no trained model is compiled in).
+// selector.hip: the features of a locked CSR; the selector network's scores
+// for A into A->plan.sel_score (false when no trained model is compiled in).
 void matrix_features_host(spira_dcsr *A, float *features);
-int spmv_learned_variant(spira_dcsr *A);
-void spmv_launch_f64(spira_dcsr *A, const double *x, double *y, int variant);
+bool spmv_selector_scores(spira_dcsr *A);
+void spmv_launch_f64(spira_dcsr *A, const double *x, double *y, int variant, bool repro = false);
 // The allocation-free launch: true when A's plan already holds what `variant`
 // (AUTO resolved to the pinned or heuristic choice) needs; the bytes of
 // scratch it takes (XPHASE_PACED counters, XSLICE/XSTREAM partial sums); and

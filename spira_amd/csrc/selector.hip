@@ -287,14 +287,15 @@ namespace {
 #include "spmv_selector_weights.inc"
 }  // namespace
 
-// AUTO's kernel variant for a single matrix: the trained network's outputs
-// (predicted log2 slowdown of every variant against the best) for A's
-// features, lowest among the variants whose size limits A fits.  The forward
-// pass runs on the device (k_dense_fwd); the parameters are compiled in from
-// the trained model.  -1 when no trained model was compiled in.
-int spmv_learned_variant(spira_dcsr *A)
+// The trained network's outputs (predicted log2 slowdown of every variant
+// against the best) for A's features, kept in A's plan; AUTO takes the lowest
+// among the variants A fits.  The forward pass runs on the device
+// (k_dense_fwd); the parameters are compiled in from the trained model.
+// false when no trained model was compiled in.
+bool spmv_selector_scores(spira_dcsr *A)
 {
-    if (!kSpmvSelTrained || A->n_rows == 0) return -1;
+    if (!kSpmvSelTrained || A->n_rows == 0) return false;
+    if (A->plan.scored) return true;
     static_assert(kSpmvSelIn == kFeat && kSpmvSelOut == SPIRA_SPMV_N_VARIANTS, "selector model shape");
     spira_dev *dev = A->dev;
     hipStream_t st = dev->stream;
@@ -316,13 +317,10 @@ int spmv_learned_variant(spira_dcsr *A)
         po += (size_t)din * dout + dout;
         ao += din;
     }
-    float out[kSpmvSelOut];
-    SPIRA_HIP(hipMemcpyAsync(out, act + ao, sizeof(out), hipMemcpyDeviceToHost, st));
+    SPIRA_HIP(hipMemcpyAsync(A->plan.sel_score, act + ao, kSpmvSelOut * sizeof(float), hipMemcpyDeviceToHost, st));
     dev_sync(dev);
-    int best = -1;
-    for (int v = 0; v < (int)kSpmvSelOut; ++v)
-        if (spmv_variant_fits(A, v) && (best < 0 || out[v] < out[best])) best = v;
-    return best;
+    A->plan.scored = true;
+    return true;
 }
 
 }  // namespace spira_amd

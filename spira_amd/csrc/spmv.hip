@@ -1065,21 +1065,34 @@ void spmv_stream_blocks(spira_dcsr *A, int which, const uint32_t **blk, uint32_t
     *nblk = A->plan.nblk[which];
 }
 
-int spmv_auto_choice(const spira_dcsr *A)
+int spmv_auto_choice(const spira_dcsr *A, bool repro)
 {
-    return A->plan.variant >= 0 ? A->plan.variant : A->plan.auto_variant;
+    const spira_plan &P = A->plan;
+    if (P.variant >= 0 && (!repro || spmv_order_fixed(P.variant))) return P.variant;
+    return repro ? P.auto_repro : P.auto_variant;
 }
 
-int spmv_resolve_auto(spira_dcsr *A)
+int spmv_resolve_auto(spira_dcsr *A, bool repro)
 {
     spira_plan &P = A->plan;
     ensure_stats(A);
-    if (P.variant >= 0) return P.variant;
-    if (P.auto_variant < 0) {
-        const int learned = spmv_learned_variant(A);
-        P.auto_variant = learned >= 0 ? learned : spmv_rule_variant(A);
+    if (P.variant >= 0 && (!repro || spmv_order_fixed(P.variant))) return P.variant;
+    int32_t &slot = repro ? P.auto_repro : P.auto_variant;
+    if (slot >= 0) return slot;
+    int best = -1;
+    if (P.tuned) {  // the autotuner's times, where it has run on this matrix
+        for (int v = 0; v < SPIRA_SPMV_N_VARIANTS; ++v)
+            if (P.tuned_ms[v] >= 0 && (!repro || spmv_order_fixed(v)) && (best < 0 || P.tuned_ms[v] < P.tuned_ms[best]))
+                best = v;
     }
-    return P.auto_variant;
+    if (best < 0 && spmv_selector_scores(A)) {
+        for (int v = 0; v < SPIRA_SPMV_N_VARIANTS; ++v)
+            if (spmv_variant_fits(A, v) && (!repro || spmv_order_fixed(v)) &&
+                (best < 0 || P.sel_score[v] < P.sel_score[best]))
+                best = v;
+    }
+    slot = best >= 0 ? best : spmv_rule_variant(A);
+    return slot;
 }
 
 bool spmv_variant_fits(const spira_dcsr *A, int v)
@@ -1099,6 +1112,7 @@ bool spmv_variant_fits(const spira_dcsr *A, int v)
     return true;
 }
 
+// The rule never picks an XPHASE variant (so it also serves reproducible AUTO).
 int spmv_rule_variant(const spira_dcsr *A)
 {
     const spira_plan &P = A->plan;
@@ -1190,12 +1204,15 @@ void spmv_launch_planned(spira_dcsr *A, const double *x, double *y, int variant,
     launch_planned(A, x, y, variant, (uint32_t *)scratch, (double *)scratch);
 }
 
-void spmv_launch_f64(spira_dcsr *A, const double *x, double *y, int variant)
+void spmv_launch_f64(spira_dcsr *A, const double *x, double *y, int variant, bool repro)
 {
+    if (repro && variant >= 0 && !spmv_order_fixed(variant))
+        fail(SPIRA_ERR_INVALID_ARGUMENT,
+             "spmv: XPHASE variants add in a varying order; drop SPIRA_REPRODUCIBLE or choose another variant");
     if (A->n_rows == 0) return;
     if (A->n_rows >= (1ull << 32) - 1) fail(SPIRA_ERR_UNSUPPORTED, "spmv: n_rows must be < 2^32 - 1");
     ensure_stats(A);
-    if (variant < 0) variant = spmv_resolve_auto(A);
+    if (variant < 0) variant = spmv_resolve_auto(A, repro);
     if (variant == SPIRA_SPMV_STREAM1K || variant == SPIRA_SPMV_STREAM2K)
         ensure_blocks(A, variant == SPIRA_SPMV_STREAM1K ? 0 : 1);
     else if (variant >= SPIRA_SPMV_XSLICE1 && variant <= SPIRA_SPMV_XSTREAM) ensure_xslice(A);

@@ -271,3 +271,40 @@ def test_autotune_pins_the_fastest_and_skips_what_cannot_run(dev):
             assert np.array_equal(y_auto.view(np.uint64), y_best.view(np.uint64))
         _check(y_auto, off, col, val, xh)
         A.close()
+
+
+def test_reproducible_mode_and_xphase_call_to_call(dev):
+    """SPIRA_REPRODUCIBLE: AUTO keeps to the variants whose summation order is
+    fixed (the fastest of them by the autotuner's times once it has run), y
+    is bit-identical on every call, an explicit XPHASE variant is refused.
+    Without it the XPHASE variants may differ between calls in the last
+    bits only (normwise within 1e-15 of each other)."""
+    import torch
+    n = 1 << 20
+    off, col, val = W.random_uniform_rows(n, n, 8, 32, seed=61)
+    A = _upload(dev, off, col, val, n)
+    xh = W.dense_x(n)
+    x = torch.from_numpy(xh).cuda()
+    ys = []
+    for _ in range(3):
+        y = torch.empty(n, dtype=torch.float64, device="cuda")
+        A.spmv(x, y, device=True, reproducible=True)
+        ys.append(y.cpu().numpy())
+    for y in ys[1:]:
+        assert np.array_equal(y.view(np.uint64), ys[0].view(np.uint64))
+    _check(ys[0], off, col, val, xh)
+    with pytest.raises(spira_amd.InvalidArgument, match="varying order"):
+        A.spmv(x, torch.empty_like(x), variant=spira_amd.VARIANTS["xphase_paced"], device=True, reproducible=True)
+    best, ms = A.autotune(x, torch.empty_like(x), reps=3)
+    y = torch.empty_like(x)
+    A.spmv(x, y, device=True, reproducible=True)
+    fixed = {v: t for v, t in ms.items() if t >= 0 and v not in (spira_amd.VARIANTS["xphase"],
+                                                                  spira_amd.VARIANTS["xphase_paced"])}
+    fastest_fixed = min(fixed, key=fixed.get)
+    y_ref = A.spmv(x, torch.empty_like(x), variant=fastest_fixed, device=True)
+    assert torch.equal(y, y_ref)
+    for name in ("xphase", "xphase_paced"):
+        a = A.spmv(x, torch.empty_like(x), variant=spira_amd.VARIANTS[name], device=True).cpu().numpy()
+        b = A.spmv(x, torch.empty_like(x), variant=spira_amd.VARIANTS[name], device=True).cpu().numpy()
+        assert oracle.normwise_rel_err(a, b) <= 1e-15
+        _check(a, off, col, val, xh)
