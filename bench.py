@@ -277,22 +277,24 @@ def run_mixed(args, np, torch, spira_amd, W, dev, stream):
            "storage": {}}
     sval = {"f32": 4, "bf16": 2, "split_bf16": 4}
     for st in ("f32", "bf16", "split_bf16"):
-        # fastest of the mixed kernels for this storage
+        # fastest of the mixed kernels for this storage (the FP64 recompute
+        # needs a row-major kernel)
         times = {}
         for v in spira_amd.MIXED_VARIANTS:
             A.spmv_mixed(x, y, storage=st, variant=v, device=True, sync=True, count=False)
             times[v] = _time_calls(torch, stream, lambda: A.spmv_mixed(x, y, storage=st, variant=v, device=True,
                                                                        sync=False, count=False), 5)
-        v = min(times, key=times.get)
         bts = W.spmv_bytes(n, n, nnz, s_val=sval[st])
-        rec = {"variant": spira_amd.VARIANT_NAMES[v], "bytes_per_call": bts}
+        rec = {"bytes_per_call": bts}
         for ratio in (0.0, args.recompute_ratio):
+            allowed = spira_amd.MIXED_VARIANTS if ratio == 0.0 else spira_amd.ROW_MAJOR_MIXED_VARIANTS
+            v = min(allowed, key=times.get)
             _, nrec = A.spmv_mixed(x, y, storage=st, variant=v, recompute_ratio=ratio, device=True, sync=True)
             err = _nrm_err(torch, y, y64)
             t = _time_calls(torch, stream, lambda: A.spmv_mixed(x, y, storage=st, variant=v, recompute_ratio=ratio,
                                                                 device=True, sync=False, count=False), reps)
             rec["recompute_off" if ratio == 0.0 else f"recompute_ratio_{ratio:g}"] = {
-                "ms": round(t, 4), "GB_s": round(bts / t / 1e6, 1), "GFLOP_s": round(2 * nnz / t / 1e6, 1),
+                "variant": spira_amd.VARIANT_NAMES[v], "ms": round(t, 4), "GB_s": round(bts / t / 1e6, 1), "GFLOP_s": round(2 * nnz / t / 1e6, 1),
                 "speedup_vs_fp64": round(t64 / t, 3), "rows_recomputed": nrec,
                 "normwise_rel_err_vs_fp64": err}
         out["storage"][st] = rec
@@ -793,9 +795,9 @@ def main():
             mixed_detail["fp64"]["normwise_rel_err_vs_fp64_oracle"] = oracle.normwise_rel_err(g64, y_or)
             mixed_detail["fp64"]["normwise_rel_err_vs_double_double"] = oracle.normwise_rel_err(g64, dd_hi, dd_lo)
             for st, rec in mixed_detail["storage"].items():
-                v = spira_amd.VARIANTS[rec["variant"]]
                 for key, ratio in (("recompute_off", 0.0), (f"recompute_ratio_{args.recompute_ratio:g}",
                                                              args.recompute_ratio)):
+                    v = spira_amd.VARIANTS[rec[key]["variant"]]
                     A.spmv_mixed(x, y, storage=st, variant=v, recompute_ratio=ratio, device=True, sync=True)
                     got = y.cpu().numpy()
                     rec[key]["normwise_rel_err_vs_fp64_oracle"] = oracle.normwise_rel_err(got, y_or)

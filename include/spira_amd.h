@@ -159,6 +159,20 @@ int spira_lock_staged_f64(spira_dcsr *A, spira_stage *s, const uint8_t *dirty, u
 int spira_spmv_f64(spira_dcsr *A, const double *x, uint64_t x_len, double *y,
                    uint64_t y_len, int variant, uint32_t flags);
 
+/* ---- SpMV of the float instantiation ----------------------------------------
+ * serial::algorithms::spmv / parallel::algorithms::spmv for
+ * <soa_tag, uint32_t, float> (serial/spmv.hpp:90-135,
+ * parallel/algorithms/spmv.hpp:85-119): y = A*x with x and y FP32 and A's
+ * values used in FP32 (a float matrix's values are exact in the handle's
+ * FP64 storage).  Checks and flags as spira_spmv_f64.  Each row is summed
+ * more accurately than the reference's FP32 loop (compensated FP32 in the VEC
+ * and STREAM kernels, FP64 in XPHASE) and rounded once to FP32: normwise
+ * relative error <= 1e-6 against the FP64 sum of the same FP32 data.
+ * variant: SPIRA_SPMV_AUTO (the FP64 AUTO choice mapped onto these kernels),
+ * VEC4..VEC32, STREAM1K, STREAM2K, XPHASE, XPHASE_PACED. */
+int spira_spmv_f32(spira_dcsr *A, const float *x, uint64_t x_len, float *y, uint64_t y_len, int variant,
+                   uint32_t flags);
+
 /* ---- allocation-free SpMV ----------------------------------------------------
  * No counterpart in the reference.  For callers that cannot allocate device
  * memory: code under HIP graph capture, tight serving loops, harnesses that
@@ -256,9 +270,12 @@ int spira_spgemm_f64(spira_dcsr *A, spira_dcsr *B, spira_dcsr *C);
  * whose cancellation ratio sum|a*x| / |sum a*x| exceeds recompute_ratio (or
  * whose result is not finite) are recomputed in FP64 from the FP64 values and
  * x; recompute_ratio <= 0 disables that pass.  variant: SPIRA_SPMV_AUTO,
- * VEC4..VEC32, STREAM1K or STREAM2K.  *n_recomputed (may be NULL; reading it
- * synchronises) receives the number of recomputed rows.  Checks and flags as
- * spira_spmv_f64. */
+ * VEC4..VEC32, STREAM1K, STREAM2K, or (without the recompute) XPHASE and
+ * XPHASE_PACED, which read the reduced values in the XPHASE plan's order and
+ * sum each row in FP64 (an FP32 value times the FP64 x, added in LDS).
+ * AUTO follows the FP64 AUTO choice (row-major kernels when the recompute is
+ * on).  *n_recomputed (may be NULL; reading it synchronises) receives the
+ * number of recomputed rows.  Checks and flags as spira_spmv_f64. */
 int spira_spmv_mixed_f64(spira_dcsr *A, int storage, const double *x, uint64_t x_len, double *y, uint64_t y_len,
                          int variant, float recompute_ratio, uint32_t flags, uint64_t *n_recomputed);
 

@@ -373,6 +373,21 @@ public:
                              reproducible_ ? SPIRA_REPRODUCIBLE : SPIRA_PTR_HOST));
     }
 
+    // The float instantiation's SpMV (FP32 x and y; the values, inserted as
+    // floats, are exact in the device's FP64 storage), same checks.
+    void spmv_into(const std::vector<float> &x, std::vector<float> &y, const char *x_msg, const char *y_msg,
+                   int variant = SPIRA_SPMV_AUTO) const
+    {
+        if (x.size() != cols_) throw std::invalid_argument(x_msg);
+        if (y.size() != rows_) throw std::invalid_argument(y_msg);
+        if (!is_locked()) throw std::logic_error("spmv: matrix must be locked");
+        if (!csr_built_) {
+            std::fill(y.begin(), y.end(), 0.0f);
+            return;
+        }
+        check(spira_spmv_f32(csr_->get(), x.data(), x.size(), y.data(), y.size(), variant, SPIRA_PTR_HOST));
+    }
+
     // Bit-identical y on every SpMV with the same matrix and x (the kernel
     // choice keeps to variants whose summation order is fixed), as the
     // reference's per-row loop gives; off by default (the fastest variant).
@@ -551,6 +566,13 @@ inline void spmv(const matrix &mat, const std::vector<double> &x, std::vector<do
                   "The size of the output vector y does not match the number of rows of the matrix");
 }
 
+// serial/spmv.hpp:90-135 (the float instantiation)
+inline void spmv(const matrix &mat, const std::vector<float> &x, std::vector<float> &y)
+{
+    mat.spmv_into(x, y, "The size of the input vector x does not match the number of columns of the matrix",
+                  "The size of the output vector y does not match the number of rows of the matrix");
+}
+
 // serial/transpose.hpp:35-157: a new locked matrix holding A^T (built on the
 // device; rows sorted, values as stored).
 inline matrix transpose(const matrix &mat)
@@ -588,6 +610,13 @@ namespace parallel::algorithms {
 // parallel/algorithms/spmv.hpp:127-161 (one launch: the partitions are row
 // ranges of the device CSR)
 inline void spmv(const parallel_matrix &mat, const std::vector<double> &x, std::vector<double> &y)
+{
+    mat.spmv_into(x, y, "spmv: x size does not match matrix column count",
+                  "spmv: y size does not match matrix row count");
+}
+
+// parallel/algorithms/spmv.hpp:85-119 (the float instantiation)
+inline void spmv(const parallel_matrix &mat, const std::vector<float> &x, std::vector<float> &y)
 {
     mat.spmv_into(x, y, "spmv: x size does not match matrix column count",
                   "spmv: y size does not match matrix row count");

@@ -61,7 +61,9 @@ VARIANTS = {
 }
 VARIANT_NAMES = {v: k for k, v in VARIANTS.items()}
 STORAGE = {"f32": 1, "bf16": 2, "split_bf16": 3}
-MIXED_VARIANTS = [VARIANTS[k] for k in ("vec4", "vec8", "vec16", "vec32", "stream1k", "stream2k")]
+MIXED_VARIANTS = [VARIANTS[k] for k in ("vec4", "vec8", "vec16", "vec32", "stream1k", "stream2k", "xphase",
+                                         "xphase_paced")]
+ROW_MAJOR_MIXED_VARIANTS = MIXED_VARIANTS[:6]  # the ones that can run the FP64 recompute
 N_VARIANTS = 15
 BATCH_XLDS_FLOATS = 36864
 BATCH_XLDS_TWO_PASS_COLS = 65536
@@ -168,6 +170,8 @@ def lib():
             "spira_dcsr_download_f64": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_uint32]),
             "spira_lock_f64": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_uint32, _u64p]),
             "spira_spmv_f64": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64, _vp, ctypes.c_uint64, ctypes.c_int,
+                                              ctypes.c_uint32]),
+            "spira_spmv_f32": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64, _vp, ctypes.c_uint64, ctypes.c_int,
                                               ctypes.c_uint32]),
             "spira_spmv_f64_noalloc": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64, _vp, ctypes.c_uint64,
                                                       ctypes.c_int, ctypes.c_uint32]),
@@ -395,6 +399,32 @@ class DeviceCSR:
         self.dev.check(lib().spira_lock_f64(self._h, _ptr(stage_ptr), _ptr(stage_col), _ptr(stage_val),
                                             _ptr(dirty), IN_DEVICE if device else 0, ctypes.byref(out)))
         return int(out.value)
+
+    def spmv_f32(self, x, y=None, variant: int = SPMV_AUTO, device: bool = False, sync: bool = True):
+        """The float instantiation's y = A*x: FP32 x and y (host numpy float32
+        arrays, or float32 device tensors with device=True), A's values in FP32."""
+        if not device:
+            x = np.ascontiguousarray(x, dtype=np.float32)
+            if y is None:
+                y = np.empty(self.n_rows, np.float32)
+            if x.size != self.n_cols:
+                raise InvalidArgument("The size of the input vector x does not match the number of columns of the matrix")
+            if not (isinstance(y, np.ndarray) and y.size == self.n_rows):
+                raise InvalidArgument("The size of the output vector y does not match the number of rows of the matrix")
+            check_host_vector(y, self.n_rows, "y", dtype=np.float32, what="spmv (float)")
+            flags = 0
+        else:
+            import torch
+            if isinstance(x, torch.Tensor) and x.numel() != self.n_cols:
+                raise InvalidArgument("The size of the input vector x does not match the number of columns of the matrix")
+            if isinstance(y, torch.Tensor) and y.numel() != self.n_rows:
+                raise InvalidArgument("The size of the output vector y does not match the number of rows of the matrix")
+            check_device_vector(x, self.n_cols, "x", self.dev.ordinal, dtype="float32", what="spmv (float)")
+            check_device_vector(y, self.n_rows, "y", self.dev.ordinal, dtype="float32", what="spmv (float)")
+            _producers_done(self.dev, x, y)
+            flags = X_DEVICE | Y_DEVICE | (0 if sync else NO_SYNC)
+        self.dev.check(lib().spira_spmv_f32(self._h, _ptr(x), self.n_cols, _ptr(y), self.n_rows, int(variant), flags))
+        return y
 
     def _vectors(self, x, y, device, what="spmv"):
         """Checked (x, y, x_len, y_len, flags) for an FP64 SpMV call.  Sizes
@@ -1159,6 +1189,18 @@ def spmv(mat: Matrix, x, y, variant: int = SPMV_AUTO, reproducible: bool = False
     x and y are numpy float64 arrays (host) or torch float64 CUDA tensors.
     reproducible: bit-identical y on every call with the same matrix and x."""
     is_dev = hasattr(x, "numel") or hasattr(y, "numel")  # torch tensors: checked as device vectors below
+    if str(getattr(y, "dtype", "")) in ("float32", "torch.float32"):
+        # the float instantiation (serial/spmv.hpp:90-135): FP32 x and y
+        xl = int(x.numel()) if hasattr(x, "numel") else int(np.asarray(x).size)
+        if xl != mat.n_cols():
+            raise InvalidArgument("The size of the input vector x does not match the number of columns of the matrix")
+        yl = int(y.numel()) if hasattr(y, "numel") else int(np.asarray(y).size)
+        if yl != mat.n_rows():
+            raise InvalidArgument("The size of the output vector y does not match the number of rows of the matrix")
+        if not mat.is_locked():
+            raise LogicError("spmv: matrix must be locked")
+        mat.device_csr().spmv_f32(x, y, variant=variant, device=is_dev)
+        return y
     xl = int(x.numel()) if hasattr(x, "numel") else int(np.asarray(x).size)
     yl = int(y.numel()) if hasattr(y, "numel") else int(np.asarray(y).size)
     if xl != mat.n_cols():

@@ -402,6 +402,40 @@ int spira_spmv_f64_noalloc(spira_dcsr *A, const double *x, uint64_t x_len, doubl
     });
 }
 
+int spira_spmv_f32(spira_dcsr *A, const float *x, uint64_t x_len, float *y, uint64_t y_len, int variant,
+                   uint32_t flags)
+{
+    if (!A) return SPIRA_ERR_INVALID_ARGUMENT;
+    spira_dev *dev = A->dev;
+    return guarded(dev, [&] {
+        // serial/spmv.hpp:98-106: sizes first, then the mode
+        if (x_len != A->n_cols)
+            fail(SPIRA_ERR_INVALID_ARGUMENT,
+                 "The size of the input vector x does not match the number of columns of the matrix");
+        if (y_len != A->n_rows)
+            fail(SPIRA_ERR_INVALID_ARGUMENT,
+                 "The size of the output vector y does not match the number of rows of the matrix");
+        if (!A->built) fail(SPIRA_ERR_LOGIC, "spmv: matrix must be locked");
+        if (variant != SPIRA_SPMV_AUTO && !mixed_variant_supported(variant))
+            fail(SPIRA_ERR_INVALID_ARGUMENT,
+                 "spmv (float): variant must be VEC4..VEC32, STREAM1K, STREAM2K, XPHASE or XPHASE_PACED");
+        if ((x_len && !x) || (y_len && !y)) fail(SPIRA_ERR_INVALID_ARGUMENT, "spmv: x or y is NULL");
+        const float *dx = x;
+        float *dy = y;
+        const bool xd = flags & SPIRA_X_DEVICE, yd = flags & SPIRA_Y_DEVICE;
+        if (!xd) {
+            float *t = (float *)dev->ws.get(WS_X, x_len * sizeof(float));
+            copy_in(dev, t, x, x_len * sizeof(float), false);
+            dx = t;
+        }
+        if (!yd) dy = (float *)dev->ws.get(WS_Y, y_len * sizeof(float));
+        spmv_f32_launch(A, dx, dy, variant);
+        if (!yd && y_len)
+            SPIRA_HIP(hipMemcpyAsync(y, dy, y_len * sizeof(float), hipMemcpyDeviceToHost, dev->stream));
+        if (!(flags & SPIRA_NO_SYNC) || !xd || !yd) dev_sync(dev);
+    });
+}
+
 int spira_spmv_mixed_f64(spira_dcsr *A, int storage, const double *x, uint64_t x_len, double *y, uint64_t y_len,
                          int variant, float recompute_ratio, uint32_t flags, uint64_t *n_recomputed)
 {
@@ -418,7 +452,8 @@ int spira_spmv_mixed_f64(spira_dcsr *A, int storage, const double *x, uint64_t x
         if (storage != SPIRA_VAL_F32 && storage != SPIRA_VAL_BF16 && storage != SPIRA_VAL_SPLIT_BF16)
             fail(SPIRA_ERR_INVALID_ARGUMENT, "mixed spmv: unknown value storage");
         if (variant != SPIRA_SPMV_AUTO && !mixed_variant_supported(variant))
-            fail(SPIRA_ERR_INVALID_ARGUMENT, "mixed spmv: variant must be VEC4..VEC32, STREAM1K or STREAM2K");
+            fail(SPIRA_ERR_INVALID_ARGUMENT,
+                 "mixed spmv: variant must be VEC4..VEC32, STREAM1K, STREAM2K, XPHASE or XPHASE_PACED");
         if ((x_len && !x) || (y_len && !y)) fail(SPIRA_ERR_INVALID_ARGUMENT, "spmv: x or y is NULL");
         const double *dx = x;
         double *dy = y;

@@ -78,6 +78,16 @@ __device__ __forceinline__ float group_sum_f(float v)
     return v;
 }
 
+// BF16 <-> FP32.  Round-to-nearest-even FP32 -> BF16 (NaN kept quiet).
+__device__ __forceinline__ float bf16_to_f32(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
+__device__ __forceinline__ uint16_t f32_to_bf16(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
 // is_zero(v) <=> |v| <= 0 (traits.hpp:73-88): +0.0 and -0.0 only; NaN and
 // subnormals are not zero.  Bit test, so no dependence on denormal flushing.
 __device__ __forceinline__ bool is_zero_f64(double v)
@@ -284,6 +294,9 @@ struct spira_plan {
     uint32_t *ph_rc = nullptr;    // row inside its block << 18 | column inside its slice
     double *ph_val = nullptr;
     uint32_t *ph_sync = nullptr;  // phase-pacing counters, one per XCD
+    float *ph_v32 = nullptr;      // the XPHASE values as FP32, BF16, split BF16 (built on demand)
+    uint16_t *ph_b16 = nullptr;
+    uint16_t *ph_shi = nullptr, *ph_slo = nullptr;
     uint32_t ph_S = 0, ph_RB = 0, ph_nb = 0, ph_width = 0;
     int32_t variant = -1;        // pinned AUTO choice (-1 = none)
     int32_t auto_variant = -1;   // AUTO's own choice once resolved (learned selector, else the rule)

@@ -32,6 +32,11 @@ bool spmv_variant_fits(const spira_dcsr *A, int variant);
 void matrix_features_host(spira_dcsr *A, float *features);
 bool spmv_selector_scores(spira_dcsr *A);
 void spmv_launch_f64(spira_dcsr *A, const double *x, double *y, int variant, bool repro = false);
+// XPHASE (paced: XPHASE_PACED) with values stored as SPIRA_VAL_* in the plan's
+// order, products and row sums in FP64: FP64 x and y, or FP32 x and y (the
+// reference's float instantiation, FP32 values).
+void spmv_xphase_reduced(spira_dcsr *A, int storage, const double *x, double *y, bool paced);
+void spmv_xphase_f32(spira_dcsr *A, const float *x, float *y, bool paced);
 // The allocation-free launch: true when A's plan already holds what `variant`
 // (AUTO resolved to the pinned or heuristic choice) needs; the bytes of
 // scratch it takes (XPHASE_PACED counters, XSLICE/XSTREAM partial sums); and
@@ -46,6 +51,11 @@ void spmv_stream_blocks(spira_dcsr *A, int which, const uint32_t **blk, uint32_t
 void spmv_mixed_launch(spira_dcsr *A, int storage, const double *x, double *y, int variant, float ratio);
 uint32_t *mixed_queue_count(spira_dcsr *A);
 bool mixed_variant_supported(int variant);
+// The reference's float instantiation: FP32 values (A's values rounded), FP32
+// x and y; each row summed in FP32 with compensation (VEC/STREAM) or in FP64
+// (XPHASE), so y is the row sum rounded once to FP32 up to the compensation's
+// error.  variant: AUTO, VEC4..VEC32, STREAM1K/2K, XPHASE, XPHASE_PACED.
+void spmv_f32_launch(spira_dcsr *A, const float *x, float *y, int variant);
 
 // lock.hip: runs the device lock pipeline on already-device-resident staged
 // arrays; installs the new CSR in A.  Returns the new nnz.

@@ -30,6 +30,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 namespace spira_amd {
 
@@ -338,13 +339,36 @@ __global__ void k_phase_sync_zero(uint32_t *sync)
     if (threadIdx.x < 8 * kPhaseSyncStride) sync[threadIdx.x] = 0u;
 }
 
-template <int U>
+// Values of the XPHASE copy, read as FP64: stored FP64, FP32, BF16 or split
+// BF16 (hi + lo, summed exactly in FP64).
+struct ph_f64 {
+    const double *v;
+    __device__ __forceinline__ double operator()(uint64_t q) const { return ld_stream(v + q); }
+};
+struct ph_f32 {
+    const float *v;
+    __device__ __forceinline__ double operator()(uint64_t q) const { return (double)ld_stream(v + q); }
+};
+struct ph_bf16 {
+    const uint16_t *v;
+    __device__ __forceinline__ double operator()(uint64_t q) const { return (double)bf16_to_f32(ld_stream(v + q)); }
+};
+struct ph_split {
+    const uint16_t *hi, *lo;
+    __device__ __forceinline__ double operator()(uint64_t q) const
+    {
+        return (double)bf16_to_f32(ld_stream(hi + q)) + (double)bf16_to_f32(ld_stream(lo + q));
+    }
+};
+
+// Products are formed and summed in FP64 whatever the stored precision (a
+// FP32 value times a FP32 x is exact in FP64), and y is written as YT.
+template <int U, class Vals, class XT, class YT>
 __global__ __launch_bounds__(kPhaseThreads) void k_spmv_xphase(uint64_t n_rows, uint32_t RB, uint32_t S,
                                                                 const uint64_t *__restrict__ goff,
                                                                 uint32_t width, const uint32_t *__restrict__ prc,
-                                                                const double *__restrict__ pval,
-                                                                const double *__restrict__ x, double *__restrict__ y,
-                                                                uint32_t *__restrict__ sync)
+                                                                Vals pval, const XT *__restrict__ x,
+                                                                YT *__restrict__ y, uint32_t *__restrict__ sync)
 {
     __shared__ double s_y[kPhaseRows];
     __shared__ uint64_t s_goff[kPhaseMaxSlices + 1];  // this block's group starts
@@ -375,10 +399,10 @@ __global__ __launch_bounds__(kPhaseThreads) void k_spmv_xphase(uint64_t n_rows, 
                 const uint32_t rc = ok[u] ? ld_stream(prc + q) : 0u;  // a masked lane reads x[cbase], adds nothing
                 rr[u] = rc >> kPhaseColBits;
                 cc[u] = cbase + (rc & ((1u << kPhaseColBits) - 1u));
-                vv[u] = ok[u] ? ld_stream(pval + q) : 0.0;
+                vv[u] = ok[u] ? pval(q) : 0.0;
             }
 #pragma unroll
-            for (int u = 0; u < U; ++u) xx[u] = x[cc[u]];
+            for (int u = 0; u < U; ++u) xx[u] = (double)x[cc[u]];
 #pragma unroll
             for (int u = 0; u < U; ++u)
                 if (ok[u])
@@ -391,7 +415,7 @@ __global__ __launch_bounds__(kPhaseThreads) void k_spmv_xphase(uint64_t n_rows, 
         }
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < nr; i += kPhaseThreads) y[r0 + i] = s_y[i];
+    for (uint32_t i = threadIdx.x; i < nr; i += kPhaseThreads) y[r0 + i] = (YT)s_y[i];
 }
 
 // XPHASE plan, built element-parallel in three passes (no per-row state, so
@@ -1010,17 +1034,67 @@ void ensure_xphase(spira_dcsr *A)
 // then make the blocks of an XCD take turns).  Only when the whole grid is
 // resident at once (one block per CU).
 // ctr: the 8 * kPhaseSyncStride pacing counters (zeroed here by a kernel).
-template <int U, bool PACE>
-void launch_xphase(spira_dcsr *A, const double *x, double *y, uint32_t *ctr)
+template <int U, bool PACE, class Vals = ph_f64, class XT = double, class YT = double>
+void launch_xphase(spira_dcsr *A, const XT *x, YT *y, uint32_t *ctr, Vals vals = Vals{})
 {
     const spira_plan &P = A->plan;
+    if constexpr (std::is_same_v<Vals, ph_f64>) vals.v = P.ph_val;
     uint32_t *sync = nullptr;
     if (PACE && A->dev->n_cu > 0 && P.ph_nb <= (uint32_t)A->dev->n_cu) {
         sync = ctr;
         k_phase_sync_zero<<<1, 8 * kPhaseSyncStride, 0, A->dev->stream>>>(sync);
     }
-    k_spmv_xphase<U><<<P.ph_nb, kPhaseThreads, 0, A->dev->stream>>>(A->n_rows, P.ph_RB, P.ph_S, P.ph_goff, P.ph_width,
-                                                                      P.ph_rc, P.ph_val, x, y, sync);
+    k_spmv_xphase<U, Vals, XT, YT><<<P.ph_nb, kPhaseThreads, 0, A->dev->stream>>>(
+        A->n_rows, P.ph_RB, P.ph_S, P.ph_goff, P.ph_width, P.ph_rc, vals, x, y, sync);
+}
+
+__global__ void k_ph_to_f32(const double *__restrict__ v, float *__restrict__ o, uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) o[i] = (float)v[i];
+}
+
+__global__ void k_ph_to_bf16(const double *__restrict__ v, uint16_t *__restrict__ o, uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) o[i] = f32_to_bf16((float)v[i]);
+}
+
+__global__ void k_ph_to_split(const double *__restrict__ v, uint16_t *__restrict__ hi, uint16_t *__restrict__ lo,
+                              uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint16_t h = f32_to_bf16((float)v[i]);
+    hi[i] = h;
+    lo[i] = f32_to_bf16((float)(v[i] - (double)bf16_to_f32(h)));
+}
+
+// The XPHASE values in reduced precision (built once per locked matrix from
+// the plan's FP64 values, in the plan's order).
+void ensure_xphase_vals(spira_dcsr *A, int storage)
+{
+    ensure_xphase(A);
+    spira_plan &P = A->plan;
+    const uint64_t n = A->nnz ? A->nnz : 1;
+    const unsigned g = grid_for(n, kThreads);
+    hipStream_t st = A->dev->stream;
+    if (storage == SPIRA_VAL_F32 && !P.ph_v32) {
+        float *o = (float *)plan_alloc(A, n * sizeof(float));
+        k_ph_to_f32<<<g, kThreads, 0, st>>>(P.ph_val, o, A->nnz);
+        P.ph_v32 = o;
+    } else if (storage == SPIRA_VAL_BF16 && !P.ph_b16) {
+        uint16_t *o = (uint16_t *)plan_alloc(A, n * sizeof(uint16_t));
+        k_ph_to_bf16<<<g, kThreads, 0, st>>>(P.ph_val, o, A->nnz);
+        P.ph_b16 = o;
+    } else if (storage == SPIRA_VAL_SPLIT_BF16 && !P.ph_shi) {
+        uint16_t *hi = (uint16_t *)plan_alloc(A, n * sizeof(uint16_t));
+        uint16_t *lo = (uint16_t *)plan_alloc(A, n * sizeof(uint16_t));
+        k_ph_to_split<<<g, kThreads, 0, st>>>(P.ph_val, hi, lo, A->nnz);
+        P.ph_shi = hi;
+        P.ph_slo = lo;
+    }
+    SPIRA_HIP(hipGetLastError());
 }
 
 // part: kSlices * n_rows partial sums (every one written before it is read).
@@ -1202,6 +1276,35 @@ void spmv_launch_planned(spira_dcsr *A, const double *x, double *y, int variant,
     if (variant < 0) variant = spmv_auto_choice(A);
     if (variant < 0) fail(SPIRA_ERR_LOGIC, "spmv: AUTO's choice is not resolved for this matrix");
     launch_planned(A, x, y, variant, (uint32_t *)scratch, (double *)scratch);
+}
+
+void spmv_xphase_reduced(spira_dcsr *A, int storage, const double *x, double *y, bool paced)
+{
+    if (A->n_rows == 0) return;
+    ensure_xphase_vals(A, storage);
+    const spira_plan &P = A->plan;
+    uint32_t *ctr = P.ph_sync;
+    if (storage == SPIRA_VAL_F32) {
+        if (paced) launch_xphase<20, true>(A, x, y, ctr, ph_f32{P.ph_v32});
+        else launch_xphase<6, false>(A, x, y, ctr, ph_f32{P.ph_v32});
+    } else if (storage == SPIRA_VAL_BF16) {
+        if (paced) launch_xphase<20, true>(A, x, y, ctr, ph_bf16{P.ph_b16});
+        else launch_xphase<6, false>(A, x, y, ctr, ph_bf16{P.ph_b16});
+    } else {
+        if (paced) launch_xphase<20, true>(A, x, y, ctr, ph_split{P.ph_shi, P.ph_slo});
+        else launch_xphase<6, false>(A, x, y, ctr, ph_split{P.ph_shi, P.ph_slo});
+    }
+    SPIRA_HIP(hipGetLastError());
+}
+
+void spmv_xphase_f32(spira_dcsr *A, const float *x, float *y, bool paced)
+{
+    if (A->n_rows == 0) return;
+    ensure_xphase_vals(A, SPIRA_VAL_F32);
+    const spira_plan &P = A->plan;
+    if (paced) launch_xphase<20, true>(A, x, y, P.ph_sync, ph_f32{P.ph_v32});
+    else launch_xphase<6, false>(A, x, y, P.ph_sync, ph_f32{P.ph_v32});
+    SPIRA_HIP(hipGetLastError());
 }
 
 void spmv_launch_f64(spira_dcsr *A, const double *x, double *y, int variant, bool repro)

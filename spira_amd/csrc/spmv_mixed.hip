@@ -22,17 +22,6 @@ namespace {
 constexpr int kThreads = 256;
 
 // ---- value decoders --------------------------------------------------------
-__device__ __forceinline__ float bf16_to_f32(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
-
-// Round-to-nearest-even FP32 -> BF16 (NaN kept quiet).
-__device__ __forceinline__ uint16_t f32_to_bf16(float f)
-{
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-
 struct vals_f32 {
     const float *v;
     __device__ __forceinline__ float operator()(uint64_t k) const { return ld_stream(v + k); }
@@ -85,11 +74,12 @@ struct dot2 {
     }
 };
 
-__device__ __forceinline__ void finish_row(const dot2 &d, uint64_t row, double *y, float ratio,
+template <class YT>
+__device__ __forceinline__ void finish_row(const dot2 &d, uint64_t row, YT *y, float ratio,
                                            uint8_t *flags, uint32_t *count)
 {
     const double v = (double)d.s + (double)d.c;
-    y[row] = v;
+    y[row] = (YT)v;
     if (ratio > 0.0f) {
         const bool bad = !isfinite(v) || (double)d.a > (double)ratio * fabs(v);
         flags[row] = bad ? 1u : 0u;
@@ -104,11 +94,12 @@ __global__ void k_x_to_f32(const double *__restrict__ x, float *__restrict__ x32
     if (i == 0 && count) *count = 0u;  // reset the recompute count (no memset: graph-safe)
 }
 
-// VEC<G>: G lanes per row.
-template <int G, class Vals>
+// VEC<G>: G lanes per row.  y is FP64 (mixed precision) or FP32 (the float
+// instantiation).
+template <int G, class Vals, class YT>
 __global__ __launch_bounds__(kThreads) void k_mixed_vec(uint64_t n_rows, const uint32_t *__restrict__ off,
                                                          const uint32_t *__restrict__ col, Vals vals,
-                                                         const float *__restrict__ x, double *__restrict__ y,
+                                                         const float *__restrict__ x, YT *__restrict__ y,
                                                          float ratio, uint8_t *flags, uint32_t *count)
 {
     const uint64_t gid = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
@@ -124,11 +115,11 @@ __global__ __launch_bounds__(kThreads) void k_mixed_vec(uint64_t n_rows, const u
 
 // STREAM<C>: LDS-staged row blocks (same plan as the FP64 kernel); the LDS
 // holds each product and its exact error.
-template <int C, class Vals>
+template <int C, class Vals, class YT>
 __global__ __launch_bounds__(kThreads) void k_mixed_stream(const uint32_t *__restrict__ blk_row,
                                                             const uint32_t *__restrict__ off,
                                                             const uint32_t *__restrict__ col, Vals vals,
-                                                            const float *__restrict__ x, double *__restrict__ y,
+                                                            const float *__restrict__ x, YT *__restrict__ y,
                                                             float ratio, uint8_t *flags, uint32_t *count)
 {
     __shared__ float s_p[2 * C];
@@ -285,8 +276,8 @@ void ensure_storage(spira_dcsr *A, int storage)
     SPIRA_HIP(hipGetLastError());
 }
 
-template <class Vals>
-void launch_mixed(spira_dcsr *A, int variant, Vals vals, const float *x32, double *y, float ratio, uint8_t *flags,
+template <class Vals, class YT>
+void launch_mixed(spira_dcsr *A, int variant, Vals vals, const float *x32, YT *y, float ratio, uint8_t *flags,
                   uint32_t *count, const uint32_t *blk, uint32_t nblk)
 {
     hipStream_t st = A->dev->stream;
@@ -294,38 +285,55 @@ void launch_mixed(spira_dcsr *A, int variant, Vals vals, const float *x32, doubl
     const uint64_t n = A->n_rows;
     switch (variant) {
     case SPIRA_SPMV_VEC4:
-        k_mixed_vec<4, Vals><<<grid_for(n * 4, kThreads), kThreads, 0, st>>>(n, off, A->col, vals, x32, y, ratio,
-                                                                            flags, count);
+        k_mixed_vec<4, Vals, YT><<<grid_for(n * 4, kThreads), kThreads, 0, st>>>(n, off, A->col, vals, x32, y,
+                                                                                ratio, flags, count);
         break;
     case SPIRA_SPMV_VEC8:
-        k_mixed_vec<8, Vals><<<grid_for(n * 8, kThreads), kThreads, 0, st>>>(n, off, A->col, vals, x32, y, ratio,
-                                                                            flags, count);
+        k_mixed_vec<8, Vals, YT><<<grid_for(n * 8, kThreads), kThreads, 0, st>>>(n, off, A->col, vals, x32, y,
+                                                                                ratio, flags, count);
         break;
     case SPIRA_SPMV_VEC16:
-        k_mixed_vec<16, Vals><<<grid_for(n * 16, kThreads), kThreads, 0, st>>>(n, off, A->col, vals, x32, y,
-                                                                              ratio, flags, count);
+        k_mixed_vec<16, Vals, YT><<<grid_for(n * 16, kThreads), kThreads, 0, st>>>(n, off, A->col, vals, x32, y,
+                                                                                  ratio, flags, count);
         break;
     case SPIRA_SPMV_VEC32:
-        k_mixed_vec<32, Vals><<<grid_for(n * 32, kThreads), kThreads, 0, st>>>(n, off, A->col, vals, x32, y,
-                                                                              ratio, flags, count);
+        k_mixed_vec<32, Vals, YT><<<grid_for(n * 32, kThreads), kThreads, 0, st>>>(n, off, A->col, vals, x32, y,
+                                                                                  ratio, flags, count);
         break;
     case SPIRA_SPMV_STREAM1K:
-        k_mixed_stream<1024, Vals><<<nblk, kThreads, 0, st>>>(blk, off, A->col, vals, x32, y, ratio, flags, count);
+        k_mixed_stream<1024, Vals, YT><<<nblk, kThreads, 0, st>>>(blk, off, A->col, vals, x32, y, ratio, flags,
+                                                                  count);
         break;
     default:
-        k_mixed_stream<2048, Vals><<<nblk, kThreads, 0, st>>>(blk, off, A->col, vals, x32, y, ratio, flags, count);
+        k_mixed_stream<2048, Vals, YT><<<nblk, kThreads, 0, st>>>(blk, off, A->col, vals, x32, y, ratio, flags,
+                                                                  count);
         break;
     }
     SPIRA_HIP(hipGetLastError());
 }
 
-}  // namespace
-
-bool mixed_variant_supported(int variant)
+bool row_major_variant(int variant)
 {
     return variant == SPIRA_SPMV_VEC4 || variant == SPIRA_SPMV_VEC8 || variant == SPIRA_SPMV_VEC16 ||
            variant == SPIRA_SPMV_VEC32 || variant == SPIRA_SPMV_STREAM1K || variant == SPIRA_SPMV_STREAM2K;
 }
+
+bool xphase_variant(int variant) { return variant == SPIRA_SPMV_XPHASE || variant == SPIRA_SPMV_XPHASE_PACED; }
+
+// The FP64 AUTO choice mapped onto the reduced-precision kernels (XPHASE
+// layouts kept; the XCD-slice copies go to the LDS row blocks; VEC widths
+// outside 4..32 to the nearest one).
+int reduced_auto(spira_dcsr *A)
+{
+    const int v = spmv_resolve_auto(A);
+    if (row_major_variant(v) || xphase_variant(v)) return v;
+    if (v >= SPIRA_SPMV_XSLICE1 && v <= SPIRA_SPMV_XSTREAM) return SPIRA_SPMV_STREAM2K;
+    return v < SPIRA_SPMV_VEC4 ? SPIRA_SPMV_VEC4 : SPIRA_SPMV_VEC32;
+}
+
+}  // namespace
+
+bool mixed_variant_supported(int variant) { return row_major_variant(variant) || xphase_variant(variant); }
 
 // WS_MIXQ: one flag byte per row, then (16-byte aligned) the recompute count.
 static uint8_t *mixed_flags(spira_dcsr *A)
@@ -346,11 +354,23 @@ void spmv_mixed_launch(spira_dcsr *A, int storage, const double *x, double *y, i
     spira_dev *dev = A->dev;
     hipStream_t st = dev->stream;
     if (variant < 0) {
-        variant = A->plan.variant >= 0 ? A->plan.variant : spmv_rule_variant(A);
-        if (!mixed_variant_supported(variant)) variant = SPIRA_SPMV_STREAM2K;
+        // the recompute needs each row's sum of |a*x|, which only the
+        // row-major kernels form (the XPHASE rows are summed by LDS atomics)
+        variant = reduced_auto(A);
+        if (ratio > 0.0f && !row_major_variant(variant)) {
+            variant = A->plan.variant >= 0 ? A->plan.variant : spmv_rule_variant(A);
+            if (!row_major_variant(variant)) variant = SPIRA_SPMV_STREAM2K;
+        }
     }
     if (!mixed_variant_supported(variant))
-        fail(SPIRA_ERR_INVALID_ARGUMENT, "mixed spmv: variant must be VEC4..VEC32, STREAM1K or STREAM2K");
+        fail(SPIRA_ERR_INVALID_ARGUMENT,
+             "mixed spmv: variant must be VEC4..VEC32, STREAM1K, STREAM2K, XPHASE or XPHASE_PACED");
+    if (xphase_variant(variant)) {
+        if (ratio > 0.0f)
+            fail(SPIRA_ERR_INVALID_ARGUMENT, "mixed spmv: the FP64 recompute needs a VEC or STREAM variant");
+        spmv_xphase_reduced(A, storage, x, y, variant == SPIRA_SPMV_XPHASE_PACED);
+        return;
+    }
     ensure_storage(A, storage);
     const uint32_t *blk = nullptr;
     uint32_t nblk = 0;
@@ -375,6 +395,27 @@ void spmv_mixed_launch(spira_dcsr *A, int storage, const double *x, double *y, i
         k_recompute_f64<<<g, kThreads, 0, st>>>(flags, A->n_rows, count, A->off, A->col, A->val, x, y);
         SPIRA_HIP(hipGetLastError());
     }
+}
+
+void spmv_f32_launch(spira_dcsr *A, const float *x, float *y, int variant)
+{
+    if (A->n_rows == 0) return;
+    if (A->nnz >= (1ull << 32)) fail(SPIRA_ERR_UNSUPPORTED, "spmv (float): nnz must be < 2^32");
+    spmv_build_plan(A);
+    if (variant < 0) variant = reduced_auto(A);
+    if (!mixed_variant_supported(variant))
+        fail(SPIRA_ERR_INVALID_ARGUMENT,
+             "spmv (float): variant must be VEC4..VEC32, STREAM1K, STREAM2K, XPHASE or XPHASE_PACED");
+    if (xphase_variant(variant)) {
+        spmv_xphase_f32(A, x, y, variant == SPIRA_SPMV_XPHASE_PACED);
+        return;
+    }
+    ensure_storage(A, SPIRA_VAL_F32);
+    const uint32_t *blk = nullptr;
+    uint32_t nblk = 0;
+    if (variant == SPIRA_SPMV_STREAM1K || variant == SPIRA_SPMV_STREAM2K)
+        spmv_stream_blocks(A, variant == SPIRA_SPMV_STREAM1K ? 0 : 1, &blk, &nblk);
+    launch_mixed(A, variant, vals_f32{A->val_f32}, x, y, 0.0f, nullptr, nullptr, blk, nblk);
 }
 
 }  // namespace spira_amd

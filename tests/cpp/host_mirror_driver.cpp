@@ -10,6 +10,7 @@
 //   row_nnz r | nnz                          -> value <n>
 //   csr                                      -> csr none | csr n nnz ; offs ; cols ; vals
 //   spmv serial|parallel|op nx ny x...       -> y <hex>...   (x given, y of size ny)
+//   spmv32 serial|parallel nx ny x...        -> y <hex>...   (the float overloads)
 //   partition t                              -> part start end ; offs ; cols ; vals
 //   bounds                                   -> bounds b...
 //   load_csr n nnz offs... cols... vals...   -> ok
@@ -159,6 +160,21 @@ std::string run(const std::string &line)
         else spira_amd::serial::algorithms::spmv(mat(), x, y);
         out << "y";
         for (auto v : y) out << ' ' << hexf(v);
+    } else if (cmd == "spmv32") {  // the float overloads: x rounded to FP32, y printed as FP64 hex
+        std::string kind;
+        std::size_t nx, ny;
+        in >> kind >> nx >> ny;
+        std::vector<float> x(nx);
+        for (auto &v : x) {
+            std::string s;
+            in >> s;
+            v = (float)parse_double(s);
+        }
+        std::vector<float> y(ny);
+        if (kind == "parallel") spira_amd::parallel::algorithms::spmv(par(), x, y);
+        else spira_amd::serial::algorithms::spmv(mat(), x, y);
+        out << "y";
+        for (auto v : y) out << ' ' << hexf((double)v);
     } else if (cmd == "partition") {
         std::size_t t;
         in >> t;

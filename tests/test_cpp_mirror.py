@@ -118,6 +118,11 @@ class CppMatrix:
         out = self.d.cmd(f"spmv {kind} {len(x)} {ny} " + " ".join(_h(v) for v in x))
         return np.array([float.fromhex(t) for t in out.split()[1:]], np.float64)
 
+    def spmv32(self, x, kind="serial", ny=None):
+        ny = self.nr if ny is None else ny
+        out = self.d.cmd(f"spmv32 {kind} {len(x)} {ny} " + " ".join(_h(v) for v in x))
+        return np.array([float.fromhex(t) for t in out.split()[1:]])
+
     def partition(self, t):
         out = self.d.cmd(f"partition {t}")
         _, rest = out.split(" ", 1)
@@ -490,3 +495,32 @@ def test_in_place_operators_on_cpp_mirror(drv):
     with pytest.raises(RuntimeError) as e:
         drv.cmd("sub")
     assert str(e.value) == "multiplication_scaler: input matrix must be locked"
+
+
+@pytest.mark.gpu
+def test_float_spmv_overloads(drv):
+    """serial/parallel spmv for the float instantiation (serial/spmv.hpp:90-135,
+    parallel/algorithms/spmv.hpp:85-119) through the C++ mirror: the known
+    answer (spmv_tests.cpp:121-139) exactly, a random case within 1e-6 of the
+    FP64 sum of its FP32 data, and the reference's size checks."""
+    for case in KATS["spmv_f32"]:
+        for kind, threads in (("serial", None), ("parallel", 2)):
+            m = CppMatrix(drv, case["n_rows"], case["n_cols"], threads)
+            for r, c, v in case["inserts"]:
+                m.insert(r, c, v)
+            m.lock()
+            assert m.spmv32(case["x"], kind).tolist() == case["y"]
+    rng = np.random.default_rng(12)
+    m = CppMatrix(drv, 300, 200)
+    rows, cols = rng.integers(0, 300, 3000), rng.integers(0, 200, 3000)
+    vals = rng.uniform(-1, 1, 3000).astype(np.float32).astype(np.float64)
+    for r, c, v in zip(rows, cols, vals):
+        m.insert(int(r), int(c), float(v))
+    m.lock()
+    off, col, val = m.csr()
+    x = rng.uniform(-1, 1, 200).astype(np.float32)
+    y = m.spmv32(x.astype(np.float64))
+    exact = oracle.spmv_f64(off, col, val, x.astype(np.float64))
+    assert oracle.normwise_rel_err(y, exact) <= 1e-6
+    with pytest.raises(ValueError, match="input vector x"):
+        m.spmv32([1.0] * 3)

@@ -38,12 +38,45 @@ TOL = {"f32": 1e-6, "split_bf16": 1e-4, "bf16": 2e-2}
 
 
 @pytest.mark.parametrize("storage", ["f32", "split_bf16", "bf16"])
-@pytest.mark.parametrize("variant", spira_amd.MIXED_VARIANTS)
+@pytest.mark.parametrize("variant", spira_amd.ROW_MAJOR_MIXED_VARIANTS)
 def test_mixed_error_bounds(powerlaw, storage, variant):
     A, csr, x, y_ref = powerlaw
     y, nrec = A.spmv_mixed(x, storage=storage, variant=variant, recompute_ratio=64.0)
     err = oracle.normwise_rel_err(y, y_ref)
     assert err <= TOL[storage], (storage, variant, err, nrec)
+
+
+@pytest.mark.parametrize("storage", ["f32", "split_bf16", "bf16"])
+@pytest.mark.parametrize("name", ["xphase", "xphase_paced"])
+def test_mixed_xphase_layout(powerlaw, storage, name):
+    """Reduced values in the XPHASE plan's order, rows summed in FP64 in LDS:
+    the error is the value rounding's alone (FP32: <= 2e-6 without any
+    recompute), and the recompute is refused (it needs row-major sums)."""
+    A, csr, x, y_ref = powerlaw
+    v = spira_amd.VARIANTS[name]
+    y, nrec = A.spmv_mixed(x, storage=storage, variant=v, recompute_ratio=0.0)
+    err = oracle.normwise_rel_err(y, y_ref)
+    assert err <= {"f32": 2e-6, "split_bf16": 1e-4, "bf16": 2e-2}[storage], (storage, name, err)
+    assert nrec == 0
+    y_row, _ = A.spmv_mixed(x, storage=storage, variant=spira_amd.VARIANTS["stream2k"], recompute_ratio=0.0)
+    assert err <= 1.5 * oracle.normwise_rel_err(y_row, y_ref) + 1e-12  # no worse than compensated FP32
+    with pytest.raises(spira_amd.InvalidArgument, match="recompute"):
+        A.spmv_mixed(x, storage=storage, variant=v, recompute_ratio=64.0)
+
+
+def test_mixed_xphase_fp32_exact_values(dev):
+    """FP32-exact values: the XPHASE layout's FP64 products and sums leave
+    only FP64 rounding (<= 1e-14 normwise against the FP64 oracle)."""
+    n = 300000
+    off, col, val = W.random_uniform_rows(n, n, 8, 32, seed=17)
+    val = val.astype(np.float32).astype(np.float64)
+    A = spira_amd.DeviceCSR(dev, n, n)
+    A.upload(off, col, val)
+    x = np.random.default_rng(2).uniform(-1, 1, n)
+    y_ref = oracle.spmv_f64(off, col, val, x)
+    for name in ("xphase", "xphase_paced"):
+        y, _ = A.spmv_mixed(x, storage="f32", variant=spira_amd.VARIANTS[name])
+        assert oracle.normwise_rel_err(y, y_ref) <= 1e-14, name
 
 
 def test_mixed_without_recompute_still_meets_fp32_bound(powerlaw):
@@ -55,8 +88,9 @@ def test_mixed_without_recompute_still_meets_fp32_bound(powerlaw):
 
 def test_recomputed_rows_are_fp64(powerlaw):
     A, (off, col, val), x, y_ref = powerlaw
-    y_plain, _ = A.spmv_mixed(x, storage="bf16", recompute_ratio=0.0)
-    y_rec, nrec = A.spmv_mixed(x, storage="bf16", recompute_ratio=4.0)
+    v = spira_amd.VARIANTS["stream2k"]  # the same row-major kernel with and without the recompute
+    y_plain, _ = A.spmv_mixed(x, storage="bf16", variant=v, recompute_ratio=0.0)
+    y_rec, nrec = A.spmv_mixed(x, storage="bf16", variant=v, recompute_ratio=4.0)
     assert nrec > 0
     changed = y_rec != y_plain
     assert changed.any()
