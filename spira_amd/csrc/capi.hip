@@ -3,6 +3,7 @@
 // the handle.
 #include "internal.hpp"
 
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -152,6 +153,8 @@ int spira_dev_create(spira_dev **out, int device_ordinal, void *stream)
         dev->arch_minor = prop.minor;
         dev->n_cu = prop.multiProcessorCount;
         dev->noalloc_scratch = scratch_of_device(device_ordinal);
+        // the XPHASE_PACED wait bound (a test forces give-ups with a small one)
+        if (const char *e = std::getenv("SPIRA_PHASE_SPIN_MAX")) dev->phase_spin_max = (uint32_t)std::strtoul(e, nullptr, 10);
         if (stream) {
             dev->stream = (hipStream_t)stream;
         } else {

@@ -334,4 +334,5 @@ struct spira_dev {
     int arch_major = 0, arch_minor = 0;
     int n_cu = 0;
     void *noalloc_scratch = nullptr;  // this GPU's copy of the fixed scratch array (capi.hip)
+    uint32_t phase_spin_max = 1u << 14;  // loads before an XPHASE_PACED wait gives up (SPIRA_PHASE_SPIN_MAX)
 };

@@ -18,8 +18,8 @@
 //   XPHASE    row blocks that keep their y in LDS; the blocks of one XCD walk
 //             L2-sized column slices in the same order (entries grouped by
 //             block and slice), so x gathers hit L2 and y is written once.
-//             XPHASE_PACED also paces the blocks of an XCD to within two
-//             slices.  Each (block, slice) group is sorted by column.
+//             XPHASE_PACED also paces the blocks of an XCD to within three
+//             slices.  Each (block, slice) group is sorted by x line.
 //
 // Values and columns are read with non-temporal loads so the x gathers keep
 // L2 and the MALL.  Blocks are renumbered so that each XCD (8 on MI355X, which
@@ -317,21 +317,29 @@ constexpr int kPhaseThreads = 1024;
 // slices at a time (they drift apart otherwise, and the XCD's L2 then holds
 // many slices' worth of x).  sync[c * kPhaseSyncStride] counts finished (block,
 // phase) pairs of XCD c; it is zeroed before the launch.  The wait only paces
-// and never decides a result: it gives up after a bounded number of sleeps,
-// and the host passes sync only when every block of the grid can be resident
-// at once (one block per CU).
+// and never decides a result: it gives up after a bounded number of sleeps
+// (spin_max loads), and the host passes sync only when every block of the
+// grid can be resident at once (one block per CU).  A block whose wait gives
+// up stops pacing for the rest of the call (other work on the GPU may hold
+// CUs, so some blocks of its XCD are not running): one timeout per block at
+// most, never one per phase.
 constexpr uint32_t kPhaseSyncStride = 16;  // counters 64 bytes apart
 constexpr uint32_t kPhaseSpinMax = 1u << 14;
 
-__device__ __forceinline__ void phase_wait(const uint32_t *ctr, uint32_t target)
+// true when the wait gave up (the same value in every thread of the block)
+__device__ __forceinline__ bool phase_wait(const uint32_t *ctr, uint32_t target, uint32_t spin_max)
 {
+    __shared__ uint32_t s_gave_up;
     if (threadIdx.x == 0) {
-        for (uint32_t i = 0; i < kPhaseSpinMax; ++i) {
+        uint32_t i = 0;
+        for (; i < spin_max; ++i) {
             if (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
             __builtin_amdgcn_s_sleep(4);
         }
+        s_gave_up = i == spin_max ? 1u : 0u;
     }
     __syncthreads();
+    return s_gave_up != 0;
 }
 
 __global__ void k_phase_sync_zero(uint32_t *sync)
@@ -368,7 +376,8 @@ __global__ __launch_bounds__(kPhaseThreads) void k_spmv_xphase(uint64_t n_rows, 
                                                                 const uint64_t *__restrict__ goff,
                                                                 uint32_t width, const uint32_t *__restrict__ prc,
                                                                 Vals pval, const XT *__restrict__ x,
-                                                                YT *__restrict__ y, uint32_t *__restrict__ sync)
+                                                                YT *__restrict__ y, uint32_t *__restrict__ sync,
+                                                                uint32_t spin_max)
 {
     __shared__ double s_y[kPhaseRows];
     __shared__ uint64_t s_goff[kPhaseMaxSlices + 1];  // this block's group starts
@@ -383,8 +392,9 @@ __global__ __launch_bounds__(kPhaseThreads) void k_spmv_xphase(uint64_t n_rows, 
     const uint32_t first = xcd * per;
     const uint32_t nbx = (gridDim.x - xcd + 7) / 8;  // blocks of this XCD
     uint32_t *ctr = sync ? sync + xcd * kPhaseSyncStride : nullptr;
+    bool pace = ctr != nullptr;  // counting continues after a give-up; waiting does not
     for (uint32_t t = 0; t < S; ++t) {
-        if (ctr && t >= 3) phase_wait(ctr, nbx * (t - 2));
+        if (pace && t >= 3 && phase_wait(ctr, nbx * (t - 2), spin_max)) pace = false;
         const uint32_t s = first + t < S ? first + t : first + t - S;
         const uint64_t e0 = s_goff[s], e1 = s_goff[s + 1];
         const uint32_t cbase = s * width;
@@ -1045,7 +1055,7 @@ void launch_xphase(spira_dcsr *A, const XT *x, YT *y, uint32_t *ctr, Vals vals =
         k_phase_sync_zero<<<1, 8 * kPhaseSyncStride, 0, A->dev->stream>>>(sync);
     }
     k_spmv_xphase<U, Vals, XT, YT><<<P.ph_nb, kPhaseThreads, 0, A->dev->stream>>>(
-        A->n_rows, P.ph_RB, P.ph_S, P.ph_goff, P.ph_width, P.ph_rc, vals, x, y, sync);
+        A->n_rows, P.ph_RB, P.ph_S, P.ph_goff, P.ph_width, P.ph_rc, vals, x, y, sync, A->dev->phase_spin_max);
 }
 
 __global__ void k_ph_to_f32(const double *__restrict__ v, float *__restrict__ o, uint64_t n)

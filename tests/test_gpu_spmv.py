@@ -308,3 +308,44 @@ def test_reproducible_mode_and_xphase_call_to_call(dev):
         b = A.spmv(x, torch.empty_like(x), variant=spira_amd.VARIANTS[name], device=True).cpu().numpy()
         assert oracle.normwise_rel_err(a, b) <= 1e-15
         _check(a, off, col, val, xh)
+
+
+_GIVE_UP_SCRIPT = r"""
+import sys, time
+import numpy as np
+sys.path.insert(0, {root!r})
+import oracle, spira_amd
+from spira_amd import workloads as W
+n = 1 << 20
+off, col, val = W.random_uniform_rows(n, n, 8, 32, seed=71)
+A = spira_amd.DeviceCSR(spira_amd.Device(0), n, n)
+A.upload(off, col, val)
+x = W.dense_x(n)
+v = spira_amd.VARIANTS["xphase_paced"]
+y = A.spmv(x, variant=v)
+t = time.perf_counter()
+for _ in range(5):
+    y = A.spmv(x, variant=v)
+dt = (time.perf_counter() - t) / 5
+err = oracle.normwise_rel_err(y, oracle.spmv_f64(off, col, val, x))
+print(f"{{err}} {{dt}}")
+"""
+
+
+def test_xphase_paced_waits_that_give_up_stop_pacing():
+    """XPHASE_PACED with the pacing wait bound forced to one load from the host
+    (SPIRA_PHASE_SPIN_MAX=1, read when the handle is created): waits give up,
+    each block stops pacing after its first give-up, the kernel exits
+    normally, y matches the oracle and a call stays well under 50 ms."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, SPIRA_PHASE_SPIN_MAX="1")
+    out = subprocess.run([sys.executable, "-c", _GIVE_UP_SCRIPT.format(root=root)], env=env, capture_output=True,
+                         text=True, timeout=240)
+    assert out.returncode == 0, out.stderr[-2000:]
+    err, dt = (float(t) for t in out.stdout.split()[-2:])
+    print(f"spin max 1: error {err:.2e}, {1e3 * dt:.3f} ms per call")
+    assert err <= 1e-12
+    assert dt < 0.05
