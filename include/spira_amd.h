@@ -272,7 +272,8 @@ int spira_spgemm_f64(spira_dcsr *A, spira_dcsr *B, spira_dcsr *C);
  * x; recompute_ratio <= 0 disables that pass.  variant: SPIRA_SPMV_AUTO,
  * VEC4..VEC32, STREAM1K, STREAM2K, or (without the recompute) XPHASE and
  * XPHASE_PACED, which read the reduced values in the XPHASE plan's order and
- * sum each row in FP64 (an FP32 value times the FP64 x, added in LDS).
+ * sum each row in FP64 (an FP32 value times the FP32 x is exact in FP64; the
+ * products are added in LDS).
  * AUTO follows the FP64 AUTO choice (row-major kernels when the recompute is
  * on).  *n_recomputed (may be NULL; reading it synchronises) receives the
  * number of recomputed rows.  Checks and flags as spira_spmv_f64. */

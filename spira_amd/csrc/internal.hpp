@@ -33,9 +33,10 @@ void matrix_features_host(spira_dcsr *A, float *features);
 bool spmv_selector_scores(spira_dcsr *A);
 void spmv_launch_f64(spira_dcsr *A, const double *x, double *y, int variant, bool repro = false);
 // XPHASE (paced: XPHASE_PACED) with values stored as SPIRA_VAL_* in the plan's
-// order, products and row sums in FP64: FP64 x and y, or FP32 x and y (the
-// reference's float instantiation, FP32 values).
-void spmv_xphase_reduced(spira_dcsr *A, int storage, const double *x, double *y, bool paced);
+// order and x in FP32 (half the x lines to gather), products and row sums in
+// FP64: FP64 y (mixed precision), or FP32 y (the reference's float
+// instantiation, FP32 values).
+void spmv_xphase_reduced(spira_dcsr *A, int storage, const float *x, double *y, bool paced);
 void spmv_xphase_f32(spira_dcsr *A, const float *x, float *y, bool paced);
 // The allocation-free launch: true when A's plan already holds what `variant`
 // (AUTO resolved to the pinned or heuristic choice) needs; the bytes of

@@ -371,8 +371,14 @@ struct ph_split {
 
 // Products are formed and summed in FP64 whatever the stored precision (a
 // FP32 value times a FP32 x is exact in FP64), and y is written as YT.
+// Block b of the grid takes row blocks b, b + grid, b + 2 grid, ... (one
+// "pass" each; with pacing the grid is one block per CU, so matrices with more
+// row blocks than CUs are paced too).  Every block runs every pass, so the
+// XCD's pacing count is the same in each; a block past the last row block
+// runs its phases empty and still counts them.
 template <int U, class Vals, class XT, class YT>
 __global__ __launch_bounds__(kPhaseThreads) void k_spmv_xphase(uint64_t n_rows, uint32_t RB, uint32_t S,
+                                                                uint32_t nb, const uint32_t *__restrict__ order,
                                                                 const uint64_t *__restrict__ goff,
                                                                 uint32_t width, const uint32_t *__restrict__ prc,
                                                                 Vals pval, const XT *__restrict__ x,
@@ -381,20 +387,25 @@ __global__ __launch_bounds__(kPhaseThreads) void k_spmv_xphase(uint64_t n_rows, 
 {
     __shared__ double s_y[kPhaseRows];
     __shared__ uint64_t s_goff[kPhaseMaxSlices + 1];  // this block's group starts
-    const uint32_t b = blockIdx.x;
-    const uint64_t r0 = (uint64_t)b * RB;
-    const uint32_t nr = (uint32_t)min((uint64_t)RB, n_rows - r0);
-    for (uint32_t i = threadIdx.x; i < nr; i += kPhaseThreads) s_y[i] = 0.0;
-    for (uint32_t i = threadIdx.x; i <= S; i += kPhaseThreads) s_goff[i] = goff[(uint64_t)b * S + i];
-    __syncthreads();
-    const uint32_t xcd = b % 8;
+    const uint32_t xcd = blockIdx.x % 8;
     const uint32_t per = S / 8;  // S is a multiple of 8; XCD c starts at slice c * S / 8
     const uint32_t first = xcd * per;
     const uint32_t nbx = (gridDim.x - xcd + 7) / 8;  // blocks of this XCD
     uint32_t *ctr = sync ? sync + xcd * kPhaseSyncStride : nullptr;
     bool pace = ctr != nullptr;  // counting continues after a give-up; waiting does not
+    const uint32_t passes = (nb + gridDim.x - 1) / gridDim.x;
+    for (uint32_t pass = 0; pass < passes; ++pass) {
+    const uint32_t slot = blockIdx.x + pass * gridDim.x;
+    const bool live = slot < nb;
+    const uint32_t b = live ? order[slot] : 0u;  // row blocks heaviest first
+    const uint64_t r0 = (uint64_t)b * RB;
+    const uint32_t nr = live ? (uint32_t)min((uint64_t)RB, n_rows - r0) : 0u;
+    for (uint32_t i = threadIdx.x; i < nr; i += kPhaseThreads) s_y[i] = 0.0;
+    for (uint32_t i = threadIdx.x; i <= S; i += kPhaseThreads) s_goff[i] = live ? goff[(uint64_t)b * S + i] : 0ull;
+    __syncthreads();
     for (uint32_t t = 0; t < S; ++t) {
-        if (pace && t >= 3 && phase_wait(ctr, nbx * (t - 2), spin_max)) pace = false;
+        const uint32_t T = pass * S + t;  // phase index across passes
+        if (pace && T >= 3 && phase_wait(ctr, nbx * (T - 2), spin_max)) pace = false;
         const uint32_t s = first + t < S ? first + t : first + t - S;
         const uint64_t e0 = s_goff[s], e1 = s_goff[s + 1];
         const uint32_t cbase = s * width;
@@ -426,6 +437,8 @@ __global__ __launch_bounds__(kPhaseThreads) void k_spmv_xphase(uint64_t n_rows, 
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < nr; i += kPhaseThreads) y[r0 + i] = (YT)s_y[i];
+    __syncthreads();  // s_y and s_goff are reused by the next pass
+    }
 }
 
 // XPHASE plan, built element-parallel in three passes (no per-row state, so
@@ -1031,6 +1044,20 @@ void ensure_xphase(spira_dcsr *A)
                                                                 P.ph_rc, P.ph_val);
         SPIRA_HIP(hipGetLastError());
     }
+    // row blocks in decreasing order of entries (the longest start first, so
+    // the light ones fill the tail)
+    {
+        std::vector<uint64_t> h(G + 1);
+        SPIRA_HIP(hipMemcpyAsync(h.data(), goff, (G + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+        dev_sync(dev);
+        std::vector<uint32_t> ord(nb);
+        for (uint32_t i = 0; i < nb; ++i) ord[i] = i;
+        std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) {
+            return h[(uint64_t)(a + 1) * S] - h[(uint64_t)a * S] > h[(uint64_t)(b + 1) * S] - h[(uint64_t)b * S];
+        });
+        P.ph_order = (uint32_t *)plan_alloc(A, nb * sizeof(uint32_t));
+        SPIRA_HIP(hipMemcpyAsync(P.ph_order, ord.data(), nb * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    }
     dev_sync(dev);  // the cursors live in the workspace: finish before they can be reused
     P.ph_S = (uint32_t)S;
     P.ph_width = (uint32_t)width;
@@ -1050,12 +1077,16 @@ void launch_xphase(spira_dcsr *A, const XT *x, YT *y, uint32_t *ctr, Vals vals =
     const spira_plan &P = A->plan;
     if constexpr (std::is_same_v<Vals, ph_f64>) vals.v = P.ph_val;
     uint32_t *sync = nullptr;
-    if (PACE && A->dev->n_cu > 0 && P.ph_nb <= (uint32_t)A->dev->n_cu) {
+    uint32_t grid = P.ph_nb;
+    if (PACE && A->dev->n_cu > 0) {
+        // one block per CU, all resident: more row blocks are taken in passes
+        grid = std::min<uint32_t>(P.ph_nb, (uint32_t)A->dev->n_cu);
         sync = ctr;
         k_phase_sync_zero<<<1, 8 * kPhaseSyncStride, 0, A->dev->stream>>>(sync);
     }
-    k_spmv_xphase<U, Vals, XT, YT><<<P.ph_nb, kPhaseThreads, 0, A->dev->stream>>>(
-        A->n_rows, P.ph_RB, P.ph_S, P.ph_goff, P.ph_width, P.ph_rc, vals, x, y, sync, A->dev->phase_spin_max);
+    k_spmv_xphase<U, Vals, XT, YT><<<grid, kPhaseThreads, 0, A->dev->stream>>>(
+        A->n_rows, P.ph_RB, P.ph_S, P.ph_nb, P.ph_order, P.ph_goff, P.ph_width, P.ph_rc, vals, x, y, sync,
+        A->dev->phase_spin_max);
 }
 
 __global__ void k_ph_to_f32(const double *__restrict__ v, float *__restrict__ o, uint64_t n)
@@ -1288,7 +1319,7 @@ void spmv_launch_planned(spira_dcsr *A, const double *x, double *y, int variant,
     launch_planned(A, x, y, variant, (uint32_t *)scratch, (double *)scratch);
 }
 
-void spmv_xphase_reduced(spira_dcsr *A, int storage, const double *x, double *y, bool paced)
+void spmv_xphase_reduced(spira_dcsr *A, int storage, const float *x, double *y, bool paced)
 {
     if (A->n_rows == 0) return;
     ensure_xphase_vals(A, storage);

@@ -368,7 +368,11 @@ void spmv_mixed_launch(spira_dcsr *A, int storage, const double *x, double *y, i
     if (xphase_variant(variant)) {
         if (ratio > 0.0f)
             fail(SPIRA_ERR_INVALID_ARGUMENT, "mixed spmv: the FP64 recompute needs a VEC or STREAM variant");
-        spmv_xphase_reduced(A, storage, x, y, variant == SPIRA_SPMV_XPHASE_PACED);
+        float *x32 = (float *)dev->ws.get(WS_X32, (A->n_cols ? A->n_cols : 1) * sizeof(float));
+        k_x_to_f32<<<grid_for(A->n_cols ? A->n_cols : 1, kThreads), kThreads, 0, st>>>(x, x32, A->n_cols,
+                                                                                         mixed_queue_count(A));
+        SPIRA_HIP(hipGetLastError());
+        spmv_xphase_reduced(A, storage, x32, y, variant == SPIRA_SPMV_XPHASE_PACED);
         return;
     }
     ensure_storage(A, storage);

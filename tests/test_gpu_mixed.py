@@ -65,14 +65,14 @@ def test_mixed_xphase_layout(powerlaw, storage, name):
 
 
 def test_mixed_xphase_fp32_exact_values(dev):
-    """FP32-exact values: the XPHASE layout's FP64 products and sums leave
-    only FP64 rounding (<= 1e-14 normwise against the FP64 oracle)."""
+    """FP32-exact values and x: the XPHASE layout's FP64 products (exact) and
+    sums leave only FP64 rounding (<= 1e-14 normwise against the FP64 oracle)."""
     n = 300000
     off, col, val = W.random_uniform_rows(n, n, 8, 32, seed=17)
     val = val.astype(np.float32).astype(np.float64)
     A = spira_amd.DeviceCSR(dev, n, n)
     A.upload(off, col, val)
-    x = np.random.default_rng(2).uniform(-1, 1, n)
+    x = np.random.default_rng(2).uniform(-1, 1, n).astype(np.float32).astype(np.float64)
     y_ref = oracle.spmv_f64(off, col, val, x)
     for name in ("xphase", "xphase_paced"):
         y, _ = A.spmv_mixed(x, storage="f32", variant=spira_amd.VARIANTS[name])
