@@ -295,19 +295,25 @@ int spira_spmv_mixed_f64(spira_dcsr *A, int storage, const double *x, uint64_t x
  * device queue's unit), and whether the matrix's X block (n_cols * k floats,
  * at most SPIRA_BATCH_XLDS_FLOATS; at k = 1 also up to
  * SPIRA_BATCH_XLDS_TWO_PASS_COLS columns, staged in two halves) is staged in
- * LDS for its gathers.
- * spira_batch_spmm runs every item in one persistent launch fed by a device
- * work queue; each output is accumulated in FP32 with compensated (Dot2)
- * summation.  The call can be captured once into a hipGraph (bound to the X
+ * LDS for its gathers, and flags: SPIRA_BATCH_STREAM reads the matrix's
+ * entries straight from global memory (items of item_nnz nonzeros, the X block
+ * staged once per item) instead of copying each item's entries into LDS beside
+ * the X block (items then also end where the LDS is full).
+ * spira_batch_spmm runs every item in one persistent launch fed by device
+ * work queues, one per XCD: matrices are dealt whole to the queues, so the
+ * CUs of an XCD share a matrix's X block in their L2, and a block whose queue
+ * is empty takes items from the others.  Each output is accumulated in FP32
+ * with compensated (Dot2) summation.  The call can be captured once into a hipGraph (bound to the X
  * and Y given) and replayed.  X and Y are device pointers; flags:
  * SPIRA_NO_SYNC.  Destroy batches before their device. */
 #define SPIRA_BATCH_XLDS_FLOATS 36864
 #define SPIRA_BATCH_XLDS_TWO_PASS_COLS 65536
 #define SPIRA_BATCH_MAX_PLOG2 5
+#define SPIRA_BATCH_STREAM 1u
 typedef struct spira_batch_sched {
     uint8_t plog2;     /* target nonzeros per lane: 1 << plog2 */
     uint8_t xlds;      /* 1: stage the X block in LDS */
-    uint16_t reserved; /* 0 */
+    uint16_t flags;    /* SPIRA_BATCH_STREAM or 0 */
     uint32_t item_nnz; /* work-item size (nonzeros + rows) */
 } spira_batch_sched;
 typedef struct spira_batch spira_batch;

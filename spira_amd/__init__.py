@@ -68,8 +68,9 @@ N_VARIANTS = 15
 BATCH_XLDS_FLOATS = 36864
 BATCH_XLDS_TWO_PASS_COLS = 65536
 BATCH_MAX_PLOG2 = 5
+BATCH_STREAM = 1
 NOALLOC_SCRATCH_BYTES = 32 << 20
-BATCH_SCHED_DTYPE = np.dtype([("plog2", "u1"), ("xlds", "u1"), ("reserved", "<u2"), ("item_nnz", "<u4")])
+BATCH_SCHED_DTYPE = np.dtype([("plog2", "u1"), ("xlds", "u1"), ("flags", "<u2"), ("item_nnz", "<u4")])
 TRANSPOSE_DROP_ZEROS = 1
 ADD_SUBTRACT = 1
 
@@ -659,10 +660,11 @@ class Batch:
         lib().spira_batch_info(self._h, None, None, None, ctypes.byref(t))
         return t.value
 
-    def set_schedule(self, k: int, plog2=None, xlds=None, item_nnz=None):
+    def set_schedule(self, k: int, plog2=None, xlds=None, item_nnz=None, stream=0):
         """Per-matrix schedule (spira_batch_sched): plog2 (target nonzeros
         per lane 2^plog2), xlds (stage the X block in LDS), item_nnz (work-item
-        size).  All three None: the library's built-in choices."""
+        size), stream (1: entries read from global memory, not staged in LDS).
+        plog2, xlds and item_nnz all None: the library's built-in choices."""
         given = [a is not None for a in (plog2, xlds, item_nnz)]
         if any(given) and not all(given):
             raise InvalidArgument("set_schedule: give plog2, xlds and item_nnz together (or none)")
@@ -673,6 +675,7 @@ class Batch:
             s["plog2"] = np.broadcast_to(np.asarray(plog2, np.int64), n)
             s["xlds"] = np.broadcast_to(np.asarray(xlds, np.int64), n)
             s["item_nnz"] = np.broadcast_to(np.asarray(item_nnz, np.int64), n)
+            s["flags"] = np.broadcast_to(np.asarray(stream, np.int64), n) * BATCH_STREAM
         self.dev.check(lib().spira_batch_set_schedule(self._h, int(k), _ptr(s)))
         self.k = int(k)
 

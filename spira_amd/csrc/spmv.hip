@@ -324,7 +324,6 @@ constexpr int kPhaseThreads = 1024;
 // CUs, so some blocks of its XCD are not running): one timeout per block at
 // most, never one per phase.
 constexpr uint32_t kPhaseSyncStride = 16;  // counters 64 bytes apart
-constexpr uint32_t kPhaseSpinMax = 1u << 14;
 
 // true when the wait gave up (the same value in every thread of the block)
 __device__ __forceinline__ bool phase_wait(const uint32_t *ctr, uint32_t target, uint32_t spin_max)

@@ -39,10 +39,14 @@ struct vals_split {
 };
 
 // ---- compensated accumulation ---------------------------------------------
+// Contraction is off inside these error-free transformations: fused into
+// FMAs (s + a * x as one rounding) the correction would count the product's
+// rounding error twice.
 struct dot2 {
     float s = 0.0f, c = 0.0f, a = 0.0f;  // sum, compensation, sum of |a*x|
     __device__ __forceinline__ void add_prod(float p, float e)
     {
+#pragma clang fp contract(off)
         const float t = s + p;
         const float bp = t - s;
         const float q = (s - (t - bp)) + (p - bp);  // TwoSum error
@@ -52,11 +56,13 @@ struct dot2 {
     }
     __device__ __forceinline__ void add(float av, float xv)
     {
+#pragma clang fp contract(off)
         const float p = av * xv;
         add_prod(p, fmaf(av, xv, -p));
     }
     __device__ __forceinline__ void merge(float s2, float c2, float a2)
     {
+#pragma clang fp contract(off)
         const float t = s + s2;
         const float bp = t - s;
         const float q = (s - (t - bp)) + (s2 - bp);

@@ -66,15 +66,16 @@ def test_constants_match_python_mirror():
     assert d["SPIRA_BATCH_XLDS_FLOATS"] == spira_amd.BATCH_XLDS_FLOATS
     assert d["SPIRA_BATCH_XLDS_TWO_PASS_COLS"] == spira_amd.BATCH_XLDS_TWO_PASS_COLS
     assert d["SPIRA_BATCH_MAX_PLOG2"] == spira_amd.BATCH_MAX_PLOG2
+    assert d["SPIRA_BATCH_STREAM"] == spira_amd.BATCH_STREAM
 
 
 def test_batch_schedule_struct_layout():
-    """spira_batch_sched is {u8 plog2, u8 xlds, u16 reserved, u32 item_nnz}:
+    """spira_batch_sched is {u8 plog2, u8 xlds, u16 flags, u32 item_nnz}:
     8 bytes, matched by the Python record type."""
     src = open(HEADER).read()
     body = re.search(r"typedef struct spira_batch_sched \{(.*?)\} spira_batch_sched;", src, re.S).group(1)
     fields = re.findall(r"(uint\d+_t)\s+(\w+);", body)
-    assert fields == [("uint8_t", "plog2"), ("uint8_t", "xlds"), ("uint16_t", "reserved"), ("uint32_t", "item_nnz")]
+    assert fields == [("uint8_t", "plog2"), ("uint8_t", "xlds"), ("uint16_t", "flags"), ("uint32_t", "item_nnz")]
     dt = spira_amd.BATCH_SCHED_DTYPE
     assert dt.itemsize == 8 and list(dt.names) == [f for _, f in fields]
     assert [dt.fields[n][1] for n in dt.names] == [0, 1, 2, 4]
