@@ -172,7 +172,10 @@ __global__ void k_reset_counters(uint32_t *counter)
 }
 
 // Groups of entries (4 columns, 4 values) from the global arrays or the
-// item's LDS copy, one storage format each.
+// item's LDS copy, one storage format each.  Global reads are ordinary loads:
+// with X in LDS nothing of the batch's is worth keeping in L2, and measured
+// on MI355X the non-temporal form is slower here (k = 1, 4096 matrices: 3.68
+// against 3.20 ms).
 template <int ST, typename CT, bool LDS>
 struct gsrc {
     const CT *col;
@@ -181,8 +184,7 @@ struct gsrc {
     template <typename V>
     static __device__ __forceinline__ V ld(const V *p)
     {
-        if constexpr (LDS) return *p;
-        else return __builtin_nontemporal_load(p);
+        return *p;
     }
     static __device__ __forceinline__ void bf16x4(const u32x2 w, float (&v)[kGroup])
     {
@@ -310,10 +312,37 @@ __device__ __forceinline__ void rows2_groups(acc2 &a0, acc2 &a1, const row_rng &
     }
 }
 
+// Lane i's value of lane i + O inside its row of 16 lanes (DPP row_shl; the
+// last O lanes of a row read 0, which no merge below uses).
+template <uint32_t O>
+__device__ __forceinline__ float lane_up(float v)
+{
+    static_assert(O >= 1 && O < 16, "DPP row shifts stay inside a row of 16 lanes");
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x100 + O, 0xf, 0xf, false));
+}
+
+template <uint32_t O>
+__device__ __forceinline__ void merge_up(acc2 &a, uint32_t span)
+{
+    if constexpr (O < 16) {
+        if (span > O) a.merge(lane_up<O>(a.s), lane_up<O>(a.c));
+    }
+}
+
+// Merge the E lanes of a row (K apart) into the row's first lane.  Round o
+// adds lane i + o to lane i: for the first lane this is the xor butterfly's
+// order, so the result does not depend on how a round moves the values.
+// Offsets inside a row of 16 lanes use DPP, the rest (E * K > 16) ds_bpermute.
 template <int K>
 __device__ __forceinline__ void merge_lanes(acc2 &a, uint32_t E)
 {
-    for (uint32_t o = K; o < K * E; o <<= 1) a.merge(__shfl_xor(a.s, (int)o, 64), __shfl_xor(a.c, (int)o, 64));
+    const uint32_t span = (uint32_t)K * E;
+    merge_up<(uint32_t)K>(a, span);
+    merge_up<2u * K>(a, span);
+    merge_up<4u * K>(a, span);
+    merge_up<8u * K>(a, span);
+    for (uint32_t o = K < 16 ? 16u : (uint32_t)K; o < span; o <<= 1)
+        a.merge(__shfl_xor(a.s, (int)o, 64), __shfl_xor(a.c, (int)o, 64));
 }
 
 // Rows p, p + S, p + 2 S, ... (< r1) of one lane group, two per step, the
@@ -353,6 +382,7 @@ __device__ __forceinline__ row_rng row_of(uint32_t g0, uint32_t g1, uint32_t rid
 // with D 16-byte loads in flight per lane, or (D = 0) straight into LDS: one
 // wavefront instruction moves 64 consecutive chunks, lane l's to dst + 16 l
 // (lanes past the end masked off); the caller waits with stage_wait<0>().
+// Ordinary (not non-temporal) loads, as for the entries.
 template <uint32_t D>
 __device__ __forceinline__ void stage16(unsigned char *dst, const unsigned char *src, uint32_t nbytes)
 {
@@ -364,8 +394,7 @@ __device__ __forceinline__ void stage16(unsigned char *dst, const unsigned char 
         for (uint32_t c0 = threadIdx.x & ~(uint32_t)(kWave - 1); c0 < n; c0 += kThreads)
             if (c0 + lane < n)
                 __builtin_amdgcn_global_load_lds((const void *)(s + c0 + lane),
-                                                 (__attribute__((address_space(3))) void *)(d + c0), 16, 0,
-                                                 2 /* non-temporal */);
+                                                 (__attribute__((address_space(3))) void *)(d + c0), 16, 0, 0);
         return;
     }
     uint32_t i = threadIdx.x;
@@ -373,12 +402,12 @@ __device__ __forceinline__ void stage16(unsigned char *dst, const unsigned char 
         for (; i + (D - 1) * kThreads < n; i += D * kThreads) {
             u32x4 v[D];
 #pragma unroll
-            for (uint32_t u = 0; u < D; ++u) v[u] = __builtin_nontemporal_load(s + i + u * kThreads);
+            for (uint32_t u = 0; u < D; ++u) v[u] = s[i + u * kThreads];
 #pragma unroll
             for (uint32_t u = 0; u < D; ++u) d[i + u * kThreads] = v[u];
         }
     }
-    for (; i < n; i += kThreads) d[i] = __builtin_nontemporal_load(s + i);
+    for (; i < n; i += kThreads) d[i] = s[i];
 }
 
 // Stage one array: the 16-byte aligned span holding elements [base, base + n)
