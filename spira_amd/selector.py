@@ -3,8 +3,9 @@
 For every matrix of a batch and every k (1, 4, 16, 32) a schedule is one of
 SCHEDULES: plog2 (target nonzeros per lane, which sets the lanes per row of
 every row-length class and so the rows per wavefront, 64 / (lanes * k)),
-xlds (stage the matrix's X block in LDS: the LDS tile) and the work-item
-size class.  The exhaustive autotuner (`exhaustive`) runs every schedule on
+xlds (stage the matrix's X block in LDS: the LDS tile), the work-item size
+class and stream (read the entries from global memory instead of copying
+each item's entries into LDS beside the X block).  The exhaustive autotuner (`exhaustive`) runs every schedule on
 the whole batch with device timestamps on every work item
 (`Batch.profile`), which gives every matrix's cost inside its batch from
 one launch per schedule.  The selector is a small MLP (hand-written HIP
@@ -30,7 +31,7 @@ from . import (BATCH_MAX_PLOG2, BATCH_XLDS_FLOATS, BATCH_XLDS_TWO_PASS_COLS, Bat
 K_LIST = (1, 4, 16, 32)
 PLOG2 = tuple(range(BATCH_MAX_PLOG2 + 1))
 ITEMS = ("default", "large")        # work-item size class
-SCHEDULES = [(p, x, t) for p in PLOG2 for x in (0, 1) for t in range(len(ITEMS))]
+SCHEDULES = [(p, x, t, st) for p in PLOG2 for x in (0, 1) for t in range(len(ITEMS)) for st in (0, 1)]
 N_OUT = len(SCHEDULES)
 HIDDEN = 64
 MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "selector_model.json")
@@ -49,14 +50,15 @@ def xlds_fits(n_cols, k: int):
 
 def valid_mask(n_cols: int, k: int) -> np.ndarray:
     fits = bool(xlds_fits(n_cols, k))
-    return np.array([(x == 0) or fits for p, x, t in SCHEDULES], dtype=bool)
+    return np.array([(x == 0) or fits for p, x, t, st in SCHEDULES], dtype=bool)
 
 
 def heuristic_schedule(n_cols: int, k: int) -> int:
     """The batch's built-in schedule (plog2 2, X block in LDS at k = 1 when
-    it fits in one or two loads, default items), as an index into SCHEDULES."""
+    it fits in one or two loads, default items, entries staged), as an index
+    into SCHEDULES."""
     x = 1 if (k == 1 and bool(xlds_fits(n_cols, k))) else 0
-    return SCHEDULES.index((2, x, 0))
+    return SCHEDULES.index((2, x, 0, 0))
 
 
 def features_for(A: DeviceCSR, k: int) -> np.ndarray:
@@ -83,7 +85,8 @@ def apply(batch: Batch, k: int, choice, n_cols):
     p = np.array([SCHEDULES[i][0] for i in choice])
     x = np.array([SCHEDULES[i][1] for i in choice]) * xlds_fits(n_cols, k)
     t = np.array([item_nnz(k, SCHEDULES[i][2]) for i in choice])
-    batch.set_schedule(k, plog2=p, xlds=x.astype(np.int64), item_nnz=t)
+    st = np.array([SCHEDULES[i][3] for i in choice])
+    batch.set_schedule(k, plog2=p, xlds=x.astype(np.int64), item_nnz=t, stream=st)
 
 
 def exhaustive(batch: Batch, k: int, n_cols, X, Y, reps: int = 2) -> np.ndarray:
@@ -93,11 +96,11 @@ def exhaustive(batch: Batch, k: int, n_cols, X, Y, reps: int = 2) -> np.ndarray:
     n = len(batch.mats)
     fits = xlds_fits(n_cols, k)
     out = np.full((n, N_OUT), np.nan)
-    for i, (p, x, t) in enumerate(SCHEDULES):
+    for i, (p, x, t, st) in enumerate(SCHEDULES):
         if x and not fits.any():
             continue
         xl = (fits if x else np.zeros(n, bool)).astype(np.int64)
-        batch.set_schedule(k, plog2=p, xlds=xl, item_nnz=item_nnz(k, t))
+        batch.set_schedule(k, plog2=p, xlds=xl, item_nnz=item_nnz(k, t), stream=st)
         batch.spmm(X, Y, sync=True)  # warm
         c = np.mean([batch.profile(X, Y) for _ in range(reps)], axis=0)
         out[:, i] = np.where(fits | (x == 0), c, np.nan)
