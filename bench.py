@@ -265,6 +265,44 @@ def _cpu_model():
     return "unknown"
 
 
+def _physical_cpus():
+    """One CPU per physical core among the CPUs this process may run on (the
+    lowest-numbered SMT sibling of each core), as bench/run_benchmark.sh
+    pins its threads away from SMT siblings."""
+    try:
+        allowed = sorted(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = list(range(os.cpu_count() or 1))
+    seen, out = set(), []
+    for c in allowed:
+        key = None
+        try:
+            with open(f"/sys/devices/system/cpu/cpu{c}/topology/thread_siblings_list") as f:
+                key = f.read().strip()
+        except OSError:
+            key = str(c)
+        if key not in seen:
+            seen.add(key)
+            out.append(c)
+    return out
+
+
+def _llc_bytes():
+    try:
+        v = os.sysconf("SC_LEVEL3_CACHE_SIZE")
+        if v and v > 0:
+            return int(v)
+    except (ValueError, OSError):
+        pass
+    try:
+        with open("/sys/devices/system/cpu/cpu0/cache/index3/size") as f:
+            t = f.read().strip().upper()
+        mult = {"K": 1 << 10, "M": 1 << 20, "G": 1 << 30}
+        return int(t[:-1]) * mult[t[-1]] if t[-1] in mult else int(t)
+    except (OSError, ValueError, KeyError):
+        return 32 << 20
+
+
 def _time_calls(torch, stream, fn, reps):
     """Mean ms per call of fn() over reps back-to-back calls on `stream`."""
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -732,56 +770,81 @@ def main():
             cores = len(os.sched_getaffinity(0))
         except AttributeError:
             cores = os.cpu_count() or 1
-        threads = max(1, min(16, cores))
-        log(f"bench: CPU baseline with {threads} threads")
-        pools = {}
-        for name in names:
-            off, col, val = mats[name]
-            n = len(off) - 1
-            p = oracle.Pool(threads, n, n)
-            p.load_csr(off, col, val)
-            pools[name] = p
-        ybuf = {name: np.empty(info[name]["n_rows"]) for name in names}
-        for name in names:  # warm
-            pools[name].spmv(info[name]["x_host"], ybuf[name])
-        reps, t_cpu = 0, 0.0
-        tstart = time.perf_counter()
-        while True:
-            for name in names:
-                pools[name].spmv(info[name]["x_host"], ybuf[name])
-            reps += 1
-            t_cpu = time.perf_counter() - tstart
-            if t_cpu >= args.cpu_seconds or reps >= 200:
-                break
+        phys = _physical_cpus()
+        threads = max(1, min(16, len(phys)))
+        llc = _llc_bytes()
+        log(f"bench: CPU baseline with {threads} pinned threads (one per physical core), LLC {llc >> 20} MiB")
         cpu_bytes = sum(W.spmv_bytes(info[k]["n_rows"], info[k]["n_rows"], info[k]["nnz"], s_ptr=8) for k in names)
-        cpu = {"value": round(cpu_bytes * reps / t_cpu / 1e9, 2), "unit": "GB/s", "cores": threads, "kind": "port",
-               "sample": f"{reps} warm steps (both matrices, full size) in {t_cpu:.1f}s; oracle/ threaded "
-                         f"parallel_matrix restatement, uniform row split, u64 offsets, scalar loop -O3 "
-                         f"-march=x86-64-v3",
-               "ms_per_step": round(1e3 * t_cpu / reps, 3),
-               "host": {"cpu_model": _cpu_model(), "nproc": os.cpu_count(), "affinity": cores}}
+        ybuf = {name: np.empty(info[name]["n_rows"]) for name in names}
+
+        def step_times(pools, budget, min_steps, max_steps, cold):
+            """Per-step seconds (one SpMV on each matrix) until budget s and min_steps."""
+            for name in names:  # warm-up step
+                pools[name].spmv(info[name]["x_host"], ybuf[name])
+            ts, spent = [], 0.0
+            while (spent < budget or len(ts) < min_steps) and len(ts) < max_steps:
+                t = 0.0
+                for name in names:
+                    if cold:  # every worker evicts its share of the LLC, untimed
+                        pools[name].flush_llc(llc)
+                    t0 = time.perf_counter()
+                    pools[name].spmv(info[name]["x_host"], ybuf[name])
+                    t += time.perf_counter() - t0
+                ts.append(t)
+                spent += t
+            return ts
+
+        def make_pools(nt, simd):
+            pools, kern = {}, None
+            for name in names:
+                off, col, val = mats[name]
+                n = len(off) - 1
+                p = oracle.Pool(nt, n, n)
+                p.pin(phys[:nt])
+                p.load_csr(off, col, val)  # each pinned worker first-touches its partition
+                kern = p.set_simd(simd)
+                pools[name] = p
+            return pools, kern
+
+        def rate(ts):
+            return round(cpu_bytes / float(np.median(ts)) / 1e9, 2)
+
+        pools, kern = make_pools(threads, True)
+        warm = step_times(pools, args.cpu_seconds * 0.4, 5, 400, cold=False)
+        cold = step_times(pools, args.cpu_seconds * 0.2, 5, 100, cold=True)
         for p in pools.values():
             p.close()
-        # one thread, same matrices (a bounded sample of steps)
-        p1 = {name: oracle.Pool(1, info[name]["n_rows"], info[name]["n_rows"]) for name in names}
-        for name in names:
-            p1[name].load_csr(*mats[name])
-        reps1, t1 = 0, 0.0
-        tstart = time.perf_counter()
-        while t1 < args.cpu_seconds / 3 and reps1 < 50:
-            for name in names:
-                p1[name].spmv(info[name]["x_host"], ybuf[name])
-            reps1 += 1
-            t1 = time.perf_counter() - tstart
-        cpu["one_thread"] = {"value": round(cpu_bytes * reps1 / t1 / 1e9, 2), "unit": "GB/s",
-                             "ms_per_step": round(1e3 * t1 / reps1, 3), "steps": reps1}
-        for p in p1.values():
+        pools, _ = make_pools(threads, False)
+        loop = step_times(pools, args.cpu_seconds * 0.15, 5, 200, cold=False)
+        for p in pools.values():
             p.close()
+        cpu = {"value": rate(warm), "unit": "GB/s", "cores": threads, "kind": "port",
+               "sample": f"median of {len(warm)} warm steps (one SpMV on each matrix, full size); oracle/ "
+                         f"threaded parallel_matrix restatement: uniform row split, one partition CSR per "
+                         f"pinned worker (one per physical core), u64 offsets, per-row {kern} gather dot "
+                         f"(the dot dispatch.cpp selects on this CPU), -O3 -march=x86-64-v3",
+               "ms_per_step": round(1e3 * float(np.median(warm)), 3),
+               "dot": kern,
+               "cold_llc": {"value": rate(cold), "unit": "GB/s", "steps": len(cold),
+                            "ms_per_step": round(1e3 * float(np.median(cold)), 3),
+                            "flush": f"each worker reads {llc >> 20} MiB before each SpMV (untimed)"},
+               "scalar_loop": {"value": rate(loop), "unit": "GB/s", "steps": len(loop),
+                               "ms_per_step": round(1e3 * float(np.median(loop)), 3)},
+               "host": {"cpu_model": _cpu_model(), "nproc": os.cpu_count(), "affinity": cores,
+                        "physical_cores": len(phys), "pinned_cpus": phys[:threads], "llc_bytes": llc}}
+        # one thread, same matrices and dot (a bounded sample of steps)
+        pools, _ = make_pools(1, True)
+        one = step_times(pools, args.cpu_seconds * 0.25, 3, 50, cold=False)
+        for p in pools.values():
+            p.close()
+        cpu["one_thread"] = {"value": rate(one), "unit": "GB/s",
+                             "ms_per_step": round(1e3 * float(np.median(one)), 3), "steps": len(one)}
         if lock_detail is not None:
             o3, c3, v3, sp3, sc3, sv3, n3, lbytes = lock_src
             tl = []
-            for rep in range(3):
+            for rep in range(5):
                 p = oracle.Pool(threads, n3, n3)
+                p.pin(phys[:threads])
                 p.load_csr(o3, c3, v3)  # install the locked matrix as the partitions' CSR
                 t = time.perf_counter()
                 p.lock(sp3, sc3, sv3)
@@ -794,6 +857,7 @@ def main():
                 p.close()
             lock_detail["cpu_baseline_ms"] = round(1e3 * float(np.median(tl)), 2)
             lock_detail["cpu_baseline_threads"] = threads
+            lock_detail["cpu_baseline_reps"] = len(tl)
         if tr_detail is not None:
             o, c, v, got = tr_src
             tt = []

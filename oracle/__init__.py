@@ -73,6 +73,10 @@ def lib():
             "spira_or_pool_create": (ctypes.c_void_p, [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64]),
             "spira_or_pool_destroy": (None, [ctypes.c_void_p]),
             "spira_or_pool_threads": (ctypes.c_uint32, [ctypes.c_void_p]),
+            "spira_or_pool_set_kernel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+            "spira_or_pool_force_kernel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+            "spira_or_pool_pin": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
+            "spira_or_pool_flush": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64]),
             "spira_or_pool_partition": (None, [ctypes.c_void_p, ctypes.c_uint32, _u64p, _u64p]),
             "spira_or_pool_load_csr": (ctypes.c_int, [ctypes.c_void_p, _u64p, _u32p, _f64p]),
             "spira_or_pool_spmv": (ctypes.c_int, [ctypes.c_void_p, _f64p, _f64p]),
@@ -296,6 +300,30 @@ class Pool:
             self.close()
         except Exception:
             pass
+
+    KERNELS = {0: "scalar loop", 1: "scalar dot", 2: "avx2", 3: "avx512"}
+
+    def set_simd(self, on=True):
+        """Row dot of spmv(): the SIMD gather dot the reference would dispatch
+        to on this CPU (AVX-512F, else AVX2+FMA), or the scalar loop.  Returns
+        the name of the dot in use."""
+        return self.KERNELS[int(lib().spira_or_pool_set_kernel(self._h, 1 if on else 0))]
+
+    def force_kernel(self, name):
+        which = {v: k for k, v in self.KERNELS.items()}[name]
+        if lib().spira_or_pool_force_kernel(self._h, which) != 0:
+            raise RuntimeError(f"{name} is not supported on this CPU")
+
+    def pin(self, cpus):
+        """Pin worker t to CPU cpus[t]."""
+        arr = (ctypes.c_int * self.n_threads)(*[int(c) for c in cpus[: self.n_threads]])
+        if len(cpus) < self.n_threads or lib().spira_or_pool_pin(self._h, arr) != 0:
+            raise RuntimeError("pinning the pool's workers failed")
+
+    def flush_llc(self, nbytes):
+        """Every worker reads an arena of nbytes (the cold-LLC mode)."""
+        if lib().spira_or_pool_flush(self._h, int(nbytes)) != 0:
+            raise MemoryError("flush arena allocation failed")
 
     def partition(self, t):
         a = ctypes.c_uint64()

@@ -8,10 +8,13 @@
 #define _GNU_SOURCE
 #include "spira_oracle.h"
 
+#include <immintrin.h>
 #include <math.h>
 #include <pthread.h>
+#include <sched.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
 int spira_or_is_zero(double v) { return fabs(v) <= 0.0; }
 static inline int is_zero_f(double v) { return fabs(v) <= 0.0; }
@@ -430,7 +433,7 @@ typedef struct {
     double *val;
 } part_t;
 
-typedef enum { JOB_NONE, JOB_LOAD, JOB_SPMV, JOB_LOCK, JOB_EXIT } job_t;
+typedef enum { JOB_NONE, JOB_LOAD, JOB_SPMV, JOB_LOCK, JOB_FLUSH, JOB_EXIT } job_t;
 
 struct spira_or_pool {
     uint32_t n;
@@ -449,6 +452,13 @@ struct spira_or_pool {
     const double *a_x;
     double *a_y;
     int failed;
+    /* row dot product of JOB_SPMV: the scalar loop above (reference order) or
+     * the per-row SIMD gather dot that dispatch.cpp:34-67 selects */
+    double (*dot)(const double *vals, const uint32_t *cols, const double *x, uint64_t n);
+    const int *cpus;     /* JOB_PIN: CPU of each worker */
+    uint64_t flush_bytes; /* JOB_FLUSH: bytes each worker reads */
+    char **arenas;       /* JOB_FLUSH: one arena per worker */
+    uint64_t sink;       /* keeps the flush reads live */
 };
 
 typedef struct {
@@ -476,12 +486,83 @@ static void job_spmv(spira_or_pool *p, part_t *q)
     const double *x = p->a_x;
     double *y = p->a_y + q->row_start;
     uint64_t n = q->row_end - q->row_start;
+    if (p->dot) {
+        for (uint64_t i = 0; i < n; ++i)
+            y[i] = p->dot(q->val + q->off[i], q->col + q->off[i], x, q->off[i + 1] - q->off[i]);
+        return;
+    }
     for (uint64_t i = 0; i < n; ++i) {
         double acc = 0.0;
         for (uint64_t k = q->off[i]; k < q->off[i + 1]; ++k)
             acc += x[q->col[k]] * q->val[k];
         y[i] = acc;
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Per-row SIMD gather dots of the CPU baseline (timing only; the oracle's  */
+/* own SpMV above keeps the reference's scalar order).  Four accumulators    */
+/* over blocks of four vectors, then single vectors, a fixed pairwise        */
+/* reduction and a scalar tail: the shape of dot_avx.cpp:6-56 (4 doubles per */
+/* gather) and dot_avx512.cpp:7-56 (8 per gather), which the reference picks */
+/* at start-up (dispatch.cpp:34-47: AVX-512F first, then AVX2 + FMA).        */
+/* ------------------------------------------------------------------------ */
+__attribute__((target("avx2,fma"))) static double dot_avx2(const double *v, const uint32_t *c,
+                                                          const double *x, uint64_t n)
+{
+    __m256d a[4] = {_mm256_setzero_pd(), _mm256_setzero_pd(), _mm256_setzero_pd(), _mm256_setzero_pd()};
+    uint64_t i = 0;
+    for (; i + 16 <= n; i += 16)
+        for (int j = 0; j < 4; ++j) {
+            const __m128i ix = _mm_loadu_si128((const __m128i *)(c + i + 4 * j));
+            a[j] = _mm256_fmadd_pd(_mm256_loadu_pd(v + i + 4 * j), _mm256_i32gather_pd(x, ix, 8), a[j]);
+        }
+    for (; i + 4 <= n; i += 4) {
+        const __m128i ix = _mm_loadu_si128((const __m128i *)(c + i));
+        a[0] = _mm256_fmadd_pd(_mm256_loadu_pd(v + i), _mm256_i32gather_pd(x, ix, 8), a[0]);
+    }
+    const __m256d s = _mm256_add_pd(_mm256_add_pd(a[0], a[1]), _mm256_add_pd(a[2], a[3]));
+    const __m128d h = _mm_add_pd(_mm256_castpd256_pd128(s), _mm256_extractf128_pd(s, 1));
+    double acc = _mm_cvtsd_f64(_mm_add_sd(h, _mm_unpackhi_pd(h, h)));
+    for (; i < n; ++i) acc += v[i] * x[c[i]];
+    return acc;
+}
+
+__attribute__((target("avx512f"))) static double dot_avx512(const double *v, const uint32_t *c,
+                                                          const double *x, uint64_t n)
+{
+    __m512d a[4] = {_mm512_setzero_pd(), _mm512_setzero_pd(), _mm512_setzero_pd(), _mm512_setzero_pd()};
+    uint64_t i = 0;
+    for (; i + 32 <= n; i += 32)
+        for (int j = 0; j < 4; ++j) {
+            const __m256i ix = _mm256_loadu_si256((const __m256i *)(c + i + 8 * j));
+            a[j] = _mm512_fmadd_pd(_mm512_loadu_pd(v + i + 8 * j), _mm512_i32gather_pd(ix, x, 8), a[j]);
+        }
+    for (; i + 8 <= n; i += 8) {
+        const __m256i ix = _mm256_loadu_si256((const __m256i *)(c + i));
+        a[0] = _mm512_fmadd_pd(_mm512_loadu_pd(v + i), _mm512_i32gather_pd(ix, x, 8), a[0]);
+    }
+    double acc = _mm512_reduce_add_pd(_mm512_add_pd(_mm512_add_pd(a[0], a[1]), _mm512_add_pd(a[2], a[3])));
+    for (; i < n; ++i) acc += x[c[i]] * v[i];
+    return acc;
+}
+
+static double dot_scalar(const double *v, const uint32_t *c, const double *x, uint64_t n)
+{
+    double acc = 0.0;
+    for (uint64_t i = 0; i < n; ++i) acc += v[i] * x[c[i]];
+    return acc;
+}
+
+/* Cold-cache mode of bench/spira_bench.cpp:43-67: read one byte per 64-byte
+ * line of an arena of the last-level cache's size.  Each worker reads its
+ * own arena, so every core's share of the LLC is evicted. */
+static void job_flush(spira_or_pool *p, uint32_t id)
+{
+    char *a = p->arenas[id];
+    uint64_t s = 0;
+    for (uint64_t i = 0; i < p->flush_bytes; i += 64) s += (unsigned char)a[i];
+    __atomic_fetch_add(&p->sink, s, __ATOMIC_RELAXED);
 }
 
 /* lock_partition (parallel_matrix.hpp:321-386) over flat staged segments. */
@@ -546,6 +627,7 @@ static void *worker_main(void *arg)
         if (job == JOB_LOAD) job_load(p, q);
         else if (job == JOB_SPMV) job_spmv(p, q);
         else if (job == JOB_LOCK) rc = job_lock(p, q);
+        else if (job == JOB_FLUSH) job_flush(p, id);
         pthread_mutex_lock(&p->mu);
         if (rc) p->failed = 1;
         if (--p->remaining == 0) pthread_cond_signal(&p->cv_done);
@@ -603,6 +685,10 @@ void spira_or_pool_destroy(spira_or_pool *p)
     for (uint32_t t = 0; t < p->n; ++t) {
         free(p->parts[t].off); free(p->parts[t].col); free(p->parts[t].val);
     }
+    if (p->arenas) {
+        for (uint32_t t = 0; t < p->n; ++t) free(p->arenas[t]);
+        free(p->arenas);
+    }
     free(p->parts);
     free(p->threads);
     pthread_mutex_destroy(&p->mu);
@@ -612,6 +698,64 @@ void spira_or_pool_destroy(spira_or_pool *p)
 }
 
 uint32_t spira_or_pool_threads(const spira_or_pool *p) { return p->n; }
+
+int spira_or_pool_set_kernel(spira_or_pool *p, int simd)
+{
+    if (!simd) {
+        p->dot = NULL;
+        return 0;
+    }
+    __builtin_cpu_init();
+    if (__builtin_cpu_supports("avx512f")) {
+        p->dot = dot_avx512;
+        return 3;
+    }
+    if (__builtin_cpu_supports("avx2") && __builtin_cpu_supports("fma")) {
+        p->dot = dot_avx2;
+        return 2;
+    }
+    p->dot = dot_scalar;
+    return 1;
+}
+
+int spira_or_pool_force_kernel(spira_or_pool *p, int which)
+{
+    __builtin_cpu_init();
+    if (which == 3 && !__builtin_cpu_supports("avx512f")) return -1;
+    if (which == 2 && !(__builtin_cpu_supports("avx2") && __builtin_cpu_supports("fma"))) return -1;
+    p->dot = which == 3 ? dot_avx512 : which == 2 ? dot_avx2 : which == 1 ? dot_scalar : NULL;
+    return 0;
+}
+
+int spira_or_pool_pin(spira_or_pool *p, const int *cpus)
+{
+    int failed = 0;
+    for (uint32_t t = 0; t < p->n; ++t) {
+        cpu_set_t set;
+        CPU_ZERO(&set);
+        CPU_SET(cpus[t], &set);
+        if (pthread_setaffinity_np(p->threads[t], sizeof(set), &set) != 0) failed = 1;
+    }
+    return failed ? -1 : 0;
+}
+
+int spira_or_pool_flush(spira_or_pool *p, uint64_t bytes)
+{
+    if (!p->arenas || p->flush_bytes != bytes) {
+        if (p->arenas) {
+            for (uint32_t t = 0; t < p->n; ++t) free(p->arenas[t]);
+            free(p->arenas);
+        }
+        p->arenas = (char **)calloc(p->n, sizeof(char *));
+        for (uint32_t t = 0; t < p->n; ++t) {
+            p->arenas[t] = (char *)malloc(bytes ? bytes : 1);
+            if (!p->arenas[t]) return -1;
+            memset(p->arenas[t], 1, bytes);
+        }
+        p->flush_bytes = bytes;
+    }
+    return pool_run(p, JOB_FLUSH);
+}
 
 void spira_or_pool_partition(const spira_or_pool *p, uint32_t t,
                              uint64_t *row_start, uint64_t *row_end)

@@ -92,6 +92,18 @@ spira_or_pool *spira_or_pool_create(uint32_t n_threads, uint64_t n_rows,
                                     uint64_t n_cols);
 void spira_or_pool_destroy(spira_or_pool *p);
 uint32_t spira_or_pool_threads(const spira_or_pool *p);
+/* Row dot product of pool SpMV: 0 = the scalar reference-order loop (the
+ * default); nonzero = the SIMD gather dot the reference's dispatch.cpp:34-67
+ * would pick on this CPU.  Returns 0 scalar loop, 1 scalar dot, 2 AVX2, 3
+ * AVX-512. */
+int spira_or_pool_set_kernel(spira_or_pool *p, int simd);
+/* Force one dot: 0 loop, 1 scalar dot, 2 AVX2, 3 AVX-512 (-1 if unsupported). */
+int spira_or_pool_force_kernel(spira_or_pool *p, int which);
+/* Pin worker t to CPU cpus[t] (pthread_setaffinity_np).  0 on success. */
+int spira_or_pool_pin(spira_or_pool *p, const int *cpus);
+/* Every worker reads one byte per 64-byte line of its own arena of `bytes`
+ * (allocated and touched on first use): the cold-LLC mode. */
+int spira_or_pool_flush(spira_or_pool *p, uint64_t bytes);
 /* Row range of partition t (uniform split). */
 void spira_or_pool_partition(const spira_or_pool *p, uint32_t t,
                              uint64_t *row_start, uint64_t *row_end);

@@ -257,6 +257,40 @@ def test_pool_matches_serial(threads):
     pool.close()
 
 
+@pytest.mark.parametrize("kernel", ["scalar dot", "avx2", "avx512"])
+def test_pool_simd_dots_within_the_dot_bound(kernel):
+    """The CPU baseline's per-row gather dots (dot_avx.cpp / dot_avx512.cpp
+    shape): every row of lengths 0..80 (all block and tail paths) within the
+    summation bound n * 2^-53 * sum |a x| of the exact dot (math.fsum)."""
+    import math
+    rng = np.random.default_rng(11)
+    lens = np.tile(np.arange(81), 3)
+    n, nc = len(lens), 5000
+    off = np.zeros(n + 1, np.uint64)
+    off[1:] = np.cumsum(lens)
+    col = rng.integers(0, nc, int(off[-1])).astype(np.uint32)
+    val = rng.normal(size=int(off[-1]))
+    x = rng.uniform(-1, 1, nc)
+    pool = oracle.Pool(3, n, nc)
+    pool.load_csr(off, col, val)
+    try:
+        pool.force_kernel(kernel)
+    except RuntimeError:
+        pool.close()
+        pytest.skip(f"{kernel} not supported on this CPU")
+    pool.pin([0, 0, 0])
+    pool.flush_llc(1 << 16)
+    y = pool.spmv(x)
+    pool.close()
+    for i in range(n):
+        a, b = int(off[i]), int(off[i + 1])
+        prods = val[a:b] * x[col[a:b]]
+        exact = math.fsum(prods)
+        bound = (b - a + 1) * 2.0 ** -53 * float(np.abs(prods).sum())
+        assert abs(y[i] - exact) <= bound, (i, b - a)
+    assert oracle.Pool(1, 2, 2).set_simd(True) in ("scalar dot", "avx2", "avx512")
+
+
 def test_pool_rejects_zero_threads_and_unlocked_spmv():
     with pytest.raises(ValueError):
         oracle.Pool(0, 10, 10)
