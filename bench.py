@@ -265,6 +265,39 @@ def _cpu_model():
     return "unknown"
 
 
+XPHASE_ROW_BLOCK = 16384  # rows per XPHASE row block (the plan's block: 128 KiB of FP64 y in LDS)
+X_LINE_DOUBLES = 16       # FP64 elements per 128-byte x line
+
+
+def xphase_ceiling(np, torch, dev, mat, kern_ms):
+    """What the XPHASE kernels' requests can cost at best on this GPU.
+    They issue two kinds from one per-CU memory pipeline: one L1-to-L2
+    request per distinct x line per row block (the entries of a block's
+    (block, slice) group are sorted by column, so lanes on one line share a
+    request; counted exactly here) and the entry stream (12 bytes per entry,
+    y written once, x brought into each of the 8 XCDs' L2).  Counters on the
+    part show the two queue behind one another in that pipeline (the same
+    requests in flight per CU with and without the gathers), so the ceiling
+    is the sum of their times at the probed rates."""
+    off, col, _ = mat
+    n_rows, n_cols, nnz = len(off) - 1, int(col.max()) + 1 if len(col) else 1, len(col)
+    g_rate, s_rate = dev.probe_ceilings()
+    o = torch.from_numpy(np.asarray(off, np.int64)).cuda()
+    rows = torch.repeat_interleave(torch.arange(n_rows, device="cuda"), o[1:] - o[:-1])
+    c = torch.from_numpy(np.asarray(col, np.int64)).cuda()
+    key = (rows // XPHASE_ROW_BLOCK) * (n_cols // X_LINE_DOUBLES + 1) + c // X_LINE_DOUBLES
+    lines = int(torch.unique(key).numel())
+    del o, rows, c, key
+    stream_bytes = 12 * nnz + 8 * n_rows + 8 * 8 * n_cols
+    t_g = lines / g_rate * 1e3
+    t_s = stream_bytes / s_rate * 1e3
+    return {"model": "x-line requests at the probed L2 gather rate + entry stream at the probed read rate",
+            "gather_Greq_s": round(g_rate / 1e9, 1), "stream_read_TB_s": round(s_rate / 1e12, 3),
+            "x_line_requests": lines, "requests_per_entry": round(lines / max(nnz, 1), 4),
+            "stream_bytes": stream_bytes, "gather_ms": round(t_g, 4), "stream_ms": round(t_s, 4),
+            "ceiling_ms": round(t_g + t_s, 4), "frac_of_ceiling": round((t_g + t_s) / kern_ms, 4)}
+
+
 def _physical_cpus():
     """One CPU per physical core among the CPUs this process may run on (the
     lowest-numbered SMT sibling of each core), as bench/run_benchmark.sh
@@ -754,6 +787,9 @@ def main():
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "bytes_per_launch": info[dominant]["bytes"], "avg_launch_ms": round(kern_ms, 5)}
+    if info[dominant]["variant"].startswith("xphase"):
+        roofline["ceiling"] = xphase_ceiling(np, torch, dev, mats[dominant], kern_ms)
+        log(f"bench: {dominant}: {roofline['ceiling']}")
 
     # ---- CPU baseline leg (oracle: threaded parallel_matrix restatement) ----
     cpu = None

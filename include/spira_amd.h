@@ -205,6 +205,15 @@ int spira_spmv_set_variant(spira_dcsr *A, int variant);
  * in *best (either output may be NULL). */
 int spira_spmv_autotune_f64(spira_dcsr *A, const double *x_dev, double *y_dev, int reps, float *ms, int *best);
 
+/* Measured ceilings of this GPU for the SpMV roofline (no counterpart in the
+ * reference; a measurement aid for benchmarks).  *gather_req_per_s: random
+ * 8-byte loads that hit L2 (every CU gathering inside its XCD's own 2 MiB
+ * slice of a 32 MiB array, 20 loads in flight per thread, each load its own
+ * L1-to-L2 request), per second.  *stream_bytes_per_s: a read-only sweep of
+ * 1 GiB with 16-byte non-temporal loads.  Allocates about 1.03 GiB from the
+ * device's cache for the duration of the call; either output may be NULL. */
+int spira_probe_ceilings(spira_dev *dev, double *gather_req_per_s, double *stream_bytes_per_s);
+
 /* ---- partitions -------------------------------------------------------------
  * compute_partition_boundaries (partition.hpp:78-126) as used by
  * parallel_matrix::rebalance() (parallel_matrix.hpp:434-497), on the device

@@ -188,6 +188,8 @@ def lib():
             "spira_stage_size": (ctypes.c_int, [_vp, _u64p]),
             "spira_stage_clear": (ctypes.c_int, [_vp]),
             "spira_lock_staged_f64": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_uint32, _u64p]),
+            "spira_probe_ceilings": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double),
+                                                    ctypes.POINTER(ctypes.c_double)]),
             "spira_spmv_autotune_f64": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
                                                        ctypes.POINTER(ctypes.c_int)]),
             "spira_spmv_mixed_f64": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_uint64, _vp, ctypes.c_uint64,
@@ -297,6 +299,14 @@ class Device:
     @property
     def stream(self) -> int:
         return lib().spira_dev_stream(self._h) or 0
+
+    def probe_ceilings(self):
+        """Measured ceilings of this GPU (spira_probe_ceilings): L2-resident
+        random 8-byte gathers in requests/s and a 1 GiB streaming read in
+        bytes/s."""
+        g, b = ctypes.c_double(), ctypes.c_double()
+        self.check(lib().spira_probe_ceilings(self._h, ctypes.byref(g), ctypes.byref(b)))
+        return g.value, b.value
 
     def close(self):
         if getattr(self, "_h", None) and not _shutdown:

@@ -491,6 +491,12 @@ int spira_spmv_choose(spira_dcsr *A, int *variant)
     });
 }
 
+int spira_probe_ceilings(spira_dev *dev, double *gather_req_per_s, double *stream_bytes_per_s)
+{
+    if (!dev) return SPIRA_ERR_INVALID_ARGUMENT;
+    return guarded(dev, [&] { probe_ceilings(dev, gather_req_per_s, stream_bytes_per_s); });
+}
+
 int spira_spmv_autotune_f64(spira_dcsr *A, const double *x_dev, double *y_dev, int reps, float *ms, int *best)
 {
     if (!A) return SPIRA_ERR_INVALID_ARGUMENT;

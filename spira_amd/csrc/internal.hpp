@@ -58,6 +58,10 @@ bool mixed_variant_supported(int variant);
 // error.  variant: AUTO, VEC4..VEC32, STREAM1K/2K, XPHASE, XPHASE_PACED.
 void spmv_f32_launch(spira_dcsr *A, const float *x, float *y, int variant);
 
+// probe.hip: measured ceilings of the part (L2-resident random 8-byte
+// gathers, one request each, in requests/s; a 1 GiB streaming read in B/s).
+void probe_ceilings(spira_dev *dev, double *gather_req_per_s, double *stream_bytes_per_s);
+
 // lock.hip: runs the device lock pipeline on already-device-resident staged
 // arrays; installs the new CSR in A.  Returns the new nnz.
 uint64_t lock_device_f64(spira_dcsr *A, const uint64_t *d_stage_ptr, const uint32_t *d_stage_col,
