@@ -851,7 +851,17 @@ def main():
         for p in pools.values():
             p.close()
         pools, _ = make_pools(threads, False)
-        loop = step_times(pools, args.cpu_seconds * 0.15, 5, 200, cold=False)
+        loop = step_times(pools, args.cpu_seconds * 0.1, 5, 200, cold=False)
+        others = {}
+        for other in ("avx2", "avx512"):  # every SIMD dot this CPU has, beside the dispatched one
+            if other == kern:
+                continue
+            try:
+                for p in pools.values():
+                    p.force_kernel(other)
+            except RuntimeError:
+                continue
+            others[other] = step_times(pools, args.cpu_seconds * 0.05, 5, 100, cold=False)
         for p in pools.values():
             p.close()
         cpu = {"value": rate(warm), "unit": "GB/s", "cores": threads, "kind": "port",
@@ -866,6 +876,7 @@ def main():
                             "flush": f"each worker reads {llc >> 20} MiB before each SpMV (untimed)"},
                "scalar_loop": {"value": rate(loop), "unit": "GB/s", "steps": len(loop),
                                "ms_per_step": round(1e3 * float(np.median(loop)), 3)},
+               "other_dots": {k: {"value": rate(v), "unit": "GB/s", "steps": len(v)} for k, v in others.items()},
                "host": {"cpu_model": _cpu_model(), "nproc": os.cpu_count(), "affinity": cores,
                         "physical_cores": len(phys), "pinned_cpus": phys[:threads], "llc_bytes": llc}}
         # one thread, same matrices and dot (a bounded sample of steps)
