@@ -205,6 +205,12 @@ int spira_spmv_set_variant(spira_dcsr *A, int variant);
  * in *best (either output may be NULL). */
 int spira_spmv_autotune_f64(spira_dcsr *A, const double *x_dev, double *y_dev, int reps, float *ms, int *best);
 
+/* The device scan every pass of this library builds on, exposed for callers and
+ * tests (no counterpart in the reference): out[0..n) = exclusive prefix sums of
+ * in[0..n), out[n] = the total (modulo 2^64); device arrays, in may equal out
+ * (and may be NULL when n == 0).  Synchronous. */
+int spira_scan_u64(spira_dev *dev, const uint64_t *in_dev, uint64_t *out_dev, uint64_t n);
+
 /* Measured ceilings of this GPU for the SpMV roofline (no counterpart in the
  * reference; a measurement aid for benchmarks).  *gather_req_per_s: random
  * 8-byte loads that hit L2 (every CU gathering inside its XCD's own 2 MiB

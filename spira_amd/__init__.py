@@ -188,6 +188,7 @@ def lib():
             "spira_stage_size": (ctypes.c_int, [_vp, _u64p]),
             "spira_stage_clear": (ctypes.c_int, [_vp]),
             "spira_lock_staged_f64": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_uint32, _u64p]),
+            "spira_scan_u64": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_uint64]),
             "spira_probe_ceilings": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double),
                                                     ctypes.POINTER(ctypes.c_double)]),
             "spira_spmv_autotune_f64": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
@@ -299,6 +300,23 @@ class Device:
     @property
     def stream(self) -> int:
         return lib().spira_dev_stream(self._h) or 0
+
+    def scan_u64(self, src, n=None, dst=None):
+        """Exclusive prefix sum of the first n words of a device int64 tensor
+        (spira_scan_u64): dst[:n] = prefixes, dst[n] = total.  n defaults to
+        src.numel(); dst (n + 1 words, may be src) defaults to a new tensor.
+        Returns dst."""
+        import torch
+        n = src.numel() if n is None else int(n)
+        if dst is None:
+            dst = torch.empty(n + 1, dtype=torch.int64, device=src.device)
+        check_device_vector(src, -1, "src", self.ordinal, dtype="int64", what="scan")
+        check_device_vector(dst, -1, "dst", self.ordinal, dtype="int64", what="scan")
+        if src.numel() < n or dst.numel() < n + 1:
+            raise InvalidArgument(f"scan: src needs {n} words and dst {n + 1}")
+        _producers_done(self, src, dst)
+        self.check(lib().spira_scan_u64(self._h, _ptr(src), _ptr(dst), n))
+        return dst
 
     def probe_ceilings(self):
         """Measured ceilings of this GPU (spira_probe_ceilings): L2-resident

@@ -178,6 +178,7 @@ void spira_dev_destroy(spira_dev *dev)
     (void)hipStreamSynchronize(dev->stream);
     dev->ws.release();
     dev->cache.release_all();
+    if (dev->scan_state) (void)hipFree(dev->scan_state);
     if (dev->own_stream) (void)hipStreamDestroy(dev->stream);
     delete dev;
 }
@@ -488,6 +489,16 @@ int spira_spmv_choose(spira_dcsr *A, int *variant)
     return guarded(A->dev, [&] {
         if (!A->built) fail(SPIRA_ERR_LOGIC, "spmv: matrix must be locked");
         *variant = A->n_rows ? spmv_resolve_auto(A) : 0;
+    });
+}
+
+int spira_scan_u64(spira_dev *dev, const uint64_t *in_dev, uint64_t *out_dev, uint64_t n)
+{
+    if (!dev || !out_dev || (!in_dev && n)) return SPIRA_ERR_INVALID_ARGUMENT;  // in is not read when n == 0
+    return guarded(dev, [&] {
+        scan_excl_u64(dev, in_dev, out_dev, n);
+        dev_sync(dev);
+        if (scan_errors(dev)) fail(SPIRA_ERR_DEVICE, "scan: a tile wait gave up");
     });
 }
 
