@@ -336,7 +336,4 @@ struct spira_dev {
     int n_cu = 0;
     void *noalloc_scratch = nullptr;  // this GPU's copy of the fixed scratch array (capi.hip)
     uint32_t phase_spin_max = 1u << 14;  // loads before an XPHASE_PACED wait gives up (SPIRA_PHASE_SPIN_MAX)
-    void *scan_state = nullptr;          // scan.hip: tile counter, error word and per-tile words
-    uint64_t scan_tiles_cap = 0;
-    uint32_t scan_epoch = 0;
 };

@@ -7,10 +7,7 @@
 namespace spira_amd {
 
 // scan.hip: out[0..n) = exclusive prefix of in, out[n] = total.  in may equal out.
-// scan_errors() (one device read) reports, and clears, whether a scan on dev
-// gave up a wait (which a correct run cannot do).
 void scan_excl_u64(spira_dev *dev, const uint64_t *in, uint64_t *out, uint64_t n);
-bool scan_errors(spira_dev *dev);
 
 // spmv.hip
 void spmv_build_plan(spira_dcsr *A);

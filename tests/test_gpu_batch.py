@@ -252,3 +252,47 @@ def test_batch_errors(dev):
     unlocked = spira_amd.DeviceCSR(dev, 5, 5)
     with pytest.raises(spira_amd.LogicError):
         spira_amd.Batch(dev, [mats[0], unlocked])
+
+
+def test_bench_generator_slice_with_selector_schedules(dev):
+    """192 matrices of the batched workload's own generator (rows log-uniform
+    in [256, 65536]: X blocks in one LDS load, in two halves, and too wide for
+    LDS at k = 4) under the learned selector's schedules, k = 1 and 4, against
+    the FP64 oracle at 1e-6; and the built-in schedule gives the same answer
+    to 1e-6."""
+    from spira_amd import selector as S
+    mats, host = [], []
+    for s in range(20_000_000, 20_000_192):
+        off, col, val = W.batch_matrix(s, device="cuda")
+        off, col, val = (off.cpu().numpy().astype(np.uint64), col.cpu().numpy().astype(np.uint32),
+                         val.cpu().numpy())
+        host.append((off, col, val))
+        A = spira_amd.DeviceCSR(dev, len(off) - 1, len(off) - 1)
+        A.upload(off, col, val)
+        mats.append(A)
+    assert max(len(o) - 1 for o, _, _ in host) > spira_amd.BATCH_XLDS_FLOATS  # two-pass matrices present
+    B = spira_amd.Batch(dev, mats, "f32")
+    sel = S.Selector.load(dev)
+    n_cols = np.array([A.n_cols for A in mats])
+    feats = S.features_by_k(mats)
+    for k in (1, 4):
+        X = np.random.default_rng(k + 40).standard_normal((B.n_cols, k)).astype(np.float32)
+        Xt = torch.from_numpy(X).cuda()
+        ref = _reference(host, X, k, B.col_base)
+        outs = []
+        for sched in ("selector", "builtin"):
+            if sched == "selector":
+                S.apply(B, k, sel.predict(feats[k], [k] * len(mats), n_cols), n_cols)
+            else:
+                B.set_schedule(k)
+            Yt = torch.full((B.n_rows, k), float("nan"), dtype=torch.float32, device="cuda")
+            B.spmm(Xt, Yt)
+            y = Yt.cpu().numpy().astype(np.float64)
+            err = oracle.normwise_rel_err(y.ravel(), ref.ravel())
+            print(f"k={k} {sched}: {err:.3e}", flush=True)
+            assert np.isfinite(y).all() and err <= 1e-6, (k, sched, err)
+            outs.append(y)
+        assert oracle.normwise_rel_err(outs[0].ravel(), outs[1].ravel()) <= 1e-6
+    B.close()
+    for A in mats:
+        A.close()

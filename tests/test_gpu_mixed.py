@@ -153,3 +153,38 @@ def test_storage_copies_dropped_by_lock(dev):
     exp = oracle.lock(*spira_amd.stage_from_coo(1000, rows, cols, np.full(1000, 2.5)), old=(off, col, val))
     y, _ = A.spmv_mixed(x, storage="f32", recompute_ratio=64.0)
     assert oracle.normwise_rel_err(y, oracle.spmv_f64(*exp, x)) <= 1e-6
+
+
+def test_mixed_full_size_powerlaw(dev):
+    """The full power-law workload (8,388,608 rows, Zipf(1.8) row lengths
+    clipped to 50,000, 70 % of columns within +-4096 of the diagonal, values
+    N(0, 1)) on the XPHASE layout for every storage, against the FP64 oracle:
+    FP64 <= 1e-12, FP32 <= 2e-6 (value and x rounding alone), split BF16 <=
+    1e-4, BF16 <= 2e-2; and the row-major FP32 path with the FP64 recompute of
+    cancelling rows <= 1e-6."""
+    import torch
+    n = 1 << 23
+    print(f"full-size power-law: {n} rows", flush=True)
+    off_t, col_t, val_t = W.powerlaw_rows_device(n, seed=W.BASE_SEED ^ 0xDEADBEEF)
+    A = spira_amd.DeviceCSR(dev, n, n)
+    A.upload(off_t, col_t, val_t, device=True)
+    off, col, val = off_t.cpu().numpy().astype(np.uint64), col_t.cpu().numpy().astype(np.uint32), val_t.cpu().numpy()
+    del off_t, col_t, val_t
+    x = np.random.default_rng(4).uniform(-1, 1, n)
+    y_ref = oracle.spmv_f64(off, col, val, x)
+    xt = torch.from_numpy(x).cuda()
+    yt = torch.empty(n, dtype=torch.float64, device="cuda")
+    for name in ("xphase", "xphase_paced"):
+        A.spmv(xt, yt, variant=spira_amd.VARIANTS[name], device=True)
+        err = oracle.normwise_rel_err(yt.cpu().numpy(), y_ref)
+        assert err <= 1e-12, (name, err)
+        for storage, tol in (("f32", 2e-6), ("split_bf16", 1e-4), ("bf16", 2e-2)):
+            y, nrec = A.spmv_mixed(xt, yt, storage=storage, variant=spira_amd.VARIANTS[name], device=True)
+            err = oracle.normwise_rel_err(yt.cpu().numpy(), y_ref)
+            print(f"{name} {storage}: {err:.3e}", flush=True)
+            assert err <= tol and nrec == 0, (name, storage, err)
+    y, nrec = A.spmv_mixed(xt, yt, storage="f32", variant=spira_amd.VARIANTS["stream1k"], recompute_ratio=16.0,
+                           device=True)
+    err = oracle.normwise_rel_err(yt.cpu().numpy(), y_ref)
+    assert err <= 1e-6 and nrec > 0, (err, nrec)
+    A.close()

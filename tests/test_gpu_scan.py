@@ -1,7 +1,7 @@
-"""The device scan (spira_scan_u64): one-pass decoupled look-back, against
-numpy's cumulative sum, out of place and in place, at tile boundaries and at
-sizes needing many look-back windows, called back to back (the per-tile state
-is reused across calls by epoch)."""
+"""The device scan (spira_scan_u64: tile sums, then tiles scanned with their
+prefix) against numpy's cumulative sum, out of place and in place, at tile
+boundaries, on both sides of the direct-prefix limit (1,024 tiles), back to
+back and with words using all 64 bits."""
 import numpy as np
 import pytest
 
@@ -46,7 +46,7 @@ def test_scan_matches_cumsum(dev, n):
 
 @pytest.mark.gpu
 def test_scan_back_to_back_and_growing(dev):
-    """Many calls in a row, sizes up and down (state reused, grown once), zeros
+    """Many calls in a row, sizes up and down (the sums workspace grows), zeros
     and words near 2^43."""
     import torch
     rng = np.random.default_rng(5)
