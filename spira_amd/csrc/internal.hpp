@@ -6,8 +6,9 @@
 
 namespace spira_amd {
 
-// scan.hip: out[0..n) = exclusive prefix of in, out[n] = total.  in may equal out.
-void scan_excl_u64(spira_dev *dev, const uint64_t *in, uint64_t *out, uint64_t n);
+// scan.hip: out[0..n) = exclusive prefix of in, out[n] = total.  in may equal
+// out.  total_copy (optional, device): a second copy of the total.
+void scan_excl_u64(spira_dev *dev, const uint64_t *in, uint64_t *out, uint64_t n, uint64_t *total_copy = nullptr);
 
 // spmv.hip
 void spmv_build_plan(spira_dcsr *A);

@@ -59,7 +59,7 @@ enum err_bits : uint32_t {
 enum ctl_word : int {
     CTL_ERR = 0,       // err_bits
     CTL_NBIG = 1,      // rows listed for k_dedup_big
-    CTL_NNZ_LO = 2,    // new nnz (k_finish)
+    CTL_NNZ_LO = 2,    // new nnz (written by the offsets scan; 8-byte aligned)
     CTL_NNZ_HI = 3,
     CTL_NLONG = 4,     // rows with more than kMaxSeg staged inserts
     CTL_LONG_INS = 5,  // staged inserts in those rows (< m < 2^32)
@@ -88,11 +88,15 @@ __device__ __forceinline__ void mark_block_rows(uint64_t r, uint64_t e0, uint64_
 // ---------------------------------------------------------------------------
 // 1. checks and binning
 // ---------------------------------------------------------------------------
-__global__ void k_check_ptr(const uint64_t *__restrict__ sptr, uint64_t n, uint64_t m, uint32_t *ctl,
-                            uint32_t *big_rows, uint32_t *long_rows)
+// One thread per staged pointer (rows 0..n) and per staged column (0..m-1):
+// pointers well formed, columns < n_cols, rows binned by insert count.
+__global__ void k_check_staged(const uint64_t *__restrict__ sptr, uint64_t n, uint64_t m,
+                               const uint32_t *__restrict__ scol, uint64_t n_cols, uint32_t *ctl,
+                               uint32_t *big_rows, uint32_t *long_rows)
 {
     uint32_t *err = ctl + CTL_ERR;
     const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < m && (uint64_t)scol[r] >= n_cols) atomicOr(err, ERR_OUT_OF_RANGE);
     if (r > n) return;
     if (r == 0 && sptr[0] != 0) atomicOr(err, ERR_BAD_PTR);
     if (r == n) {
@@ -112,12 +116,6 @@ __global__ void k_check_ptr(const uint64_t *__restrict__ sptr, uint64_t n, uint6
     } else if (k > kSmallSeg) {
         big_rows[atomicAdd(ctl + CTL_NBIG, 1u)] = (uint32_t)r;
     }
-}
-
-__global__ void k_check_cols(const uint32_t *__restrict__ scol, uint64_t m, uint64_t n_cols, uint32_t *err)
-{
-    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < m && (uint64_t)scol[s] >= n_cols) atomicOr(err, ERR_OUT_OF_RANGE);
 }
 
 // ---------------------------------------------------------------------------
@@ -746,12 +744,6 @@ __global__ __launch_bounds__(kThreads) void k_scatter_ded(uint64_t m, const uint
 }
 
 // The new nnz next to the error bits, so one copy brings both back.
-__global__ void k_finish(const uint64_t *__restrict__ nnz_new, uint32_t *__restrict__ ctl)
-{
-    ctl[CTL_NNZ_LO] = (uint32_t)*nnz_new;
-    ctl[CTL_NNZ_HI] = (uint32_t)(*nnz_new >> 32);
-}
-
 // ---------------------------------------------------------------------------
 // load_csr validation (matrix.hpp:524-575), first failure in the reference's
 // check order wins: key = row << 32 | step, step 0 = offsets of the row,
@@ -913,12 +905,9 @@ uint64_t lock_device_f64(spira_dcsr *A, const uint64_t *sptr, const uint32_t *sc
     uint32_t *big = (uint32_t *)dev->ws.get(WS_BIG, (n + 1) * sizeof(uint32_t));
     uint32_t *long_rows = (uint32_t *)dev->ws.get(WS_LONG_ROWS, (m / (kMaxSeg + 1) + 1) * sizeof(uint32_t));
     SPIRA_HIP(hipMemsetAsync(ctl, 0, CTL_WORDS * sizeof(uint32_t), st));
-    k_check_ptr<<<grid_for(n + 1, kThreads), kThreads, 0, st>>>(sptr, n, m, ctl, big, long_rows);
+    k_check_staged<<<grid_for(std::max<uint64_t>(n + 1, m), kThreads), kThreads, 0, st>>>(sptr, n, m, scol, A->n_cols,
+                                                                                     ctl, big, long_rows);
     SPIRA_HIP(hipGetLastError());
-    if (m > 0) {
-        k_check_cols<<<grid_for(m, kThreads), kThreads, 0, st>>>(scol, m, A->n_cols, ctl);
-        SPIRA_HIP(hipGetLastError());
-    }
 
     uint32_t *dcol = (uint32_t *)dev->ws.get(WS_DED_COL, m * sizeof(uint32_t));
     double *dval = (double *)dev->ws.get(WS_DED_VAL, m * sizeof(double));
@@ -985,7 +974,8 @@ uint64_t lock_device_f64(spira_dcsr *A, const uint64_t *sptr, const uint32_t *sc
                                                                          PZ, cnt, ctl);
                 SPIRA_HIP(hipGetLastError());
             }
-            scan_excl_u64(dev, cnt, new_off, n);
+            // new offsets; the total (the new nnz) also goes to the control words
+            scan_excl_u64(dev, cnt, new_off, n, (uint64_t *)(ctl + CTL_NNZ_LO));
             // ---- 5. scatter ------------------------------------------------
             if (total_merge > 0) {
                 k_scatter_merge<<<grid_for(total_merge, kPerBlockMerge), kThreads, 0, st>>>(
@@ -997,8 +987,6 @@ uint64_t lock_device_f64(spira_dcsr *A, const uint64_t *sptr, const uint32_t *sc
                                                                            new_col, new_val, ctl, stab);
                 SPIRA_HIP(hipGetLastError());
             }
-            k_finish<<<1, 1, 0, st>>>(new_off + n, ctl);
-            SPIRA_HIP(hipGetLastError());
             SPIRA_HIP(hipMemcpyAsync(hctl, ctl, sizeof(hctl), hipMemcpyDeviceToHost, st));
             dev_sync(dev);
         };

@@ -6,8 +6,8 @@
 // the tiles before it itself (at most 8 KiB read from L2, spread over its
 // 256 threads), otherwise it reads the prefix that k_sums_scan (one block)
 // wrote over the sums.  The tile is staged in LDS, scanned and written; the
-// last tile also writes the total at out[n].  Two launches up to 4,194,304
-// words, three above.  Input and output may be the same array: each tile is
+// last tile also writes the total at out[n] (and, if asked, a second copy).
+// Two launches up to 4,194,304 words, three above.  Input and output may be the same array: each tile is
 // read whole before it is written and no block reads another tile's words.
 #include "internal.hpp"
 
@@ -84,7 +84,8 @@ __global__ __launch_bounds__(kScanThreads) void k_sums_scan(uint64_t *sums, uint
 // otherwise sums holds their exclusive prefix.
 template <bool DIRECT>
 __global__ __launch_bounds__(kScanThreads) void k_tile_scan(const uint64_t *in, uint64_t *out, uint64_t n,
-                                                            const uint64_t *__restrict__ sums, uint64_t nt)
+                                                            const uint64_t *__restrict__ sums, uint64_t nt,
+                                                            uint64_t *__restrict__ total_copy)
 {
     __shared__ uint64_t s_tile[kScanTile];
     __shared__ uint64_t s_prefix;
@@ -126,12 +127,15 @@ __global__ __launch_bounds__(kScanThreads) void k_tile_scan(const uint64_t *in, 
         const uint64_t k = base + j;
         if (k < n) out[k] = s_tile[j];
     }
-    if (threadIdx.x == 0 && blockIdx.x == nt - 1) out[n] = s_prefix + tot;
+    if (threadIdx.x == 0 && blockIdx.x == nt - 1) {
+        out[n] = s_prefix + tot;
+        if (total_copy) *total_copy = s_prefix + tot;
+    }
 }
 
 }  // namespace
 
-void scan_excl_u64(spira_dev *dev, const uint64_t *in, uint64_t *out, uint64_t n)
+void scan_excl_u64(spira_dev *dev, const uint64_t *in, uint64_t *out, uint64_t n, uint64_t *total_copy)
 {
     hipStream_t s = dev->stream;
     const uint64_t nt = n == 0 ? 1 : (n + kScanTile - 1) / kScanTile;
@@ -140,11 +144,11 @@ void scan_excl_u64(spira_dev *dev, const uint64_t *in, uint64_t *out, uint64_t n
     k_tile_sums<<<(unsigned)nt, kScanThreads, 0, s>>>(in, n, sums);
     SPIRA_HIP(hipGetLastError());
     if (nt <= kDirectTiles) {
-        k_tile_scan<true><<<(unsigned)nt, kScanThreads, 0, s>>>(in, out, n, sums, nt);
+        k_tile_scan<true><<<(unsigned)nt, kScanThreads, 0, s>>>(in, out, n, sums, nt, total_copy);
     } else {
         k_sums_scan<<<1, kScanThreads, 0, s>>>(sums, nt);
         SPIRA_HIP(hipGetLastError());
-        k_tile_scan<false><<<(unsigned)nt, kScanThreads, 0, s>>>(in, out, n, sums, nt);
+        k_tile_scan<false><<<(unsigned)nt, kScanThreads, 0, s>>>(in, out, n, sums, nt, total_copy);
     }
     SPIRA_HIP(hipGetLastError());
 }
