@@ -6,16 +6,19 @@
 // (parallel/algorithms/transpose.hpp:37-69: the same entries inserted into a
 // new matrix whose lock() removes explicit zeros).
 //
-//   1. k_transpose_keys: per entry a 64-bit key, (column << 32) | input row,
-//      with the sentinel column n_cols for a dropped zero; the input row comes
-//      from the block's row range (a wavefront-wide search) and a search inside.
-//   2. A stable LSD radix sort (rocPRIM) of (key, value) over the column bits
-//      only: entries come out grouped by output row and, inside a row, in
+//   1. k_key_rows: one thread per row marks the row owning the first entry of
+//      every key block (a row table), so no block searches for its rows.
+//   2. k_transpose_keys: per entry a 64-bit key, (column << 32) | input row,
+//      with the sentinel column n_cols for a dropped zero.
+//   3. A stable LSD radix sort (rocPRIM onesweep, 8-bit digits; wider digits
+//      were measured slower per pass than the passes they save) of
+//      (key, value) over the column bits only:
+//      entries come out grouped by output row and, inside a row, in
 //      input-entry order, which is input-row order -- the reference's fill
 //      order.  The values travel with the keys, so nothing is gathered later.
-//   3. k_transpose_offsets: output offsets = lower bounds of each row in the
-//      sorted keys (no atomics, no second pass over A).
-//   4. k_transpose_cols: output column = the low half of the sorted key.
+//   4. k_transpose_out: one streaming pass over the sorted keys writes the
+//      output columns (the low halves) and, where the column changes, the
+//      output offsets of every row in between.
 #include "internal.hpp"
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -27,20 +30,34 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kItems = 8;         // entries per thread in the key pass
 constexpr int kRowCache = 2048;   // row offsets a key block keeps in LDS
+constexpr uint64_t kKeyBlock = (uint64_t)kThreads * kItems;  // entries per key block
+
+
+// tab[b] = row owning entry b * kKeyBlock (b = 0 .. ceil(nnz / kKeyBlock));
+// rows are non-decreasing in the entry index, so a block's rows are
+// [tab[b], tab[b + 1]].
+__global__ void k_key_rows(uint64_t n_rows, uint64_t nnz, const uint64_t *__restrict__ off, uint32_t *__restrict__ tab)
+{
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_rows) return;
+    const uint64_t e0 = off[r], e1 = off[r + 1];
+    if (e0 == e1) return;
+    for (uint64_t b = (e0 + kKeyBlock - 1) / kKeyBlock; b * kKeyBlock < e1; ++b) tab[b] = (uint32_t)r;
+    if (e1 == nnz) tab[(nnz + kKeyBlock - 1) / kKeyBlock] = (uint32_t)r;  // the row owning the last entry
+}
 
 __global__ __launch_bounds__(kThreads) void k_transpose_keys(uint64_t n_rows, uint64_t nnz, uint64_t n_cols,
                                                               const uint64_t *__restrict__ off,
                                                               const uint32_t *__restrict__ col,
                                                               const double *__restrict__ val, int drop_zeros,
+                                                              const uint32_t *__restrict__ tab,
                                                               uint64_t *__restrict__ key)
 {
     __shared__ uint64_t s_off[kRowCache + 1];
-    const uint64_t first = (uint64_t)blockIdx.x * kThreads * kItems;
-    if (first >= nnz) return;  // whole blocks leave together: the searches below need every lane
-    const uint64_t last = min(first + (uint64_t)kThreads * kItems, nnz) - 1;
-    const uint64_t lo = wave_row_of(off, 0, n_rows, first);
-    const uint64_t hi = wave_row_of(off, lo, n_rows, last);
-    const uint64_t nr = hi - lo + 1;  // rows the block's entries lie in (the same in every wavefront)
+    const uint64_t first = (uint64_t)blockIdx.x * kKeyBlock;
+    if (first >= nnz) return;
+    const uint64_t lo = tab[blockIdx.x], hi = tab[blockIdx.x + 1];
+    const uint64_t nr = hi - lo + 1;  // rows the block's entries lie in
     if (nr <= kRowCache) {
         for (uint64_t i = threadIdx.x; i <= nr; i += kThreads) s_off[i] = off[lo + i];  // off[lo .. hi + 1]
         __syncthreads();
@@ -70,31 +87,29 @@ __global__ __launch_bounds__(kThreads) void k_transpose_keys(uint64_t n_rows, ui
     }
 }
 
-// t_off[r] = number of sorted keys whose column is below r, r = 0 .. n_cols
-// (dropped zeros carry the column n_cols and sort last, so t_off[n_cols] is
-// the output nnz).
-__global__ void k_transpose_offsets(const uint64_t *__restrict__ skey, uint64_t nnz, uint64_t n_cols,
-                                    uint64_t *__restrict__ t_off)
+// Sorted keys -> output columns (the low halves) and offsets: t_off[c] =
+// number of keys whose column is below c, c = 0 .. n_cols (dropped zeros carry
+// the column n_cols and sort last, so t_off[n_cols] is the output nnz).  The
+// entry where the column steps up from c0 to c1 writes t_off[c0 + 1 .. c1];
+// the first entry writes t_off[0 .. c1], the last t_off[c1 + 1 .. n_cols].
+__global__ void k_transpose_out(const uint64_t *__restrict__ skey, uint64_t nnz, uint64_t n_cols,
+                                uint64_t *__restrict__ t_off, uint32_t *__restrict__ t_col)
 {
-    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r <= n_cols;
-         r += (uint64_t)gridDim.x * blockDim.x) {
-        uint64_t a = 0, b = nnz;
-        while (a < b) {
-            const uint64_t mid = a + (b - a) / 2;
-            if ((skey[mid] >> 32) < r) a = mid + 1;
-            else b = mid;
-        }
-        t_off[r] = a;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (nnz == 0) {
+        for (uint64_t c = t0; c <= n_cols; c += stride) t_off[c] = 0;
+        return;
     }
-}
-
-__global__ void k_transpose_cols(const uint64_t *__restrict__ skey, const uint64_t *__restrict__ t_off,
-                                 uint64_t n_cols, uint32_t *__restrict__ t_col)
-{
-    const uint64_t nz = t_off[n_cols];
-    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nz;
-         p += (uint64_t)gridDim.x * blockDim.x)
-        t_col[p] = (uint32_t)skey[p];
+    for (uint64_t p = t0; p < nnz; p += stride) {
+        const uint64_t k = skey[p];
+        const uint64_t c = k >> 32;
+        t_col[p] = (uint32_t)k;
+        const uint64_t from = p == 0 ? 0 : (skey[p - 1] >> 32) + 1;
+        for (uint64_t q = from; q <= c; ++q) t_off[q] = p;
+        if (p == nnz - 1)
+            for (uint64_t q = c + 1; q <= n_cols; ++q) t_off[q] = nnz;
+    }
 }
 
 unsigned key_bits(uint64_t max_key)
@@ -143,25 +158,26 @@ extern "C" int spira_transpose_f64(spira_dcsr *A, spira_dcsr *out, uint32_t flag
             key = (uint64_t *)dev_alloc(dev, kb);
             skey = (uint64_t *)dev_alloc(dev, kb);
             if (nnz > 0) {
-                k_transpose_keys<<<grid_for(nnz, kThreads * kItems), kThreads, 0, st>>>(
-                    n, nnz, nc, A->off, A->col, A->val, drop ? 1 : 0, key);
+                const uint64_t nkb = (nnz + kKeyBlock - 1) / kKeyBlock;
+                uint32_t *tab = (uint32_t *)dev->ws.get(WS_MISC2, (nkb + 1) * sizeof(uint32_t));
+                k_key_rows<<<grid_for(n, kThreads), kThreads, 0, st>>>(n, nnz, A->off, tab);
+                SPIRA_HIP(hipGetLastError());
+                k_transpose_keys<<<(unsigned)nkb, kThreads, 0, st>>>(n, nnz, nc, A->off, A->col, A->val, drop ? 1 : 0,
+                                                                     tab, key);
                 SPIRA_HIP(hipGetLastError());
                 // columns 0 .. n_cols - 1, and the sentinel n_cols when zeros
                 // are dropped, in bits 32 and up
                 const unsigned bits = key_bits(drop ? nc : nc - 1);
                 size_t tmp_bytes = 0;
-                SPIRA_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, key, skey, A->val, t_val, (unsigned)nnz, 32,
-                                                    32 + bits, st));
+                SPIRA_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, key, skey, A->val, t_val,
+                                                                 (unsigned)nnz, 32, 32 + bits, st));
                 void *tmp = dev->ws.get(WS_SORT, tmp_bytes ? tmp_bytes : 8);
-                SPIRA_HIP(rocprim::radix_sort_pairs(tmp, tmp_bytes, key, skey, A->val, t_val, (unsigned)nnz, 32,
-                                                    32 + bits, st));
+                SPIRA_HIP(rocprim::radix_sort_pairs(tmp, tmp_bytes, key, skey, A->val, t_val,
+                                                                 (unsigned)nnz, 32, 32 + bits, st));
             }
-            k_transpose_offsets<<<grid_for(nc + 1, kThreads, 8192), kThreads, 0, st>>>(skey, nnz, nc, t_off);
+            k_transpose_out<<<grid_for(std::max<uint64_t>(nnz, 1), kThreads, 16384), kThreads, 0, st>>>(
+                skey, nnz, nc, t_off, t_col);
             SPIRA_HIP(hipGetLastError());
-            if (nnz > 0) {
-                k_transpose_cols<<<grid_for(nnz, kThreads, 16384), kThreads, 0, st>>>(skey, t_off, nc, t_col);
-                SPIRA_HIP(hipGetLastError());
-            }
             const uint64_t nnz_out = read_scalar(dev, t_off + nc);
             release_tmp();
             // install (out may be A itself for a square matrix: everything above
