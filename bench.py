@@ -509,6 +509,7 @@ def run_batch(args, np, torch, spira_amd, W, dev, stream):
     nnz = int(batch.nnz)
     n_cols = np.array([A.n_cols for A in mats])
     log(f"bench: batch: {len(mats)} matrices, {N} rows, {nnz} nnz built in {time.time() - t0:.1f}s")
+    col_bytes = 2 if max(A.n_cols for A in mats) <= 65536 else 4
     sel = None
     if os.path.exists(S.MODEL_PATH):
         try:
@@ -571,7 +572,11 @@ def run_batch(args, np, torch, spira_amd, W, dev, stream):
             t = _time_calls(torch, stream, lambda: batch.replay(sync=False), reps)
             rec[f"{name}_ms"] = round(t, 4)
             rec[f"{name}_GFLOP_s"] = round(2 * nnz * k / t / 1e6, 1)
-            rec[f"{name}_GB_s"] = round(W.spmv_bytes(N, N, nnz, s_val=4, s_x=4, s_y=4, k=k) / t / 1e6, 1)
+            # bytes at the batch's device widths: FP32 values, u16 columns
+            # (every matrix has at most 65,536 columns), u32 row offset and
+            # u32 row id per stored row, FP32 X and Y
+            rec[f"{name}_GB_s"] = round(W.spmv_bytes(N, N, nnz, s_val=4, s_col=col_bytes, s_ptr=8, s_x=4, s_y=4,
+                                                     k=k) / t / 1e6, 1)
         # error of the last schedule's result against the FP64 path
         batch.replay(sync=True)
         plain()
