@@ -26,11 +26,13 @@
 #include <algorithm>
 #include <cstddef>
 #include <cstdint>
+#include <exception>
 #include <memory>
 #include <new>
 #include <optional>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -541,6 +543,65 @@ public:
         return p;
     }
     [[nodiscard]] const std::vector<size_type> &partition_bounds() const noexcept { return bounds_; }
+
+    // One staged row as parallel_fill hands it out (row.hpp:162-168): insert()
+    // checks the column and appends to the row's open-mode buffer.
+    class row_ref {
+    public:
+        void insert(index_type col, const value_type &val)
+        {
+            if ((size_type)col >= m_->cols_) throw std::out_of_range("Column index out of range");
+            m_->buf_[r_].emplace_back(col, val);
+            m_->dirty_[r_] = 1;
+        }
+        [[nodiscard]] size_type size() const noexcept { return m_->buf_[r_].size(); }
+
+    private:
+        friend class parallel_matrix;
+        row_ref(parallel_matrix *m, size_type r) : m_(m), r_(r) {}
+        parallel_matrix *m_;
+        size_type r_;
+    };
+    // A partition's rows: rows[i] is global row row_start + i.
+    class partition_rows {
+    public:
+        [[nodiscard]] row_ref operator[](size_type i) const { return row_ref(m_, a_ + i); }
+        [[nodiscard]] size_type size() const noexcept { return z_ - a_; }
+
+    private:
+        friend class parallel_matrix;
+        partition_rows(parallel_matrix *m, size_type a, size_type z) : m_(m), a_(a), z_(z) {}
+        parallel_matrix *m_;
+        size_type a_, z_;
+    };
+
+    // parallel_matrix.hpp:166-189: f(rows, row_start, row_end, t) runs once per
+    // partition, each on its own host thread, and stages inserts into that
+    // partition's rows only (distinct rows: no locking).  The matrix stays
+    // open; lock() then builds or merges on the device as after insert().
+    // The first exception thrown by f is rethrown after every thread joined;
+    // inserts staged before it stay staged.
+    template <class Func>
+    void parallel_fill(Func &&f)
+    {
+        if (!is_open()) throw std::logic_error("parallel_fill: matrix must be open");
+        const size_type nt = n_threads();
+        std::vector<std::exception_ptr> err(nt);
+        std::vector<std::thread> th;
+        th.reserve(nt);
+        for (size_type t = 0; t < nt; ++t)
+            th.emplace_back([this, &f, &err, t] {
+                try {
+                    partition_rows rows(this, bounds_[t], bounds_[t + 1]);
+                    f(rows, bounds_[t], bounds_[t + 1], t);
+                } catch (...) {
+                    err[t] = std::current_exception();
+                }
+            });
+        for (auto &x : th) x.join();
+        for (auto &e : err)
+            if (e) std::rethrow_exception(e);
+    }
 
 protected:
     const char *insert_error() const override { return "parallel_matrix::insert() requires open mode"; }

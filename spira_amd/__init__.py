@@ -888,6 +888,41 @@ class _RowBuffers:
         return d
 
 
+class _RowRef:
+    """One staged row handed out by ParallelMatrix.parallel_fill."""
+
+    __slots__ = ("_p", "_r")
+
+    def __init__(self, part, r):
+        self._p, self._r = part, r
+
+    def insert(self, col: int, val: float):
+        """row::insert (row.hpp:162-168)."""
+        p = self._p
+        if col < 0 or col >= p.n_cols:
+            raise OutOfRange("Column index out of range")
+        p.r.append(self._r)
+        p.c.append(int(col))
+        p.v.append(float(val))
+
+
+class _PartitionRows:
+    """A partition's rows for parallel_fill: rows[i] is global row row_start + i."""
+
+    def __init__(self, row_start, row_end, n_cols):
+        self.a, self.z, self.n_cols = int(row_start), int(row_end), int(n_cols)
+        self.r, self.c, self.v = [], [], []
+
+    def __len__(self):
+        return self.z - self.a
+
+    def __getitem__(self, i):
+        i = int(i)
+        if i < 0 or i >= self.z - self.a:
+            raise IndexError("partition row index out of range")
+        return _RowRef(self, self.a + i)
+
+
 class Matrix:
     """spira::matrix<soa_tag, uint32_t, double> with the device backend."""
 
@@ -1113,6 +1148,38 @@ class ParallelMatrix(Matrix):
 
     def partition_bounds(self):
         return list(self._bounds)
+
+    def parallel_fill(self, f):
+        """parallel_matrix::parallel_fill (parallel_matrix.hpp:166-189):
+        f(rows, row_start, row_end, t) runs once per partition on a worker
+        thread; rows[global_row - row_start].insert(col, val) stages an insert
+        into that partition's rows (row::insert, row.hpp:162-168, checks the
+        column).  The matrix stays open; lock() then builds or merges as after
+        insert().  Each partition's inserts are appended in partition order,
+        so every row keeps its own insertion order.  The first exception
+        raised by f is re-raised after every worker finished, and inserts
+        staged before it stay staged (as in the C++ mirror)."""
+        if self._locked:
+            raise LogicError("parallel_fill: matrix must be open")
+        from concurrent.futures import ThreadPoolExecutor
+
+        parts = [_PartitionRows(self._bounds[t], self._bounds[t + 1], self._n_cols)
+                 for t in range(self._n_threads)]
+
+        def work(t):
+            f(parts[t], self._bounds[t], self._bounds[t + 1], t)
+
+        with ThreadPoolExecutor(max_workers=self._n_threads) as ex:
+            futs = [ex.submit(work, t) for t in range(self._n_threads)]
+            errs = [fu.exception() for fu in futs]
+        for p in parts:
+            if p.r:
+                rows = np.asarray(p.r, np.int64)
+                self._buf.extend(rows, np.asarray(p.c, np.uint32), np.asarray(p.v, np.float64))
+                self._dirty[rows] = 1
+        for e in errs:
+            if e is not None:
+                raise e
 
     def rebalance(self):
         """parallel_matrix::rebalance() (parallel_matrix.hpp:434-497): move the

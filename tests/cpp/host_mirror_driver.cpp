@@ -5,6 +5,8 @@
 //
 //   new serial R C | new parallel R C T      -> ok
 //   insert r c v | mark_dirty r | lock | open | clear | rebalance -> ok
+//   parallel_fill n r c v ...                -> ok  (parallel_matrix::parallel_fill, the
+//                                               triples dealt to their owning partitions)
 //   get r c | accumulate r                   -> value <hex>
 //   contains r c | is_locked                 -> value 0|1
 //   row_nnz r | nnz                          -> value <n>
@@ -33,6 +35,7 @@
 #include <memory>
 #include <sstream>
 #include <string>
+#include <tuple>
 #include <vector>
 
 namespace {
@@ -100,6 +103,26 @@ std::string run(const std::string &line)
         in >> r >> c >> v;
         if (g_par) g_par->insert((uint32_t)r, (uint32_t)c, parse_double(v));
         else mat().insert((uint32_t)r, (uint32_t)c, parse_double(v));
+        out << "ok";
+    } else if (cmd == "parallel_fill") {
+        // parallel_fill n r c v ... : the triples dealt to their owning
+        // partitions in the given order, then inserted by parallel_fill.
+        std::size_t n;
+        in >> n;
+        auto &p = par();
+        std::vector<std::vector<std::tuple<std::size_t, uint32_t, double>>> by_part(p.n_threads());
+        for (std::size_t i = 0; i < n; ++i) {
+            long long r, c;
+            std::string v;
+            in >> r >> c >> v;
+            by_part[p.owner((std::size_t)r)].emplace_back((std::size_t)r, (uint32_t)c, parse_double(v));
+        }
+        p.parallel_fill([&](auto &rows, std::size_t r_start, std::size_t r_end, std::size_t t) {
+            for (const auto &[r, c, v] : by_part[t]) {
+                if (r < r_start || r >= r_end) throw std::logic_error("driver: triple outside its partition");
+                rows[r - r_start].insert(c, v);
+            }
+        });
         out << "ok";
     } else if (cmd == "mark_dirty") {
         long long r;

@@ -133,6 +133,10 @@ class CppMatrix:
     def bounds(self):
         return [int(t) for t in self.d.cmd("bounds").split()[1:]]
 
+    def parallel_fill(self, rows, cols, vals):
+        self.d.cmd(f"parallel_fill {len(rows)} "
+                   + " ".join(f"{int(r)} {int(c)} {_h(v)}" for r, c, v in zip(rows, cols, vals)))
+
     def load_csr(self, off, col, val):
         self.d.cmd(f"load_csr {len(off) - 1} {len(col)} " + " ".join(str(int(v)) for v in off) + " "
                    + " ".join(str(int(v)) for v in col) + " " + " ".join(_h(v) for v in val))
@@ -199,6 +203,38 @@ def test_multicycle_lock_bit_exact_against_oracle_model(drv, parallel):
         assert m.nnz() == ref.nnz()
         m.open()
         ref.open()
+
+
+@pytest.mark.gpu
+def test_parallel_fill_bit_exact_against_oracle_model(drv):
+    """parallel_matrix::parallel_fill (one host thread per partition, each
+    inserting its own rows) over open/lock cycles with a rebalance between
+    them: the CSR equals the oracle model fed the same triples in order."""
+    rng = np.random.default_rng(17)
+    n_rows, n_cols = 500, 120
+    m = CppMatrix(drv, n_rows, n_cols, threads=6)
+    ref = OracleMatrix(n_rows, n_cols, parallel=True)
+    for cycle in range(4):
+        n = int(rng.integers(0, 3000))
+        rows = rng.integers(0, n_rows, n)
+        cols = rng.integers(0, 30, n)
+        vals = np.where(rng.random(n) < 0.1, -0.0, rng.uniform(-1, 1, n))
+        m.parallel_fill(rows, cols, vals)
+        for r, c, v in zip(rows, cols, vals):
+            ref.insert(int(r), int(c), float(v))
+        assert m.nnz() == ref.nnz()
+        m.lock()
+        ref.lock()
+        assert csr_equal(m.csr(), ref.csr()), f"cycle {cycle}"
+        if cycle == 1:
+            m.rebalance()
+        m.open()
+        ref.open()
+    with pytest.raises(IndexError, match="Column index out of range"):
+        m.parallel_fill([0], [n_cols], [1.0])
+    m.lock()
+    with pytest.raises(RuntimeError, match="must be open"):
+        m.parallel_fill([0], [0], [1.0])
 
 
 @pytest.mark.gpu

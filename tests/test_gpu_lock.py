@@ -11,7 +11,7 @@ import pytest
 import oracle
 import spira_amd
 from spira_amd import workloads as W
-from oracle_model import GOLDEN, brute_lock, csr_equal, load_kats, random_stream, run_scenario, stage
+from oracle_model import GOLDEN, OracleMatrix, brute_lock, csr_equal, load_kats, random_stream, run_scenario, stage
 
 pytestmark = pytest.mark.gpu
 KATS = load_kats()
@@ -411,3 +411,70 @@ def test_device_partition_boundaries_random_and_rebalance(dev):
     pm.rebalance()
     off = pm.csr()[0]
     assert pm.partition_bounds() == oracle.partition_bounds(np.diff(off.astype(np.int64)), 4).tolist()
+
+
+def test_parallel_fill_locks_like_inserts_in_order(dev):
+    """parallel_fill (parallel_matrix.hpp:166-189) over four partitions, one
+    worker thread each, over several open/lock cycles with a rebalance in
+    between: every cycle's CSR equals the oracle model fed the same triples
+    through insert() in order (duplicates in a row keep their order, so the
+    last write still wins), bit for bit."""
+    n_rows, n_cols, T = 2000, 300, 4
+    pm = spira_amd.ParallelMatrix(n_rows, n_cols, T, dev)
+    ref = OracleMatrix(n_rows, n_cols, parallel=True)
+    rng = np.random.default_rng(41)
+    for cycle in range(4):
+        n = int(rng.integers(0, 20000))
+        rows = rng.integers(0, n_rows, n)
+        cols = rng.integers(0, 40, n)
+        vals = np.where(rng.random(n) < 0.1, 0.0, rng.uniform(-1, 1, n))
+        bounds = pm.partition_bounds()
+        owner = np.searchsorted(bounds[1:-1], rows, side="right")
+        by_part = [np.nonzero(owner == t)[0] for t in range(T)]
+        seen = []
+
+        def fill(part_rows, r0, r1, t):
+            seen.append((t, r0, r1, len(part_rows)))
+            for i in by_part[t]:
+                part_rows[int(rows[i]) - r0].insert(int(cols[i]), float(vals[i]))
+
+        pm.parallel_fill(fill)
+        assert sorted(seen) == [(t, bounds[t], bounds[t + 1], bounds[t + 1] - bounds[t]) for t in range(T)]
+        for i in range(n):
+            ref.insert(int(rows[i]), int(cols[i]), float(vals[i]))
+        assert pm.nnz() == ref.nnz()
+        pm.lock()
+        ref.lock()
+        assert csr_equal(pm.csr(), ref.csr()), f"cycle {cycle}"
+        if cycle == 1:
+            pm.rebalance()
+        pm.open()
+        ref.open()
+    pm.lock()
+    with pytest.raises(spira_amd.LogicError, match="must be open"):
+        pm.parallel_fill(lambda *a: None)
+
+
+def test_parallel_fill_errors(dev):
+    """row::insert's column check (row.hpp:162-168) inside parallel_fill, and
+    an exception from the fill function re-raised after every worker ended."""
+    pm = spira_amd.ParallelMatrix(8, 5, 2, dev)
+
+    def bad_col(rows, r0, r1, t):
+        if t == 1:
+            rows[0].insert(5, 1.0)
+
+    with pytest.raises(spira_amd.OutOfRange, match="Column index out of range"):
+        pm.parallel_fill(bad_col)
+
+    def good_then_raise(rows, r0, r1, t):
+        rows[0].insert(t, 2.0 + t)
+        if t == 0:
+            raise KeyError("from the fill function")
+
+    with pytest.raises(KeyError):
+        pm.parallel_fill(good_then_raise)
+    pm.lock()  # the inserts staged before the exception are kept
+    off, col, val = pm.csr()
+    assert off.tolist() == [0, 1, 1, 1, 1, 2, 2, 2, 2]
+    assert col.tolist() == [0, 1] and val.tolist() == [2.0, 3.0]

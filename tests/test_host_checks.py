@@ -54,3 +54,52 @@ def test_top_level_spmv_refuses_cpu_tensors_before_any_call():
         A.spmv(np.zeros(4), np.zeros(2))
     with pytest.raises(InvalidArgument, match="float64"):
         A.spmv(np.zeros(4), np.zeros(3, np.float32))
+
+
+def _hostonly_parallel_matrix(n_rows, n_cols, n_threads):
+    """A ParallelMatrix's open-mode staging without a device handle (the
+    staging path of parallel_fill makes no device call)."""
+    pm = object.__new__(spira_amd.ParallelMatrix)
+    pm._n_rows, pm._n_cols, pm._n_threads = n_rows, n_cols, n_threads
+    pm._bounds = [t * n_rows // n_threads for t in range(n_threads + 1)]
+    pm._locked = False
+    pm._buf = spira_amd._RowBuffers()
+    pm._dirty = np.zeros(n_rows, np.uint8)
+    return pm
+
+
+def test_parallel_fill_stages_each_row_in_insertion_order():
+    """What lock() would stage after parallel_fill equals what it stages after
+    the same triples went through insert() in order: per-row segments in
+    per-row insertion order (threads run concurrently, but each owns its rows)."""
+    rng = np.random.default_rng(3)
+    n_rows, n_cols, T = 997, 50, 5
+    pm = _hostonly_parallel_matrix(n_rows, n_cols, T)
+    n = 30000
+    rows = rng.integers(0, n_rows, n)
+    cols = rng.integers(0, n_cols, n)
+    vals = rng.standard_normal(n)
+    owner = np.searchsorted(pm.partition_bounds()[1:-1], rows, side="right")
+
+    def fill(part_rows, r0, r1, t):
+        assert len(part_rows) == r1 - r0
+        for i in np.nonzero(owner == t)[0]:
+            part_rows[int(rows[i]) - r0].insert(int(cols[i]), float(vals[i]))
+
+    pm.parallel_fill(fill)
+    got = spira_amd.stage_from_coo(n_rows, *pm._buf.coo())
+    exp = spira_amd.stage_from_coo(n_rows, rows, cols, vals)
+    for a, b in zip(got, exp):
+        assert np.array_equal(a, b)
+    assert np.array_equal(pm._dirty, np.bincount(rows, minlength=n_rows).astype(bool).astype(np.uint8))
+
+
+def test_parallel_fill_host_checks():
+    pm = _hostonly_parallel_matrix(6, 4, 3)
+    with pytest.raises(spira_amd.OutOfRange, match="Column index out of range"):
+        pm.parallel_fill(lambda rows, a, b, t: rows[0].insert(4, 1.0))
+    with pytest.raises(IndexError):
+        pm.parallel_fill(lambda rows, a, b, t: rows[2].insert(0, 1.0))
+    pm._locked = True
+    with pytest.raises(spira_amd.LogicError, match="must be open"):
+        pm.parallel_fill(lambda *a: None)
