@@ -148,7 +148,12 @@ int spira_lock_staged_f64(spira_dcsr *A, spira_stage *s, const uint8_t *dirty, u
  * sequential FP64 loop <= 1e-12.  variant: SPIRA_SPMV_*; SPIRA_SPMV_AUTO is
  * the pinned variant (spira_spmv_set_variant, the autotuner) or else the
  * learned selector's choice for A (computed on the device once per locked
- * matrix).  Every variant except XPHASE and XPHASE_PACED adds each row's
+ * matrix).  When that choice copies A into a layout of its own (XSLICE,
+ * XSTREAM, XPHASE: several passes over A, worth some 5-10 calls), the copy is
+ * made by the second AUTO call after a lock or upload; the first AUTO call
+ * runs the fastest variant that needs no copy (VEC, STREAM), so a matrix
+ * multiplied once per lock never pays for it (with SPIRA_REPRODUCIBLE the
+ * first call builds it, so that every call runs one variant).  Every variant except XPHASE and XPHASE_PACED adds each row's
  * products in a fixed order, so y is the same bits on every call; the XPHASE
  * variants add into LDS with atomics, whose order (and so the last bits of
  * y) can change between calls.  With SPIRA_REPRODUCIBLE, AUTO keeps to the
@@ -181,7 +186,9 @@ int spira_spmv_f32(spira_dcsr *A, const float *x, uint64_t x_len, float *y, uint
  * SPIRA_Y_DEVICE, and may add SPIRA_NO_SYNC), calling no allocator: A's SpMV
  * plan for the variant must already exist (one spira_spmv_f64 call with the
  * same variant after the last lock or upload builds it; SPIRA_ERR_LOGIC
- * otherwise), and the scratch its kernels use -- the XPHASE_PACED pacing
+ * otherwise).  AUTO runs AUTO's choice once its plan exists, else the variant
+ * the last AUTO call of spira_spmv_f64 ran (the plan-free one of the first
+ * call after a lock).  The scratch its kernels use -- the XPHASE_PACED pacing
  * counters (512 bytes) or the XSLICE/XSTREAM partial sums (8 * n_rows
  * doubles) -- comes from one fixed array of SPIRA_NOALLOC_SCRATCH_BYTES per
  * GPU, held until the process ends.  A variant needing more scratch gives
@@ -192,7 +199,8 @@ int spira_spmv_f32(spira_dcsr *A, const float *x, uint64_t x_len, float *y, uint
 int spira_spmv_f64_noalloc(spira_dcsr *A, const double *x, uint64_t x_len, double *y, uint64_t y_len, int variant,
                            uint32_t flags);
 
-/* Variant the AUTO path would use now for A (after plan build). */
+/* Variant AUTO settles on for A: the one it runs from the second AUTO call
+ * after a lock or upload (the first may run a plan-free variant instead). */
 int spira_spmv_choose(spira_dcsr *A, int *variant);
 /* Pin the AUTO choice for A (autotuner / selector output). */
 int spira_spmv_set_variant(spira_dcsr *A, int variant);

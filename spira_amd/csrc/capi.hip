@@ -153,6 +153,7 @@ int spira_dev_create(spira_dev **out, int device_ordinal, void *stream)
         dev->arch_minor = prop.minor;
         dev->n_cu = prop.multiProcessorCount;
         dev->noalloc_scratch = scratch_of_device(device_ordinal);
+        SPIRA_HIP(hipHostMalloc(&dev->host_words, kHostWordsBytes, hipHostMallocDefault));
         // the XPHASE_PACED wait bound (a test forces give-ups with a small one)
         if (const char *e = std::getenv("SPIRA_PHASE_SPIN_MAX")) dev->phase_spin_max = (uint32_t)std::strtoul(e, nullptr, 10);
         if (stream) {
@@ -164,6 +165,7 @@ int spira_dev_create(spira_dev **out, int device_ordinal, void *stream)
         dev->ws.stream = dev->stream;
     });
     if (rc != SPIRA_OK) {
+        if (dev && dev->host_words) (void)hipHostFree(dev->host_words);
         delete dev;
         return rc;
     }
@@ -179,6 +181,7 @@ void spira_dev_destroy(spira_dev *dev)
     dev->ws.release();
     dev->cache.release_all();
     if (dev->own_stream) (void)hipStreamDestroy(dev->stream);
+    if (dev->host_words) (void)hipHostFree(dev->host_words);
     delete dev;
 }
 
@@ -392,7 +395,14 @@ int spira_spmv_f64_noalloc(spira_dcsr *A, const double *x, uint64_t x_len, doubl
         if (repro && variant >= 0 && !spmv_order_fixed(variant))
             fail(SPIRA_ERR_INVALID_ARGUMENT,
                  "spmv: XPHASE variants add in a varying order; drop SPIRA_REPRODUCIBLE or choose another variant");
-        const int v = variant < 0 ? spmv_auto_choice(A, repro) : variant;
+        int v = variant < 0 ? spmv_auto_choice(A, repro) : variant;
+        // AUTO whose choice has no plan yet: the variant the last AUTO call of
+        // spira_spmv_f64 ran (a plan-free one on the first call after a lock)
+        // if it fits the call's flags
+        const int ran = A->plan.auto_ran;
+        if (variant < 0 && v >= 0 && ran >= 0 && !spmv_plan_ready(A, v) && spmv_plan_ready(A, ran) &&
+            (!repro || spmv_order_fixed(ran)))
+            v = ran;
         if (v < 0 || !A->plan.stats_valid) fail(SPIRA_ERR_LOGIC, unprepared);
         const uint64_t need = spmv_scratch_bytes(A, v);
         if (need > kNoAllocScratchBytes)

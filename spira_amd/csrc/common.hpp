@@ -267,7 +267,7 @@ enum ws_slot : int {
     WS_LONG_KEY = 22,   // lock: (list index, column) keys, unsorted then sorted
     WS_LONG_IDX = 23,   // lock: insertion index of each key, unsorted then sorted
     WS_LONG_P = 24,     // lock: prefix of the kept keys
-    WS_SEL = 25,        // spmv AUTO: selector parameters and activations
+    WS_SEL = 25,        // matrix features and the SpMV selector's scores
 };
 
 }  // namespace spira_amd
@@ -302,6 +302,7 @@ struct spira_plan {
     int32_t variant = -1;        // pinned AUTO choice (-1 = none)
     int32_t auto_variant = -1;   // AUTO's own choice once resolved (learned selector, else the rule)
     int32_t auto_repro = -1;     // ... among the variants with a fixed summation order
+    int32_t auto_ran = -1;       // the variant the last AUTO call ran (-1: no AUTO call since the lock)
     bool scored = false;         // sel_score holds the selector's outputs for this matrix
     float sel_score[SPIRA_SPMV_N_VARIANTS] = {};
     bool tuned = false;          // tuned_ms holds the autotuner's times (-1: did not run)
@@ -335,5 +336,6 @@ struct spira_dev {
     int arch_major = 0, arch_minor = 0;
     int n_cu = 0;
     void *noalloc_scratch = nullptr;  // this GPU's copy of the fixed scratch array (capi.hip)
+    void *host_words = nullptr;       // pinned host memory for small read-backs (kHostWordsBytes)
     uint32_t phase_spin_max = 1u << 14;  // loads before an XPHASE_PACED wait gives up (SPIRA_PHASE_SPIN_MAX)
 };

@@ -4,11 +4,19 @@
 
 #include "common.hpp"
 
+#include <cstring>
+
 namespace spira_amd {
 
 // scan.hip: out[0..n) = exclusive prefix of in, out[n] = total.  in may equal
 // out.  total_copy (optional, device): a second copy of the total.
 void scan_excl_u64(spira_dev *dev, const uint64_t *in, uint64_t *out, uint64_t n, uint64_t *total_copy = nullptr);
+// The same scan when the kernel that wrote `in` also wrote the sum of every
+// chunk of `chunk` words (chunk divides 4,096) into scan_chunk_sums(dev, n,
+// chunk)'s array; the scan then skips its own summing pass.
+uint64_t *scan_chunk_sums(spira_dev *dev, uint64_t n, uint32_t chunk);
+void scan_excl_u64_chunks(spira_dev *dev, const uint64_t *in, uint64_t *out, uint64_t n, uint64_t *sums,
+                          uint32_t chunk, uint64_t *total_copy = nullptr);
 
 // spmv.hip
 void spmv_build_plan(spira_dcsr *A);
@@ -75,13 +83,30 @@ void validate_csr_device(spira_dcsr *A, uint64_t nnz, const uint64_t *d_off, con
 // Synchronise the handle's stream (throws on failure).
 inline void dev_sync(spira_dev *dev) { SPIRA_HIP(hipStreamSynchronize(dev->stream)); }
 
+// Copy `bytes` of device memory to host memory after the stream's queued
+// work and wait.  Up to kHostWordsBytes go through the handle's pinned
+// buffer: an asynchronous copy into pinned memory, where pageable memory
+// would take a staged, blocking copy.
+constexpr size_t kHostWordsBytes = 65536;
+inline void read_back(spira_dev *dev, void *dst, const void *dsrc, size_t bytes)
+{
+    if (bytes == 0) return;
+    if (dev->host_words && bytes <= kHostWordsBytes) {
+        SPIRA_HIP(hipMemcpyAsync(dev->host_words, dsrc, bytes, hipMemcpyDeviceToHost, dev->stream));
+        dev_sync(dev);
+        std::memcpy(dst, dev->host_words, bytes);
+    } else {
+        SPIRA_HIP(hipMemcpyAsync(dst, dsrc, bytes, hipMemcpyDeviceToHost, dev->stream));
+        dev_sync(dev);
+    }
+}
+
 // Read one device word.
 template <class T>
 inline T read_scalar(spira_dev *dev, const T *dptr)
 {
     T v{};
-    SPIRA_HIP(hipMemcpyAsync(&v, dptr, sizeof(T), hipMemcpyDeviceToHost, dev->stream));
-    dev_sync(dev);
+    read_back(dev, &v, dptr, sizeof(T));
     return v;
 }
 

@@ -336,6 +336,20 @@ __device__ __forceinline__ row_range block_rows(const uint64_t *ptr, uint64_t n_
 // the block before any of them leaves.
 constexpr uint32_t kRowCache = 2048;
 
+// The block's sum of v, written by thread 0 to *dst (every thread calls it).
+__device__ __forceinline__ void block_sum_to(uint64_t v, uint64_t *dst)
+{
+    __shared__ uint64_t s_w[kThreads / 64];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (int w = 0; w < kThreads / 64; ++w) t += s_w[w];
+        *dst = t;
+    }
+}
+
 struct row_cache {
     uint64_t lo, hi;
     bool cached;
@@ -387,7 +401,8 @@ __global__ __launch_bounds__(kThreads) void k_ded_flags(uint64_t n, uint64_t m, 
                                                          const uint32_t *__restrict__ old_col,
                                                          uint64_t *__restrict__ flags, uint32_t *__restrict__ qpos,
                                                          const uint64_t *__restrict__ stab,
-                                                         const uint32_t *__restrict__ ctl)
+                                                         const uint32_t *__restrict__ ctl,
+                                                         uint64_t *__restrict__ chunk_sums)
 {
     __shared__ uint64_t s_rows[kRowCache + 1];
     const uint64_t first = (uint64_t)blockIdx.x * kPerBlock;
@@ -402,6 +417,7 @@ __global__ __launch_bounds__(kThreads) void k_ded_flags(uint64_t n, uint64_t m, 
         vs[it] = s < m ? dval[s] : 0.0;
     }
     const row_cache rc = make_row_cache(table_rows(stab), sptr, s_rows);
+    uint64_t fsum = 0;  // this thread's flags, for the block's chunk sum
 #pragma unroll
     for (int it = 0; it < kItems; ++it) {
         const uint64_t s = first + (uint64_t)it * kThreads + threadIdx.x;
@@ -424,8 +440,10 @@ __global__ __launch_bounds__(kThreads) void k_ded_flags(uint64_t n, uint64_t m, 
             f = ((uint64_t)(match ? 1u : 0u) << 32) | (nz ? 1u : 0u);
         }
         flags[s] = f;
+        fsum += f;
         if (merge) qpos[s] = q;
     }
+    block_sum_to(fsum, chunk_sums + blockIdx.x);  // the scan's chunk of kPerBlock flags
 }
 
 // Old entries that a dirty row drops because they are explicit zeros (and not
@@ -466,27 +484,37 @@ __global__ __launch_bounds__(kThreads) void k_old_zero_flags(uint64_t n, uint64_
 // ---------------------------------------------------------------------------
 // 4. row counts
 // ---------------------------------------------------------------------------
-__global__ void k_row_counts(uint64_t n, const uint64_t *__restrict__ sptr, const uint64_t *__restrict__ P,
-                             const uint8_t *__restrict__ dirty, int merge, const uint64_t *__restrict__ old_off,
-                             const uint64_t *__restrict__ PZ, uint64_t *__restrict__ cnt,
-                             const uint32_t *__restrict__ ctl)
+// One thread per row; each block also writes the sum of its kThreads counts
+// (the offsets scan's chunk).
+__global__ __launch_bounds__(kThreads) void k_row_counts(uint64_t n, const uint64_t *__restrict__ sptr,
+                                                          const uint64_t *__restrict__ P,
+                                                          const uint8_t *__restrict__ dirty, int merge,
+                                                          const uint64_t *__restrict__ old_off,
+                                                          const uint64_t *__restrict__ PZ, uint64_t *__restrict__ cnt,
+                                                          const uint32_t *__restrict__ ctl,
+                                                          uint64_t *__restrict__ chunk_sums)
 {
+    if (ctl[CTL_ERR]) return;  // the same word for every block: the whole grid returns
     const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n || ctl[CTL_ERR]) return;
-    const uint64_t pa = P[sptr[r]], pb = P[sptr[r + 1]];
-    const uint64_t nz = (pb & 0xffffffffull) - (pa & 0xffffffffull);
-    const uint64_t mt = (pb >> 32) - (pa >> 32);
-    if (!merge) {
-        cnt[r] = nz;
-        return;
+    uint64_t c = 0;
+    if (r < n) {
+        const uint64_t pa = P[sptr[r]], pb = P[sptr[r + 1]];
+        const uint64_t nz = (pb & 0xffffffffull) - (pa & 0xffffffffull);
+        const uint64_t mt = (pb >> 32) - (pa >> 32);
+        if (!merge) {
+            c = nz;
+        } else {
+            const uint64_t ob = old_off[r], oe = old_off[r + 1];
+            if (!row_dirty(dirty, sptr, r)) {
+                c = oe - ob;
+            } else {
+                const uint64_t zeros = PZ ? (PZ[oe] - PZ[ob]) : 0;
+                c = (oe - ob) - mt - zeros + nz;
+            }
+        }
+        cnt[r] = c;
     }
-    const uint64_t ob = old_off[r], oe = old_off[r + 1];
-    if (!row_dirty(dirty, sptr, r)) {
-        cnt[r] = oe - ob;
-        return;
-    }
-    uint64_t zeros = PZ ? (PZ[oe] - PZ[ob]) : 0;
-    cnt[r] = (oe - ob) - mt - zeros + nz;
+    block_sum_to(c, chunk_sums + blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -804,10 +832,7 @@ void validate_csr_device(spira_dcsr *A, uint64_t nnz, const uint64_t *d_off, con
 {
     spira_dev *dev = A->dev;
     const uint64_t n = A->n_rows;
-    uint64_t ends[2];
-    SPIRA_HIP(hipMemcpyAsync(&ends[0], d_off, sizeof(uint64_t), hipMemcpyDeviceToHost, dev->stream));
-    SPIRA_HIP(hipMemcpyAsync(&ends[1], d_off + n, sizeof(uint64_t), hipMemcpyDeviceToHost, dev->stream));
-    dev_sync(dev);
+    const uint64_t ends[2] = {read_scalar(dev, d_off), read_scalar(dev, d_off + n)};
     if (ends[0] != 0) fail(SPIRA_ERR_INVALID_ARGUMENT, "spira::matrix::load_csr: CSR offsets[0] must be zero");
     if (ends[1] != nnz)
         fail(SPIRA_ERR_INVALID_ARGUMENT, "spira::matrix::load_csr: CSR offsets[n_rows] does not equal nnz");
@@ -822,14 +847,12 @@ void validate_csr_device(spira_dcsr *A, uint64_t nnz, const uint64_t *d_off, con
     // Entries are only checked once offsets are known to be monotone (the
     // reference stops at the first bad row as well).
     unsigned long long h[2];
-    SPIRA_HIP(hipMemcpyAsync(h, bad, sizeof(h), hipMemcpyDeviceToHost, dev->stream));
-    dev_sync(dev);
+    read_back(dev, h, bad, sizeof(h));
     if (h[0] == ~0ull && nnz > 0) {
         k_validate_entries<<<grid_for(nnz, kThreads * kItems), kThreads, 0, dev->stream>>>(n, nnz, A->n_cols, d_off,
                                                                                            d_col, d_val, bad, zs);
         SPIRA_HIP(hipGetLastError());
-        SPIRA_HIP(hipMemcpyAsync(h, bad, sizeof(h), hipMemcpyDeviceToHost, dev->stream));
-        dev_sync(dev);
+        read_back(dev, h, bad, sizeof(h));
     }
     if (h[0] != ~0ull) {
         const uint64_t step = h[0] & 0xffffffffull;
@@ -954,13 +977,14 @@ uint64_t lock_device_f64(spira_dcsr *A, const uint64_t *sptr, const uint32_t *sc
                 SPIRA_HIP(hipGetLastError());
             }
             // ---- 3. flags + scan ------------------------------------------
+            uint64_t *psums = scan_chunk_sums(dev, m, kPerBlock);
             if (m > 0) {
                 k_ded_flags<<<grid_for(m, kPerBlock), kThreads, 0, st>>>(n, m, sptr, dcnt, dcol, dval, dirty,
                                                                          merge ? 1 : 0, A->off, A->col, P, qpos,
-                                                                         stab, ctl);
+                                                                         stab, ctl, psums);
                 SPIRA_HIP(hipGetLastError());
             }
-            scan_excl_u64(dev, P, P, m);
+            scan_excl_u64_chunks(dev, P, P, m, psums, kPerBlock);
             // Old explicit zeros (only possible after load_csr with zeros).
             if (zeros) {
                 k_old_zero_flags<<<grid_for(A->nnz, kThreads * kItems), kThreads, 0, st>>>(
@@ -969,13 +993,14 @@ uint64_t lock_device_f64(spira_dcsr *A, const uint64_t *sptr, const uint32_t *sc
                 scan_excl_u64(dev, PZ, PZ, A->nnz);
             }
             // ---- 4. counts + offsets --------------------------------------
+            uint64_t *csums = scan_chunk_sums(dev, n, kThreads);
             if (n > 0) {
                 k_row_counts<<<grid_for(n, kThreads), kThreads, 0, st>>>(n, sptr, P, dirty, merge ? 1 : 0, A->off,
-                                                                         PZ, cnt, ctl);
+                                                                         PZ, cnt, ctl, csums);
                 SPIRA_HIP(hipGetLastError());
             }
             // new offsets; the total (the new nnz) also goes to the control words
-            scan_excl_u64(dev, cnt, new_off, n, (uint64_t *)(ctl + CTL_NNZ_LO));
+            scan_excl_u64_chunks(dev, cnt, new_off, n, csums, kThreads, (uint64_t *)(ctl + CTL_NNZ_LO));
             // ---- 5. scatter ------------------------------------------------
             if (total_merge > 0) {
                 k_scatter_merge<<<grid_for(total_merge, kPerBlockMerge), kThreads, 0, st>>>(
@@ -987,8 +1012,7 @@ uint64_t lock_device_f64(spira_dcsr *A, const uint64_t *sptr, const uint32_t *sc
                                                                            new_col, new_val, ctl, stab);
                 SPIRA_HIP(hipGetLastError());
             }
-            SPIRA_HIP(hipMemcpyAsync(hctl, ctl, sizeof(hctl), hipMemcpyDeviceToHost, st));
-            dev_sync(dev);
+            read_back(dev, hctl, ctl, sizeof(hctl));
         };
         steps();
         if (hctl[CTL_ERR] == ERR_LONG_ROWS) {

@@ -1,6 +1,7 @@
 // selector.hip -- matrix features and a small MLP (forward, backward, Adam)
-// for the learned schedule selector.  Everything here runs on the device; the
-// host only moves the (tiny) parameter and feature arrays.
+// for the learned selectors.  Everything here runs on the device; the host
+// only moves the (tiny) parameter, feature and score arrays.  The SpMV
+// kernel selector's trained parameters are compiled into the code object.
 //
 // Features of a CSR (SPIRA_FEATURES entries, all O(1)-scaled):
 //   0 log2(n_rows+1)/20   1 log2(nnz+1)/30   2 log2(mean_len+1)/10
@@ -24,10 +25,17 @@ constexpr int kThreads = 256;
 constexpr int kFeat = SPIRA_FEATURES;
 constexpr int kStat = 16;  // accumulated statistics
 
-// stats: 0 sum len, 1 sum len^2, 2 max len, 3..10 bins, 11 sum spread, 12 sum band, 13 max band
+constexpr unsigned kFeatBlocks = 2048;  // partial-sum blocks of the feature pass (grid-stride over rows)
+
+// stats: 0 sum len, 1 sum len^2, 2 max len, 3..10 bins, 11 sum spread, 12 sum band, 13 max band.
+// Every statistic is a sum or max of integers below 2^53, so it is exact in
+// FP64 whatever the order; each block writes its own partials (no atomics)
+// and k_feature_fold adds them up.
+__device__ __forceinline__ bool stat_is_max(int i) { return i == 2 || i == 13; }
+
 __global__ __launch_bounds__(kThreads) void k_feature_stats(const uint64_t *__restrict__ off,
                                                              const uint32_t *__restrict__ col, uint64_t n_rows,
-                                                             double *__restrict__ stats)
+                                                             double *__restrict__ part)
 {
     __shared__ double s[kThreads / 64][kStat];
     double loc[kStat] = {};
@@ -59,7 +67,7 @@ __global__ __launch_bounds__(kThreads) void k_feature_stats(const uint64_t *__re
         double v = loc[i];
         for (int o = 32; o > 0; o >>= 1) {
             const double t = __shfl_xor(v, o, 64);
-            v = (i == 2 || i == 13) ? fmax(v, t) : v + t;
+            v = stat_is_max(i) ? fmax(v, t) : v + t;
         }
         if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6][i] = v;
     }
@@ -67,14 +75,54 @@ __global__ __launch_bounds__(kThreads) void k_feature_stats(const uint64_t *__re
     if (threadIdx.x < kStat) {
         const int i = threadIdx.x;
         double v = s[0][i];
-        for (int w = 1; w < kThreads / 64; ++w) v = (i == 2 || i == 13) ? fmax(v, s[w][i]) : v + s[w][i];
-        if (i == 2 || i == 13) {
-            // non-negative doubles order like their bit patterns
-            atomicMax(reinterpret_cast<unsigned long long *>(&stats[i]), (unsigned long long)__double_as_longlong(v));
-        } else {
-            atomicAdd(&stats[i], v);
-        }
+        for (int w = 1; w < kThreads / 64; ++w) v = stat_is_max(i) ? fmax(v, s[w][i]) : v + s[w][i];
+        part[(uint64_t)blockIdx.x * kStat + i] = v;
     }
+}
+
+// One block: the partials of nb blocks folded into the statistics, then the
+// features (the list at the top of this file; slot 17 left 0) into feat.
+__global__ __launch_bounds__(kThreads) void k_feature_fold(const double *__restrict__ part, uint32_t nb,
+                                                            uint64_t n_rows, uint64_t nnz, uint64_t n_cols,
+                                                            float *__restrict__ feat)
+{
+    __shared__ double s[kThreads / 64][kStat];
+    __shared__ double h[kStat];
+    for (int i = 0; i < kStat; ++i) {
+        double v = 0.0;
+        for (uint32_t j = threadIdx.x; j < nb; j += kThreads)
+            v = stat_is_max(i) ? fmax(v, part[(uint64_t)j * kStat + i]) : v + part[(uint64_t)j * kStat + i];
+        for (int o = 32; o > 0; o >>= 1) {
+            const double t = __shfl_xor(v, o, 64);
+            v = stat_is_max(i) ? fmax(v, t) : v + t;
+        }
+        if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6][i] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kStat) {
+        const int i = threadIdx.x;
+        double v = s[0][i];
+        for (int w = 1; w < kThreads / 64; ++w) v = stat_is_max(i) ? fmax(v, s[w][i]) : v + s[w][i];
+        h[i] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double n = (double)(n_rows ? n_rows : 1), nc = (double)(n_cols ? n_cols : 1);
+    const double mean = h[0] / n;
+    const double var = h[1] / n - mean * mean;
+    const double cv = mean > 0 ? sqrt(var > 0 ? var : 0.0) / mean : 0.0;
+    const double nonempty = n - h[3] > 0 ? n - h[3] : 1.0;
+    feat[0] = (float)(log2((double)n_rows + 1) / 20.0);
+    feat[1] = (float)(log2((double)nnz + 1) / 30.0);
+    feat[2] = (float)(log2(mean + 1) / 10.0);
+    feat[3] = (float)(fmin(cv, 4.0) / 4.0);
+    feat[4] = (float)(log2(h[2] + 1) / 16.0);
+    for (int b = 0; b < 8; ++b) feat[5 + b] = (float)(h[3 + b] / n);
+    feat[13] = (float)(h[11] / nonempty / nc);
+    feat[14] = (float)(h[12] / nonempty / nc);
+    feat[15] = (float)(h[13] / nc);
+    feat[16] = (float)(log2((double)n_cols + 1) / 20.0);
+    feat[17] = 0.0f;
 }
 
 // ---- MLP kernels ----------------------------------------------------------
@@ -248,77 +296,92 @@ void forward(spira_mlp *N, const float *X, uint32_t B)
 
 namespace spira_amd {
 
+namespace {
+#include "spmv_selector_weights.inc"
+
+// The selector network's forward pass in one block: h1 = relu(W1 f + b1),
+// h2 = relu(W2 h1 + b2), out = W3 h2 + b3, with the parameters read in place
+// (kSpmvSelParams, compiled into the code object).  Each output is summed in
+// the order k_dense_fwd uses (bias first, then fmaf over the inputs in order),
+// so the scores equal the network's own forward pass bit for bit.
+__global__ __launch_bounds__(kThreads) void k_sel_forward(const float *__restrict__ feat, float *__restrict__ out)
+{
+    __shared__ float s_a[kSpmvSelIn > kSpmvSelHidden ? kSpmvSelIn : kSpmvSelHidden];
+    __shared__ float s_b[kSpmvSelHidden];
+    const uint32_t t = threadIdx.x;
+    if (t < kSpmvSelIn) s_a[t] = feat[t];
+    __syncthreads();
+    const float *P = kSpmvSelParams;
+    if (t < kSpmvSelHidden) {  // layer 1
+        const float *W = P, *b = P + kSpmvSelIn * kSpmvSelHidden;
+        float acc = b[t];
+        for (uint32_t i = 0; i < kSpmvSelIn; ++i) acc = fmaf(W[t * kSpmvSelIn + i], s_a[i], acc);
+        s_b[t] = fmaxf(acc, 0.0f);
+    }
+    __syncthreads();
+    P += kSpmvSelIn * kSpmvSelHidden + kSpmvSelHidden;
+    if (t < kSpmvSelHidden) {  // layer 2
+        const float *W = P, *b = P + kSpmvSelHidden * kSpmvSelHidden;
+        float acc = b[t];
+        for (uint32_t i = 0; i < kSpmvSelHidden; ++i) acc = fmaf(W[t * kSpmvSelHidden + i], s_b[i], acc);
+        s_a[t] = fmaxf(acc, 0.0f);
+    }
+    __syncthreads();
+    P += kSpmvSelHidden * kSpmvSelHidden + kSpmvSelHidden;
+    if (t < kSpmvSelOut) {  // layer 3
+        const float *W = P, *b = P + kSpmvSelHidden * kSpmvSelOut;
+        float acc = b[t];
+        for (uint32_t i = 0; i < kSpmvSelHidden; ++i) acc = fmaf(W[t * kSpmvSelHidden + i], s_a[i], acc);
+        out[t] = acc;
+    }
+}
+
+static_assert(kSpmvSelNParams == kSpmvSelIn * kSpmvSelHidden + kSpmvSelHidden + kSpmvSelHidden * kSpmvSelHidden +
+                                     kSpmvSelHidden + kSpmvSelHidden * kSpmvSelOut + kSpmvSelOut,
+              "selector parameter count");
+
+// Queue the feature pass of A on its stream: kFeat floats at feat (device).
+void queue_features(spira_dcsr *A, float *feat)
+{
+    spira_dev *dev = A->dev;
+    const uint32_t nb = A->n_rows ? (uint32_t)grid_for(A->n_rows, kThreads, kFeatBlocks) : 0;
+    double *part = (double *)dev->ws.get(WS_MISC, ((size_t)nb + 1) * kStat * sizeof(double));
+    if (nb) {
+        k_feature_stats<<<nb, kThreads, 0, dev->stream>>>(A->off, A->col, A->n_rows, part);
+        SPIRA_HIP(hipGetLastError());
+    }
+    k_feature_fold<<<1, kThreads, 0, dev->stream>>>(part, nb, A->n_rows, A->nnz, A->n_cols, feat);
+    SPIRA_HIP(hipGetLastError());
+}
+
+}  // namespace
+
 // The features of a locked CSR (the list at the top of this file), computed
 // on the device; slot 17 is left 0.
 void matrix_features_host(spira_dcsr *A, float *features)
 {
     spira_dev *dev = A->dev;
-    double *st = (double *)dev->ws.get(WS_MISC, kStat * sizeof(double));
-    SPIRA_HIP(hipMemsetAsync(st, 0, kStat * sizeof(double), dev->stream));
-    double h[kStat] = {};
-    if (A->n_rows > 0) {
-        const unsigned g = grid_for(A->n_rows, kThreads, 1024);
-        k_feature_stats<<<g, kThreads, 0, dev->stream>>>(A->off, A->col, A->n_rows, st);
-        SPIRA_HIP(hipGetLastError());
-        SPIRA_HIP(hipMemcpyAsync(h, st, sizeof(h), hipMemcpyDeviceToHost, dev->stream));
-    }
-    dev_sync(dev);
-    const double n = (double)(A->n_rows ? A->n_rows : 1), nc = (double)(A->n_cols ? A->n_cols : 1);
-    const double mean = h[0] / n;
-    const double var = h[1] / n - mean * mean;
-    const double cv = mean > 0 ? std::sqrt(var > 0 ? var : 0.0) / mean : 0.0;
-    const double nonempty = n - h[3] > 0 ? n - h[3] : 1.0;
-    float f[kFeat] = {};
-    f[0] = (float)(std::log2((double)A->n_rows + 1) / 20.0);
-    f[1] = (float)(std::log2((double)A->nnz + 1) / 30.0);
-    f[2] = (float)(std::log2(mean + 1) / 10.0);
-    f[3] = (float)(std::min(cv, 4.0) / 4.0);
-    f[4] = (float)(std::log2(h[2] + 1) / 16.0);
-    for (int b = 0; b < 8; ++b) f[5 + b] = (float)(h[3 + b] / n);
-    f[13] = (float)(h[11] / nonempty / nc);
-    f[14] = (float)(h[12] / nonempty / nc);
-    f[15] = (float)(h[13] / nc);
-    f[16] = (float)(std::log2((double)A->n_cols + 1) / 20.0);
-    f[17] = 0.0f;
-    std::copy(f, f + kFeat, features);
+    float *feat = (float *)dev->ws.get(WS_SEL, kFeat * sizeof(float));
+    queue_features(A, feat);
+    read_back(dev, features, feat, kFeat * sizeof(float));
 }
-
-namespace {
-#include "spmv_selector_weights.inc"
-}  // namespace
 
 // The trained network's outputs (predicted log2 slowdown of every variant
 // against the best) for A's features, kept in A's plan; AUTO takes the lowest
-// among the variants A fits.  The forward pass runs on the device
-// (k_dense_fwd); the parameters are compiled in from the trained model.
-// false when no trained model was compiled in.
+// among the variants A fits.  Features and forward pass run on the device,
+// back to back, and only the scores come back.  false when no trained model
+// was compiled in.
 bool spmv_selector_scores(spira_dcsr *A)
 {
     if (!kSpmvSelTrained || A->n_rows == 0) return false;
     if (A->plan.scored) return true;
     static_assert(kSpmvSelIn == kFeat && kSpmvSelOut == SPIRA_SPMV_N_VARIANTS, "selector model shape");
     spira_dev *dev = A->dev;
-    hipStream_t st = dev->stream;
-    float f[kFeat];
-    matrix_features_host(A, f);
-    constexpr uint32_t dims[4] = {kSpmvSelIn, kSpmvSelHidden, kSpmvSelHidden, kSpmvSelOut};
-    constexpr size_t n_params = sizeof(kSpmvSelParams) / sizeof(float);
-    const size_t n_act = dims[0] + dims[1] + dims[2] + dims[3];
-    float *buf = (float *)dev->ws.get(WS_SEL, (n_params + n_act) * sizeof(float));
-    float *act = buf + n_params;
-    SPIRA_HIP(hipMemcpyAsync(buf, kSpmvSelParams, n_params * sizeof(float), hipMemcpyHostToDevice, st));
-    SPIRA_HIP(hipMemcpyAsync(act, f, kFeat * sizeof(float), hipMemcpyHostToDevice, st));
-    size_t po = 0, ao = 0;
-    for (int l = 0; l < 3; ++l) {
-        const uint32_t din = dims[l], dout = dims[l + 1];
-        const float *W = buf + po, *b = buf + po + (size_t)din * dout;
-        k_dense_fwd<<<1, kThreads, 0, st>>>(act + ao, W, b, act + ao + din, 1, din, dout, l < 2 ? 1 : 0);
-        SPIRA_HIP(hipGetLastError());
-        po += (size_t)din * dout + dout;
-        ao += din;
-    }
-    SPIRA_HIP(hipMemcpyAsync(A->plan.sel_score, act + ao, kSpmvSelOut * sizeof(float), hipMemcpyDeviceToHost, st));
-    dev_sync(dev);
+    float *feat = (float *)dev->ws.get(WS_SEL, (kFeat + kSpmvSelOut) * sizeof(float));
+    queue_features(A, feat);
+    k_sel_forward<<<1, kThreads, 0, dev->stream>>>(feat, feat + kFeat);
+    SPIRA_HIP(hipGetLastError());
+    read_back(dev, A->plan.sel_score, feat + kFeat, kSpmvSelOut * sizeof(float));
     A->plan.scored = true;
     return true;
 }

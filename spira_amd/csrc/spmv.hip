@@ -883,8 +883,7 @@ void ensure_stats(spira_dcsr *A)
         k_fold_stats<<<1, kThreads, 0, dev->stream>>>(bmax, bsq, nb, omax, osq);
         SPIRA_HIP(hipGetLastError());
         uint64_t h[2];
-        SPIRA_HIP(hipMemcpyAsync(h, omax, sizeof(h), hipMemcpyDeviceToHost, dev->stream));
-        dev_sync(dev);
+        read_back(dev, h, omax, sizeof(h));
         double sumsq;
         std::memcpy(&sumsq, &h[1], sizeof(double));
         P.max_len = h[0];
@@ -1033,8 +1032,7 @@ void ensure_xphase(spira_dcsr *A)
             // a sub-group's entry count is held in 32 bits by the line pass
             uint64_t mx = 0;
             std::vector<uint64_t> h(GS + 1);
-            SPIRA_HIP(hipMemcpyAsync(h.data(), gsub, (GS + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-            dev_sync(dev);
+            read_back(dev, h.data(), gsub, (GS + 1) * sizeof(uint64_t));
             for (uint64_t g = 0; g < GS; ++g) mx = std::max(mx, h[g + 1] - h[g]);
             if (mx >= (1ull << 32)) fail(SPIRA_ERR_UNSUPPORTED, "spmv: an XPHASE group holds 2^32 entries or more");
         }
@@ -1047,8 +1045,7 @@ void ensure_xphase(spira_dcsr *A)
     // the light ones fill the tail)
     {
         std::vector<uint64_t> h(G + 1);
-        SPIRA_HIP(hipMemcpyAsync(h.data(), goff, (G + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-        dev_sync(dev);
+        read_back(dev, h.data(), goff, (G + 1) * sizeof(uint64_t));
         std::vector<uint32_t> ord(nb);
         for (uint32_t i = 0; i < nb; ++i) ord[i] = i;
         std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) {
@@ -1347,6 +1344,23 @@ void spmv_xphase_f32(spira_dcsr *A, const float *x, float *y, bool paced)
     SPIRA_HIP(hipGetLastError());
 }
 
+namespace {
+
+// The fastest variant that needs no copy of A (VEC: the offsets only; STREAM:
+// a row-block list from one pass over the offsets), by the selector's scores
+// where they exist, else by the row-length rule.
+int spmv_light_variant(const spira_dcsr *A)
+{
+    const spira_plan &P = A->plan;
+    if (!P.scored) return spmv_rule_variant(A);
+    int best = SPIRA_SPMV_VEC1;
+    for (int v = SPIRA_SPMV_VEC1; v <= SPIRA_SPMV_STREAM2K; ++v)
+        if (P.sel_score[v] < P.sel_score[best]) best = v;
+    return best;
+}
+
+}  // namespace
+
 void spmv_launch_f64(spira_dcsr *A, const double *x, double *y, int variant, bool repro)
 {
     if (repro && variant >= 0 && !spmv_order_fixed(variant))
@@ -1355,7 +1369,17 @@ void spmv_launch_f64(spira_dcsr *A, const double *x, double *y, int variant, boo
     if (A->n_rows == 0) return;
     if (A->n_rows >= (1ull << 32) - 1) fail(SPIRA_ERR_UNSUPPORTED, "spmv: n_rows must be < 2^32 - 1");
     ensure_stats(A);
-    if (variant < 0) variant = spmv_resolve_auto(A, repro);
+    if (variant < 0) {
+        variant = spmv_resolve_auto(A, repro);
+        // A choice that copies A into its own layout (XSLICE, XSTREAM, XPHASE:
+        // several passes over A, worth ~5-10 calls) is built on the second AUTO
+        // call after a lock; the first runs the fastest plan-free variant, so
+        // a matrix multiplied once after each lock never pays for the copy.
+        // Reproducible calls keep one variant from the first call on.
+        if (!repro && variant >= SPIRA_SPMV_XSLICE1 && A->plan.auto_ran < 0 && !spmv_plan_ready(A, variant))
+            variant = spmv_light_variant(A);
+        A->plan.auto_ran = variant;
+    }
     if (variant == SPIRA_SPMV_STREAM1K || variant == SPIRA_SPMV_STREAM2K)
         ensure_blocks(A, variant == SPIRA_SPMV_STREAM1K ? 0 : 1);
     else if (variant >= SPIRA_SPMV_XSLICE1 && variant <= SPIRA_SPMV_XSTREAM) ensure_xslice(A);

@@ -164,14 +164,16 @@ class SpmvSelector:
 
 
 def write_inc(params, path: str = INC_PATH, source: str = "spira_amd/spmv_selector_model.json"):
-    """The parameters as the C++ include the library compiles in (layout
-    W1[hidden][in], b1, W2, b2, W3[out][hidden], b3)."""
+    """The parameters as the HIP include the library compiles in (layout
+    W1[hidden][in], b1, W2, b2, W3[out][hidden], b3): a device array, read in
+    place by the selector's forward kernel."""
     p = np.asarray(params, np.float32)
     lines = ["// Parameters of the single-matrix SpMV kernel selector, written by",
              f"// spira_amd/spmv_selector.py (write_inc) from {source}.",
              f"constexpr unsigned kSpmvSelIn = {N_FEATURES}, kSpmvSelHidden = {HIDDEN}, kSpmvSelOut = {N_OUT};",
              "constexpr int kSpmvSelTrained = 1;",
-             f"constexpr float kSpmvSelParams[{p.size}] = {{"]
+             f"constexpr unsigned kSpmvSelNParams = {p.size};",
+             f"__device__ const float kSpmvSelParams[{p.size}] = {{"]
     for i in range(0, p.size, 6):
         lines.append("    " + ", ".join(f"{float(v):.9e}f" for v in p[i:i + 6]) + ",")
     lines.append("};")

@@ -79,3 +79,41 @@ def test_auto_on_the_hbm_scale_matrices(dev, name):
     print(f"{name}: AUTO {spira_amd.VARIANT_NAMES[v]} {ms[v]:.4f} ms, best {spira_amd.VARIANT_NAMES[best]} "
           f"{ms[best]:.4f} ms")
     assert ms[best] / ms[v] >= 0.9
+
+
+def test_auto_builds_a_copying_plan_on_the_second_call(dev):
+    """AUTO whose choice copies A into its own layout (here XPHASE on a
+    1,048,576-row random matrix) runs a plan-free variant on the first call
+    after an upload and builds the copy on the second; both calls match the
+    oracle.  Seen through the allocation-free entry point, which refuses a
+    variant without a plan: after the first call the choice has none and
+    AUTO there reruns the first call's variant bit for bit; after the second
+    the choice runs.  A re-upload starts over."""
+    n = 1 << 20
+    off, col, val = W.random_uniform_rows(n, n, 8, 32, seed=77)
+    A = spira_amd.DeviceCSR(dev, n, n)
+    xh = W.dense_x(n)
+    x = torch.from_numpy(xh).cuda()
+    y_ref = oracle.spmv_f64(off, col, val, xh)
+    for cycle in range(2):
+        A.upload(off, col, val)
+        y1 = torch.empty(n, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        A.spmv(x, y1, device=True)  # AUTO, first call
+        full = A.choose_variant()
+        assert full >= spira_amd.VARIANTS["xslice1"], spira_amd.VARIANT_NAMES[full]
+        with pytest.raises(spira_amd.LogicError, match="no SpMV plan"):
+            A.spmv_noalloc(x, torch.empty_like(y1), variant=full)
+        y1n = torch.empty_like(y1)
+        A.spmv_noalloc(x, y1n)  # AUTO: the first call's plan-free variant
+        assert np.array_equal(y1.cpu().numpy().view(np.uint64), y1n.cpu().numpy().view(np.uint64))
+        assert oracle.normwise_rel_err(y1.cpu().numpy(), y_ref) <= 1e-12
+        y2 = torch.empty_like(y1)
+        A.spmv(x, y2, device=True)  # AUTO, second call: builds the choice's plan
+        assert A.choose_variant() == full
+        assert oracle.normwise_rel_err(y2.cpu().numpy(), y_ref) <= 1e-12
+        y3 = torch.empty_like(y1)
+        A.spmv_noalloc(x, y3, variant=full)  # the plan exists now
+        A.spmv_noalloc(x, y3)
+        assert oracle.normwise_rel_err(y3.cpu().numpy(), y_ref) <= 1e-12
+    A.close()
