@@ -40,6 +40,7 @@ bool spmv_variant_fits(const spira_dcsr *A, int variant);
 // for A into A->plan.sel_score (false when no trained model is compiled in).
 void matrix_features_host(spira_dcsr *A, float *features);
 bool spmv_selector_scores(spira_dcsr *A);
+bool spmv_stats_and_scores(spira_dcsr *A, uint64_t *max_len, double *sumsq);
 void spmv_launch_f64(spira_dcsr *A, const double *x, double *y, int variant, bool repro = false);
 // XPHASE (paced: XPHASE_PACED) with values stored as SPIRA_VAL_* in the plan's
 // order and x in FP32 (half the x lines to gather), products and row sums in

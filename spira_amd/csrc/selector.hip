@@ -81,17 +81,23 @@ __global__ __launch_bounds__(kThreads) void k_feature_stats(const uint64_t *__re
 }
 
 // One block: the partials of nb blocks folded into the statistics, then the
-// features (the list at the top of this file; slot 17 left 0) into feat.
+// features (the list at the top of this file; slot 17 left 0) into feat, and
+// the statistics into stats (kStat doubles) when it is not null.
 __global__ __launch_bounds__(kThreads) void k_feature_fold(const double *__restrict__ part, uint32_t nb,
                                                             uint64_t n_rows, uint64_t nnz, uint64_t n_cols,
-                                                            float *__restrict__ feat)
+                                                            float *__restrict__ feat, double *__restrict__ stats)
 {
     __shared__ double s[kThreads / 64][kStat];
     __shared__ double h[kStat];
+    double loc[kStat] = {};
+    for (uint32_t j = threadIdx.x; j < nb; j += kThreads) {  // one block's partials: kStat contiguous words
+        const double *pj = part + (uint64_t)j * kStat;
+#pragma unroll
+        for (int i = 0; i < kStat; ++i) loc[i] = stat_is_max(i) ? fmax(loc[i], pj[i]) : loc[i] + pj[i];
+    }
+#pragma unroll
     for (int i = 0; i < kStat; ++i) {
-        double v = 0.0;
-        for (uint32_t j = threadIdx.x; j < nb; j += kThreads)
-            v = stat_is_max(i) ? fmax(v, part[(uint64_t)j * kStat + i]) : v + part[(uint64_t)j * kStat + i];
+        double v = loc[i];
         for (int o = 32; o > 0; o >>= 1) {
             const double t = __shfl_xor(v, o, 64);
             v = stat_is_max(i) ? fmax(v, t) : v + t;
@@ -104,6 +110,7 @@ __global__ __launch_bounds__(kThreads) void k_feature_fold(const double *__restr
         double v = s[0][i];
         for (int w = 1; w < kThreads / 64; ++w) v = stat_is_max(i) ? fmax(v, s[w][i]) : v + s[w][i];
         h[i] = v;
+        if (stats) stats[i] = v;
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
@@ -340,8 +347,9 @@ static_assert(kSpmvSelNParams == kSpmvSelIn * kSpmvSelHidden + kSpmvSelHidden + 
                                      kSpmvSelHidden + kSpmvSelHidden * kSpmvSelOut + kSpmvSelOut,
               "selector parameter count");
 
-// Queue the feature pass of A on its stream: kFeat floats at feat (device).
-void queue_features(spira_dcsr *A, float *feat)
+// Queue the feature pass of A on its stream: kFeat floats at feat (device),
+// and the kStat statistics at stats when it is not null.
+void queue_features(spira_dcsr *A, float *feat, double *stats = nullptr)
 {
     spira_dev *dev = A->dev;
     const uint32_t nb = A->n_rows ? (uint32_t)grid_for(A->n_rows, kThreads, kFeatBlocks) : 0;
@@ -350,7 +358,7 @@ void queue_features(spira_dcsr *A, float *feat)
         k_feature_stats<<<nb, kThreads, 0, dev->stream>>>(A->off, A->col, A->n_rows, part);
         SPIRA_HIP(hipGetLastError());
     }
-    k_feature_fold<<<1, kThreads, 0, dev->stream>>>(part, nb, A->n_rows, A->nnz, A->n_cols, feat);
+    k_feature_fold<<<1, kThreads, 0, dev->stream>>>(part, nb, A->n_rows, A->nnz, A->n_cols, feat, stats);
     SPIRA_HIP(hipGetLastError());
 }
 
@@ -371,6 +379,32 @@ void matrix_features_host(spira_dcsr *A, float *features)
 // among the variants A fits.  Features and forward pass run on the device,
 // back to back, and only the scores come back.  false when no trained model
 // was compiled in.
+// Row-length statistics (longest row, sum of squared lengths) and the
+// selector's scores of A from one feature pass and one read-back: what the
+// first AUTO call needs.  false (nothing done) when no trained model was
+// compiled in.
+bool spmv_stats_and_scores(spira_dcsr *A, uint64_t *max_len, double *sumsq)
+{
+    if (!kSpmvSelTrained || A->n_rows == 0) return false;
+    spira_dev *dev = A->dev;
+    struct words {
+        double stats[kStat];
+        float feat[kFeat];
+        float score[kSpmvSelOut];
+    };
+    words *w = (words *)dev->ws.get(WS_SEL, sizeof(words));
+    queue_features(A, w->feat, w->stats);
+    k_sel_forward<<<1, kThreads, 0, dev->stream>>>(w->feat, w->score);
+    SPIRA_HIP(hipGetLastError());
+    words h;
+    read_back(dev, &h, w, sizeof(h));
+    *max_len = (uint64_t)h.stats[2];
+    *sumsq = h.stats[1];
+    std::copy(h.score, h.score + kSpmvSelOut, A->plan.sel_score);
+    A->plan.scored = true;
+    return true;
+}
+
 bool spmv_selector_scores(spira_dcsr *A)
 {
     if (!kSpmvSelTrained || A->n_rows == 0) return false;

@@ -866,35 +866,42 @@ void launch_stream(spira_dcsr *A, const uint32_t *blk, uint32_t nblk, const OffT
     k_spmv_stream<C, OffT><<<nblk, kThreads, 0, A->dev->stream>>>(blk, off, A->col, A->val, x, y);
 }
 
-void ensure_stats(spira_dcsr *A)
+// Row-length statistics and the 32-bit offsets.  for_auto: the call resolves
+// AUTO, so the selector's feature pass (which yields the same statistics)
+// runs instead of the plain one, with its scores, under one read-back.
+void ensure_stats(spira_dcsr *A, bool for_auto = false)
 {
     spira_plan &P = A->plan;
     if (P.stats_valid) return;
     spira_dev *dev = A->dev;
-    if (A->n_rows > 0) {
-        const uint64_t nb = grid_for(A->n_rows, kThreads);
-        char *w = (char *)dev->ws.get(WS_MISC, nb * 16 + 64);
-        uint64_t *bmax = (uint64_t *)w;
-        double *bsq = (double *)(w + nb * 8);
-        uint64_t *omax = (uint64_t *)(w + nb * 16);
-        double *osq = (double *)(w + nb * 16 + 8);
-        k_row_stats<<<(unsigned)nb, kThreads, 0, dev->stream>>>(A->off, A->n_rows, bmax, bsq);
-        SPIRA_HIP(hipGetLastError());
-        k_fold_stats<<<1, kThreads, 0, dev->stream>>>(bmax, bsq, nb, omax, osq);
-        SPIRA_HIP(hipGetLastError());
-        uint64_t h[2];
-        read_back(dev, h, omax, sizeof(h));
-        double sumsq;
-        std::memcpy(&sumsq, &h[1], sizeof(double));
-        P.max_len = h[0];
-        P.mean_len = (double)A->nnz / (double)A->n_rows;
-        double var = sumsq / (double)A->n_rows - P.mean_len * P.mean_len;
-        P.cv_len = P.mean_len > 0 ? std::sqrt(var > 0 ? var : 0.0) / P.mean_len : 0.0;
-    }
-    if (A->nnz < (1ull << 32) && !P.off32) {
+    if (A->nnz < (1ull << 32) && !P.off32) {  // queued first: it runs while the statistics come back
         P.off32 = (uint32_t *)plan_alloc(A, (A->n_rows + 1) * sizeof(uint32_t));
         k_off32<<<grid_for(A->n_rows + 1, kThreads), kThreads, 0, dev->stream>>>(A->off, P.off32, A->n_rows + 1);
         SPIRA_HIP(hipGetLastError());
+    }
+    if (A->n_rows > 0) {
+        uint64_t max_len = 0;
+        double sumsq = 0.0;
+        if (!(for_auto && spmv_stats_and_scores(A, &max_len, &sumsq))) {
+            const uint64_t nb = grid_for(A->n_rows, kThreads);
+            char *w = (char *)dev->ws.get(WS_MISC, nb * 16 + 64);
+            uint64_t *bmax = (uint64_t *)w;
+            double *bsq = (double *)(w + nb * 8);
+            uint64_t *omax = (uint64_t *)(w + nb * 16);
+            double *osq = (double *)(w + nb * 16 + 8);
+            k_row_stats<<<(unsigned)nb, kThreads, 0, dev->stream>>>(A->off, A->n_rows, bmax, bsq);
+            SPIRA_HIP(hipGetLastError());
+            k_fold_stats<<<1, kThreads, 0, dev->stream>>>(bmax, bsq, nb, omax, osq);
+            SPIRA_HIP(hipGetLastError());
+            uint64_t h[2];
+            read_back(dev, h, omax, sizeof(h));
+            max_len = h[0];
+            std::memcpy(&sumsq, &h[1], sizeof(double));
+        }
+        P.max_len = max_len;
+        P.mean_len = (double)A->nnz / (double)A->n_rows;
+        double var = sumsq / (double)A->n_rows - P.mean_len * P.mean_len;
+        P.cv_len = P.mean_len > 0 ? std::sqrt(var > 0 ? var : 0.0) / P.mean_len : 0.0;
     }
     P.stats_valid = true;
 }
@@ -1368,7 +1375,7 @@ void spmv_launch_f64(spira_dcsr *A, const double *x, double *y, int variant, boo
              "spmv: XPHASE variants add in a varying order; drop SPIRA_REPRODUCIBLE or choose another variant");
     if (A->n_rows == 0) return;
     if (A->n_rows >= (1ull << 32) - 1) fail(SPIRA_ERR_UNSUPPORTED, "spmv: n_rows must be < 2^32 - 1");
-    ensure_stats(A);
+    ensure_stats(A, variant < 0);
     if (variant < 0) {
         variant = spmv_resolve_auto(A, repro);
         // A choice that copies A into its own layout (XSLICE, XSTREAM, XPHASE:
