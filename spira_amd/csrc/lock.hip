@@ -522,7 +522,6 @@ __global__ __launch_bounds__(kThreads) void k_row_counts(uint64_t n, const uint6
 // ---------------------------------------------------------------------------
 constexpr uint32_t kSoRows = 384;   // rows of one re-lock scatter block staged in LDS
 constexpr uint32_t kSoIns = 1024;   // deduped inserts of those rows staged in LDS (< 2^16: 16-bit counts)
-constexpr uint32_t kCleanRow = 0xffffffffu;
 
 // Largest r in [lo, hi) with old_off[r] + sptr[r] <= p (both non-decreasing).
 __device__ __forceinline__ uint64_t row_of_key(const uint64_t *old_off, const uint64_t *sptr, uint64_t lo, uint64_t hi,
@@ -562,10 +561,10 @@ __global__ __launch_bounds__(kThreads) void k_scatter_merge(uint64_t total, cons
                                                              double *__restrict__ new_val, uint32_t *__restrict__ err,
                                                              const uint64_t *__restrict__ mtab)
 {
-    __shared__ uint64_t s_key[kSoRows + 1];  // key(lo .. hi+1)
-    __shared__ uint64_t s_oo[kSoRows + 1];   // old_off[lo .. hi+1]
+    __shared__ uint64_t s_key[kSoRows + 1];  // key(lo .. hi+1) = old_off + sptr
     __shared__ uint64_t s_no[kSoRows + 1];   // new_off[lo .. hi+1]
-    __shared__ uint32_t s_db[kSoRows];       // sptr[r] - ibase, or kCleanRow
+    __shared__ uint32_t s_sp[kSoRows + 1];   // sptr[lo .. hi+1] - ibase (< m < 2^32)
+    __shared__ uint8_t s_dirty[kSoRows];
     __shared__ uint16_t s_dn[kSoRows];       // dcnt[r]
     __shared__ uint32_t s_dcol[kSoIns];      // dcol[ibase ..)
     __shared__ uint32_t s_P[kSoIns + 1];     // (matches << 16 | nonzeros) of P[ibase + t] - P[ibase]
@@ -584,10 +583,10 @@ __global__ __launch_bounds__(kThreads) void k_scatter_merge(uint64_t total, cons
             const uint64_t r = rr.lo + j;
             const uint64_t oo = old_off[r], sp = sptr[r];
             s_key[j] = oo + sp;
-            s_oo[j] = oo;
+            s_sp[j] = (uint32_t)(sp - ibase);
             s_no[j] = new_off[r];
             if (j < nr) {
-                s_db[j] = row_dirty(dirty, sptr, r) ? (uint32_t)(sp - ibase) : kCleanRow;
+                s_dirty[j] = row_dirty(dirty, sptr, r) ? 1 : 0;
                 s_dn[j] = (uint16_t)dcnt[r];
                 // the block's positions that row j owns
                 const uint64_t a = max(oo + sp, first), b = min(old_off[r + 1] + sptr[r + 1], last + 1);
@@ -614,13 +613,14 @@ __global__ __launch_bounds__(kThreads) void k_scatter_merge(uint64_t total, cons
         row_view w;
         if (staged) {
             const uint32_t a = s_rid[p - first];
+            const uint32_t sp = s_sp[a];
             w.key = s_key[a];
-            w.ob = s_oo[a];
-            w.olen = s_oo[a + 1] - w.ob;
+            w.ob = w.key - ibase - sp;
+            w.olen = s_key[a + 1] - ibase - s_sp[a + 1] - w.ob;
             w.no = s_no[a];
             w.nlen = s_no[a + 1] - w.no;
-            w.dirty = s_db[a] != kCleanRow;
-            w.db = ibase + (w.dirty ? s_db[a] : 0u);
+            w.dirty = s_dirty[a] != 0;
+            w.db = ibase + (w.dirty ? sp : 0u);
             w.dn = s_dn[a];
         } else {
             const uint64_t r = row_of_key(old_off, sptr, rr.lo, rr.hi + 1, p);
@@ -650,43 +650,41 @@ __global__ __launch_bounds__(kThreads) void k_scatter_merge(uint64_t total, cons
     };
     auto col_of = [&](uint64_t t) -> uint32_t { return staged ? s_dcol[t - ibase] : dcol[t]; };
 
-    // Pass 1: classify every position and issue all of its loads.
-    uint32_t cs[kItemsMerge], qs[kItemsMerge];
+    // Pass 1: classify every position and issue the loads of its entry.
+    uint32_t cs[kItemsMerge];
     double vs[kItemsMerge];
-    uint8_t kind[kItemsMerge];  // 0 nothing, 1 old entry, 2 deduped insert
+    uint32_t kinds = 0;  // 2 bits per position: 0 nothing, 1 old entry, 2 deduped insert
 #pragma unroll
     for (int it = 0; it < kItemsMerge; ++it) {
         const uint64_t p = first + (uint64_t)it * kThreads + threadIdx.x;
-        kind[it] = 0;
         cs[it] = 0;
-        qs[it] = 0;
         vs[it] = 0.0;
         if (p > last) continue;
         const row_view w = view(p);
         const uint64_t i = p - w.key;
         if (i < w.olen) {
-            kind[it] = 1;
+            kinds |= 1u << (2 * it);
             cs[it] = ld_stream(old_col + w.ob + i);
             vs[it] = ld_stream(old_val + w.ob + i);
         } else if (w.dirty && i - w.olen < w.dn) {  // clean rows keep no insert; slots past the run hold nothing
             const uint64_t s = w.db + (i - w.olen);
-            kind[it] = 2;
+            kinds |= 2u << (2 * it);
             cs[it] = col_of(s);
             vs[it] = dval[s];
-            qs[it] = qpos[s];
         }
     }
     // Pass 2: output slots and stores.
 #pragma unroll
     for (int it = 0; it < kItemsMerge; ++it) {
-        if (kind[it] == 0) continue;
+        const uint32_t kind = (kinds >> (2 * it)) & 3u;
+        if (kind == 0) continue;
         const uint64_t p = first + (uint64_t)it * kThreads + threadIdx.x;
         const row_view w = view(p);
         const uint64_t i = p - w.key;
         const uint32_t c = cs[it];
         const double v = vs[it];
         uint64_t out;
-        if (kind[it] == 1) {
+        if (kind == 1) {
             const uint64_t k = w.ob + i;
             if (!w.dirty) {
                 out = w.no + i;  // clean row: verbatim (csr_build.hpp:362-377)
@@ -713,7 +711,7 @@ __global__ __launch_bounds__(kThreads) void k_scatter_merge(uint64_t total, cons
             prefix(s + 1, tm, tz);
             if (tz == sz) continue;  // zero insert: a delete marker, not an entry
             prefix(w.db, am, az);
-            const uint64_t q = qs[it];  // old entries of the row before this column
+            const uint64_t q = qpos[s];  // old entries of the row before this column
             const uint64_t before_z = PZ ? (PZ[w.ob + q] - PZ[w.ob]) : 0;
             out = w.no + (uint32_t)(sz - az) + q - (uint32_t)(sm - am) - before_z;
         }
