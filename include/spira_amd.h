@@ -45,6 +45,11 @@ typedef struct spira_dcsr spira_dcsr; /* device CSR: u64 offsets, u32 cols, f64 
 #define SPIRA_REPRODUCIBLE 32u /* SpMV: bit-identical y on every call with the
                                   same A and x (AUTO keeps to variants whose
                                   summation order is fixed) */
+#define SPIRA_COUNT_GIVEN 64u  /* lock with SPIRA_IN_DEVICE: *out_nnz holds the
+                                  staged count m (the length of stage_col and
+                                  stage_val) on entry, so stage_ptr[n_rows] is
+                                  not read back first; m != stage_ptr[n_rows]
+                                  gives SPIRA_ERR_INVALID_ARGUMENT */
 
 /* ---- SpMV kernel variants ------------------------------------------------ */
 #define SPIRA_SPMV_AUTO (-1)      /* cached choice, else the built-in heuristic */
@@ -111,7 +116,8 @@ int spira_dcsr_download_f64(const spira_dcsr *A, uint64_t *offsets, uint32_t *co
  * receives the new nnz.  Column indices >= n_cols give SPIRA_ERR_OUT_OF_RANGE
  * (matrix.hpp:167-171).  Rows may hold any number of staged inserts
  * (fewer than 2^32 in one call, as for the whole call).
- * flags: SPIRA_IN_DEVICE. */
+ * flags: SPIRA_IN_DEVICE, SPIRA_COUNT_GIVEN (with SPIRA_IN_DEVICE; out_nnz must
+ * then be given). */
 int spira_lock_f64(spira_dcsr *A, const uint64_t *stage_ptr, const uint32_t *stage_col,
                    const double *stage_val, const uint8_t *dirty, uint32_t flags,
                    uint64_t *out_nnz);

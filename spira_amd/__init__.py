@@ -52,6 +52,7 @@ IN_DEVICE = 4
 OUT_DEVICE = 8
 NO_SYNC = 16
 REPRODUCIBLE = 32
+COUNT_GIVEN = 64
 
 SPMV_AUTO = -1
 VARIANTS = {
@@ -423,10 +424,18 @@ class DeviceCSR:
                 dirty = np.ascontiguousarray(dirty, dtype=np.uint8)
                 if dirty.size != self.n_rows:
                     raise InvalidArgument("lock: dirty must have n_rows entries")
+            flags = 0
         else:
             _producers_done(self.dev, stage_ptr, stage_col, stage_val, dirty)
+            flags = IN_DEVICE
+            if hasattr(stage_col, "numel") and hasattr(stage_val, "numel"):
+                # the staged count is the arrays' length: no device read of stage_ptr[n]
+                if stage_col.numel() != stage_val.numel():
+                    raise InvalidArgument("lock: stage_col and stage_val differ in length")
+                out.value = int(stage_col.numel())
+                flags |= COUNT_GIVEN
         self.dev.check(lib().spira_lock_f64(self._h, _ptr(stage_ptr), _ptr(stage_col), _ptr(stage_val),
-                                            _ptr(dirty), IN_DEVICE if device else 0, ctypes.byref(out)))
+                                            _ptr(dirty), flags, ctypes.byref(out)))
         return int(out.value)
 
     def spmv_f32(self, x, y=None, variant: int = SPMV_AUTO, device: bool = False, sync: bool = True):

@@ -301,7 +301,10 @@ int spira_lock_f64(spira_dcsr *A, const uint64_t *stage_ptr, const uint32_t *sta
         const bool in_dev = (flags & SPIRA_IN_DEVICE) != 0;
         const uint64_t n = A->n_rows;
         uint64_t m = 0;
-        if (in_dev) {
+        if (in_dev && (flags & SPIRA_COUNT_GIVEN)) {
+            if (!out_nnz) fail(SPIRA_ERR_INVALID_ARGUMENT, "lock: SPIRA_COUNT_GIVEN needs out_nnz to hold the count");
+            m = *out_nnz;  // checked against stage_ptr[n] on the device before any column is read
+        } else if (in_dev) {
             m = read_scalar(dev, stage_ptr + n);
         } else {
             m = stage_ptr[n];

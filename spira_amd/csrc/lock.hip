@@ -96,7 +96,9 @@ __global__ void k_check_staged(const uint64_t *__restrict__ sptr, uint64_t n, ui
 {
     uint32_t *err = ctl + CTL_ERR;
     const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < m && (uint64_t)scol[r] >= n_cols) atomicOr(err, ERR_OUT_OF_RANGE);
+    // Columns are read only when the caller's count agrees with stage_ptr[n]
+    // (a count given by the caller may be wrong: then only ERR_BAD_PTR is set).
+    if (r < m && sptr[n] == m && (uint64_t)scol[r] >= n_cols) atomicOr(err, ERR_OUT_OF_RANGE);
     if (r > n) return;
     if (r == 0 && sptr[0] != 0) atomicOr(err, ERR_BAD_PTR);
     if (r == n) {
@@ -830,7 +832,15 @@ void validate_csr_device(spira_dcsr *A, uint64_t nnz, const uint64_t *d_off, con
 {
     spira_dev *dev = A->dev;
     const uint64_t n = A->n_rows;
-    const uint64_t ends[2] = {read_scalar(dev, d_off), read_scalar(dev, d_off + n)};
+    uint64_t ends[2];
+    {  // both words into the pinned buffer, one wait
+        uint64_t *hw = (uint64_t *)dev->host_words;
+        SPIRA_HIP(hipMemcpyAsync(hw, d_off, sizeof(uint64_t), hipMemcpyDeviceToHost, dev->stream));
+        SPIRA_HIP(hipMemcpyAsync(hw + 1, d_off + n, sizeof(uint64_t), hipMemcpyDeviceToHost, dev->stream));
+        dev_sync(dev);
+        ends[0] = hw[0];
+        ends[1] = hw[1];
+    }
     if (ends[0] != 0) fail(SPIRA_ERR_INVALID_ARGUMENT, "spira::matrix::load_csr: CSR offsets[0] must be zero");
     if (ends[1] != nnz)
         fail(SPIRA_ERR_INVALID_ARGUMENT, "spira::matrix::load_csr: CSR offsets[n_rows] does not equal nnz");

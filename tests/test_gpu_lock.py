@@ -478,3 +478,48 @@ def test_parallel_fill_errors(dev):
     off, col, val = pm.csr()
     assert off.tolist() == [0, 1, 1, 1, 1, 2, 2, 2, 2]
     assert col.tolist() == [0, 1] and val.tolist() == [2.0, 3.0]
+
+
+def test_lock_with_the_staged_count_given(dev):
+    """SPIRA_COUNT_GIVEN: the caller's staged count replaces the read of
+    stage_ptr[n] (device inputs).  The right count gives the same CSR as
+    without it, bit for bit; a count that disagrees with stage_ptr[n] is
+    refused before any column is read, and the matrix is left as it was."""
+    import ctypes
+
+    import torch
+    n = 5000
+    off, col, val = W.random_uniform_rows(n, n, 4, 20, seed=91)
+    rng = np.random.default_rng(92)
+    rows = rng.integers(0, n, 30000)
+    cols = rng.integers(0, n, 30000)
+    vals = rng.uniform(-1, 1, 30000)
+    sp, sc, sv = spira_amd.stage_from_coo(n, rows, cols, vals)
+    d_sp = torch.from_numpy(sp.astype(np.int64)).cuda()
+    d_sc = torch.from_numpy(sc.astype(np.int32)).cuda()
+    d_sv = torch.from_numpy(sv).cuda()
+    torch.cuda.synchronize()
+    ref = oracle.lock(sp, sc, sv, old=(off, col, val))
+    L = spira_amd.lib()
+    A = spira_amd.DeviceCSR(dev, n, n)
+    for given in (False, True):
+        A.upload(off, col, val)
+        out = ctypes.c_uint64(sc.size if given else 0)
+        flags = spira_amd.IN_DEVICE | (spira_amd.COUNT_GIVEN if given else 0)
+        rc = L.spira_lock_f64(A.handle, d_sp.data_ptr(), d_sc.data_ptr(), d_sv.data_ptr(), None, flags,
+                              ctypes.byref(out))
+        assert rc == 0
+        assert out.value == ref[1].size
+        assert csr_equal(A.download(), ref)
+    for wrong in (sc.size + 1, sc.size - 1, 0):
+        A.upload(off, col, val)
+        out = ctypes.c_uint64(wrong)
+        rc = L.spira_lock_f64(A.handle, d_sp.data_ptr(), d_sc.data_ptr(), d_sv.data_ptr(), None,
+                              spira_amd.IN_DEVICE | spira_amd.COUNT_GIVEN, ctypes.byref(out))
+        assert rc == spira_amd.ERR_INVALID_ARGUMENT, wrong
+        assert b"stage_ptr" in L.spira_dev_last_error(dev.handle)
+        assert csr_equal(A.download(), (off, col, val))
+    rc = L.spira_lock_f64(A.handle, d_sp.data_ptr(), d_sc.data_ptr(), d_sv.data_ptr(), None,
+                          spira_amd.IN_DEVICE | spira_amd.COUNT_GIVEN, None)
+    assert rc == spira_amd.ERR_INVALID_ARGUMENT
+    A.close()
