@@ -6,11 +6,13 @@
 // and B's row; a shared column holds a + b, a lone one its value; the output's
 // first lock() drops every zero.  A - B adds B scaled by -1
 // (matrix_operators.hpp:32-41, scalars.hpp:23-35).  Rows are classified by
-// la + lb: up to 64 entries are merged in registers (k_add_lanes<G>, a
-// bitonic network on column << 1 | source, the shared column's two entries
-// then neighbours), longer rows along the merge path (k_add_long); both write
-// a row's kept entries to its scratch area (A.off[r] + B.off[r]), and a scan
-// of the counts and a gather build C.
+// la + lb: up to 64 entries are merged in registers (a bitonic merge of A's
+// row and B's reversed row on column << 1 | source, the shared column's two
+// entries then neighbours), longer rows along the merge path (k_add_long).
+// A first pass counts each row's kept entries from the columns
+// (k_add_count_lanes; values only where a + b may cancel or a matrix may hold
+// explicit zeros), a scan gives C's offsets, and the merge pass
+// (k_add_lanes) writes every row straight into C.
 //
 // SpGEMM (serial/spgemm.hpp:12-50, parallel/algorithms/spgemm.hpp:24-69):
 // Gustavson rows.  For output (i, j) the reference adds the products
@@ -639,13 +641,12 @@ __global__ __launch_bounds__(kThreads) void k_spgemm_gather(uint64_t n_rows, uin
 // / 64 (k_add_lanes<G>), or longer (k_add_long).
 constexpr int kAddClasses = 4;
 constexpr uint32_t kAddCap[kAddClasses - 1] = {16, 32, 64};
+constexpr int kAddRows = 4;  // rows per lane group and step of k_add_lanes
 
-// base[i] = A.off[i] + B.off[i]: row i's scratch area (room for la + lb
-// entries); cnt[i] = 0 for empty rows; rows with entries go to their class list.
+// cnt[i] = 0 for empty rows; rows with entries go to their class list.
 __global__ __launch_bounds__(kThreads) void k_add_classify(uint64_t n_rows, const uint64_t *__restrict__ a_off,
                                                             const uint64_t *__restrict__ b_off,
-                                                            uint64_t *__restrict__ base, uint64_t *__restrict__ cnt,
-                                                            uint32_t *__restrict__ lists,
+                                                            uint64_t *__restrict__ cnt, uint32_t *__restrict__ lists,
                                                             uint32_t *__restrict__ list_n)
 {
     const uint64_t r0 = (uint64_t)blockIdx.x * kClassifyRows;
@@ -657,15 +658,12 @@ __global__ __launch_bounds__(kThreads) void k_add_classify(uint64_t n_rows, cons
         if (i < n_rows) {
             const uint64_t a0 = a_off[i], b0 = b_off[i];
             const uint64_t t = (a_off[i + 1] - a0) + (b_off[i + 1] - b0);
-            base[i] = a0 + b0;
             cnt[i] = 0;
             if (t > 0) {
                 c = kAddClasses - 1;
                 for (int q = kAddClasses - 2; q >= 0; --q)
                     if (t <= kAddCap[q]) c = q;
             }
-        } else if (i == n_rows) {
-            base[i] = a_off[i] + b_off[i];
         }
         cls[it] = c;
     }
@@ -673,76 +671,188 @@ __global__ __launch_bounds__(kThreads) void k_add_classify(uint64_t n_rows, cons
 }
 
 // Rows with at most G entries in A's and B's row together, 64 / G rows per
-// wavefront, in registers: lane g holds A's entry g or B's entry g - la, keyed
-// column << 1 | (0 for A, 1 for B); a bitonic network sorts the keys, so a
-// column both rows hold sits on two neighbouring lanes, A's first: that lane
-// writes a + b (b * -1 for A - B), the next writes nothing; zeros are dropped.
-template <int G>
+// wavefront, in registers, keyed column << 1 | (0 for A, 1 for B): lanes
+// 0 .. la-1 hold A's row in order, lanes G-lb .. G-1 hold B's row reversed and
+// the lanes between hold the largest key, so the G keys rise and then fall
+// (both rows are sorted with distinct columns) and one bitonic merge of
+// log2(G) stages sorts them.  A column both rows hold then sits on two
+// neighbouring lanes, A's first: that lane writes a + b (b * -1 for A - B),
+// the next writes nothing; zeros are dropped.
+// Each group of G lanes takes R rows per step: their list entries, offsets and
+// entries are loaded together before the R networks run, so R rows' memory
+// latency overlaps.
+// The entries go straight to C's row (c_off from the counts of
+// k_add_count_lanes, which keeps exactly the same entries); a row never writes
+// past its counted length.
+template <int G, int R>
 __global__ __launch_bounds__(256) void k_add_lanes(const uint32_t *__restrict__ list, uint32_t n_list,
                                                     const uint64_t *__restrict__ a_off,
                                                     const uint32_t *__restrict__ a_col,
                                                     const double *__restrict__ a_val,
                                                     const uint64_t *__restrict__ b_off,
                                                     const uint32_t *__restrict__ b_col,
-                                                    const double *__restrict__ b_val, int subtract,
-                                                    const uint64_t *__restrict__ base, uint64_t *__restrict__ cnt,
-                                                    uint32_t *__restrict__ t_col, double *__restrict__ t_val)
+                                                    const double *__restrict__ b_val, int subtract, int zeros,
+                                                    const uint64_t *__restrict__ c_off, uint32_t *__restrict__ c_col,
+                                                    double *__restrict__ c_val)
 {
 #pragma clang fp contract(off)
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t g = lane & (G - 1);
     const uint32_t groups = blockDim.x / G;
     const uint64_t gmask = G == kWave ? ~0ull : (((1ull << G) - 1) << (lane & ~(uint32_t)(G - 1)));
-    for (uint64_t l0 = (uint64_t)blockIdx.x * groups; l0 < n_list; l0 += (uint64_t)gridDim.x * groups) {
-        const uint64_t li = l0 + threadIdx.x / G;
-        const bool active = li < n_list;
-        const uint32_t i = active ? list[li] : 0;
-        uint64_t a0 = 0, b0 = 0;
-        uint32_t la = 0, lb = 0;
-        if (active) {
-            a0 = a_off[i];
-            la = (uint32_t)(a_off[i + 1] - a0);
-            b0 = b_off[i];
-            lb = (uint32_t)(b_off[i + 1] - b0);  // la + lb <= G (classified)
+    for (uint64_t l0 = (uint64_t)blockIdx.x * groups * R; l0 < n_list; l0 += (uint64_t)gridDim.x * groups * R) {
+        const uint64_t lb0 = l0 + (uint64_t)(threadIdx.x / G) * R;  // this group's first list entry
+        uint32_t row[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) row[j] = lb0 + j < n_list ? list[lb0 + j] : 0xffffffffu;
+        uint64_t a0[R], b0[R];
+        uint32_t la[R], lb[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            a0[j] = b0[j] = 0;
+            la[j] = lb[j] = 0;
+            if (row[j] != 0xffffffffu) {
+                const uint32_t i = row[j];
+                a0[j] = a_off[i];
+                la[j] = (uint32_t)(a_off[i + 1] - a0[j]);
+                b0[j] = b_off[i];
+                lb[j] = (uint32_t)(b_off[i + 1] - b0[j]);  // la + lb <= G (classified)
+            }
         }
-        uint64_t key = ~0ull;
-        double v = 0.0;
-        if (g < la) {
-            key = (uint64_t)a_col[a0 + g] << 1;
-            v = a_val[a0 + g];
-        } else if (g < la + lb) {
-            const uint64_t k = b0 + (g - la);
-            key = ((uint64_t)b_col[k] << 1) | 1u;
-            v = scale_b(b_val[k], subtract);
+        uint64_t key[R];
+        double v[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            key[j] = ~0ull;
+            v[j] = 0.0;
+            if (g < la[j]) {
+                key[j] = (uint64_t)a_col[a0[j] + g] << 1;
+                v[j] = a_val[a0[j] + g];
+            } else if (g >= G - lb[j]) {  // B's row reversed: lane G-1 holds its first entry
+                const uint64_t k = b0[j] + (G - 1 - g);
+                key[j] = ((uint64_t)b_col[k] << 1) | 1u;
+                v[j] = scale_b(b_val[k], subtract);
+            }
         }
 #pragma unroll
-        for (int size = 2; size <= G; size <<= 1) {
+        for (int stride = G >> 1; stride > 0; stride >>= 1) {
+            const bool lower = (g & stride) == 0;
 #pragma unroll
-            for (int stride = size >> 1; stride > 0; stride >>= 1) {
-                const uint64_t ok = xor_lane64(key, (uint32_t)stride);
-                const double ov = __builtin_bit_cast(double, xor_lane64(__builtin_bit_cast(uint64_t, v), (uint32_t)stride));
-                const bool up = (g & size) == 0, lower = (g & stride) == 0;
-                if (lower == up ? ok < key : ok > key) {
-                    key = ok;
-                    v = ov;
+            for (int j = 0; j < R; ++j) {
+                const uint64_t ok = xor_lane64(key[j], (uint32_t)stride);
+                const double ov =
+                    __builtin_bit_cast(double, xor_lane64(__builtin_bit_cast(uint64_t, v[j]), (uint32_t)stride));
+                if (lower ? ok < key[j] : ok > key[j]) {  // the lower lane keeps the smaller key
+                    key[j] = ok;
+                    v[j] = ov;
                 }
             }
         }
-        const uint32_t n = la + lb;
-        const uint64_t nk = next_lane64(key), pk = prev_lane64(key);
-        const double nv = __builtin_bit_cast(double, next_lane64(__builtin_bit_cast(uint64_t, v)));
-        const bool valid = g < n;
-        const bool dup = (key & 1u) && g > 0 && (pk >> 1) == (key >> 1);  // B's entry of a shared column
-        double out = v;
-        if (!(key & 1u) && g + 1 < n && (nk >> 1) == (key >> 1)) out = v + nv;
-        const bool keep = valid && !dup && !is_zero_f64(out);
-        const uint64_t bal = __ballot(keep) & gmask;
-        if (keep) {
-            const uint64_t pos = base[i] + __popcll(bal & ((1ull << lane) - 1));
-            t_col[pos] = (uint32_t)(key >> 1);
-            t_val[pos] = out;
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const uint32_t n = la[j] + lb[j];
+            const uint64_t nk = next_lane64(key[j]), pk = prev_lane64(key[j]);
+            const double nv = __builtin_bit_cast(double, next_lane64(__builtin_bit_cast(uint64_t, v[j])));
+            const bool valid = g < n;
+            const bool dup = (key[j] & 1u) && g > 0 && (pk >> 1) == (key[j] >> 1);  // B's entry of a shared column
+            double out = v[j];
+            const bool shared = !(key[j] & 1u) && g + 1 < n && (nk >> 1) == (key[j] >> 1);
+            if (shared) out = v[j] + nv;
+            // the counting pass's rule: a lone entry is tested for zero only
+            // when a matrix may hold explicit zeros (otherwise it never is one)
+            const bool keep = valid && !dup && (!(shared || zeros) || !is_zero_f64(out));
+            const uint64_t bal = __ballot(keep) & gmask;
+            if (keep) {
+                const uint64_t c0 = c_off[row[j]], rank = __popcll(bal & ((1ull << lane) - 1));
+                if (rank < c_off[row[j] + 1] - c0) {
+                    c_col[c0 + rank] = (uint32_t)(key[j] >> 1);
+                    c_val[c0 + rank] = out;
+                }
+            }
         }
-        if (active && g == 0) cnt[i] = __popcll(bal);
+    }
+}
+
+// The number of entries k_add_lanes keeps in each row, from the columns: the
+// same bitonic merge on keys (column << 7 | source << 6 | index in its row),
+// values read only for a column both rows hold (a + b may cancel) and, when
+// either matrix may hold explicit zeros, for every entry.
+template <int G, int R>
+__global__ __launch_bounds__(256) void k_add_count_lanes(const uint32_t *__restrict__ list, uint32_t n_list,
+                                                          const uint64_t *__restrict__ a_off,
+                                                          const uint32_t *__restrict__ a_col,
+                                                          const double *__restrict__ a_val,
+                                                          const uint64_t *__restrict__ b_off,
+                                                          const uint32_t *__restrict__ b_col,
+                                                          const double *__restrict__ b_val, int subtract, int zeros,
+                                                          uint64_t *__restrict__ cnt)
+{
+#pragma clang fp contract(off)
+    static_assert(G <= 64, "6-bit entry index");
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t g = lane & (G - 1);
+    const uint32_t groups = blockDim.x / G;
+    const uint64_t gmask = G == kWave ? ~0ull : (((1ull << G) - 1) << (lane & ~(uint32_t)(G - 1)));
+    for (uint64_t l0 = (uint64_t)blockIdx.x * groups * R; l0 < n_list; l0 += (uint64_t)gridDim.x * groups * R) {
+        const uint64_t lb0 = l0 + (uint64_t)(threadIdx.x / G) * R;
+        uint32_t row[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) row[j] = lb0 + j < n_list ? list[lb0 + j] : 0xffffffffu;
+        uint64_t a0[R], b0[R];
+        uint32_t la[R], lb[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            a0[j] = b0[j] = 0;
+            la[j] = lb[j] = 0;
+            if (row[j] != 0xffffffffu) {
+                const uint32_t i = row[j];
+                a0[j] = a_off[i];
+                la[j] = (uint32_t)(a_off[i + 1] - a0[j]);
+                b0[j] = b_off[i];
+                lb[j] = (uint32_t)(b_off[i + 1] - b0[j]);
+            }
+        }
+        uint64_t key[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            key[j] = ~0ull;
+            if (g < la[j]) {
+                key[j] = ((uint64_t)a_col[a0[j] + g] << 7) | g;
+            } else if (g >= G - lb[j]) {
+                const uint32_t k = G - 1 - g;
+                key[j] = ((uint64_t)b_col[b0[j] + k] << 7) | 64u | k;
+            }
+        }
+#pragma unroll
+        for (int stride = G >> 1; stride > 0; stride >>= 1) {
+            const bool lower = (g & stride) == 0;
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const uint64_t ok = xor_lane64(key[j], (uint32_t)stride);
+                if (lower ? ok < key[j] : ok > key[j]) key[j] = ok;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const uint32_t n = la[j] + lb[j];
+            const uint64_t k = key[j], nk = next_lane64(k), pk = prev_lane64(k);
+            const bool from_b = (k >> 6) & 1u;
+            const bool shared = !from_b && g + 1 < n && (nk >> 7) == (k >> 7);  // A's entry; B holds the column too
+            const bool dup = from_b && g > 0 && (pk >> 7) == (k >> 7);         // B's entry of a shared column
+            bool keep = g < n && !dup;
+            if (keep && (shared || zeros)) {
+                double out;
+                if (!from_b) {
+                    out = a_val[a0[j] + (k & 63u)];
+                    if (shared) out = out + scale_b(b_val[b0[j] + (nk & 63u)], subtract);
+                } else {
+                    out = scale_b(b_val[b0[j] + (k & 63u)], subtract);
+                }
+                keep = !is_zero_f64(out);
+            }
+            const uint64_t bal = __ballot(keep) & gmask;
+            if (row[j] != 0xffffffffu && g == 0) cnt[row[j]] = __popcll(bal);
+        }
     }
 }
 
@@ -753,6 +863,9 @@ __global__ __launch_bounds__(256) void k_add_lanes(const uint32_t *__restrict__ 
 // and a workgroup scan of the kept counts places the entries.
 constexpr int kAddItems = 8;
 
+// COUNT: only the kept count of each row, into cnt; otherwise the entries go
+// to C's row (c_off), never past its counted length.
+template <bool COUNT>
 __global__ __launch_bounds__(kThreads) void k_add_long(const uint32_t *__restrict__ list, uint32_t n_list,
                                                         const uint64_t *__restrict__ a_off,
                                                         const uint32_t *__restrict__ a_col,
@@ -760,15 +873,15 @@ __global__ __launch_bounds__(kThreads) void k_add_long(const uint32_t *__restric
                                                         const uint64_t *__restrict__ b_off,
                                                         const uint32_t *__restrict__ b_col,
                                                         const double *__restrict__ b_val, int subtract,
-                                                        const uint64_t *__restrict__ base, uint64_t *__restrict__ cnt,
-                                                        uint32_t *__restrict__ t_col, double *__restrict__ t_val)
+                                                        const uint64_t *__restrict__ c_off, uint64_t *__restrict__ cnt,
+                                                        uint32_t *__restrict__ c_col, double *__restrict__ c_val)
 {
 #pragma clang fp contract(off)
     __shared__ uint32_t s_w[kThreads / kWave];
     for (uint32_t li = blockIdx.x; li < n_list; li += gridDim.x) {
         const uint32_t i = list[li];
         const uint64_t a0 = a_off[i], la = a_off[i + 1] - a0, b0 = b_off[i], lb = b_off[i + 1] - b0;
-        const uint64_t n = la + lb, dst = base[i];
+        const uint64_t n = la + lb, dst = COUNT ? 0 : c_off[i], room = COUNT ? 0 : c_off[i + 1] - dst;
         const uint32_t *ac = a_col + a0, *bc = b_col + b0;
         const double *av = a_val + a0, *bv = b_val + b0;
         uint64_t carry = 0;
@@ -812,18 +925,22 @@ __global__ __launch_bounds__(kThreads) void k_add_long(const uint32_t *__restric
             }
             uint32_t tot;
             const uint32_t incl = block_incl_scan<kThreads>(nk, s_w, tot);
-            uint64_t pos = dst + carry + (incl - nk);
+            if (!COUNT) {
+                uint64_t rank = carry + (incl - nk);
 #pragma unroll
-            for (int t = 0; t < kAddItems; ++t) {
-                if (kp[t]) {
-                    t_col[pos] = oc[t];
-                    t_val[pos] = ov[t];
-                    ++pos;
+                for (int t = 0; t < kAddItems; ++t) {
+                    if (kp[t]) {
+                        if (rank < room) {
+                            c_col[dst + rank] = oc[t];
+                            c_val[dst + rank] = ov[t];
+                        }
+                        ++rank;
+                    }
                 }
             }
             carry += tot;
         }
-        if (threadIdx.x == 0) cnt[i] = carry;
+        if (COUNT && threadIdx.x == 0) cnt[i] = carry;
     }
 }
 
@@ -914,44 +1031,54 @@ extern "C" int spira_add_f64(spira_dcsr *A, spira_dcsr *B, spira_dcsr *C, uint32
         hipStream_t st = dev->stream;
         scratch_set tmp{dev, {}};
         uint64_t *cnt = (uint64_t *)tmp.get((n + 1) * sizeof(uint64_t));   // row counts, then C's offsets
-        uint64_t *base = (uint64_t *)tmp.get((n + 1) * sizeof(uint64_t));  // scratch area of each row
         uint32_t *lists = (uint32_t *)tmp.get((uint64_t)kAddClasses * (n ? n : 1) * sizeof(uint32_t));
         uint32_t *list_n = (uint32_t *)dev->ws.get(WS_MISC, 8 * sizeof(uint32_t));
         SPIRA_HIP(hipMemsetAsync(list_n, 0, 8 * sizeof(uint32_t), st));
-        k_add_classify<<<(unsigned)((n + kClassifyRows) / kClassifyRows), kThreads, 0, st>>>(n, A->off, B->off, base,
-                                                                                         cnt, lists, list_n);
+        k_add_classify<<<(unsigned)((n + kClassifyRows) / kClassifyRows), kThreads, 0, st>>>(n, A->off, B->off, cnt,
+                                                                                         lists, list_n);
         SPIRA_HIP(hipGetLastError());
         uint32_t ln[8] = {};
-        SPIRA_HIP(hipMemcpyAsync(ln, list_n, sizeof(ln), hipMemcpyDeviceToHost, st));
-        const uint64_t total = read_scalar(dev, base + n);  // nnz(A) + nnz(B); synchronises ln too
-        uint32_t *t_col = (uint32_t *)tmp.get(total * sizeof(uint32_t));
-        double *t_val = (double *)tmp.get(total * sizeof(double));
+        read_back(dev, ln, list_n, sizeof(ln));
         const uint32_t *l[kAddClasses];
         for (int c = 0; c < kAddClasses; ++c) l[c] = lists + (uint64_t)c * n;
         auto blocks = [](uint32_t rows, uint32_t per_block, uint32_t cap) {
             return (unsigned)std::min<uint64_t>((rows + per_block - 1) / per_block, cap);
         };
+        // 1. the kept count of every row (empty rows were set to 0), 2. C's
+        // offsets, 3. the merged rows written straight into C.
+        const int zeros = (A->has_zeros || B->has_zeros) ? 1 : 0;
         if (ln[0])
-            k_add_lanes<16><<<blocks(ln[0], 16, 65536), 256, 0, st>>>(l[0], ln[0], A->off, A->col, A->val, B->off,
-                                                                     B->col, B->val, sub, base, cnt, t_col, t_val);
+            k_add_count_lanes<16, kAddRows><<<blocks(ln[0], 16 * kAddRows, 65536), 256, 0, st>>>(
+                l[0], ln[0], A->off, A->col, A->val, B->off, B->col, B->val, sub, zeros, cnt);
         if (ln[1])
-            k_add_lanes<32><<<blocks(ln[1], 8, 65536), 256, 0, st>>>(l[1], ln[1], A->off, A->col, A->val, B->off,
-                                                                    B->col, B->val, sub, base, cnt, t_col, t_val);
+            k_add_count_lanes<32, kAddRows><<<blocks(ln[1], 8 * kAddRows, 65536), 256, 0, st>>>(
+                l[1], ln[1], A->off, A->col, A->val, B->off, B->col, B->val, sub, zeros, cnt);
         if (ln[2])
-            k_add_lanes<64><<<blocks(ln[2], 4, 65536), 256, 0, st>>>(l[2], ln[2], A->off, A->col, A->val, B->off,
-                                                                    B->col, B->val, sub, base, cnt, t_col, t_val);
+            k_add_count_lanes<64, kAddRows><<<blocks(ln[2], 4 * kAddRows, 65536), 256, 0, st>>>(
+                l[2], ln[2], A->off, A->col, A->val, B->off, B->col, B->val, sub, zeros, cnt);
         if (ln[3])
-            k_add_long<<<blocks(ln[3], 1, 16384), kThreads, 0, st>>>(l[3], ln[3], A->off, A->col, A->val, B->off,
-                                                                    B->col, B->val, sub, base, cnt, t_col, t_val);
+            k_add_long<true><<<blocks(ln[3], 1, 16384), kThreads, 0, st>>>(l[3], ln[3], A->off, A->col, A->val,
+                                                                          B->off, B->col, B->val, sub, nullptr, cnt,
+                                                                          nullptr, nullptr);
         SPIRA_HIP(hipGetLastError());
         scan_excl_u64(dev, cnt, cnt, n);
         const uint64_t nnz = read_scalar(dev, cnt + n);
         uint32_t *c_col = (uint32_t *)tmp.get(nnz * sizeof(uint32_t));
         double *c_val = (double *)tmp.get(nnz * sizeof(double));
         if (nnz > 0) {
-            const uint64_t nb = (nnz + (uint64_t)kThreads * kItems - 1) / ((uint64_t)kThreads * kItems);
-            if (nb >= (1ull << 31)) fail(SPIRA_ERR_UNSUPPORTED, "add: too many output entries");
-            k_spgemm_gather<<<(unsigned)nb, kThreads, 0, st>>>(n, nnz, cnt, base, t_col, t_val, c_col, c_val);
+            if (ln[0])
+                k_add_lanes<16, kAddRows><<<blocks(ln[0], 16 * kAddRows, 65536), 256, 0, st>>>(
+                    l[0], ln[0], A->off, A->col, A->val, B->off, B->col, B->val, sub, zeros, cnt, c_col, c_val);
+            if (ln[1])
+                k_add_lanes<32, kAddRows><<<blocks(ln[1], 8 * kAddRows, 65536), 256, 0, st>>>(
+                    l[1], ln[1], A->off, A->col, A->val, B->off, B->col, B->val, sub, zeros, cnt, c_col, c_val);
+            if (ln[2])
+                k_add_lanes<64, kAddRows><<<blocks(ln[2], 4 * kAddRows, 65536), 256, 0, st>>>(
+                    l[2], ln[2], A->off, A->col, A->val, B->off, B->col, B->val, sub, zeros, cnt, c_col, c_val);
+            if (ln[3])
+                k_add_long<false><<<blocks(ln[3], 1, 16384), kThreads, 0, st>>>(l[3], ln[3], A->off, A->col,
+                                                                               A->val, B->off, B->col, B->val, sub,
+                                                                               cnt, nullptr, c_col, c_val);
             SPIRA_HIP(hipGetLastError());
         }
         tmp.keep(cnt);
