@@ -543,10 +543,84 @@ __device__ __forceinline__ uint64_t row_of_key(const uint64_t *old_off, const ui
 // entry and every surviving deduped insert computes its own output slot from
 // the prefix sums, and both parts of a row's new range are written by the
 // same block close together in time, so its cache lines leave L2 whole.
-// The block's rows (key, old_off, new_off, dirty bit, deduped-insert range)
-// and their deduped inserts (columns, prefix counts relative to the first)
-// are loaded into LDS once, in parallel; blocks whose rows or inserts do not
-// fit read the same quantities from global memory.
+//
+// Position p of row r is old entry k = p - sptr[r] when k < old_off[r + 1],
+// else deduped slot s = p - old_off[r + 1] (a surviving insert when the row is
+// dirty and s - sptr[r] < dcnt[r]; later slots hold nothing).  A block whose
+// rows and deduped inserts fit in LDS (the common case) stages the rows
+// (sptr and old-end offsets relative to the block, new offsets, dirty bit,
+// deduped count), their deduped columns and 16-bit prefix counts and the row
+// of each position, and does its index arithmetic in 32 bits; other blocks
+// read the same quantities from global memory (scatter_merge_global).
+
+// One block of the re-lock scatter with every row quantity read from global
+// memory (a binary search finds each position's row).
+__device__ __forceinline__ void scatter_merge_global(uint64_t first, uint64_t last, uint64_t rlo, uint64_t rhi,
+                                                  const uint64_t *__restrict__ old_off,
+                                                  const uint32_t *__restrict__ old_col,
+                                                  const double *__restrict__ old_val,
+                                                  const uint64_t *__restrict__ sptr,
+                                                  const uint32_t *__restrict__ dcnt,
+                                                  const uint32_t *__restrict__ dcol,
+                                                  const double *__restrict__ dval, const uint64_t *__restrict__ P,
+                                                  const uint32_t *__restrict__ qpos, const uint64_t *__restrict__ PZ,
+                                                  const uint8_t *__restrict__ dirty,
+                                                  const uint64_t *__restrict__ new_off, uint32_t *__restrict__ new_col,
+                                                  double *__restrict__ new_val, uint32_t *__restrict__ err)
+{
+    for (int it = 0; it < kItemsMerge; ++it) {
+        const uint64_t p = first + (uint64_t)it * kThreads + threadIdx.x;
+        if (p > last) break;
+        const uint64_t r = row_of_key(old_off, sptr, rlo, rhi + 1, p);
+        const uint64_t ob = old_off[r], db = sptr[r], key = ob + db, olen = old_off[r + 1] - ob;
+        const uint64_t no = new_off[r], nlen = new_off[r + 1] - no;
+        const bool dty = row_dirty(dirty, sptr, r);
+        const uint64_t dn = dcnt[r];
+        const uint64_t i = p - key;
+        uint64_t out;
+        uint32_t c;
+        double v;
+        if (i < olen) {
+            const uint64_t k = ob + i;
+            c = old_col[k];
+            v = old_val[k];
+            if (!dty) {
+                out = no + i;  // clean row: verbatim (csr_build.hpp:362-377)
+            } else {
+                const uint64_t de = db + dn;
+                uint64_t lo = db, hi = de;  // lower bound of c in the row's deduped columns
+                while (lo < hi) {
+                    const uint64_t mid = (lo + hi) >> 1;
+                    if (dcol[mid] < c) lo = mid + 1;
+                    else hi = mid;
+                }
+                if (lo < de && dcol[lo] == c) continue;  // buffer wins (:430-436)
+                if (PZ && is_zero_f64(v)) continue;      // old zero dropped (:420-421)
+                const uint64_t a = P[db], b = P[lo];
+                const uint64_t before_z = PZ ? (PZ[k] - PZ[ob]) : 0;
+                out = no + i - (uint32_t)((b >> 32) - (a >> 32)) - before_z + (uint32_t)((uint32_t)b - (uint32_t)a);
+            }
+        } else if (dty && i - olen < dn) {  // clean rows keep no insert; slots past the run hold nothing
+            const uint64_t s = db + (i - olen);
+            const uint64_t a = P[db], sw = P[s], tw = P[s + 1];
+            if ((uint32_t)tw == (uint32_t)sw) continue;  // zero insert: a delete marker, not an entry
+            c = dcol[s];
+            v = dval[s];
+            const uint64_t q = qpos[s];  // old entries of the row before this column
+            const uint64_t before_z = PZ ? (PZ[ob + q] - PZ[ob]) : 0;
+            out = no + (uint32_t)((uint32_t)sw - (uint32_t)a) + q - (uint32_t)((sw >> 32) - (a >> 32)) - before_z;
+        } else {
+            continue;
+        }
+        if (out - no >= nlen) {  // every slot must fall inside the row's new range; never write outside
+            atomicOr(err, ERR_INTERNAL);
+            continue;
+        }
+        new_col[out] = c;
+        new_val[out] = v;
+    }
+}
+
 __global__ __launch_bounds__(kThreads) void k_scatter_merge(uint64_t total, const uint64_t *__restrict__ old_off,
                                                              const uint32_t *__restrict__ old_col,
                                                              const double *__restrict__ old_val,
@@ -563,15 +637,16 @@ __global__ __launch_bounds__(kThreads) void k_scatter_merge(uint64_t total, cons
                                                              double *__restrict__ new_val, uint32_t *__restrict__ err,
                                                              const uint64_t *__restrict__ mtab)
 {
-    __shared__ uint64_t s_key[kSoRows + 1];  // key(lo .. hi+1) = old_off + sptr
     __shared__ uint64_t s_no[kSoRows + 1];   // new_off[lo .. hi+1]
-    __shared__ uint32_t s_sp[kSoRows + 1];   // sptr[lo .. hi+1] - ibase (< m < 2^32)
+    __shared__ int64_t s_ob[kSoRows];        // old_off[r] - K0 (old index of the row's first entry, block-relative)
+    __shared__ uint32_t s_sp[kSoRows];       // sptr[r] - ibase
+    __shared__ int32_t s_oe[kSoRows];        // old_off[r + 1] - K0, clamped to +-2^30
     __shared__ uint8_t s_dirty[kSoRows];
     __shared__ uint16_t s_dn[kSoRows];       // dcnt[r]
     __shared__ uint32_t s_dcol[kSoIns];      // dcol[ibase ..)
     __shared__ uint32_t s_P[kSoIns + 1];     // (matches << 16 | nonzeros) of P[ibase + t] - P[ibase]
     __shared__ uint16_t s_rid[kPerBlockMerge];  // local row of each position of the block
-    static_assert(kSoRows <= 65536, "local rows are 16-bit");
+    static_assert(kSoRows <= 65536 && kSoIns < 65536, "local rows and 16-bit prefix counts");
     const uint64_t first = (uint64_t)blockIdx.x * kPerBlockMerge;
     if (first >= total || *err) return;
     const uint64_t last = min(first + kPerBlockMerge, total) - 1;
@@ -579,78 +654,41 @@ __global__ __launch_bounds__(kThreads) void k_scatter_merge(uint64_t total, cons
     const uint64_t nr = rr.hi - rr.lo + 1;
     const uint64_t ibase = sptr[rr.lo];
     const uint64_t ni = sptr[rr.hi] + dcnt[rr.hi] - ibase;  // deduped inserts of rows lo..hi lie in [ibase, ibase+ni)
-    const bool staged = nr <= kSoRows && ni <= kSoIns;
-    if (staged) {
-        for (uint32_t j = threadIdx.x; j <= nr; j += kThreads) {
-            const uint64_t r = rr.lo + j;
-            const uint64_t oo = old_off[r], sp = sptr[r];
-            s_key[j] = oo + sp;
+    if (nr > kSoRows || ni > kSoIns) {  // the same for every thread of the block
+        scatter_merge_global(first, last, rr.lo, rr.hi, old_off, old_col, old_val, sptr, dcnt, dcol, dval, P, qpos,
+                             PZ, dirty, new_off, new_col, new_val, err);
+        return;
+    }
+    // Block-relative indices: position q = p - first; old entry k = K0 + d
+    // with d = q - (sptr[r] - ibase) in [-kSoIns, kPerBlockMerge); deduped slot
+    // s = ibase + (q - s_oe[r]).  first >= key(lo) >= sptr[lo], so K0 >= 0.
+    const uint64_t K0 = first - ibase;
+    for (uint32_t j = threadIdx.x; j <= nr; j += kThreads) {
+        const uint64_t r = rr.lo + j;
+        s_no[j] = new_off[r];
+        if (j < nr) {
+            const uint64_t oo = old_off[r], oe = old_off[r + 1], sp = sptr[r];
+            const int64_t oe_rel = (int64_t)(oe - K0);
+            s_ob[j] = (int64_t)(oo - K0);
             s_sp[j] = (uint32_t)(sp - ibase);
-            s_no[j] = new_off[r];
-            if (j < nr) {
-                s_dirty[j] = row_dirty(dirty, sptr, r) ? 1 : 0;
-                s_dn[j] = (uint16_t)dcnt[r];
-                // the block's positions that row j owns
-                const uint64_t a = max(oo + sp, first), b = min(old_off[r + 1] + sptr[r + 1], last + 1);
-                for (uint64_t p = a; p < b; ++p) s_rid[p - first] = (uint16_t)j;
-            }
-        }
-        const uint64_t p0 = P[ibase];
-        for (uint32_t t = threadIdx.x; t <= ni; t += kThreads) {
-            const uint64_t pt = P[ibase + t];
-            const uint32_t dm = (uint32_t)(pt >> 32) - (uint32_t)(p0 >> 32);
-            const uint32_t dz = (uint32_t)pt - (uint32_t)p0;
-            s_P[t] = (dm << 16) | dz;
-            if (t < ni) s_dcol[t] = dcol[ibase + t];
+            s_oe[j] = (int32_t)max<int64_t>(-(1ll << 30), min<int64_t>(oe_rel, 1ll << 30));
+            s_dirty[j] = row_dirty(dirty, sptr, r) ? 1 : 0;
+            s_dn[j] = (uint16_t)dcnt[r];
+            // the block's positions that row j owns
+            const uint64_t a = max(oo + sp, first), b = min(oe + sptr[r + 1], last + 1);
+            for (uint64_t p = a; p < b; ++p) s_rid[p - first] = (uint16_t)j;
         }
     }
+    const uint64_t p0 = P[ibase];
+    for (uint32_t t = threadIdx.x; t <= ni; t += kThreads) {
+        const uint64_t pt = P[ibase + t];
+        const uint32_t dm = (uint32_t)(pt >> 32) - (uint32_t)(p0 >> 32);
+        const uint32_t dz = (uint32_t)pt - (uint32_t)p0;
+        s_P[t] = (dm << 16) | dz;
+        if (t < ni) s_dcol[t] = dcol[ibase + t];
+    }
     __syncthreads();
-
-    // Row of position p and what the scatter needs of it, from LDS or memory.
-    struct row_view {
-        uint64_t key, ob, olen, no, nlen, db, dn;  // db: absolute index of the row's first staged slot
-        bool dirty;
-    };
-    auto view = [&](uint64_t p) {
-        row_view w;
-        if (staged) {
-            const uint32_t a = s_rid[p - first];
-            const uint32_t sp = s_sp[a];
-            w.key = s_key[a];
-            w.ob = w.key - ibase - sp;
-            w.olen = s_key[a + 1] - ibase - s_sp[a + 1] - w.ob;
-            w.no = s_no[a];
-            w.nlen = s_no[a + 1] - w.no;
-            w.dirty = s_dirty[a] != 0;
-            w.db = ibase + (w.dirty ? sp : 0u);
-            w.dn = s_dn[a];
-        } else {
-            const uint64_t r = row_of_key(old_off, sptr, rr.lo, rr.hi + 1, p);
-            w.ob = old_off[r];
-            w.db = sptr[r];
-            w.key = w.ob + w.db;
-            w.olen = old_off[r + 1] - w.ob;
-            w.no = new_off[r];
-            w.nlen = new_off[r + 1] - w.no;
-            w.dirty = row_dirty(dirty, sptr, r);
-            w.dn = dcnt[r];
-        }
-        return w;
-    };
-    // (matches, nonzeros) before deduped slot t, relative to any other slot of
-    // the block's rows: only differences are used.
-    auto prefix = [&](uint64_t t, uint32_t &pm, uint32_t &pz) {
-        if (staged) {
-            const uint32_t w = s_P[t - ibase];
-            pm = w >> 16;
-            pz = w & 0xffffu;
-        } else {
-            const uint64_t w = P[t];
-            pm = (uint32_t)(w >> 32);
-            pz = (uint32_t)w;
-        }
-    };
-    auto col_of = [&](uint64_t t) -> uint32_t { return staged ? s_dcol[t - ibase] : dcol[t]; };
+    const uint32_t nq = (uint32_t)(last - first + 1);
 
     // Pass 1: classify every position and issue the loads of its entry.
     uint32_t cs[kItemsMerge];
@@ -658,21 +696,25 @@ __global__ __launch_bounds__(kThreads) void k_scatter_merge(uint64_t total, cons
     uint32_t kinds = 0;  // 2 bits per position: 0 nothing, 1 old entry, 2 deduped insert
 #pragma unroll
     for (int it = 0; it < kItemsMerge; ++it) {
-        const uint64_t p = first + (uint64_t)it * kThreads + threadIdx.x;
+        const uint32_t q = (uint32_t)it * kThreads + threadIdx.x;
         cs[it] = 0;
         vs[it] = 0.0;
-        if (p > last) continue;
-        const row_view w = view(p);
-        const uint64_t i = p - w.key;
-        if (i < w.olen) {
+        if (q >= nq) continue;
+        const uint32_t j = s_rid[q];
+        const int32_t sp = (int32_t)s_sp[j], oe = s_oe[j];
+        const int32_t d = (int32_t)q - sp;
+        if (d < oe) {
+            const uint64_t k = K0 + (int64_t)d;
             kinds |= 1u << (2 * it);
-            cs[it] = ld_stream(old_col + w.ob + i);
-            vs[it] = ld_stream(old_val + w.ob + i);
-        } else if (w.dirty && i - w.olen < w.dn) {  // clean rows keep no insert; slots past the run hold nothing
-            const uint64_t s = w.db + (i - w.olen);
-            kinds |= 2u << (2 * it);
-            cs[it] = col_of(s);
-            vs[it] = dval[s];
+            cs[it] = ld_stream(old_col + k);
+            vs[it] = ld_stream(old_val + k);
+        } else {
+            const int32_t srel = (int32_t)q - oe;  // deduped slot, relative to ibase
+            if (s_dirty[j] && srel - sp < (int32_t)s_dn[j]) {  // clean rows keep no insert; slots past the run hold nothing
+                kinds |= 2u << (2 * it);
+                cs[it] = s_dcol[srel];
+                vs[it] = dval[ibase + (uint32_t)srel];
+            }
         }
     }
     // Pass 2: output slots and stores.
@@ -680,44 +722,43 @@ __global__ __launch_bounds__(kThreads) void k_scatter_merge(uint64_t total, cons
     for (int it = 0; it < kItemsMerge; ++it) {
         const uint32_t kind = (kinds >> (2 * it)) & 3u;
         if (kind == 0) continue;
-        const uint64_t p = first + (uint64_t)it * kThreads + threadIdx.x;
-        const row_view w = view(p);
-        const uint64_t i = p - w.key;
+        const uint32_t q = (uint32_t)it * kThreads + threadIdx.x;
+        const uint32_t j = s_rid[q];
+        const uint32_t sp = s_sp[j];
+        const uint64_t no = s_no[j], nlen = s_no[j + 1] - no;
         const uint32_t c = cs[it];
         const double v = vs[it];
+        const uint32_t wa = s_P[sp];  // prefix counts at the row's first deduped slot
         uint64_t out;
         if (kind == 1) {
-            const uint64_t k = w.ob + i;
-            if (!w.dirty) {
-                out = w.no + i;  // clean row: verbatim (csr_build.hpp:362-377)
+            const int32_t d = (int32_t)q - (int32_t)sp;
+            const uint64_t i = (uint64_t)((int64_t)d - s_ob[j]);  // index in the row's old entries
+            if (!s_dirty[j]) {
+                out = no + i;  // clean row: verbatim (csr_build.hpp:362-377)
             } else {
-                const uint64_t de = w.db + w.dn;
-                uint64_t lo = w.db, hi = de;  // lower bound of c in the row's deduped columns
+                const uint32_t de = sp + s_dn[j];
+                uint32_t lo = sp, hi = de;  // lower bound of c in the row's deduped columns
                 while (lo < hi) {
-                    const uint64_t mid = (lo + hi) >> 1;
-                    if (col_of(mid) < c) lo = mid + 1;
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (s_dcol[mid] < c) lo = mid + 1;
                     else hi = mid;
                 }
-                if (lo < de && col_of(lo) == c) continue;  // buffer wins (:430-436)
+                if (lo < de && s_dcol[lo] == c) continue;  // buffer wins (:430-436)
                 if (PZ && is_zero_f64(v)) continue;        // old zero dropped (:420-421)
-                uint32_t am, az, pm, pz;
-                prefix(w.db, am, az);
-                prefix(lo, pm, pz);
-                const uint64_t before_z = PZ ? (PZ[k] - PZ[w.ob]) : 0;
-                out = w.no + i - (uint32_t)(pm - am) - before_z + (uint32_t)(pz - az);
+                const uint32_t wb = s_P[lo];
+                const uint64_t ob = K0 + (uint64_t)s_ob[j];
+                const uint64_t before_z = PZ ? (PZ[ob + i] - PZ[ob]) : 0;
+                out = no + i - ((wb >> 16) - (wa >> 16)) - before_z + ((wb & 0xffffu) - (wa & 0xffffu));
             }
         } else {
-            const uint64_t s = w.db + (i - w.olen);
-            uint32_t am, az, sm, sz, tm, tz;
-            prefix(s, sm, sz);
-            prefix(s + 1, tm, tz);
-            if (tz == sz) continue;  // zero insert: a delete marker, not an entry
-            prefix(w.db, am, az);
-            const uint64_t q = qpos[s];  // old entries of the row before this column
-            const uint64_t before_z = PZ ? (PZ[w.ob + q] - PZ[w.ob]) : 0;
-            out = w.no + (uint32_t)(sz - az) + q - (uint32_t)(sm - am) - before_z;
+            const uint32_t srel = q - (uint32_t)s_oe[j];
+            const uint32_t ws = s_P[srel], wt = s_P[srel + 1];
+            if ((wt & 0xffffu) == (ws & 0xffffu)) continue;  // zero insert: a delete marker, not an entry
+            const uint64_t qo = qpos[ibase + srel];  // old entries of the row before this column
+            const uint64_t ob = K0 + (uint64_t)s_ob[j];
+            const uint64_t before_z = PZ ? (PZ[ob + qo] - PZ[ob]) : 0;
+            out = no + ((ws & 0xffffu) - (wa & 0xffffu)) + qo - ((ws >> 16) - (wa >> 16)) - before_z;
         }
-        const uint64_t no = w.no, nlen = w.nlen;
         // Every slot must fall inside the row's new range; never write outside.
         if (out - no >= nlen) {
             atomicOr(err, ERR_INTERNAL);

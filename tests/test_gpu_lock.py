@@ -523,3 +523,40 @@ def test_lock_with_the_staged_count_given(dev):
                           spira_amd.IN_DEVICE | spira_amd.COUNT_GIVEN, None)
     assert rc == spira_amd.ERR_INVALID_ARGUMENT
     A.close()
+
+
+def test_relock_scatter_staged_and_global_blocks(dev):
+    """One re-lock whose scatter blocks take both paths: a region of one-entry
+    rows (a block of 2,048 positions spans more rows than it stages in LDS),
+    regular rows (staged), and a row with thousands of distinct inserts (more
+    deduped inserts in a block than it stages), over an old CSR with explicit
+    zeros and a dirty mask wider than the inserted rows; bit-exact against the
+    oracle."""
+    rng = np.random.default_rng(123)
+    n, nc = 20000, 50000
+    lens = np.concatenate([np.ones(8000, np.int64), np.full(8000, 20, np.int64), rng.integers(0, 40, 4000)])
+    off = np.zeros(n + 1, np.uint64)
+    np.cumsum(lens, out=off[1:])
+    col = np.concatenate([np.sort(rng.choice(nc, int(k), replace=False)) for k in lens]).astype(np.uint32)
+    val = rng.uniform(-1, 1, col.size)
+    val[rng.random(col.size) < 0.2] = 0.0  # explicit zeros: only load_csr keeps them
+    A = spira_amd.DeviceCSR(dev, n, nc)
+    A.upload(off, col, val)
+    rows = [np.arange(0, 8000, 2), np.repeat(np.arange(8000, 16000), 2), np.full(3000, 17000)]
+    rows = np.concatenate(rows)
+    cols = rng.integers(0, nc, rows.size)
+    own = rows[(rows >= 8000) & (rows < 16000)]
+    hit = rng.random(own.size) < 0.25  # a quarter of the regular rows' inserts overwrite
+    sel = np.nonzero((rows >= 8000) & (rows < 16000))[0][hit]
+    for k in sel:
+        r = rows[k]
+        cols[k] = col[int(off[r]) + int(rng.integers(0, lens[r]))]
+    vals = rng.uniform(-1, 1, rows.size)
+    vals[rng.random(rows.size) < 0.1] = 0.0
+    ptr, c, v = stage(n, rows, cols, vals)
+    dirty = (rng.random(n) < 0.1).astype(np.uint8)
+    dirty[np.bincount(rows, minlength=n) > 0] = 1
+    A.lock(ptr, c, v, dirty=dirty)
+    exp = oracle.lock(ptr, c, v, old=(off, col, val), dirty=dirty)
+    assert csr_equal(A.download(), exp)
+    A.close()
