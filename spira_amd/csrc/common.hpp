@@ -55,6 +55,29 @@ __device__ __forceinline__ T ld_stream(const T *p)
     return __builtin_nontemporal_load(p);
 }
 
+// Unsigned 32-bit division by a divisor fixed for a launch: one multiply-high,
+// one add and one shift (Granlund & Montgomery, "Division by invariant
+// integers using multiplication", PLDI 1994, Figure 4.1).  For 1 <= d < 2^32,
+// l = ceil(log2 d) and m = floor(2^32 (2^l - d) / d) + 1 (< 2^32):
+// n / d = (mulhi(m, n) + n) >> l, the sum taken in 33 bits.
+struct udiv32 {
+    uint32_t mul = 1, shift = 0;  // d = 1
+};
+inline udiv32 make_udiv32(uint32_t d)  // d >= 1 (d <= 1 gives the identity)
+{
+    udiv32 f;
+    if (d <= 1) return f;
+    uint32_t l = 0;
+    while ((1ull << l) < d) ++l;
+    f.mul = (uint32_t)(((1ull << 32) * ((1ull << l) - d)) / d + 1);
+    f.shift = l;
+    return f;
+}
+__device__ __forceinline__ uint32_t udiv(uint32_t n, udiv32 f)
+{
+    return (uint32_t)(((uint64_t)__umulhi(n, f.mul) + n) >> f.shift);
+}
+
 // Sum over aligned groups of G lanes (G a power of two <= 64).
 template <int G>
 __device__ __forceinline__ double group_sum(double v)

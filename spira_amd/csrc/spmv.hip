@@ -485,23 +485,63 @@ __global__ void k_xphase_chunk_rows(const uint64_t *__restrict__ off, uint64_t n
     if (e1 == nnz) tab[(nnz + kPlanChunk - 1) / kPlanChunk] = r;
 }
 
+// In-place running maximum of s[0..n) (n <= kPlanChunk) by a kPlanThreads
+// block, kPlanItems consecutive values per thread.
+__device__ __forceinline__ void block_max_scan_u16(uint16_t *s, uint32_t n)
+{
+    __shared__ uint32_t s_w[kPlanThreads / kWave];
+    const uint32_t a = threadIdx.x * kPlanItems;
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x >> 6;
+    uint32_t v[kPlanItems];
+    uint32_t m = 0;
+#pragma unroll
+    for (int i = 0; i < kPlanItems; ++i) {
+        v[i] = a + i < n ? s[a + i] : 0u;
+        m = max(m, v[i]);
+        v[i] = m;
+    }
+    uint32_t inc = m;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const uint32_t t = __shfl_up(inc, d, kWave);
+        if (lane >= d) inc = max(inc, t);
+    }
+    if (lane == kWave - 1) s_w[wid] = inc;
+    __syncthreads();
+    uint32_t ex = __shfl_up(inc, 1, kWave);
+    if (lane == 0) ex = 0;
+    for (int w = 0; w < wid; ++w) ex = max(ex, s_w[w]);
+#pragma unroll
+    for (int i = 0; i < kPlanItems; ++i)
+        if (a + i < n) s[a + i] = (uint16_t)max(v[i], ex);
+    __syncthreads();
+}
+
+// Entries of one chunk (kPlanChunk of them) binned by (row block, slice,
+// sub-slice).  The row of each entry: every non-empty row of the chunk marks
+// its first entry with its index in the chunk's row range and a running
+// maximum spreads the marks over the row's entries (up to kPlanRowCache rows;
+// a chunk touching more searches the offsets in global memory).  Divisions by
+// the row-block size and the slice width are multiply-shifts (udiv32).
 template <bool SCATTER>
-__global__ __launch_bounds__(kPlanThreads) void k_xphase_bin(uint64_t nnz, uint64_t n_rows, uint32_t RB, uint32_t S,
-                                                              uint32_t SUB, uint32_t width,
-                                                              const uint64_t *__restrict__ off,
+__global__ __launch_bounds__(kPlanThreads) void k_xphase_bin(uint64_t nnz, uint64_t n_rows, uint32_t RB, udiv32 rb_div,
+                                                              uint32_t S, uint32_t SUB, uint32_t width,
+                                                              udiv32 width_div, const uint64_t *__restrict__ off,
                                                               const uint32_t *__restrict__ col,
                                                               const double *__restrict__ val,
                                                               const uint64_t *__restrict__ tab,
                                                               unsigned long long *__restrict__ gctr,
                                                               uint32_t *__restrict__ trc, double *__restrict__ tval)
 {
-    __shared__ uint64_t s_off[kPlanRowCache + 1];
+    static_assert(kPlanRowCache <= 65536, "16-bit local rows");
+    __shared__ uint16_t s_rid[kPlanChunk];   // local row of each entry of the chunk
     __shared__ uint32_t s_cnt[kPlanLocalGroups];
     __shared__ uint64_t s_base[SCATTER ? kPlanLocalGroups : 1];
     const uint64_t first = (uint64_t)blockIdx.x * kPlanChunk;
     if (first >= nnz) return;
     const uint64_t last = min(first + kPlanChunk, nnz) - 1;
-    // the entries first: their loads are in flight while the row cache fills
+    const uint32_t nq = (uint32_t)(last - first + 1);
+    // the entries first: their loads are in flight while the row marks fill
     uint32_t cc[kPlanItems];
     double v[kPlanItems];
 #pragma unroll
@@ -511,43 +551,42 @@ __global__ __launch_bounds__(kPlanThreads) void k_xphase_bin(uint64_t nnz, uint6
         if (SCATTER) v[it] = k <= last ? ld_stream(val + k) : 0.0;
     }
     const uint64_t rlo = tab[blockIdx.x], rhi = tab[blockIdx.x + 1];
-    const bool cached = rhi - rlo + 1 <= kPlanRowCache;
-    if (cached)
-        for (uint64_t j = threadIdx.x; j <= rhi - rlo + 1; j += kPlanThreads) s_off[j] = off[rlo + j];
+    const uint64_t nr = rhi - rlo + 1;
+    const bool cached = nr <= kPlanRowCache;
     const uint64_t gps = (uint64_t)S * SUB;  // sub-groups per block
-    const uint64_t b0 = rlo / RB;
-    const uint64_t nlg = (rhi / RB - b0 + 1) * gps;  // sub-groups this chunk can touch
+    const uint32_t b0 = udiv((uint32_t)rlo, rb_div);
+    const uint64_t nlg = (uint64_t)(udiv((uint32_t)rhi, rb_div) - b0 + 1) * gps;  // sub-groups this chunk can touch
     const bool local = nlg <= kPlanLocalGroups;
+    if (cached)
+        for (uint32_t j = threadIdx.x; j < nq; j += kPlanThreads) s_rid[j] = 0;  // row rlo owns entry `first`
     if (local)
         for (uint32_t j = threadIdx.x; j < nlg; j += kPlanThreads) s_cnt[j] = 0;
     __syncthreads();
+    if (cached) {
+        // rows after rlo start after `first`; a non-empty one marks its first entry
+        for (uint32_t j = threadIdx.x + 1; j < nr; j += kPlanThreads) {
+            const uint64_t a = off[rlo + j];
+            if (a <= last && off[rlo + j + 1] > a) s_rid[a - first] = (uint16_t)j;
+        }
+        __syncthreads();
+        block_max_scan_u16(s_rid, nq);
+    }
 
     uint32_t lg[kPlanItems], rc[kPlanItems];
 #pragma unroll
     for (int it = 0; it < kPlanItems; ++it) {
-        const uint64_t k = first + (uint64_t)it * kPlanThreads + threadIdx.x;
+        const uint32_t q = (uint32_t)it * kPlanThreads + threadIdx.x;
         lg[it] = 0xffffffffu;
-        if (k > last) continue;
+        if (q >= nq) continue;
         const uint32_t c = cc[it];
-        uint64_t r;
-        if (cached) {
-            uint32_t a = 0, e = (uint32_t)(rhi - rlo + 1);
-            while (e - a > 1) {
-                const uint32_t mid = (a + e) >> 1;
-                if (s_off[mid] <= k) a = mid;
-                else e = mid;
-            }
-            r = rlo + a;
-        } else {
-            r = row_of(off, rlo, rhi + 1, k);
-        }
-        const uint64_t b = r / RB;
-        const uint32_t sl = c / width;
+        const uint32_t r = cached ? (uint32_t)rlo + s_rid[q] : (uint32_t)row_of(off, rlo, rhi + 1, first + q);
+        const uint32_t b = udiv(r, rb_div);
+        const uint32_t sl = udiv(c, width_div);
         const uint32_t cl = c - sl * width;
-        rc[it] = ((uint32_t)(r - b * RB) << kPhaseColBits) | cl;
-        lg[it] = (uint32_t)((b - b0) * gps + (uint64_t)sl * SUB + (cl >> kSubBits));
+        rc[it] = ((r - b * RB) << kPhaseColBits) | cl;
+        lg[it] = (uint32_t)((uint64_t)(b - b0) * gps + (uint64_t)sl * SUB + (cl >> kSubBits));
         if (!local) {
-            const uint64_t g = b0 * gps + lg[it];
+            const uint64_t g = (uint64_t)b0 * gps + lg[it];
             if (SCATTER) {
                 const uint64_t p = atomicAdd(gctr + g, 1ull);
                 trc[p] = rc[it];
@@ -565,10 +604,10 @@ __global__ __launch_bounds__(kPlanThreads) void k_xphase_bin(uint64_t nnz, uint6
         const uint32_t n = s_cnt[j];
         if (n == 0) continue;
         if (SCATTER) {
-            s_base[j] = atomicAdd(gctr + b0 * gps + j, (unsigned long long)n);
+            s_base[j] = atomicAdd(gctr + (uint64_t)b0 * gps + j, (unsigned long long)n);
             s_cnt[j] = 0;  // now the running rank inside the reserved run
         } else {
-            atomicAdd(gctr + b0 * gps + j, (unsigned long long)n);
+            atomicAdd(gctr + (uint64_t)b0 * gps + j, (unsigned long long)n);
         }
     }
     if (!SCATTER) return;
@@ -1022,8 +1061,8 @@ void ensure_xphase(spira_dcsr *A)
         k_xphase_chunk_rows<<<grid_for(n, kThreads), kThreads, 0, st>>>(A->off, n, A->nnz, tab);
         SPIRA_HIP(hipGetLastError());
         k_xphase_bin<false><<<(unsigned)chunks, kPlanThreads, 0, st>>>(
-            A->nnz, n, (uint32_t)rb, (uint32_t)S, (uint32_t)SUB, (uint32_t)width, A->off, A->col, A->val, tab,
-            (unsigned long long *)gsub, nullptr, nullptr);
+            A->nnz, n, (uint32_t)rb, make_udiv32((uint32_t)rb), (uint32_t)S, (uint32_t)SUB, (uint32_t)width,
+            make_udiv32((uint32_t)width), A->off, A->col, A->val, tab, (unsigned long long *)gsub, nullptr, nullptr);
         SPIRA_HIP(hipGetLastError());
     }
     scan_excl_u64(dev, gsub, gsub, GS);  // sub-group starts; gsub[GS] = nnz
@@ -1031,9 +1070,9 @@ void ensure_xphase(spira_dcsr *A)
     SPIRA_HIP(hipGetLastError());
     if (chunks > 0) {
         SPIRA_HIP(hipMemcpyAsync(cur, gsub, GS * sizeof(uint64_t), hipMemcpyDeviceToDevice, st));
-        k_xphase_bin<true><<<(unsigned)chunks, kPlanThreads, 0, st>>>(A->nnz, n, (uint32_t)rb, (uint32_t)S,
-                                                                       (uint32_t)SUB, (uint32_t)width, A->off,
-                                                                       A->col, A->val, tab, cur, trc, tval);
+        k_xphase_bin<true><<<(unsigned)chunks, kPlanThreads, 0, st>>>(
+            A->nnz, n, (uint32_t)rb, make_udiv32((uint32_t)rb), (uint32_t)S, (uint32_t)SUB, (uint32_t)width,
+            make_udiv32((uint32_t)width), A->off, A->col, A->val, tab, cur, trc, tval);
         SPIRA_HIP(hipGetLastError());
         if (A->nnz >= (1ull << 32)) {
             // a sub-group's entry count is held in 32 bits by the line pass
