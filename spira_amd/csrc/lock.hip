@@ -394,6 +394,11 @@ __device__ __forceinline__ row_range table_rows(const uint64_t *__restrict__ tab
 // ---------------------------------------------------------------------------
 // 3. flags of deduped entries
 // ---------------------------------------------------------------------------
+// One thread per kItems staged slots.  The row of each slot: the non-empty
+// rows of the block mark their first slot with their index in the block's row
+// range and a running maximum spreads the marks (blocks touching more than
+// kRowCache rows search sptr in global memory).  Index arithmetic inside a
+// row is 32-bit (slot counts and old-row lengths are below 2^32).
 __global__ __launch_bounds__(kThreads) void k_ded_flags(uint64_t n, uint64_t m, const uint64_t *__restrict__ sptr,
                                                          const uint32_t *__restrict__ dcnt,
                                                          const uint32_t *__restrict__ dcol,
@@ -406,44 +411,92 @@ __global__ __launch_bounds__(kThreads) void k_ded_flags(uint64_t n, uint64_t m, 
                                                          const uint32_t *__restrict__ ctl,
                                                          uint64_t *__restrict__ chunk_sums)
 {
-    __shared__ uint64_t s_rows[kRowCache + 1];
+    static_assert(kRowCache <= 65536, "16-bit local rows");
+    __shared__ uint16_t s_rid[kPerBlock];  // local row of each slot of the block
+    __shared__ uint32_t s_w[kThreads / kWave];
     const uint64_t first = (uint64_t)blockIdx.x * kPerBlock;
     if (first >= m || ctl[CTL_ERR]) return;
-    // slot data first: these loads are in flight while the row cache fills
+    const uint32_t nq = (uint32_t)(min<uint64_t>(first + kPerBlock, m) - first);
+    // slot data first: these loads are in flight while the row marks fill
     uint32_t cs[kItems];
     double vs[kItems];
 #pragma unroll
     for (int it = 0; it < kItems; ++it) {
-        const uint64_t s = first + (uint64_t)it * kThreads + threadIdx.x;
-        cs[it] = s < m ? dcol[s] : 0u;
-        vs[it] = s < m ? dval[s] : 0.0;
+        const uint32_t q = (uint32_t)it * kThreads + threadIdx.x;
+        cs[it] = q < nq ? dcol[first + q] : 0u;
+        vs[it] = q < nq ? dval[first + q] : 0.0;
     }
-    const row_cache rc = make_row_cache(table_rows(stab), sptr, s_rows);
+    const row_range rr = table_rows(stab);
+    const uint64_t nr = rr.hi - rr.lo + 1;
+    const bool cached = nr <= kRowCache;  // the same for every thread of the block
+    if (cached) {
+        for (uint32_t q = threadIdx.x; q < nq; q += kThreads) s_rid[q] = 0;  // row lo owns slot `first`
+        __syncthreads();
+        // rows after lo start after `first`; a non-empty one marks its first slot
+        for (uint32_t j = threadIdx.x + 1; j < nr; j += kThreads) {
+            const uint64_t a = sptr[rr.lo + j];
+            if (a < first + nq && sptr[rr.lo + j + 1] > a) s_rid[a - first] = (uint16_t)j;
+        }
+        __syncthreads();
+        // running maximum over the kItems consecutive slots of each thread, then across threads
+        const uint32_t a0 = threadIdx.x * kItems;
+        uint32_t v[kItems], mx = 0;
+#pragma unroll
+        for (int i = 0; i < kItems; ++i) {
+            v[i] = a0 + i < nq ? s_rid[a0 + i] : 0u;
+            mx = max(mx, v[i]);
+            v[i] = mx;
+        }
+        const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x >> 6;
+        uint32_t inc = mx;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            const uint32_t t = __shfl_up(inc, d, kWave);
+            if (lane >= d) inc = max(inc, t);
+        }
+        if (lane == kWave - 1) s_w[wid] = inc;
+        __syncthreads();
+        uint32_t ex = __shfl_up(inc, 1, kWave);
+        if (lane == 0) ex = 0;
+        for (int w = 0; w < wid; ++w) ex = max(ex, s_w[w]);
+#pragma unroll
+        for (int i = 0; i < kItems; ++i)
+            if (a0 + i < nq) s_rid[a0 + i] = (uint16_t)max(v[i], ex);
+        __syncthreads();
+    }
     uint64_t fsum = 0;  // this thread's flags, for the block's chunk sum
 #pragma unroll
     for (int it = 0; it < kItems; ++it) {
-        const uint64_t s = first + (uint64_t)it * kThreads + threadIdx.x;
-        if (s >= m) break;
-        const uint64_t r = rc.find(s);
-        const uint64_t j = s - rc.at(r);
+        const uint32_t q = (uint32_t)it * kThreads + threadIdx.x;
+        if (q >= nq) break;
+        const uint64_t s = first + q;
+        const uint64_t r = cached ? rr.lo + s_rid[q] : row_of(sptr, rr.lo, rr.hi + 1, s);
+        const uint32_t j = (uint32_t)s - (uint32_t)sptr[r];  // slot in the row (< 2^32)
         uint64_t f = 0;
-        uint32_t q = 0;
+        uint32_t qp = 0;
         if (j < dcnt[r]) {
             const bool d = merge ? row_dirty(dirty, sptr, r) : true;
             const uint32_t c = cs[it];
             const bool nz = d && !is_zero_f64(vs[it]);
             bool match = false;
             if (merge && d) {
-                const uint64_t ob = old_off[r], oe = old_off[r + 1];
-                const uint64_t p = lower_bound_u32(old_col, ob, oe, c);
-                match = p < oe && old_col[p] == c;
-                q = (uint32_t)(p - ob);
+                const uint64_t ob = old_off[r];
+                const uint32_t *row = old_col + ob;
+                uint32_t lo = 0, hi = (uint32_t)(old_off[r + 1] - ob);  // lower bound of c in the old row
+                const uint32_t len = hi;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (row[mid] < c) lo = mid + 1;
+                    else hi = mid;
+                }
+                match = lo < len && row[lo] == c;
+                qp = lo;
             }
             f = ((uint64_t)(match ? 1u : 0u) << 32) | (nz ? 1u : 0u);
         }
         flags[s] = f;
         fsum += f;
-        if (merge) qpos[s] = q;
+        if (merge) qpos[s] = qp;
     }
     block_sum_to(fsum, chunk_sums + blockIdx.x);  // the scan's chunk of kPerBlock flags
 }
