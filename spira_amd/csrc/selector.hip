@@ -25,7 +25,7 @@ constexpr int kThreads = 256;
 constexpr int kFeat = SPIRA_FEATURES;
 constexpr int kStat = 16;  // accumulated statistics
 
-constexpr unsigned kFeatBlocks = 2048;  // partial-sum blocks of the feature pass (grid-stride over rows)
+constexpr unsigned kFeatBlocks = 512;  // partial-sum blocks of the feature pass (grid-stride over rows)
 
 // stats: 0 sum len, 1 sum len^2, 2 max len, 3..10 bins, 11 sum spread, 12 sum band, 13 max band.
 // Every statistic is a sum or max of integers below 2^53, so it is exact in
@@ -83,14 +83,16 @@ __global__ __launch_bounds__(kThreads) void k_feature_stats(const uint64_t *__re
 // One block: the partials of nb blocks folded into the statistics, then the
 // features (the list at the top of this file; slot 17 left 0) into feat, and
 // the statistics into stats (kStat doubles) when it is not null.
-__global__ __launch_bounds__(kThreads) void k_feature_fold(const double *__restrict__ part, uint32_t nb,
-                                                            uint64_t n_rows, uint64_t nnz, uint64_t n_cols,
-                                                            float *__restrict__ feat, double *__restrict__ stats)
+constexpr int kFoldThreads = 1024;
+
+__global__ __launch_bounds__(kFoldThreads) void k_feature_fold(const double *__restrict__ part, uint32_t nb,
+                                                                uint64_t n_rows, uint64_t nnz, uint64_t n_cols,
+                                                                float *__restrict__ feat, double *__restrict__ stats)
 {
-    __shared__ double s[kThreads / 64][kStat];
+    __shared__ double s[kFoldThreads / 64][kStat];
     __shared__ double h[kStat];
     double loc[kStat] = {};
-    for (uint32_t j = threadIdx.x; j < nb; j += kThreads) {  // one block's partials: kStat contiguous words
+    for (uint32_t j = threadIdx.x; j < nb; j += kFoldThreads) {  // one block's partials: kStat contiguous words
         const double *pj = part + (uint64_t)j * kStat;
 #pragma unroll
         for (int i = 0; i < kStat; ++i) loc[i] = stat_is_max(i) ? fmax(loc[i], pj[i]) : loc[i] + pj[i];
@@ -108,28 +110,40 @@ __global__ __launch_bounds__(kThreads) void k_feature_fold(const double *__restr
     if (threadIdx.x < kStat) {
         const int i = threadIdx.x;
         double v = s[0][i];
-        for (int w = 1; w < kThreads / 64; ++w) v = stat_is_max(i) ? fmax(v, s[w][i]) : v + s[w][i];
+        for (int w = 1; w < kFoldThreads / 64; ++w) v = stat_is_max(i) ? fmax(v, s[w][i]) : v + s[w][i];
         h[i] = v;
         if (stats) stats[i] = v;
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
+    // one thread per feature
+    const int t = threadIdx.x;
+    if (t >= kFeat) return;
     const double n = (double)(n_rows ? n_rows : 1), nc = (double)(n_cols ? n_cols : 1);
     const double mean = h[0] / n;
-    const double var = h[1] / n - mean * mean;
-    const double cv = mean > 0 ? sqrt(var > 0 ? var : 0.0) / mean : 0.0;
-    const double nonempty = n - h[3] > 0 ? n - h[3] : 1.0;
-    feat[0] = (float)(log2((double)n_rows + 1) / 20.0);
-    feat[1] = (float)(log2((double)nnz + 1) / 30.0);
-    feat[2] = (float)(log2(mean + 1) / 10.0);
-    feat[3] = (float)(fmin(cv, 4.0) / 4.0);
-    feat[4] = (float)(log2(h[2] + 1) / 16.0);
-    for (int b = 0; b < 8; ++b) feat[5 + b] = (float)(h[3 + b] / n);
-    feat[13] = (float)(h[11] / nonempty / nc);
-    feat[14] = (float)(h[12] / nonempty / nc);
-    feat[15] = (float)(h[13] / nc);
-    feat[16] = (float)(log2((double)n_cols + 1) / 20.0);
-    feat[17] = 0.0f;
+    double f = 0.0;
+    if (t == 0) {
+        f = log2((double)n_rows + 1) / 20.0;
+    } else if (t == 1) {
+        f = log2((double)nnz + 1) / 30.0;
+    } else if (t == 2) {
+        f = log2(mean + 1) / 10.0;
+    } else if (t == 3) {
+        const double var = h[1] / n - mean * mean;
+        const double cv = mean > 0 ? sqrt(var > 0 ? var : 0.0) / mean : 0.0;
+        f = fmin(cv, 4.0) / 4.0;
+    } else if (t == 4) {
+        f = log2(h[2] + 1) / 16.0;
+    } else if (t <= 12) {
+        f = h[3 + (t - 5)] / n;
+    } else if (t <= 14) {
+        const double nonempty = n - h[3] > 0 ? n - h[3] : 1.0;
+        f = h[t == 13 ? 11 : 12] / nonempty / nc;
+    } else if (t == 15) {
+        f = h[13] / nc;
+    } else if (t == 16) {
+        f = log2((double)n_cols + 1) / 20.0;
+    }
+    feat[t] = (float)f;  // slot 17 stays 0
 }
 
 // ---- MLP kernels ----------------------------------------------------------
@@ -358,7 +372,7 @@ void queue_features(spira_dcsr *A, float *feat, double *stats = nullptr)
         k_feature_stats<<<nb, kThreads, 0, dev->stream>>>(A->off, A->col, A->n_rows, part);
         SPIRA_HIP(hipGetLastError());
     }
-    k_feature_fold<<<1, kThreads, 0, dev->stream>>>(part, nb, A->n_rows, A->nnz, A->n_cols, feat, stats);
+    k_feature_fold<<<1, kFoldThreads, 0, dev->stream>>>(part, nb, A->n_rows, A->nnz, A->n_cols, feat, stats);
     SPIRA_HIP(hipGetLastError());
 }
 
