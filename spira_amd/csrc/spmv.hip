@@ -346,27 +346,68 @@ __global__ void k_phase_sync_zero(uint32_t *sync)
     if (threadIdx.x < 8 * kPhaseSyncStride) sync[threadIdx.x] = 0u;
 }
 
-// Values of the XPHASE copy, read as FP64: stored FP64, FP32, BF16 or split
-// BF16 (hi + lo, summed exactly in FP64).
+// Values of the XPHASE copy: stored FP64, FP32, BF16 or split BF16 (hi + lo,
+// summed exactly in FP64).  load() reads the stored form and wide() turns it
+// into FP64 where the product is formed, after the step's gathers are issued.
 struct ph_f64 {
     const double *v;
-    __device__ __forceinline__ double operator()(uint64_t q) const { return ld_stream(v + q); }
+    using raw = double;
+    __device__ __forceinline__ raw load(uint64_t q) const { return ld_stream(v + q); }
+    __device__ __forceinline__ static double wide(raw r) { return r; }
 };
 struct ph_f32 {
     const float *v;
-    __device__ __forceinline__ double operator()(uint64_t q) const { return (double)ld_stream(v + q); }
+    using raw = float;
+    __device__ __forceinline__ raw load(uint64_t q) const { return ld_stream(v + q); }
+    __device__ __forceinline__ static double wide(raw r) { return (double)r; }
 };
 struct ph_bf16 {
     const uint16_t *v;
-    __device__ __forceinline__ double operator()(uint64_t q) const { return (double)bf16_to_f32(ld_stream(v + q)); }
+    using raw = uint16_t;
+    __device__ __forceinline__ raw load(uint64_t q) const { return ld_stream(v + q); }
+    __device__ __forceinline__ static double wide(raw r) { return (double)bf16_to_f32(r); }
 };
 struct ph_split {
     const uint16_t *hi, *lo;
-    __device__ __forceinline__ double operator()(uint64_t q) const
+    using raw = uint32_t;  // hi in the low half, lo in the high half
+    __device__ __forceinline__ raw load(uint64_t q) const
     {
-        return (double)bf16_to_f32(ld_stream(hi + q)) + (double)bf16_to_f32(ld_stream(lo + q));
+        return (uint32_t)ld_stream(hi + q) | ((uint32_t)ld_stream(lo + q) << 16);
+    }
+    __device__ __forceinline__ static double wide(raw r)
+    {
+        return (double)bf16_to_f32((uint16_t)r) + (double)bf16_to_f32((uint16_t)(r >> 16));
     }
 };
+
+// One step of U entries per thread from entry k on, each slot guarded
+// against the group's end e1.
+template <int U, class Vals, class XT>
+__device__ __forceinline__ void xphase_guarded_step(uint64_t k, uint64_t e1, uint32_t cbase,
+                                                    const uint32_t *__restrict__ prc, Vals pval,
+                                                    const XT *__restrict__ x, double *s_y)
+{
+    uint32_t rr[U], cc[U];
+    typename Vals::raw vr[U];
+    double xx[U];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint64_t q = k + (uint64_t)u * kPhaseThreads;
+        ok[u] = q < e1;
+        const uint32_t rc = ok[u] ? ld_stream(prc + q) : 0u;  // a masked lane reads x[cbase], adds nothing
+        rr[u] = rc >> kPhaseColBits;
+        cc[u] = cbase + (rc & ((1u << kPhaseColBits) - 1u));
+        vr[u] = ok[u] ? pval.load(q) : typename Vals::raw{};
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) xx[u] = (double)x[cc[u]];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (ok[u])
+            __hip_atomic_fetch_add(&s_y[rr[u]], Vals::wide(vr[u]) * xx[u], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 
 // Products are formed and summed in FP64 whatever the stored precision (a
 // FP32 value times a FP32 x is exact in FP64), and y is written as YT.
@@ -382,7 +423,7 @@ __global__ __launch_bounds__(kPhaseThreads) void k_spmv_xphase(uint64_t n_rows, 
                                                                 uint32_t width, const uint32_t *__restrict__ prc,
                                                                 Vals pval, const XT *__restrict__ x,
                                                                 YT *__restrict__ y, uint32_t *__restrict__ sync,
-                                                                uint32_t spin_max)
+                                                                uint32_t spin_max, uint32_t full_steps)
 {
     __shared__ double s_y[kPhaseRows];
     __shared__ uint64_t s_goff[kPhaseMaxSlices + 1];  // this block's group starts
@@ -408,26 +449,40 @@ __global__ __launch_bounds__(kPhaseThreads) void k_spmv_xphase(uint64_t n_rows, 
         const uint32_t s = first + t < S ? first + t : first + t - S;
         const uint64_t e0 = s_goff[s], e1 = s_goff[s + 1];
         const uint32_t cbase = s * width;
-        for (uint64_t k = e0 + threadIdx.x; k < e1; k += (uint64_t)U * kPhaseThreads) {
-            uint32_t rr[U], cc[U];
-            double vv[U], xx[U];
-            bool ok[U];
+        // Reduced value types with full_steps: full steps (every slot of every
+        // thread inside the group) load without branches, so every load of
+        // the step is in flight before the first wait, and one guarded step
+        // takes the group's remainder.  Otherwise (FP64 values, or
+        // full_steps clear) every step guards its slots.  Each form is kept
+        // where it measured fastest; the compiler schedules the guarded step
+        // differently inside a loop and as a single step, hence both.
+        constexpr uint64_t kStep = (uint64_t)U * kPhaseThreads;
+        if constexpr (std::is_same_v<Vals, ph_f64>) {
+            for (uint64_t k = e0 + threadIdx.x; k < e1; k += kStep) xphase_guarded_step<U>(k, e1, cbase, prc, pval, x, s_y);
+        } else if (full_steps) {
+            uint64_t kb = e0;  // the block's step base
+            for (; kb + kStep <= e1; kb += kStep) {
+                const uint64_t k = kb + threadIdx.x;
+                uint32_t rc[U];
+                typename Vals::raw vr[U];
+                XT xv[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint64_t q = k + (uint64_t)u * kPhaseThreads;
-                ok[u] = q < e1;
-                const uint32_t rc = ok[u] ? ld_stream(prc + q) : 0u;  // a masked lane reads x[cbase], adds nothing
-                rr[u] = rc >> kPhaseColBits;
-                cc[u] = cbase + (rc & ((1u << kPhaseColBits) - 1u));
-                vv[u] = ok[u] ? pval(q) : 0.0;
+                for (int u = 0; u < U; ++u) {
+                    rc[u] = ld_stream(prc + k + (uint64_t)u * kPhaseThreads);
+                    vr[u] = pval.load(k + (uint64_t)u * kPhaseThreads);
+                }
+                __builtin_amdgcn_sched_barrier(0);  // all stream loads issued before the first wait
+#pragma unroll
+                for (int u = 0; u < U; ++u) xv[u] = x[cbase + (rc[u] & ((1u << kPhaseColBits) - 1u))];
+                __builtin_amdgcn_sched_barrier(0);  // all gathers issued before the first add
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    __hip_atomic_fetch_add(&s_y[rc[u] >> kPhaseColBits], Vals::wide(vr[u]) * (double)xv[u],
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
-#pragma unroll
-            for (int u = 0; u < U; ++u) xx[u] = (double)x[cc[u]];
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-                if (ok[u])
-                    __hip_atomic_fetch_add(&s_y[rr[u]], vv[u] * xx[u], __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (kb < e1) xphase_guarded_step<U>(kb + threadIdx.x, e1, cbase, prc, pval, x, s_y);
+        } else {
+            for (uint64_t k = e0 + threadIdx.x; k < e1; k += kStep) xphase_guarded_step<U>(k, e1, cbase, prc, pval, x, s_y);
         }
         if (ctr) {
             __syncthreads();  // the whole block is done with phase t
@@ -1113,6 +1168,23 @@ void ensure_xphase(spira_dcsr *A)
 // then make the blocks of an XCD take turns).  Only when the whole grid is
 // resident at once (one block per CU).
 // ctr: the 8 * kPhaseSyncStride pacing counters (zeroed here by a kernel).
+// Branch-free full steps for the reduced value types (the FP64 kernels
+// ignore the flag).  The paced kernel (20 entries per thread per step) always
+// takes them.  The unpaced one (6 per step) does unless the longest row holds
+// 1/32 or more of an average row block's entries: that row's entries then
+// meet in most wavefronts of the column-sorted groups (two or more lanes of a
+// wavefront adding into one LDS word), and such matrices ran faster with
+// every step guarded (FP32 values, rows of up to 50,000 entries: 868 against
+// 911 us, while rows clipped at 1,024 ran 406 against 369 us and the
+// Laplacian 64.4 against 39.3 us).  Without row statistics: guarded.
+bool xphase_full_steps(const spira_dcsr *A, bool paced)
+{
+    const spira_plan &P = A->plan;
+    if (paced) return true;
+    if (!P.stats_valid || P.ph_nb == 0) return false;
+    return P.max_len * 32 < A->nnz / P.ph_nb + 1;
+}
+
 template <int U, bool PACE, class Vals = ph_f64, class XT = double, class YT = double>
 void launch_xphase(spira_dcsr *A, const XT *x, YT *y, uint32_t *ctr, Vals vals = Vals{})
 {
@@ -1128,7 +1200,7 @@ void launch_xphase(spira_dcsr *A, const XT *x, YT *y, uint32_t *ctr, Vals vals =
     }
     k_spmv_xphase<U, Vals, XT, YT><<<grid, kPhaseThreads, 0, A->dev->stream>>>(
         A->n_rows, P.ph_RB, P.ph_S, P.ph_nb, P.ph_order, P.ph_goff, P.ph_width, P.ph_rc, vals, x, y, sync,
-        A->dev->phase_spin_max);
+        A->dev->phase_spin_max, xphase_full_steps(A, PACE) ? 1u : 0u);
 }
 
 __global__ void k_ph_to_f32(const double *__restrict__ v, float *__restrict__ o, uint64_t n)
@@ -1364,6 +1436,7 @@ void spmv_launch_planned(spira_dcsr *A, const double *x, double *y, int variant,
 void spmv_xphase_reduced(spira_dcsr *A, int storage, const float *x, double *y, bool paced)
 {
     if (A->n_rows == 0) return;
+    ensure_stats(A);
     ensure_xphase_vals(A, storage);
     const spira_plan &P = A->plan;
     uint32_t *ctr = P.ph_sync;
@@ -1383,6 +1456,7 @@ void spmv_xphase_reduced(spira_dcsr *A, int storage, const float *x, double *y, 
 void spmv_xphase_f32(spira_dcsr *A, const float *x, float *y, bool paced)
 {
     if (A->n_rows == 0) return;
+    ensure_stats(A);
     ensure_xphase_vals(A, SPIRA_VAL_F32);
     const spira_plan &P = A->plan;
     if (paced) launch_xphase<20, true>(A, x, y, P.ph_sync, ph_f32{P.ph_v32});
