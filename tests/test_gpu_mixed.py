@@ -79,6 +79,69 @@ def test_mixed_xphase_fp32_exact_values(dev):
         assert oracle.normwise_rel_err(y, y_ref) <= 1e-14, name
 
 
+def _one_group(n_rows, n_cols, nnz, seed, long_row=0):
+    """n_rows x n_cols CSR with exactly nnz distinct entries at random places
+    (plus, with long_row > 0, row 0 filled with long_row leading columns), FP32-
+    exact values: with 16,384 rows or fewer and at most 2^18 columns it is one
+    (row block, slice) group of the XPHASE plan."""
+    rng = np.random.default_rng(seed)
+    keys = rng.choice(n_rows * n_cols, size=nnz, replace=False).astype(np.int64)
+    if long_row:
+        keys = np.union1d(keys, np.arange(long_row, dtype=np.int64))
+    keys.sort()
+    rows, cols = keys // n_cols, (keys % n_cols).astype(np.uint32)
+    off = np.zeros(n_rows + 1, np.uint64)
+    np.cumsum(np.bincount(rows, minlength=n_rows), out=off[1:])
+    val = rng.standard_normal(keys.size).astype(np.float32).astype(np.float64)
+    return off, cols, val
+
+
+# entries per thread per step: 6 (XPHASE) and 20 (XPHASE_PACED), 1024 threads
+STEP_SIZES = [1, 6143, 6144, 6145, 12288, 12289, 20479, 20480, 20481, 40960, 40961]
+
+
+@pytest.mark.parametrize("nnz", STEP_SIZES)
+def test_xphase_reduced_steps_at_group_boundaries(dev, nnz):
+    """The reduced-precision XPHASE kernels take a group's full steps without
+    branches and its remainder in one guarded step: groups of exactly one or
+    two steps, and one entry either side, are exact with FP32-exact data
+    (FP64 products and sums: <= 1e-14 normwise against the FP64 oracle; the
+    FP32 result within FP32 rounding of it)."""
+    n_rows, n_cols = 16384, 50000
+    off, col, val = _one_group(n_rows, n_cols, nnz, seed=nnz)
+    A = spira_amd.DeviceCSR(dev, n_rows, n_cols)
+    A.upload(off, col, val)
+    x = np.random.default_rng(nnz + 1).uniform(-1, 1, n_cols).astype(np.float32)
+    y_ref = oracle.spmv_f64(off, col, val, x.astype(np.float64))
+    for name in ("xphase", "xphase_paced"):
+        v = spira_amd.VARIANTS[name]
+        y, _ = A.spmv_mixed(x.astype(np.float64), storage="f32", variant=v)
+        assert oracle.normwise_rel_err(y, y_ref) <= 1e-14, (name, nnz)
+        y32 = A.spmv_f32(x, variant=v)
+        assert np.array_equal(y32, y_ref.astype(np.float32)) or \
+            oracle.normwise_rel_err(y32.astype(np.float64), y_ref) <= 1e-7, (name, nnz)
+
+
+@pytest.mark.parametrize("long_row", [0, 4000])
+def test_xphase_reduced_long_row_and_second_block(dev, long_row):
+    """A row holding more than 1/32 of an average row block's entries makes
+    the unpaced reduced kernel guard every step; both forms, and a second row
+    block whose only group is a remainder, give the exact FP64 sums."""
+    n_rows, n_cols = 20000, 30000  # two row blocks: 16,384 rows and 3,616 rows
+    off, col, val = _one_group(n_rows, n_cols, 50000, seed=7 + long_row, long_row=long_row)
+    A = spira_amd.DeviceCSR(dev, n_rows, n_cols)
+    A.upload(off, col, val)
+    x = np.random.default_rng(8).uniform(-1, 1, n_cols).astype(np.float32).astype(np.float64)
+    y_ref = oracle.spmv_f64(off, col, val, x)
+    for name in ("xphase", "xphase_paced"):
+        y, _ = A.spmv_mixed(x, storage="f32", variant=spira_amd.VARIANTS[name])
+        assert oracle.normwise_rel_err(y, y_ref) <= 1e-14, name
+        for st in ("bf16", "split_bf16"):  # the other stored forms take the same steps
+            yb, _ = A.spmv_mixed(x, storage=st, variant=spira_amd.VARIANTS[name])
+            yb_row, _ = A.spmv_mixed(x, storage=st, variant=spira_amd.VARIANTS["vec8"])
+            assert oracle.normwise_rel_err(yb, y_ref) <= 1.5 * oracle.normwise_rel_err(yb_row, y_ref) + 1e-12
+
+
 def test_mixed_without_recompute_still_meets_fp32_bound(powerlaw):
     A, csr, x, y_ref = powerlaw
     y, nrec = A.spmv_mixed(x, storage="f32", recompute_ratio=0.0)
