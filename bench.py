@@ -778,6 +778,27 @@ def main():
             detail[name]["auto_fraction_of_autotuned"] = round(
                 min(t for t in at.values() if t >= 0) / at[spira_amd.VARIANT_NAMES[av]], 4)
 
+    # The reference's float instantiation (serial/spmv.hpp:90-135) on the same
+    # matrices: spira_spmv_f32 with AUTO's choice, A's values and x rounded to
+    # FP32, FP32 y (row sums in FP64), against the FP64 result above.
+    for name in names:
+        A, n = dA[name], info[name]["n_rows"]
+        with torch.cuda.stream(stream):
+            x32 = xs[name].float()
+            y32 = torch.empty(n, dtype=torch.float32, device="cuda")
+        stream.synchronize()
+        A.spmv_f32(x32, y32, device=True, sync=True)  # builds the FP32 copy of the values
+        t32 = _time_calls(torch, stream, lambda: A.spmv_f32(x32, y32, device=True, sync=False), 50)
+        A.spmv(xs[name], ys[name], device=True, sync=True)
+        b32 = W.spmv_bytes(n, n, info[name]["nnz"], s_val=4, s_col=4, s_ptr=4, s_x=4, s_y=4)
+        detail[name]["float_instantiation"] = {
+            "path": "spira_spmv_f32 (AUTO): FP32 values, x and y, FP64 row sums",
+            "ms": round(t32, 5), "GB_s": round(b32 / (t32 * 1e-3) / 1e9, 1), "bytes_per_call": b32,
+            "speedup_vs_fp64": round(detail[name]["ms"] / t32, 3),
+            "normwise_rel_err_vs_fp64": _nrm_err(torch, y32, ys[name])}
+        log(f"bench: {name}: float instantiation {detail[name]['float_instantiation']}")
+        del x32, y32
+
     lock_detail, lock_src = (None, None) if args.no_lock else run_lock(args, np, torch, spira_amd, W, dev, stream)
     tr_detail, tr_src = (None, None) if args.no_transpose else run_transpose(
         args, np, torch, spira_amd, W, dev, stream, None if lock_src is None else lock_src[:3])
