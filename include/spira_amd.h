@@ -179,7 +179,8 @@ int spira_spmv_f64(spira_dcsr *A, const double *x, uint64_t x_len, double *y,
  * more accurately than the reference's FP32 loop (compensated FP32 in the VEC
  * and STREAM kernels, FP64 in XPHASE) and rounded once to FP32: normwise
  * relative error <= 1e-6 against the FP64 sum of the same FP32 data.
- * variant: SPIRA_SPMV_AUTO (the FP64 AUTO choice mapped onto these kernels),
+ * variant: SPIRA_SPMV_AUTO (the FP64 AUTO choice mapped onto these kernels,
+ * XPHASE_PACED run as XPHASE),
  * VEC4..VEC32, STREAM1K, STREAM2K, XPHASE, XPHASE_PACED. */
 int spira_spmv_f32(spira_dcsr *A, const float *x, uint64_t x_len, float *y, uint64_t y_len, int variant,
                    uint32_t flags);
@@ -303,8 +304,9 @@ int spira_spgemm_f64(spira_dcsr *A, spira_dcsr *B, spira_dcsr *C);
  * XPHASE_PACED, which read the reduced values in the XPHASE plan's order and
  * sum each row in FP64 (an FP32 value times the FP32 x is exact in FP64; the
  * products are added in LDS).
- * AUTO follows the FP64 AUTO choice (row-major kernels when the recompute is
- * on).  *n_recomputed (may be NULL; reading it synchronises) receives the
+ * AUTO follows the FP64 AUTO choice, with XPHASE_PACED run as XPHASE (the
+ * unpaced kernel is the faster one for reduced values), and row-major kernels
+ * when the recompute is on.  *n_recomputed (may be NULL; reading it synchronises) receives the
  * number of recomputed rows.  Checks and flags as spira_spmv_f64. */
 int spira_spmv_mixed_f64(spira_dcsr *A, int storage, const double *x, uint64_t x_len, double *y, uint64_t y_len,
                          int variant, float recompute_ratio, uint32_t flags, uint64_t *n_recomputed);

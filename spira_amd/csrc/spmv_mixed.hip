@@ -326,13 +326,17 @@ bool row_major_variant(int variant)
 
 bool xphase_variant(int variant) { return variant == SPIRA_SPMV_XPHASE || variant == SPIRA_SPMV_XPHASE_PACED; }
 
-// The FP64 AUTO choice mapped onto the reduced-precision kernels (XPHASE
-// layouts kept; the XCD-slice copies go to the LDS row blocks; VEC widths
-// outside 4..32 to the nearest one).
+// The FP64 AUTO choice mapped onto the reduced-precision kernels (the XPHASE
+// layout kept, unpaced: with FP32 x a slice covers twice the columns, and the
+// unpaced kernel measured faster on every matrix tried -- random 4M rows 298
+// against 306 us, power-law 8.4M rows 867 against 1,075 us, the 2048^2
+// Laplacian 39.8 against 108 us; the XCD-slice copies go to the LDS row
+// blocks; VEC widths outside 4..32 to the nearest one).
 int reduced_auto(spira_dcsr *A)
 {
     const int v = spmv_resolve_auto(A);
-    if (row_major_variant(v) || xphase_variant(v)) return v;
+    if (xphase_variant(v)) return SPIRA_SPMV_XPHASE;
+    if (row_major_variant(v)) return v;
     if (v >= SPIRA_SPMV_XSLICE1 && v <= SPIRA_SPMV_XSTREAM) return SPIRA_SPMV_STREAM2K;
     return v < SPIRA_SPMV_VEC4 ? SPIRA_SPMV_VEC4 : SPIRA_SPMV_VEC32;
 }
