@@ -320,7 +320,7 @@ struct spira_plan {
     uint32_t *ph_order = nullptr; // row blocks by decreasing entry count
     float *ph_v32 = nullptr;      // the XPHASE values as FP32, BF16, split BF16 (built on demand)
     uint16_t *ph_b16 = nullptr;
-    uint16_t *ph_shi = nullptr, *ph_slo = nullptr;
+    uint32_t *ph_s32 = nullptr;   // split BF16: hi in the low half, lo in the high half
     uint32_t ph_S = 0, ph_RB = 0, ph_nb = 0, ph_width = 0;
     int32_t variant = -1;        // pinned AUTO choice (-1 = none)
     int32_t auto_variant = -1;   // AUTO's own choice once resolved (learned selector, else the rule)

@@ -368,12 +368,9 @@ struct ph_bf16 {
     __device__ __forceinline__ static double wide(raw r) { return (double)bf16_to_f32(r); }
 };
 struct ph_split {
-    const uint16_t *hi, *lo;
+    const uint32_t *v;     // one word per entry: one load instead of two
     using raw = uint32_t;  // hi in the low half, lo in the high half
-    __device__ __forceinline__ raw load(uint64_t q) const
-    {
-        return (uint32_t)ld_stream(hi + q) | ((uint32_t)ld_stream(lo + q) << 16);
-    }
+    __device__ __forceinline__ raw load(uint64_t q) const { return ld_stream(v + q); }
     __device__ __forceinline__ static double wide(raw r)
     {
         return (double)bf16_to_f32((uint16_t)r) + (double)bf16_to_f32((uint16_t)(r >> 16));
@@ -1215,14 +1212,13 @@ __global__ void k_ph_to_bf16(const double *__restrict__ v, uint16_t *__restrict_
     if (i < n) o[i] = f32_to_bf16((float)v[i]);
 }
 
-__global__ void k_ph_to_split(const double *__restrict__ v, uint16_t *__restrict__ hi, uint16_t *__restrict__ lo,
-                              uint64_t n)
+__global__ void k_ph_to_split(const double *__restrict__ v, uint32_t *__restrict__ o, uint64_t n)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint16_t h = f32_to_bf16((float)v[i]);
-    hi[i] = h;
-    lo[i] = f32_to_bf16((float)(v[i] - (double)bf16_to_f32(h)));
+    const uint16_t l = f32_to_bf16((float)(v[i] - (double)bf16_to_f32(h)));
+    o[i] = (uint32_t)h | ((uint32_t)l << 16);
 }
 
 // The XPHASE values in reduced precision (built once per locked matrix from
@@ -1242,12 +1238,10 @@ void ensure_xphase_vals(spira_dcsr *A, int storage)
         uint16_t *o = (uint16_t *)plan_alloc(A, n * sizeof(uint16_t));
         k_ph_to_bf16<<<g, kThreads, 0, st>>>(P.ph_val, o, A->nnz);
         P.ph_b16 = o;
-    } else if (storage == SPIRA_VAL_SPLIT_BF16 && !P.ph_shi) {
-        uint16_t *hi = (uint16_t *)plan_alloc(A, n * sizeof(uint16_t));
-        uint16_t *lo = (uint16_t *)plan_alloc(A, n * sizeof(uint16_t));
-        k_ph_to_split<<<g, kThreads, 0, st>>>(P.ph_val, hi, lo, A->nnz);
-        P.ph_shi = hi;
-        P.ph_slo = lo;
+    } else if (storage == SPIRA_VAL_SPLIT_BF16 && !P.ph_s32) {
+        uint32_t *o = (uint32_t *)plan_alloc(A, n * sizeof(uint32_t));
+        k_ph_to_split<<<g, kThreads, 0, st>>>(P.ph_val, o, A->nnz);
+        P.ph_s32 = o;
     }
     SPIRA_HIP(hipGetLastError());
 }
@@ -1447,8 +1441,8 @@ void spmv_xphase_reduced(spira_dcsr *A, int storage, const float *x, double *y, 
         if (paced) launch_xphase<20, true>(A, x, y, ctr, ph_bf16{P.ph_b16});
         else launch_xphase<6, false>(A, x, y, ctr, ph_bf16{P.ph_b16});
     } else {
-        if (paced) launch_xphase<20, true>(A, x, y, ctr, ph_split{P.ph_shi, P.ph_slo});
-        else launch_xphase<6, false>(A, x, y, ctr, ph_split{P.ph_shi, P.ph_slo});
+        if (paced) launch_xphase<20, true>(A, x, y, ctr, ph_split{P.ph_s32});
+        else launch_xphase<6, false>(A, x, y, ctr, ph_split{P.ph_s32});
     }
     SPIRA_HIP(hipGetLastError());
 }
