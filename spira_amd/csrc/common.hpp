@@ -317,6 +317,8 @@ struct spira_plan {
     uint32_t *ph_rc = nullptr;    // row inside its block << 18 | column inside its slice
     double *ph_val = nullptr;
     uint32_t *ph_sync = nullptr;  // phase-pacing counters, one per XCD
+    bool ph_sync_init = false;    // ph_sync zeroed (before this plan's first paced call)
+    uint32_t ph_epoch = 0;        // paced calls on ph_sync since it was zeroed
     uint32_t *ph_order = nullptr; // row blocks by decreasing entry count
     float *ph_v32 = nullptr;      // the XPHASE values as FP32, BF16, split BF16 (built on demand)
     uint16_t *ph_b16 = nullptr;

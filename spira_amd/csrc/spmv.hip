@@ -332,7 +332,8 @@ __device__ __forceinline__ bool phase_wait(const uint32_t *ctr, uint32_t target,
     if (threadIdx.x == 0) {
         uint32_t i = 0;
         for (; i < spin_max; ++i) {
-            if (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
+            // counts and targets are taken mod 2^32 (the counters run on across calls)
+            if ((int32_t)(__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) >= 0) break;
             __builtin_amdgcn_s_sleep(4);
         }
         s_gave_up = i == spin_max ? 1u : 0u;
@@ -420,7 +421,8 @@ __global__ __launch_bounds__(kPhaseThreads) void k_spmv_xphase(uint64_t n_rows, 
                                                                 uint32_t width, const uint32_t *__restrict__ prc,
                                                                 Vals pval, const XT *__restrict__ x,
                                                                 YT *__restrict__ y, uint32_t *__restrict__ sync,
-                                                                uint32_t spin_max, uint32_t full_steps)
+                                                                uint32_t spin_max, uint32_t full_steps,
+                                                                uint32_t epoch)
 {
     __shared__ double s_y[kPhaseRows];
     __shared__ uint64_t s_goff[kPhaseMaxSlices + 1];  // this block's group starts
@@ -431,6 +433,16 @@ __global__ __launch_bounds__(kPhaseThreads) void k_spmv_xphase(uint64_t n_rows, 
     uint32_t *ctr = sync ? sync + xcd * kPhaseSyncStride : nullptr;
     bool pace = ctr != nullptr;  // counting continues after a give-up; waiting does not
     const uint32_t passes = (nb + gridDim.x - 1) / gridDim.x;
+    // every call on these counters adds nbx * passes * S to this XCD's count:
+    // the count before this call (mod 2^32)
+    const uint32_t base = epoch * (nbx * passes * S);
+    if (pace) {  // a count behind base means the counters missed a call: pace nothing
+        __shared__ uint32_t s_behind;
+        if (threadIdx.x == 0)
+            s_behind = (int32_t)(__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - base) < 0;
+        __syncthreads();
+        if (s_behind) pace = false;
+    }
     for (uint32_t pass = 0; pass < passes; ++pass) {
     const uint32_t slot = blockIdx.x + pass * gridDim.x;
     const bool live = slot < nb;
@@ -442,7 +454,7 @@ __global__ __launch_bounds__(kPhaseThreads) void k_spmv_xphase(uint64_t n_rows, 
     __syncthreads();
     for (uint32_t t = 0; t < S; ++t) {
         const uint32_t T = pass * S + t;  // phase index across passes
-        if (pace && T >= 3 && phase_wait(ctr, nbx * (T - 2), spin_max)) pace = false;
+        if (pace && T >= 3 && phase_wait(ctr, base + nbx * (T - 2), spin_max)) pace = false;
         const uint32_t s = first + t < S ? first + t : first + t - S;
         const uint64_t e0 = s_goff[s], e1 = s_goff[s + 1];
         const uint32_t cbase = s * width;
@@ -1182,22 +1194,34 @@ bool xphase_full_steps(const spira_dcsr *A, bool paced)
     return P.max_len * 32 < A->nnz / P.ph_nb + 1;
 }
 
+// The pacing counters of a plan (ctr == the plan's ph_sync) are zeroed before
+// its first paced call only: each call adds a fixed count per XCD, so the
+// kernel offsets its targets by the plan's call number (epoch).  Counters
+// shared with other kernels (the allocation-free scratch) are zeroed before
+// every call (epoch 0).
 template <int U, bool PACE, class Vals = ph_f64, class XT = double, class YT = double>
 void launch_xphase(spira_dcsr *A, const XT *x, YT *y, uint32_t *ctr, Vals vals = Vals{})
 {
-    const spira_plan &P = A->plan;
+    spira_plan &P = A->plan;
     if constexpr (std::is_same_v<Vals, ph_f64>) vals.v = P.ph_val;
     uint32_t *sync = nullptr;
-    uint32_t grid = P.ph_nb;
+    uint32_t grid = P.ph_nb, epoch = 0;
     if (PACE && A->dev->n_cu > 0) {
         // one block per CU, all resident: more row blocks are taken in passes
         grid = std::min<uint32_t>(P.ph_nb, (uint32_t)A->dev->n_cu);
         sync = ctr;
-        k_phase_sync_zero<<<1, 8 * kPhaseSyncStride, 0, A->dev->stream>>>(sync);
+        const uint64_t per_call = (uint64_t)((grid + 7) / 8) * ((P.ph_nb + grid - 1) / grid) * P.ph_S;
+        const bool own = ctr == P.ph_sync && per_call < (1ull << 30);  // targets compared mod 2^32
+        if (!own || !P.ph_sync_init) {
+            k_phase_sync_zero<<<1, 8 * kPhaseSyncStride, 0, A->dev->stream>>>(sync);
+            P.ph_epoch = 0;
+            P.ph_sync_init = own;
+        }
+        if (own) epoch = P.ph_epoch++;
     }
     k_spmv_xphase<U, Vals, XT, YT><<<grid, kPhaseThreads, 0, A->dev->stream>>>(
         A->n_rows, P.ph_RB, P.ph_S, P.ph_nb, P.ph_order, P.ph_goff, P.ph_width, P.ph_rc, vals, x, y, sync,
-        A->dev->phase_spin_max, xphase_full_steps(A, PACE) ? 1u : 0u);
+        A->dev->phase_spin_max, xphase_full_steps(A, PACE) ? 1u : 0u, epoch);
 }
 
 __global__ void k_ph_to_f32(const double *__restrict__ v, float *__restrict__ o, uint64_t n)
