@@ -316,7 +316,9 @@ constexpr int kPhaseThreads = 1024;
 // has finished phase t - 3, so an XCD's blocks gather from at most three
 // slices at a time (they drift apart otherwise, and the XCD's L2 then holds
 // many slices' worth of x).  sync[c * kPhaseSyncStride] counts finished (block,
-// phase) pairs of XCD c; it is zeroed before the launch.  The wait only paces
+// phase) pairs of XCD c, mod 2^32, and runs on across calls: a call adds
+// nbx * passes * S to it, so the kernel offsets its targets by epoch (the
+// calls since the counters were zeroed) times that.  The wait only paces
 // and never decides a result: it gives up after a bounded number of sleeps
 // (spin_max loads), and the host passes sync only when every block of the
 // grid can be resident at once (one block per CU).  A block whose wait gives
