@@ -46,7 +46,9 @@ void spmv_launch_f64(spira_dcsr *A, const double *x, double *y, int variant, boo
 // order and x in FP32 (half the x lines to gather), products and row sums in
 // FP64: FP64 y (mixed precision), or FP32 y (the reference's float
 // instantiation, FP32 values).
+// x already rounded to FP32, or FP64 x rounded where the kernel gathers it
 void spmv_xphase_reduced(spira_dcsr *A, int storage, const float *x, double *y, bool paced);
+void spmv_xphase_reduced(spira_dcsr *A, int storage, const double *x, double *y, bool paced);
 void spmv_xphase_f32(spira_dcsr *A, const float *x, float *y, bool paced);
 // The allocation-free launch: true when A's plan already holds what `variant`
 // (AUTO resolved to the pinned or heuristic choice) needs; the bytes of

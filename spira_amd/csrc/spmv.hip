@@ -380,9 +380,18 @@ struct ph_split {
     }
 };
 
+// x as used by the products: RX rounds an FP64 x to FP32 first (the mixed
+// path's x, rounded where it is gathered instead of in a pass of its own).
+template <bool RX, class XT>
+__device__ __forceinline__ double x_used(XT v)
+{
+    if constexpr (RX) return (double)(float)v;
+    else return (double)v;
+}
+
 // One step of U entries per thread from entry k on, each slot guarded
 // against the group's end e1.
-template <int U, class Vals, class XT>
+template <int U, bool RX, class Vals, class XT>
 __device__ __forceinline__ void xphase_guarded_step(uint64_t k, uint64_t e1, uint32_t cbase,
                                                     const uint32_t *__restrict__ prc, Vals pval,
                                                     const XT *__restrict__ x, double *s_y)
@@ -401,7 +410,7 @@ __device__ __forceinline__ void xphase_guarded_step(uint64_t k, uint64_t e1, uin
         vr[u] = ok[u] ? pval.load(q) : typename Vals::raw{};
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) xx[u] = (double)x[cc[u]];
+    for (int u = 0; u < U; ++u) xx[u] = x_used<RX>(x[cc[u]]);
 #pragma unroll
     for (int u = 0; u < U; ++u)
         if (ok[u])
@@ -416,7 +425,7 @@ __device__ __forceinline__ void xphase_guarded_step(uint64_t k, uint64_t e1, uin
 // row blocks than CUs are paced too).  Every block runs every pass, so the
 // XCD's pacing count is the same in each; a block past the last row block
 // runs its phases empty and still counts them.
-template <int U, class Vals, class XT, class YT>
+template <int U, class Vals, class XT, class YT, bool RX = false>
 __global__ __launch_bounds__(kPhaseThreads) void k_spmv_xphase(uint64_t n_rows, uint32_t RB, uint32_t S,
                                                                 uint32_t nb, const uint32_t *__restrict__ order,
                                                                 const uint64_t *__restrict__ goff,
@@ -469,7 +478,7 @@ __global__ __launch_bounds__(kPhaseThreads) void k_spmv_xphase(uint64_t n_rows, 
         // differently inside a loop and as a single step, hence both.
         constexpr uint64_t kStep = (uint64_t)U * kPhaseThreads;
         if constexpr (std::is_same_v<Vals, ph_f64>) {
-            for (uint64_t k = e0 + threadIdx.x; k < e1; k += kStep) xphase_guarded_step<U>(k, e1, cbase, prc, pval, x, s_y);
+            for (uint64_t k = e0 + threadIdx.x; k < e1; k += kStep) xphase_guarded_step<U, RX>(k, e1, cbase, prc, pval, x, s_y);
         } else if (full_steps) {
             uint64_t kb = e0;  // the block's step base
             for (; kb + kStep <= e1; kb += kStep) {
@@ -488,12 +497,12 @@ __global__ __launch_bounds__(kPhaseThreads) void k_spmv_xphase(uint64_t n_rows, 
                 __builtin_amdgcn_sched_barrier(0);  // all gathers issued before the first add
 #pragma unroll
                 for (int u = 0; u < U; ++u)
-                    __hip_atomic_fetch_add(&s_y[rc[u] >> kPhaseColBits], Vals::wide(vr[u]) * (double)xv[u],
+                    __hip_atomic_fetch_add(&s_y[rc[u] >> kPhaseColBits], Vals::wide(vr[u]) * x_used<RX>(xv[u]),
                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
-            if (kb < e1) xphase_guarded_step<U>(kb + threadIdx.x, e1, cbase, prc, pval, x, s_y);
+            if (kb < e1) xphase_guarded_step<U, RX>(kb + threadIdx.x, e1, cbase, prc, pval, x, s_y);
         } else {
-            for (uint64_t k = e0 + threadIdx.x; k < e1; k += kStep) xphase_guarded_step<U>(k, e1, cbase, prc, pval, x, s_y);
+            for (uint64_t k = e0 + threadIdx.x; k < e1; k += kStep) xphase_guarded_step<U, RX>(k, e1, cbase, prc, pval, x, s_y);
         }
         if (ctr) {
             __syncthreads();  // the whole block is done with phase t
@@ -1201,7 +1210,7 @@ bool xphase_full_steps(const spira_dcsr *A, bool paced)
 // kernel offsets its targets by the plan's call number (epoch).  Counters
 // shared with other kernels (the allocation-free scratch) are zeroed before
 // every call (epoch 0).
-template <int U, bool PACE, class Vals = ph_f64, class XT = double, class YT = double>
+template <int U, bool PACE, class Vals = ph_f64, class XT = double, class YT = double, bool RX = false>
 void launch_xphase(spira_dcsr *A, const XT *x, YT *y, uint32_t *ctr, Vals vals = Vals{})
 {
     spira_plan &P = A->plan;
@@ -1221,7 +1230,7 @@ void launch_xphase(spira_dcsr *A, const XT *x, YT *y, uint32_t *ctr, Vals vals =
         }
         if (own) epoch = P.ph_epoch++;
     }
-    k_spmv_xphase<U, Vals, XT, YT><<<grid, kPhaseThreads, 0, A->dev->stream>>>(
+    k_spmv_xphase<U, Vals, XT, YT, RX><<<grid, kPhaseThreads, 0, A->dev->stream>>>(
         A->n_rows, P.ph_RB, P.ph_S, P.ph_nb, P.ph_order, P.ph_goff, P.ph_width, P.ph_rc, vals, x, y, sync,
         A->dev->phase_spin_max, xphase_full_steps(A, PACE) ? 1u : 0u, epoch);
 }
@@ -1453,24 +1462,41 @@ void spmv_launch_planned(spira_dcsr *A, const double *x, double *y, int variant,
     launch_planned(A, x, y, variant, (uint32_t *)scratch, (double *)scratch);
 }
 
-void spmv_xphase_reduced(spira_dcsr *A, int storage, const float *x, double *y, bool paced)
+namespace {
+
+// XT = float: x rounded to FP32 beforehand; XT = double: rounded where gathered.
+template <class XT>
+void xphase_reduced(spira_dcsr *A, int storage, const XT *x, double *y, bool paced)
 {
     if (A->n_rows == 0) return;
     ensure_stats(A);
     ensure_xphase_vals(A, storage);
     const spira_plan &P = A->plan;
     uint32_t *ctr = P.ph_sync;
+    constexpr bool RX = std::is_same_v<XT, double>;
     if (storage == SPIRA_VAL_F32) {
-        if (paced) launch_xphase<20, true>(A, x, y, ctr, ph_f32{P.ph_v32});
-        else launch_xphase<6, false>(A, x, y, ctr, ph_f32{P.ph_v32});
+        if (paced) launch_xphase<20, true, ph_f32, XT, double, RX>(A, x, y, ctr, ph_f32{P.ph_v32});
+        else launch_xphase<6, false, ph_f32, XT, double, RX>(A, x, y, ctr, ph_f32{P.ph_v32});
     } else if (storage == SPIRA_VAL_BF16) {
-        if (paced) launch_xphase<20, true>(A, x, y, ctr, ph_bf16{P.ph_b16});
-        else launch_xphase<6, false>(A, x, y, ctr, ph_bf16{P.ph_b16});
+        if (paced) launch_xphase<20, true, ph_bf16, XT, double, RX>(A, x, y, ctr, ph_bf16{P.ph_b16});
+        else launch_xphase<6, false, ph_bf16, XT, double, RX>(A, x, y, ctr, ph_bf16{P.ph_b16});
     } else {
-        if (paced) launch_xphase<20, true>(A, x, y, ctr, ph_split{P.ph_s32});
-        else launch_xphase<6, false>(A, x, y, ctr, ph_split{P.ph_s32});
+        if (paced) launch_xphase<20, true, ph_split, XT, double, RX>(A, x, y, ctr, ph_split{P.ph_s32});
+        else launch_xphase<6, false, ph_split, XT, double, RX>(A, x, y, ctr, ph_split{P.ph_s32});
     }
     SPIRA_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void spmv_xphase_reduced(spira_dcsr *A, int storage, const float *x, double *y, bool paced)
+{
+    xphase_reduced(A, storage, x, y, paced);
+}
+
+void spmv_xphase_reduced(spira_dcsr *A, int storage, const double *x, double *y, bool paced)
+{
+    xphase_reduced(A, storage, x, y, paced);
 }
 
 void spmv_xphase_f32(spira_dcsr *A, const float *x, float *y, bool paced)

@@ -3,7 +3,8 @@
 //
 // Values are stored as FP32 (4 B), BF16 (2 B) or split BF16 (hi + lo, 4 B),
 // copied once from A's FP64 values and dropped at the next lock.  x is
-// rounded to FP32 once per call.  Each row is accumulated with the Dot2
+// rounded to FP32 once per call (the XPHASE kernels of banded matrices round
+// it where they gather it instead).  Each row is accumulated with the Dot2
 // scheme of Ogita, Rump and Oishi ("Accurate sum and dot product", SIAM J. Sci.
 // Comput. 26(6), 2005): p = a*x, e = fma(a, x, -p) (the exact product error),
 // (s, q) = TwoSum(s, p) (Knuth), c += q + e, result s + c.  The same pass sums
@@ -378,11 +379,21 @@ void spmv_mixed_launch(spira_dcsr *A, int storage, const double *x, double *y, i
     if (xphase_variant(variant)) {
         if (ratio > 0.0f)
             fail(SPIRA_ERR_INVALID_ARGUMENT, "mixed spmv: the FP64 recompute needs a VEC or STREAM variant");
-        float *x32 = (float *)dev->ws.get(WS_X32, (A->n_cols ? A->n_cols : 1) * sizeof(float));
-        k_x_to_f32<<<grid_for(A->n_cols ? A->n_cols : 1, kThreads), kThreads, 0, st>>>(x, x32, A->n_cols,
-                                                                                         mixed_queue_count(A));
-        SPIRA_HIP(hipGetLastError());
-        spmv_xphase_reduced(A, storage, x32, y, variant == SPIRA_SPMV_XPHASE_PACED);
+        // A matrix that AUTO runs on unpaced XPHASE (a row block meets few
+        // slices) gathers FP64 x and rounds it there: no pass of its own
+        // (2048^2 Laplacian, FP32 values: 62.6 -> 46.9 us).  Scattered
+        // columns take x rounded to FP32 first: half the bytes behind each
+        // gathered line (random 4M rows: 313 against 410 us gathering FP64).
+        const bool paced = variant == SPIRA_SPMV_XPHASE_PACED;
+        if (spmv_resolve_auto(A) == SPIRA_SPMV_XPHASE) {
+            spmv_xphase_reduced(A, storage, x, y, paced);
+        } else {
+            float *x32 = (float *)dev->ws.get(WS_X32, (A->n_cols ? A->n_cols : 1) * sizeof(float));
+            k_x_to_f32<<<grid_for(A->n_cols ? A->n_cols : 1, kThreads), kThreads, 0, st>>>(x, x32, A->n_cols,
+                                                                                             mixed_queue_count(A));
+            SPIRA_HIP(hipGetLastError());
+            spmv_xphase_reduced(A, storage, x32, y, paced);
+        }
         return;
     }
     ensure_storage(A, storage);
