@@ -310,6 +310,52 @@ def test_reproducible_mode_and_xphase_call_to_call(dev):
         _check(a, off, col, val, xh)
 
 
+def test_xphase_paced_counters_shared_across_calls_and_kernels(dev):
+    """A plan's pacing counters run on across calls (zeroed before its first
+    paced call only): FP64, float-instantiation and mixed-precision paced
+    calls interleaved on one matrix keep the oracle's results, and no wait
+    runs to its spin bound (each would cost milliseconds: the 60 calls here
+    take about 10 ms)."""
+    import time
+    import torch
+    n = 2_000_000
+    off, col, val = W.random_uniform_rows(n, n, 8, 16, seed=61)
+    val = val.astype(np.float32).astype(np.float64)  # FP32-exact, so every path has an exact reference
+    A = spira_amd.DeviceCSR(dev, n, n)
+    A.upload(off, col, val)
+    xh = np.random.default_rng(62).uniform(-1, 1, n).astype(np.float32).astype(np.float64)
+    y_ref = oracle.spmv_f64(off, col, val, xh)
+    x64 = torch.from_numpy(xh).cuda()
+    x32 = x64.float()
+    y64 = torch.empty(n, dtype=torch.float64, device="cuda")
+    y32 = torch.empty(n, dtype=torch.float32, device="cuda")
+    v = spira_amd.VARIANTS["xphase_paced"]
+    A.spmv(x64, y64, variant=v, device=True)  # plan, first paced call
+    A.spmv_f32(x32, y32, variant=v, device=True)
+    A.spmv_mixed(x64, y64, storage="f32", variant=v, device=True, count=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(20):
+        A.spmv(x64, y64, variant=v, device=True)
+        assert oracle.normwise_rel_err(y64.cpu().numpy(), y_ref) <= 1e-14, i
+        A.spmv_f32(x32, y32, variant=v, device=True)
+        assert oracle.normwise_rel_err(y32.cpu().numpy().astype(np.float64), y_ref) <= 1e-7, i
+        A.spmv_mixed(x64, y64, storage="f32", variant=v, device=True, count=False)
+        assert oracle.normwise_rel_err(y64.cpu().numpy(), y_ref) <= 1e-14, i
+    dt = time.perf_counter() - t0
+    print(f"60 interleaved paced calls (and their checks): {1e3 * dt:.1f} ms")
+    assert dt < 5.0  # the checks dominate; 60 waits run to their bound would add ~0.3 s of device time
+    t0 = time.perf_counter()
+    for _ in range(20):
+        A.spmv(x64, y64, variant=v, device=True, sync=False)
+        A.spmv_f32(x32, y32, variant=v, device=True, sync=False)
+        A.spmv_mixed(x64, y64, storage="f32", variant=v, device=True, sync=False, count=False)
+    A.dev.synchronize()
+    dt = time.perf_counter() - t0
+    print(f"60 interleaved paced calls: {1e3 * dt:.1f} ms")
+    assert dt < 0.15, dt
+
+
 _GIVE_UP_SCRIPT = r"""
 import sys, time
 import numpy as np
